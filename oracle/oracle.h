@@ -1,0 +1,84 @@
+/* CPU restatement of the reference contact step: TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the parity checker for the HIP path. Only tests/, the smoke check and the
+ * benchmark's CPU-baseline leg may load it; the product never links or calls it.
+ *
+ * Every routine follows the reference kernel it restates in operation order, with
+ * floating-point contraction off (built with -ffp-contract=off), float32 state, the int64
+ * fixed-point scale 1e10 and float64 quantisation. Citations are to files of the reference
+ * repository antonbj3/3fold-motion-engine (src/motion_engine/...).
+ *
+ * Parity pins (see tests/test_oracle_golden.py and tests/test_oracle_invariants.py):
+ *   - body build, world points, bead contacts, relaxed Jacobi, integrate: the reference's
+ *     archived trajectories reports/position_solve_matrix/inputs/cross_hardware_l4.npz
+ *     (jacobi stack8 x120, lattice1000 x40) and per-step contact counts;
+ *   - box narrow phase (an extension; the reference's bead model cannot express boxes):
+ *     closed forms (resting box -> 4 corners at known depth; aligned stacked boxes -> the
+ *     overlap rectangle), brute-force SAT on tiny inputs;
+ *   - pair build, JP colouring: invariants (proper colouring, sortedness) + brute force;
+ *   - coloured chunk solvers: physics checks (resting force = Mg, stick/slide at atan mu).
+ *     The PCG hash behind the colouring priorities is "parity unpinned" (the reference's
+ *     ColorCache archive needs the bead manifold reduction, not restated here).
+ */
+#ifndef MOTION_ORACLE_H
+#define MOTION_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_JACOBI = 0, OR_PAIR_CHUNKS = 1, OR_COLOR_CACHE = 2 };
+
+typedef struct or_params {
+    int solver;        /* OR_JACOBI / OR_PAIR_CHUNKS / OR_COLOR_CACHE */
+    int vit, pit;      /* velocity / position iterations */
+    float relax;       /* Jacobi relaxation (0.2) */
+    float t2d;         /* coloured friction t2 factor (1.0) */
+    int warm_start;    /* coloured solvers: warm start by feature key */
+    float R;           /* bead radius (bead geometry) */
+    float margin;      /* box contact margin (0 = penetrating points only) */
+    int cap;           /* contact capacity */
+    int n_colors;      /* 0 = JP colouring as is; K > 0 = split into max(K, JP colours) classes */
+} or_params;
+
+typedef struct or_world or_world;
+
+or_world *or_create(int n_bodies, int n_beads, const or_params *prm);
+void or_destroy(or_world *w);
+/* SoA body state: x[3N], q[4N] (xyzw), v[3N], w[3N], invM[N], IbInv[9N] row-major, kin[N],
+ * half[3N] (box half extents; unused in bead mode), mu[N], scene[N]. */
+void or_set_bodies(or_world *w, const float *x, const float *q, const float *v, const float *om,
+                   const float *invM, const float *IbInv, const int *kin, const float *half,
+                   const float *mu, const int *scene);
+void or_set_rest(or_world *w, const float *rest); /* bead geometry: rest[3P] */
+void or_set_velocity(or_world *w, const float *v, const float *om);
+void or_set_mass(or_world *w, const double *M);      /* fp64 masses for contact forces */
+int or_step(or_world *w, double dt, int substeps);   /* 0 ok; 4 capacity; 2 colouring */
+void or_get_state(const or_world *w, float *x, float *q, float *v, float *om);
+void or_get_aux(const or_world *w, float *pv, float *po, float *invIw);
+void or_contact_forces(const or_world *w, double *F);
+/* counters: [n_contacts, n_pairs, n_colors, cache_hits, cache_misses, overflow] */
+void or_counters(const or_world *w, int64_t *out);
+int or_get_contacts(const or_world *w, int64_t *key, int *bi, int *bj, float *pA, float *pB,
+                    float *n, float *pen, float *jn, float *jt1, float *jt2, float *jp);
+int or_get_pairs(const or_world *w, int *pbi, int *pbj, int *pstart, int *pcount, int *color);
+
+/* single-stage entry points for unit tests */
+void or_quat_to_matrix(const float *q, float *R);
+void or_quat_rotate(const float *q, const float *x, float *out);
+void or_integrate(int N, float *x, float *q, const float *v, const float *om, const float *pv,
+                  const float *po, const float *invM, float dt);
+int or_box_box(const float *xa, const float *qa, const float *ha, const float *xb,
+               const float *qb, const float *hb, float margin, float *p, float *n, float *pen,
+               int *code);
+int or_box_ground(const float *x, const float *q, const float *h, float margin, float *p,
+                  float *pen, int *code);
+uint32_t or_pcg(uint32_t s);
+int64_t or_jp_key(int rank);
+double or_time_steps(or_world *w, double dt, int substeps, int steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
