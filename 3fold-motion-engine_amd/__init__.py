@@ -1,1 +1,8 @@
-"""MI355X-native contact step (HIP/gfx950) behind the reference ContactEngine protocol."""
+"""MI355X-native contact step (hand-written gfx950 HIP kernels behind a C ABI) behind the
+reference ContactEngine protocol (antonbj3/3fold-motion-engine, src/motion_engine/contact_engine.py)."""
+from .protocol import BodyState, ContactEngine
+from .engines import ColorCacheContactEngine, GraphColoredContactEngine, RelaxedJacobiContactEngine
+from . import bodies, scenes
+
+__all__ = ["BodyState", "ContactEngine", "RelaxedJacobiContactEngine", "GraphColoredContactEngine",
+           "ColorCacheContactEngine", "bodies", "scenes"]

@@ -1,0 +1,78 @@
+"""ctypes binding of libmotion_hip.so (the C ABI in include/motion_hip.h).
+
+The library must be built in-tree (``build.py``); there is no fallback: a missing library, or a
+process without a ROCm GPU, raises instead of running anything else."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libmotion_hip.so"
+
+ME_OK, ME_EARG, ME_ERUNTIME, ME_ETHREAD, ME_ECAPACITY, ME_ENONFINITE = 0, 1, 2, 3, 4, 5
+SOLVER_JACOBI, SOLVER_PAIR_CHUNKS, SOLVER_COLOR_CACHE = 0, 1, 2
+FLAG_TILED = 1
+
+(BUF_KEY, BUF_BI, BUF_BJ, BUF_PA, BUF_PB, BUF_N, BUF_PEN, BUF_JN, BUF_JT1, BUF_JT2, BUF_JP,
+ BUF_PBI, BUF_PBJ, BUF_PSTART, BUF_PCOUNT, BUF_COLOR, BUF_VSNAP, BUF_PV, BUF_PO, BUF_INVIW,
+ BUF_ISLAND) = range(21)
+
+COUNTERS = ("n_contacts", "n_pairs", "n_colors", "cache_hits", "cache_misses", "overflow",
+            "n_candidates", "n_islands", "n_tiles", "max_tile_bodies")
+OVF_CONTACTS, OVF_PAIRS, OVF_CAND, OVF_COLORS, OVF_TILE = 1, 2, 4, 8, 16
+
+
+class Config(C.Structure):
+    _fields_ = [("solver", C.c_int32), ("n_bodies", C.c_int32), ("n_beads", C.c_int32),
+                ("vit", C.c_int32), ("pit", C.c_int32), ("relax", C.c_float), ("t2d", C.c_float),
+                ("warm_start", C.c_int32), ("bead_radius", C.c_float), ("margin", C.c_float),
+                ("max_pairs", C.c_int32), ("max_contacts", C.c_int32), ("n_colors", C.c_int32),
+                ("tile_bodies", C.c_int32), ("flags", C.c_int32)]
+
+
+class Bodies(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("x", "q", "v", "w", "inv_mass", "inv_inertia", "half",
+                                           "mu", "kinematic", "scene", "rest")]
+
+
+class MotionError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise MotionError(f"{LIB_PATH} is not built: run build.py (hipcc --offload-arch=gfx950)")
+        L = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        L.me_workspace_size.argtypes = [C.POINTER(Config), C.POINTER(C.c_size_t)]
+        L.me_create.argtypes = [C.POINTER(Config), C.POINTER(Bodies), vp, C.c_size_t, vp, C.POINTER(vp)]
+        L.me_destroy.argtypes = [vp]
+        L.me_set_bodies.argtypes = [vp, C.POINTER(Bodies)]
+        L.me_step.argtypes = [vp, C.c_double, C.c_int, vp]
+        L.me_buffer.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int64)]
+        L.me_read_counters.argtypes = [vp, C.POINTER(C.c_int64), vp]
+        L.me_last_error.restype = C.c_char_p
+        for f in ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
+                  "me_buffer", "me_read_counters"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc == ME_OK:
+        return
+    msg = lib().me_last_error().decode(errors="replace")
+    if rc == ME_EARG:
+        raise ValueError(f"{what}: {msg}")
+    raise MotionError(f"{what}: {msg} (code {rc})")
+
+
+EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
+            "me_buffer", "me_read_counters", "me_last_error")

@@ -1,0 +1,802 @@
+// Pipeline kernels of the contact step (gfx950). One thread per body, candidate pair, contact or
+// pair as noted; every count a later kernel needs stays on the device (istat slots).
+#pragma once
+#include "geometry.hpp"
+#include "solvers.hpp"
+
+namespace me {
+
+// int32 status slots shared by the kernels of one engine
+enum {
+    IS_C = 0, IS_P, IS_CAND, IS_NCOL, IS_HITS, IS_MISSES, IS_OVF, IS_ISL, IS_TILES, IS_MAXTB,
+    IS_MAXEXT, IS_LEFT, IS_WN, IS_CACHED_P, IS_CACHED_NCOL, IS_MISMATCH, IS_CELL, IS_NBODYSORT,
+    IS_NPAIRSORT, IS_COUNT = 32
+};
+enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16 };
+
+#define ME_NCOLMAX 512
+#define ME_MAXCAND 32
+#define ME_BEADCAND 64
+#define ME_THREADS 256
+
+__global__ void k_fill_i32(int *p, int n, int val) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = val;
+}
+__global__ void k_fill_f32(float *p, int n, float val) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = val;
+}
+__global__ void k_fill_i64(int64_t *p, int n, int64_t val) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = val;
+}
+__global__ void k_copy_f32(float *dst, const float *src, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// ── exclusive scan (3 phases, 2048 items per block, deterministic) ──
+#define SCAN_ITEMS 8
+#define SCAN_BLOCK (ME_THREADS * SCAN_ITEMS)
+
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int *total) {
+    // wave64 inclusive scan, then across the 4 waves of the block
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int k = 0; k < ME_THREADS / 64; ++k) {
+        if (k < wv) base += sh[k];
+        tot += sh[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
+}
+
+__device__ __forceinline__ int scan_len(const int *n_dev, int n_host, int cap) {
+    int n = n_dev ? *n_dev : n_host;
+    return n < cap ? n : cap;
+}
+
+__global__ void __launch_bounds__(ME_THREADS) k_scan_reduce(const int *in, const int *n_dev, int n_host, int cap, int *bsum) {
+    __shared__ int sh[ME_THREADS / 64];
+    int n = scan_len(n_dev, n_host, cap);
+    int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+        if (base + k < n) s += in[base + k];
+    int tot;
+    block_excl_scan(s, sh, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(int *bsum, int nb, int *total) {
+    __shared__ int sh[16];
+    // each of 1024 threads owns ceil(nb/1024) consecutive block sums
+    int per = (nb + 1023) / 1024;
+    int b0 = threadIdx.x * per;
+    int s = 0;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nb) s += bsum[b0 + k];
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int k = 0; k < 16; ++k) {
+        if (k < wv) base += sh[k];
+        tot += sh[k];
+    }
+    int run = base + x - s;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nb) {
+            int t = bsum[b0 + k];
+            bsum[b0 + k] = run;
+            run += t;
+        }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
+__global__ void __launch_bounds__(ME_THREADS) k_scan_down(const int *in, int *out, const int *n_dev, int n_host, int cap,
+                                                          const int *bsum) {
+    __shared__ int sh[ME_THREADS / 64];
+    int n = scan_len(n_dev, n_host, cap);
+    int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    int vals[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        vals[k] = (base + k < n) ? in[base + k] : 0;
+        s += vals[k];
+    }
+    int tot;
+    int run = block_excl_scan(s, sh, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += vals[k];
+    }
+}
+
+// ── A2 gravity, A3 world inverse inertia, box AABBs, velocity snapshot ──
+struct Bodies {
+    float *x, *q, *v, *w;
+    const float *invM, *IbInv, *half, *mu;
+    const int *kin, *scene;
+    const float *rest;
+};
+
+__global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb, float *vsnap, int *istat,
+                           float dt, int box_mode) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    f3 v = ld3(b.v, i);
+    if (b.invM[i] > 0.0f) {  // k_grav (contact_engine_gpu.py:41-44)
+        v = v + mk(0.0f, 0.0f, -9.81f) * dt;
+        st3(b.v, i, v);
+    }
+    st3(vsnap, i, v);
+    m3 R = quat_to_matrix(ld4(b.q, i));  // k_invIw (contact_engine_gpu.py:47-53)
+    stm(Rw, i, R);
+    if (b.kin[i] == 1) {
+        m3 z;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) z.m[k / 3][k % 3] = 0.0f;
+        stm(invIw, i, z);
+    } else {
+        m3 Ib = ldm(b.IbInv, i);
+        m3 t = mm(R, Ib);
+        m3 r = mm(t, transpose(R));
+        stm(invIw, i, r);
+    }
+    if (box_mode) {
+        f3 h = ld3(b.half, i), c = ld3(b.x, i);
+        float e0 = fabsf(R.m[0][0]) * h.x + fabsf(R.m[0][1]) * h.y + fabsf(R.m[0][2]) * h.z;
+        float e1 = fabsf(R.m[1][0]) * h.x + fabsf(R.m[1][1]) * h.y + fabsf(R.m[1][2]) * h.z;
+        float e2 = fabsf(R.m[2][0]) * h.x + fabsf(R.m[2][1]) * h.y + fabsf(R.m[2][2]) * h.z;
+        aabb[6 * i + 0] = c.x - e0; aabb[6 * i + 1] = c.y - e1; aabb[6 * i + 2] = c.z - e2;
+        aabb[6 * i + 3] = c.x + e0; aabb[6 * i + 4] = c.y + e1; aabb[6 * i + 5] = c.z + e2;
+        // largest AABB half extent actually stored (hi - centre), as the oracle measures it
+        float m = fmaxf(fmaxf((c.x + e0) - c.x, (c.y + e1) - c.y), (c.z + e2) - c.z);
+        atomicMax(&istat[IS_MAXEXT], __float_as_int(m));
+    }
+}
+
+// cell size of the body grid: (2 * max half extent + margin + slack) * 1.001, so that any two
+// overlapping AABBs have centre cells at most one apart even after rounding of x / cell
+__global__ void k_grid_cell(int *istat, float margin) {
+    float hmax = __int_as_float(istat[IS_MAXEXT]);
+    float cell = (2.0f * hmax + margin + ME_BP_SLACK) * 1.001f;
+    if (!(cell > 0.0f)) cell = 1.0f;
+    istat[IS_CELL] = __float_as_int(cell);
+}
+
+__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, int s) {
+    return ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u) ^
+           ((uint32_t)s * 2654435761u);
+}
+
+// hash grid: bucket per item (box centre or bead point), bucket counts
+__global__ void k_grid_count(int n, const float *pos, const int *scene_of, int div, const int *istat, float cell_fixed,
+                             int T, int *cellc, int *bucket, int *bcount) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float cell = cell_fixed > 0.0f ? cell_fixed : __int_as_float(istat[IS_CELL]);
+    f3 p = ld3(pos, i);
+    int cx = (int)floorf(p.x / cell), cy = (int)floorf(p.y / cell), cz = (int)floorf(p.z / cell);
+    int s = scene_of ? scene_of[i / div] : 0;
+    cellc[4 * i] = cx; cellc[4 * i + 1] = cy; cellc[4 * i + 2] = cz; cellc[4 * i + 3] = s;
+    int bk = (int)(cell_hash(cx, cy, cz, s) & (uint32_t)(T - 1));
+    bucket[i] = bk;
+    atomicAdd(&bcount[bk], 1);
+}
+__global__ void k_grid_scatter(int n, const int *bucket, const int *bstart, int *bcursor, int *items) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int bk = bucket[i];
+    items[bstart[bk] + atomicAdd(&bcursor[bk], 1)] = i;
+}
+
+__device__ __forceinline__ bool aabb_overlap(const float *a, const float *b, float m) {
+    return a[0] <= b[3] + m && b[0] <= a[3] + m && a[1] <= b[4] + m && b[1] <= a[4] + m && a[2] <= b[5] + m &&
+           b[2] <= a[5] + m;
+}
+
+// ── A6 body pairs: ground candidate + AABB overlaps with higher bodies of the same scene, in key order ──
+__global__ void k_box_cand(int N, Bodies b, const float *aabb, const int *cellc, const int *bstart, const int *bcount,
+                           const int *items, int T, float margin, int *cand_n, int *cand_tmp, int *istat) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    int list[ME_MAXCAND];
+    int nc = 0;
+    bool over = false;
+    float m = margin + ME_BP_SLACK;
+    const float *A = aabb + 6 * a;
+    bool dynA = b.invM[a] > 0.0f;
+    if (dynA && A[2] < m) list[nc++] = -1;
+    int cx = cellc[4 * a], cy = cellc[4 * a + 1], cz = cellc[4 * a + 2], s = cellc[4 * a + 3];
+    int first = nc;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int X = cx + dx, Y = cy + dy, Z = cz + dz;
+                int bk = (int)(cell_hash(X, Y, Z, s) & (uint32_t)(T - 1));
+                int st = bstart[bk], cnt = bcount[bk];
+                for (int t = 0; t < cnt; ++t) {
+                    int j = items[st + t];
+                    if (j <= a) continue;
+                    if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z || cellc[4 * j + 3] != s)
+                        continue;
+                    if (!dynA && !(b.invM[j] > 0.0f)) continue;
+                    if (!aabb_overlap(A, aabb + 6 * j, m)) continue;
+                    if (nc < ME_MAXCAND) {
+                        int k = nc++;  // insertion keeps ascending body order
+                        while (k > first && list[k - 1] > j) { list[k] = list[k - 1]; --k; }
+                        list[k] = j;
+                    } else {
+                        over = true;
+                    }
+                }
+            }
+    if (over) atomicOr(&istat[IS_OVF], OVF_CAND);
+    cand_n[a] = nc;
+    for (int k = 0; k < nc; ++k) cand_tmp[a * ME_MAXCAND + k] = list[k];
+}
+__global__ void k_box_cand_emit(int N, const int *cand_n, const int *cand_off, const int *cand_tmp, int cap,
+                                int *cand_a, int *cand_b, int *istat) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    int o = cand_off[a], n = cand_n[a];
+    for (int k = 0; k < n; ++k) {
+        if (o + k < cap) {
+            cand_a[o + k] = a;
+            cand_b[o + k] = cand_tmp[a * ME_MAXCAND + k];
+        } else {
+            atomicOr(&istat[IS_OVF], OVF_PAIRS);
+        }
+    }
+}
+
+__device__ __forceinline__ float pair_mu(const float *mu, int a, int b) {
+    if (b < 0) return mu[a];
+    float ma = mu[a], mb = mu[b];
+    return ma == mb ? ma : sqrtf(ma * mb);
+}
+
+struct Slots {
+    int *n;
+    int64_t *key;
+    float *p, *nrm, *pen, *mu;
+};
+
+// ── narrow phase per candidate pair, up to 4 contacts in fixed slots ──
+__global__ void k_narrow(int cap, const int *istat, Bodies b, const float *Rw, const int *cand_a, const int *cand_b,
+                         float margin, int N, Slots S) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = istat[IS_CAND];
+    if (n > cap) n = cap;
+    if (p >= n) return;
+    int a = cand_a[p], bb = cand_b[p];
+    m3 RA = ldm(Rw, a);
+    f3 cA = ld3(b.x, a), hA = ld3(b.half, a);
+    f3 pp[4], nn[4];
+    float pen[4];
+    int code[4];
+    int k;
+    int64_t pk = (int64_t)a * (int64_t)(N + 1) + (int64_t)bb + 1;
+    if (bb < 0) {
+        k = box_ground(cA, RA, hA, margin, pp, pen, code);
+        for (int i = 0; i < k; ++i) nn[i] = mk(0.0f, 0.0f, 1.0f);
+    } else {
+        m3 RB = ldm(Rw, bb);
+        k = box_box(cA, RA, hA, ld3(b.x, bb), RB, ld3(b.half, bb), margin, pp, nn, pen, code);
+    }
+    float mu = pair_mu(b.mu, a, bb);
+    S.n[p] = k;
+    for (int i = 0; i < k; ++i) {
+        int s = 4 * p + i;
+        S.key[s] = pk * 8192 + code[i];
+        st3(S.p, s, pp[i]);
+        st3(S.nrm, s, nn[i]);
+        S.pen[s] = pen[i];
+        S.mu[s] = mu;
+    }
+}
+
+__global__ void k_pair_flag(int cap, const int *istat, const int *slot_n, int *flag) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = istat[IS_CAND];
+    if (n > cap) n = cap;
+    if (p >= n) return;
+    flag[p] = slot_n[p] > 0 ? 1 : 0;
+}
+
+struct Contacts {
+    int64_t *key;
+    int *bi, *bj;
+    float *pA, *pB, *n, *pen, *mu, *jn, *jt1, *jt2, *jp;
+};
+struct Pairs {
+    int *pbi, *pbj, *pstart, *pcount, *col, *porder;
+};
+
+__global__ void k_compact(int cap, int capC, int capP, int *istat, const int *cand_a, const int *cand_b, Slots S,
+                          const int *slot_off, const int *pair_off, Contacts Cc, Pairs Pp) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = istat[IS_CAND];
+    if (n > cap) n = cap;
+    if (p >= n) return;
+    int k = S.n[p];
+    if (k == 0) return;
+    int o = slot_off[p], g = pair_off[p];
+    if (g >= capP || o + k > capC) {
+        atomicOr(&istat[IS_OVF], OVF_CONTACTS);
+        return;
+    }
+    int a = cand_a[p], bb = cand_b[p];
+    Pp.pbi[g] = a; Pp.pbj[g] = bb; Pp.pstart[g] = o; Pp.pcount[g] = k;
+    for (int i = 0; i < k; ++i) {
+        int s = 4 * p + i, c = o + i;
+        Cc.key[c] = S.key[s];
+        Cc.bi[c] = a; Cc.bj[c] = bb;
+        f3 pt = ld3(S.p, s);
+        st3(Cc.pA, c, pt); st3(Cc.pB, c, pt);
+        st3(Cc.n, c, ld3(S.nrm, s));
+        Cc.pen[c] = S.pen[s];
+        Cc.mu[c] = S.mu[s];
+    }
+}
+
+// per-contact colouring (pair_chunks=False, contact_engine_gpu_colored.py:1105-1136): every
+// contact is a chunk of its own
+__global__ void k_pairs_per_contact(int capC, int *istat, const int *bi, const int *bj, Pairs Pp) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int C = istat[IS_C];
+    if (C > capC) C = capC;
+    if (c == 0) istat[IS_P] = C;
+    if (c >= C) return;
+    Pp.pbi[c] = bi[c]; Pp.pbj[c] = bj[c]; Pp.pstart[c] = c; Pp.pcount[c] = 1;
+}
+
+// clamp counts after compaction so later kernels stay in bounds
+__global__ void k_clamp_counts(int *istat, int capC, int capP) {
+    if (istat[IS_C] > capC) { istat[IS_C] = capC; atomicOr(&istat[IS_OVF], OVF_CONTACTS); }
+    if (istat[IS_P] > capP) { istat[IS_P] = capP; atomicOr(&istat[IS_OVF], OVF_PAIRS); }
+}
+
+// ── bead geometry (the reference's model): A4 world points, A5 ground + bead-pair contacts in feature-key order ──
+__global__ void k_bead_points(int N, int P, Bodies b, float *allp) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N * P) return;
+    int bi = g / P;
+    st3(allp, g, ld3(b.x, bi) + quat_rotate(ld4(b.q, bi), ld3(b.rest, g % P)));
+}
+
+template <bool EMIT>
+__global__ void k_bead_contacts(int NP, int P, const float *allp, const int *cellc, const int *bstart, const int *bcount,
+                                const int *items, int T, float R, const float *mu, int *cnt, const int *off, int capC,
+                                Contacts Cc, int *istat) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NP) return;
+    f3 xi = ld3(allp, g);
+    int bi = g / P;
+    int list[ME_BEADCAND];
+    int nc = 0;
+    int cx = cellc[4 * g], cy = cellc[4 * g + 1], cz = cellc[4 * g + 2];
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int X = cx + dx, Y = cy + dy, Z = cz + dz;
+                int bk = (int)(cell_hash(X, Y, Z, 0) & (uint32_t)(T - 1));
+                int st = bstart[bk], c = bcount[bk];
+                for (int t = 0; t < c; ++t) {
+                    int j = items[st + t];
+                    if (j <= g || j / P == bi) continue;
+                    if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z) continue;
+                    f3 dv = xi - ld3(allp, j);
+                    float dist = length(dv);
+                    if (!(dist < 2.0f * R && dist > 1e-9f)) continue;
+                    if (nc < ME_BEADCAND) {
+                        int k = nc++;
+                        while (k > 0 && list[k - 1] > j) { list[k] = list[k - 1]; --k; }
+                        list[k] = j;
+                    } else {
+                        atomicOr(&istat[IS_OVF], OVF_CAND);
+                    }
+                }
+            }
+    float s = xi.z - R;
+    int ground = s < 0.0f ? 1 : 0;
+    if (!EMIT) {
+        cnt[g] = ground + nc;
+        return;
+    }
+    int o = off[g];
+    int64_t stride = (int64_t)NP + 1;
+    float m = mu[0];
+    int c = o;
+    if (ground) {
+        if (c < capC) {
+            Cc.key[c] = (int64_t)g * stride; Cc.bi[c] = bi; Cc.bj[c] = -1;
+            st3(Cc.pA, c, xi); st3(Cc.pB, c, mk(0.0f, 0.0f, 0.0f)); st3(Cc.n, c, mk(0.0f, 0.0f, 1.0f));
+            Cc.pen[c] = -s; Cc.mu[c] = m;
+        }
+        c++;
+    }
+    for (int k = 0; k < nc; ++k, ++c) {
+        if (c >= capC) break;
+        int j = list[k];
+        f3 xj = ld3(allp, j);
+        f3 dv = xi - xj;
+        float dist = length(dv);
+        Cc.key[c] = (int64_t)g * stride + (int64_t)j + 1; Cc.bi[c] = bi; Cc.bj[c] = j / P;
+        st3(Cc.pA, c, xi); st3(Cc.pB, c, xj); st3(Cc.n, c, dv / dist);
+        Cc.pen[c] = 2.0f * R - dist; Cc.mu[c] = m;
+    }
+    if (o + ground + nc > capC) atomicOr(&istat[IS_OVF], OVF_CONTACTS);
+}
+
+// ── A8-A10 relaxed Jacobi with int64 fixed-point accumulation ──
+// _acc (contact_engine_gpu.py:32-38): int64(round(f64(d) * f64(sign * scale))) with atomic add
+__device__ __forceinline__ void acc3(int64_t *acc, int N, int b, int off, f3 d, float sign, float scale) {
+    double sc = (double)(sign * scale);
+    atomicAdd((unsigned long long *)&acc[(size_t)(off + 0) * N + b], (unsigned long long)(int64_t)round((double)d.x * sc));
+    atomicAdd((unsigned long long *)&acc[(size_t)(off + 1) * N + b], (unsigned long long)(int64_t)round((double)d.y * sc));
+    atomicAdd((unsigned long long *)&acc[(size_t)(off + 2) * N + b], (unsigned long long)(int64_t)round((double)d.z * sc));
+}
+
+struct JacArgs {
+    const int *bi, *bj;
+    const float *pA, *pB, *n, *pen, *mu;
+    float *jn, *jt1, *jt2, *jp;
+    const float *x, *invM, *invIw;
+    const float *v, *w, *pv, *po;
+    int64_t *acc;
+    int N;
+};
+
+// k_jac_vel (contact_engine_gpu.py:82-131)
+__global__ void k_jac_vel(const int *istat, JacArgs A, float scale) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= istat[IS_C]) return;
+    int bi = A.bi[c], bj = A.bj[c];
+    f3 nrm = ld3(A.n, c), rA = ld3(A.pA, c) - ld3(A.x, bi);
+    float iMA = A.invM[bi];
+    m3 IA = ldm(A.invIw, bi), IB = IA;
+    f3 va = ld3(A.v, bi) + cross(ld3(A.w, bi), rA);
+    f3 vb = mk(0.0f, 0.0f, 0.0f), rB = mk(0.0f, 0.0f, 0.0f);
+    float iMB = 0.0f;
+    if (bj >= 0) {
+        rB = ld3(A.pB, c) - ld3(A.x, bj);
+        vb = ld3(A.v, bj) + cross(ld3(A.w, bj), rB);
+        iMB = A.invM[bj];
+        IB = ldm(A.invIw, bj);
+    }
+    f3 vrel = va - vb;
+    float vn = dot(vrel, nrm);
+    f3 vt = vrel - nrm * vn;
+    float mt = length(vt);
+    f3 t1 = vt / mt;
+    if (mt <= 5e-3f) t1 = fixed_t1(nrm);
+    f3 t2 = cross(nrm, t1);
+    f3 rnA = cross(rA, nrm);
+    float meff = iMA + dot(rnA, mv(IA, rnA));
+    if (bj >= 0) {
+        f3 rnB = cross(rB, nrm);
+        meff = meff + iMB + dot(rnB, mv(IB, rnB));
+    }
+    if (meff < 1e-12f) return;
+    float jo = A.jn[c];
+    float dj = -vn / meff;
+    float nw = wmax(0.0f, jo + dj);
+    dj = nw - jo;
+    A.jn[c] = nw;
+    f3 Jn = nrm * dj;
+    int N = A.N;
+    acc3(A.acc, N, bi, 0, Jn * iMA, 1.0f, scale);
+    acc3(A.acc, N, bi, 3, mv(IA, cross(rA, Jn)), 1.0f, scale);
+    if (bj >= 0) {
+        acc3(A.acc, N, bj, 0, Jn * iMB, -1.0f, scale);
+        acc3(A.acc, N, bj, 3, mv(IB, cross(rB, Jn)), -1.0f, scale);
+    }
+    f3 rtA1 = cross(rA, t1);
+    float meft1 = iMA + dot(rtA1, mv(IA, rtA1));
+    f3 rtA2 = cross(rA, t2);
+    float meft2 = iMA + dot(rtA2, mv(IA, rtA2));
+    if (bj >= 0) {
+        f3 rtB1 = cross(rB, t1);
+        meft1 = meft1 + iMB + dot(rtB1, mv(IB, rtB1));
+        f3 rtB2 = cross(rB, t2);
+        meft2 = meft2 + iMB + dot(rtB2, mv(IB, rtB2));
+    }
+    float o1 = A.jt1[c], o2 = A.jt2[c];
+    float a1 = o1 - dot(vrel, t1) / meft1;
+    float a2 = o2 - dot(vrel, t2) / meft2;
+    float mag = sqrtf(a1 * a1 + a2 * a2);
+    float lim = A.mu[c] * nw;
+    if (mag > lim && mag > 1e-12f) { a1 = a1 * lim / mag; a2 = a2 * lim / mag; }
+    A.jt1[c] = a1;
+    A.jt2[c] = a2;
+    f3 Jt = t1 * (a1 - o1) + t2 * (0.5f * (a2 - o2));
+    acc3(A.acc, N, bi, 0, Jt * iMA, 1.0f, scale);
+    acc3(A.acc, N, bi, 3, mv(IA, cross(rA, Jt)), 1.0f, scale);
+    if (bj >= 0) {
+        acc3(A.acc, N, bj, 0, Jt * iMB, -1.0f, scale);
+        acc3(A.acc, N, bj, 3, mv(IB, cross(rB, Jt)), -1.0f, scale);
+    }
+}
+
+// k_jac_pos (contact_engine_gpu.py:146-169)
+__global__ void k_jac_pos(const int *istat, JacArgs A, float dt, float scale) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= istat[IS_C]) return;
+    int bi = A.bi[c], bj = A.bj[c];
+    f3 nrm = ld3(A.n, c), rA = ld3(A.pA, c) - ld3(A.x, bi);
+    float iMA = A.invM[bi];
+    m3 IA = ldm(A.invIw, bi), IB = IA;
+    f3 rel = ld3(A.pv, bi) + cross(ld3(A.po, bi), rA);
+    f3 rB = mk(0.0f, 0.0f, 0.0f);
+    float iMB = 0.0f;
+    if (bj >= 0) {
+        rB = ld3(A.pB, c) - ld3(A.x, bj);
+        rel = rel - (ld3(A.pv, bj) + cross(ld3(A.po, bj), rB));
+        iMB = A.invM[bj];
+        IB = ldm(A.invIw, bj);
+    }
+    f3 rnA = cross(rA, nrm);
+    float meff = iMA + dot(rnA, mv(IA, rnA));
+    if (bj >= 0) {
+        f3 rnB = cross(rB, nrm);
+        meff = meff + iMB + dot(rnB, mv(IB, rnB));
+    }
+    if (meff < 1e-12f) return;
+    float bias = 0.2f * wmax(A.pen[c] - 1e-4f, 0.0f) / dt;
+    float dj = (bias - dot(rel, nrm)) / meff;
+    float jo = A.jp[c];
+    float nw = wmax(0.0f, jo + dj);
+    dj = nw - jo;
+    A.jp[c] = nw;
+    f3 Jn = nrm * dj;
+    int N = A.N;
+    acc3(A.acc, N, bi, 0, Jn * iMA, 1.0f, scale);
+    acc3(A.acc, N, bi, 3, mv(IA, cross(rA, Jn)), 1.0f, scale);
+    if (bj >= 0) {
+        acc3(A.acc, N, bj, 0, Jn * iMB, -1.0f, scale);
+        acc3(A.acc, N, bj, 3, mv(IB, cross(rB, Jn)), -1.0f, scale);
+    }
+}
+
+// k_apply (contact_engine_gpu.py:134-143)
+__global__ void k_jac_apply(int N, float *v, float *w, int64_t *acc, float relax, float scale) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float inv = 1.0f / scale;
+    double dinv = (double)inv;
+    f3 dv = mk((float)((double)acc[i] * dinv), (float)((double)acc[(size_t)N + i] * dinv),
+               (float)((double)acc[(size_t)2 * N + i] * dinv));
+    f3 dw = mk((float)((double)acc[(size_t)3 * N + i] * dinv), (float)((double)acc[(size_t)4 * N + i] * dinv),
+               (float)((double)acc[(size_t)5 * N + i] * dinv));
+    st3(v, i, ld3(v, i) + relax * dv);
+    st3(w, i, ld3(w, i) + relax * dw);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[(size_t)k * N + i] = 0;
+}
+
+// Jacobi substep set-up: impulses to zero; pv, po, accumulators to zero only when there are
+// contacts (the reference zeroes pv/po inside `if C > 0`, contact_engine_gpu.py:266-269)
+__global__ void k_jac_reset(int N, int capC, const int *istat, float *pv, float *po, int64_t *acc, float *jn, float *jt1,
+                            float *jt2, float *jp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int C = istat[IS_C];
+    if (C <= 0) return;
+    if (i < N) {
+        st3(pv, i, mk(0.0f, 0.0f, 0.0f));
+        st3(po, i, mk(0.0f, 0.0f, 0.0f));
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[(size_t)k * N + i] = 0;
+    }
+    if (i < C && i < capC) { jn[i] = 0.0f; jt1[i] = 0.0f; jt2[i] = 0.0f; jp[i] = 0.0f; }
+}
+
+// ── coloured solvers ──
+// A14 warm start by binary search in the previous step's kept keys (contact_engine_gpu_colored.py:602-626)
+__global__ void k_warm_lookup(int capC, const int *istat, int warm, const int64_t *key, const int64_t *wkey,
+                              const float *wjn, const float *wjt1, const float *wjt2, float *jn, float *jt1, float *jt2,
+                              float *jp) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int C = istat[IS_C];
+    if (c >= C || c >= capC) return;
+    jn[c] = 0.0f; jt1[c] = 0.0f; jt2[c] = 0.0f; jp[c] = 0.0f;
+    int wn = istat[IS_WN];
+    if (!warm || wn == 0) return;
+    int64_t k = key[c];
+    int lo = 0, hi = wn - 1, hit = -1;
+    while (lo <= hi) {
+        int mid = (lo + hi) / 2;
+        int64_t km = wkey[mid];
+        if (km == k) { hit = mid; lo = hi + 1; }
+        else if (km < k) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    if (hit >= 0) { jn[c] = wjn[hit]; jt1[c] = wjt1[hit]; jt2[c] = wjt2[hit]; }
+}
+// A23 warm store (contact_engine_gpu_colored.py:1089-1093)
+__global__ void k_warm_store(int capC, int *istat, const int64_t *key, const float *jn, const float *jt1,
+                             const float *jt2, int64_t *wkey, float *wjn, float *wjt1, float *wjt2) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int C = istat[IS_C];
+    if (C > capC) C = capC;
+    if (c == 0) istat[IS_WN] = C;
+    if (c >= C) return;
+    wkey[c] = key[c]; wjn[c] = jn[c]; wjt1[c] = jt1[c]; wjt2[c] = jt2[c];
+}
+
+// ColorCache compare (contact_engine_gpu_color_cache.py:7-18, 37-54)
+__global__ void k_cc_compare(int N, int *istat, const int *pbi, const int *pbj, const int *ca, const int *cb,
+                             const float *invM, const float *cinvM) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int P = istat[IS_P];
+    if (P != istat[IS_CACHED_P]) {
+        if (i == 0) istat[IS_MISMATCH] = 1;
+        return;
+    }
+    if (i < P && (pbi[i] != ca[i] || pbj[i] != cb[i])) atomicMax(&istat[IS_MISMATCH], 1);
+    if (i < N && invM[i] != cinvM[i]) atomicMax(&istat[IS_MISMATCH], 1);
+}
+
+// PCG hash (Jarzynski & Olano 2020), behind rand_init(seed, offset) / randi
+__device__ __forceinline__ uint32_t pcg(uint32_t s) {
+    uint32_t b = s * 747796405u + 2891336453u;
+    uint32_t c = ((b >> ((b >> 28u) + 4u)) ^ b) * 277803737u;
+    return (c >> 22u) ^ c;
+}
+// k_prio (contact_engine_gpu_colored.py:137-148)
+__device__ __forceinline__ int64_t jp_key(int rank) {
+    uint32_t st = pcg(pcg((uint32_t)7919 + pcg((uint32_t)rank)));
+    int64_t p = (int64_t)(int32_t)st;
+    if (p < 0) p = -p;
+    return p * (int64_t)4294967296LL + (int64_t)rank;
+}
+
+// A17 JP rounds over the pair graph (k_claim / k_win, contact_engine_gpu_colored.py:151-185)
+__global__ void k_jp_init(int capP, const int *istat, int *col) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < istat[IS_P] && p < capP) col[p] = -1;
+}
+__global__ void k_jp_claim(int capP, const int *istat, const int *col, const int *pbi, const int *pbj,
+                           const float *invM, unsigned long long *claim) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= istat[IS_P] || p >= capP || col[p] >= 0) return;
+    unsigned long long k = (unsigned long long)jp_key(p);
+    int a = pbi[p], b = pbj[p];
+    if (invM[a] > 0.0f) atomicMin(&claim[a], k);
+    if (b >= 0 && invM[b] > 0.0f) atomicMin(&claim[b], k);
+}
+__global__ void k_jp_win(int capP, int *istat, int cur, int *col, const int *pbi, const int *pbj, const float *invM,
+                         const unsigned long long *claim) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= istat[IS_P] || p >= capP || col[p] >= 0) return;
+    unsigned long long k = (unsigned long long)jp_key(p);
+    int a = pbi[p], b = pbj[p];
+    bool ok = true;
+    if (invM[a] > 0.0f && claim[a] != k) ok = false;
+    if (b >= 0 && invM[b] > 0.0f && claim[b] != k) ok = false;
+    if (ok) col[p] = cur;
+    else atomicAdd(&istat[IS_LEFT], 1);
+}
+
+// colour class sizes; then (single thread) the split of classes into max(K, JP colours) classes
+__global__ void k_color_count(int capP, const int *istat, const int *col, int *ccount) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= istat[IS_P] || p >= capP) return;
+    atomicAdd(&ccount[col[p]], 1);
+}
+// sub[c] colours for JP class c: the extra K - ncol classes go one at a time to the class with the
+// most pairs per sub-colour (lowest class first); base[c] = first final colour of class c
+__global__ void k_color_split_plan(int *istat, int K, const int *ccount, int *sub, int *base, int maxcol) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int nc = istat[IS_NCOL];
+    for (int c = 0; c < nc; ++c) sub[c] = 1;
+    int tot = nc;
+    if (K > nc && nc > 0) {
+        for (int e = nc; e < K && e < maxcol; ++e) {
+            int bc = 0;
+            for (int c = 1; c < nc; ++c)
+                if ((long long)ccount[c] * sub[bc] > (long long)ccount[bc] * sub[c]) bc = c;
+            sub[bc]++;
+        }
+        tot = 0;
+        for (int c = 0; c < nc; ++c) tot += sub[c];
+    }
+    base[0] = 0;
+    for (int c = 0; c < nc; ++c) base[c + 1] = base[c] + sub[c];
+    istat[IS_NCOL] = tot;
+}
+__global__ void k_color_split_apply(int capP, const int *istat, int *col, const int *sub, const int *base) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= istat[IS_P] || p >= capP) return;
+    int c = col[p];
+    col[p] = base[c] + p % sub[c];
+}
+// group pairs by colour (order inside a colour is free: its pairs touch disjoint dynamic bodies)
+__global__ void k_color_scatter(int capP, const int *istat, const int *col, const int *cstart, int *cfill, int *porder) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= istat[IS_P] || p >= capP) return;
+    int c = col[p];
+    porder[cstart[c] + atomicAdd(&cfill[c], 1)] = p;
+}
+__global__ void k_cc_store(int N, int capP, int *istat, const int *pbi, const int *pbj, const int *col, const float *invM,
+                           int *ca, int *cb, int *ccol, float *cinvM, int hit) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int P = istat[IS_P];
+    if (i == 0) {
+        if (hit) istat[IS_HITS] += 1;
+        else { istat[IS_MISSES] += 1; istat[IS_CACHED_P] = P; istat[IS_CACHED_NCOL] = istat[IS_NCOL]; }
+    }
+    if (hit) return;
+    if (i < P && i < capP) { ca[i] = pbi[i]; cb[i] = pbj[i]; ccol[i] = col[i]; }
+    if (i < N) cinvM[i] = invM[i];
+}
+__global__ void k_cc_restore(int capP, int *istat, const int *ccol, int *col) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) istat[IS_NCOL] = istat[IS_CACHED_NCOL];
+    if (i < istat[IS_P] && i < capP) col[i] = ccol[i];
+}
+
+struct SweepArgs {
+    BodyView B;
+    ContactView Cv;
+    const int *pbi, *pbj, *pstart, *pcount, *porder, *cstart, *ccount;
+};
+
+// one launch per colour: one thread per pair of colour ci
+__global__ void k_sweep_warm(SweepArgs S, int ci) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.ccount[ci]) return;
+    int p = S.porder[S.cstart[ci] + t];
+    apply_warm_pair(S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p]);
+}
+template <bool FIXED>
+__global__ void k_sweep_vel(SweepArgs S, int ci, float t2d) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.ccount[ci]) return;
+    int p = S.porder[S.cstart[ci] + t];
+    chunk_vel<FIXED>(S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p], t2d);
+}
+__global__ void k_sweep_pos(SweepArgs S, int ci, float dt) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.ccount[ci]) return;
+    int p = S.porder[S.cstart[ci] + t];
+    chunk_pos(S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p], dt);
+}
+
+__global__ void k_zero_pvpo(int N, float *pv, float *po) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    st3(pv, i, mk(0.0f, 0.0f, 0.0f));
+    st3(po, i, mk(0.0f, 0.0f, 0.0f));
+}
+
+// A11 integrate (contact_engine_gpu.py:172-180)
+__global__ void k_integrate(int N, Bodies b, const float *pv, const float *po, float dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (b.invM[i] <= 0.0f) return;
+    integrate_body(b.x, b.q, ld3(b.v, i), ld3(b.w, i), ld3(pv, i), ld3(po, i), i, dt);
+}
+
+}  // namespace me

@@ -1,0 +1,602 @@
+// libmotion_hip.so: the C ABI of include/motion_hip.h and the host orchestration of one contact
+// step on one MI355X. All device memory is the caller's (body arrays + one workspace block).
+#include "../../include/motion_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+
+#include "kernels.hpp"
+#include "tiled.hpp"
+
+using namespace me;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            return fail(ME_ERUNTIME, std::string(#expr) + ": " + hipGetErrorString(_e));         \
+    } while (0)
+
+#define LAUNCH_CHECK()                                                                           \
+    do {                                                                                         \
+        hipError_t _e = hipGetLastError();                                                       \
+        if (_e != hipSuccess) return fail(ME_ERUNTIME, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+inline int nblk(long long n) { return (int)((n + ME_THREADS - 1) / ME_THREADS); }
+inline int pow2_at_least(long long n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+// bump allocator over the workspace (sizes in bytes, 256-byte aligned)
+struct Bump {
+    char *base;
+    size_t off = 0;
+    template <class T>
+    T *take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += sizeof(T) * (count ? count : 1);
+        return p;
+    }
+};
+
+}  // namespace
+
+struct me_engine {
+    me_config cfg;
+    me_bodies b;
+    int N, P, NP, T;
+    int cap_cand, capC, capP;
+    std::thread::id owner;
+    // per body
+    float *invIw, *Rw, *aabb, *pv, *po, *vsnap, *allp;
+    int64_t *acc;
+    unsigned long long *claim;
+    // grid
+    int *cellc, *bucket, *bcount, *bstart, *bcursor, *items;
+    // candidates and narrow-phase slots
+    int *cand_n, *cand_off, *cand_tmp, *cand_a, *cand_b;
+    Slots S;
+    int *slot_off, *pair_flag, *pair_off;
+    // contacts, pairs
+    Contacts Cc;
+    Pairs Pp;
+    int *cache_a, *cache_b, *cache_col;
+    float *cache_invM;
+    int *ccount, *cstart, *cfill, *csub, *cbase;
+    // warm store
+    int64_t *wkey;
+    float *wjn, *wjt1, *wjt2;
+    // tiled solve
+    TiledBuffers tb;
+    // scan scratch + status
+    int *bsum;
+    int nbsum;
+    int *istat;
+    int *hstat;  // pinned host mirror
+};
+
+namespace {
+
+void layout(me_engine *e, char *base, size_t *bytes) {
+    const me_config &c = e->cfg;
+    int N = c.n_bodies, P = c.n_beads > 0 ? c.n_beads : 0;
+    long long NP = (long long)N * (P > 0 ? P : 1);
+    e->N = N; e->P = P; e->NP = (int)NP;
+    long long items = P > 0 ? NP : N;
+    e->T = pow2_at_least(2 * items > 64 ? 2 * items : 64);
+    e->cap_cand = c.max_pairs > 0 ? c.max_pairs : (P > 0 ? 1 : 16 * N + 1024);
+    e->capC = c.max_contacts > 0 ? c.max_contacts : (P > 0 ? 400000 : 4 * e->cap_cand);
+    e->capP = (P > 0 || (c.flags & ME_FLAG_PER_CONTACT)) ? e->capC : e->cap_cand;
+    Bump B{base};
+    e->invIw = B.take<float>(9 * (size_t)N);
+    e->Rw = B.take<float>(9 * (size_t)N);
+    e->aabb = B.take<float>(6 * (size_t)N);
+    e->pv = B.take<float>(3 * (size_t)N);
+    e->po = B.take<float>(3 * (size_t)N);
+    e->vsnap = B.take<float>(3 * (size_t)N);
+    e->allp = B.take<float>(3 * (size_t)(P > 0 ? NP : 1));
+    e->acc = B.take<int64_t>(6 * (size_t)N);
+    e->claim = B.take<unsigned long long>((size_t)N);
+    e->cellc = B.take<int>(4 * (size_t)items);
+    e->bucket = B.take<int>((size_t)items);
+    e->bcount = B.take<int>((size_t)e->T);
+    e->bstart = B.take<int>((size_t)e->T);
+    e->bcursor = B.take<int>((size_t)e->T);
+    e->items = B.take<int>((size_t)items);
+    long long scan_n = items > e->cap_cand ? items : e->cap_cand;
+    if (e->T > scan_n) scan_n = e->T;
+    if (e->capC > scan_n) scan_n = e->capC;
+    e->cand_n = B.take<int>((size_t)items);
+    e->cand_off = B.take<int>((size_t)items);
+    e->cand_tmp = B.take<int>(P > 0 ? 1 : (size_t)N * ME_MAXCAND);
+    e->cand_a = B.take<int>((size_t)e->cap_cand);
+    e->cand_b = B.take<int>((size_t)e->cap_cand);
+    size_t ns = P > 0 ? 1 : 4 * (size_t)e->cap_cand;
+    e->S.n = B.take<int>((size_t)e->cap_cand);
+    e->S.key = B.take<int64_t>(ns);
+    e->S.p = B.take<float>(3 * ns);
+    e->S.nrm = B.take<float>(3 * ns);
+    e->S.pen = B.take<float>(ns);
+    e->S.mu = B.take<float>(ns);
+    e->slot_off = B.take<int>((size_t)e->cap_cand);
+    e->pair_flag = B.take<int>((size_t)e->cap_cand);
+    e->pair_off = B.take<int>((size_t)e->cap_cand);
+    size_t C = (size_t)e->capC, Pc = (size_t)e->capP;
+    e->Cc.key = B.take<int64_t>(C);
+    e->Cc.bi = B.take<int>(C);
+    e->Cc.bj = B.take<int>(C);
+    e->Cc.pA = B.take<float>(3 * C);
+    e->Cc.pB = B.take<float>(3 * C);
+    e->Cc.n = B.take<float>(3 * C);
+    e->Cc.pen = B.take<float>(C);
+    e->Cc.mu = B.take<float>(C);
+    e->Cc.jn = B.take<float>(C);
+    e->Cc.jt1 = B.take<float>(C);
+    e->Cc.jt2 = B.take<float>(C);
+    e->Cc.jp = B.take<float>(C);
+    e->Pp.pbi = B.take<int>(Pc);
+    e->Pp.pbj = B.take<int>(Pc);
+    e->Pp.pstart = B.take<int>(Pc);
+    e->Pp.pcount = B.take<int>(Pc);
+    e->Pp.col = B.take<int>(Pc);
+    e->Pp.porder = B.take<int>(Pc);
+    e->cache_a = B.take<int>(Pc);
+    e->cache_b = B.take<int>(Pc);
+    e->cache_col = B.take<int>(Pc);
+    e->cache_invM = B.take<float>((size_t)N);
+    e->ccount = B.take<int>(ME_NCOLMAX + 1);
+    e->cstart = B.take<int>(ME_NCOLMAX + 1);
+    e->cfill = B.take<int>(ME_NCOLMAX + 1);
+    e->csub = B.take<int>(ME_NCOLMAX + 1);
+    e->cbase = B.take<int>(ME_NCOLMAX + 2);
+    e->wkey = B.take<int64_t>(C);
+    e->wjn = B.take<float>(C);
+    e->wjt1 = B.take<float>(C);
+    e->wjt2 = B.take<float>(C);
+    tiled_layout(e->tb, B, N, (int)Pc, (int)C);
+    e->nbsum = (int)((scan_n + SCAN_BLOCK - 1) / SCAN_BLOCK) + 1;
+    e->bsum = B.take<int>((size_t)e->nbsum);
+    e->istat = B.take<int>(IS_COUNT);
+    *bytes = (B.off + 255) & ~size_t(255);
+}
+
+int validate(const me_config *c) {
+    if (!c) return fail(ME_EARG, "me_config is NULL");
+    if (c->n_bodies <= 0) return fail(ME_EARG, "n_bodies must be positive");
+    if (c->solver < 0 || c->solver > 2) return fail(ME_EARG, "solver must be 0 (jacobi), 1 (pair chunks) or 2 (color cache)");
+    if (c->vit < 0 || c->pit < 0) return fail(ME_EARG, "iteration counts must be >= 0");
+    if (c->n_beads < 0) return fail(ME_EARG, "n_beads must be >= 0");
+    if (c->n_beads > 0 && c->solver != ME_SOLVER_JACOBI)
+        return fail(ME_EARG, "bead geometry is supported by the Jacobi solver only (the bead manifold reduction is not built)");
+    if (c->n_colors < 0 || c->n_colors > ME_NCOLMAX) return fail(ME_EARG, "n_colors must be in [0, 512]");
+    if ((long long)c->n_bodies * (c->n_beads > 0 ? c->n_beads : 1) > (1LL << 30))
+        return fail(ME_EARG, "too many bodies/beads");
+    if (!(c->margin >= 0.0f)) return fail(ME_EARG, "margin must be >= 0");
+    return ME_OK;
+}
+
+// exclusive scan in -> out over n elements (n_dev device count or n_host), total -> *total_dev
+int scan(me_engine *e, const int *in, int *out, const int *n_dev, int n_host, int cap, int *total_dev, hipStream_t s) {
+    int nb = (cap + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    if (nb < 1) nb = 1;
+    if (nb > e->nbsum) return fail(ME_ERUNTIME, "scan scratch too small");
+    if (nb > 1024 * 64) return fail(ME_ERUNTIME, "scan too large");
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(ME_THREADS), 0, s, in, n_dev, n_host, cap, e->bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, e->bsum, nb, total_dev);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(ME_THREADS), 0, s, in, out, n_dev, n_host, cap, e->bsum);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+Bodies bodies_of(me_engine *e) {
+    Bodies b;
+    b.x = e->b.x; b.q = e->b.q; b.v = e->b.v; b.w = e->b.w;
+    b.invM = e->b.inv_mass; b.IbInv = e->b.inv_inertia; b.half = e->b.half; b.mu = e->b.mu;
+    b.kin = e->b.kinematic; b.scene = e->b.scene; b.rest = e->b.rest;
+    return b;
+}
+
+int read_stat(me_engine *e, hipStream_t s) {
+    HIP_TRY(hipMemcpyAsync(e->hstat, e->istat, sizeof(int) * IS_COUNT, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ME_OK;
+}
+
+// grid over n items with positions pos; cell from IS_CELL (cell_fixed <= 0) or fixed
+int build_grid(me_engine *e, int n, const float *pos, const int *scene, int div, float cell_fixed, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(e->T)), dim3(ME_THREADS), 0, s, e->bcount, e->T, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(e->T)), dim3(ME_THREADS), 0, s, e->bcursor, e->T, 0);
+    hipLaunchKernelGGL(k_grid_count, dim3(nblk(n)), dim3(ME_THREADS), 0, s, n, pos, scene, div, e->istat, cell_fixed,
+                       e->T, e->cellc, e->bucket, e->bcount);
+    int rc = scan(e, e->bcount, e->bstart, nullptr, e->T, e->T, nullptr, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n)), dim3(ME_THREADS), 0, s, n, e->bucket, e->bstart, e->bcursor,
+                       e->items);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+// contact generation: box geometry (A6 body pairs + narrow phase) or bead geometry (A4 + A5)
+int generate(me_engine *e, hipStream_t s) {
+    Bodies b = bodies_of(e);
+    int N = e->N;
+    if (e->P > 0) {
+        int NP = e->NP, P = e->P;
+        float R = e->cfg.bead_radius;
+        hipLaunchKernelGGL(k_bead_points, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, N, P, b, e->allp);
+        int rc = build_grid(e, NP, e->allp, nullptr, 1, 2.0f * R, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_bead_contacts<false>, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, NP, P, e->allp, e->cellc,
+                           e->bstart, e->bcount, e->items, e->T, R, e->b.mu, e->cand_n, e->cand_off, e->capC, e->Cc,
+                           e->istat);
+        rc = scan(e, e->cand_n, e->cand_off, nullptr, NP, NP, e->istat + IS_C, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_bead_contacts<true>, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, NP, P, e->allp, e->cellc,
+                           e->bstart, e->bcount, e->items, e->T, R, e->b.mu, e->cand_n, e->cand_off, e->capC, e->Cc,
+                           e->istat);
+        hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP);
+        LAUNCH_CHECK();
+        return ME_OK;
+    }
+    hipLaunchKernelGGL(k_grid_cell, dim3(1), dim3(1), 0, s, e->istat, e->cfg.margin);
+    int rc = build_grid(e, N, e->b.x, e->b.scene, 1, -1.0f, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_box_cand, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cellc, e->bstart,
+                       e->bcount, e->items, e->T, e->cfg.margin, e->cand_n, e->cand_tmp, e->istat);
+    rc = scan(e, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_box_cand_emit, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->cand_n, e->cand_off,
+                       e->cand_tmp, e->cap_cand, e->cand_a, e->cand_b, e->istat);
+    int cc = e->cap_cand;
+    hipLaunchKernelGGL(k_narrow, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, b, e->Rw, e->cand_a, e->cand_b,
+                       e->cfg.margin, N, e->S);
+    rc = scan(e, e->S.n, e->slot_off, e->istat + IS_CAND, 0, cc, e->istat + IS_C, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pair_flag, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, e->S.n, e->pair_flag);
+    rc = scan(e, e->pair_flag, e->pair_off, e->istat + IS_CAND, 0, cc, e->istat + IS_P, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_compact, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->capC, e->capP, e->istat, e->cand_a,
+                       e->cand_b, e->S, e->slot_off, e->pair_off, e->Cc, e->Pp);
+    hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP);
+    if (e->cfg.flags & ME_FLAG_PER_CONTACT)
+        hipLaunchKernelGGL(k_pairs_per_contact, dim3(nblk(e->capC)), dim3(ME_THREADS), 0, s, e->capC, e->istat,
+                           e->Cc.bi, e->Cc.bj, e->Pp);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+// A12 relaxed Jacobi substep body (contact_engine_gpu.py:266-275)
+int solve_jacobi(me_engine *e, float sdt, hipStream_t s) {
+    int rc = read_stat(e, s);
+    if (rc) return rc;
+    int C = e->hstat[IS_C];
+    if (C <= 0) return ME_OK;
+    int N = e->N;
+    int gridN = nblk(N > C ? N : C);
+    hipLaunchKernelGGL(k_jac_reset, dim3(gridN), dim3(ME_THREADS), 0, s, N, e->capC, e->istat, e->pv, e->po, e->acc,
+                       e->Cc.jn, e->Cc.jt1, e->Cc.jt2, e->Cc.jp);
+    JacArgs A;
+    A.bi = e->Cc.bi; A.bj = e->Cc.bj; A.pA = e->Cc.pA; A.pB = e->Cc.pB; A.n = e->Cc.n; A.pen = e->Cc.pen;
+    A.mu = e->Cc.mu; A.jn = e->Cc.jn; A.jt1 = e->Cc.jt1; A.jt2 = e->Cc.jt2; A.jp = e->Cc.jp;
+    A.x = e->b.x; A.invM = e->b.inv_mass; A.invIw = e->invIw; A.v = e->b.v; A.w = e->b.w; A.pv = e->pv; A.po = e->po;
+    A.acc = e->acc; A.N = N;
+    const float S = 1.0e10f;
+    for (int it = 0; it < e->cfg.vit; ++it) {
+        hipLaunchKernelGGL(k_jac_vel, dim3(nblk(C)), dim3(ME_THREADS), 0, s, e->istat, A, S);
+        hipLaunchKernelGGL(k_jac_apply, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->b.v, e->b.w, e->acc,
+                           e->cfg.relax, S);
+    }
+    for (int it = 0; it < e->cfg.pit; ++it) {
+        hipLaunchKernelGGL(k_jac_pos, dim3(nblk(C)), dim3(ME_THREADS), 0, s, e->istat, A, sdt, S);
+        hipLaunchKernelGGL(k_jac_apply, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->pv, e->po, e->acc, e->cfg.relax, S);
+    }
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+SweepArgs sweep_args(me_engine *e) {
+    SweepArgs S;
+    S.B.v = e->b.v; S.B.w = e->b.w; S.B.pv = e->pv; S.B.po = e->po; S.B.x = e->b.x; S.B.invM = e->b.inv_mass;
+    S.B.invIw = e->invIw;
+    S.Cv.pA = e->Cc.pA; S.Cv.pB = e->Cc.pB; S.Cv.n = e->Cc.n; S.Cv.pen = e->Cc.pen; S.Cv.mu = e->Cc.mu;
+    S.Cv.jn = e->Cc.jn; S.Cv.jt1 = e->Cc.jt1; S.Cv.jt2 = e->Cc.jt2; S.Cv.jp = e->Cc.jp;
+    S.pbi = e->Pp.pbi; S.pbj = e->Pp.pbj; S.pstart = e->Pp.pstart; S.pcount = e->Pp.pcount; S.porder = e->Pp.porder;
+    S.cstart = e->cstart; S.ccount = e->ccount;
+    return S;
+}
+
+// JP colouring rounds with one read-back per round (contact_engine_gpu_colored.py:1178-1208)
+int color_jp(me_engine *e, int P, hipStream_t s) {
+    int capP = e->capP;
+    hipLaunchKernelGGL(k_jp_init, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->Pp.col);
+    int cur = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_fill_i64, dim3(nblk(e->N)), dim3(ME_THREADS), 0, s, (int64_t *)e->claim, e->N, INT64_MAX);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_LEFT, 1, 0);
+        hipLaunchKernelGGL(k_jp_claim, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->Pp.col, e->Pp.pbi,
+                           e->Pp.pbj, e->b.inv_mass, e->claim);
+        hipLaunchKernelGGL(k_jp_win, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, cur, e->Pp.col, e->Pp.pbi,
+                           e->Pp.pbj, e->b.inv_mass, e->claim);
+        LAUNCH_CHECK();
+        cur++;
+        int rc = read_stat(e, s);
+        if (rc) return rc;
+        if (e->hstat[IS_LEFT] == 0) break;
+        if (cur > P + 1) return fail(ME_ERUNTIME, "colouring did not terminate");
+    }
+    if (cur > ME_NCOLMAX) return fail(ME_ECAPACITY, "colour count exceeds the per-colour capacity 512");
+    // class sizes, optional split into n_colors classes, final sizes
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(ME_THREADS), 0, s, e->istat + IS_NCOL, 1, cur);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, e->ccount, ME_NCOLMAX + 1, 0);
+    hipLaunchKernelGGL(k_color_count, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->Pp.col, e->ccount);
+    if (e->cfg.n_colors > cur) {
+        hipLaunchKernelGGL(k_color_split_plan, dim3(1), dim3(1), 0, s, e->istat, e->cfg.n_colors, e->ccount, e->csub,
+                           e->cbase, ME_NCOLMAX);
+        hipLaunchKernelGGL(k_color_split_apply, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->Pp.col,
+                           e->csub, e->cbase);
+    }
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+// A17-A21 coloured pair-chunk solve: warm lookup, (cached) colouring, grouping, sweeps, warm store
+int solve_colored(me_engine *e, float sdt, hipStream_t s) {
+    bool cc_mode = e->cfg.solver == ME_SOLVER_COLOR_CACHE;
+    int capC = e->capC, capP = e->capP, N = e->N;
+    hipLaunchKernelGGL(k_zero_pvpo, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->pv, e->po);
+    int rc = read_stat(e, s);
+    if (rc) return rc;
+    int C = e->hstat[IS_C], P = e->hstat[IS_P];
+    if (C <= 0) {  // no contacts: warm store emptied (contact_engine_gpu_deferred_force.py:227-235)
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_WN, 1, 0);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_NCOL, 1, 0);
+        LAUNCH_CHECK();
+        return ME_OK;
+    }
+    hipLaunchKernelGGL(k_warm_lookup, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->cfg.warm_start,
+                       e->Cc.key, e->wkey, e->wjn, e->wjt1, e->wjt2, e->Cc.jn, e->Cc.jt1, e->Cc.jt2, e->Cc.jp);
+    bool hit = false;
+    if (cc_mode) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_MISMATCH, 1, 0);
+        int g = P > N ? P : N;
+        hipLaunchKernelGGL(k_cc_compare, dim3(nblk(g)), dim3(ME_THREADS), 0, s, N, e->istat, e->Pp.pbi, e->Pp.pbj,
+                           e->cache_a, e->cache_b, e->b.inv_mass, e->cache_invM);
+        LAUNCH_CHECK();
+        rc = read_stat(e, s);
+        if (rc) return rc;
+        hit = e->hstat[IS_MISMATCH] == 0;
+    }
+    if (hit) {
+        hipLaunchKernelGGL(k_cc_restore, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->cache_col,
+                           e->Pp.col);
+    } else {
+        rc = color_jp(e, P, s);
+        if (rc) return rc;
+    }
+    if (cc_mode) {
+        int g = P > N ? P : N;
+        hipLaunchKernelGGL(k_cc_store, dim3(nblk(g)), dim3(ME_THREADS), 0, s, N, capP, e->istat, e->Pp.pbi, e->Pp.pbj,
+                           e->Pp.col, e->b.inv_mass, e->cache_a, e->cache_b, e->cache_col, e->cache_invM, hit ? 1 : 0);
+    }
+    rc = read_stat(e, s);
+    if (rc) return rc;
+    int ncol = e->hstat[IS_NCOL];
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, e->ccount, ME_NCOLMAX + 1, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, e->cfill, ME_NCOLMAX + 1, 0);
+    hipLaunchKernelGGL(k_color_count, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->Pp.col, e->ccount);
+    rc = scan(e, e->ccount, e->cstart, nullptr, ncol, ME_NCOLMAX + 1, nullptr, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_color_scatter, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->Pp.col, e->cstart,
+                       e->cfill, e->Pp.porder);
+    SweepArgs S = sweep_args(e);
+    dim3 g(nblk(P)), t(ME_THREADS);
+    if (cc_mode)
+        for (int ci = 0; ci < ncol; ++ci) hipLaunchKernelGGL(k_sweep_warm, g, t, 0, s, S, ci);
+    for (int it = 0; it < e->cfg.vit; ++it)
+        for (int ci = 0; ci < ncol; ++ci) {
+            if (cc_mode) hipLaunchKernelGGL(k_sweep_vel<true>, g, t, 0, s, S, ci, 1.0f);
+            else hipLaunchKernelGGL(k_sweep_vel<false>, g, t, 0, s, S, ci, e->cfg.t2d);
+        }
+    for (int it = 0; it < e->cfg.pit; ++it)
+        for (int ci = 0; ci < ncol; ++ci) hipLaunchKernelGGL(k_sweep_pos, g, t, 0, s, S, ci, sdt);
+    if (e->cfg.warm_start)
+        hipLaunchKernelGGL(k_warm_store, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->Cc.key, e->Cc.jn,
+                           e->Cc.jt1, e->Cc.jt2, e->wkey, e->wjn, e->wjt1, e->wjt2);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+int substep(me_engine *e, double sdt64, hipStream_t s) {
+    float sdt = (float)sdt64;
+    int N = e->N;
+    Bodies b = bodies_of(e);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_MAXEXT, 1, 0);
+    hipLaunchKernelGGL(k_body_pre, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->invIw, e->Rw, e->aabb, e->vsnap,
+                       e->istat, sdt, e->P == 0 ? 1 : 0);
+    LAUNCH_CHECK();
+    int rc = generate(e, s);
+    if (rc) return rc;
+    if (e->cfg.solver == ME_SOLVER_JACOBI) {
+        rc = solve_jacobi(e, sdt, s);
+    } else if (e->cfg.flags & ME_FLAG_TILED) {
+        rc = tiled_solve(e->tb, tiled_args_of(e, sdt), s);
+    } else {
+        rc = solve_colored(e, sdt, s);
+    }
+    if (rc) return rc;
+    if (!(e->cfg.flags & ME_FLAG_TILED) || e->cfg.solver == ME_SOLVER_JACOBI)
+        hipLaunchKernelGGL(k_integrate, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->pv, e->po, sdt);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+}  // namespace
+
+// tiled-solve argument bundle (declared in tiled.hpp)
+TiledArgs tiled_args_of(me_engine *e, float sdt) {
+    TiledArgs a;
+    a.N = e->N; a.capC = e->capC; a.capP = e->capP;
+    a.solver = e->cfg.solver; a.vit = e->cfg.vit; a.pit = e->cfg.pit; a.t2d = e->cfg.t2d;
+    a.warm = e->cfg.warm_start; a.n_colors = e->cfg.n_colors; a.tile_bodies = e->cfg.tile_bodies;
+    a.dt = sdt;
+    a.b = bodies_of(e);
+    a.invIw = e->invIw; a.pv = e->pv; a.po = e->po;
+    a.Cc = e->Cc; a.Pp = e->Pp;
+    a.cache_a = e->cache_a; a.cache_b = e->cache_b; a.cache_col = e->cache_col; a.cache_invM = e->cache_invM;
+    a.wkey = e->wkey; a.wjn = e->wjn; a.wjt1 = e->wjt1; a.wjt2 = e->wjt2;
+    a.istat = e->istat;
+    a.bsum = e->bsum; a.nbsum = e->nbsum;
+    return a;
+}
+
+extern "C" {
+
+int me_workspace_size(const me_config *cfg, size_t *bytes) {
+    int rc = validate(cfg);
+    if (rc) return rc;
+    if (!bytes) return fail(ME_EARG, "bytes is NULL");
+    me_engine tmp{};
+    tmp.cfg = *cfg;
+    layout(&tmp, nullptr, bytes);
+    return ME_OK;
+}
+
+int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, size_t bytes, void *stream,
+              me_engine **out) {
+    int rc = validate(cfg);
+    if (rc) return rc;
+    if (!bodies || !out || !workspace) return fail(ME_EARG, "NULL argument");
+    if (((uintptr_t)workspace & 255) != 0) return fail(ME_EARG, "workspace must be 256-byte aligned");
+    if (!bodies->x || !bodies->q || !bodies->v || !bodies->w || !bodies->inv_mass || !bodies->inv_inertia ||
+        !bodies->mu || !bodies->kinematic || !bodies->scene)
+        return fail(ME_EARG, "body array pointer is NULL");
+    if (cfg->n_beads > 0 && !bodies->rest) return fail(ME_EARG, "bead geometry needs rest positions");
+    if (cfg->n_beads == 0 && !bodies->half) return fail(ME_EARG, "box geometry needs half extents");
+    size_t need = 0;
+    me_engine *e = new me_engine{};
+    e->cfg = *cfg;
+    layout(e, nullptr, &need);
+    if (bytes < need) {
+        delete e;
+        return fail(ME_EARG, "workspace too small: need " + std::to_string(need) + " bytes");
+    }
+    layout(e, (char *)workspace, &need);
+    e->b = *bodies;
+    e->owner = std::this_thread::get_id();
+    if (hipHostMalloc((void **)&e->hstat, sizeof(int) * IS_COUNT, hipHostMallocDefault) != hipSuccess) {
+        delete e;
+        return fail(ME_ERUNTIME, "hipHostMalloc failed");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // zero the workspace with a fill kernel (no memset nodes, so a caller may capture later steps)
+    size_t words = need / 4;
+    for (size_t off = 0; off < words; off += (size_t)1 << 30) {
+        int n = (int)((words - off) < ((size_t)1 << 30) ? (words - off) : ((size_t)1 << 30));
+        hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(ME_THREADS), 0, s, (int *)workspace + off, n, 0);
+    }
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_CACHED_P, 1, -1);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        hipHostFree(e->hstat);
+        delete e;
+        return fail(ME_ERUNTIME, "workspace initialisation failed");
+    }
+    *out = e;
+    return ME_OK;
+}
+
+int me_destroy(me_engine *e) {
+    if (!e) return fail(ME_EARG, "engine is NULL");
+    if (e->hstat) hipHostFree(e->hstat);
+    delete e;
+    return ME_OK;
+}
+
+int me_set_bodies(me_engine *e, const me_bodies *bodies) {
+    if (!e || !bodies) return fail(ME_EARG, "NULL argument");
+    if (std::this_thread::get_id() != e->owner) return fail(ME_ETHREAD, "engine used from a thread that does not own it");
+    e->b = *bodies;
+    return ME_OK;
+}
+
+int me_step(me_engine *e, double dt, int substeps, void *stream) {
+    if (!e) return fail(ME_EARG, "engine is NULL");
+    if (std::this_thread::get_id() != e->owner) return fail(ME_ETHREAD, "engine used from a thread that does not own it");
+    if (substeps < 1) return fail(ME_EARG, "substeps must be >= 1");
+    if (!(dt > 0.0) || dt != dt) return fail(ME_EARG, "dt must be positive and finite");
+    hipStream_t s = (hipStream_t)stream;
+    double sdt = dt / substeps;
+    for (int k = 0; k < substeps; ++k) {
+        int rc = substep(e, sdt, s);
+        if (rc) return rc;
+    }
+    return ME_OK;
+}
+
+int me_buffer(me_engine *e, int id, void **ptr, int64_t *count) {
+    if (!e || !ptr || !count) return fail(ME_EARG, "NULL argument");
+    int64_t C = e->capC, P = e->capP, N = e->N;
+    switch (id) {
+        case ME_BUF_KEY: *ptr = e->Cc.key; *count = C; break;
+        case ME_BUF_BI: *ptr = e->Cc.bi; *count = C; break;
+        case ME_BUF_BJ: *ptr = e->Cc.bj; *count = C; break;
+        case ME_BUF_PA: *ptr = e->Cc.pA; *count = 3 * C; break;
+        case ME_BUF_PB: *ptr = e->Cc.pB; *count = 3 * C; break;
+        case ME_BUF_N: *ptr = e->Cc.n; *count = 3 * C; break;
+        case ME_BUF_PEN: *ptr = e->Cc.pen; *count = C; break;
+        case ME_BUF_JN: *ptr = e->Cc.jn; *count = C; break;
+        case ME_BUF_JT1: *ptr = e->Cc.jt1; *count = C; break;
+        case ME_BUF_JT2: *ptr = e->Cc.jt2; *count = C; break;
+        case ME_BUF_JP: *ptr = e->Cc.jp; *count = C; break;
+        case ME_BUF_PBI: *ptr = e->Pp.pbi; *count = P; break;
+        case ME_BUF_PBJ: *ptr = e->Pp.pbj; *count = P; break;
+        case ME_BUF_PSTART: *ptr = e->Pp.pstart; *count = P; break;
+        case ME_BUF_PCOUNT: *ptr = e->Pp.pcount; *count = P; break;
+        case ME_BUF_COLOR: *ptr = e->Pp.col; *count = P; break;
+        case ME_BUF_VSNAP: *ptr = e->vsnap; *count = 3 * N; break;
+        case ME_BUF_PV: *ptr = e->pv; *count = 3 * N; break;
+        case ME_BUF_PO: *ptr = e->po; *count = 3 * N; break;
+        case ME_BUF_INVIW: *ptr = e->invIw; *count = 9 * N; break;
+        case ME_BUF_ISLAND: *ptr = e->tb.root; *count = N; break;
+        default: return fail(ME_EARG, "unknown buffer id");
+    }
+    return ME_OK;
+}
+
+int me_read_counters(me_engine *e, int64_t *out, void *stream) {
+    if (!e || !out) return fail(ME_EARG, "NULL argument");
+    int rc = read_stat(e, (hipStream_t)stream);
+    if (rc) return rc;
+    const int *h = e->hstat;
+    out[ME_CNT_CONTACTS] = h[IS_C];
+    out[ME_CNT_PAIRS] = h[IS_P];
+    out[ME_CNT_COLORS] = h[IS_NCOL];
+    out[ME_CNT_CACHE_HITS] = h[IS_HITS];
+    out[ME_CNT_CACHE_MISSES] = h[IS_MISSES];
+    out[ME_CNT_OVERFLOW] = h[IS_OVF];
+    out[ME_CNT_CANDIDATES] = h[IS_CAND];
+    out[ME_CNT_ISLANDS] = h[IS_ISL];
+    out[ME_CNT_TILES] = h[IS_TILES];
+    out[ME_CNT_MAX_TILE_BODIES] = h[IS_MAXTB];
+    return ME_OK;
+}
+
+const char *me_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// Device vector math with the reference kernels' operation order (Warp builtins as used in
+// src/motion_engine/contact_engine_gpu*.py). Compiled with -ffp-contract=off and IEEE div/sqrt,
+// so every expression below rounds exactly as written:
+//   dot(a,b)   = a.x*b.x + a.y*b.y + a.z*b.z (left to right)
+//   cross(a,b) = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)
+//   M*v        = row i: (M[i][0]*v.x + M[i][1]*v.y) + M[i][2]*v.z
+//   M*N        = element ij: ((0 + M[i][0]*N[0][j]) + M[i][1]*N[1][j]) + M[i][2]*N[2][j]
+//   quat_rotate(q,x) = x*(2w*w-1) + qv*(2*(qv.x)) + cross(qv,x)*w*2; quat_to_matrix = its columns
+//   quat normalize = q * (1/|q|); vector normalize = v / |v|
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace me {
+
+struct f3 { float x, y, z; };
+struct f4 { float x, y, z, w; };
+struct m3 { float m[3][3]; };
+
+__host__ __device__ __forceinline__ f3 mk(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+__host__ __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ __forceinline__ f3 operator-(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__host__ __device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ __forceinline__ f3 operator*(float s, f3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+__host__ __device__ __forceinline__ f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__host__ __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__host__ __device__ __forceinline__ f3 normalize(f3 a) {
+    float l = length(a);
+    if (l > 0.0f) return a / l;
+    return mk(0.0f, 0.0f, 0.0f);
+}
+// max(a, b) = a > b ? a : b (explicit: same signed-zero and NaN behaviour as the oracle)
+__host__ __device__ __forceinline__ float wmax(float a, float b) { return a > b ? a : b; }
+
+__host__ __device__ __forceinline__ f3 mv(const m3 &M, f3 b) {
+    return mk(M.m[0][0] * b.x + M.m[0][1] * b.y + M.m[0][2] * b.z,
+              M.m[1][0] * b.x + M.m[1][1] * b.y + M.m[1][2] * b.z,
+              M.m[2][0] * b.x + M.m[2][1] * b.y + M.m[2][2] * b.z);
+}
+__host__ __device__ __forceinline__ m3 mm(const m3 &A, const m3 &B) {
+    m3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s += A.m[i][k] * B.m[k][j];
+            r.m[i][j] = s;
+        }
+    return r;
+}
+__host__ __device__ __forceinline__ m3 transpose(const m3 &A) {
+    m3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = A.m[j][i];
+    return r;
+}
+__host__ __device__ __forceinline__ f3 col(const m3 &R, int i) { return mk(R.m[0][i], R.m[1][i], R.m[2][i]); }
+
+__host__ __device__ __forceinline__ f3 quat_rotate(f4 q, f3 x) {
+    float c = 2.0f * q.w * q.w - 1.0f;
+    float d = 2.0f * (q.x * x.x + q.y * x.y + q.z * x.z);
+    f3 qv = mk(q.x, q.y, q.z);
+    return (x * c + qv * d) + (cross(qv, x) * q.w) * 2.0f;
+}
+__host__ __device__ __forceinline__ m3 quat_to_matrix(f4 q) {
+    f3 c1 = quat_rotate(q, mk(1.0f, 0.0f, 0.0f));
+    f3 c2 = quat_rotate(q, mk(0.0f, 1.0f, 0.0f));
+    f3 c3 = quat_rotate(q, mk(0.0f, 0.0f, 1.0f));
+    m3 R;
+    R.m[0][0] = c1.x; R.m[1][0] = c1.y; R.m[2][0] = c1.z;
+    R.m[0][1] = c2.x; R.m[1][1] = c2.y; R.m[2][1] = c2.z;
+    R.m[0][2] = c3.x; R.m[1][2] = c3.y; R.m[2][2] = c3.z;
+    return R;
+}
+__host__ __device__ __forceinline__ f4 qmul(f4 a, f4 b) {
+    f4 r;
+    r.x = a.w * b.x + b.w * a.x + a.y * b.z - b.y * a.z;
+    r.y = a.w * b.y + b.w * a.y + a.z * b.x - b.z * a.x;
+    r.z = a.w * b.z + b.w * a.z + a.x * b.y - b.x * a.y;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    return r;
+}
+__host__ __device__ __forceinline__ f4 qnormalize(f4 q) {
+    float l = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    f4 r;
+    if (l > 0.0f) {
+        float inv = 1.0f / l;
+        r.x = q.x * inv; r.y = q.y * inv; r.z = q.z * inv; r.w = q.w * inv;
+    } else {
+        r.x = 0.0f; r.y = 0.0f; r.z = 0.0f; r.w = 1.0f;
+    }
+    return r;
+}
+
+// SoA loads / stores
+__device__ __forceinline__ f3 ld3(const float *p, int i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void st3(float *p, int i, f3 a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
+__device__ __forceinline__ f4 ld4(const float *p, int i) {
+    f4 r; r.x = p[4 * i]; r.y = p[4 * i + 1]; r.z = p[4 * i + 2]; r.w = p[4 * i + 3]; return r;
+}
+__device__ __forceinline__ m3 ldm(const float *p, int i) {
+    m3 r;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r.m[k / 3][k % 3] = p[9 * i + k];
+    return r;
+}
+__device__ __forceinline__ void stm(float *p, int i, const m3 &M) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) p[9 * i + k] = M.m[k / 3][k % 3];
+}
+
+}  // namespace me

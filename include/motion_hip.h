@@ -1,0 +1,153 @@
+/* motion_hip — MI355X (gfx950) contact step behind the reference ContactEngine protocol.
+ *
+ * C ABI of libmotion_hip.so. Plain pointers and sizes only. Every device buffer belongs to the
+ * caller: the body state arrays (me_bodies) and one workspace block whose size me_workspace_size
+ * reports. The library itself calls no hipMalloc; it keeps a small pinned host block for counter
+ * read-backs. Return codes follow the reference's only C ABI
+ * (src/motion_engine/body_pairs_rt_native/api.cpp:130-163): 0 ok, 1 bad argument, 2 runtime
+ * failure, 3 wrong thread, plus 4 capacity exceeded and 5 non-finite state.
+ *
+ * Reference interfaces replaced (antonbj3/3fold-motion-engine, src/motion_engine/...):
+ *   me_create / me_destroy     the engine constructors + _build
+ *                              (contact_engine_gpu.py:195-239, contact_engine_gpu_colored.py:798-889,
+ *                               contact_engine_gpu_color_cache.py:21-35)
+ *   me_step                    ContactEngine.step(dt, substeps) (contact_engine.py:52;
+ *                              contact_engine_gpu.py:252-280, contact_engine_gpu_colored.py:1302-1375,
+ *                              contact_engine_gpu_deferred_force.py:13-64)
+ *   me_buffer                  the harness-visible solver arrays (ckeys, sbi, sbj, spA, spB, sn, spen,
+ *                              jn, jt1, jt2, jp; tests/test_island_partition.py:70-71)
+ *   me_read_counters           n_contacts, n_contacts_solved, n_pairs, n_colors, color_cache_hits/misses
+ *   me_last_error              the exception text of the reference's ValueError / RuntimeError
+ * get_state, set_kinematic and contact_forces (contact_engine.py:53-55) read or write the caller's
+ * body tensors directly; contact_forces uses the ME_BUF_VSNAP snapshot.
+ */
+#ifndef MOTION_HIP_H
+#define MOTION_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ME_OK 0
+#define ME_EARG 1
+#define ME_ERUNTIME 2
+#define ME_ETHREAD 3
+#define ME_ECAPACITY 4
+#define ME_ENONFINITE 5
+
+#define ME_SOLVER_JACOBI 0       /* RelaxedJacobiContactEngine (contact_engine_gpu.py:191) */
+#define ME_SOLVER_PAIR_CHUNKS 1  /* GraphColoredContactEngine, pair_chunks=True (contact_engine_gpu_colored.py:793) */
+#define ME_SOLVER_COLOR_CACHE 2  /* ColorCacheContactEngine (contact_engine_gpu_color_cache.py:21) */
+
+#define ME_FLAG_TILED 1          /* solve islands in LDS-resident tiles (one launch per substep) */
+#define ME_FLAG_PER_CONTACT 2    /* coloured solver without pair chunks: every contact its own chunk
+                                    (GraphColoredContactEngine pair_chunks=False) */
+
+typedef struct me_config {
+    int32_t solver;        /* ME_SOLVER_* */
+    int32_t n_bodies;      /* N */
+    int32_t n_beads;       /* 0 = box geometry; P > 0 = bead geometry (the reference's body model) */
+    int32_t vit, pit;      /* velocity / position iterations */
+    float relax;           /* Jacobi relaxation (reference 0.2) */
+    float t2d;             /* pair-chunk friction t2 factor (reference 1.0) */
+    int32_t warm_start;    /* coloured solvers: carry impulses by feature key */
+    float bead_radius;     /* bead geometry: R (reference 0.05) */
+    float margin;          /* box geometry: contact margin (0 = penetrating points only) */
+    int32_t max_pairs;     /* capacity: candidate body pairs (box) */
+    int32_t max_contacts;  /* capacity: contacts */
+    int32_t n_colors;      /* 0 = Jones-Plassmann colouring as is; K > 0: split to max(K, JP) classes */
+    int32_t tile_bodies;   /* tiled solve: nominal bodies per tile */
+    int32_t flags;         /* ME_FLAG_* */
+} me_config;
+
+/* Body state, device pointers, SoA float32 (the caller's tensors). */
+typedef struct me_bodies {
+    float *x;              /* [N,3] centre of mass */
+    float *q;              /* [N,4] orientation, xyzw */
+    float *v;              /* [N,3] linear velocity */
+    float *w;              /* [N,3] angular velocity */
+    float *inv_mass;       /* [N]   0 for kinematic bodies */
+    float *inv_inertia;    /* [N,3,3] body-frame inverse inertia, row-major */
+    float *half;           /* [N,3] box half extents (box geometry) */
+    float *mu;             /* [N]   friction coefficient */
+    int32_t *kinematic;    /* [N]   1 = kinematic */
+    int32_t *scene;        /* [N]   scene id; bodies of different scenes never touch */
+    float *rest;           /* [P,3] bead rest positions (bead geometry) */
+} me_bodies;
+
+typedef struct me_engine me_engine;
+
+/* buffer ids for me_buffer */
+enum {
+    ME_BUF_KEY = 0,    /* int64 [C]   feature key per contact (sorted) */
+    ME_BUF_BI,         /* int32 [C]   body A */
+    ME_BUF_BJ,         /* int32 [C]   body B (-1 = ground) */
+    ME_BUF_PA,         /* float [C,3] contact point on A */
+    ME_BUF_PB,         /* float [C,3] contact point on B */
+    ME_BUF_N,          /* float [C,3] normal, B -> A */
+    ME_BUF_PEN,        /* float [C]   penetration */
+    ME_BUF_JN,         /* float [C]   accumulated normal impulse */
+    ME_BUF_JT1,        /* float [C] */
+    ME_BUF_JT2,        /* float [C] */
+    ME_BUF_JP,         /* float [C]   accumulated position impulse */
+    ME_BUF_PBI,        /* int32 [Pp]  pair body A */
+    ME_BUF_PBJ,        /* int32 [Pp]  pair body B */
+    ME_BUF_PSTART,     /* int32 [Pp]  first contact of the pair */
+    ME_BUF_PCOUNT,     /* int32 [Pp]  contacts of the pair */
+    ME_BUF_COLOR,      /* int32 [Pp]  colour of the pair */
+    ME_BUF_VSNAP,      /* float [N,3] velocity after gravity, before the contact solve */
+    ME_BUF_PV,         /* float [N,3] pseudo-velocity of the last substep */
+    ME_BUF_PO,         /* float [N,3] pseudo angular velocity */
+    ME_BUF_INVIW,      /* float [N,3,3] world inverse inertia */
+    ME_BUF_ISLAND,     /* int32 [N]   island root (tiled solve) */
+    ME_BUF_COUNT_
+};
+
+/* counters for me_read_counters */
+enum {
+    ME_CNT_CONTACTS = 0, /* contacts generated in the last substep */
+    ME_CNT_PAIRS,        /* body pairs with contacts */
+    ME_CNT_COLORS,       /* colours of the last solve */
+    ME_CNT_CACHE_HITS,
+    ME_CNT_CACHE_MISSES,
+    ME_CNT_OVERFLOW,     /* capacity overflow flags (bit 0 contacts, 1 pairs, 2 candidates) */
+    ME_CNT_CANDIDATES,   /* broad-phase candidate pairs */
+    ME_CNT_ISLANDS,      /* islands of the last tiled solve */
+    ME_CNT_TILES,        /* tiles of the last tiled solve */
+    ME_CNT_MAX_TILE_BODIES,
+    ME_CNT_COUNT_
+};
+
+/* Workspace bytes for a configuration (no device work). */
+int me_workspace_size(const me_config *cfg, size_t *bytes);
+
+/* Create an engine over caller-owned body arrays and a caller-owned workspace of at least
+ * me_workspace_size bytes (256-byte aligned). Zeroes the workspace on `stream`. */
+int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, size_t bytes,
+              void *stream, me_engine **out);
+int me_destroy(me_engine *e);
+
+/* Re-point the engine at (possibly re-allocated) body arrays; same N. */
+int me_set_bodies(me_engine *e, const me_bodies *bodies);
+
+/* Advance dt in `substeps` equal substeps (ContactEngine.step). Work is queued on `stream`
+ * (a hipStream_t; NULL = the null stream). Returns after queueing; counters and the overflow
+ * flag are read with me_read_counters. */
+int me_step(me_engine *e, double dt, int substeps, void *stream);
+
+/* Device pointer and element count of an internal buffer (a view into the workspace). */
+int me_buffer(me_engine *e, int id, void **ptr, int64_t *count);
+
+/* Copy ME_CNT_COUNT_ counters to host memory (synchronises `stream`). */
+int me_read_counters(me_engine *e, int64_t *out, void *stream);
+
+/* Text of the last error on the calling thread. */
+const char *me_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

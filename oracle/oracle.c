@@ -319,7 +319,7 @@ static void bead_contacts(or_world *w) {
         st3(w->allp, g, p);
     }
     float h = 2.0f * R;
-    cellrec *cr = (cellrec *)malloc(sizeof(cellrec) * (size_t)(NP ? NP : 1));
+    cellrec *cr = (cellrec *)malloc(sizeof(cellrec) * (NP > 0 ? (size_t)NP : (size_t)1));
     for (int g = 0; g < NP; ++g) {
         v3 p = ld3(w->allp, g);
         cr[g].cell = cell_key((int64_t)floorf(p.x / h), (int64_t)floorf(p.y / h), (int64_t)floorf(p.z / h), 0);
@@ -621,7 +621,7 @@ static void box_contacts(or_world *w) {
             float e = w->aabb[6 * i + 3 + k] - w->x[3 * i + k];
             if (e > hmax) hmax = e;
         }
-    float cell = 2.0f * hmax + margin + BP_SLACK;
+    float cell = (2.0f * hmax + margin + BP_SLACK) * 1.001f;
     if (!(cell > 0.0f)) cell = 1.0f;
     cellrec *cr = (cellrec *)malloc(sizeof(cellrec) * (size_t)(N ? N : 1));
     for (int i = 0; i < N; ++i) {
@@ -637,7 +637,7 @@ static void box_contacts(or_world *w) {
         m33 RA = ldm(w->Rw, a);
         v3 cA = ld3(w->x, a), hA = ld3(w->half, a);
         v3 p[4], nn[4]; float pen[4]; int code[4];
-        if (w->aabb[6 * a + 2] < margin + BP_SLACK) {
+        if (w->invM[a] > 0.0f && w->aabb[6 * a + 2] < margin + BP_SLACK) {
             int k = box_ground_impl(cA, &RA, hA, margin, p, pen, code);
             int64_t pk = (int64_t)a * N1 + 0;
             for (int i = 0; i < k; ++i)
@@ -815,6 +815,13 @@ static void jac_apply(or_world *w, float *vv, float *ww, float relax, float scal
  * contiguous run of equal (bi, bj). */
 static void build_pairs(or_world *w) {
     w->Pp = 0;
+    if (w->prm.per_contact) { /* contact_engine_gpu_colored.py:1105-1136: the contact graph */
+        for (int c = 0; c < w->C; ++c) {
+            w->pstart[c] = c; w->pcount[c] = 1; w->pbi[c] = w->cbi[c]; w->pbj[c] = w->cbj[c];
+        }
+        w->Pp = w->C;
+        return;
+    }
     for (int c = 0; c < w->C; ++c) {
         if (c == 0 || w->cbi[c] != w->cbi[c - 1] || w->cbj[c] != w->cbj[c - 1]) {
             int p = w->Pp++;
@@ -867,8 +874,8 @@ int64_t or_jp_key(int rank) {
 }
 
 /* A17 JP colouring of the pair graph (k_claim / k_win, contact_engine_gpu_colored.py:151-185,
- * rounds at :1178-1208); then, for n_colors = K > 0, every JP class is split into sub-colours
- * (round robin over its pairs in rank order) until there are max(K, JP colours) classes: the extra
+ * rounds at :1178-1208); then, for n_colors = K > 0, every JP class c is split into sub[c]
+ * sub-colours (pair rank mod sub[c]) so that there are max(K, JP colours) classes: the extra
  * classes go one at a time to the class with the most pairs per sub-colour (lowest class first). */
 static int color_pairs(or_world *w) {
     int P = w->Pp, N = w->N;
@@ -901,7 +908,7 @@ static int color_pairs(or_world *w) {
     int K = w->prm.n_colors;
     if (K > cur && cur > 0) {
         int *cnt = (int *)calloc((size_t)cur, sizeof(int)), *sub = (int *)calloc((size_t)cur, sizeof(int));
-        int *base = (int *)calloc((size_t)cur + 1, sizeof(int)), *rk = (int *)calloc((size_t)cur, sizeof(int));
+        int *base = (int *)calloc((size_t)cur + 1, sizeof(int));
         for (int p = 0; p < P; ++p) cnt[w->col[p]]++;
         for (int c = 0; c < cur; ++c) sub[c] = 1;
         for (int e = cur; e < K; ++e) {
@@ -914,11 +921,10 @@ static int color_pairs(or_world *w) {
         for (int c = 0; c < cur; ++c) base[c + 1] = base[c] + sub[c];
         for (int p = 0; p < P; ++p) {
             int c = w->col[p];
-            w->col[p] = base[c] + rk[c] % sub[c];
-            rk[c]++;
+            w->col[p] = base[c] + p % sub[c];
         }
         cur = base[cur];
-        free(cnt); free(sub); free(base); free(rk);
+        free(cnt); free(sub); free(base);
     }
     return cur;
 }

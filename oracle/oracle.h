@@ -40,6 +40,7 @@ typedef struct or_params {
     float margin;      /* box contact margin (0 = penetrating points only) */
     int cap;           /* contact capacity */
     int n_colors;      /* 0 = JP colouring as is; K > 0 = split into max(K, JP colours) classes */
+    int per_contact;   /* coloured solver without pair chunks: every contact its own chunk */
 } or_params;
 
 typedef struct or_world or_world;

@@ -19,7 +19,8 @@ SOLVERS = {"jacobi": JACOBI, "pair_chunks": PAIR_CHUNKS, "color_cache": COLOR_CA
 class Params(C.Structure):
     _fields_ = [("solver", C.c_int), ("vit", C.c_int), ("pit", C.c_int), ("relax", C.c_float),
                 ("t2d", C.c_float), ("warm_start", C.c_int), ("R", C.c_float),
-                ("margin", C.c_float), ("cap", C.c_int), ("n_colors", C.c_int)]
+                ("margin", C.c_float), ("cap", C.c_int), ("n_colors", C.c_int),
+                ("per_contact", C.c_int)]
 
 
 def build() -> Path:
@@ -83,14 +84,14 @@ class OracleWorld:
     """One simulated world on the CPU restatement."""
 
     def __init__(self, bodies, solver="jacobi", vit=40, pit=20, relax=0.2, t2d=1.0,
-                 warm_start=True, R=0.05, margin=0.0, cap=None, n_colors=0):
+                 warm_start=True, R=0.05, margin=0.0, cap=None, n_colors=0, per_contact=False):
         L = lib()
         N = bodies.n
         P = int(len(bodies.rest))
         if cap is None:
             cap = max(1024, 4 * N * 8) if P == 0 else max(1024, N * P * 8)
         prm = Params(SOLVERS[solver], vit, pit, relax, t2d, int(warm_start), R, margin, cap,
-                     n_colors)
+                     n_colors, int(per_contact))
         self._prm = prm
         self.N, self.P, self.cap = N, P, cap
         self.h = L.or_create(N, P, C.byref(prm))
