@@ -57,9 +57,11 @@ def lib():
         L.me_step.argtypes = [vp, C.c_double, C.c_int, vp]
         L.me_buffer.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int64)]
         L.me_read_counters.argtypes = [vp, C.POINTER(C.c_int64), vp]
+        L.me_profile.argtypes = [vp, C.c_int]
+        L.me_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.me_last_error.restype = C.c_char_p
         for f in ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
-                  "me_buffer", "me_read_counters"):
+                  "me_buffer", "me_read_counters", "me_profile", "me_profile_read"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -75,4 +77,5 @@ def check(rc: int, what: str) -> None:
 
 
 EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
-            "me_buffer", "me_read_counters", "me_last_error")
+            "me_buffer", "me_read_counters", "me_profile", "me_profile_read", "me_last_error")
+K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL = range(5)

@@ -10,7 +10,7 @@ namespace me {
 enum {
     IS_C = 0, IS_P, IS_CAND, IS_NCOL, IS_HITS, IS_MISSES, IS_OVF, IS_ISL, IS_TILES, IS_MAXTB,
     IS_MAXEXT, IS_LEFT, IS_WN, IS_CACHED_P, IS_CACHED_NCOL, IS_MISMATCH, IS_CELL, IS_NBODYSORT,
-    IS_NPAIRSORT, IS_COUNT = 32
+    IS_NPAIRSORT, IS_NCOLJP, IS_COUNT = 32
 };
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16 };
 
@@ -768,20 +768,20 @@ __global__ void k_sweep_warm(SweepArgs S, int ci) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.ccount[ci]) return;
     int p = S.porder[S.cstart[ci] + t];
-    apply_warm_pair(S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p]);
+    apply_warm_pair(S.B, S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p]);
 }
 template <bool FIXED>
 __global__ void k_sweep_vel(SweepArgs S, int ci, float t2d) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.ccount[ci]) return;
     int p = S.porder[S.cstart[ci] + t];
-    chunk_vel<FIXED>(S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p], t2d);
+    chunk_vel<FIXED>(S.B, S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p], t2d);
 }
 __global__ void k_sweep_pos(SweepArgs S, int ci, float dt) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.ccount[ci]) return;
     int p = S.porder[S.cstart[ci] + t];
-    chunk_pos(S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p], dt);
+    chunk_pos(S.B, S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p], dt);
 }
 
 __global__ void k_zero_pvpo(int N, float *pv, float *po) {

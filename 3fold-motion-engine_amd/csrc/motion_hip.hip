@@ -8,6 +8,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include "kernels.hpp"
 #include "tiled.hpp"
@@ -90,9 +91,27 @@ struct me_engine {
     int nbsum;
     int *istat;
     int *hstat;  // pinned host mirror
+    // HIP-event timing of selected kernels (measurement only)
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_ev[ME_K_COUNT_];
+    int prof_used[ME_K_COUNT_] = {0};
 };
 
 namespace {
+
+const int kProfMax = 8192;
+
+// event pair around one launch of kernel `id` on stream s (first kProfMax launches)
+void prof_mark(me_engine *e, int id, hipStream_t s) {
+    if (!e->prof_on || e->prof_used[id] >= 2 * kProfMax) return;
+    std::vector<hipEvent_t> &v = e->prof_ev[id];
+    if ((int)v.size() <= e->prof_used[id]) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) return;
+        v.push_back(ev);
+    }
+    (void)hipEventRecord(v[e->prof_used[id]++], s);
+}
 
 void layout(me_engine *e, char *base, size_t *bytes) {
     const me_config &c = e->cfg;
@@ -265,8 +284,10 @@ int generate(me_engine *e, hipStream_t s) {
     hipLaunchKernelGGL(k_box_cand_emit, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->cand_n, e->cand_off,
                        e->cand_tmp, e->cap_cand, e->cand_a, e->cand_b, e->istat);
     int cc = e->cap_cand;
+    prof_mark(e, ME_K_NARROW, s);
     hipLaunchKernelGGL(k_narrow, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, b, e->Rw, e->cand_a, e->cand_b,
                        e->cfg.margin, N, e->S);
+    prof_mark(e, ME_K_NARROW, s);
     rc = scan(e, e->S.n, e->slot_off, e->istat + IS_CAND, 0, cc, e->istat + IS_C, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pair_flag, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, e->S.n, e->pair_flag);
@@ -299,7 +320,9 @@ int solve_jacobi(me_engine *e, float sdt, hipStream_t s) {
     A.acc = e->acc; A.N = N;
     const float S = 1.0e10f;
     for (int it = 0; it < e->cfg.vit; ++it) {
+        prof_mark(e, ME_K_JAC_VEL, s);
         hipLaunchKernelGGL(k_jac_vel, dim3(nblk(C)), dim3(ME_THREADS), 0, s, e->istat, A, S);
+        prof_mark(e, ME_K_JAC_VEL, s);
         hipLaunchKernelGGL(k_jac_apply, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->b.v, e->b.w, e->acc,
                            e->cfg.relax, S);
     }
@@ -411,11 +434,17 @@ int solve_colored(me_engine *e, float sdt, hipStream_t s) {
         for (int ci = 0; ci < ncol; ++ci) hipLaunchKernelGGL(k_sweep_warm, g, t, 0, s, S, ci);
     for (int it = 0; it < e->cfg.vit; ++it)
         for (int ci = 0; ci < ncol; ++ci) {
+            prof_mark(e, ME_K_SWEEP_VEL, s);
             if (cc_mode) hipLaunchKernelGGL(k_sweep_vel<true>, g, t, 0, s, S, ci, 1.0f);
             else hipLaunchKernelGGL(k_sweep_vel<false>, g, t, 0, s, S, ci, e->cfg.t2d);
+            prof_mark(e, ME_K_SWEEP_VEL, s);
         }
     for (int it = 0; it < e->cfg.pit; ++it)
-        for (int ci = 0; ci < ncol; ++ci) hipLaunchKernelGGL(k_sweep_pos, g, t, 0, s, S, ci, sdt);
+        for (int ci = 0; ci < ncol; ++ci) {
+            prof_mark(e, ME_K_SWEEP_POS, s);
+            hipLaunchKernelGGL(k_sweep_pos, g, t, 0, s, S, ci, sdt);
+            prof_mark(e, ME_K_SWEEP_POS, s);
+        }
     if (e->cfg.warm_start)
         hipLaunchKernelGGL(k_warm_store, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->Cc.key, e->Cc.jn,
                            e->Cc.jt1, e->Cc.jt2, e->wkey, e->wjn, e->wjt1, e->wjt2);
@@ -436,7 +465,9 @@ int substep(me_engine *e, double sdt64, hipStream_t s) {
     if (e->cfg.solver == ME_SOLVER_JACOBI) {
         rc = solve_jacobi(e, sdt, s);
     } else if (e->cfg.flags & ME_FLAG_TILED) {
+        prof_mark(e, ME_K_TILED, s);
         rc = tiled_solve(e->tb, tiled_args_of(e, sdt), s);
+        prof_mark(e, ME_K_TILED, s);
     } else {
         rc = solve_colored(e, sdt, s);
     }
@@ -463,6 +494,7 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.wkey = e->wkey; a.wjn = e->wjn; a.wjt1 = e->wjt1; a.wjt2 = e->wjt2;
     a.istat = e->istat;
     a.bsum = e->bsum; a.nbsum = e->nbsum;
+    a.claim = e->claim;
     return a;
 }
 
@@ -523,6 +555,8 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
 
 int me_destroy(me_engine *e) {
     if (!e) return fail(ME_EARG, "engine is NULL");
+    for (int k = 0; k < ME_K_COUNT_; ++k)
+        for (hipEvent_t ev : e->prof_ev[k]) (void)hipEventDestroy(ev);
     if (e->hstat) hipHostFree(e->hstat);
     delete e;
     return ME_OK;
@@ -594,6 +628,31 @@ int me_read_counters(me_engine *e, int64_t *out, void *stream) {
     out[ME_CNT_ISLANDS] = h[IS_ISL];
     out[ME_CNT_TILES] = h[IS_TILES];
     out[ME_CNT_MAX_TILE_BODIES] = h[IS_MAXTB];
+    return ME_OK;
+}
+
+int me_profile(me_engine *e, int enable) {
+    if (!e) return fail(ME_EARG, "engine is NULL");
+    e->prof_on = enable != 0;
+    if (e->prof_on)  // a new measurement window; disabling keeps the recorded events for reading
+        for (int k = 0; k < ME_K_COUNT_; ++k) e->prof_used[k] = 0;
+    return ME_OK;
+}
+
+int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launches) {
+    if (!e || !total_ms || !launches) return fail(ME_EARG, "NULL argument");
+    if (kernel < 0 || kernel >= ME_K_COUNT_) return fail(ME_EARG, "unknown kernel id");
+    double tot = 0.0;
+    int n = e->prof_used[kernel] / 2;
+    for (int k = 0; k < n; ++k) {
+        hipEvent_t a = e->prof_ev[kernel][2 * k], b = e->prof_ev[kernel][2 * k + 1];
+        HIP_TRY(hipEventSynchronize(b));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = n;
     return ME_OK;
 }
 

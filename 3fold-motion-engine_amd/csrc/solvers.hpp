@@ -23,17 +23,18 @@ __device__ __forceinline__ f3 fixed_t1(f3 nrm) {
     return normalize(aa - nrm * dot(aa, nrm));
 }
 
-// apply_warm of one pair (contact_engine_gpu_applied_warm.py:7-54); a, b are BodyView indices,
-// ga, gb the global body ids (b < 0: ground)
-__device__ __forceinline__ void apply_warm_pair(const BodyView &B, const ContactView &Cv, int a, int b,
-                                                int st, int cnt) {
+// apply_warm of one pair (contact_engine_gpu_applied_warm.py:7-54); body a is BA[a], body b is BB[b]
+// (b < 0: ground)
+__device__ __forceinline__ void apply_warm_pair(const BodyView &BA, const BodyView &BB, const ContactView &Cv,
+                                                int a, int b, int st, int cnt) {
+    const BodyView &B = BA;
     f3 va = ld3(B.v, a), wa = ld3(B.w, a);
     f3 vb = mk(0.0f, 0.0f, 0.0f), wb = mk(0.0f, 0.0f, 0.0f);
-    if (b >= 0) { vb = ld3(B.v, b); wb = ld3(B.w, b); }
+    if (b >= 0) { vb = ld3(BB.v, b); wb = ld3(BB.w, b); }
     m3 IA = ldm(B.invIw, a), IB;
-    if (b >= 0) IB = ldm(B.invIw, b);
-    float iMA = B.invM[a], iMB = b >= 0 ? B.invM[b] : 0.0f;
-    f3 xa = ld3(B.x, a), xb = b >= 0 ? ld3(B.x, b) : mk(0.0f, 0.0f, 0.0f);
+    if (b >= 0) IB = ldm(BB.invIw, b);
+    float iMA = B.invM[a], iMB = b >= 0 ? BB.invM[b] : 0.0f;
+    f3 xa = ld3(B.x, a), xb = b >= 0 ? ld3(BB.x, b) : mk(0.0f, 0.0f, 0.0f);
     for (int k = 0; k < cnt; ++k) {
         int c = st + k;
         f3 n = ld3(Cv.n, c);
@@ -50,22 +51,22 @@ __device__ __forceinline__ void apply_warm_pair(const BodyView &B, const Contact
         }
     }
     if (iMA > 0.0f) { st3(B.v, a, va); st3(B.w, a, wa); }
-    if (b >= 0 && iMB > 0.0f) { st3(B.v, b, vb); st3(B.w, b, wb); }
+    if (b >= 0 && iMB > 0.0f) { st3(BB.v, b, vb); st3(BB.w, b, wb); }
 }
 
 // pair-chunk velocity sweep. FIXED: normal-derived frame (k_fixed_frame,
 // contact_engine_gpu_fixed_frame.py:7-77 / unfused_solve.py:117-220); otherwise the slip frame of
 // k_chunk_vel_r (contact_engine_gpu_colored.py:413-490)
 template <bool FIXED>
-__device__ __forceinline__ void chunk_vel(const BodyView &B, const ContactView &Cv, int bi, int bj, int st,
-                                          int cnt, float t2d) {
+__device__ __forceinline__ void chunk_vel(const BodyView &B, const BodyView &BB, const ContactView &Cv, int bi,
+                                          int bj, int st, int cnt, float t2d) {
     float iMA = B.invM[bi];
     m3 IA = ldm(B.invIw, bi), IB = IA;
     f3 vA = ld3(B.v, bi), wA = ld3(B.w, bi);
     float iMB = 0.0f;
     f3 vB = mk(0.0f, 0.0f, 0.0f), wB = mk(0.0f, 0.0f, 0.0f);
     f3 xA = ld3(B.x, bi), xB = mk(0.0f, 0.0f, 0.0f);
-    if (bj >= 0) { iMB = B.invM[bj]; IB = ldm(B.invIw, bj); vB = ld3(B.v, bj); wB = ld3(B.w, bj); xB = ld3(B.x, bj); }
+    if (bj >= 0) { iMB = BB.invM[bj]; IB = ldm(BB.invIw, bj); vB = ld3(BB.v, bj); wB = ld3(BB.w, bj); xB = ld3(BB.x, bj); }
     for (int k = 0; k < cnt; ++k) {
         int c = st + k;
         f3 nrm = ld3(Cv.n, c), rA = ld3(Cv.pA, c) - xA;
@@ -128,20 +129,20 @@ __device__ __forceinline__ void chunk_vel(const BodyView &B, const ContactView &
         }
     }
     if (iMA > 0.0f) { st3(B.v, bi, vA); st3(B.w, bi, wA); }
-    if (bj >= 0 && iMB > 0.0f) { st3(B.v, bj, vB); st3(B.w, bj, wB); }
+    if (bj >= 0 && iMB > 0.0f) { st3(BB.v, bj, vB); st3(BB.w, bj, wB); }
 }
 
 // pair-chunk position sweep (k_chunk_pos_r, contact_engine_gpu_colored.py:511-554;
 // unfused_position.py:7-65)
-__device__ __forceinline__ void chunk_pos(const BodyView &B, const ContactView &Cv, int bi, int bj, int st,
-                                          int cnt, float dt) {
+__device__ __forceinline__ void chunk_pos(const BodyView &B, const BodyView &BB, const ContactView &Cv, int bi,
+                                          int bj, int st, int cnt, float dt) {
     float iMA = B.invM[bi];
     m3 IA = ldm(B.invIw, bi), IB = IA;
     f3 pvA = ld3(B.pv, bi), poA = ld3(B.po, bi);
     float iMB = 0.0f;
     f3 pvB = mk(0.0f, 0.0f, 0.0f), poB = mk(0.0f, 0.0f, 0.0f);
     f3 xA = ld3(B.x, bi), xB = mk(0.0f, 0.0f, 0.0f);
-    if (bj >= 0) { iMB = B.invM[bj]; IB = ldm(B.invIw, bj); pvB = ld3(B.pv, bj); poB = ld3(B.po, bj); xB = ld3(B.x, bj); }
+    if (bj >= 0) { iMB = BB.invM[bj]; IB = ldm(BB.invIw, bj); pvB = ld3(BB.pv, bj); poB = ld3(BB.po, bj); xB = ld3(BB.x, bj); }
     for (int k = 0; k < cnt; ++k) {
         int c = st + k;
         f3 nrm = ld3(Cv.n, c), rA = ld3(Cv.pA, c) - xA;
@@ -166,7 +167,7 @@ __device__ __forceinline__ void chunk_pos(const BodyView &B, const ContactView &
         if (bj >= 0 && iMB > 0.0f) { pvB = pvB - Jn * iMB; poB = poB - mv(IB, cross(rB, Jn)); }
     }
     if (iMA > 0.0f) { st3(B.pv, bi, pvA); st3(B.po, bi, poA); }
-    if (bj >= 0 && iMB > 0.0f) { st3(B.pv, bj, pvB); st3(B.po, bj, poB); }
+    if (bj >= 0 && iMB > 0.0f) { st3(BB.pv, bj, pvB); st3(BB.po, bj, poB); }
 }
 
 // integrate one body (k_integ, contact_engine_gpu.py:172-180)

@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Contact-step benchmark on one MI355X.
+
+Workload (default): the 10 000-box pile -- per-axis edges U(0.05, 0.15) m, density 1000, 5 layers
+on a 0.2 m grid with jitter and random yaw, mu U(0.3, 0.8) per body -- stepped by the ColorCache
+solver with 16 colour classes, 2 substeps, 16 velocity and 10 position iterations, dt = 1/240,
+float32, warm-started. A "step" is one ContactEngine.step(dt, substeps=2) over the whole pile; the
+warm-up steps let the pile fall into contact first (warm-up + timed = 600 steps by default).
+
+Prints ONE JSON line: body-steps/s (whole job), ms/step, the roofline of the dominant kernel
+(algorithmic bytes per launch / HIP-event launch time, against 8 TB/s) and the CPU oracle timed
+on this host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "body-steps/s at 10k boxes & 256-scene batch; adjoint ms/step; HBM % of peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(pairs, invM, solver, V, P, S, N, C):
+    """Compulsory SoA traffic, each field once per pass: velocity chunk per pair
+    20 + 100*n_dyn + 64*k B; position chunk and warm apply 20 + 100*n_dyn + 52*k B."""
+    a, b, k = pairs["pbi"], pairs["pbj"], pairs["pcount"].astype(np.int64)
+    ndyn = (invM[a] > 0).astype(np.int64) + ((b >= 0) & (invM[np.maximum(b, 0)] > 0)).astype(np.int64)
+    vel = int(np.sum(20 + 100 * ndyn + 64 * k))
+    pos = int(np.sum(20 + 100 * ndyn + 52 * k))
+    warm = pos if solver == "color_cache" else 0
+    pre = 116 * N + 108 * C + 24 * N
+    integ = 108 * N
+    return dict(vel_sweep=vel, pos_sweep=pos, warm=warm,
+                step=S * (pre + warm + V * vel + P * pos + integ))
+
+
+def cpu_baseline(x, q, v, w, bodies_set, solver, V, P, S, n_colors, budget_s):
+    from oracle import oracle as orc
+    bs = bodies_set
+    bs.x[:], bs.q[:], bs.v[:], bs.w[:] = x, q, v, w
+    o = orc.OracleWorld(bs, solver, vit=V, pit=P, n_colors=n_colors)
+    steps, secs = 0, 0.0
+    while secs < budget_s and steps < 400:
+        secs += o.time_steps(1 / 240, S, 2)
+        steps += 2
+    return steps, secs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--bodies", type=int, default=10000)
+    ap.add_argument("--solver", default="color_cache", choices=["color_cache", "pair_chunks", "jacobi"])
+    ap.add_argument("--colors", type=int, default=16)
+    ap.add_argument("--substeps", type=int, default=2)
+    ap.add_argument("--vit", type=int, default=16)
+    ap.add_argument("--pit", type=int, default=10)
+    ap.add_argument("--tiled", type=int, default=0)
+    ap.add_argument("--tile-bodies", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              f"(independent scenes and islands shard without communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    L = pkg._lib if hasattr(pkg, "_lib") else importlib.import_module("3fold-motion-engine_amd._lib")
+    L = importlib.import_module("3fold-motion-engine_amd._lib")
+    import ctypes as C
+
+    bodies = pkg.scenes.box_pile(a.bodies, seed=a.seed)
+    cls = {"color_cache": pkg.ColorCacheContactEngine, "pair_chunks": pkg.GraphColoredContactEngine,
+           "jacobi": pkg.RelaxedJacobiContactEngine}[a.solver]
+    kw = dict(vit=a.vit, pit=a.pit, check_every_step=False)
+    if a.solver != "jacobi":
+        kw.update(n_colors=a.colors, tiled=bool(a.tiled), tile_bodies=a.tile_bodies)
+    e = cls(**kw)
+    e.add_boxes(bodies)
+    dt, S = 1 / 240, a.substeps
+    t_w = time.perf_counter()
+    for i in range(a.warmup):
+        e.step(dt, S)
+        if (i + 1) % 100 == 0:
+            torch.cuda.synchronize()
+            print(f"[bench] warmup {i + 1}/{a.warmup} {time.perf_counter() - t_w:.1f}s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    e.refresh_counters()
+    x0, q0, v0, w0 = e.state_numpy()
+    Lib = L.lib()
+    L.check(Lib.me_profile(e._h, 1), "me_profile")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        e.step(dt, S)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.check(Lib.me_profile(e._h, 0), "me_profile")
+    cnt = e.refresh_counters()
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / a.steps
+    value = a.bodies * a.steps / elapsed
+
+    # dominant kernel: the profiled kernel with the largest summed time
+    names = {L.K_SWEEP_VEL: "k_sweep_vel", L.K_SWEEP_POS: "k_sweep_pos", L.K_NARROW: "k_narrow",
+             L.K_TILED: "k_tiled_solve", L.K_JAC_VEL: "k_jac_vel"}
+    prof = {}
+    for kid, nm in names.items():
+        tot, n = C.c_double(), C.c_int64()
+        L.check(Lib.me_profile_read(e._h, kid, C.byref(tot), C.byref(n)), "me_profile_read")
+        if n.value:
+            prof[nm] = (tot.value, n.value)
+    pairs = e.pairs()
+    ab = algorithmic_bytes(pairs, bodies.invM, a.solver, a.vit, a.pit, S, a.bodies, cnt["n_contacts"])
+    ncol = max(cnt["n_colors"], 1)
+    per_launch = {"k_sweep_vel": ab["vel_sweep"] / ncol, "k_sweep_pos": ab["pos_sweep"] / ncol,
+                  "k_tiled_solve": ab["step"] / S}
+    roofline = None
+    if prof:
+        dom = max((k for k in prof if k in per_launch), key=lambda k: prof[k][0], default=None)
+        if dom:
+            avg_s = prof[dom][0] / prof[dom][1] / 1e3
+            ach = per_launch[dom] / avg_s / 1e9
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 3), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None,
+                        "bytes_per_launch": int(per_launch[dom]), "avg_launch_us": round(avg_s * 1e6, 3),
+                        "launches_timed": prof[dom][1]}
+    cpu = None
+    if not a.no_cpu:
+        steps_c, secs_c = cpu_baseline(x0, q0, v0, w0, bodies, a.solver, a.vit, a.pit, S,
+                                       a.colors if a.solver != "jacobi" else 0, a.cpu_seconds)
+        cpu = {"value": round(a.bodies * steps_c / secs_c, 1), "unit": "body-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"CPU oracle (C, -O2, contraction off, 1 thread) from the GPU state at the start "
+                         f"of the timed region: {steps_c} steps of the same {a.bodies}-box pile in "
+                         f"{secs_c:.1f} s on {os.cpu_count()} visible host cores"}
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box pile)",
+           "config": {"workload": f"{a.bodies}-box pile, {a.solver}, {a.colors} colours, S{S} V{a.vit} P{a.pit}, "
+                                  f"dt 1/240" + (", tiled" if a.tiled else ", plain launches"),
+                      "bodies": a.bodies, "solver": a.solver, "colors": a.colors, "substeps": S,
+                      "vit": a.vit, "pit": a.pit, "tiled": bool(a.tiled),
+                      "contacts": cnt["n_contacts"], "pairs": cnt["n_pairs"], "colors_used": cnt["n_colors"]},
+           "roofline": roofline, "cpu_baseline": cpu,
+           "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in prof.items()}}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

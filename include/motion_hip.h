@@ -144,6 +144,21 @@ int me_buffer(me_engine *e, int id, void **ptr, int64_t *count);
 /* Copy ME_CNT_COUNT_ counters to host memory (synchronises `stream`). */
 int me_read_counters(me_engine *e, int64_t *out, void *stream);
 
+/* kernel ids for me_profile_read */
+enum {
+    ME_K_SWEEP_VEL = 0,  /* per-colour pair-chunk velocity sweep */
+    ME_K_SWEEP_POS,      /* per-colour pair-chunk position sweep */
+    ME_K_NARROW,         /* box narrow phase */
+    ME_K_TILED,          /* island-tiled solve (one launch per substep) */
+    ME_K_JAC_VEL,        /* Jacobi velocity pass */
+    ME_K_COUNT_
+};
+
+/* Time launches of the kernels above with HIP events on the step's stream (the first 8192
+ * launches per kernel after enabling). Not part of the reference interface: measurement only. */
+int me_profile(me_engine *e, int enable);
+int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launches);
+
 /* Text of the last error on the calling thread. */
 const char *me_last_error(void);
 

@@ -84,14 +84,17 @@ def test_box_stack16_matches_oracle(oracle, kind):
     np.testing.assert_array_equal(e.contact_forces(), o.forces())
 
 
-@pytest.mark.parametrize("kind,n_colors,per_contact", [("color_cache", 0, False), ("pair_chunks", 0, False),
-                                                       ("color_cache", 16, False), ("pair_chunks", 0, True)])
-def test_box_pile_matches_oracle(oracle, kind, n_colors, per_contact):
-    _say(f"{kind} box pile 400 x60 S2 V16 P10 colors={n_colors} per_contact={per_contact}")
+@pytest.mark.parametrize("kind,n_colors,per_contact,tiled", [
+    ("color_cache", 0, False, False), ("pair_chunks", 0, False, False), ("color_cache", 16, False, False),
+    ("pair_chunks", 0, True, False), ("color_cache", 16, False, True), ("color_cache", 0, False, True),
+    ("pair_chunks", 8, False, True), ("pair_chunks", 0, True, True)])
+def test_box_pile_matches_oracle(oracle, kind, n_colors, per_contact, tiled):
+    _say(f"{kind} box pile 400 x60 S2 V16 P10 colors={n_colors} per_contact={per_contact} tiled={tiled}")
     pkg = _pkg()
     b = pkg.scenes.box_pile(400, seed=3)
     kw = dict(vit=16, pit=10, n_colors=n_colors)
-    e = _engine(kind, pair_chunks=not per_contact, **kw) if kind == "pair_chunks" else _engine(kind, **kw)
+    ekw = dict(kw, tiled=tiled, tile_bodies=32)
+    e = _engine(kind, pair_chunks=not per_contact, **ekw) if kind == "pair_chunks" else _engine(kind, **ekw)
     e.add_boxes(b)
     o = oracle.OracleWorld(b, kind, per_contact=per_contact, **kw)
     for s in range(60):
@@ -129,3 +132,27 @@ def test_colouring_is_proper_and_runs_repeat():
         runs.append(e.state_numpy())
     for a, b2 in zip(*runs):
         assert np.array_equal(a, b2)
+
+
+@pytest.mark.parametrize("tile_bodies", [8, 64, 256])
+def test_tiled_chain_island_beyond_lds(oracle, tile_bodies):
+    """A row of 600 touching cubes is one island larger than the LDS tile capacity (512 bodies):
+    that tile runs on global memory; small tiles make islands straddle tile boundaries."""
+    _say(f"color_cache tiled chain 600 + pile 200, tile_bodies={tile_bodies}")
+    pkg = _pkg()
+    n = 600
+    c = np.zeros((n, 3))
+    c[:, 0] = np.arange(n) * 0.0995
+    c[:, 2] = 0.0499
+    row = pkg.bodies.box_bodies(c, [0.05, 0.05, 0.05], 1000.0, mu=0.6)
+    pile = pkg.scenes.box_pile(200, seed=5, origin=(0.0, 3.0, 0.0))
+    b = pkg.scenes.concat([row, pile])
+    e = _engine("color_cache", vit=8, pit=4, n_colors=16, tiled=True, tile_bodies=tile_bodies)
+    e.add_boxes(b)
+    o = oracle.OracleWorld(b, "color_cache", vit=8, pit=4, n_colors=16)
+    for s in range(30):
+        e.step(1 / 240, 2)
+        o.step(1 / 240, 2)
+        _assert_state_equal(e, o, f"step {s}")
+    assert e._counters["max_tile_bodies"] >= n
+    _assert_contacts_equal(e, o, "final")
