@@ -465,9 +465,14 @@ int substep(me_engine *e, double sdt64, hipStream_t s) {
     if (e->cfg.solver == ME_SOLVER_JACOBI) {
         rc = solve_jacobi(e, sdt, s);
     } else if (e->cfg.flags & ME_FLAG_TILED) {
+        TiledArgs ta = tiled_args_of(e, sdt);
+        rc = tiled_prepare(e->tb, ta, s);
+        if (rc) return fail(rc, "tiled solve: launch failed");
         prof_mark(e, ME_K_TILED, s);
-        rc = tiled_solve(e->tb, tiled_args_of(e, sdt), s);
+        rc = tiled_sweeps(e->tb, ta, s);
         prof_mark(e, ME_K_TILED, s);
+        if (rc) return fail(rc, "tiled solve: launch failed");
+        rc = tiled_finish(e->tb, ta, s);
     } else {
         rc = solve_colored(e, sdt, s);
     }

@@ -32,6 +32,10 @@ struct TiledBuffers {
     int *bsorted, *bslot, *psorted;
     int *tb0, *tb1, *tp0, *tp1;
     int *ccount_jp, *csub, *cbase;
+    float *xs, *invMs, *invIws;  // body constants in island order (index = position in bsorted)
+    int *psorted2;               // big tiles: pairs counting-sorted by colour
+    int *rec;                    // big tiles: 8 ints per colour-sorted pair (p, a, b, la, lb, st, cnt, flags)
+    float *prep;                 // 20 floats per contact: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu
 };
 
 struct TiledArgs {
@@ -52,7 +56,6 @@ struct TiledArgs {
 
 template <class BumpT>
 inline void tiled_layout(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
-    (void)capC;
     size_t n = (size_t)N, p = (size_t)capP;
     t.parent = B.template take<int>(n);
     t.root = B.template take<int>(n);
@@ -73,6 +76,12 @@ inline void tiled_layout(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
     t.ccount_jp = B.template take<int>(ME_NCOLMAX + 1);
     t.csub = B.template take<int>(ME_NCOLMAX + 1);
     t.cbase = B.template take<int>(ME_NCOLMAX + 2);
+    t.xs = B.template take<float>(3 * n);
+    t.invMs = B.template take<float>(n);
+    t.invIws = B.template take<float>(9 * n);
+    t.psorted2 = B.template take<int>(p);
+    t.rec = B.template take<int>(8 * p);
+    t.prep = B.template take<float>(20 * (size_t)capC);
 }
 
 // IS_MISMATCH after k_t_mode: 0 = ColorCache hit, 1 = colour this substep, 2 = no contacts
@@ -164,7 +173,7 @@ __global__ void k_t_pair_root(int capP, const int *istat, const int *pbi, const 
     if (r >= 0) atomicAdd(&T.cnt_p[r], 1);
 }
 
-__global__ void k_t_scatter_b(int N, TiledBuffers T) {
+__global__ void k_t_scatter_b(int N, TiledBuffers T, const float *x, const float *invM, const float *invIw) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int r = T.root[i];
@@ -172,6 +181,11 @@ __global__ void k_t_scatter_b(int N, TiledBuffers T) {
     int pos = T.boff[r] + atomicAdd(&T.fill_b[r], 1);
     T.bsorted[pos] = i;
     T.bslot[i] = pos;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) T.xs[3 * pos + k] = x[3 * i + k];
+    T.invMs[pos] = invM[i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) T.invIws[9 * pos + k] = invIw[9 * i + k];
 }
 
 __global__ void k_t_scatter_p(int capP, const int *istat, TiledBuffers T) {
@@ -286,124 +300,240 @@ __global__ void k_t_plan(int *istat, int K, TiledBuffers T) {
     istat[IS_NCOL] = tot;
 }
 
-// the sweeps of one tile; SOLVER: ME_SOLVER_PAIR_CHUNKS (slip frame, no warm apply) or
-// ME_SOLVER_COLOR_CACHE (fixed frame, warm apply)
-template <int WG, int SOLVER>
-__global__ void __launch_bounds__(WG) k_t_solve(TiledArgs A, TiledBuffers T) {
-    int t = blockIdx.x;
-    int b0 = T.tb0[t];
-    if (b0 == INT_MAX) return;
-    int nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - T.tp0[t];
-    if (np < 0) np = 0;
-    const int mode = A.istat[IS_MISMATCH];
-    int ncol = A.istat[IS_NCOL];
-    if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
-    __shared__ float sv[3 * T_MAXB], sw[3 * T_MAXB], spv[3 * T_MAXB], spo[3 * T_MAXB], sx[3 * T_MAXB];
-    __shared__ float sinvM[T_MAXB], sI[9 * T_MAXB];
-    __shared__ int plist[T_MAXP], pcol[T_MAXP];
-    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1];
-    __shared__ int maxc;
-    const bool lds = nb <= T_MAXB && np <= T_MAXP;
-    if (threadIdx.x == 0) {
-        atomicAdd(&A.istat[IS_TILES], 1);
-        atomicMax(&A.istat[IS_MAXTB], nb);
-        maxc = 0;
-    }
-    BodyView G;
-    G.v = A.b.v; G.w = A.b.w; G.pv = A.pv; G.po = A.po; G.x = A.b.x; G.invM = A.b.invM; G.invIw = A.invIw;
-    BodyView L;
-    L.v = sv; L.w = sw; L.pv = spv; L.po = spo; L.x = sx; L.invM = sinvM; L.invIw = sI;
-    ContactView Cv;
-    Cv.pA = A.Cc.pA; Cv.pB = A.Cc.pB; Cv.n = A.Cc.n; Cv.pen = A.Cc.pen; Cv.mu = A.Cc.mu;
-    Cv.jn = A.Cc.jn; Cv.jt1 = A.Cc.jt1; Cv.jt2 = A.Cc.jt2; Cv.jp = A.Cc.jp;
-    if (lds) {
-        for (int j = threadIdx.x; j < nb; j += WG) {
-            int i = T.bsorted[b0 + j];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                sv[3 * j + k] = A.b.v[3 * i + k];
-                sw[3 * j + k] = A.b.w[3 * i + k];
-                sx[3 * j + k] = A.b.x[3 * i + k];
-                spv[3 * j + k] = 0.0f;
-                spo[3 * j + k] = 0.0f;
-            }
-            sinvM[j] = A.b.invM[i];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) sI[9 * j + k] = A.invIw[9 * i + k];
-        }
-    }
-    // final colours (split on a fresh colouring) and, in LDS mode, a counting sort by colour
-    for (int c = threadIdx.x; c <= ncol; c += WG) { cst[c] = 0; cfl[c] = 0; }
-    __syncthreads();
-    for (int k = threadIdx.x; k < np; k += WG) {
-        int p = T.psorted[p0 + k];
-        int c = A.Pp.col[p];
-        if (mode == 1 && c >= 0 && c < ME_NCOLMAX) {
-            c = T.cbase[c] + p % T.csub[c];
-            A.Pp.col[p] = c;
-        }
-        if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;  // only after a colour overflow (flagged)
-        if (lds) {
-            pcol[k] = c;
-            atomicAdd(&cst[c], 1);
-        }
-        atomicMax(&maxc, c + 1);
-    }
-    __syncthreads();
-    if (lds && threadIdx.x == 0) {
-        int run = 0;
-        for (int c = 0; c < maxc; ++c) {
-            int n = cst[c];
-            cst[c] = run;
-            run += n;
-        }
-        cst[maxc] = run;
-    }
-    __syncthreads();
-    if (lds)
-        for (int k = threadIdx.x; k < np; k += WG) {
-            int c = pcol[k];
-            plist[cst[c] + atomicAdd(&cfl[c], 1)] = T.psorted[p0 + k];
-        }
-    __syncthreads();
-    const int nc = maxc;
-    const int *pbi = A.Pp.pbi, *pbj = A.Pp.pbj, *pst = A.Pp.pstart, *pcn = A.Pp.pcount;
+// final colour of every pair: on a fresh colouring, JP class c becomes base[c] + rank mod sub[c]
+__global__ void k_t_apply_split(int capP, const int *istat, int *col, TiledBuffers T) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (istat[IS_MISMATCH] != 1 || p >= istat[IS_P] || p >= capP) return;
+    int c = col[p];
+    if (c >= 0 && c < ME_NCOLMAX) col[p] = T.cbase[c] + p % T.csub[c];
+}
+
+#define T_SMALL_P 256   // pairs of a small tile: one per thread, constants in registers
+#define T_SMALL_B 512   // bodies of a small tile (velocities in LDS)
+#define T_BIG_B 6400    // bodies of a big tile whose v, w (then pv, po) fit the dynamic LDS (150 KB)
+
+// per-contact constants of one substep's solve (identical to recomputing them every sweep: their
+// inputs -- normal, contact points, centres, world inverse inertia -- do not change in the solve)
+struct CCon {
+    f3 n, rA, rB, t1, t2;
+    float meff, meft1, meft2, bias, mu;
+    float jn, jt1, jt2, jp;
+};
+
+struct PairCache {
+    int cnt, la, lb;
+    bool hasB, dynA, dynB;
+    float iMA, iMB;
+    m3 IA, IB;
+    f3 vAs, wAs, vBs, wBs, pvBs, poBs, pvAs, poAs;  // static bodies: constant state
+    CCon c[4];
+};
+
+template <bool FIXED>
+__device__ __forceinline__ void pc_load(PairCache &P, const TiledArgs &A, const TiledBuffers &T, int p, int b0) {
+    int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
     const float *invM = A.b.invM;
-    // one colour step of one pass: 0 = warm apply, 1 = velocity, 2 = position
-    auto solve_pair = [&](int p, int pass) {
-        int a = pbi[p], b = pbj[p];
-        bool dA = lds && invM[a] > 0.0f, dB = lds && b >= 0 && invM[b] > 0.0f;
-        const BodyView &VA = dA ? L : G;
-        const BodyView &VB = dB ? L : G;
-        int ia = dA ? T.bslot[a] - b0 : a;
-        int ib = b < 0 ? -1 : (dB ? T.bslot[b] - b0 : b);
-        if (pass == 0) apply_warm_pair(VA, VB, Cv, ia, ib, pst[p], pcn[p]);
-        else if (pass == 1) chunk_vel<SOLVER == ME_SOLVER_COLOR_CACHE>(VA, VB, Cv, ia, ib, pst[p], pcn[p],
-                                                                      SOLVER == ME_SOLVER_COLOR_CACHE ? 1.0f : A.t2d);
-        else chunk_pos(VA, VB, Cv, ia, ib, pst[p], pcn[p], A.dt);
-    };
-    auto sweep = [&](int pass) {
-        for (int c = 0; c < nc; ++c) {
-            if (lds) {
-                for (int k = cst[c] + threadIdx.x; k < cst[c + 1]; k += WG) solve_pair(plist[k], pass);
-            } else {
-                for (int k = threadIdx.x; k < np; k += WG) {
-                    int p = T.psorted[p0 + k];
-                    int pc = A.Pp.col[p];
-                    if (pc < 0 || pc >= ncol) pc = ncol > 0 ? ncol - 1 : 0;
-                    if (pc == c) solve_pair(p, pass);
-                }
-            }
-            __syncthreads();
+    P.iMA = invM[a];
+    P.dynA = P.iMA > 0.0f;
+    P.hasB = b >= 0;
+    P.iMB = P.hasB ? invM[b] : 0.0f;
+    P.dynB = P.hasB && P.iMB > 0.0f;
+    P.la = P.dynA ? T.bslot[a] - b0 : a;
+    P.lb = P.dynB ? T.bslot[b] - b0 : b;
+    P.IA = ldm(A.invIw, a);
+    P.IB = P.hasB ? ldm(A.invIw, b) : P.IA;
+    f3 z = mk(0.0f, 0.0f, 0.0f);
+    f3 xA = ld3(A.b.x, a), xB = P.hasB ? ld3(A.b.x, b) : z;
+    P.vAs = P.dynA ? z : ld3(A.b.v, a); P.wAs = P.dynA ? z : ld3(A.b.w, a);
+    P.pvAs = P.dynA ? z : ld3(A.pv, a); P.poAs = P.dynA ? z : ld3(A.po, a);
+    P.vBs = (P.hasB && !P.dynB) ? ld3(A.b.v, b) : z; P.wBs = (P.hasB && !P.dynB) ? ld3(A.b.w, b) : z;
+    P.pvBs = (P.hasB && !P.dynB) ? ld3(A.pv, b) : z; P.poBs = (P.hasB && !P.dynB) ? ld3(A.po, b) : z;
+    int st = A.Pp.pstart[p];
+    P.cnt = A.Pp.pcount[p];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= P.cnt) break;
+        int c = st + k;
+        CCon &C = P.c[k];
+        C.n = ld3(A.Cc.n, c);
+        C.rA = ld3(A.Cc.pA, c) - xA;
+        C.rB = P.hasB ? ld3(A.Cc.pB, c) - xB : z;
+        C.mu = A.Cc.mu[c];
+        C.jn = A.Cc.jn[c]; C.jt1 = A.Cc.jt1[c]; C.jt2 = A.Cc.jt2[c]; C.jp = A.Cc.jp[c];
+        f3 rnA = cross(C.rA, C.n);
+        float meff = P.iMA + dot(rnA, mv(P.IA, rnA));
+        if (P.hasB) {
+            f3 rnB = cross(C.rB, C.n);
+            meff = meff + P.iMB + dot(rnB, mv(P.IB, rnB));
         }
-    };
-    if (np > 0) {
-        if (SOLVER == ME_SOLVER_COLOR_CACHE) sweep(0);
-        for (int it = 0; it < A.vit; ++it) sweep(1);
-        for (int it = 0; it < A.pit; ++it) sweep(2);
+        C.meff = meff;
+        C.bias = 0.2f * wmax(A.Cc.pen[c] - 1e-4f, 0.0f) / A.dt;
+        C.t1 = fixed_t1(C.n);
+        C.t2 = cross(C.n, C.t1);
+        if (FIXED) {
+            f3 rtA1 = cross(C.rA, C.t1);
+            float m1 = P.iMA + dot(rtA1, mv(P.IA, rtA1));
+            f3 rtA2 = cross(C.rA, C.t2);
+            float m2 = P.iMA + dot(rtA2, mv(P.IA, rtA2));
+            if (P.hasB) {
+                f3 rtB1 = cross(C.rB, C.t1);
+                m1 = m1 + P.iMB + dot(rtB1, mv(P.IB, rtB1));
+                f3 rtB2 = cross(C.rB, C.t2);
+                m2 = m2 + P.iMB + dot(rtB2, mv(P.IB, rtB2));
+            }
+            C.meft1 = m1;
+            C.meft2 = m2;
+        }
     }
-    __syncthreads();
-    // write back and integrate (k_integ, contact_engine_gpu.py:172-180)
+}
+
+__device__ __forceinline__ void pc_store(const PairCache &P, const TiledArgs &A, int p) {
+    int st = A.Pp.pstart[p];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= P.cnt) break;
+        int c = st + k;
+        A.Cc.jn[c] = P.c[k].jn; A.Cc.jt1[c] = P.c[k].jt1; A.Cc.jt2[c] = P.c[k].jt2; A.Cc.jp[c] = P.c[k].jp;
+    }
+}
+
+// apply_warm (contact_engine_gpu_applied_warm.py:7-54) on the cached pair
+__device__ __forceinline__ void pc_warm(const PairCache &P, float *sv, float *sw) {
+    f3 va = P.dynA ? ld3(sv, P.la) : P.vAs, wa = P.dynA ? ld3(sw, P.la) : P.wAs;
+    f3 vb = mk(0.0f, 0.0f, 0.0f), wb = mk(0.0f, 0.0f, 0.0f);
+    if (P.hasB) { vb = P.dynB ? ld3(sv, P.lb) : P.vBs; wb = P.dynB ? ld3(sw, P.lb) : P.wBs; }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= P.cnt) break;
+        const CCon &C = P.c[k];
+        f3 imp = (C.jn * C.n + C.jt1 * C.t1) + C.jt2 * C.t2;
+        if (P.iMA > 0.0f) {
+            va = va + P.iMA * imp;
+            wa = wa + mv(P.IA, cross(C.rA, imp));
+        }
+        if (P.hasB && P.iMB > 0.0f) {
+            vb = vb - P.iMB * imp;
+            wb = wb - mv(P.IB, cross(C.rB, imp));
+        }
+    }
+    if (P.iMA > 0.0f) { st3(sv, P.la, va); st3(sw, P.la, wa); }
+    if (P.hasB && P.iMB > 0.0f) { st3(sv, P.lb, vb); st3(sw, P.lb, wb); }
+}
+
+// chunk velocity sweep on the cached pair, expression for expression as chunk_vel
+template <bool FIXED>
+__device__ __forceinline__ void pc_vel(PairCache &P, float *sv, float *sw, float t2d) {
+    f3 vA = P.dynA ? ld3(sv, P.la) : P.vAs, wA = P.dynA ? ld3(sw, P.la) : P.wAs;
+    f3 vB = mk(0.0f, 0.0f, 0.0f), wB = mk(0.0f, 0.0f, 0.0f);
+    if (P.hasB) { vB = P.dynB ? ld3(sv, P.lb) : P.vBs; wB = P.dynB ? ld3(sw, P.lb) : P.wBs; }
+    const float iMA = P.iMA, iMB = P.iMB;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= P.cnt) break;
+        CCon &C = P.c[k];
+        const f3 nrm = C.n, rA = C.rA, rB = C.rB;
+        f3 va = vA + cross(wA, rA);
+        f3 vb = mk(0.0f, 0.0f, 0.0f);
+        if (P.hasB) vb = vB + cross(wB, rB);
+        f3 vrel = va - vb;
+        float vn = dot(vrel, nrm);
+        f3 t1, t2;
+        float meft1, meft2;
+        if (FIXED) {
+            t1 = C.t1; t2 = C.t2;
+        } else {
+            f3 vt = vrel - nrm * vn;
+            float mt = length(vt);
+            t1 = vt / mt;
+            if (mt <= 5e-3f) t1 = C.t1;
+            t2 = cross(nrm, t1);
+        }
+        if (!(C.meff >= 1e-12f)) continue;
+        float jo = C.jn;
+        float dj = -vn / C.meff;
+        float nw = wmax(0.0f, jo + dj);
+        dj = nw - jo;
+        C.jn = nw;
+        f3 Jn = nrm * dj;
+        if (iMA > 0.0f) { vA = vA + Jn * iMA; wA = wA + mv(P.IA, cross(rA, Jn)); }
+        if (P.hasB && iMB > 0.0f) { vB = vB - Jn * iMB; wB = wB - mv(P.IB, cross(rB, Jn)); }
+        va = vA + cross(wA, rA);
+        vb = mk(0.0f, 0.0f, 0.0f);
+        if (P.hasB) vb = vB + cross(wB, rB);
+        vrel = va - vb;
+        if (FIXED) {
+            meft1 = C.meft1; meft2 = C.meft2;
+        } else {
+            f3 rtA1 = cross(rA, t1);
+            meft1 = iMA + dot(rtA1, mv(P.IA, rtA1));
+            f3 rtA2 = cross(rA, t2);
+            meft2 = iMA + dot(rtA2, mv(P.IA, rtA2));
+            if (P.hasB) {
+                f3 rtB1 = cross(rB, t1);
+                meft1 = meft1 + iMB + dot(rtB1, mv(P.IB, rtB1));
+                f3 rtB2 = cross(rB, t2);
+                meft2 = meft2 + iMB + dot(rtB2, mv(P.IB, rtB2));
+            }
+        }
+        if (meft1 >= 1e-12f && meft2 >= 1e-12f) {
+            float o1 = C.jt1, o2 = C.jt2;
+            float a1 = o1 - dot(vrel, t1) / meft1;
+            float a2 = o2 - dot(vrel, t2) / meft2;
+            float mag = sqrtf(a1 * a1 + a2 * a2);
+            float lim = C.mu * nw;
+            if (mag > lim && mag > 1e-12f) { a1 = a1 * lim / mag; a2 = a2 * lim / mag; }
+            C.jt1 = a1;
+            C.jt2 = a2;
+            f3 Jt = t1 * (a1 - o1) + t2 * (t2d * (a2 - o2));
+            if (iMA > 0.0f) { vA = vA + Jt * iMA; wA = wA + mv(P.IA, cross(rA, Jt)); }
+            if (P.hasB && iMB > 0.0f) { vB = vB - Jt * iMB; wB = wB - mv(P.IB, cross(rB, Jt)); }
+        }
+    }
+    if (iMA > 0.0f) { st3(sv, P.la, vA); st3(sw, P.la, wA); }
+    if (P.hasB && iMB > 0.0f) { st3(sv, P.lb, vB); st3(sw, P.lb, wB); }
+}
+
+// chunk position sweep on the cached pair, as chunk_pos
+__device__ __forceinline__ void pc_pos(PairCache &P, float *spv, float *spo) {
+    f3 pvA = P.dynA ? ld3(spv, P.la) : P.pvAs, poA = P.dynA ? ld3(spo, P.la) : P.poAs;
+    f3 pvB = mk(0.0f, 0.0f, 0.0f), poB = mk(0.0f, 0.0f, 0.0f);
+    if (P.hasB) { pvB = P.dynB ? ld3(spv, P.lb) : P.pvBs; poB = P.dynB ? ld3(spo, P.lb) : P.poBs; }
+    const float iMA = P.iMA, iMB = P.iMB;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= P.cnt) break;
+        CCon &C = P.c[k];
+        const f3 nrm = C.n, rA = C.rA, rB = C.rB;
+        f3 rel = pvA + cross(poA, rA);
+        if (P.hasB) rel = rel - (pvB + cross(poB, rB));
+        if (!(C.meff >= 1e-12f)) continue;
+        float dj = (C.bias - dot(rel, nrm)) / C.meff;
+        float jo = C.jp;
+        float nw = wmax(0.0f, jo + dj);
+        dj = nw - jo;
+        C.jp = nw;
+        f3 Jn = nrm * dj;
+        if (iMA > 0.0f) { pvA = pvA + Jn * iMA; poA = poA + mv(P.IA, cross(rA, Jn)); }
+        if (P.hasB && iMB > 0.0f) { pvB = pvB - Jn * iMB; poB = poB - mv(P.IB, cross(rB, Jn)); }
+    }
+    if (iMA > 0.0f) { st3(spv, P.la, pvA); st3(spo, P.la, poA); }
+    if (P.hasB && iMB > 0.0f) { st3(spv, P.lb, pvB); st3(spo, P.lb, poB); }
+}
+
+__device__ __forceinline__ void tile_bounds(const TiledBuffers &T, int t, int &b0, int &nb, int &p0, int &np) {
+    b0 = T.tb0[t];
+    nb = T.tb1[t] - b0;
+    p0 = T.tp0[t];
+    np = T.tp1[t] - T.tp0[t];
+    if (np < 0) np = 0;
+}
+
+__device__ __forceinline__ bool is_small_tile(int nb, int np) { return np <= T_SMALL_P && nb <= T_SMALL_B; }
+
+// write back the LDS body state and integrate (k_integ, contact_engine_gpu.py:172-180)
+template <int WG>
+__device__ __forceinline__ void tile_writeback(const TiledArgs &A, const TiledBuffers &T, int b0, int nb, bool lds,
+                                               const float *sv, const float *sw, const float *spv, const float *spo) {
     for (int j = threadIdx.x; j < nb; j += WG) {
         int i = T.bsorted[b0 + j];
         f3 v, w, pv, po;
@@ -414,6 +544,272 @@ __global__ void __launch_bounds__(WG) k_t_solve(TiledArgs A, TiledBuffers T) {
             v = ld3(A.b.v, i); w = ld3(A.b.w, i); pv = ld3(A.pv, i); po = ld3(A.po, i);
         }
         if (A.b.invM[i] > 0.0f) integrate_body(A.b.x, A.b.q, v, w, pv, po, i, A.dt);
+    }
+}
+
+// small tiles: thread t owns the t-th pair of the tile in colour order, its constants and impulses
+// in registers; body velocities in LDS; one workgroup barrier per colour step
+template <bool FIXED>
+__global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers T) {
+    int t = blockIdx.x;
+    if (T.tb0[t] == INT_MAX) return;
+    int b0, nb, p0, np;
+    tile_bounds(T, t, b0, nb, p0, np);
+    if (!is_small_tile(nb, np)) return;
+    __shared__ float sv[3 * T_SMALL_B], sw[3 * T_SMALL_B], spv[3 * T_SMALL_B], spo[3 * T_SMALL_B];
+    __shared__ int cnt[ME_NCOLMAX + 1], fill[ME_NCOLMAX + 1], plist[T_SMALL_P], plcol[T_SMALL_P];
+    __shared__ int maxc;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        atomicAdd(&A.istat[IS_TILES], 1);
+        atomicMax(&A.istat[IS_MAXTB], nb);
+        maxc = 0;
+    }
+    int ncol = A.istat[IS_NCOL];
+    if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
+    for (int j = tid; j < nb; j += T_SMALL_P) {
+        int i = T.bsorted[b0 + j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sv[3 * j + k] = A.b.v[3 * i + k];
+            sw[3 * j + k] = A.b.w[3 * i + k];
+            spv[3 * j + k] = 0.0f;
+            spo[3 * j + k] = 0.0f;
+        }
+    }
+    for (int c = tid; c <= ncol; c += T_SMALL_P) { cnt[c] = 0; fill[c] = 0; }
+    __syncthreads();
+    int myc = -1;
+    if (tid < np) {
+        myc = A.Pp.col[T.psorted[p0 + tid]];
+        if (myc < 0 || myc >= ncol) myc = ncol > 0 ? ncol - 1 : 0;  // only after a flagged colour overflow
+        atomicAdd(&cnt[myc], 1);
+        atomicMax(&maxc, myc + 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int c = 0; c < maxc; ++c) { int n = cnt[c]; cnt[c] = run; run += n; }
+    }
+    __syncthreads();
+    if (tid < np) {
+        int pos = cnt[myc] + atomicAdd(&fill[myc], 1);
+        plist[pos] = T.psorted[p0 + tid];
+        plcol[pos] = myc;
+    }
+    __syncthreads();
+    const int nc = maxc;
+    PairCache P;
+    int p = -1, c_own = -1;
+    if (tid < np) {
+        p = plist[tid];
+        c_own = plcol[tid];
+        pc_load<FIXED>(P, A, T, p, b0);
+    }
+    if (np > 0) {
+        if (FIXED) {
+            for (int c = 0; c < nc; ++c) {
+                if (c_own == c) pc_warm(P, sv, sw);
+                __syncthreads();
+            }
+        }
+        const float t2d = FIXED ? 1.0f : A.t2d;
+        for (int it = 0; it < A.vit; ++it)
+            for (int c = 0; c < nc; ++c) {
+                if (c_own == c) pc_vel<FIXED>(P, sv, sw, t2d);
+                __syncthreads();
+            }
+        for (int it = 0; it < A.pit; ++it)
+            for (int c = 0; c < nc; ++c) {
+                if (c_own == c) pc_pos(P, spv, spo);
+                __syncthreads();
+            }
+    }
+    if (p >= 0) pc_store(P, A, p);
+    __syncthreads();
+    tile_writeback<T_SMALL_P>(A, T, b0, nb, true, sv, sw, spv, spo);
+}
+
+// per-contact constants of the substep (k_t_prep): the same expressions pc_load evaluates
+template <bool FIXED>
+__global__ void k_t_prep(TiledArgs A, TiledBuffers T) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int C = A.istat[IS_C];
+    if (C > A.capC) C = A.capC;
+    if (c >= C) return;
+    int a = A.Cc.bi[c], b = A.Cc.bj[c];
+    const float *invM = A.b.invM;
+    float iMA = invM[a], iMB = b >= 0 ? invM[b] : 0.0f;
+    m3 IA = ldm(A.invIw, a), IB = b >= 0 ? ldm(A.invIw, b) : IA;
+    f3 n = ld3(A.Cc.n, c), rA = ld3(A.Cc.pA, c) - ld3(A.b.x, a);
+    f3 rB = b >= 0 ? ld3(A.Cc.pB, c) - ld3(A.b.x, b) : mk(0.0f, 0.0f, 0.0f);
+    f3 rnA = cross(rA, n);
+    float meff = iMA + dot(rnA, mv(IA, rnA));
+    if (b >= 0) {
+        f3 rnB = cross(rB, n);
+        meff = meff + iMB + dot(rnB, mv(IB, rnB));
+    }
+    f3 t1 = fixed_t1(n), t2 = cross(n, t1);
+    float m1 = 0.0f, m2 = 0.0f;
+    if (FIXED) {
+        f3 rtA1 = cross(rA, t1);
+        m1 = iMA + dot(rtA1, mv(IA, rtA1));
+        f3 rtA2 = cross(rA, t2);
+        m2 = iMA + dot(rtA2, mv(IA, rtA2));
+        if (b >= 0) {
+            f3 rtB1 = cross(rB, t1);
+            m1 = m1 + iMB + dot(rtB1, mv(IB, rtB1));
+            f3 rtB2 = cross(rB, t2);
+            m2 = m2 + iMB + dot(rtB2, mv(IB, rtB2));
+        }
+    }
+    float *o = T.prep + 20 * (size_t)c;
+    o[0] = n.x; o[1] = n.y; o[2] = n.z; o[3] = rA.x; o[4] = rA.y; o[5] = rA.z; o[6] = rB.x; o[7] = rB.y; o[8] = rB.z;
+    o[9] = t1.x; o[10] = t1.y; o[11] = t1.z; o[12] = t2.x; o[13] = t2.y; o[14] = t2.z;
+    o[15] = meff; o[16] = m1; o[17] = m2;
+    o[18] = 0.2f * wmax(A.Cc.pen[c] - 1e-4f, 0.0f) / A.dt;
+    o[19] = A.Cc.mu[c];
+}
+
+// PairCache of a colour-sorted big-tile record; v / w (or pv / po) come from the sweep arrays
+__device__ __forceinline__ void pc_from_rec(PairCache &P, const TiledArgs &A, const TiledBuffers &T, const int *r) {
+    int a = r[1], b = r[2], flags = r[7];
+    P.la = r[3]; P.lb = r[4]; P.cnt = r[6];
+    P.dynA = flags & 1; P.dynB = (flags & 2) != 0; P.hasB = b >= 0;
+    const float *invM = A.b.invM;
+    P.iMA = invM[a];
+    P.iMB = P.hasB ? invM[b] : 0.0f;
+    P.IA = ldm(A.invIw, a);
+    P.IB = P.hasB ? ldm(A.invIw, b) : P.IA;
+    f3 z = mk(0.0f, 0.0f, 0.0f);
+    P.vAs = P.dynA ? z : ld3(A.b.v, a); P.wAs = P.dynA ? z : ld3(A.b.w, a);
+    P.pvAs = P.dynA ? z : ld3(A.pv, a); P.poAs = P.dynA ? z : ld3(A.po, a);
+    bool sB = P.hasB && !P.dynB;
+    P.vBs = sB ? ld3(A.b.v, b) : z; P.wBs = sB ? ld3(A.b.w, b) : z;
+    P.pvBs = sB ? ld3(A.pv, b) : z; P.poBs = sB ? ld3(A.po, b) : z;
+    int st = r[5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= P.cnt) break;
+        int c = st + k;
+        const float *o = T.prep + 20 * (size_t)c;
+        CCon &C = P.c[k];
+        C.n = mk(o[0], o[1], o[2]); C.rA = mk(o[3], o[4], o[5]); C.rB = mk(o[6], o[7], o[8]);
+        C.t1 = mk(o[9], o[10], o[11]); C.t2 = mk(o[12], o[13], o[14]);
+        C.meff = o[15]; C.meft1 = o[16]; C.meft2 = o[17]; C.bias = o[18]; C.mu = o[19];
+        C.jn = A.Cc.jn[c]; C.jt1 = A.Cc.jt1[c]; C.jt2 = A.Cc.jt2[c]; C.jp = A.Cc.jp[c];
+    }
+}
+
+// big tiles: 1024 threads; pairs counting-sorted by colour into records in global scratch; body
+// velocities (then pseudo-velocities) in dynamic LDS when the tile fits, global memory otherwise
+#define T_BIG_WG 512
+template <int SOLVER>
+__global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T) {
+    constexpr bool FIXED = SOLVER == ME_SOLVER_COLOR_CACHE;
+    int t = blockIdx.x;
+    if (T.tb0[t] == INT_MAX) return;
+    int b0, nb, p0, np;
+    tile_bounds(T, t, b0, nb, p0, np);
+    if (is_small_tile(nb, np)) return;
+    extern __shared__ float dsm[];
+    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1];
+    __shared__ int maxc;
+    const int tid = threadIdx.x;
+    const bool lds = nb <= T_BIG_B;
+    if (tid == 0) {
+        atomicAdd(&A.istat[IS_TILES], 1);
+        atomicMax(&A.istat[IS_MAXTB], nb);
+        maxc = 0;
+    }
+    int ncol = A.istat[IS_NCOL];
+    if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
+    float *bv = lds ? dsm : A.b.v, *bw = lds ? dsm + 3 * T_BIG_B : A.b.w;
+    float *bpv = lds ? dsm : A.pv, *bpo = lds ? dsm + 3 * T_BIG_B : A.po;
+    if (lds)
+        for (int j = tid; j < nb; j += T_BIG_WG) {
+            int i = T.bsorted[b0 + j];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                bv[3 * j + k] = A.b.v[3 * i + k];
+                bw[3 * j + k] = A.b.w[3 * i + k];
+            }
+        }
+    for (int c = tid; c <= ncol; c += T_BIG_WG) { cst[c] = 0; cfl[c] = 0; }
+    __syncthreads();
+    for (int k = tid; k < np; k += T_BIG_WG) {
+        int c = A.Pp.col[T.psorted[p0 + k]];
+        if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+        atomicAdd(&cst[c], 1);
+        atomicMax(&maxc, c + 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int c = 0; c < maxc; ++c) { int n = cst[c]; cst[c] = run; run += n; }
+        cst[maxc] = run;
+    }
+    __syncthreads();
+    const float *invM = A.b.invM;
+    for (int k = tid; k < np; k += T_BIG_WG) {
+        int p = T.psorted[p0 + k];
+        int c = A.Pp.col[p];
+        if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+        int pos = cst[c] + atomicAdd(&cfl[c], 1);
+        int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
+        bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
+        int *r = T.rec + 8 * (size_t)(p0 + pos);
+        r[0] = p; r[1] = a; r[2] = b;
+        r[3] = (lds && dA) ? T.bslot[a] - b0 : a;
+        r[4] = b < 0 ? -1 : ((lds && dB) ? T.bslot[b] - b0 : b);
+        r[5] = A.Pp.pstart[p]; r[6] = A.Pp.pcount[p];
+        r[7] = (dA ? 1 : 0) | (dB ? 2 : 0);
+    }
+    __syncthreads();
+    const int nc = maxc;
+    const float t2d = FIXED ? 1.0f : A.t2d;
+    auto sweep = [&](int pass) {
+        for (int c = 0; c < nc; ++c) {
+            for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIG_WG) {
+                const int *r = T.rec + 8 * (size_t)(p0 + k);
+                PairCache P;
+                pc_from_rec(P, A, T, r);
+                if (pass == 0) pc_warm(P, bv, bw);
+                else if (pass == 1) pc_vel<FIXED>(P, bv, bw, t2d);
+                else pc_pos(P, bpv, bpo);
+                if (pass != 0) pc_store(P, A, r[0]);
+            }
+            __syncthreads();
+        }
+    };
+    if (np > 0) {
+        if (FIXED) sweep(0);
+        for (int it = 0; it < A.vit; ++it) sweep(1);
+    }
+    __syncthreads();
+    // velocities back to global memory; the LDS region now holds the pseudo-velocities
+    for (int j = tid; j < nb; j += T_BIG_WG) {
+        int i = T.bsorted[b0 + j];
+        if (lds) {
+            st3(A.b.v, i, ld3(bv, j));
+            st3(A.b.w, i, ld3(bw, j));
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int j = tid; j < nb; j += T_BIG_WG) {
+            st3(bpv, j, mk(0.0f, 0.0f, 0.0f));
+            st3(bpo, j, mk(0.0f, 0.0f, 0.0f));
+        }
+    __syncthreads();
+    if (np > 0)
+        for (int it = 0; it < A.pit; ++it) sweep(2);
+    __syncthreads();
+    for (int j = tid; j < nb; j += T_BIG_WG) {
+        int i = T.bsorted[b0 + j];
+        f3 pv = lds ? ld3(bpv, j) : ld3(A.pv, i), po = lds ? ld3(bpo, j) : ld3(A.po, i);
+        if (lds) { st3(A.pv, i, pv); st3(A.po, i, po); }
+        if (A.b.invM[i] > 0.0f) integrate_body(A.b.x, A.b.q, ld3(A.b.v, i), ld3(A.b.w, i), pv, po, i, A.dt);
     }
 }
 
@@ -429,22 +825,35 @@ inline int tiled_scan(const TiledArgs &A, const int *in, int *out, int n, int *t
 
 inline int tblk(long long n) { return (int)((n + ME_THREADS - 1) / ME_THREADS); }
 
-template <int WG>
-inline void launch_tiles(const TiledArgs &A, TiledBuffers &T, int ntile, hipStream_t s) {
-    hipLaunchKernelGGL(k_t_color<WG>, dim3(ntile), dim3(WG), 0, s, A, T);
-    hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
-    if (A.solver == ME_SOLVER_COLOR_CACHE)
-        hipLaunchKernelGGL((k_t_solve<WG, ME_SOLVER_COLOR_CACHE>), dim3(ntile), dim3(WG), 0, s, A, T);
-    else
-        hipLaunchKernelGGL((k_t_solve<WG, ME_SOLVER_PAIR_CHUNKS>), dim3(ntile), dim3(WG), 0, s, A, T);
+template <int SOLVER>
+inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStream_t s) {
+    const size_t dyn = 6 * sizeof(float) * T_BIG_B;
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)k_t_big<SOLVER>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
+            hipSuccess)
+            return ME_ERUNTIME;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
+    hipLaunchKernelGGL(k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
+    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile), dim3(T_BIG_WG), dyn, s, A, T);
+    return ME_OK;
 }
 
-// the whole coloured solve + integration of one substep
-inline int tiled_solve(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
+// the coloured solve + integration of one substep, in three parts so that the sweep kernels can be
+// timed on their own: prepare (warm start, cache, islands, tiles, colouring), sweeps, finish
+inline int tiled_ntile(const TiledArgs &A) {
+    int TB = A.tile_bodies > 0 ? A.tile_bodies : 128;
+    if (TB > T_SMALL_B) TB = T_SMALL_B;
+    return (A.N + TB - 1) / TB;
+}
+
+inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     int N = A.N, capP = A.capP, capC = A.capC;
-    int TB = A.tile_bodies > 0 ? A.tile_bodies : 64;
-    if (TB > T_MAXB) TB = T_MAXB;
-    int ntile = (N + TB - 1) / TB;
+    int TB = A.tile_bodies > 0 ? A.tile_bodies : 128;
+    if (TB > T_SMALL_B) TB = T_SMALL_B;
+    int ntile = tiled_ntile(A);
     bool cc = A.solver == ME_SOLVER_COLOR_CACHE;
     int g = capP > N ? capP : N;
     hipLaunchKernelGGL(k_zero_pvpo, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, A.pv, A.po);
@@ -467,15 +876,28 @@ inline int tiled_solve(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     if (rc) return rc;
     rc = tiled_scan(A, T.cnt_p, T.poff, N, A.istat + IS_NPAIRSORT, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T);
+    hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T, A.b.x, A.b.invM, A.invIw);
     hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
     hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
-    if (TB <= 64) launch_tiles<64>(A, T, ntile, s);
-    else launch_tiles<256>(A, T, ntile, s);
+    hipLaunchKernelGGL(k_t_color<256>, dim3(ntile), dim3(256), 0, s, A, T);
+    hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
+    hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);
+    return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;
+}
+
+inline int tiled_sweeps(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
+    int ntile = tiled_ntile(A);
+    return A.solver == ME_SOLVER_COLOR_CACHE ? launch_solve<ME_SOLVER_COLOR_CACHE>(A, T, ntile, s)
+                                             : launch_solve<ME_SOLVER_PAIR_CHUNKS>(A, T, ntile, s);
+}
+
+inline int tiled_finish(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
+    int N = A.N, capP = A.capP, capC = A.capC;
+    int g = capP > N ? capP : N;
     if (A.warm)
         hipLaunchKernelGGL(k_warm_store, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.Cc.key, A.Cc.jn,
                            A.Cc.jt1, A.Cc.jt2, A.wkey, A.wjn, A.wjt1, A.wjt2);
-    if (cc)
+    if (A.solver == ME_SOLVER_COLOR_CACHE)
         hipLaunchKernelGGL(k_t_cc_store, dim3(tblk(g)), dim3(ME_THREADS), 0, s, N, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
                            A.Pp.col, A.b.invM, A.cache_a, A.cache_b, A.cache_col, A.cache_invM);
     return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;

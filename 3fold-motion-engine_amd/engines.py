@@ -39,7 +39,7 @@ class _HipContactEngine:
     _solver = _lib.SOLVER_JACOBI
 
     def __init__(s, dims=(0.3, 0.3, 0.2), vit=40, pit=20, mu=0.5, relax=0.2, t2_damp=1.0,
-                 warm_start=True, margin=0.0, n_colors=0, tiled=False, tile_bodies=256,
+                 warm_start=True, margin=0.0, n_colors=0, tiled=False, tile_bodies=128,
                  max_pairs=None, max_contacts=None, device="cuda:0", check_every_step=True,
                  profile=False, per_contact=False):
         if vit < 0 or pit < 0:

@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--vit", type=int, default=16)
     ap.add_argument("--pit", type=int, default=10)
     ap.add_argument("--tiled", type=int, default=0)
-    ap.add_argument("--tile-bodies", type=int, default=256)
+    ap.add_argument("--tile-bodies", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
@@ -81,7 +81,6 @@ def main():
 
     import torch
     pkg = importlib.import_module("3fold-motion-engine_amd")
-    L = pkg._lib if hasattr(pkg, "_lib") else importlib.import_module("3fold-motion-engine_amd._lib")
     L = importlib.import_module("3fold-motion-engine_amd._lib")
     import ctypes as C
 
@@ -102,6 +101,7 @@ def main():
             print(f"[bench] warmup {i + 1}/{a.warmup} {time.perf_counter() - t_w:.1f}s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     e.refresh_counters()
+    print(f"[bench] at timed start: {e._counters}", file=sys.stderr, flush=True)
     x0, q0, v0, w0 = e.state_numpy()
     Lib = L.lib()
     L.check(Lib.me_profile(e._h, 1), "me_profile")
@@ -113,6 +113,7 @@ def main():
     t1 = time.perf_counter()
     L.check(Lib.me_profile(e._h, 0), "me_profile")
     cnt = e.refresh_counters()
+    print(f"[bench] at timed end: {cnt}", file=sys.stderr, flush=True)
     elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / a.steps
     value = a.bodies * a.steps / elapsed
@@ -130,7 +131,7 @@ def main():
     ab = algorithmic_bytes(pairs, bodies.invM, a.solver, a.vit, a.pit, S, a.bodies, cnt["n_contacts"])
     ncol = max(cnt["n_colors"], 1)
     per_launch = {"k_sweep_vel": ab["vel_sweep"] / ncol, "k_sweep_pos": ab["pos_sweep"] / ncol,
-                  "k_tiled_solve": ab["step"] / S}
+                  "k_tiled_solve": ab["warm"] + a.vit * ab["vel_sweep"] + a.pit * ab["pos_sweep"]}
     roofline = None
     if prof:
         dom = max((k for k in prof if k in per_launch), key=lambda k: prof[k][0], default=None)

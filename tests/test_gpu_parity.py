@@ -134,13 +134,13 @@ def test_colouring_is_proper_and_runs_repeat():
         assert np.array_equal(a, b2)
 
 
-@pytest.mark.parametrize("tile_bodies", [8, 64, 256])
-def test_tiled_chain_island_beyond_lds(oracle, tile_bodies):
-    """A row of 600 touching cubes is one island larger than the LDS tile capacity (512 bodies):
-    that tile runs on global memory; small tiles make islands straddle tile boundaries."""
-    _say(f"color_cache tiled chain 600 + pile 200, tile_bodies={tile_bodies}")
+@pytest.mark.parametrize("tile_bodies,n,steps", [(8, 600, 30), (64, 600, 30), (256, 600, 30), (128, 6600, 6)])
+def test_tiled_chain_island_beyond_lds(oracle, tile_bodies, n, steps):
+    """A row of touching cubes is one island: 600 bodies exceed a small tile (big-tile kernel, LDS
+    state), 6600 bodies exceed the big tile's LDS capacity (global-memory state); small tiles make
+    islands straddle tile boundaries."""
+    _say(f"color_cache tiled chain {n} + pile 200, tile_bodies={tile_bodies}")
     pkg = _pkg()
-    n = 600
     c = np.zeros((n, 3))
     c[:, 0] = np.arange(n) * 0.0995
     c[:, 2] = 0.0499
@@ -150,7 +150,7 @@ def test_tiled_chain_island_beyond_lds(oracle, tile_bodies):
     e = _engine("color_cache", vit=8, pit=4, n_colors=16, tiled=True, tile_bodies=tile_bodies)
     e.add_boxes(b)
     o = oracle.OracleWorld(b, "color_cache", vit=8, pit=4, n_colors=16)
-    for s in range(30):
+    for s in range(steps):
         e.step(1 / 240, 2)
         o.step(1 / 240, 2)
         _assert_state_equal(e, o, f"step {s}")
