@@ -16,7 +16,7 @@ FLAG_TILED = 1
 
 (BUF_KEY, BUF_BI, BUF_BJ, BUF_PA, BUF_PB, BUF_N, BUF_PEN, BUF_JN, BUF_JT1, BUF_JT2, BUF_JP,
  BUF_PBI, BUF_PBJ, BUF_PSTART, BUF_PCOUNT, BUF_COLOR, BUF_VSNAP, BUF_PV, BUF_PO, BUF_INVIW,
- BUF_ISLAND) = range(21)
+ BUF_ISLAND, BUF_DEBUG) = range(22)
 
 COUNTERS = ("n_contacts", "n_pairs", "n_colors", "cache_hits", "cache_misses", "overflow",
             "n_candidates", "n_islands", "n_tiles", "max_tile_bodies")

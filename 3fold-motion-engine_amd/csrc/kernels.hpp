@@ -256,6 +256,63 @@ __global__ void k_box_cand(int N, Bodies b, const float *aabb, const int *cellc,
     cand_n[a] = nc;
     for (int k = 0; k < nc; ++k) cand_tmp[a * ME_MAXCAND + k] = list[k];
 }
+// broad phase in two parallel passes: one thread per (body, neighbour cell) collects the higher
+// bodies of that cell whose AABB overlaps (k_box_cell); one thread per body merges its 27 lists
+// in ascending order behind its ground candidate (k_box_merge)
+#define ME_CELLCAND 16
+__global__ void k_box_cell(int N, Bodies b, const float *aabb, const int *cellc, const int *bstart, const int *bcount,
+                           const int *items, int T, float margin, int *cc_n, int *cc_list, int *istat) {
+    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= 27LL * N) return;
+    int a = (int)(gid / 27), cl = (int)(gid % 27);
+    int X = cellc[4 * a] + cl % 3 - 1, Y = cellc[4 * a + 1] + (cl / 3) % 3 - 1, Z = cellc[4 * a + 2] + cl / 9 - 1;
+    int s = cellc[4 * a + 3];
+    float m = margin + ME_BP_SLACK;
+    const float *A = aabb + 6 * a;
+    bool dynA = b.invM[a] > 0.0f;
+    int bk = (int)(cell_hash(X, Y, Z, s) & (uint32_t)(T - 1));
+    int st = bstart[bk], cnt = bcount[bk];
+    int n = 0;
+    for (int t = 0; t < cnt; ++t) {
+        int j = items[st + t];
+        if (j <= a) continue;
+        if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z || cellc[4 * j + 3] != s) continue;
+        if (!dynA && !(b.invM[j] > 0.0f)) continue;
+        if (!aabb_overlap(A, aabb + 6 * j, m)) continue;
+        if (n < ME_CELLCAND) cc_list[gid * ME_CELLCAND + n] = j;
+        else atomicOr(&istat[IS_OVF], OVF_CAND);
+        n++;
+    }
+    cc_n[gid] = n < ME_CELLCAND ? n : ME_CELLCAND;
+}
+__global__ void k_box_merge(int N, Bodies b, const float *aabb, float margin, const int *cc_n, const int *cc_list,
+                            int *cand_n, int *cand_tmp, int *istat) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    int list[ME_MAXCAND];
+    int nc = 0;
+    bool over = false;
+    if (b.invM[a] > 0.0f && aabb[6 * a + 2] < margin + ME_BP_SLACK) list[nc++] = -1;
+    int first = nc;
+    for (int cl = 0; cl < 27; ++cl) {
+        long long g = 27LL * a + cl;
+        int n = cc_n[g];
+        for (int t = 0; t < n; ++t) {
+            int j = cc_list[g * ME_CELLCAND + t];
+            if (nc < ME_MAXCAND) {
+                int k = nc++;
+                while (k > first && list[k - 1] > j) { list[k] = list[k - 1]; --k; }
+                list[k] = j;
+            } else {
+                over = true;
+            }
+        }
+    }
+    if (over) atomicOr(&istat[IS_OVF], OVF_CAND);
+    cand_n[a] = nc;
+    for (int k = 0; k < nc; ++k) cand_tmp[a * ME_MAXCAND + k] = list[k];
+}
+
 __global__ void k_box_cand_emit(int N, const int *cand_n, const int *cand_off, const int *cand_tmp, int cap,
                                 int *cand_a, int *cand_b, int *istat) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;

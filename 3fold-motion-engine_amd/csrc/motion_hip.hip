@@ -73,6 +73,7 @@ struct me_engine {
     int *cellc, *bucket, *bcount, *bstart, *bcursor, *items;
     // candidates and narrow-phase slots
     int *cand_n, *cand_off, *cand_tmp, *cand_a, *cand_b;
+    int *cc_n, *cc_list;  // per (body, neighbour cell) candidate lists
     Slots S;
     int *slot_off, *pair_flag, *pair_off;
     // contacts, pairs
@@ -91,6 +92,8 @@ struct me_engine {
     int nbsum;
     int *istat;
     int *hstat;  // pinned host mirror
+    hipStream_t side = nullptr;  // second stream for the big-tile sweeps (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // HIP-event timing of selected kernels (measurement only)
     bool prof_on = false;
     std::vector<hipEvent_t> prof_ev[ME_K_COUNT_];
@@ -145,6 +148,8 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->cand_n = B.take<int>((size_t)items);
     e->cand_off = B.take<int>((size_t)items);
     e->cand_tmp = B.take<int>(P > 0 ? 1 : (size_t)N * ME_MAXCAND);
+    e->cc_n = B.take<int>(P > 0 ? 1 : 27 * (size_t)N);
+    e->cc_list = B.take<int>(P > 0 ? 1 : 27 * (size_t)ME_CELLCAND * N);
     e->cand_a = B.take<int>((size_t)e->cap_cand);
     e->cand_b = B.take<int>((size_t)e->cap_cand);
     size_t ns = P > 0 ? 1 : 4 * (size_t)e->cap_cand;
@@ -277,8 +282,10 @@ int generate(me_engine *e, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_cell, dim3(1), dim3(1), 0, s, e->istat, e->cfg.margin);
     int rc = build_grid(e, N, e->b.x, e->b.scene, 1, -1.0f, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_box_cand, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cellc, e->bstart,
-                       e->bcount, e->items, e->T, e->cfg.margin, e->cand_n, e->cand_tmp, e->istat);
+    hipLaunchKernelGGL(k_box_cell, dim3(nblk(27LL * N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cellc, e->bstart,
+                       e->bcount, e->items, e->T, e->cfg.margin, e->cc_n, e->cc_list, e->istat);
+    hipLaunchKernelGGL(k_box_merge, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cfg.margin, e->cc_n,
+                       e->cc_list, e->cand_n, e->cand_tmp, e->istat);
     rc = scan(e, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_box_cand_emit, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->cand_n, e->cand_off,
@@ -500,6 +507,7 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.istat = e->istat;
     a.bsum = e->bsum; a.nbsum = e->nbsum;
     a.claim = e->claim;
+    a.side = e->side; a.ev_fork = e->ev_fork; a.ev_join = e->ev_join;
     return a;
 }
 
@@ -541,6 +549,13 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
         delete e;
         return fail(ME_ERUNTIME, "hipHostMalloc failed");
     }
+    if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipHostFree(e->hstat);
+        delete e;
+        return fail(ME_ERUNTIME, "stream/event creation failed");
+    }
     hipStream_t s = (hipStream_t)stream;
     // zero the workspace with a fill kernel (no memset nodes, so a caller may capture later steps)
     size_t words = need / 4;
@@ -562,7 +577,10 @@ int me_destroy(me_engine *e) {
     if (!e) return fail(ME_EARG, "engine is NULL");
     for (int k = 0; k < ME_K_COUNT_; ++k)
         for (hipEvent_t ev : e->prof_ev[k]) (void)hipEventDestroy(ev);
-    if (e->hstat) hipHostFree(e->hstat);
+    if (e->hstat) (void)hipHostFree(e->hstat);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+    if (e->side) (void)hipStreamDestroy(e->side);
     delete e;
     return ME_OK;
 }
@@ -613,6 +631,7 @@ int me_buffer(me_engine *e, int id, void **ptr, int64_t *count) {
         case ME_BUF_PO: *ptr = e->po; *count = 3 * N; break;
         case ME_BUF_INVIW: *ptr = e->invIw; *count = 9 * N; break;
         case ME_BUF_ISLAND: *ptr = e->tb.root; *count = N; break;
+        case ME_BUF_DEBUG: *ptr = e->tb.dbg; *count = 16; break;
         default: return fail(ME_EARG, "unknown buffer id");
     }
     return ME_OK;

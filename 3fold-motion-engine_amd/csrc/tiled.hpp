@@ -23,8 +23,9 @@
 
 namespace me {
 
-#define T_MAXB 512   // bodies of a tile resident in LDS
-#define T_MAXP 1024  // pairs of a tile sorted by colour in LDS
+#define T_COLB 8192   // bodies of a tile whose JP claims sit in LDS
+#define T_COLP 16384  // pairs of a tile whose colours sit in LDS
+#define T_COL_DYN (8 * T_COLB + 4 * T_COLP)
 
 struct TiledBuffers {
     int *parent, *root, *proot;
@@ -36,6 +37,9 @@ struct TiledBuffers {
     int *psorted2;               // big tiles: pairs counting-sorted by colour
     int *rec;                    // big tiles: 8 ints per colour-sorted pair (p, a, b, la, lb, st, cnt, flags)
     float *prep;                 // 20 floats per contact: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu
+    float *sprep;                // big tiles: 4 x 24 floats per colour-sorted pair (prep + impulses)
+    int *srec;                   // big tiles: 4 ints per colour-sorted pair (p, a, b, first contact)
+    long long *dbg;              // phase timestamps (wall clock) of the last big tile of >= 1000 bodies
 };
 
 struct TiledArgs {
@@ -52,6 +56,7 @@ struct TiledArgs {
     int *istat, *bsum;
     int nbsum;
     unsigned long long *claim;
+    void *side, *ev_fork, *ev_join;  // side stream and fork/join events of the sweep kernels
 };
 
 template <class BumpT>
@@ -78,10 +83,13 @@ inline void tiled_layout(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
     t.cbase = B.template take<int>(ME_NCOLMAX + 2);
     t.xs = B.template take<float>(3 * n);
     t.invMs = B.template take<float>(n);
-    t.invIws = B.template take<float>(9 * n);
+    t.invIws = B.template take<float>(12 * n);
     t.psorted2 = B.template take<int>(p);
     t.rec = B.template take<int>(8 * p);
     t.prep = B.template take<float>(20 * (size_t)capC);
+    t.sprep = B.template take<float>(96 * p);
+    t.srec = B.template take<int>(4 * p);
+    t.dbg = B.template take<long long>(16);
 }
 
 // IS_MISMATCH after k_t_mode: 0 = ColorCache hit, 1 = colour this substep, 2 = no contacts
@@ -185,7 +193,8 @@ __global__ void k_t_scatter_b(int N, TiledBuffers T, const float *x, const float
     for (int k = 0; k < 3; ++k) T.xs[3 * pos + k] = x[3 * i + k];
     T.invMs[pos] = invM[i];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) T.invIws[9 * pos + k] = invIw[9 * i + k];
+    for (int k = 0; k < 9; ++k) T.invIws[12 * pos + k] = invIw[9 * i + k];
+    T.invIws[12 * pos + 9] = 0.0f; T.invIws[12 * pos + 10] = 0.0f; T.invIws[12 * pos + 11] = 0.0f;
 }
 
 __global__ void k_t_scatter_p(int capP, const int *istat, TiledBuffers T) {
@@ -218,10 +227,11 @@ __global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T) {
     if (b0 == INT_MAX) return;
     int nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - T.tp0[t];
     if (np <= 0) return;
-    __shared__ unsigned long long sclaim[T_MAXB];
-    __shared__ int colL[T_MAXP];
+    extern __shared__ unsigned long long cdsm[];
+    unsigned long long *sclaim = cdsm;
+    int *colL = reinterpret_cast<int *>(cdsm + T_COLB);
     __shared__ int left;
-    const bool lds = nb <= T_MAXB && np <= T_MAXP;
+    const bool lds = nb <= T_COLB && np <= T_COLP;
     unsigned long long *claim = lds ? sclaim : A.claim;
     const float *invM = A.b.invM;
     int *gcol = A.Pp.col;
@@ -310,7 +320,7 @@ __global__ void k_t_apply_split(int capP, const int *istat, int *col, TiledBuffe
 
 #define T_SMALL_P 256   // pairs of a small tile: one per thread, constants in registers
 #define T_SMALL_B 512   // bodies of a small tile (velocities in LDS)
-#define T_BIG_B 6400    // bodies of a big tile whose v, w (then pv, po) fit the dynamic LDS (150 KB)
+#define T_BIG_B 5000    // bodies of a big tile whose v, w (then pv, po) sit in LDS (120 KB)
 
 // per-contact constants of one substep's solve (identical to recomputing them every sweep: their
 // inputs -- normal, contact points, centres, world inverse inertia -- do not change in the solve)
@@ -397,7 +407,101 @@ __device__ __forceinline__ void pc_store(const PairCache &P, const TiledArgs &A,
     }
 }
 
-// apply_warm (contact_engine_gpu_applied_warm.py:7-54) on the cached pair
+// one contact of apply_warm (contact_engine_gpu_applied_warm.py:7-54)
+__device__ __forceinline__ void cc_warm(const CCon &C, f3 &va, f3 &wa, f3 &vb, f3 &wb, bool hasB, float iMA,
+                                        float iMB, const m3 &IA, const m3 &IB) {
+    f3 imp = (C.jn * C.n + C.jt1 * C.t1) + C.jt2 * C.t2;
+    if (iMA > 0.0f) {
+        va = va + iMA * imp;
+        wa = wa + mv(IA, cross(C.rA, imp));
+    }
+    if (hasB && iMB > 0.0f) {
+        vb = vb - iMB * imp;
+        wb = wb - mv(IB, cross(C.rB, imp));
+    }
+}
+
+// one contact of the chunk velocity sweep, expression for expression as chunk_vel
+template <bool FIXED>
+__device__ __forceinline__ void cc_vel(CCon &C, f3 &vA, f3 &wA, f3 &vB, f3 &wB, bool hasB, float iMA, float iMB,
+                                       const m3 &IA, const m3 &IB, float t2d) {
+    const f3 nrm = C.n, rA = C.rA, rB = C.rB;
+    f3 va = vA + cross(wA, rA);
+    f3 vb = mk(0.0f, 0.0f, 0.0f);
+    if (hasB) vb = vB + cross(wB, rB);
+    f3 vrel = va - vb;
+    float vn = dot(vrel, nrm);
+    f3 t1, t2;
+    float meft1, meft2;
+    if (FIXED) {
+        t1 = C.t1; t2 = C.t2;
+    } else {
+        f3 vt = vrel - nrm * vn;
+        float mt = length(vt);
+        t1 = vt / mt;
+        if (mt <= 5e-3f) t1 = C.t1;
+        t2 = cross(nrm, t1);
+    }
+    if (!(C.meff >= 1e-12f)) return;
+    float jo = C.jn;
+    float dj = -vn / C.meff;
+    float nw = wmax(0.0f, jo + dj);
+    dj = nw - jo;
+    C.jn = nw;
+    f3 Jn = nrm * dj;
+    if (iMA > 0.0f) { vA = vA + Jn * iMA; wA = wA + mv(IA, cross(rA, Jn)); }
+    if (hasB && iMB > 0.0f) { vB = vB - Jn * iMB; wB = wB - mv(IB, cross(rB, Jn)); }
+    va = vA + cross(wA, rA);
+    vb = mk(0.0f, 0.0f, 0.0f);
+    if (hasB) vb = vB + cross(wB, rB);
+    vrel = va - vb;
+    if (FIXED) {
+        meft1 = C.meft1; meft2 = C.meft2;
+    } else {
+        f3 rtA1 = cross(rA, t1);
+        meft1 = iMA + dot(rtA1, mv(IA, rtA1));
+        f3 rtA2 = cross(rA, t2);
+        meft2 = iMA + dot(rtA2, mv(IA, rtA2));
+        if (hasB) {
+            f3 rtB1 = cross(rB, t1);
+            meft1 = meft1 + iMB + dot(rtB1, mv(IB, rtB1));
+            f3 rtB2 = cross(rB, t2);
+            meft2 = meft2 + iMB + dot(rtB2, mv(IB, rtB2));
+        }
+    }
+    if (meft1 >= 1e-12f && meft2 >= 1e-12f) {
+        float o1 = C.jt1, o2 = C.jt2;
+        float a1 = o1 - dot(vrel, t1) / meft1;
+        float a2 = o2 - dot(vrel, t2) / meft2;
+        float mag = sqrtf(a1 * a1 + a2 * a2);
+        float lim = C.mu * nw;
+        if (mag > lim && mag > 1e-12f) { a1 = a1 * lim / mag; a2 = a2 * lim / mag; }
+        C.jt1 = a1;
+        C.jt2 = a2;
+        f3 Jt = t1 * (a1 - o1) + t2 * (t2d * (a2 - o2));
+        if (iMA > 0.0f) { vA = vA + Jt * iMA; wA = wA + mv(IA, cross(rA, Jt)); }
+        if (hasB && iMB > 0.0f) { vB = vB - Jt * iMB; wB = wB - mv(IB, cross(rB, Jt)); }
+    }
+}
+
+// one contact of the chunk position sweep, as chunk_pos
+__device__ __forceinline__ void cc_pos(CCon &C, f3 &pvA, f3 &poA, f3 &pvB, f3 &poB, bool hasB, float iMA, float iMB,
+                                       const m3 &IA, const m3 &IB) {
+    const f3 nrm = C.n, rA = C.rA, rB = C.rB;
+    f3 rel = pvA + cross(poA, rA);
+    if (hasB) rel = rel - (pvB + cross(poB, rB));
+    if (!(C.meff >= 1e-12f)) return;
+    float dj = (C.bias - dot(rel, nrm)) / C.meff;
+    float jo = C.jp;
+    float nw = wmax(0.0f, jo + dj);
+    dj = nw - jo;
+    C.jp = nw;
+    f3 Jn = nrm * dj;
+    if (iMA > 0.0f) { pvA = pvA + Jn * iMA; poA = poA + mv(IA, cross(rA, Jn)); }
+    if (hasB && iMB > 0.0f) { pvB = pvB - Jn * iMB; poB = poB - mv(IB, cross(rB, Jn)); }
+}
+
+// the cached pair: bodies from LDS (dynamic) or the cached static state
 __device__ __forceinline__ void pc_warm(const PairCache &P, float *sv, float *sw) {
     f3 va = P.dynA ? ld3(sv, P.la) : P.vAs, wa = P.dynA ? ld3(sw, P.la) : P.wAs;
     f3 vb = mk(0.0f, 0.0f, 0.0f), wb = mk(0.0f, 0.0f, 0.0f);
@@ -405,119 +509,45 @@ __device__ __forceinline__ void pc_warm(const PairCache &P, float *sv, float *sw
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= P.cnt) break;
-        const CCon &C = P.c[k];
-        f3 imp = (C.jn * C.n + C.jt1 * C.t1) + C.jt2 * C.t2;
-        if (P.iMA > 0.0f) {
-            va = va + P.iMA * imp;
-            wa = wa + mv(P.IA, cross(C.rA, imp));
-        }
-        if (P.hasB && P.iMB > 0.0f) {
-            vb = vb - P.iMB * imp;
-            wb = wb - mv(P.IB, cross(C.rB, imp));
-        }
+        cc_warm(P.c[k], va, wa, vb, wb, P.hasB, P.iMA, P.iMB, P.IA, P.IB);
     }
     if (P.iMA > 0.0f) { st3(sv, P.la, va); st3(sw, P.la, wa); }
     if (P.hasB && P.iMB > 0.0f) { st3(sv, P.lb, vb); st3(sw, P.lb, wb); }
 }
 
-// chunk velocity sweep on the cached pair, expression for expression as chunk_vel
 template <bool FIXED>
 __device__ __forceinline__ void pc_vel(PairCache &P, float *sv, float *sw, float t2d) {
     f3 vA = P.dynA ? ld3(sv, P.la) : P.vAs, wA = P.dynA ? ld3(sw, P.la) : P.wAs;
     f3 vB = mk(0.0f, 0.0f, 0.0f), wB = mk(0.0f, 0.0f, 0.0f);
     if (P.hasB) { vB = P.dynB ? ld3(sv, P.lb) : P.vBs; wB = P.dynB ? ld3(sw, P.lb) : P.wBs; }
-    const float iMA = P.iMA, iMB = P.iMB;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= P.cnt) break;
-        CCon &C = P.c[k];
-        const f3 nrm = C.n, rA = C.rA, rB = C.rB;
-        f3 va = vA + cross(wA, rA);
-        f3 vb = mk(0.0f, 0.0f, 0.0f);
-        if (P.hasB) vb = vB + cross(wB, rB);
-        f3 vrel = va - vb;
-        float vn = dot(vrel, nrm);
-        f3 t1, t2;
-        float meft1, meft2;
-        if (FIXED) {
-            t1 = C.t1; t2 = C.t2;
-        } else {
-            f3 vt = vrel - nrm * vn;
-            float mt = length(vt);
-            t1 = vt / mt;
-            if (mt <= 5e-3f) t1 = C.t1;
-            t2 = cross(nrm, t1);
-        }
-        if (!(C.meff >= 1e-12f)) continue;
-        float jo = C.jn;
-        float dj = -vn / C.meff;
-        float nw = wmax(0.0f, jo + dj);
-        dj = nw - jo;
-        C.jn = nw;
-        f3 Jn = nrm * dj;
-        if (iMA > 0.0f) { vA = vA + Jn * iMA; wA = wA + mv(P.IA, cross(rA, Jn)); }
-        if (P.hasB && iMB > 0.0f) { vB = vB - Jn * iMB; wB = wB - mv(P.IB, cross(rB, Jn)); }
-        va = vA + cross(wA, rA);
-        vb = mk(0.0f, 0.0f, 0.0f);
-        if (P.hasB) vb = vB + cross(wB, rB);
-        vrel = va - vb;
-        if (FIXED) {
-            meft1 = C.meft1; meft2 = C.meft2;
-        } else {
-            f3 rtA1 = cross(rA, t1);
-            meft1 = iMA + dot(rtA1, mv(P.IA, rtA1));
-            f3 rtA2 = cross(rA, t2);
-            meft2 = iMA + dot(rtA2, mv(P.IA, rtA2));
-            if (P.hasB) {
-                f3 rtB1 = cross(rB, t1);
-                meft1 = meft1 + iMB + dot(rtB1, mv(P.IB, rtB1));
-                f3 rtB2 = cross(rB, t2);
-                meft2 = meft2 + iMB + dot(rtB2, mv(P.IB, rtB2));
-            }
-        }
-        if (meft1 >= 1e-12f && meft2 >= 1e-12f) {
-            float o1 = C.jt1, o2 = C.jt2;
-            float a1 = o1 - dot(vrel, t1) / meft1;
-            float a2 = o2 - dot(vrel, t2) / meft2;
-            float mag = sqrtf(a1 * a1 + a2 * a2);
-            float lim = C.mu * nw;
-            if (mag > lim && mag > 1e-12f) { a1 = a1 * lim / mag; a2 = a2 * lim / mag; }
-            C.jt1 = a1;
-            C.jt2 = a2;
-            f3 Jt = t1 * (a1 - o1) + t2 * (t2d * (a2 - o2));
-            if (iMA > 0.0f) { vA = vA + Jt * iMA; wA = wA + mv(P.IA, cross(rA, Jt)); }
-            if (P.hasB && iMB > 0.0f) { vB = vB - Jt * iMB; wB = wB - mv(P.IB, cross(rB, Jt)); }
-        }
+        cc_vel<FIXED>(P.c[k], vA, wA, vB, wB, P.hasB, P.iMA, P.iMB, P.IA, P.IB, t2d);
     }
-    if (iMA > 0.0f) { st3(sv, P.la, vA); st3(sw, P.la, wA); }
-    if (P.hasB && iMB > 0.0f) { st3(sv, P.lb, vB); st3(sw, P.lb, wB); }
+    if (P.iMA > 0.0f) { st3(sv, P.la, vA); st3(sw, P.la, wA); }
+    if (P.hasB && P.iMB > 0.0f) { st3(sv, P.lb, vB); st3(sw, P.lb, wB); }
 }
 
-// chunk position sweep on the cached pair, as chunk_pos
 __device__ __forceinline__ void pc_pos(PairCache &P, float *spv, float *spo) {
     f3 pvA = P.dynA ? ld3(spv, P.la) : P.pvAs, poA = P.dynA ? ld3(spo, P.la) : P.poAs;
     f3 pvB = mk(0.0f, 0.0f, 0.0f), poB = mk(0.0f, 0.0f, 0.0f);
     if (P.hasB) { pvB = P.dynB ? ld3(spv, P.lb) : P.pvBs; poB = P.dynB ? ld3(spo, P.lb) : P.poBs; }
-    const float iMA = P.iMA, iMB = P.iMB;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= P.cnt) break;
-        CCon &C = P.c[k];
-        const f3 nrm = C.n, rA = C.rA, rB = C.rB;
-        f3 rel = pvA + cross(poA, rA);
-        if (P.hasB) rel = rel - (pvB + cross(poB, rB));
-        if (!(C.meff >= 1e-12f)) continue;
-        float dj = (C.bias - dot(rel, nrm)) / C.meff;
-        float jo = C.jp;
-        float nw = wmax(0.0f, jo + dj);
-        dj = nw - jo;
-        C.jp = nw;
-        f3 Jn = nrm * dj;
-        if (iMA > 0.0f) { pvA = pvA + Jn * iMA; poA = poA + mv(P.IA, cross(rA, Jn)); }
-        if (P.hasB && iMB > 0.0f) { pvB = pvB - Jn * iMB; poB = poB - mv(P.IB, cross(rB, Jn)); }
+        cc_pos(P.c[k], pvA, poA, pvB, poB, P.hasB, P.iMA, P.iMB, P.IA, P.IB);
     }
-    if (iMA > 0.0f) { st3(spv, P.la, pvA); st3(spo, P.la, poA); }
-    if (P.hasB && iMB > 0.0f) { st3(spv, P.lb, pvB); st3(spo, P.lb, poB); }
+    if (P.iMA > 0.0f) { st3(spv, P.la, pvA); st3(spo, P.la, poA); }
+    if (P.hasB && P.iMB > 0.0f) { st3(spv, P.lb, pvB); st3(spo, P.lb, poB); }
+}
+
+// workgroup barrier that orders LDS only: s_waitcnt lgkmcnt(0) + s_barrier, global stores stay in
+// flight (for sweeps whose cross-thread state is all in LDS)
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 __device__ __forceinline__ void tile_bounds(const TiledBuffers &T, int t, int &b0, int &nb, int &p0, int &np) {
@@ -610,19 +640,19 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers
         if (FIXED) {
             for (int c = 0; c < nc; ++c) {
                 if (c_own == c) pc_warm(P, sv, sw);
-                __syncthreads();
+                lds_barrier();
             }
         }
         const float t2d = FIXED ? 1.0f : A.t2d;
         for (int it = 0; it < A.vit; ++it)
             for (int c = 0; c < nc; ++c) {
                 if (c_own == c) pc_vel<FIXED>(P, sv, sw, t2d);
-                __syncthreads();
+                lds_barrier();
             }
         for (int it = 0; it < A.pit; ++it)
             for (int c = 0; c < nc; ++c) {
                 if (c_own == c) pc_pos(P, spv, spo);
-                __syncthreads();
+                lds_barrier();
             }
     }
     if (p >= 0) pc_store(P, A, p);
@@ -671,147 +701,7 @@ __global__ void k_t_prep(TiledArgs A, TiledBuffers T) {
     o[19] = A.Cc.mu[c];
 }
 
-// PairCache of a colour-sorted big-tile record; v / w (or pv / po) come from the sweep arrays
-__device__ __forceinline__ void pc_from_rec(PairCache &P, const TiledArgs &A, const TiledBuffers &T, const int *r) {
-    int a = r[1], b = r[2], flags = r[7];
-    P.la = r[3]; P.lb = r[4]; P.cnt = r[6];
-    P.dynA = flags & 1; P.dynB = (flags & 2) != 0; P.hasB = b >= 0;
-    const float *invM = A.b.invM;
-    P.iMA = invM[a];
-    P.iMB = P.hasB ? invM[b] : 0.0f;
-    P.IA = ldm(A.invIw, a);
-    P.IB = P.hasB ? ldm(A.invIw, b) : P.IA;
-    f3 z = mk(0.0f, 0.0f, 0.0f);
-    P.vAs = P.dynA ? z : ld3(A.b.v, a); P.wAs = P.dynA ? z : ld3(A.b.w, a);
-    P.pvAs = P.dynA ? z : ld3(A.pv, a); P.poAs = P.dynA ? z : ld3(A.po, a);
-    bool sB = P.hasB && !P.dynB;
-    P.vBs = sB ? ld3(A.b.v, b) : z; P.wBs = sB ? ld3(A.b.w, b) : z;
-    P.pvBs = sB ? ld3(A.pv, b) : z; P.poBs = sB ? ld3(A.po, b) : z;
-    int st = r[5];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k >= P.cnt) break;
-        int c = st + k;
-        const float *o = T.prep + 20 * (size_t)c;
-        CCon &C = P.c[k];
-        C.n = mk(o[0], o[1], o[2]); C.rA = mk(o[3], o[4], o[5]); C.rB = mk(o[6], o[7], o[8]);
-        C.t1 = mk(o[9], o[10], o[11]); C.t2 = mk(o[12], o[13], o[14]);
-        C.meff = o[15]; C.meft1 = o[16]; C.meft2 = o[17]; C.bias = o[18]; C.mu = o[19];
-        C.jn = A.Cc.jn[c]; C.jt1 = A.Cc.jt1[c]; C.jt2 = A.Cc.jt2[c]; C.jp = A.Cc.jp[c];
-    }
-}
-
-// big tiles: 1024 threads; pairs counting-sorted by colour into records in global scratch; body
-// velocities (then pseudo-velocities) in dynamic LDS when the tile fits, global memory otherwise
-#define T_BIG_WG 512
-template <int SOLVER>
-__global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T) {
-    constexpr bool FIXED = SOLVER == ME_SOLVER_COLOR_CACHE;
-    int t = blockIdx.x;
-    if (T.tb0[t] == INT_MAX) return;
-    int b0, nb, p0, np;
-    tile_bounds(T, t, b0, nb, p0, np);
-    if (is_small_tile(nb, np)) return;
-    extern __shared__ float dsm[];
-    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1];
-    __shared__ int maxc;
-    const int tid = threadIdx.x;
-    const bool lds = nb <= T_BIG_B;
-    if (tid == 0) {
-        atomicAdd(&A.istat[IS_TILES], 1);
-        atomicMax(&A.istat[IS_MAXTB], nb);
-        maxc = 0;
-    }
-    int ncol = A.istat[IS_NCOL];
-    if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
-    float *bv = lds ? dsm : A.b.v, *bw = lds ? dsm + 3 * T_BIG_B : A.b.w;
-    float *bpv = lds ? dsm : A.pv, *bpo = lds ? dsm + 3 * T_BIG_B : A.po;
-    if (lds)
-        for (int j = tid; j < nb; j += T_BIG_WG) {
-            int i = T.bsorted[b0 + j];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                bv[3 * j + k] = A.b.v[3 * i + k];
-                bw[3 * j + k] = A.b.w[3 * i + k];
-            }
-        }
-    for (int c = tid; c <= ncol; c += T_BIG_WG) { cst[c] = 0; cfl[c] = 0; }
-    __syncthreads();
-    for (int k = tid; k < np; k += T_BIG_WG) {
-        int c = A.Pp.col[T.psorted[p0 + k]];
-        if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
-        atomicAdd(&cst[c], 1);
-        atomicMax(&maxc, c + 1);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int c = 0; c < maxc; ++c) { int n = cst[c]; cst[c] = run; run += n; }
-        cst[maxc] = run;
-    }
-    __syncthreads();
-    const float *invM = A.b.invM;
-    for (int k = tid; k < np; k += T_BIG_WG) {
-        int p = T.psorted[p0 + k];
-        int c = A.Pp.col[p];
-        if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
-        int pos = cst[c] + atomicAdd(&cfl[c], 1);
-        int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
-        bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
-        int *r = T.rec + 8 * (size_t)(p0 + pos);
-        r[0] = p; r[1] = a; r[2] = b;
-        r[3] = (lds && dA) ? T.bslot[a] - b0 : a;
-        r[4] = b < 0 ? -1 : ((lds && dB) ? T.bslot[b] - b0 : b);
-        r[5] = A.Pp.pstart[p]; r[6] = A.Pp.pcount[p];
-        r[7] = (dA ? 1 : 0) | (dB ? 2 : 0);
-    }
-    __syncthreads();
-    const int nc = maxc;
-    const float t2d = FIXED ? 1.0f : A.t2d;
-    auto sweep = [&](int pass) {
-        for (int c = 0; c < nc; ++c) {
-            for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIG_WG) {
-                const int *r = T.rec + 8 * (size_t)(p0 + k);
-                PairCache P;
-                pc_from_rec(P, A, T, r);
-                if (pass == 0) pc_warm(P, bv, bw);
-                else if (pass == 1) pc_vel<FIXED>(P, bv, bw, t2d);
-                else pc_pos(P, bpv, bpo);
-                if (pass != 0) pc_store(P, A, r[0]);
-            }
-            __syncthreads();
-        }
-    };
-    if (np > 0) {
-        if (FIXED) sweep(0);
-        for (int it = 0; it < A.vit; ++it) sweep(1);
-    }
-    __syncthreads();
-    // velocities back to global memory; the LDS region now holds the pseudo-velocities
-    for (int j = tid; j < nb; j += T_BIG_WG) {
-        int i = T.bsorted[b0 + j];
-        if (lds) {
-            st3(A.b.v, i, ld3(bv, j));
-            st3(A.b.w, i, ld3(bw, j));
-        }
-    }
-    __syncthreads();
-    if (lds)
-        for (int j = tid; j < nb; j += T_BIG_WG) {
-            st3(bpv, j, mk(0.0f, 0.0f, 0.0f));
-            st3(bpo, j, mk(0.0f, 0.0f, 0.0f));
-        }
-    __syncthreads();
-    if (np > 0)
-        for (int it = 0; it < A.pit; ++it) sweep(2);
-    __syncthreads();
-    for (int j = tid; j < nb; j += T_BIG_WG) {
-        int i = T.bsorted[b0 + j];
-        f3 pv = lds ? ld3(bpv, j) : ld3(A.pv, i), po = lds ? ld3(bpo, j) : ld3(A.po, i);
-        if (lds) { st3(A.pv, i, pv); st3(A.po, i, po); }
-        if (A.b.invM[i] > 0.0f) integrate_body(A.b.x, A.b.q, ld3(A.b.v, i), ld3(A.b.w, i), pv, po, i, A.dt);
-    }
-}
+#include "tiled_big.hpp"
 
 inline int tiled_scan(const TiledArgs &A, const int *in, int *out, int n, int *total, hipStream_t s) {
     int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
@@ -825,9 +715,12 @@ inline int tiled_scan(const TiledArgs &A, const int *in, int *out, int n, int *t
 
 inline int tblk(long long n) { return (int)((n + ME_THREADS - 1) / ME_THREADS); }
 
+// small tiles on stream s, big tiles on the side stream s2 (disjoint islands), joined on s
 template <int SOLVER>
 inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStream_t s) {
-    const size_t dyn = 6 * sizeof(float) * T_BIG_B;
+    hipStream_t s2 = (hipStream_t)A.side;
+    hipEvent_t fork = (hipEvent_t)A.ev_fork, join = (hipEvent_t)A.ev_join;
+    const size_t dyn = T_BIG_DYN;
     static bool attr_set = false;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void *)k_t_big<SOLVER>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
@@ -836,8 +729,10 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
         attr_set = true;
     }
     hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
+    if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
-    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile), dim3(T_BIG_WG), dyn, s, A, T);
+    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile), dim3(T_BIG_WG), dyn, s2, A, T);
+    if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return ME_ERUNTIME;
     return ME_OK;
 }
 
@@ -879,7 +774,14 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T, A.b.x, A.b.invM, A.invIw);
     hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
     hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
-    hipLaunchKernelGGL(k_t_color<256>, dim3(ntile), dim3(256), 0, s, A, T);
+    static bool col_attr = false;
+    if (!col_attr) {
+        if (hipFuncSetAttribute((const void *)k_t_color<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)T_COL_DYN) != hipSuccess)
+            return ME_ERUNTIME;
+        col_attr = true;
+    }
+    hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile), dim3(1024), T_COL_DYN, s, A, T);
     hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
     hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);
     return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;

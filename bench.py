@@ -72,6 +72,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large"])
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--seq-rounds", type=int, default=3)
+    ap.add_argument("--autotune", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -79,12 +83,19 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if a.workload == "batched":
+        return bench_batched(a)
+    if a.workload == "large":
+        a.bodies = 100000 if a.bodies == 10000 else a.bodies
+        a.tiled = 1
+
     import torch
     pkg = importlib.import_module("3fold-motion-engine_amd")
     L = importlib.import_module("3fold-motion-engine_amd._lib")
     import ctypes as C
 
-    bodies = pkg.scenes.box_pile(a.bodies, seed=a.seed)
+    bodies = (pkg.scenes.box_pile(a.bodies, seed=a.seed, mass_ratio=100.0) if a.workload == "large"
+              else pkg.scenes.box_pile(a.bodies, seed=a.seed))
     cls = {"color_cache": pkg.ColorCacheContactEngine, "pair_chunks": pkg.GraphColoredContactEngine,
            "jacobi": pkg.RelaxedJacobiContactEngine}[a.solver]
     kw = dict(vit=a.vit, pit=a.pit, check_every_step=False)
@@ -127,6 +138,11 @@ def main():
         L.check(Lib.me_profile_read(e._h, kid, C.byref(tot), C.byref(n)), "me_profile_read")
         if n.value:
             prof[nm] = (tot.value, n.value)
+    dbg = e.buffer(L.BUF_DEBUG, np.int64, (16,))
+    if dbg[6] > dbg[0] > 0:
+        ph = np.diff(dbg[:7]) / 100.0
+        print(f"[bench] big tile ({dbg[7]} bodies, {dbg[8]} pairs) phases us: load {ph[0]:.1f} count {ph[1]:.1f} "
+              f"records {ph[2]:.1f} vel {ph[3]:.1f} pos {ph[4]:.1f} writeback {ph[5]:.1f}", file=sys.stderr, flush=True)
     pairs = e.pairs()
     ab = algorithmic_bytes(pairs, bodies.invM, a.solver, a.vit, a.pit, S, a.bodies, cnt["n_contacts"])
     ncol = max(cnt["n_colors"], 1)
@@ -161,6 +177,82 @@ def main():
                       "contacts": cnt["n_contacts"], "pairs": cnt["n_pairs"], "colors_used": cnt["n_colors"]},
            "roofline": roofline, "cpu_baseline": cpu,
            "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in prof.items()}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(a):
+    """256 independent piles (N_s log-uniform in [8, 2048]) in one tiled engine -- one launch sequence
+    per substep for every scene -- against the same scenes stepped one engine after another with
+    plain per-colour launches. Both start from the batched state after the warm-up steps."""
+    import torch
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    bodies, sizes = pkg.scenes.batched_piles(a.scenes, seed=a.seed)
+    N = bodies.n
+    dt = 1 / 240
+    kw = dict(n_colors=a.colors, vit=a.vit, pit=a.pit, tile_bodies=a.tile_bodies)
+    S = a.substeps
+    tune = None
+    if a.autotune:
+        at = importlib.import_module("3fold-motion-engine_amd.autotune")
+        mk = lambda **k: pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, **k)
+        warm = pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, **kw)
+        warm.add_boxes(bodies)
+        for _ in range(a.warmup):
+            warm.step(dt, S)
+        st = warm.state_numpy()
+        res = at.autotune(mk, bodies, warm_state=st, log=lambda m: print(m, file=sys.stderr, flush=True))
+        if res.best:
+            tune = res.best
+            kw = res.as_kwargs()
+            S = tune.substeps
+        del warm
+    e = pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, **kw)
+    e.add_boxes(bodies)
+    for i in range(a.warmup):
+        e.step(dt, S)
+    torch.cuda.synchronize()
+    e.refresh_counters()
+    x0, q0, v0, w0 = e.state_numpy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        e.step(dt, S)
+    torch.cuda.synchronize()
+    t_b = (time.perf_counter() - t0) / a.steps
+    cnt = e.refresh_counters()
+    # sequential plain launches: one engine per scene, same state
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    engines = []
+    for k in range(a.scenes):
+        sl = slice(offs[k], offs[k + 1])
+        part = pkg.bodies.BodySet(x=x0[sl].copy(), q=q0[sl].copy(), v=v0[sl].copy(), w=w0[sl].copy(),
+                                  invM=bodies.invM[sl], IbInv=bodies.IbInv[sl], kin=bodies.kin[sl],
+                                  half=bodies.half[sl], mu=bodies.mu[sl],
+                                  scene=np.zeros(offs[k + 1] - offs[k], np.int32), M=bodies.M[sl])
+        ek = pkg.ColorCacheContactEngine(tiled=False, check_every_step=False, **kw)
+        ek.add_boxes(part)
+        ek.step(dt, S)
+        engines.append(ek)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(a.seq_rounds):
+        for ek in engines:
+            ek.step(dt, S)
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) / a.seq_rounds
+    out = {"metric": METRIC, "value": round(N / t_b, 1), "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(1e3 * t_b, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box piles)",
+           "config": {"workload": f"{a.scenes} independent box piles, N_s log-uniform [8, 2048], one tiled launch "
+                                  f"sequence per substep", "bodies": int(N), "scenes": a.scenes,
+                      "colors": kw.get("n_colors"), "substeps": S, "vit": kw.get("vit"), "pit": kw.get("pit"),
+                      "autotuned": tune is not None, "pairs": cnt["n_pairs"], "islands": cnt["n_islands"]},
+           "sequential_plain_launches": {"value": round(N / t_s, 1), "unit": "body-steps/s",
+                                         "ms_per_round": round(1e3 * t_s, 3)},
+           "speedup_vs_sequential": round(t_s / t_b, 2)}
+    if tune is not None:
+        out["autotune"] = {"colours": tune.colours, "substeps": tune.substeps, "vit": tune.vit, "pit": tune.pit,
+                           "max_pen": tune.max_pen}
     print(json.dumps(out), flush=True)
 
 

@@ -103,6 +103,7 @@ enum {
     ME_BUF_PO,         /* float [N,3] pseudo angular velocity */
     ME_BUF_INVIW,      /* float [N,3,3] world inverse inertia */
     ME_BUF_ISLAND,     /* int32 [N]   island root (tiled solve) */
+    ME_BUF_DEBUG,      /* int64 [16]  phase timestamps of the last big tile (100 MHz wall clock) */
     ME_BUF_COUNT_
 };
 

