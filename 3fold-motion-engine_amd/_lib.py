@@ -16,7 +16,7 @@ FLAG_TILED = 1
 
 (BUF_KEY, BUF_BI, BUF_BJ, BUF_PA, BUF_PB, BUF_N, BUF_PEN, BUF_JN, BUF_JT1, BUF_JT2, BUF_JP,
  BUF_PBI, BUF_PBJ, BUF_PSTART, BUF_PCOUNT, BUF_COLOR, BUF_VSNAP, BUF_PV, BUF_PO, BUF_INVIW,
- BUF_ISLAND, BUF_DEBUG) = range(22)
+ BUF_ISLAND, BUF_DEBUG, BUF_BIG_WORK) = range(23)
 
 COUNTERS = ("n_contacts", "n_pairs", "n_colors", "cache_hits", "cache_misses", "overflow",
             "n_candidates", "n_islands", "n_tiles", "max_tile_bodies")
@@ -78,4 +78,4 @@ def check(rc: int, what: str) -> None:
 
 EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
             "me_buffer", "me_read_counters", "me_profile", "me_profile_read", "me_last_error")
-K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL = range(5)
+K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL, K_TILE_BIG = range(6)

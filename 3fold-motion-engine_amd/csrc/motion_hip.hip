@@ -104,16 +104,22 @@ namespace {
 
 const int kProfMax = 8192;
 
-// event pair around one launch of kernel `id` on stream s (first kProfMax launches)
-void prof_mark(me_engine *e, int id, hipStream_t s) {
-    if (!e->prof_on || e->prof_used[id] >= 2 * kProfMax) return;
+// next timing event of kernel `id` (two per launch, the first kProfMax launches); null when off
+hipEvent_t prof_next(me_engine *e, int id) {
+    if (!e->prof_on || e->prof_used[id] >= 2 * kProfMax) return nullptr;
     std::vector<hipEvent_t> &v = e->prof_ev[id];
     if ((int)v.size() <= e->prof_used[id]) {
         hipEvent_t ev;
-        if (hipEventCreate(&ev) != hipSuccess) return;
+        if (hipEventCreate(&ev) != hipSuccess) return nullptr;
         v.push_back(ev);
     }
-    (void)hipEventRecord(v[e->prof_used[id]++], s);
+    return v[e->prof_used[id]++];
+}
+
+// event pair around one launch of kernel `id` on stream s
+void prof_mark(me_engine *e, int id, hipStream_t s) {
+    hipEvent_t ev = prof_next(e, id);
+    if (ev) (void)hipEventRecord(ev, s);
 }
 
 void layout(me_engine *e, char *base, size_t *bytes) {
@@ -473,6 +479,11 @@ int substep(me_engine *e, double sdt64, hipStream_t s) {
         rc = solve_jacobi(e, sdt, s);
     } else if (e->cfg.flags & ME_FLAG_TILED) {
         TiledArgs ta = tiled_args_of(e, sdt);
+        if (e->prof_on && e->prof_used[ME_K_TILE_BIG] + 2 <= 2 * kProfMax) {
+            ta.pev0 = prof_next(e, ME_K_TILE_BIG);
+            ta.pev1 = ta.pev0 ? prof_next(e, ME_K_TILE_BIG) : nullptr;
+            if (!ta.pev1) ta.pev0 = nullptr;
+        }
         rc = tiled_prepare(e->tb, ta, s);
         if (rc) return fail(rc, "tiled solve: launch failed");
         prof_mark(e, ME_K_TILED, s);
@@ -508,6 +519,7 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.bsum = e->bsum; a.nbsum = e->nbsum;
     a.claim = e->claim;
     a.side = e->side; a.ev_fork = e->ev_fork; a.ev_join = e->ev_join;
+    a.pev0 = nullptr; a.pev1 = nullptr;
     return a;
 }
 
@@ -565,7 +577,7 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
     }
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_CACHED_P, 1, -1);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
-        hipHostFree(e->hstat);
+        (void)hipHostFree(e->hstat);
         delete e;
         return fail(ME_ERUNTIME, "workspace initialisation failed");
     }
@@ -632,6 +644,7 @@ int me_buffer(me_engine *e, int id, void **ptr, int64_t *count) {
         case ME_BUF_INVIW: *ptr = e->invIw; *count = 9 * N; break;
         case ME_BUF_ISLAND: *ptr = e->tb.root; *count = N; break;
         case ME_BUF_DEBUG: *ptr = e->tb.dbg; *count = 16; break;
+        case ME_BUF_BIG_WORK: *ptr = e->tb.bigw; *count = 4; break;
         default: return fail(ME_EARG, "unknown buffer id");
     }
     return ME_OK;
@@ -658,8 +671,12 @@ int me_read_counters(me_engine *e, int64_t *out, void *stream) {
 int me_profile(me_engine *e, int enable) {
     if (!e) return fail(ME_EARG, "engine is NULL");
     e->prof_on = enable != 0;
-    if (e->prof_on)  // a new measurement window; disabling keeps the recorded events for reading
+    if (e->prof_on) {  // a new measurement window; disabling keeps the recorded events for reading
         for (int k = 0; k < ME_K_COUNT_; ++k) e->prof_used[k] = 0;
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemset(e->tb.bigw, 0, 4 * sizeof(long long)));
+        HIP_TRY(hipDeviceSynchronize());
+    }
     return ME_OK;
 }
 

@@ -39,7 +39,8 @@ struct TiledBuffers {
     float *prep;                 // 20 floats per contact: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu
     float *sprep;                // big tiles: 4 x 24 floats per colour-sorted pair (prep + impulses)
     int *srec;                   // big tiles: 4 ints per colour-sorted pair (p, a, b, first contact)
-    long long *dbg;              // phase timestamps (wall clock) of the last big tile of >= 1000 bodies
+    long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
+    long long *bigw;             // big-tile work since the last reset: tiles, pairs, dynamic pair ends, contacts
 };
 
 struct TiledArgs {
@@ -57,6 +58,7 @@ struct TiledArgs {
     int nbsum;
     unsigned long long *claim;
     void *side, *ev_fork, *ev_join;  // side stream and fork/join events of the sweep kernels
+    void *pev0, *pev1;               // timing events around the big-tile kernel (null: not timed)
 };
 
 template <class BumpT>
@@ -90,6 +92,7 @@ inline void tiled_layout(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
     t.sprep = B.template take<float>(96 * p);
     t.srec = B.template take<int>(4 * p);
     t.dbg = B.template take<long long>(16);
+    t.bigw = B.template take<long long>(4);
 }
 
 // IS_MISMATCH after k_t_mode: 0 = ColorCache hit, 1 = colour this substep, 2 = no contacts
@@ -104,6 +107,7 @@ __global__ void k_t_init(int N, TiledBuffers T, int *istat) {
     if (i == 0) {
         istat[IS_NCOLJP] = 0; istat[IS_ISL] = 0; istat[IS_TILES] = 0; istat[IS_MAXTB] = 0;
         istat[IS_MISMATCH] = 0;
+        T.dbg[7] = 0;
     }
 }
 
@@ -219,20 +223,12 @@ __global__ void k_t_tiles(int N, int TB, int *istat, TiledBuffers T) {
 // JP rounds of one tile (k_claim / k_win of contact_engine_gpu_colored.py:151-185, restricted to the
 // tile's islands). LDS mode: claims by the body's slot in the tile, colours in LDS. Oversized
 // tiles: claims in the global claim array by body id (tiles never share a dynamic body).
-template <int WG>
-__global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T) {
-    if (A.istat[IS_MISMATCH] != 1) return;
-    int t = blockIdx.x;
-    int b0 = T.tb0[t];
-    if (b0 == INT_MAX) return;
-    int nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - T.tp0[t];
-    if (np <= 0) return;
-    extern __shared__ unsigned long long cdsm[];
-    unsigned long long *sclaim = cdsm;
-    int *colL = reinterpret_cast<int *>(cdsm + T_COLB);
-    __shared__ int left;
-    const bool lds = nb <= T_COLB && np <= T_COLP;
-    unsigned long long *claim = lds ? sclaim : A.claim;
+// LDSM: claims and colours in LDS (claim and colL point into it, so their accesses compile to ds_*
+// instructions); otherwise the global claim array and the pair colours.
+template <int WG, bool LDSM>
+__device__ __forceinline__ void t_color_tile(const TiledArgs &A, const TiledBuffers &T, int b0, int nb, int p0, int np,
+                                             unsigned long long *claim, int *colL, int &left) {
+    constexpr bool lds = LDSM;
     const float *invM = A.b.invM;
     int *gcol = A.Pp.col;
     auto slot = [&](int body) { return lds ? T.bslot[body] - b0 : body; };
@@ -285,6 +281,22 @@ __global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T) {
     }
     if (ovf) atomicOr(&A.istat[IS_OVF], OVF_COLORS);
     if (threadIdx.x == 0) atomicMax(&A.istat[IS_NCOLJP], nround);
+}
+
+template <int WG>
+__global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T) {
+    if (A.istat[IS_MISMATCH] != 1) return;
+    int t = blockIdx.x;
+    int b0 = T.tb0[t];
+    if (b0 == INT_MAX) return;
+    int nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - T.tp0[t];
+    if (np <= 0) return;
+    extern __shared__ unsigned long long cdsm[];
+    __shared__ int left;
+    if (nb <= T_COLB && np <= T_COLP)
+        t_color_tile<WG, true>(A, T, b0, nb, p0, np, cdsm, reinterpret_cast<int *>(cdsm + T_COLB), left);
+    else
+        t_color_tile<WG, false>(A, T, b0, nb, p0, np, A.claim, nullptr, left);
 }
 
 // split of the JP classes (same rule as k_color_split_plan)
@@ -731,7 +743,9 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
+    if (A.pev0 && hipEventRecord((hipEvent_t)A.pev0, s2) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile), dim3(T_BIG_WG), dyn, s2, A, T);
+    if (A.pev1 && hipEventRecord((hipEvent_t)A.pev1, s2) != hipSuccess) return ME_ERUNTIME;
     if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return ME_ERUNTIME;
     return ME_OK;
 }

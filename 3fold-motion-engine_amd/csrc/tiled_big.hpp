@@ -1,4 +1,4 @@
-// Big tiles (more pairs than one small tile holds): one 512-thread workgroup per tile.
+// Big tiles (more pairs than one small tile holds): one 1024-thread workgroup per tile.
 //
 // LDS mode (tile within T_BIG_B bodies and T_BIG_P pairs): the tile's pairs are counting-sorted by
 // colour; each sorted pair keeps a 32-bit word in LDS (body slots, contact count, flags) and its
@@ -37,84 +37,6 @@ __device__ __forceinline__ m3 ld_m12(const float *o) {
     M.m[0][0] = a.x; M.m[0][1] = a.y; M.m[0][2] = a.z; M.m[1][0] = a.w; M.m[1][1] = b.x; M.m[1][2] = b.y;
     M.m[2][0] = b.z; M.m[2][1] = b.w; M.m[2][2] = c.x;
     return M;
-}
-
-// PairCache of the k-th colour-sorted pair of an LDS-mode big tile (q = p0 + k)
-__device__ __forceinline__ void pc_from_word(PairCache &P, const TiledArgs &A, const TiledBuffers &T, unsigned w,
-                                             int q, int b0) {
-    P.la = w & 8191;
-    P.lb = (w >> 13) & 8191;
-    P.cnt = (int)((w >> 26) & 3) + 1;
-    P.dynA = (w >> 28) & 1;
-    P.dynB = (w >> 29) & 1;
-    P.hasB = (w >> 30) & 1;
-    f3 z = mk(0.0f, 0.0f, 0.0f);
-    P.vAs = z; P.wAs = z; P.pvAs = z; P.poAs = z; P.vBs = z; P.wBs = z; P.pvBs = z; P.poBs = z;
-    if (P.dynA) {
-        P.iMA = T.invMs[b0 + P.la];
-        P.IA = ld_m12(T.invIws + 12 * (size_t)(b0 + P.la));
-    } else {
-        int a = T.srec[4 * (size_t)q + 1];
-        P.iMA = A.b.invM[a];
-        P.IA = ldm(A.invIw, a);
-        P.vAs = ld3(A.b.v, a); P.wAs = ld3(A.b.w, a); P.pvAs = ld3(A.pv, a); P.poAs = ld3(A.po, a);
-    }
-    P.iMB = 0.0f;
-    P.IB = P.IA;
-    if (P.dynB) {
-        P.iMB = T.invMs[b0 + P.lb];
-        P.IB = ld_m12(T.invIws + 12 * (size_t)(b0 + P.lb));
-    } else if (P.hasB) {
-        int b = T.srec[4 * (size_t)q + 2];
-        P.iMB = A.b.invM[b];
-        P.IB = ldm(A.invIw, b);
-        P.vBs = ld3(A.b.v, b); P.wBs = ld3(A.b.w, b); P.pvBs = ld3(A.pv, b); P.poBs = ld3(A.po, b);
-    }
-    const float *o = T.sprep + 96 * (size_t)q;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k >= P.cnt) break;
-        ld_sprep(o + 24 * k, P.c[k]);
-    }
-}
-
-__device__ __forceinline__ void pc_store_sprep(const PairCache &P, const TiledBuffers &T, int q) {
-    float *o = T.sprep + 96 * (size_t)q;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k >= P.cnt) break;
-        *reinterpret_cast<float4 *>(o + 24 * k + 20) = make_float4(P.c[k].jn, P.c[k].jt1, P.c[k].jt2, P.c[k].jp);
-    }
-}
-
-// PairCache of a colour-sorted global-mode record (8 ints: p, a, b, la, lb, st, cnt, flags)
-__device__ __forceinline__ void pc_from_rec(PairCache &P, const TiledArgs &A, const TiledBuffers &T, const int *r) {
-    int a = r[1], b = r[2], flags = r[7];
-    P.la = r[3]; P.lb = r[4]; P.cnt = r[6];
-    P.dynA = flags & 1; P.dynB = (flags & 2) != 0; P.hasB = b >= 0;
-    const float *invM = A.b.invM;
-    P.iMA = invM[a];
-    P.iMB = P.hasB ? invM[b] : 0.0f;
-    P.IA = ldm(A.invIw, a);
-    P.IB = P.hasB ? ldm(A.invIw, b) : P.IA;
-    f3 z = mk(0.0f, 0.0f, 0.0f);
-    P.vAs = P.dynA ? z : ld3(A.b.v, a); P.wAs = P.dynA ? z : ld3(A.b.w, a);
-    P.pvAs = P.dynA ? z : ld3(A.pv, a); P.poAs = P.dynA ? z : ld3(A.po, a);
-    bool sB = P.hasB && !P.dynB;
-    P.vBs = sB ? ld3(A.b.v, b) : z; P.wBs = sB ? ld3(A.b.w, b) : z;
-    P.pvBs = sB ? ld3(A.pv, b) : z; P.poBs = sB ? ld3(A.po, b) : z;
-    int st = r[5];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k >= P.cnt) break;
-        int c = st + k;
-        const float *o = T.prep + 20 * (size_t)c;
-        CCon &C = P.c[k];
-        C.n = mk(o[0], o[1], o[2]); C.rA = mk(o[3], o[4], o[5]); C.rB = mk(o[6], o[7], o[8]);
-        C.t1 = mk(o[9], o[10], o[11]); C.t2 = mk(o[12], o[13], o[14]);
-        C.meff = o[15]; C.meft1 = o[16]; C.meft2 = o[17]; C.bias = o[18]; C.mu = o[19];
-        C.jn = A.Cc.jn[c]; C.jt1 = A.Cc.jt1[c]; C.jt2 = A.Cc.jt2[c]; C.jp = A.Cc.jp[c];
-    }
 }
 
 __device__ __forceinline__ void ld_prep20(const float *o, CCon &C) {
@@ -207,86 +129,26 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
     if (hasB && iMB > 0.0f) { st3(s0, lb, xB); st3(s1, lb, yB); }
 }
 
-// LDS-mode pair data loaded ahead of its colour step: inverse mass and inertia of dynamic bodies and
-// the contact records (constants + impulses). gfx9 counts stores in vmcnt, so loads issued after a
-// step's impulse stores would wait for them; loading the next step's pair before the current pair
-// is solved and stored keeps the stores off the critical path.
-struct BigPre {
-    unsigned w;
-    int q;
-    float iMA, iMB;
-    m3 IA, IB;
-    CCon c[4];
-};
-
-__device__ __forceinline__ void big_prefetch(BigPre &B, const TiledBuffers &T, unsigned w, int q, int b0) {
-    B.w = w;
-    B.q = q;
-    int la = w & 8191, lb = (w >> 13) & 8191, cnt = (int)((w >> 26) & 3) + 1;
-    bool dynA = (w >> 28) & 1, dynB = (w >> 29) & 1;
-    if (dynA) {
-        B.iMA = T.invMs[b0 + la];
-        B.IA = ld_m12(T.invIws + 12 * (size_t)(b0 + la));
-    }
-    if (dynB) {
-        B.iMB = T.invMs[b0 + lb];
-        B.IB = ld_m12(T.invIws + 12 * (size_t)(b0 + lb));
-    }
-    const float *o = T.sprep + 96 * (size_t)q;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (k < cnt) ld_sprep(o + 24 * k, B.c[k]);
-}
-
-// solve one prefetched pair (static bodies, which are rare, are read here)
-template <int PASS, bool FIXED>
-__device__ __forceinline__ void big_solve(BigPre &B, const TiledArgs &A, const TiledBuffers &T, float *s0, float *s1,
-                                          float t2d) {
-    unsigned w = B.w;
-    int q = B.q;
-    int la = w & 8191, lb = (w >> 13) & 8191, cnt = (int)((w >> 26) & 3) + 1;
-    bool dynA = (w >> 28) & 1, dynB = (w >> 29) & 1, hasB = (w >> 30) & 1;
-    const float *gs0 = PASS == 2 ? A.pv : A.b.v, *gs1 = PASS == 2 ? A.po : A.b.w;
-    f3 z = mk(0.0f, 0.0f, 0.0f);
-    float iMA, iMB = 0.0f;
-    m3 IA, IB;
-    f3 xA, yA, xB = z, yB = z;
-    if (dynA) {
-        iMA = B.iMA; IA = B.IA;
-        xA = ld3(s0, la); yA = ld3(s1, la);
-    } else {
-        int a = T.srec[4 * (size_t)q + 1];
-        iMA = A.b.invM[a]; IA = ldm(A.invIw, a);
-        xA = ld3(gs0, a); yA = ld3(gs1, a);
-    }
-    IB = IA;
-    if (dynB) {
-        iMB = B.iMB; IB = B.IB;
-        xB = ld3(s0, lb); yB = ld3(s1, lb);
-    } else if (hasB) {
-        int b = T.srec[4 * (size_t)q + 2];
-        iMB = A.b.invM[b]; IB = ldm(A.invIw, b);
-        xB = ld3(gs0, b); yB = ld3(gs1, b);
-    }
-    float *o = T.sprep + 96 * (size_t)q;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k >= cnt) break;
-        CCon &C = B.c[k];
-        if (PASS == 0) {
-            cc_warm(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
-        } else {
-            if (PASS == 1) cc_vel<FIXED>(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB, t2d);
-            else cc_pos(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
-            *reinterpret_cast<float4 *>(o + 24 * k + 20) = make_float4(C.jn, C.jt1, C.jt2, C.jp);
-        }
-    }
-    if (iMA > 0.0f) { st3(s0, la, xA); st3(s1, la, yA); }
-    if (hasB && iMB > 0.0f) { st3(s0, lb, xB); st3(s1, lb, yB); }
-}
-
 #define T_BIG_WG 1024
 #define T_BIG_DYN (6 * sizeof(float) * T_BIG_B + sizeof(unsigned) * T_BIG_P)
+
+// `reps` sweeps of one pass over an LDS-mode big tile, contacts streamed (big_pair). A colour step
+// of a big island is bound by VALU issue: every active wave runs the whole per-pair instruction chain,
+// and a colour of more than 256 pairs puts two waves on a SIMD. Prefetching the next pair across
+// colour steps (to keep loads clear of the step's impulse stores, which gfx9 counts in vmcnt) costs
+// more instructions than the latency it hides, since the second wave on each SIMD covers it.
+template <int PASS, bool FIXED>
+__device__ __forceinline__ void big_sweeps_stream(const TiledArgs &A, const TiledBuffers &T, const int *cst,
+                                                  const unsigned *pword, int nc, int reps, int p0, int b0, float *s0,
+                                                  float *s1, float t2d) {
+    const int tid = threadIdx.x;
+    for (int it = 0; it < reps; ++it)
+        for (int c = 0; c < nc; ++c) {
+            for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIG_WG)
+                big_pair<PASS, FIXED, true>(A, T, pword[k], nullptr, p0 + k, b0, s0, s1, t2d);
+            lds_barrier();
+        }
+}
 
 template <int SOLVER>
 __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T) {
@@ -297,33 +159,40 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
     tile_bounds(T, t, b0, nb, p0, np);
     if (is_small_tile(nb, np)) return;
     extern __shared__ float dsm[];
-    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1];
-    __shared__ int maxc;
+    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
+    __shared__ int maxc, wdyn, wcon;
     const int tid = threadIdx.x;
     const bool lds = nb <= T_BIG_B && np <= T_BIG_P;
-    if (tid == 0 && nb >= 1000) { T.dbg[0] = wall_clock64(); T.dbg[7] = nb; T.dbg[8] = np; }
+    // phase stamps of the largest big tile (>= 1000 bodies) of the launch
+    if (tid == 0 && nb >= 1000 && atomicMax(&T.dbg[7], (long long)nb) <= nb) { T.dbg[0] = wall_clock64(); T.dbg[8] = np; }
+    auto stamp = [&](int k) {
+        if (tid == 0 && nb >= 1000 && (long long)atomicAdd(reinterpret_cast<unsigned long long *>(&T.dbg[7]), 0ull) == nb) T.dbg[k] = wall_clock64();
+    };
     unsigned *pword = reinterpret_cast<unsigned *>(dsm + 6 * T_BIG_B);
     if (tid == 0) {
         atomicAdd(&A.istat[IS_TILES], 1);
         atomicMax(&A.istat[IS_MAXTB], nb);
         maxc = 0;
+        wdyn = 0;
+        wcon = 0;
     }
     int ncol = A.istat[IS_NCOL];
     if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
-    float *bv = lds ? dsm : A.b.v, *bw = lds ? dsm + 3 * T_BIG_B : A.b.w;
-    float *bpv = lds ? dsm : A.pv, *bpo = lds ? dsm + 3 * T_BIG_B : A.po;
+    // LDS-mode body state (velocities, then pseudo-velocities) by tile slot; kept as LDS pointers so
+    // every access compiles to ds_* instructions rather than generic (flat) ones
+    float *lv = dsm, *lw = dsm + 3 * T_BIG_B;
     if (lds)
         for (int j = tid; j < nb; j += T_BIG_WG) {
             int i = T.bsorted[b0 + j];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                bv[3 * j + k] = A.b.v[3 * i + k];
-                bw[3 * j + k] = A.b.w[3 * i + k];
+                lv[3 * j + k] = A.b.v[3 * i + k];
+                lw[3 * j + k] = A.b.w[3 * i + k];
             }
         }
-    for (int c = tid; c <= ncol; c += T_BIG_WG) { cst[c] = 0; cfl[c] = 0; }
+    for (int c = tid; c <= ncol; c += T_BIG_WG) { cst[c] = 0; cfl[c] = 0; cfe[c] = 0; }
     __syncthreads();
-    if (tid == 0 && nb >= 1000) T.dbg[1] = wall_clock64();
+    stamp(1);
     for (int k = tid; k < np; k += T_BIG_WG) {
         int c = A.Pp.col[T.psorted[p0 + k]];
         if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
@@ -337,17 +206,22 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
         cst[maxc] = run;
     }
     __syncthreads();
-    if (tid == 0 && nb >= 1000) T.dbg[2] = wall_clock64();
+    stamp(2);
     const float *invM = A.b.invM;
+    int mdyn = 0, mcon = 0;
     for (int k = tid; k < np; k += T_BIG_WG) {
         int p = T.psorted[p0 + k];
         int c = A.Pp.col[p];
         if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
-        int pos = cst[c] + atomicAdd(&cfl[c], 1);
-        int q = p0 + pos;
         int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
         bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
+        // within a colour (order-free), pairs with two dynamic bodies first and one-sided pairs
+        // (ground, static body) last, so most waves skip one side's updates as a whole
+        int pos = (dA && dB) ? cst[c] + atomicAdd(&cfl[c], 1) : cst[c + 1] - 1 - atomicAdd(&cfe[c], 1);
+        int q = p0 + pos;
         int st = A.Pp.pstart[p], cnt = A.Pp.pcount[p];
+        mdyn += (int)dA + (int)dB;
+        mcon += cnt;
         if (lds) {
             int la = dA ? T.bslot[a] - b0 : 0, lb = dB ? T.bslot[b] - b0 : 0;
             pword[pos] = pw_make(la, lb, cnt, dA, dB, b >= 0);
@@ -370,56 +244,63 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
             r[7] = (dA ? 1 : 0) | (dB ? 2 : 0);
         }
     }
+    if (mdyn) atomicAdd(&wdyn, mdyn);
+    if (mcon) atomicAdd(&wcon, mcon);
     __syncthreads();
-    if (tid == 0 && nb >= 1000) T.dbg[3] = wall_clock64();
+    if (tid == 0) {  // work of this tile for the algorithmic-byte count of the launch
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[0]), (unsigned long long)(1ll));
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[1]), (unsigned long long)((long long)np));
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[2]), (unsigned long long)((long long)wdyn));
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[3]), (unsigned long long)((long long)wcon));
+    }
+    stamp(3);
     const int nc = maxc;
     const float t2d = FIXED ? 1.0f : A.t2d;
+    // global mode: body state in global memory, a full barrier per colour step
     auto sweep = [&](int pass) {
         for (int c = 0; c < nc; ++c) {
             for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIG_WG) {
                 int q = p0 + k;
                 const int *r = T.rec + 8 * (size_t)q;
-                unsigned wd = lds ? pword[k] : 0u;
-                if (lds) {
-                    if (pass == 0) big_pair<0, FIXED, true>(A, T, wd, r, q, b0, bv, bw, t2d);
-                    else if (pass == 1) big_pair<1, FIXED, true>(A, T, wd, r, q, b0, bv, bw, t2d);
-                    else big_pair<2, FIXED, true>(A, T, wd, r, q, b0, bpv, bpo, t2d);
-                } else {
-                    if (pass == 0) big_pair<0, FIXED, false>(A, T, wd, r, q, b0, bv, bw, t2d);
-                    else if (pass == 1) big_pair<1, FIXED, false>(A, T, wd, r, q, b0, bv, bw, t2d);
-                    else big_pair<2, FIXED, false>(A, T, wd, r, q, b0, bpv, bpo, t2d);
-                }
+                if (pass == 0) big_pair<0, FIXED, false>(A, T, 0u, r, q, b0, A.b.v, A.b.w, t2d);
+                else if (pass == 1) big_pair<1, FIXED, false>(A, T, 0u, r, q, b0, A.b.v, A.b.w, t2d);
+                else big_pair<2, FIXED, false>(A, T, 0u, r, q, b0, A.pv, A.po, t2d);
             }
-            // LDS mode: only LDS state crosses threads (each record is read and written by one thread),
-            // so the colour step ends with an LDS-only barrier and leaves impulse stores in flight
-            if (lds) lds_barrier();
-            else __syncthreads();
+            __syncthreads();
         }
     };
     if (np > 0) {
-        if (FIXED) sweep(0);
-        for (int it = 0; it < A.vit; ++it) sweep(1);
+        if (lds) {
+            if (FIXED) big_sweeps_stream<0, FIXED>(A, T, cst, pword, nc, 1, p0, b0, lv, lw, t2d);
+            big_sweeps_stream<1, FIXED>(A, T, cst, pword, nc, A.vit, p0, b0, lv, lw, t2d);
+        } else {
+            if (FIXED) sweep(0);
+            for (int it = 0; it < A.vit; ++it) sweep(1);
+        }
     }
     __syncthreads();
-    if (tid == 0 && nb >= 1000) T.dbg[4] = wall_clock64();
+    stamp(4);
     // velocities back to global memory; the LDS region now holds the pseudo-velocities
     if (lds)
         for (int j = tid; j < nb; j += T_BIG_WG) {
             int i = T.bsorted[b0 + j];
-            st3(A.b.v, i, ld3(bv, j));
-            st3(A.b.w, i, ld3(bw, j));
+            st3(A.b.v, i, ld3(lv, j));
+            st3(A.b.w, i, ld3(lw, j));
         }
     __syncthreads();
     if (lds)
         for (int j = tid; j < nb; j += T_BIG_WG) {
-            st3(bpv, j, mk(0.0f, 0.0f, 0.0f));
-            st3(bpo, j, mk(0.0f, 0.0f, 0.0f));
+            st3(lv, j, mk(0.0f, 0.0f, 0.0f));
+            st3(lw, j, mk(0.0f, 0.0f, 0.0f));
         }
     __syncthreads();
-    if (np > 0)
-        for (int it = 0; it < A.pit; ++it) sweep(2);
+    if (np > 0) {
+        if (lds) big_sweeps_stream<2, FIXED>(A, T, cst, pword, nc, A.pit, p0, b0, lv, lw, t2d);
+        else
+            for (int it = 0; it < A.pit; ++it) sweep(2);
+    }
     __syncthreads();
-    if (tid == 0 && nb >= 1000) T.dbg[5] = wall_clock64();
+    stamp(5);
     if (lds)  // impulses back to the contact arrays
         for (int k = tid; k < np; k += T_BIG_WG) {
             int q = p0 + k;
@@ -435,10 +316,17 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
         }
     for (int j = tid; j < nb; j += T_BIG_WG) {
         int i = T.bsorted[b0 + j];
-        f3 pv = lds ? ld3(bpv, j) : ld3(A.pv, i), po = lds ? ld3(bpo, j) : ld3(A.po, i);
+        f3 pv, po;
+        if (lds) {
+            pv = ld3(lv, j);
+            po = ld3(lw, j);
+        } else {
+            pv = ld3(A.pv, i);
+            po = ld3(A.po, i);
+        }
         if (lds) { st3(A.pv, i, pv); st3(A.po, i, po); }
         if (A.b.invM[i] > 0.0f) integrate_body(A.b.x, A.b.q, ld3(A.b.v, i), ld3(A.b.w, i), pv, po, i, A.dt);
     }
     __syncthreads();
-    if (tid == 0 && nb >= 1000) T.dbg[6] = wall_clock64();
+    stamp(6);
 }

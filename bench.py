@@ -67,7 +67,7 @@ def main():
     ap.add_argument("--substeps", type=int, default=2)
     ap.add_argument("--vit", type=int, default=16)
     ap.add_argument("--pit", type=int, default=10)
-    ap.add_argument("--tiled", type=int, default=0)
+    ap.add_argument("--tiled", type=int, default=1)
     ap.add_argument("--tile-bodies", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--seq-rounds", type=int, default=3)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, one plain device-to-device copy of MIB MiB (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -131,7 +134,7 @@ def main():
 
     # dominant kernel: the profiled kernel with the largest summed time
     names = {L.K_SWEEP_VEL: "k_sweep_vel", L.K_SWEEP_POS: "k_sweep_pos", L.K_NARROW: "k_narrow",
-             L.K_TILED: "k_tiled_solve", L.K_JAC_VEL: "k_jac_vel"}
+             L.K_TILED: "k_tiled_solve", L.K_JAC_VEL: "k_jac_vel", L.K_TILE_BIG: "k_t_big"}
     prof = {}
     for kid, nm in names.items():
         tot, n = C.c_double(), C.c_int64()
@@ -140,14 +143,27 @@ def main():
             prof[nm] = (tot.value, n.value)
     dbg = e.buffer(L.BUF_DEBUG, np.int64, (16,))
     if dbg[6] > dbg[0] > 0:
-        ph = np.diff(dbg[:7]) / 100.0
-        print(f"[bench] big tile ({dbg[7]} bodies, {dbg[8]} pairs) phases us: load {ph[0]:.1f} count {ph[1]:.1f} "
-              f"records {ph[2]:.1f} vel {ph[3]:.1f} pos {ph[4]:.1f} writeback {ph[5]:.1f}", file=sys.stderr, flush=True)
+        ph = np.diff(dbg[:7]).astype(np.float64)
+        fr = ph / ph.sum()
+        print(f"[bench] largest big tile ({dbg[7]} bodies, {dbg[8]} pairs), last substep, share of its time: "
+              f"load {fr[0]:.3f} count {fr[1]:.3f} records {fr[2]:.3f} vel {fr[3]:.3f} pos {fr[4]:.3f} "
+              f"writeback {fr[5]:.3f} ({int(ph.sum())} wall-clock ticks)", file=sys.stderr, flush=True)
     pairs = e.pairs()
     ab = algorithmic_bytes(pairs, bodies.invM, a.solver, a.vit, a.pit, S, a.bodies, cnt["n_contacts"])
     ncol = max(cnt["n_colors"], 1)
-    per_launch = {"k_sweep_vel": ab["vel_sweep"] / ncol, "k_sweep_pos": ab["pos_sweep"] / ncol,
-                  "k_tiled_solve": ab["warm"] + a.vit * ab["vel_sweep"] + a.pit * ab["pos_sweep"]}
+    per_launch = {"k_sweep_vel": ab["vel_sweep"] / ncol, "k_sweep_pos": ab["pos_sweep"] / ncol}
+    if "k_t_big" in prof:
+        # the big-tile kernel is the critical path of the tiled solve (the small tiles run beside it on
+        # a second stream); its algorithmic bytes come from the work it counted over the same window
+        tiles, bp, bd, bc = (int(v) for v in e.buffer(L.BUF_BIG_WORK, np.int64, (4,)))
+        nl = prof["k_t_big"][1]
+        vel_b = 20 * bp + 100 * bd + 64 * bc
+        pos_b = 20 * bp + 100 * bd + 52 * bc
+        warm_b = pos_b if a.solver == "color_cache" else 0
+        per_launch["k_t_big"] = (warm_b + a.vit * vel_b + a.pit * pos_b) / max(nl, 1)
+        big_work = {"tiles": tiles / max(nl, 1), "pairs": bp / max(nl, 1), "dyn_ends": bd / max(nl, 1),
+                    "contacts": bc / max(nl, 1)}
+        print(f"[bench] k_t_big per launch: {big_work}", file=sys.stderr, flush=True)
     roofline = None
     if prof:
         dom = max((k for k in prof if k in per_launch), key=lambda k: prof[k][0], default=None)
@@ -177,7 +193,35 @@ def main():
                       "contacts": cnt["n_contacts"], "pairs": cnt["n_pairs"], "colors_used": cnt["n_colors"]},
            "roofline": roofline, "cpu_baseline": cpu,
            "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in prof.items()}}
+    traffic = _traffic_from_profile(roofline["kernel"] if roofline else None, out["config"]["workload"])
+    if roofline and traffic:
+        roofline["traffic"] = traffic
     print(json.dumps(out), flush=True)
+    if a.calibrate_copy:
+        _copy_kernel(a.calibrate_copy)
+
+
+def _copy_kernel(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] calibration copy: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+
+
+def _traffic_from_profile(kernel, workload):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE runs,
+    each scaled by the reading for a plain copy of known size) on this workload, if recorded."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not kernel or not f.exists():
+        return None
+    rec = json.loads(f.read_text()).get(kernel)
+    if not rec or rec.get("workload") != workload:
+        return None
+    return rec.get("bytes_per_launch")
 
 
 def bench_batched(a):

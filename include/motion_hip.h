@@ -103,7 +103,9 @@ enum {
     ME_BUF_PO,         /* float [N,3] pseudo angular velocity */
     ME_BUF_INVIW,      /* float [N,3,3] world inverse inertia */
     ME_BUF_ISLAND,     /* int32 [N]   island root (tiled solve) */
-    ME_BUF_DEBUG,      /* int64 [16]  phase timestamps of the last big tile (100 MHz wall clock) */
+    ME_BUF_DEBUG,      /* int64 [16]  phase timestamps of the largest big tile (wall clock) */
+    ME_BUF_BIG_WORK,   /* int64 [4]   big-tile work since me_profile(e, 1): tiles, pairs, dynamic pair
+                                      ends, contacts (for the algorithmic bytes of ME_K_TILE_BIG) */
     ME_BUF_COUNT_
 };
 
@@ -152,6 +154,7 @@ enum {
     ME_K_NARROW,         /* box narrow phase */
     ME_K_TILED,          /* island-tiled solve (one launch per substep) */
     ME_K_JAC_VEL,        /* Jacobi velocity pass */
+    ME_K_TILE_BIG,       /* big-tile kernel of the tiled solve (islands beyond one small tile) */
     ME_K_COUNT_
 };
 
