@@ -12,14 +12,14 @@ LIB_PATH = HERE / "libmotion_hip.so"
 
 ME_OK, ME_EARG, ME_ERUNTIME, ME_ETHREAD, ME_ECAPACITY, ME_ENONFINITE = 0, 1, 2, 3, 4, 5
 SOLVER_JACOBI, SOLVER_PAIR_CHUNKS, SOLVER_COLOR_CACHE = 0, 1, 2
-FLAG_TILED = 1
+FLAG_TILED, FLAG_PER_CONTACT, FLAG_GRAPH = 1, 2, 4
 
 (BUF_KEY, BUF_BI, BUF_BJ, BUF_PA, BUF_PB, BUF_N, BUF_PEN, BUF_JN, BUF_JT1, BUF_JT2, BUF_JP,
  BUF_PBI, BUF_PBJ, BUF_PSTART, BUF_PCOUNT, BUF_COLOR, BUF_VSNAP, BUF_PV, BUF_PO, BUF_INVIW,
  BUF_ISLAND, BUF_DEBUG, BUF_BIG_WORK) = range(23)
 
 COUNTERS = ("n_contacts", "n_pairs", "n_colors", "cache_hits", "cache_misses", "overflow",
-            "n_candidates", "n_islands", "n_tiles", "max_tile_bodies")
+            "n_candidates", "n_islands", "n_tiles", "max_tile_bodies", "graph_captures")
 OVF_CONTACTS, OVF_PAIRS, OVF_CAND, OVF_COLORS, OVF_TILE = 1, 2, 4, 8, 16
 
 

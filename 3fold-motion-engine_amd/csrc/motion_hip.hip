@@ -94,6 +94,16 @@ struct me_engine {
     int *hstat;  // pinned host mirror
     hipStream_t side = nullptr;  // second stream for the big-tile sweeps (default priority)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // whole-step graphs (ME_FLAG_GRAPH): captured and launched on gs, keyed by (dt, substeps)
+    struct Graph {
+        double dt;
+        int substeps;
+        hipGraphExec_t exec;
+    };
+    hipStream_t gs = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::vector<Graph> graphs;
+    int64_t n_captures = 0;
     // HIP-event timing of selected kernels (measurement only)
     bool prof_on = false;
     std::vector<hipEvent_t> prof_ev[ME_K_COUNT_];
@@ -219,6 +229,9 @@ int validate(const me_config *c) {
     if ((long long)c->n_bodies * (c->n_beads > 0 ? c->n_beads : 1) > (1LL << 30))
         return fail(ME_EARG, "too many bodies/beads");
     if (!(c->margin >= 0.0f)) return fail(ME_EARG, "margin must be >= 0");
+    if ((c->flags & ME_FLAG_GRAPH) &&
+        (!(c->flags & ME_FLAG_TILED) || c->n_beads > 0 || c->solver == ME_SOLVER_JACOBI))
+        return fail(ME_EARG, "graph capture needs the tiled solve, box geometry and a coloured solver");
     return ME_OK;
 }
 
@@ -501,6 +514,62 @@ int substep(me_engine *e, double sdt64, hipStream_t s) {
     return ME_OK;
 }
 
+const size_t kGraphMax = 8;
+
+void drop_graphs(me_engine *e) {
+    for (auto &g : e->graphs) (void)hipGraphExecDestroy(g.exec);
+    e->graphs.clear();
+}
+
+// everything an engine holds besides the caller's memory
+void release(me_engine *e) {
+    drop_graphs(e);
+    for (int k = 0; k < ME_K_COUNT_; ++k)
+        for (hipEvent_t ev : e->prof_ev[k]) (void)hipEventDestroy(ev);
+    if (e->hstat) (void)hipHostFree(e->hstat);
+    for (hipEvent_t ev : {e->ev_fork, e->ev_join, e->ev_in, e->ev_out})
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->side) (void)hipStreamDestroy(e->side);
+    if (e->gs) (void)hipStreamDestroy(e->gs);
+    delete e;
+}
+
+// one step as a graph launch: capture every substep on gs the first time a (dt, substeps) pair is
+// seen, then replay; gs is ordered after the caller's stream s and s after the replay
+int step_graph(me_engine *e, double dt, int substeps, hipStream_t s) {
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : e->graphs)
+        if (g.dt == dt && g.substeps == substeps) exec = g.exec;
+    if (!exec) {
+        if (e->graphs.size() >= kGraphMax) {
+            (void)hipGraphExecDestroy(e->graphs.front().exec);
+            e->graphs.erase(e->graphs.begin());
+        }
+        HIP_TRY(hipStreamBeginCapture(e->gs, hipStreamCaptureModeThreadLocal));
+        double sdt = dt / substeps;
+        int rc = ME_OK;
+        for (int k = 0; k < substeps && rc == ME_OK; ++k) rc = substep(e, sdt, e->gs);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->gs, &g);
+        if (rc != ME_OK || ce != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return rc != ME_OK ? rc : fail(ME_ERUNTIME, "step capture failed");
+        }
+        hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return fail(ME_ERUNTIME, "graph instantiation failed");
+        e->graphs.push_back({dt, substeps, exec});
+        e->n_captures++;
+    }
+    HIP_TRY(hipEventRecord(e->ev_in, s));
+    HIP_TRY(hipStreamWaitEvent(e->gs, e->ev_in, 0));
+    HIP_TRY(hipGraphLaunch(exec, e->gs));
+    HIP_TRY(hipEventRecord(e->ev_out, e->gs));
+    HIP_TRY(hipStreamWaitEvent(s, e->ev_out, 0));
+    return ME_OK;
+}
+
 }  // namespace
 
 // tiled-solve argument bundle (declared in tiled.hpp)
@@ -558,15 +627,23 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
     e->b = *bodies;
     e->owner = std::this_thread::get_id();
     if (hipHostMalloc((void **)&e->hstat, sizeof(int) * IS_COUNT, hipHostMallocDefault) != hipSuccess) {
-        delete e;
+        e->hstat = nullptr;
+        release(e);
         return fail(ME_ERUNTIME, "hipHostMalloc failed");
     }
+    bool graph = (cfg->flags & ME_FLAG_GRAPH) != 0;
     if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
-        (void)hipHostFree(e->hstat);
-        delete e;
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess ||
+        (graph && (hipStreamCreateWithFlags(&e->gs, hipStreamNonBlocking) != hipSuccess ||
+                   hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming) != hipSuccess ||
+                   hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming) != hipSuccess))) {
+        release(e);
         return fail(ME_ERUNTIME, "stream/event creation failed");
+    }
+    if (tiled_set_attributes() != ME_OK) {
+        release(e);
+        return fail(ME_ERUNTIME, "setting the tile kernels' LDS limits failed");
     }
     hipStream_t s = (hipStream_t)stream;
     // zero the workspace with a fill kernel (no memset nodes, so a caller may capture later steps)
@@ -577,8 +654,7 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
     }
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_CACHED_P, 1, -1);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
-        (void)hipHostFree(e->hstat);
-        delete e;
+        release(e);
         return fail(ME_ERUNTIME, "workspace initialisation failed");
     }
     *out = e;
@@ -587,13 +663,7 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
 
 int me_destroy(me_engine *e) {
     if (!e) return fail(ME_EARG, "engine is NULL");
-    for (int k = 0; k < ME_K_COUNT_; ++k)
-        for (hipEvent_t ev : e->prof_ev[k]) (void)hipEventDestroy(ev);
-    if (e->hstat) (void)hipHostFree(e->hstat);
-    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
-    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
-    if (e->side) (void)hipStreamDestroy(e->side);
-    delete e;
+    release(e);
     return ME_OK;
 }
 
@@ -601,6 +671,7 @@ int me_set_bodies(me_engine *e, const me_bodies *bodies) {
     if (!e || !bodies) return fail(ME_EARG, "NULL argument");
     if (std::this_thread::get_id() != e->owner) return fail(ME_ETHREAD, "engine used from a thread that does not own it");
     e->b = *bodies;
+    drop_graphs(e);  // captured kernels hold the old body pointers
     return ME_OK;
 }
 
@@ -610,6 +681,7 @@ int me_step(me_engine *e, double dt, int substeps, void *stream) {
     if (substeps < 1) return fail(ME_EARG, "substeps must be >= 1");
     if (!(dt > 0.0) || dt != dt) return fail(ME_EARG, "dt must be positive and finite");
     hipStream_t s = (hipStream_t)stream;
+    if ((e->cfg.flags & ME_FLAG_GRAPH) && !e->prof_on) return step_graph(e, dt, substeps, s);
     double sdt = dt / substeps;
     for (int k = 0; k < substeps; ++k) {
         int rc = substep(e, sdt, s);
@@ -665,6 +737,7 @@ int me_read_counters(me_engine *e, int64_t *out, void *stream) {
     out[ME_CNT_ISLANDS] = h[IS_ISL];
     out[ME_CNT_TILES] = h[IS_TILES];
     out[ME_CNT_MAX_TILE_BODIES] = h[IS_MAXTB];
+    out[ME_CNT_GRAPH_CAPTURES] = e->n_captures;
     return ME_OK;
 }
 

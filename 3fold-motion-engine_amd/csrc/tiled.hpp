@@ -727,19 +727,25 @@ inline int tiled_scan(const TiledArgs &A, const int *in, int *out, int n, int *t
 
 inline int tblk(long long n) { return (int)((n + ME_THREADS - 1) / ME_THREADS); }
 
+// dynamic-LDS limits of the tile kernels beyond the 64 KB default; set on the current device before
+// the first launch (at engine creation, so that no attribute call falls inside a stream capture)
+inline int tiled_set_attributes() {
+    if (hipFuncSetAttribute((const void *)k_t_big<ME_SOLVER_PAIR_CHUNKS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)T_BIG_DYN) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_t_big<ME_SOLVER_COLOR_CACHE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)T_BIG_DYN) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_t_color<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)T_COL_DYN) != hipSuccess)
+        return ME_ERUNTIME;
+    return ME_OK;
+}
+
 // small tiles on stream s, big tiles on the side stream s2 (disjoint islands), joined on s
 template <int SOLVER>
 inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStream_t s) {
     hipStream_t s2 = (hipStream_t)A.side;
     hipEvent_t fork = (hipEvent_t)A.ev_fork, join = (hipEvent_t)A.ev_join;
     const size_t dyn = T_BIG_DYN;
-    static bool attr_set = false;
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void *)k_t_big<SOLVER>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
-            hipSuccess)
-            return ME_ERUNTIME;
-        attr_set = true;
-    }
     hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
@@ -788,13 +794,6 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T, A.b.x, A.b.invM, A.invIw);
     hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
     hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
-    static bool col_attr = false;
-    if (!col_attr) {
-        if (hipFuncSetAttribute((const void *)k_t_color<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)T_COL_DYN) != hipSuccess)
-            return ME_ERUNTIME;
-        col_attr = true;
-    }
     hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile), dim3(1024), T_COL_DYN, s, A, T);
     hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
     hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);

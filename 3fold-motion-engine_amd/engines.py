@@ -41,7 +41,7 @@ class _HipContactEngine:
     def __init__(s, dims=(0.3, 0.3, 0.2), vit=40, pit=20, mu=0.5, relax=0.2, t2_damp=1.0,
                  warm_start=True, margin=0.0, n_colors=0, tiled=False, tile_bodies=128,
                  max_pairs=None, max_contacts=None, device="cuda:0", check_every_step=True,
-                 profile=False, per_contact=False):
+                 profile=False, per_contact=False, graph_capture=False):
         if vit < 0 or pit < 0:
             raise ValueError("iteration counts must be >= 0")
         if not 0 <= int(n_colors) <= 512:
@@ -57,6 +57,8 @@ class _HipContactEngine:
         s.device = device
         s.check_every_step = bool(check_every_step)
         s.per_contact = bool(per_contact)
+        # whole-step hipGraph replay (the tiled coloured solve on box bodies; otherwise eager)
+        s.graph_capture = bool(graph_capture)
         s.profile = bool(profile)
         s.prof_ms = {}
         s._centers, s._dens, s._kin = [], [], []
@@ -66,7 +68,6 @@ class _HipContactEngine:
         s._h = None
         s._sdt = None
         s._counters = dict.fromkeys(_lib.COUNTERS, 0)
-        s.n_graph_captures = 0
 
     # ── construction (data in) ──
     def add_body(s, center, density=700., kin=False):
@@ -107,7 +108,9 @@ class _HipContactEngine:
         raise ValueError("no bodies added")
 
     def _config(s, N, P):
-        flags = (_lib.FLAG_TILED if s.tiled else 0) | (2 if s.per_contact else 0)
+        flags = (_lib.FLAG_TILED if s.tiled else 0) | (_lib.FLAG_PER_CONTACT if s.per_contact else 0)
+        if s.graph_capture and s.tiled and P == 0 and s._solver != _lib.SOLVER_JACOBI:
+            flags |= _lib.FLAG_GRAPH
         return _lib.Config(s._solver, N, P, s.vit, s.pit, s.relax, s.t2_damp, int(s.warm_start),
                            R_BEAD, s.margin,
                            int(s.max_pairs or 0), int(s.max_contacts or (_MAXC if P else 0)),
@@ -230,6 +233,7 @@ class _HipContactEngine:
     n_colors = property(lambda s: s._counters["n_colors"])
     color_cache_hits = property(lambda s: s._counters["cache_hits"])
     color_cache_misses = property(lambda s: s._counters["cache_misses"])
+    n_graph_captures = property(lambda s: s._counters["graph_captures"])
 
     # ── data out ──
     def get_state(s):
@@ -303,8 +307,8 @@ class RelaxedJacobiContactEngine(_HipContactEngine):
         super().__init__(dims=dims, vit=vit, pit=pit, mu=mu, relax=relax, **kw)
 
 
-_COLORED_IGNORED = ("manifold_reduce", "manifold_max_points", "manifold_on_host", "graph_capture",
-                    "graph_mode", "chunk_mode", "trim_unused_priorities")
+_COLORED_IGNORED = ("manifold_reduce", "manifold_max_points", "manifold_on_host", "graph_mode", "chunk_mode",
+                    "trim_unused_priorities")
 
 
 class GraphColoredContactEngine(_HipContactEngine):

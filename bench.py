@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--pit", type=int, default=10)
     ap.add_argument("--tiled", type=int, default=1)
     ap.add_argument("--tile-bodies", type=int, default=128)
+    ap.add_argument("--graph", type=int, default=0, help="replay each step as one captured hipGraph (tiled path)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
@@ -103,7 +104,7 @@ def main():
            "jacobi": pkg.RelaxedJacobiContactEngine}[a.solver]
     kw = dict(vit=a.vit, pit=a.pit, check_every_step=False)
     if a.solver != "jacobi":
-        kw.update(n_colors=a.colors, tiled=bool(a.tiled), tile_bodies=a.tile_bodies)
+        kw.update(n_colors=a.colors, tiled=bool(a.tiled), tile_bodies=a.tile_bodies, graph_capture=bool(a.graph))
     e = cls(**kw)
     e.add_boxes(bodies)
     dt, S = 1 / 240, a.substeps
@@ -118,17 +119,33 @@ def main():
     print(f"[bench] at timed start: {e._counters}", file=sys.stderr, flush=True)
     x0, q0, v0, w0 = e.state_numpy()
     Lib = L.lib()
-    L.check(Lib.me_profile(e._h, 1), "me_profile")
+    # kernel timing needs per-launch events, so steps taken while profiling run eagerly; with --graph
+    # the timed region replays graphs and a second window of the same length is profiled eagerly
+    if not a.graph:
+        L.check(Lib.me_profile(e._h, 1), "me_profile")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
         e.step(dt, S)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    elapsed = t1 - t0
+    eager_ms = None
+    if a.graph:
+        cnt = e.refresh_counters()
+        print(f"[bench] graph replays: {1e3 * elapsed / a.steps:.4f} ms/step, {cnt['graph_captures']} capture(s)",
+              file=sys.stderr, flush=True)
+        L.check(Lib.me_profile(e._h, 1), "me_profile")
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for i in range(a.steps):
+            e.step(dt, S)
+        torch.cuda.synchronize()
+        eager_ms = 1e3 * (time.perf_counter() - t2) / a.steps
+        print(f"[bench] profiled eager window after it: {eager_ms:.4f} ms/step", file=sys.stderr, flush=True)
     L.check(Lib.me_profile(e._h, 0), "me_profile")
     cnt = e.refresh_counters()
     print(f"[bench] at timed end: {cnt}", file=sys.stderr, flush=True)
-    elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / a.steps
     value = a.bodies * a.steps / elapsed
 
@@ -189,10 +206,13 @@ def main():
            "config": {"workload": f"{a.bodies}-box pile, {a.solver}, {a.colors} colours, S{S} V{a.vit} P{a.pit}, "
                                   f"dt 1/240" + (", tiled" if a.tiled else ", plain launches"),
                       "bodies": a.bodies, "solver": a.solver, "colors": a.colors, "substeps": S,
-                      "vit": a.vit, "pit": a.pit, "tiled": bool(a.tiled),
+                      "vit": a.vit, "pit": a.pit, "tiled": bool(a.tiled), "graph": bool(a.graph),
                       "contacts": cnt["n_contacts"], "pairs": cnt["n_pairs"], "colors_used": cnt["n_colors"]},
            "roofline": roofline, "cpu_baseline": cpu,
            "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in prof.items()}}
+    if eager_ms is not None:
+        out["kernel_timing_window"] = {"mode": "eager, right after the timed graph replays", "steps": a.steps,
+                                       "ms_per_step": round(eager_ms, 4)}
     traffic = _traffic_from_profile(roofline["kernel"] if roofline else None, out["config"]["workload"])
     if roofline and traffic:
         roofline["traffic"] = traffic
@@ -250,7 +270,7 @@ def bench_batched(a):
             kw = res.as_kwargs()
             S = tune.substeps
         del warm
-    e = pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, **kw)
+    e = pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, graph_capture=bool(a.graph), **kw)
     e.add_boxes(bodies)
     for i in range(a.warmup):
         e.step(dt, S)
@@ -290,7 +310,7 @@ def bench_batched(a):
            "config": {"workload": f"{a.scenes} independent box piles, N_s log-uniform [8, 2048], one tiled launch "
                                   f"sequence per substep", "bodies": int(N), "scenes": a.scenes,
                       "colors": kw.get("n_colors"), "substeps": S, "vit": kw.get("vit"), "pit": kw.get("pit"),
-                      "autotuned": tune is not None, "pairs": cnt["n_pairs"], "islands": cnt["n_islands"]},
+                      "autotuned": tune is not None, "graph": bool(a.graph), "pairs": cnt["n_pairs"], "islands": cnt["n_islands"]},
            "sequential_plain_launches": {"value": round(N / t_s, 1), "unit": "body-steps/s",
                                          "ms_per_round": round(1e3 * t_s, 3)},
            "speedup_vs_sequential": round(t_s / t_b, 2)}

@@ -45,6 +45,11 @@ extern "C" {
 #define ME_FLAG_TILED 1          /* solve islands in LDS-resident tiles (one launch per substep) */
 #define ME_FLAG_PER_CONTACT 2    /* coloured solver without pair chunks: every contact its own chunk
                                     (GraphColoredContactEngine pair_chunks=False) */
+#define ME_FLAG_GRAPH 4          /* me_step replays one hipGraph of the whole step (every substep, every
+                                    sweep), captured on the first call for each (dt, substeps), at most 8
+                                    kept; the graph solve capture of contact_engine_gpu_colored.py:1228-1299.
+                                    Needs ME_FLAG_TILED, box geometry and a coloured solver. Steps taken
+                                    while me_profile is on run eagerly. */
 
 typedef struct me_config {
     int32_t solver;        /* ME_SOLVER_* */
@@ -121,6 +126,7 @@ enum {
     ME_CNT_ISLANDS,      /* islands of the last tiled solve */
     ME_CNT_TILES,        /* tiles of the last tiled solve */
     ME_CNT_MAX_TILE_BODIES,
+    ME_CNT_GRAPH_CAPTURES, /* whole-step graphs captured so far (ME_FLAG_GRAPH) */
     ME_CNT_COUNT_
 };
 

@@ -30,3 +30,13 @@ def test_workspace_size_and_argument_errors():
     assert b"solver" in L.me_last_error()
     bead_cc = lib_mod.Config(2, 10, 18, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 0, 256, 0)
     assert L.me_workspace_size(ctypes.byref(bead_cc), ctypes.byref(n)) == lib_mod.ME_EARG
+    # whole-step graphs need the tiled coloured solve on box bodies
+    g_plain = lib_mod.Config(2, 100, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64, lib_mod.FLAG_GRAPH)
+    assert L.me_workspace_size(ctypes.byref(g_plain), ctypes.byref(n)) == lib_mod.ME_EARG
+    assert b"graph" in L.me_last_error()
+    g_jac = lib_mod.Config(0, 100, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 0, 64,
+                           lib_mod.FLAG_GRAPH | lib_mod.FLAG_TILED)
+    assert L.me_workspace_size(ctypes.byref(g_jac), ctypes.byref(n)) == lib_mod.ME_EARG
+    g_ok = lib_mod.Config(2, 100, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64,
+                          lib_mod.FLAG_GRAPH | lib_mod.FLAG_TILED)
+    assert L.me_workspace_size(ctypes.byref(g_ok), ctypes.byref(n)) == 0

@@ -183,3 +183,35 @@ def test_batched_scenes_match_oracle_and_stay_independent(oracle):
     assert e.n_pairs == o.counters()["n_pairs"]
     xs = solo.state()[0]
     assert np.all(np.isfinite(xs))
+
+
+def test_graph_replay_matches_eager(oracle):
+    """Whole-step hipGraphs (ME_FLAG_GRAPH): the engine replaying captured steps equals the eager
+    engine bit for bit after every replay, with eager work between replays (state and counter
+    read-backs); the big-tile kernel on the side stream is inside the graph (a 600-box chain); a new
+    dt captures a second graph; the end state equals the oracle."""
+    _say("color_cache tiled chain 600 + pile 200 x36: graph replay vs eager vs oracle, dt change")
+    pkg = _pkg()
+    c = np.zeros((600, 3))
+    c[:, 0] = np.arange(600) * 0.0995
+    c[:, 2] = 0.0499
+    row = pkg.bodies.box_bodies(c, [0.05, 0.05, 0.05], 1000.0, mu=0.6)
+    b = pkg.scenes.concat([row, pkg.scenes.box_pile(200, seed=5, origin=(0.0, 3.0, 0.0))])
+    kw = dict(vit=8, pit=4, n_colors=16, tiled=True, tile_bodies=64)
+    eg = _engine("color_cache", graph_capture=True, **kw)
+    ee = _engine("color_cache", **kw)
+    eg.add_boxes(b)
+    ee.add_boxes(b)
+    o = oracle.OracleWorld(b, "color_cache", vit=8, pit=4, n_colors=16)
+    for s in range(36):
+        dt = 1 / 240 if s < 24 else 1 / 300
+        eg.step(dt, 2)
+        ee.step(dt, 2)
+        o.step(dt, 2)
+        for name, a, r in zip("xqvw", eg.state_numpy(), ee.state_numpy()):
+            assert np.array_equal(a, r), f"step {s}: graph replay differs from eager in {name}"
+        assert (eg.n_contacts, eg.n_pairs, eg.n_colors) == (ee.n_contacts, ee.n_pairs, ee.n_colors)
+    assert eg.n_graph_captures == 2 and ee.n_graph_captures == 0
+    assert eg._counters["max_tile_bodies"] >= 600
+    _assert_state_equal(eg, o, "final")
+    _assert_contacts_equal(eg, o, "final")
