@@ -12,7 +12,8 @@ enum {
     IS_MAXEXT, IS_LEFT, IS_WN, IS_CACHED_P, IS_CACHED_NCOL, IS_MISMATCH, IS_CELL, IS_NBODYSORT,
     IS_NPAIRSORT, IS_NCOLJP, IS_COUNT = 32
 };
-enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16 };
+enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,
+       OVF_TAPE = 32 /* a recorded (differentiated) substep met a tile beyond one small tile */ };
 
 #define ME_NCOLMAX 512
 #define ME_MAXCAND 32
@@ -691,6 +692,26 @@ __global__ void k_warm_lookup(int capC, const int *istat, int warm, const int64_
     }
     if (hit >= 0) { jn[c] = wjn[hit]; jt1[c] = wjt1[hit]; jt2[c] = wjt2[hit]; }
 }
+// the warm-start match of every contact (k_warm_lookup's binary search), kept for the adjoint
+__global__ void k_tape_widx(int capC, const int *istat, int warm, const int64_t *key, const int64_t *wkey, int *widx) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int C = istat[IS_C];
+    if (c >= C || c >= capC) return;
+    int wn = istat[IS_WN], hit = -1;
+    if (warm && wn > 0) {
+        int64_t k = key[c];
+        int lo = 0, hi = wn - 1;
+        while (lo <= hi) {
+            int mid = (lo + hi) / 2;
+            int64_t km = wkey[mid];
+            if (km == k) { hit = mid; lo = hi + 1; }
+            else if (km < k) lo = mid + 1;
+            else hi = mid - 1;
+        }
+    }
+    widx[c] = hit;
+}
+
 // A23 warm store (contact_engine_gpu_colored.py:1089-1093)
 __global__ void k_warm_store(int capC, int *istat, const int64_t *key, const float *jn, const float *jt1,
                              const float *jt2, int64_t *wkey, float *wjn, float *wjt1, float *wjt2) {

@@ -12,6 +12,7 @@
 
 #include "kernels.hpp"
 #include "tiled.hpp"
+#include "adjoint_kernels.hpp"
 
 using namespace me;
 
@@ -102,6 +103,9 @@ struct me_engine {
     };
     hipStream_t gs = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // recording of the current substep for its adjoint (me_step_taped / me_adjoint_substep)
+    bool taping = false;
+    AdjTape tape{};
     std::vector<Graph> graphs;
     int64_t n_captures = 0;
     // HIP-event timing of selected kernels (measurement only)
@@ -570,6 +574,39 @@ int step_graph(me_engine *e, double dt, int substeps, hipStream_t s) {
     return ME_OK;
 }
 
+// layout of a tape block (see AdjTape), 256-byte aligned arrays
+size_t tape_layout(const me_engine *e, char *base, AdjTape *t) {
+    size_t N = (size_t)e->N, C = (size_t)e->capC, S = (size_t)(e->cfg.vit + e->cfg.pit);
+    Bump B{base};
+    AdjTape x{};
+    x.xpre = B.take<float>(3 * N);
+    x.qpre = B.take<float>(4 * N);
+    x.widx = B.take<int>(C);
+    x.sst = B.take<float>(S * 6 * N);
+    x.simp = B.take<float>(S * 4 * C);
+    x.cbar = B.take<float>(20 * C);
+    x.jbi = B.take<float>(3 * C);
+    x.gbar = B.take<float>(7 * C);
+    x.Ibar = B.take<float>(9 * N);
+    x.pvb = B.take<float>(3 * N);
+    x.pob = B.take<float>(3 * N);
+    if (t) *t = x;
+    return B.off;
+}
+
+int tape_check(me_engine *e, void *tape, size_t bytes) {
+    if (!e) return fail(ME_EARG, "engine is NULL");
+    if (std::this_thread::get_id() != e->owner) return fail(ME_ETHREAD, "engine used from a thread that does not own it");
+    const me_config &c = e->cfg;
+    if (c.solver != ME_SOLVER_COLOR_CACHE || !(c.flags & ME_FLAG_TILED) || c.n_beads > 0)
+        return fail(ME_EARG, "the adjoint covers the tiled ColorCache solver on box bodies");
+    if (!tape || ((uintptr_t)tape & 255) != 0) return fail(ME_EARG, "tape must be a 256-byte aligned device block");
+    size_t need = tape_layout(e, nullptr, nullptr);
+    if (bytes < need) return fail(ME_EARG, "tape too small: need " + std::to_string(need) + " bytes");
+    tape_layout(e, (char *)tape, &e->tape);
+    return ME_OK;
+}
+
 }  // namespace
 
 // tiled-solve argument bundle (declared in tiled.hpp)
@@ -589,6 +626,8 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.claim = e->claim;
     a.side = e->side; a.ev_fork = e->ev_fork; a.ev_join = e->ev_join;
     a.pev0 = nullptr; a.pev1 = nullptr;
+    a.taped = e->taping ? 1 : 0;
+    a.tape = e->tape;
     return a;
 }
 
@@ -767,6 +806,60 @@ int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launche
     }
     *total_ms = tot;
     *launches = n;
+    return ME_OK;
+}
+
+int me_tape_size(me_engine *e, size_t *bytes) {
+    if (!e || !bytes) return fail(ME_EARG, "NULL argument");
+    *bytes = tape_layout(e, nullptr, nullptr);
+    return ME_OK;
+}
+
+int me_step_taped(me_engine *e, double dt, void *tape, size_t bytes, void *stream) {
+    int rc = tape_check(e, tape, bytes);
+    if (rc) return rc;
+    if (!(dt > 0.0) || dt != dt) return fail(ME_EARG, "dt must be positive and finite");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_tape_xq, dim3(nblk(e->N)), dim3(ME_THREADS), 0, s, e->N, e->b.x, e->b.q, e->tape.xpre,
+                       e->tape.qpre);
+    e->taping = true;
+    rc = substep(e, dt, s);
+    e->taping = false;
+    return rc;
+}
+
+int me_adjoint_substep(me_engine *e, double dt, void *tape, size_t bytes, float *xbar, float *qbar, float *vbar,
+                       float *wbar, const float *jbar_end, float *jbar_prev, float *mubar, void *stream) {
+    int rc = tape_check(e, tape, bytes);
+    if (rc) return rc;
+    if (!xbar || !qbar || !vbar || !wbar || !jbar_end || !jbar_prev || !mubar) return fail(ME_EARG, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    int N = e->N, capC = e->capC;
+    const AdjTape &t = e->tape;
+    auto zero = [&](float *p, size_t n) {
+        for (size_t off = 0; off < n; off += (size_t)1 << 30) {
+            int m = (int)((n - off) < ((size_t)1 << 30) ? (n - off) : ((size_t)1 << 30));
+            hipLaunchKernelGGL(k_fill_i32, dim3(nblk(m)), dim3(ME_THREADS), 0, s, (int *)(p + off), m, 0);
+        }
+    };
+    zero(t.cbar, 20 * (size_t)capC);
+    zero(t.jbi, 3 * (size_t)capC);
+    zero(t.gbar, 7 * (size_t)capC);
+    zero(t.Ibar, 9 * (size_t)N);
+    zero(t.pvb, 3 * (size_t)N);
+    zero(t.pob, 3 * (size_t)N);
+    zero(jbar_prev, 3 * (size_t)capC);
+    e->taping = true;
+    TiledArgs A = tiled_args_of(e, (float)dt);
+    e->taping = false;
+    int ntile = tiled_ntile(A);
+    hipLaunchKernelGGL(k_adj_integ, dim3(nblk(N)), dim3(ME_THREADS), 0, s, A, xbar, qbar, vbar, wbar);
+    hipLaunchKernelGGL(k_adj_tiles, dim3(ntile), dim3(T_SMALL_P), 0, s, A, e->tb, vbar, wbar, jbar_end);
+    hipLaunchKernelGGL(k_adj_contacts, dim3(nblk(capC)), dim3(ME_THREADS), 0, s, A, xbar, mubar, jbar_prev);
+    hipLaunchKernelGGL(k_adj_narrow, dim3(nblk(e->cap_cand)), dim3(ME_THREADS), 0, s, A, e->cap_cand, e->cand_a,
+                       e->cand_b, e->S.n, e->slot_off, e->cfg.margin, xbar, qbar);
+    hipLaunchKernelGGL(k_adj_body, dim3(nblk(N)), dim3(ME_THREADS), 0, s, A, qbar);
+    LAUNCH_CHECK();
     return ME_OK;
 }
 

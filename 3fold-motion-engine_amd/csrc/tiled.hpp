@@ -43,6 +43,22 @@ struct TiledBuffers {
     long long *bigw;             // big-tile work since the last reset: tiles, pairs, dynamic pair ends, contacts
 };
 
+// Recording of one substep for its adjoint (all device pointers into one caller buffer; V, P =
+// velocity and position iterations): the centres and orientations at the substep start, the warm-start
+// match of every contact, and the body state and impulses at the start of every sweep. The remaining
+// arrays are scratch of the reverse pass.
+struct AdjTape {
+    float *xpre, *qpre;  // [3N], [4N]
+    int *widx;           // [capC] index of the warm-start match in the previous substep's contacts, -1 none
+    float *sst;          // [(V+P) * 6N] sweep-start (v, w) then (pv, po), by body
+    float *simp;         // [(V+P) * 4 * capC] sweep-start (jn, jt1, jt2, jp), by contact
+    float *cbar;         // [20 * capC] adjoint of the per-contact constants (k_t_prep layout)
+    float *jbi;          // [3 * capC] adjoint of the warm-started impulses
+    float *gbar;         // [7 * capC] adjoint of contact point, normal, penetration
+    float *Ibar;         // [9N] adjoint of the world inverse inertia
+    float *pvb, *pob;    // [3N] adjoint of the pseudo-velocities
+};
+
 struct TiledArgs {
     int N, capC, capP, solver, vit, pit, warm, n_colors, tile_bodies;
     float t2d, dt;
@@ -59,6 +75,8 @@ struct TiledArgs {
     unsigned long long *claim;
     void *side, *ev_fork, *ev_join;  // side stream and fork/join events of the sweep kernels
     void *pev0, *pev1;               // timing events around the big-tile kernel (null: not timed)
+    int taped;                       // record the substep into tape (k_t_small<.., true>)
+    AdjTape tape;
 };
 
 template <class BumpT>
@@ -591,7 +609,31 @@ __device__ __forceinline__ void tile_writeback(const TiledArgs &A, const TiledBu
 
 // small tiles: thread t owns the t-th pair of the tile in colour order, its constants and impulses
 // in registers; body velocities in LDS; one workgroup barrier per colour step
-template <bool FIXED>
+// sweep-start record of a small tile: its bodies' state (LDS) and this thread's pair's impulses
+__device__ __forceinline__ void tape_sweep(const TiledArgs &A, const TiledBuffers &T, int b0, int nb, int sweep,
+                                           const float *s0, const float *s1, const PairCache &P, int p) {
+    float *st = A.tape.sst + (size_t)sweep * 6 * A.N;
+    for (int j = threadIdx.x; j < nb; j += T_SMALL_P) {
+        int i = T.bsorted[b0 + j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            st[6 * (size_t)i + k] = s0[3 * j + k];
+            st[6 * (size_t)i + 3 + k] = s1[3 * j + k];
+        }
+    }
+    if (p >= 0) {
+        int c0 = A.Pp.pstart[p];
+        float *si = A.tape.simp + (size_t)sweep * 4 * A.capC;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k >= P.cnt) break;
+            float *o = si + 4 * (size_t)(c0 + k);
+            o[0] = P.c[k].jn; o[1] = P.c[k].jt1; o[2] = P.c[k].jt2; o[3] = P.c[k].jp;
+        }
+    }
+}
+
+template <bool FIXED, bool TAPE>
 __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers T) {
     int t = blockIdx.x;
     if (T.tb0[t] == INT_MAX) return;
@@ -656,16 +698,23 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers
             }
         }
         const float t2d = FIXED ? 1.0f : A.t2d;
-        for (int it = 0; it < A.vit; ++it)
+        for (int it = 0; it < A.vit; ++it) {
+            if (TAPE) tape_sweep(A, T, b0, nb, it, sv, sw, P, p);
             for (int c = 0; c < nc; ++c) {
                 if (c_own == c) pc_vel<FIXED>(P, sv, sw, t2d);
                 lds_barrier();
             }
-        for (int it = 0; it < A.pit; ++it)
+        }
+        for (int it = 0; it < A.pit; ++it) {
+            if (TAPE) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, p);
             for (int c = 0; c < nc; ++c) {
                 if (c_own == c) pc_pos(P, spv, spo);
                 lds_barrier();
             }
+        }
+    } else if (TAPE) {
+        for (int it = 0; it < A.vit; ++it) tape_sweep(A, T, b0, nb, it, sv, sw, P, -1);
+        for (int it = 0; it < A.pit; ++it) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, -1);
     }
     if (p >= 0) pc_store(P, A, p);
     __syncthreads();
@@ -748,7 +797,11 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     const size_t dyn = T_BIG_DYN;
     hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
-    hipLaunchKernelGGL(k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
+    if (A.taped)
+        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
+    else
+        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>), dim3(ntile), dim3(T_SMALL_P), 0, s, A,
+                           T);
     if (A.pev0 && hipEventRecord((hipEvent_t)A.pev0, s2) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile), dim3(T_BIG_WG), dyn, s2, A, T);
     if (A.pev1 && hipEventRecord((hipEvent_t)A.pev1, s2) != hipSuccess) return ME_ERUNTIME;
@@ -772,6 +825,9 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     bool cc = A.solver == ME_SOLVER_COLOR_CACHE;
     int g = capP > N ? capP : N;
     hipLaunchKernelGGL(k_zero_pvpo, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, A.pv, A.po);
+    if (A.taped)  // before the lookup: the warm store still holds the previous substep's contacts
+        hipLaunchKernelGGL(k_tape_widx, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
+                           A.wkey, A.tape.widx);
     hipLaunchKernelGGL(k_warm_lookup, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
                        A.wkey, A.wjn, A.wjt1, A.wjt2, A.Cc.jn, A.Cc.jt1, A.Cc.jt2, A.Cc.jp);
     hipLaunchKernelGGL(k_t_init, dim3(tblk(N > ME_NCOLMAX + 1 ? N : ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, N, T,

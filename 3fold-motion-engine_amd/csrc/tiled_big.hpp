@@ -158,6 +158,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
     int b0, nb, p0, np;
     tile_bounds(T, t, b0, nb, p0, np);
     if (is_small_tile(nb, np)) return;
+    if (A.taped && threadIdx.x == 0) atomicOr(&A.istat[IS_OVF], OVF_TAPE);  // no recording for big tiles
     extern __shared__ float dsm[];
     __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
     __shared__ int maxc, wdyn, wcon;

@@ -169,6 +169,32 @@ enum {
 int me_profile(me_engine *e, int enable);
 int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launches);
 
+/* Adjoint of the step (tiled ColorCache solver, box bodies, islands within one small tile).
+ * No counterpart entry points in the reference's solver classes; they replace the role of the
+ * reference's batched NCP adjoint (ncp/ncp_batch_gpu.py:1172-1187, BatchProxADMMGPU.grad_affine)
+ * for the contact step itself. A caller differentiates a rollout substep by substep, last first:
+ * restore the engine's state (bodies + workspace) from before the substep, record it with
+ * me_step_taped, then me_adjoint_substep turns the adjoint of the state after the substep into the
+ * adjoint of the state before it. */
+
+/* Bytes of the caller-owned tape block (256-byte aligned device memory) for this engine. */
+int me_tape_size(me_engine *e, size_t *bytes);
+
+/* One substep of length dt (as one substep of me_step), recorded into the tape: the centres and
+ * orientations at its start, the warm-start matches, and the body state and impulses at the start
+ * of every sweep. A tile beyond one small tile sets overflow bit 32 (not recorded). */
+int me_step_taped(me_engine *e, double dt, void *tape, size_t bytes, void *stream);
+
+/* Reverse of the recorded substep; the engine must hold the state right after me_step_taped.
+ * In: xbar [N,3], qbar [N,4], vbar [N,3], wbar [N,3] = adjoint of the state after the substep;
+ *     jbar_end [capC,3] = adjoint of the substep's final (jn, jt1, jt2), the values the next substep
+ *     warm-starts from (zero for the last substep).
+ * Out: xbar, qbar, vbar, wbar = adjoint of the state before the substep (in place);
+ *     jbar_prev [capC,3] = adjoint of the previous substep's final impulses, by its contact index;
+ *     mubar [N] += adjoint of the per-body friction coefficients. */
+int me_adjoint_substep(me_engine *e, double dt, void *tape, size_t bytes, float *xbar, float *qbar, float *vbar,
+                       float *wbar, const float *jbar_end, float *jbar_prev, float *mubar, void *stream);
+
 /* Text of the last error on the calling thread. */
 const char *me_last_error(void);
 

@@ -19,26 +19,26 @@
 #include <string.h>
 #include <time.h>
 
-typedef struct { float x, y, z; } v3;
-typedef struct { float x, y, z, w; } qt;
-typedef struct { float m[3][3]; } m33;
+typedef struct { real x, y, z; } v3;
+typedef struct { real x, y, z, w; } qt;
+typedef struct { real m[3][3]; } m33;
 
-static inline v3 V3(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 V3(real x, real y, real z) { v3 r = {x, y, z}; return r; }
 static inline v3 vadd(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
 static inline v3 vsub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
 static inline v3 vneg(v3 a) { return V3(-a.x, -a.y, -a.z); }
-static inline v3 vmul(v3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
-static inline v3 smul(float s, v3 a) { return V3(s * a.x, s * a.y, s * a.z); }
-static inline v3 vdiv(v3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
-static inline float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vmul(v3 a, real s) { return V3(a.x * s, a.y * s, a.z * s); }
+static inline v3 smul(real s, v3 a) { return V3(s * a.x, s * a.y, s * a.z); }
+static inline v3 vdiv(v3 a, real s) { return V3(a.x / s, a.y / s, a.z / s); }
+static inline real vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline v3 vcross(v3 a, v3 b) {
     return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-static inline float vlen(v3 a) { return sqrtf(vdot(a, a)); }
+static inline real vlen(v3 a) { return RSQRT(vdot(a, a)); }
 /* max(a, b) as a > b ? a : b (explicit, so signed zeros and NaNs behave the same everywhere) */
-static inline float wmax(float a, float b) { return a > b ? a : b; }
+static inline real wmax(real a, real b) { return a > b ? a : b; }
 static inline v3 vnormalize(v3 a) {
-    float l = vlen(a);
+    real l = vlen(a);
     if (l > 0.0f) return vdiv(a, l);
     return V3(0.0f, 0.0f, 0.0f);
 }
@@ -51,7 +51,7 @@ static inline m33 mm(const m33 *A, const m33 *B) {
     m33 r;
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
-            float s = 0.0f;
+            real s = 0.0f;
             for (int k = 0; k < 3; ++k) s += A->m[i][k] * B->m[k][j];
             r.m[i][j] = s;
         }
@@ -63,25 +63,25 @@ static inline m33 mtrans(const m33 *A) {
         for (int j = 0; j < 3; ++j) r.m[i][j] = A->m[j][i];
     return r;
 }
-static inline v3 ld3(const float *p, int i) { return V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
-static inline void st3(float *p, int i, v3 a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
-static inline qt ldq(const float *p, int i) {
+static inline v3 ld3(const real *p, int i) { return V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+static inline void st3(real *p, int i, v3 a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
+static inline qt ldq(const real *p, int i) {
     qt r = {p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]};
     return r;
 }
-static inline m33 ldm(const float *p, int i) {
+static inline m33 ldm(const real *p, int i) {
     m33 r;
     for (int k = 0; k < 9; ++k) r.m[k / 3][k % 3] = p[9 * i + k];
     return r;
 }
-static inline void stm(float *p, int i, const m33 *M) {
+static inline void stm(real *p, int i, const m33 *M) {
     for (int k = 0; k < 9; ++k) p[9 * i + k] = M->m[k / 3][k % 3];
 }
 
 /* Warp quat_rotate: c = 2*w*w - 1; d = 2*(qv.x); x*c + qv*d + cross(qv, x)*w*2 */
 static inline v3 quat_rotate(qt q, v3 x) {
-    float c = 2.0f * q.w * q.w - 1.0f;
-    float d = 2.0f * (q.x * x.x + q.y * x.y + q.z * x.z);
+    real c = 2.0f * q.w * q.w - 1.0f;
+    real d = 2.0f * (q.x * x.x + q.y * x.y + q.z * x.z);
     v3 qv = V3(q.x, q.y, q.z);
     return vadd(vadd(vmul(x, c), vmul(qv, d)), vmul(vmul(vcross(qv, x), q.w), 2.0f));
 }
@@ -104,10 +104,10 @@ static inline qt qmul(qt a, qt b) {
     return r;
 }
 static inline qt qnormalize(qt q) {
-    float l = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    real l = RSQRT(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     qt r;
     if (l > 0.0f) {
-        float inv = 1.0f / l;
+        real inv = 1.0f / l;
         r.x = q.x * inv; r.y = q.y * inv; r.z = q.z * inv; r.w = q.w * inv;
     } else {
         r.x = 0.0f; r.y = 0.0f; r.z = 0.0f; r.w = 1.0f;
@@ -119,9 +119,9 @@ static inline qt qnormalize(qt q) {
 struct or_world {
     int N, P;
     or_params prm;
-    float *x, *q, *v, *w, *invM, *IbInv, *invIw, *pv, *po, *half, *mu, *rest, *allp, *vsnap;
-    float *Rw; /* 9N world rotation */
-    float *aabb; /* 6N lo/hi */
+    real *x, *q, *v, *w, *invM, *IbInv, *invIw, *pv, *po, *half, *mu, *rest, *allp, *vsnap;
+    real *Rw; /* 9N world rotation */
+    real *aabb; /* 6N lo/hi */
     int *kin, *scene;
     double *M, *force;
     double force_dt;
@@ -129,7 +129,7 @@ struct or_world {
     int C, cap;
     int64_t *key;
     int *cbi, *cbj;
-    float *cpA, *cpB, *cn, *cpen, *cmu, *jn, *jt1, *jt2, *jp;
+    real *cpA, *cpB, *cn, *cpen, *cmu, *jn, *jt1, *jt2, *jp;
     /* pairs */
     int Pp, pcap;
     int *pbi, *pbj, *pstart, *pcount, *col, *porder, *cstart, *ccount;
@@ -137,11 +137,11 @@ struct or_world {
     /* warm store */
     int wn;
     int64_t *wkey;
-    float *wjn, *wjt1, *wjt2;
+    real *wjn, *wjt1, *wjt2;
     /* colour cache */
     int cached_P, cached_ncol;
     int *cache_a, *cache_b, *cache_col;
-    float *cache_invM;
+    real *cache_invM;
     int64_t hits, misses, overflow;
     /* Jacobi accumulators */
     int64_t *acc;
@@ -155,25 +155,25 @@ static void *xc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
 or_world *or_create(int N, int P, const or_params *prm) {
     or_world *w = (or_world *)xc(1, sizeof(or_world));
     w->N = N; w->P = P; w->prm = *prm;
-    w->x = xc(3 * N, 4); w->q = xc(4 * N, 4); w->v = xc(3 * N, 4); w->w = xc(3 * N, 4);
-    w->invM = xc(N, 4); w->IbInv = xc(9 * N, 4); w->invIw = xc(9 * N, 4); w->pv = xc(3 * N, 4);
-    w->po = xc(3 * N, 4); w->half = xc(3 * N, 4); w->mu = xc(N, 4); w->vsnap = xc(3 * N, 4);
-    w->Rw = xc(9 * N, 4); w->aabb = xc(6 * N, 4);
+    w->x = xc(3 * N, sizeof(real)); w->q = xc(4 * N, sizeof(real)); w->v = xc(3 * N, sizeof(real)); w->w = xc(3 * N, sizeof(real));
+    w->invM = xc(N, sizeof(real)); w->IbInv = xc(9 * N, sizeof(real)); w->invIw = xc(9 * N, sizeof(real)); w->pv = xc(3 * N, sizeof(real));
+    w->po = xc(3 * N, sizeof(real)); w->half = xc(3 * N, sizeof(real)); w->mu = xc(N, sizeof(real)); w->vsnap = xc(3 * N, sizeof(real));
+    w->Rw = xc(9 * N, sizeof(real)); w->aabb = xc(6 * N, sizeof(real));
     w->kin = xc(N, 4); w->scene = xc(N, 4); w->M = xc(N, 8); w->force = xc(3 * N, 8);
-    w->rest = xc(3 * (P > 0 ? P : 1), 4);
-    w->allp = xc(3 * (size_t)N * (P > 0 ? P : 1), 4);
+    w->rest = xc(3 * (P > 0 ? P : 1), sizeof(real));
+    w->allp = xc(3 * (size_t)N * (P > 0 ? P : 1), sizeof(real));
     w->cap = prm->cap;
     int cap = prm->cap;
     w->key = xc(cap, 8); w->cbi = xc(cap, 4); w->cbj = xc(cap, 4);
-    w->cpA = xc(3 * cap, 4); w->cpB = xc(3 * cap, 4); w->cn = xc(3 * cap, 4);
-    w->cpen = xc(cap, 4); w->cmu = xc(cap, 4); w->jn = xc(cap, 4); w->jt1 = xc(cap, 4);
-    w->jt2 = xc(cap, 4); w->jp = xc(cap, 4);
+    w->cpA = xc(3 * cap, sizeof(real)); w->cpB = xc(3 * cap, sizeof(real)); w->cn = xc(3 * cap, sizeof(real));
+    w->cpen = xc(cap, sizeof(real)); w->cmu = xc(cap, sizeof(real)); w->jn = xc(cap, sizeof(real)); w->jt1 = xc(cap, sizeof(real));
+    w->jt2 = xc(cap, sizeof(real)); w->jp = xc(cap, sizeof(real));
     w->pcap = cap;
     w->pbi = xc(cap, 4); w->pbj = xc(cap, 4); w->pstart = xc(cap, 4); w->pcount = xc(cap, 4);
     w->col = xc(cap, 4); w->porder = xc(cap, 4); w->cstart = xc(cap + 1, 4); w->ccount = xc(cap + 1, 4);
-    w->wkey = xc(cap, 8); w->wjn = xc(cap, 4); w->wjt1 = xc(cap, 4); w->wjt2 = xc(cap, 4);
+    w->wkey = xc(cap, 8); w->wjn = xc(cap, sizeof(real)); w->wjt1 = xc(cap, sizeof(real)); w->wjt2 = xc(cap, sizeof(real));
     w->cache_a = xc(cap, 4); w->cache_b = xc(cap, 4); w->cache_col = xc(cap, 4);
-    w->cache_invM = xc(N, 4);
+    w->cache_invM = xc(N, sizeof(real));
     w->cached_P = -1;
     w->acc = xc(6 * (size_t)N, 8);
     w->tmpk = xc(cap > N ? cap : N, 8);
@@ -194,27 +194,27 @@ void or_destroy(or_world *w) {
     free(w);
 }
 
-void or_set_bodies(or_world *w, const float *x, const float *q, const float *v, const float *om,
-                   const float *invM, const float *IbInv, const int *kin, const float *half,
-                   const float *mu, const int *scene) {
+void or_set_bodies(or_world *w, const real *x, const real *q, const real *v, const real *om,
+                   const real *invM, const real *IbInv, const int *kin, const real *half,
+                   const real *mu, const int *scene) {
     int N = w->N;
-    memcpy(w->x, x, 12 * (size_t)N); memcpy(w->q, q, 16 * (size_t)N);
-    memcpy(w->v, v, 12 * (size_t)N); memcpy(w->w, om, 12 * (size_t)N);
-    memcpy(w->invM, invM, 4 * (size_t)N); memcpy(w->IbInv, IbInv, 36 * (size_t)N);
+    memcpy(w->x, x, 3 * sizeof(real) * (size_t)N); memcpy(w->q, q, 4 * sizeof(real) * (size_t)N);
+    memcpy(w->v, v, 3 * sizeof(real) * (size_t)N); memcpy(w->w, om, 3 * sizeof(real) * (size_t)N);
+    memcpy(w->invM, invM, sizeof(real) * (size_t)N); memcpy(w->IbInv, IbInv, 9 * sizeof(real) * (size_t)N);
     memcpy(w->kin, kin, 4 * (size_t)N);
-    if (half) memcpy(w->half, half, 12 * (size_t)N);
-    if (mu) memcpy(w->mu, mu, 4 * (size_t)N);
+    if (half) memcpy(w->half, half, 3 * sizeof(real) * (size_t)N);
+    if (mu) memcpy(w->mu, mu, sizeof(real) * (size_t)N);
     if (scene) memcpy(w->scene, scene, 4 * (size_t)N);
     for (int i = 0; i < N; ++i) w->M[i] = invM[i] > 0.0f ? 1.0 / (double)invM[i] : INFINITY;
 }
 void or_set_mass(or_world *w, const double *M) { memcpy(w->M, M, 8 * (size_t)w->N); }
-void or_set_rest(or_world *w, const float *rest) { memcpy(w->rest, rest, 12 * (size_t)w->P); }
-void or_set_velocity(or_world *w, const float *v, const float *om) {
-    memcpy(w->v, v, 12 * (size_t)w->N); memcpy(w->w, om, 12 * (size_t)w->N);
+void or_set_rest(or_world *w, const real *rest) { memcpy(w->rest, rest, 3 * sizeof(real) * (size_t)w->P); }
+void or_set_velocity(or_world *w, const real *v, const real *om) {
+    memcpy(w->v, v, 3 * sizeof(real) * (size_t)w->N); memcpy(w->w, om, 3 * sizeof(real) * (size_t)w->N);
 }
 
 /* ── A2 k_grav (contact_engine_gpu.py:41-44) ── */
-static void grav(or_world *w, float dt) {
+static void grav(or_world *w, real dt) {
     for (int i = 0; i < w->N; ++i)
         if (w->invM[i] > 0.0f) {
             v3 g = vmul(V3(0.0f, 0.0f, -9.81f), dt);
@@ -241,8 +241,8 @@ static void inv_inertia_world(or_world *w) {
 }
 
 /* ── A11 k_integ (contact_engine_gpu.py:172-180) ── */
-static void integrate_range(int N, float *x, float *q, const float *v, const float *om,
-                            const float *pv, const float *po, const float *invM, float dt) {
+static void integrate_range(int N, real *x, real *q, const real *v, const real *om,
+                            const real *pv, const real *po, const real *invM, real dt) {
     for (int i = 0; i < N; ++i) {
         if (invM[i] <= 0.0f) continue;
         st3(x, i, vadd(ld3(x, i), vmul(vadd(ld3(v, i), ld3(pv, i)), dt)));
@@ -257,15 +257,15 @@ static void integrate_range(int N, float *x, float *q, const float *v, const flo
         q[4 * i] = r.x; q[4 * i + 1] = r.y; q[4 * i + 2] = r.z; q[4 * i + 3] = r.w;
     }
 }
-void or_integrate(int N, float *x, float *q, const float *v, const float *om, const float *pv,
-                  const float *po, const float *invM, float dt) {
+void or_integrate(int N, real *x, real *q, const real *v, const real *om, const real *pv,
+                  const real *po, const real *invM, real dt) {
     integrate_range(N, x, q, v, om, pv, po, invM, dt);
 }
-void or_quat_to_matrix(const float *q, float *R) {
+void or_quat_to_matrix(const real *q, real *R) {
     m33 M = quat_to_matrix(ldq(q, 0));
     stm(R, 0, &M);
 }
-void or_quat_rotate(const float *q, const float *x, float *out) {
+void or_quat_rotate(const real *q, const real *x, real *out) {
     st3(out, 0, quat_rotate(ldq(q, 0), ld3(x, 0)));
 }
 
@@ -275,8 +275,8 @@ static int cmp_i64(const void *a, const void *b) {
     return x < y ? -1 : x > y;
 }
 
-static int push_contact(or_world *w, int64_t key, int bi, int bj, v3 pA, v3 pB, v3 n, float pen,
-                        float mu) {
+static int push_contact(or_world *w, int64_t key, int bi, int bj, v3 pA, v3 pB, v3 n, real pen,
+                        real mu) {
     if (w->C >= w->cap) { w->overflow++; return 0; }
     int c = w->C++;
     w->key[c] = key; w->cbi[c] = bi; w->cbj[c] = bj;
@@ -310,19 +310,19 @@ static int find_cell(const cellrec *cr, int n, int64_t key) {
 
 static void bead_contacts(or_world *w) {
     int N = w->N, P = w->P, NP = N * P;
-    float R = w->prm.R;
-    float mu = w->mu[0];
+    real R = w->prm.R;
+    real mu = w->mu[0];
     for (int g = 0; g < NP; ++g) {
         int b = g / P;
         qt q = ldq(w->q, b);
         v3 p = vadd(ld3(w->x, b), quat_rotate(q, ld3(w->rest, g % P)));
         st3(w->allp, g, p);
     }
-    float h = 2.0f * R;
+    real h = 2.0f * R;
     cellrec *cr = (cellrec *)malloc(sizeof(cellrec) * (NP > 0 ? (size_t)NP : (size_t)1));
     for (int g = 0; g < NP; ++g) {
         v3 p = ld3(w->allp, g);
-        cr[g].cell = cell_key((int64_t)floorf(p.x / h), (int64_t)floorf(p.y / h), (int64_t)floorf(p.z / h), 0);
+        cr[g].cell = cell_key((int64_t)RFLOOR(p.x / h), (int64_t)RFLOOR(p.y / h), (int64_t)RFLOOR(p.z / h), 0);
         cr[g].idx = g;
     }
     qsort(cr, NP, sizeof(cellrec), cmp_cell);
@@ -332,11 +332,11 @@ static void bead_contacts(or_world *w) {
     for (int g = 0; g < NP; ++g) {
         v3 xi = ld3(w->allp, g);
         int bi = g / P;
-        float s = xi.z - R;
+        real s = xi.z - R;
         if (s < 0.0f)
             push_contact(w, (int64_t)g * stride + 0, bi, -1, xi, V3(0.0f, 0.0f, 0.0f),
                          V3(0.0f, 0.0f, 1.0f), -s, mu);
-        int64_t cx = (int64_t)floorf(xi.x / h), cy = (int64_t)floorf(xi.y / h), cz = (int64_t)floorf(xi.z / h);
+        int64_t cx = (int64_t)RFLOOR(xi.x / h), cy = (int64_t)RFLOOR(xi.y / h), cz = (int64_t)RFLOOR(xi.z / h);
         int nc = 0;
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
@@ -357,7 +357,7 @@ static void bead_contacts(or_world *w) {
             int j = cand[a];
             v3 xj = ld3(w->allp, j);
             v3 dv = vsub(xi, xj);
-            float dist = vlen(dv);
+            real dist = vlen(dv);
             if (dist < 2.0f * R && dist > 1e-9f)
                 push_contact(w, (int64_t)g * stride + (int64_t)j + 1, bi, j / P, xi, xj,
                              vdiv(dv, dist), 2.0f * R - dist, mu);
@@ -376,15 +376,15 @@ static inline v3 box_vertex(v3 c, const m33 *R, v3 h, int k) {
     return vadd(c, mv(R, l));
 }
 static inline v3 mcol(const m33 *R, int i) { return V3(R->m[0][i], R->m[1][i], R->m[2][i]); }
-static inline float hget(v3 h, int i) { return i == 0 ? h.x : (i == 1 ? h.y : h.z); }
+static inline real hget(v3 h, int i) { return i == 0 ? h.x : (i == 1 ? h.y : h.z); }
 
 /* face (axis a, sign s) polygon, ncp_ref._face_polygon order: (-,-), (-,+), (+,+), (+,-) */
-static void face_poly(v3 c, const m33 *R, v3 h, int a, float s, v3 out[4]) {
+static void face_poly(v3 c, const m33 *R, v3 h, int a, real s, v3 out[4]) {
     int o0 = a == 0 ? 1 : 0, o1 = a == 2 ? 1 : 2;
-    static const float S0[4] = {-1.0f, -1.0f, 1.0f, 1.0f};
-    static const float S1[4] = {-1.0f, 1.0f, 1.0f, -1.0f};
+    static const real S0[4] = {-1.0f, -1.0f, 1.0f, 1.0f};
+    static const real S1[4] = {-1.0f, 1.0f, 1.0f, -1.0f};
     for (int k = 0; k < 4; ++k) {
-        float l[3];
+        real l[3];
         l[a] = s * hget(h, a);
         l[o0] = S0[k] * hget(h, o0);
         l[o1] = S1[k] * hget(h, o1);
@@ -396,13 +396,13 @@ static void face_poly(v3 c, const m33 *R, v3 h, int a, float s, v3 out[4]) {
 #define MAXPOLY 16
 typedef struct { v3 p[MAXPOLY]; int code[MAXPOLY]; int edge[MAXPOLY]; int n; } poly_t;
 
-static void clip_poly(const poly_t *in, v3 sn, float d, int plane, poly_t *out) {
+static void clip_poly(const poly_t *in, v3 sn, real d, int plane, poly_t *out) {
     out->n = 0;
     int m = in->n;
     for (int i = 0; i < m; ++i) {
         v3 a = in->p[i], b = in->p[(i + 1) % m];
-        float da = vdot(sn, a) - d;
-        float db = vdot(sn, b) - d;
+        real da = vdot(sn, a) - d;
+        real db = vdot(sn, b) - d;
         int E = in->edge[i];
         if (da <= CLIP_EPS && out->n < MAXPOLY) {
             out->p[out->n] = a;
@@ -413,7 +413,7 @@ static void clip_poly(const poly_t *in, v3 sn, float d, int plane, poly_t *out) 
         int enter = da > CLIP_EPS && db < -CLIP_EPS;
         int exitp = da < -CLIP_EPS && db > CLIP_EPS;
         if ((enter || exitp) && out->n < MAXPOLY) {
-            float t = da / (da - db);
+            real t = da / (da - db);
             out->p[out->n] = vadd(a, smul(t, vsub(b, a)));
             out->code[out->n] = E < 4 ? 4 + 4 * E + plane : 20 + 4 * (E - 4) + plane;
             out->edge[out->n] = exitp ? 4 + plane : E;
@@ -424,7 +424,7 @@ static void clip_poly(const poly_t *in, v3 sn, float d, int plane, poly_t *out) 
 
 /* deepest first (stable by gap), then greedily the candidate farthest (squared distance) from
  * those kept; returns kept indices ordered by ascending feature code */
-static int reduce4(int n, const v3 *p, const float *gap, const int *code, int *keep) {
+static int reduce4(int n, const v3 *p, const real *gap, const int *code, int *keep) {
     int ord[MAXPOLY];
     for (int i = 0; i < n; ++i) ord[i] = i;
     for (int a = 1; a < n; ++a) {
@@ -440,13 +440,13 @@ static int reduce4(int n, const v3 *p, const float *gap, const int *code, int *k
         int used[MAXPOLY] = {0};
         keep[0] = ord[0]; used[0] = 1; nk = 1;
         while (nk < 4) {
-            int best = -1; float bd = -1.0f;
+            int best = -1; real bd = -1.0f;
             for (int r = 1; r < n; ++r) {
                 if (used[r]) continue;
-                float md = 3.4e38f;
+                real md = 3.4e38f;
                 for (int k = 0; k < nk; ++k) {
                     v3 e = vsub(p[ord[r]], p[keep[k]]);
-                    float d2 = vdot(e, e);
+                    real d2 = vdot(e, e);
                     if (d2 < md) md = d2;
                 }
                 if (md > bd) { bd = md; best = r; }
@@ -463,8 +463,8 @@ static int reduce4(int n, const v3 *p, const float *gap, const int *code, int *k
 }
 
 /* box vs ground plane z = 0, normal +z (ground -> box); up to 4 vertices with z < margin */
-static int box_ground_impl(v3 c, const m33 *R, v3 h, float margin, v3 *pout, float *pen, int *code) {
-    v3 p[8]; float gap[8]; int cd[8]; int n = 0;
+static int box_ground_impl(v3 c, const m33 *R, v3 h, real margin, v3 *pout, real *pen, int *code) {
+    v3 p[8]; real gap[8]; int cd[8]; int n = 0;
     for (int k = 0; k < 8; ++k) {
         v3 vk = box_vertex(c, R, h, k);
         if (vk.z < margin) { p[n] = vk; gap[n] = vk.z; cd[n] = k; n++; }
@@ -475,37 +475,37 @@ static int box_ground_impl(v3 c, const m33 *R, v3 h, float margin, v3 *pout, flo
     return nk;
 }
 
-static inline float sat_overlap(v3 ax, const m33 *RA, v3 hA, const m33 *RB, v3 hB, v3 d) {
-    float ra = fabsf(vdot(ax, mcol(RA, 0))) * hA.x + fabsf(vdot(ax, mcol(RA, 1))) * hA.y +
-               fabsf(vdot(ax, mcol(RA, 2))) * hA.z;
-    float rb = fabsf(vdot(ax, mcol(RB, 0))) * hB.x + fabsf(vdot(ax, mcol(RB, 1))) * hB.y +
-               fabsf(vdot(ax, mcol(RB, 2))) * hB.z;
-    return ra + rb - fabsf(vdot(ax, d));
+static inline real sat_overlap(v3 ax, const m33 *RA, v3 hA, const m33 *RB, v3 hB, v3 d) {
+    real ra = RFABS(vdot(ax, mcol(RA, 0))) * hA.x + RFABS(vdot(ax, mcol(RA, 1))) * hA.y +
+               RFABS(vdot(ax, mcol(RA, 2))) * hA.z;
+    real rb = RFABS(vdot(ax, mcol(RB, 0))) * hB.x + RFABS(vdot(ax, mcol(RB, 1))) * hB.y +
+               RFABS(vdot(ax, mcol(RB, 2))) * hB.z;
+    return ra + rb - RFABS(vdot(ax, d));
 }
 
 /* box A vs box B; normal n points from B to A; code = feature code (13 bits) */
-static int box_box_impl(v3 cA, const m33 *RA, v3 hA, v3 cB, const m33 *RB, v3 hB, float margin,
-                        v3 *pout, v3 *nout, float *pen, int *code) {
+static int box_box_impl(v3 cA, const m33 *RA, v3 hA, v3 cB, const m33 *RB, v3 hB, real margin,
+                        v3 *pout, v3 *nout, real *pen, int *code) {
     v3 d = vsub(cA, cB);
-    float best = 3.4e38f; int bfrom = -1;
+    real best = 3.4e38f; int bfrom = -1;
     for (int i = 0; i < 6; ++i) {
         v3 ax = i < 3 ? mcol(RA, i) : mcol(RB, i - 3);
-        float o = sat_overlap(ax, RA, hA, RB, hB, d);
+        real o = sat_overlap(ax, RA, hA, RB, hB, d);
         if (o < best) { best = o; bfrom = i; }
     }
     if (bfrom < 0 || best < -margin) return 0;
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
             v3 L = vcross(mcol(RA, i), mcol(RB, j));
-            float l2 = vdot(L, L);
+            real l2 = vdot(L, L);
             if (l2 < 1e-12f) continue;
-            v3 ax = vdiv(L, sqrtf(l2));
+            v3 ax = vdiv(L, RSQRT(l2));
             if (sat_overlap(ax, RA, hA, RB, hB, d) < -margin) return 0;
         }
     v3 n = bfrom < 3 ? mcol(RA, bfrom) : mcol(RB, bfrom - 3);
     if (vdot(n, d) < 0.0f) n = vneg(n);
     int refIsB = bfrom >= 3, ai = refIsB ? bfrom - 3 : bfrom;
-    v3 refc, incc, refh, inch; const m33 *refR, *incR; v3 ref_n; float ref_sign;
+    v3 refc, incc, refh, inch; const m33 *refR, *incR; v3 ref_n; real ref_sign;
     if (!refIsB) {
         refc = cA; refR = RA; refh = hA; incc = cB; incR = RB; inch = hB;
         ref_n = vneg(n);
@@ -515,11 +515,11 @@ static int box_box_impl(v3 cA, const m33 *RA, v3 hA, v3 cB, const m33 *RB, v3 hB
         ref_n = n;
         ref_sign = vdot(mcol(RB, ai), n) > 0.0f ? 1.0f : -1.0f;
     }
-    int bi = 0; float bs = 1.0f, bv = 3.4e38f;
+    int bi = 0; real bs = 1.0f, bv = 3.4e38f;
     for (int i = 0; i < 3; ++i)
         for (int si = 0; si < 2; ++si) {
-            float s = si == 0 ? -1.0f : 1.0f;
-            float val = vdot(vmul(mcol(incR, i), s), ref_n);
+            real s = si == 0 ? -1.0f : 1.0f;
+            real val = vdot(vmul(mcol(incR, i), s), ref_n);
             if (val < bv) { bv = val; bi = i; bs = s; }
         }
     poly_t A, B;
@@ -532,7 +532,7 @@ static int box_box_impl(v3 cA, const m33 *RA, v3 hA, v3 cB, const m33 *RB, v3 hB
     for (int i = 0; i < 4; ++i) {
         v3 e = vsub(rp[(i + 1) % 4], rp[i]);
         v3 sn = vcross(e, ref_n);
-        float nl = vlen(sn);
+        real nl = vlen(sn);
         if (nl < 1e-12f) continue;
         sn = vdiv(sn, nl);
         if (vdot(sn, vsub(refc, rp[i])) > 0.0f) sn = vneg(sn);
@@ -541,7 +541,7 @@ static int box_box_impl(v3 cA, const m33 *RA, v3 hA, v3 cB, const m33 *RB, v3 hB
         if (cur->n == 0) return 0;
     }
     /* drop coincident points, keep penetrating ones */
-    v3 p[MAXPOLY]; float gap[MAXPOLY]; int cd[MAXPOLY]; int n2 = 0;
+    v3 p[MAXPOLY]; real gap[MAXPOLY]; int cd[MAXPOLY]; int n2 = 0;
     for (int i = 0; i < cur->n; ++i) {
         int dup = 0;
         for (int k = 0; k < i; ++k) {
@@ -549,7 +549,7 @@ static int box_box_impl(v3 cA, const m33 *RA, v3 hA, v3 cB, const m33 *RB, v3 hB
             if (vdot(e, e) <= 1e-12f) { dup = 1; break; }
         }
         if (dup) continue;
-        float g = vdot(ref_n, vsub(cur->p[i], rp[0]));
+        real g = vdot(ref_n, vsub(cur->p[i], rp[0]));
         if (g < margin) {
             p[n2] = vsub(cur->p[i], vmul(ref_n, 0.5f * g));
             gap[n2] = g; cd[n2] = cur->code[i]; n2++;
@@ -566,15 +566,15 @@ static int box_box_impl(v3 cA, const m33 *RA, v3 hA, v3 cB, const m33 *RB, v3 hB
     return nk;
 }
 
-int or_box_box(const float *xa, const float *qa, const float *ha, const float *xb, const float *qb,
-               const float *hb, float margin, float *p, float *n, float *pen, int *code) {
+int or_box_box(const real *xa, const real *qa, const real *ha, const real *xb, const real *qb,
+               const real *hb, real margin, real *p, real *n, real *pen, int *code) {
     m33 RA = quat_to_matrix(ldq(qa, 0)), RB = quat_to_matrix(ldq(qb, 0));
     v3 pp[4], nn[4];
     int k = box_box_impl(ld3(xa, 0), &RA, ld3(ha, 0), ld3(xb, 0), &RB, ld3(hb, 0), margin, pp, nn, pen, code);
     for (int i = 0; i < k; ++i) { st3(p, i, pp[i]); st3(n, i, nn[i]); }
     return k;
 }
-int or_box_ground(const float *x, const float *q, const float *h, float margin, float *p, float *pen,
+int or_box_ground(const real *x, const real *q, const real *h, real margin, real *p, real *pen,
                   int *code) {
     m33 R = quat_to_matrix(ldq(q, 0));
     v3 pp[4];
@@ -583,10 +583,10 @@ int or_box_ground(const float *x, const float *q, const float *h, float margin, 
     return k;
 }
 
-static inline float pair_mu(const or_world *w, int a, int b) {
+static inline real pair_mu(const or_world *w, int a, int b) {
     if (b < 0) return w->mu[a];
-    float ma = w->mu[a], mb = w->mu[b];
-    return ma == mb ? ma : sqrtf(ma * mb);
+    real ma = w->mu[a], mb = w->mu[b];
+    return ma == mb ? ma : RSQRT(ma * mb);
 }
 
 /* body AABB half extents ext_k = sum_j |R_kj| h_j */
@@ -594,16 +594,16 @@ static void box_aabbs(or_world *w) {
     for (int i = 0; i < w->N; ++i) {
         m33 R = ldm(w->Rw, i);
         v3 h = ld3(w->half, i), c = ld3(w->x, i);
-        float e[3];
+        real e[3];
         for (int k = 0; k < 3; ++k)
-            e[k] = fabsf(R.m[k][0]) * h.x + fabsf(R.m[k][1]) * h.y + fabsf(R.m[k][2]) * h.z;
+            e[k] = RFABS(R.m[k][0]) * h.x + RFABS(R.m[k][1]) * h.y + RFABS(R.m[k][2]) * h.z;
         w->aabb[6 * i + 0] = c.x - e[0]; w->aabb[6 * i + 1] = c.y - e[1]; w->aabb[6 * i + 2] = c.z - e[2];
         w->aabb[6 * i + 3] = c.x + e[0]; w->aabb[6 * i + 4] = c.y + e[1]; w->aabb[6 * i + 5] = c.z + e[2];
     }
 }
 /* broad-phase slack: candidate tests are conservative by this much beyond the contact margin */
 #define BP_SLACK 1e-4f
-static inline int aabb_overlap(const float *a, const float *b, float m) {
+static inline int aabb_overlap(const real *a, const real *b, real m) {
     return a[0] <= b[3] + m && b[0] <= a[3] + m && a[1] <= b[4] + m && b[1] <= a[4] + m &&
            a[2] <= b[5] + m && b[2] <= a[5] + m;
 }
@@ -613,20 +613,20 @@ static inline int aabb_overlap(const float *a, const float *b, float m) {
  * narrow phase then writes each pair's contacts in feature-code order. */
 static void box_contacts(or_world *w) {
     int N = w->N;
-    float margin = w->prm.margin;
+    real margin = w->prm.margin;
     box_aabbs(w);
-    float hmax = 0.0f;
+    real hmax = 0.0f;
     for (int i = 0; i < N; ++i)
         for (int k = 0; k < 3; ++k) {
-            float e = w->aabb[6 * i + 3 + k] - w->x[3 * i + k];
+            real e = w->aabb[6 * i + 3 + k] - w->x[3 * i + k];
             if (e > hmax) hmax = e;
         }
-    float cell = (2.0f * hmax + margin + BP_SLACK) * 1.001f;
+    real cell = (2.0f * hmax + margin + BP_SLACK) * 1.001f;
     if (!(cell > 0.0f)) cell = 1.0f;
     cellrec *cr = (cellrec *)malloc(sizeof(cellrec) * (size_t)(N ? N : 1));
     for (int i = 0; i < N; ++i) {
-        cr[i].cell = cell_key((int64_t)floorf(w->x[3 * i] / cell), (int64_t)floorf(w->x[3 * i + 1] / cell),
-                              (int64_t)floorf(w->x[3 * i + 2] / cell), w->scene[i]);
+        cr[i].cell = cell_key((int64_t)RFLOOR(w->x[3 * i] / cell), (int64_t)RFLOOR(w->x[3 * i + 1] / cell),
+                              (int64_t)RFLOOR(w->x[3 * i + 2] / cell), w->scene[i]);
         cr[i].idx = i;
     }
     qsort(cr, N, sizeof(cellrec), cmp_cell);
@@ -636,7 +636,7 @@ static void box_contacts(or_world *w) {
     for (int a = 0; a < N; ++a) {
         m33 RA = ldm(w->Rw, a);
         v3 cA = ld3(w->x, a), hA = ld3(w->half, a);
-        v3 p[4], nn[4]; float pen[4]; int code[4];
+        v3 p[4], nn[4]; real pen[4]; int code[4];
         if (w->invM[a] > 0.0f && w->aabb[6 * a + 2] < margin + BP_SLACK) {
             int k = box_ground_impl(cA, &RA, hA, margin, p, pen, code);
             int64_t pk = (int64_t)a * N1 + 0;
@@ -644,7 +644,7 @@ static void box_contacts(or_world *w) {
                 push_contact(w, pk * 8192 + code[i], a, -1, p[i], p[i], V3(0.0f, 0.0f, 1.0f), pen[i],
                              pair_mu(w, a, -1));
         }
-        int64_t cx = (int64_t)floorf(cA.x / cell), cy = (int64_t)floorf(cA.y / cell), cz = (int64_t)floorf(cA.z / cell);
+        int64_t cx = (int64_t)RFLOOR(cA.x / cell), cy = (int64_t)RFLOOR(cA.y / cell), cz = (int64_t)RFLOOR(cA.z / cell);
         int nc = 0;
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
@@ -677,22 +677,22 @@ static void box_contacts(or_world *w) {
 }
 
 /* ───────────────────────── A8-A10 relaxed Jacobi (contact_engine_gpu.py:32-38, 82-169) ───────────────────────── */
-static inline void acc3(int64_t *acc, int N, int b, int off, v3 d, float sign, float scale) {
+static inline void acc3(int64_t *acc, int N, int b, int off, v3 d, real sign, real scale) {
     double sc = (double)(sign * scale);
     acc[(off + 0) * (size_t)N + b] += (int64_t)round((double)d.x * sc);
     acc[(off + 1) * (size_t)N + b] += (int64_t)round((double)d.y * sc);
     acc[(off + 2) * (size_t)N + b] += (int64_t)round((double)d.z * sc);
 }
 
-static void jac_vel(or_world *w, int c, float scale) {
+static void jac_vel(or_world *w, int c, real scale) {
     int N = w->N;
     int bi = w->cbi[c], bj = w->cbj[c];
     v3 nrm = ld3(w->cn, c), rA = vsub(ld3(w->cpA, c), ld3(w->x, bi));
-    float iMA = w->invM[bi];
+    real iMA = w->invM[bi];
     m33 IA = ldm(w->invIw, bi), IB;
     v3 va = vadd(ld3(w->v, bi), vcross(ld3(w->w, bi), rA));
     v3 vb = V3(0.0f, 0.0f, 0.0f), rB = V3(0.0f, 0.0f, 0.0f);
-    float iMB = 0.0f;
+    real iMB = 0.0f;
     if (bj >= 0) {
         rB = vsub(ld3(w->cpB, c), ld3(w->x, bj));
         vb = vadd(ld3(w->v, bj), vcross(ld3(w->w, bj), rB));
@@ -700,25 +700,25 @@ static void jac_vel(or_world *w, int c, float scale) {
         IB = ldm(w->invIw, bj);
     }
     v3 vrel = vsub(va, vb);
-    float vn = vdot(vrel, nrm);
+    real vn = vdot(vrel, nrm);
     v3 vt = vsub(vrel, vmul(nrm, vn));
-    float mt = vlen(vt);
+    real mt = vlen(vt);
     v3 t1 = vdiv(vt, mt);
     if (mt <= 5e-3f) {
         v3 aa = V3(1.0f, 0.0f, 0.0f);
-        if (fabsf(nrm.x) >= 0.9f) aa = V3(0.0f, 1.0f, 0.0f);
+        if (RFABS(nrm.x) >= 0.9f) aa = V3(0.0f, 1.0f, 0.0f);
         t1 = vnormalize(vsub(aa, vmul(nrm, vdot(aa, nrm))));
     }
     v3 t2 = vcross(nrm, t1);
     v3 rnA = vcross(rA, nrm);
-    float meff = iMA + vdot(rnA, mv(&IA, rnA));
+    real meff = iMA + vdot(rnA, mv(&IA, rnA));
     if (bj >= 0) {
         v3 rnB = vcross(rB, nrm);
         meff = meff + iMB + vdot(rnB, mv(&IB, rnB));
     }
     if (meff < 1e-12f) return;
-    float dj = -vn / meff;
-    float nw = wmax(0.0f, w->jn[c] + dj);
+    real dj = -vn / meff;
+    real nw = wmax(0.0f, w->jn[c] + dj);
     dj = nw - w->jn[c];
     w->jn[c] = nw;
     v3 Jn = vmul(nrm, dj);
@@ -729,20 +729,20 @@ static void jac_vel(or_world *w, int c, float scale) {
         acc3(w->acc, N, bj, 3, mv(&IB, vcross(rB, Jn)), -1.0f, scale);
     }
     v3 rtA1 = vcross(rA, t1);
-    float meft1 = iMA + vdot(rtA1, mv(&IA, rtA1));
+    real meft1 = iMA + vdot(rtA1, mv(&IA, rtA1));
     v3 rtA2 = vcross(rA, t2);
-    float meft2 = iMA + vdot(rtA2, mv(&IA, rtA2));
+    real meft2 = iMA + vdot(rtA2, mv(&IA, rtA2));
     if (bj >= 0) {
         v3 rtB1 = vcross(rB, t1);
         meft1 = meft1 + iMB + vdot(rtB1, mv(&IB, rtB1));
         v3 rtB2 = vcross(rB, t2);
         meft2 = meft2 + iMB + vdot(rtB2, mv(&IB, rtB2));
     }
-    float o1 = w->jt1[c], o2 = w->jt2[c];
-    float a1 = o1 - vdot(vrel, t1) / meft1;
-    float a2 = o2 - vdot(vrel, t2) / meft2;
-    float mag = sqrtf(a1 * a1 + a2 * a2);
-    float lim = w->cmu[c] * w->jn[c];
+    real o1 = w->jt1[c], o2 = w->jt2[c];
+    real a1 = o1 - vdot(vrel, t1) / meft1;
+    real a2 = o2 - vdot(vrel, t2) / meft2;
+    real mag = RSQRT(a1 * a1 + a2 * a2);
+    real lim = w->cmu[c] * w->jn[c];
     if (mag > lim && mag > 1e-12f) {
         a1 = a1 * lim / mag;
         a2 = a2 * lim / mag;
@@ -757,15 +757,15 @@ static void jac_vel(or_world *w, int c, float scale) {
     }
 }
 
-static void jac_pos(or_world *w, int c, float dt, float scale) {
+static void jac_pos(or_world *w, int c, real dt, real scale) {
     int N = w->N;
     int bi = w->cbi[c], bj = w->cbj[c];
     v3 nrm = ld3(w->cn, c), rA = vsub(ld3(w->cpA, c), ld3(w->x, bi));
-    float iMA = w->invM[bi];
+    real iMA = w->invM[bi];
     m33 IA = ldm(w->invIw, bi), IB;
     v3 rel = vadd(ld3(w->pv, bi), vcross(ld3(w->po, bi), rA));
     v3 rB = V3(0.0f, 0.0f, 0.0f);
-    float iMB = 0.0f;
+    real iMB = 0.0f;
     if (bj >= 0) {
         rB = vsub(ld3(w->cpB, c), ld3(w->x, bj));
         rel = vsub(rel, vadd(ld3(w->pv, bj), vcross(ld3(w->po, bj), rB)));
@@ -773,15 +773,15 @@ static void jac_pos(or_world *w, int c, float dt, float scale) {
         IB = ldm(w->invIw, bj);
     }
     v3 rnA = vcross(rA, nrm);
-    float meff = iMA + vdot(rnA, mv(&IA, rnA));
+    real meff = iMA + vdot(rnA, mv(&IA, rnA));
     if (bj >= 0) {
         v3 rnB = vcross(rB, nrm);
         meff = meff + iMB + vdot(rnB, mv(&IB, rnB));
     }
     if (meff < 1e-12f) return;
-    float bias = 0.2f * wmax(w->cpen[c] - 1e-4f, 0.0f) / dt;
-    float dj = (bias - vdot(rel, nrm)) / meff;
-    float nw = wmax(0.0f, w->jp[c] + dj);
+    real bias = 0.2f * wmax(w->cpen[c] - 1e-4f, 0.0f) / dt;
+    real dj = (bias - vdot(rel, nrm)) / meff;
+    real nw = wmax(0.0f, w->jp[c] + dj);
     dj = nw - w->jp[c];
     w->jp[c] = nw;
     v3 Jn = vmul(nrm, dj);
@@ -794,16 +794,16 @@ static void jac_pos(or_world *w, int c, float dt, float scale) {
 }
 
 /* k_apply (contact_engine_gpu.py:134-143): dv = f32(f64(acc) * f64(f32(1/scale))), v += relax*dv */
-static void jac_apply(or_world *w, float *vv, float *ww, float relax, float scale) {
+static void jac_apply(or_world *w, real *vv, real *ww, real relax, real scale) {
     int N = w->N;
-    float inv = 1.0f / scale;
+    real inv = 1.0f / scale;
     for (int i = 0; i < N; ++i) {
-        v3 dv = V3((float)((double)w->acc[0 * (size_t)N + i] * (double)inv),
-                   (float)((double)w->acc[1 * (size_t)N + i] * (double)inv),
-                   (float)((double)w->acc[2 * (size_t)N + i] * (double)inv));
-        v3 dw = V3((float)((double)w->acc[3 * (size_t)N + i] * (double)inv),
-                   (float)((double)w->acc[4 * (size_t)N + i] * (double)inv),
-                   (float)((double)w->acc[5 * (size_t)N + i] * (double)inv));
+        v3 dv = V3((real)((double)w->acc[0 * (size_t)N + i] * (double)inv),
+                   (real)((double)w->acc[1 * (size_t)N + i] * (double)inv),
+                   (real)((double)w->acc[2 * (size_t)N + i] * (double)inv));
+        v3 dw = V3((real)((double)w->acc[3 * (size_t)N + i] * (double)inv),
+                   (real)((double)w->acc[4 * (size_t)N + i] * (double)inv),
+                   (real)((double)w->acc[5 * (size_t)N + i] * (double)inv));
         st3(vv, i, vadd(ld3(vv, i), smul(relax, dv)));
         st3(ww, i, vadd(ld3(ww, i), smul(relax, dw)));
         for (int k = 0; k < 6; ++k) w->acc[k * (size_t)N + i] = 0;
@@ -852,9 +852,9 @@ static void warm_lookup(or_world *w) {
 static void warm_store(or_world *w) {
     w->wn = w->C;
     memcpy(w->wkey, w->key, 8 * (size_t)w->C);
-    memcpy(w->wjn, w->jn, 4 * (size_t)w->C);
-    memcpy(w->wjt1, w->jt1, 4 * (size_t)w->C);
-    memcpy(w->wjt2, w->jt2, 4 * (size_t)w->C);
+    memcpy(w->wjn, w->jn, sizeof(real) * (size_t)w->C);
+    memcpy(w->wjt1, w->jt1, sizeof(real) * (size_t)w->C);
+    memcpy(w->wjt2, w->jt2, sizeof(real) * (size_t)w->C);
 }
 
 /* PCG hash (Jarzynski & Olano 2020, "Hash Functions for GPU Rendering"), the generator behind the
@@ -961,7 +961,7 @@ static int color_cached(or_world *w) {
     memcpy(w->cache_a, w->pbi, sizeof(int) * (size_t)P);
     memcpy(w->cache_b, w->pbj, sizeof(int) * (size_t)P);
     memcpy(w->cache_col, w->col, sizeof(int) * (size_t)P);
-    memcpy(w->cache_invM, w->invM, sizeof(float) * (size_t)w->N);
+    memcpy(w->cache_invM, w->invM, sizeof(real) * (size_t)w->N);
     w->cached_P = P; w->cached_ncol = nc;
     w->misses++;
     return nc;
@@ -970,7 +970,7 @@ static int color_cached(or_world *w) {
 /* ───────────────────────── A18-A20 coloured chunk sweeps ───────────────────────── */
 static inline v3 fixed_t1(v3 nrm) {
     v3 aa = V3(1.0f, 0.0f, 0.0f);
-    if (fabsf(nrm.x) >= 0.9f) aa = V3(0.0f, 1.0f, 0.0f);
+    if (RFABS(nrm.x) >= 0.9f) aa = V3(0.0f, 1.0f, 0.0f);
     return vnormalize(vsub(aa, vmul(nrm, vdot(aa, nrm))));
 }
 
@@ -1004,12 +1004,12 @@ static void apply_warm_pair(or_world *w, int p) {
 /* chunk velocity sweep: fixed = 1 -> k_fixed_frame (contact_engine_gpu_fixed_frame.py:7-77, the
  * noncontracting form contact_engine_gpu_unfused_solve.py:117-220); fixed = 0 -> k_chunk_vel_r
  * (contact_engine_gpu_colored.py:413-490, slip-direction frame) */
-static void chunk_vel(or_world *w, int p, int fixed, float t2d) {
+static void chunk_vel(or_world *w, int p, int fixed, real t2d) {
     int bi = w->pbi[p], bj = w->pbj[p];
-    float iMA = w->invM[bi];
+    real iMA = w->invM[bi];
     m33 IA = ldm(w->invIw, bi), IB = ldm(w->invIw, bi);
     v3 vA = ld3(w->v, bi), wA = ld3(w->w, bi);
-    float iMB = 0.0f;
+    real iMB = 0.0f;
     v3 vB = V3(0.0f, 0.0f, 0.0f), wB = V3(0.0f, 0.0f, 0.0f);
     if (bj >= 0) { iMB = w->invM[bj]; IB = ldm(w->invIw, bj); vB = ld3(w->v, bj); wB = ld3(w->w, bj); }
     for (int k = 0; k < w->pcount[p]; ++k) {
@@ -1019,26 +1019,26 @@ static void chunk_vel(or_world *w, int p, int fixed, float t2d) {
         v3 vb = V3(0.0f, 0.0f, 0.0f), rB = V3(0.0f, 0.0f, 0.0f);
         if (bj >= 0) { rB = vsub(ld3(w->cpB, c), ld3(w->x, bj)); vb = vadd(vB, vcross(wB, rB)); }
         v3 vrel = vsub(va, vb);
-        float vn = vdot(vrel, nrm);
+        real vn = vdot(vrel, nrm);
         v3 t1;
         if (fixed) {
             t1 = fixed_t1(nrm);
         } else {
             v3 vt = vsub(vrel, vmul(nrm, vn));
-            float mt = vlen(vt);
+            real mt = vlen(vt);
             t1 = vdiv(vt, mt);
             if (mt <= 5e-3f) t1 = fixed_t1(nrm);
         }
         v3 t2 = vcross(nrm, t1);
         v3 rnA = vcross(rA, nrm);
-        float meff = iMA + vdot(rnA, mv(&IA, rnA));
+        real meff = iMA + vdot(rnA, mv(&IA, rnA));
         if (bj >= 0) {
             v3 rnB = vcross(rB, nrm);
             meff = meff + iMB + vdot(rnB, mv(&IB, rnB));
         }
         if (!(meff >= 1e-12f)) continue;
-        float dj = -vn / meff;
-        float nw = wmax(0.0f, w->jn[c] + dj);
+        real dj = -vn / meff;
+        real nw = wmax(0.0f, w->jn[c] + dj);
         dj = nw - w->jn[c];
         w->jn[c] = nw;
         v3 Jn = vmul(nrm, dj);
@@ -1049,9 +1049,9 @@ static void chunk_vel(or_world *w, int p, int fixed, float t2d) {
         if (bj >= 0) vb = vadd(vB, vcross(wB, rB));
         vrel = vsub(va, vb);
         v3 rtA1 = vcross(rA, t1);
-        float meft1 = iMA + vdot(rtA1, mv(&IA, rtA1));
+        real meft1 = iMA + vdot(rtA1, mv(&IA, rtA1));
         v3 rtA2 = vcross(rA, t2);
-        float meft2 = iMA + vdot(rtA2, mv(&IA, rtA2));
+        real meft2 = iMA + vdot(rtA2, mv(&IA, rtA2));
         if (bj >= 0) {
             v3 rtB1 = vcross(rB, t1);
             meft1 = meft1 + iMB + vdot(rtB1, mv(&IB, rtB1));
@@ -1059,11 +1059,11 @@ static void chunk_vel(or_world *w, int p, int fixed, float t2d) {
             meft2 = meft2 + iMB + vdot(rtB2, mv(&IB, rtB2));
         }
         if (meft1 >= 1e-12f && meft2 >= 1e-12f) {
-            float o1 = w->jt1[c], o2 = w->jt2[c];
-            float a1 = o1 - vdot(vrel, t1) / meft1;
-            float a2 = o2 - vdot(vrel, t2) / meft2;
-            float mag = sqrtf(a1 * a1 + a2 * a2);
-            float lim = w->cmu[c] * w->jn[c];
+            real o1 = w->jt1[c], o2 = w->jt2[c];
+            real a1 = o1 - vdot(vrel, t1) / meft1;
+            real a2 = o2 - vdot(vrel, t2) / meft2;
+            real mag = RSQRT(a1 * a1 + a2 * a2);
+            real lim = w->cmu[c] * w->jn[c];
             if (mag > lim && mag > 1e-12f) { a1 = a1 * lim / mag; a2 = a2 * lim / mag; }
             w->jt1[c] = a1; w->jt2[c] = a2;
             v3 Jt = vadd(vmul(t1, a1 - o1), vmul(t2, t2d * (a2 - o2)));
@@ -1076,12 +1076,12 @@ static void chunk_vel(or_world *w, int p, int fixed, float t2d) {
 }
 
 /* k_chunk_pos_r (contact_engine_gpu_colored.py:511-554; unfused_position.py:7-65) */
-static void chunk_pos(or_world *w, int p, float dt) {
+static void chunk_pos(or_world *w, int p, real dt) {
     int bi = w->pbi[p], bj = w->pbj[p];
-    float iMA = w->invM[bi];
+    real iMA = w->invM[bi];
     m33 IA = ldm(w->invIw, bi), IB = ldm(w->invIw, bi);
     v3 pvA = ld3(w->pv, bi), poA = ld3(w->po, bi);
-    float iMB = 0.0f;
+    real iMB = 0.0f;
     v3 pvB = V3(0.0f, 0.0f, 0.0f), poB = V3(0.0f, 0.0f, 0.0f);
     if (bj >= 0) { iMB = w->invM[bj]; IB = ldm(w->invIw, bj); pvB = ld3(w->pv, bj); poB = ld3(w->po, bj); }
     for (int k = 0; k < w->pcount[p]; ++k) {
@@ -1091,15 +1091,15 @@ static void chunk_pos(or_world *w, int p, float dt) {
         v3 rB = V3(0.0f, 0.0f, 0.0f);
         if (bj >= 0) { rB = vsub(ld3(w->cpB, c), ld3(w->x, bj)); rel = vsub(rel, vadd(pvB, vcross(poB, rB))); }
         v3 rnA = vcross(rA, nrm);
-        float meff = iMA + vdot(rnA, mv(&IA, rnA));
+        real meff = iMA + vdot(rnA, mv(&IA, rnA));
         if (bj >= 0) {
             v3 rnB = vcross(rB, nrm);
             meff = meff + iMB + vdot(rnB, mv(&IB, rnB));
         }
         if (!(meff >= 1e-12f)) continue;
-        float bias = 0.2f * wmax(w->cpen[c] - 1e-4f, 0.0f) / dt;
-        float dj = (bias - vdot(rel, nrm)) / meff;
-        float nw = wmax(0.0f, w->jp[c] + dj);
+        real bias = 0.2f * wmax(w->cpen[c] - 1e-4f, 0.0f) / dt;
+        real dj = (bias - vdot(rel, nrm)) / meff;
+        real nw = wmax(0.0f, w->jp[c] + dj);
         dj = nw - w->jp[c];
         w->jp[c] = nw;
         v3 Jn = vmul(nrm, dj);
@@ -1115,28 +1115,28 @@ static void generate(or_world *w) {
     if (w->P > 0) bead_contacts(w); else box_contacts(w);
 }
 
-static void forces_from(or_world *w, const float *before, double sdt) {
+static void forces_from(or_world *w, const real *before, double sdt) {
     for (int i = 0; i < w->N; ++i)
         for (int k = 0; k < 3; ++k) {
             if (w->kin[i] != 0) { w->force[3 * i + k] = 0.0; continue; }
-            float dv = w->v[3 * i + k] - before[3 * i + k];
+            real dv = w->v[3 * i + k] - before[3 * i + k];
             w->force[3 * i + k] = w->M[i] * (double)dv / sdt;
         }
 }
 
 /* A12 Jacobi step (contact_engine_gpu.py:252-280) */
 static int step_jacobi(or_world *w, double sdt64) {
-    float sdt = (float)sdt64;
-    const float S = 1.0e10f;
+    real sdt = (real)sdt64;
+    const real S = 1.0e10f;
     grav(w, sdt);
     inv_inertia_world(w);
     generate(w);
-    memcpy(w->vsnap, w->v, 12 * (size_t)w->N);
+    memcpy(w->vsnap, w->v, 3 * sizeof(real) * (size_t)w->N);
     int C = w->C;
     if (C > 0) {
-        memset(w->jn, 0, 4 * (size_t)C); memset(w->jt1, 0, 4 * (size_t)C);
-        memset(w->jt2, 0, 4 * (size_t)C); memset(w->jp, 0, 4 * (size_t)C);
-        memset(w->pv, 0, 12 * (size_t)w->N); memset(w->po, 0, 12 * (size_t)w->N);
+        memset(w->jn, 0, sizeof(real) * (size_t)C); memset(w->jt1, 0, sizeof(real) * (size_t)C);
+        memset(w->jt2, 0, sizeof(real) * (size_t)C); memset(w->jp, 0, sizeof(real) * (size_t)C);
+        memset(w->pv, 0, 3 * sizeof(real) * (size_t)w->N); memset(w->po, 0, 3 * sizeof(real) * (size_t)w->N);
         memset(w->acc, 0, 48 * (size_t)w->N);
         for (int it = 0; it < w->prm.vit; ++it) {
             for (int c = 0; c < C; ++c) jac_vel(w, c, S);
@@ -1156,13 +1156,13 @@ static int step_jacobi(or_world *w, double sdt64) {
 /* coloured pair-chunk step (contact_engine_gpu_colored.py:1302-1375, pair_chunks path) and the
  * ColorCache step (contact_engine_gpu_deferred_force.py:13-64 with the colour cache) */
 static int step_colored(or_world *w, double sdt64, int colorcache) {
-    float sdt = (float)sdt64;
+    real sdt = (real)sdt64;
     grav(w, sdt);
     inv_inertia_world(w);
     generate(w);
-    memcpy(w->vsnap, w->v, 12 * (size_t)w->N);
+    memcpy(w->vsnap, w->v, 3 * sizeof(real) * (size_t)w->N);
     int C = w->C;
-    memset(w->pv, 0, 12 * (size_t)w->N); memset(w->po, 0, 12 * (size_t)w->N);
+    memset(w->pv, 0, 3 * sizeof(real) * (size_t)w->N); memset(w->po, 0, 3 * sizeof(real) * (size_t)w->N);
     if (C > 0) {
         warm_lookup(w);
         build_pairs(w);
@@ -1210,38 +1210,38 @@ double or_time_steps(or_world *w, double dt, int substeps, int steps) {
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
 
-void or_get_state(const or_world *w, float *x, float *q, float *v, float *om) {
+void or_get_state(const or_world *w, real *x, real *q, real *v, real *om) {
     int N = w->N;
-    if (x) memcpy(x, w->x, 12 * (size_t)N);
-    if (q) memcpy(q, w->q, 16 * (size_t)N);
-    if (v) memcpy(v, w->v, 12 * (size_t)N);
-    if (om) memcpy(om, w->w, 12 * (size_t)N);
+    if (x) memcpy(x, w->x, 3 * sizeof(real) * (size_t)N);
+    if (q) memcpy(q, w->q, 4 * sizeof(real) * (size_t)N);
+    if (v) memcpy(v, w->v, 3 * sizeof(real) * (size_t)N);
+    if (om) memcpy(om, w->w, 3 * sizeof(real) * (size_t)N);
 }
-void or_get_aux(const or_world *w, float *pv, float *po, float *invIw) {
+void or_get_aux(const or_world *w, real *pv, real *po, real *invIw) {
     int N = w->N;
-    if (pv) memcpy(pv, w->pv, 12 * (size_t)N);
-    if (po) memcpy(po, w->po, 12 * (size_t)N);
-    if (invIw) memcpy(invIw, w->invIw, 36 * (size_t)N);
+    if (pv) memcpy(pv, w->pv, 3 * sizeof(real) * (size_t)N);
+    if (po) memcpy(po, w->po, 3 * sizeof(real) * (size_t)N);
+    if (invIw) memcpy(invIw, w->invIw, 9 * sizeof(real) * (size_t)N);
 }
 void or_contact_forces(const or_world *w, double *F) { memcpy(F, w->force, 24 * (size_t)w->N); }
 void or_counters(const or_world *w, int64_t *out) {
     out[0] = w->C; out[1] = w->Pp; out[2] = w->ncol; out[3] = w->hits; out[4] = w->misses;
     out[5] = w->overflow;
 }
-int or_get_contacts(const or_world *w, int64_t *key, int *bi, int *bj, float *pA, float *pB, float *n,
-                    float *pen, float *jn, float *jt1, float *jt2, float *jp) {
+int or_get_contacts(const or_world *w, int64_t *key, int *bi, int *bj, real *pA, real *pB, real *n,
+                    real *pen, real *jn, real *jt1, real *jt2, real *jp) {
     int C = w->C;
     if (key) memcpy(key, w->key, 8 * (size_t)C);
     if (bi) memcpy(bi, w->cbi, 4 * (size_t)C);
     if (bj) memcpy(bj, w->cbj, 4 * (size_t)C);
-    if (pA) memcpy(pA, w->cpA, 12 * (size_t)C);
-    if (pB) memcpy(pB, w->cpB, 12 * (size_t)C);
-    if (n) memcpy(n, w->cn, 12 * (size_t)C);
-    if (pen) memcpy(pen, w->cpen, 4 * (size_t)C);
-    if (jn) memcpy(jn, w->jn, 4 * (size_t)C);
-    if (jt1) memcpy(jt1, w->jt1, 4 * (size_t)C);
-    if (jt2) memcpy(jt2, w->jt2, 4 * (size_t)C);
-    if (jp) memcpy(jp, w->jp, 4 * (size_t)C);
+    if (pA) memcpy(pA, w->cpA, 3 * sizeof(real) * (size_t)C);
+    if (pB) memcpy(pB, w->cpB, 3 * sizeof(real) * (size_t)C);
+    if (n) memcpy(n, w->cn, 3 * sizeof(real) * (size_t)C);
+    if (pen) memcpy(pen, w->cpen, sizeof(real) * (size_t)C);
+    if (jn) memcpy(jn, w->jn, sizeof(real) * (size_t)C);
+    if (jt1) memcpy(jt1, w->jt1, sizeof(real) * (size_t)C);
+    if (jt2) memcpy(jt2, w->jt2, sizeof(real) * (size_t)C);
+    if (jp) memcpy(jp, w->jp, sizeof(real) * (size_t)C);
     return C;
 }
 int or_get_pairs(const or_world *w, int *pbi, int *pbj, int *pstart, int *pcount, int *color) {

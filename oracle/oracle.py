@@ -11,6 +11,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
+LIB64 = HERE / "_build" / "liboracle64.so"  # the same restatement in float64 (FD reference of the adjoint)
 
 JACOBI, PAIR_CHUNKS, COLOR_CACHE = 0, 1, 2
 SOLVERS = {"jacobi": JACOBI, "pair_chunks": PAIR_CHUNKS, "color_cache": COLOR_CACHE}
@@ -28,16 +29,17 @@ def build() -> Path:
     return LIB
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not LIB.exists():
+def lib(fp64=False):
+    if fp64 not in _libs:
+        path = LIB64 if fp64 else LIB
+        if not path.exists():
             build()
-        L = C.CDLL(str(LIB))
+        L = C.CDLL(str(path))
         vp = C.c_void_p
+        real = C.c_double if fp64 else C.c_float
         L.or_create.restype = vp
         L.or_create.argtypes = [C.c_int, C.c_int, C.POINTER(Params)]
         L.or_destroy.argtypes = [vp]
@@ -57,10 +59,10 @@ def lib():
         L.or_get_pairs.restype = C.c_int
         L.or_quat_to_matrix.argtypes = [vp, vp]
         L.or_quat_rotate.argtypes = [vp, vp, vp]
-        L.or_integrate.argtypes = [C.c_int] + [vp] * 8 + [C.c_float]
-        L.or_box_box.argtypes = [vp] * 6 + [C.c_float] + [vp] * 4
+        L.or_integrate.argtypes = [C.c_int] + [vp] * 8 + [real]
+        L.or_box_box.argtypes = [vp] * 6 + [real] + [vp] * 4
         L.or_box_box.restype = C.c_int
-        L.or_box_ground.argtypes = [vp] * 3 + [C.c_float] + [vp] * 3
+        L.or_box_ground.argtypes = [vp] * 3 + [real] + [vp] * 3
         L.or_box_ground.restype = C.c_int
         L.or_pcg.argtypes = [C.c_uint32]
         L.or_pcg.restype = C.c_uint32
@@ -68,8 +70,8 @@ def lib():
         L.or_jp_key.restype = C.c_int64
         L.or_time_steps.argtypes = [vp, C.c_double, C.c_int, C.c_int]
         L.or_time_steps.restype = C.c_double
-        _lib = L
-    return _lib
+        _libs[fp64] = L
+    return _libs[fp64]
 
 
 def _p(a):
@@ -81,11 +83,14 @@ def _f32(a, shape):
 
 
 class OracleWorld:
-    """One simulated world on the CPU restatement."""
+    """One simulated world on the CPU restatement (fp64=True: the float64 build)."""
 
     def __init__(self, bodies, solver="jacobi", vit=40, pit=20, relax=0.2, t2d=1.0,
-                 warm_start=True, R=0.05, margin=0.0, cap=None, n_colors=0, per_contact=False):
-        L = lib()
+                 warm_start=True, R=0.05, margin=0.0, cap=None, n_colors=0, per_contact=False, fp64=False):
+        self.fp64 = bool(fp64)
+        self.dtype = np.float64 if fp64 else np.float32
+        L = lib(self.fp64)
+        _f32 = self._arr
         N = bodies.n
         P = int(len(bodies.rest))
         if cap is None:
@@ -110,55 +115,58 @@ class OracleWorld:
             rest = _f32(bodies.rest, (P, 3))
             L.or_set_rest(self.h, _p(rest))
 
+    def _arr(self, a, shape):
+        return np.ascontiguousarray(np.asarray(a, self.dtype).reshape(shape))
+
     def __del__(self):
         try:
-            lib().or_destroy(self.h)
+            lib(self.fp64).or_destroy(self.h)
         except Exception:
             pass
 
     def step(self, dt, substeps=1):
-        rc = lib().or_step(self.h, float(dt), int(substeps))
+        rc = lib(self.fp64).or_step(self.h, float(dt), int(substeps))
         if rc == 4:
             raise RuntimeError("oracle: contact capacity exceeded")
         if rc:
             raise RuntimeError(f"oracle: step failed ({rc})")
 
     def time_steps(self, dt, substeps, steps):
-        return lib().or_time_steps(self.h, float(dt), int(substeps), int(steps))
+        return lib(self.fp64).or_time_steps(self.h, float(dt), int(substeps), int(steps))
 
     def state(self):
         N = self.N
-        x = np.zeros((N, 3), np.float32); q = np.zeros((N, 4), np.float32)
-        v = np.zeros((N, 3), np.float32); w = np.zeros((N, 3), np.float32)
-        lib().or_get_state(self.h, _p(x), _p(q), _p(v), _p(w))
+        x = np.zeros((N, 3), self.dtype); q = np.zeros((N, 4), self.dtype)
+        v = np.zeros((N, 3), self.dtype); w = np.zeros((N, 3), self.dtype)
+        lib(self.fp64).or_get_state(self.h, _p(x), _p(q), _p(v), _p(w))
         return x, q, v, w
 
     def aux(self):
         N = self.N
-        pv = np.zeros((N, 3), np.float32); po = np.zeros((N, 3), np.float32)
-        I = np.zeros((N, 3, 3), np.float32)
-        lib().or_get_aux(self.h, _p(pv), _p(po), _p(I))
+        pv = np.zeros((N, 3), self.dtype); po = np.zeros((N, 3), self.dtype)
+        I = np.zeros((N, 3, 3), self.dtype)
+        lib(self.fp64).or_get_aux(self.h, _p(pv), _p(po), _p(I))
         return pv, po, I
 
     def forces(self):
         F = np.zeros((self.N, 3), np.float64)
-        lib().or_contact_forces(self.h, _p(F))
+        lib(self.fp64).or_contact_forces(self.h, _p(F))
         return F
 
     def counters(self):
         out = np.zeros(6, np.int64)
-        lib().or_counters(self.h, _p(out))
+        lib(self.fp64).or_counters(self.h, _p(out))
         return dict(n_contacts=int(out[0]), n_pairs=int(out[1]), n_colors=int(out[2]),
                     cache_hits=int(out[3]), cache_misses=int(out[4]), overflow=int(out[5]))
 
     def contacts(self):
         C_ = self.counters()["n_contacts"]
         key = np.zeros(C_, np.int64); bi = np.zeros(C_, np.int32); bj = np.zeros(C_, np.int32)
-        pA = np.zeros((C_, 3), np.float32); pB = np.zeros((C_, 3), np.float32)
-        n = np.zeros((C_, 3), np.float32); pen = np.zeros(C_, np.float32)
-        jn = np.zeros(C_, np.float32); jt1 = np.zeros(C_, np.float32)
-        jt2 = np.zeros(C_, np.float32); jp = np.zeros(C_, np.float32)
-        lib().or_get_contacts(self.h, _p(key), _p(bi), _p(bj), _p(pA), _p(pB), _p(n), _p(pen),
+        pA = np.zeros((C_, 3), self.dtype); pB = np.zeros((C_, 3), self.dtype)
+        n = np.zeros((C_, 3), self.dtype); pen = np.zeros(C_, self.dtype)
+        jn = np.zeros(C_, self.dtype); jt1 = np.zeros(C_, self.dtype)
+        jt2 = np.zeros(C_, self.dtype); jp = np.zeros(C_, self.dtype)
+        lib(self.fp64).or_get_contacts(self.h, _p(key), _p(bi), _p(bj), _p(pA), _p(pB), _p(n), _p(pen),
                               _p(jn), _p(jt1), _p(jt2), _p(jp))
         return dict(key=key, bi=bi, bj=bj, pA=pA, pB=pB, n=n, pen=pen, jn=jn, jt1=jt1, jt2=jt2,
                     jp=jp)
@@ -167,7 +175,7 @@ class OracleWorld:
         P = self.counters()["n_pairs"]
         a = np.zeros(P, np.int32); b = np.zeros(P, np.int32); s = np.zeros(P, np.int32)
         c = np.zeros(P, np.int32); col = np.zeros(P, np.int32)
-        lib().or_get_pairs(self.h, _p(a), _p(b), _p(s), _p(c), _p(col))
+        lib(self.fp64).or_get_pairs(self.h, _p(a), _p(b), _p(s), _p(c), _p(col))
         return dict(pbi=a, pbj=b, pstart=s, pcount=c, color=col)
 
 
