@@ -2,7 +2,7 @@
 reference ContactEngine protocol (antonbj3/3fold-motion-engine, src/motion_engine/contact_engine.py)."""
 from .protocol import BodyState, ContactEngine
 from .engines import ColorCacheContactEngine, GraphColoredContactEngine, RelaxedJacobiContactEngine
-from . import bodies, scenes
+from . import adjoint, bodies, scenes
 
 __all__ = ["BodyState", "ContactEngine", "RelaxedJacobiContactEngine", "GraphColoredContactEngine",
-           "ColorCacheContactEngine", "bodies", "scenes"]
+           "ColorCacheContactEngine", "adjoint", "bodies", "scenes"]

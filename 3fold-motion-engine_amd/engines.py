@@ -221,7 +221,8 @@ class _HipContactEngine:
         ovf = s._counters["overflow"]
         if ovf:
             what = [n for bit, n in ((1, "contacts"), (2, "body pairs"), (4, "broad-phase candidates"),
-                                     (8, "colours"), (16, "tile")) if ovf & bit]
+                                     (8, "colours"), (16, "tile"), (32, "recorded tile for the adjoint"))
+                    if ovf & bit]
             raise RuntimeError(f"{s.name}.step: capacity exceeded ({', '.join(what)}); raise max_pairs / "
                                f"max_contacts")
         return s._counters
