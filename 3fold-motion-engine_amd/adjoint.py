@@ -97,6 +97,7 @@ class Rollout:
         """Forward: set initial velocities (N,3), (N,3) and friction (N,), step T steps; returns x_T."""
         torch = _torch()
         e = self.eng
+        _lib.check(_lib.lib().me_adopt(e._h), "me_adopt")  # forward and backward may run on different threads
         with torch.no_grad():
             self._restore(self._init)
             e.v.copy_(v0.reshape(-1, 3))
@@ -121,6 +122,7 @@ class Rollout:
         torch = _torch()
         e = self.eng
         L = _lib.lib()
+        _lib.check(L.me_adopt(e._h), "me_adopt")
         dev = e.x.device
         xbar = xbar_T.reshape(-1, 3).to(torch.float32).clone().contiguous()
         qbar = torch.zeros((self.N, 4), dtype=torch.float32, device=dev)

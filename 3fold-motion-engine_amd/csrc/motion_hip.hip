@@ -706,6 +706,12 @@ int me_destroy(me_engine *e) {
     return ME_OK;
 }
 
+int me_adopt(me_engine *e) {
+    if (!e) return fail(ME_EARG, "engine is NULL");
+    e->owner = std::this_thread::get_id();
+    return ME_OK;
+}
+
 int me_set_bodies(me_engine *e, const me_bodies *bodies) {
     if (!e || !bodies) return fail(ME_EARG, "NULL argument");
     if (std::this_thread::get_id() != e->owner) return fail(ME_ETHREAD, "engine used from a thread that does not own it");

@@ -93,6 +93,19 @@ def batched_piles(n_scenes=256, lo=8, hi=2048, seed=0):
     return concat(parts), sizes
 
 
+def fixed_piles(n_scenes=1024, n=64, seed=0, vel_sigma=0.1, target_half=0.5):
+    """n_scenes independent piles of n boxes (box_pile draws per scene from one generator, scene ids
+    0..n_scenes-1), then initial velocities v0, w0 ~ N(0, vel_sigma) (N*n_scenes x 3 each) and targets
+    U(-target_half, target_half)^3 per box, in that order. Returns (BodySet, targets)."""
+    rng = np.random.default_rng(seed)
+    b = concat([box_pile(n, rng=rng, scene=s) for s in range(n_scenes)])
+    N = b.n
+    b.v[:] = rng.normal(0.0, vel_sigma, size=(N, 3))
+    b.w[:] = rng.normal(0.0, vel_sigma, size=(N, 3))
+    targets = rng.uniform(-target_half, target_half, size=(N, 3))
+    return b, targets
+
+
 def concat(parts) -> BodySet:
     cat = lambda k: np.concatenate([getattr(b, k) for b in parts])
     return BodySet(**{k: cat(k) for k in ("x", "q", "v", "w", "invM", "IbInv", "kin", "half", "mu",

@@ -73,7 +73,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large"])
+    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large", "adjoint"])
+    ap.add_argument("--rollouts", type=int, default=3, help="adjoint workload: timed forward+backward rollouts")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--seq-rounds", type=int, default=3)
     ap.add_argument("--autotune", action="store_true")
@@ -89,6 +90,8 @@ def main():
 
     if a.workload == "batched":
         return bench_batched(a)
+    if a.workload == "adjoint":
+        return bench_adjoint(a)
     if a.workload == "large":
         a.bodies = 100000 if a.bodies == 10000 else a.bodies
         a.tiled = 1
@@ -242,6 +245,60 @@ def _traffic_from_profile(kernel, workload):
     if not rec or rec.get("workload") != workload:
         return None
     return rec.get("bytes_per_launch")
+
+
+def bench_adjoint(a):
+    """Differentiable rollout: 1024 scenes of 64-box piles (v0, w0 ~ N(0, 0.1), targets U(-0.5, 0.5)^3),
+    32 steps of the ColorCache step (S2 V16 P10, 16 colours), checkpoints every k = 8 steps; loss =
+    sum of squared distances of the final centres to the targets; gradient w.r.t. v0, w0 and the
+    per-body friction through the hand-written adjoint behind torch.autograd.Function. One warm-up
+    rollout, then --rollouts timed forward + backward rollouts."""
+    import torch
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    n_sc, n, T, k = 1024, 64, 32, 8
+    b, target = pkg.scenes.fixed_piles(n_sc, n, seed=a.seed)
+    N = b.n
+    dt, S = 1 / 240, a.substeps
+    ro = pkg.adjoint.Rollout(b, steps=T, dt=dt, substeps=S, vit=a.vit, pit=a.pit, n_colors=a.colors, k=k,
+                             tile_bodies=64)
+    fn = pkg.adjoint.contact_rollout(ro)
+    dev = ro.eng.x.device
+    f = lambda x: torch.tensor(np.asarray(x), dtype=torch.float32, device=dev)
+    tgt = f(target)
+    times = []
+    gnorm = None
+    for r in range(1 + a.rollouts):
+        v0 = f(b.v).requires_grad_(True)
+        w0 = f(b.w).requires_grad_(True)
+        mu = f(b.mu).requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xT = fn(v0, w0, mu)
+        loss = torch.sum((xT - tgt) ** 2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        print(f"[bench] rollout {r}: forward {1e3 * (t1 - t0):.1f} ms, backward {1e3 * (t2 - t1):.1f} ms, "
+              f"loss {loss.item():.4f}", file=sys.stderr, flush=True)
+        if r > 0:
+            times.append((t1 - t0, t2 - t1))
+        gnorm = [float(torch.linalg.vector_norm(g).item()) for g in (v0.grad, w0.grad, mu.grad)]
+    fw = sum(t[0] for t in times) / len(times)
+    bw = sum(t[1] for t in times) / len(times)
+    ms_per_step = 1e3 * (fw + bw) / T
+    out = {"metric": METRIC, "value": round(N * T / (fw + bw), 1), "unit": "body-steps/s", "n_gpus": 1,
+           "steps": T * a.rollouts, "warmup": T, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box piles)",
+           "config": {"workload": f"{n_sc} x {n}-box piles, {T} steps, checkpoint k={k}, ColorCache S{S} V{a.vit} "
+                                  f"P{a.pit}, forward + hand-written adjoint (grad of v0, w0, mu)",
+                      "bodies": N, "scenes": n_sc, "steps_per_rollout": T, "checkpoint_k": k},
+           "adjoint": {"forward_ms_per_step": round(1e3 * fw / T, 4), "backward_ms_per_step": round(1e3 * bw / T, 4),
+                       "adjoint_ms_per_step": round(ms_per_step, 4), "rollouts_timed": len(times),
+                       "grad_norms_v0_w0_mu": gnorm},
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
 
 
 def bench_batched(a):

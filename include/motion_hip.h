@@ -139,6 +139,12 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
               void *stream, me_engine **out);
 int me_destroy(me_engine *e);
 
+/* Hand the engine to the calling thread: every entry point checks that it runs on the owning
+ * thread (ME_ETHREAD otherwise, the convention of body_pairs_rt_native/api.cpp:130-163). For a
+ * caller whose work moves between threads, e.g. a backward pass run by torch's autograd thread;
+ * the previous owner must not use the engine afterwards. */
+int me_adopt(me_engine *e);
+
 /* Re-point the engine at (possibly re-allocated) body arrays; same N. */
 int me_set_bodies(me_engine *e, const me_bodies *bodies);
 
