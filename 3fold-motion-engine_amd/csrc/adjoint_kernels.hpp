@@ -137,8 +137,9 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
         P.IB = P.f.hasB ? g33(A.invIw, b) : P.IA;
         P.c0 = A.Pp.pstart[p];
         P.cnt = A.Pp.pcount[p];
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (k >= P.cnt) break;
+            if (k >= P.cnt) continue;
             const float *o = T.prep + 20 * (size_t)(P.c0 + k);
             Con<float> &C = P.C[k];
             C.n = ad::v3<float>(o[0], o[1], o[2]); C.rA = ad::v3<float>(o[3], o[4], o[5]);
@@ -176,7 +177,9 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
         float jp0[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (p >= 0) {
             const float *si = A.tape.simp + (size_t)(V + s) * 4 * A.capC;
-            for (int k = 0; k < P.cnt; ++k) jp0[k] = si[4 * (size_t)(P.c0 + k) + 3];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < P.cnt) jp0[k] = si[4 * (size_t)(P.c0 + k) + 3];
         }
         __syncthreads();
         PairBodies<float> pre;
@@ -184,7 +187,9 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
             if (c_own == c) {
                 pre = adj_load(P, sv, sw);
                 PairBodies<float> B = pre;
-                for (int k = 0; k < P.cnt; ++k) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (k >= P.cnt) continue;
                     float jp = jp0[k];
                     ad::pos_fwd(B, P.C[k], jp, P.f, P.iMA, P.iMB, P.IA, P.IB);
                 }
@@ -198,14 +203,19 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
                 PairBodies<float> chain[4];
                 float jpre[4];
                 PairBodies<float> B = pre;
-                for (int k = 0; k < P.cnt; ++k) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (k >= P.cnt) continue;
                     chain[k] = B;
                     jpre[k] = jp0[k];
                     float jp = jp0[k];
                     ad::pos_fwd(B, P.C[k], jp, P.f, P.iMA, P.iMB, P.IA, P.IB);
                 }
-                for (int k = P.cnt - 1; k >= 0; --k)
-                    ad::pos_rev(chain[k], P.C[k], jpre[k], P.f, P.iMA, P.iMB, P.IA, P.IB, Bb, jpb[k], cb[k], IAb, IBb);
+#pragma unroll
+                for (int k = 3; k >= 0; --k)
+                    if (k < P.cnt)
+                        ad::pos_rev(chain[k], P.C[k], jpre[k], P.f, P.iMA, P.iMB, P.IA, P.IB, Bb, jpb[k], cb[k], IAb,
+                                    IBb);
                 adj_store(P, bv, bw, Bb);
             }
             __syncthreads();
@@ -218,8 +228,10 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
         P.stat.vA = P.f.dynA ? z : g3(A.b.v, P.a); P.stat.wA = P.f.dynA ? z : g3(A.b.w, P.a);
         bool sB = P.f.hasB && !P.f.dynB;
         P.stat.vB = sB ? g3(A.b.v, P.b) : z; P.stat.wB = sB ? g3(A.b.w, P.b) : z;
-        for (int k = 0; k < P.cnt; ++k)
-            for (int m = 0; m < 3; ++m) jb[k][m] = jbar_end[3 * (size_t)(P.c0 + k) + m];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < P.cnt)
+                for (int m = 0; m < 3; ++m) jb[k][m] = jbar_end[3 * (size_t)(P.c0 + k) + m];
     }
     __syncthreads();
     for (int j = tid; j < nb; j += T_SMALL_P) {
@@ -237,8 +249,10 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
         float j0[4][3];
         if (p >= 0) {
             const float *si = A.tape.simp + (size_t)s * 4 * A.capC;
-            for (int k = 0; k < P.cnt; ++k)
-                for (int m = 0; m < 3; ++m) j0[k][m] = si[4 * (size_t)(P.c0 + k) + m];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < P.cnt)
+                    for (int m = 0; m < 3; ++m) j0[k][m] = si[4 * (size_t)(P.c0 + k) + m];
         }
         __syncthreads();
         PairBodies<float> pre;
@@ -246,7 +260,9 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
             if (c_own == c) {
                 pre = adj_load(P, sv, sw);
                 PairBodies<float> B = pre;
-                for (int k = 0; k < P.cnt; ++k) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (k >= P.cnt) continue;
                     float jn = j0[k][0], jt1 = j0[k][1], jt2 = j0[k][2];
                     ad::vel_fwd(B, P.C[k], jn, jt1, jt2, P.f, P.iMA, P.iMB, P.IA, P.IB);
                 }
@@ -259,14 +275,18 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
                 PairBodies<float> Bb = adj_load_bar(P, bv, bw);
                 PairBodies<float> chain[4];
                 PairBodies<float> B = pre;
-                for (int k = 0; k < P.cnt; ++k) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (k >= P.cnt) continue;
                     chain[k] = B;
                     float jn = j0[k][0], jt1 = j0[k][1], jt2 = j0[k][2];
                     ad::vel_fwd(B, P.C[k], jn, jt1, jt2, P.f, P.iMA, P.iMB, P.IA, P.IB);
                 }
-                for (int k = P.cnt - 1; k >= 0; --k)
-                    ad::vel_rev(chain[k], P.C[k], j0[k][0], j0[k][1], j0[k][2], P.f, P.iMA, P.iMB, P.IA, P.IB, Bb,
-                                jb[k][0], jb[k][1], jb[k][2], cb[k], IAb, IBb);
+#pragma unroll
+                for (int k = 3; k >= 0; --k)
+                    if (k < P.cnt)
+                        ad::vel_rev(chain[k], P.C[k], j0[k][0], j0[k][1], j0[k][2], P.f, P.iMA, P.iMB, P.IA, P.IB, Bb,
+                                    jb[k][0], jb[k][1], jb[k][2], cb[k], IAb, IBb);
                 adj_store(P, bv, bw, Bb);
             }
             __syncthreads();
@@ -277,7 +297,9 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
     if (p >= 0 && np > 0 && V > 0) {
         PairBodies<float> Bb = adj_load_bar(P, bv, bw);
         const float *si = A.tape.simp;
-        for (int k = 0; k < P.cnt; ++k) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k >= P.cnt) continue;
             const float *o = si + 4 * (size_t)(P.c0 + k);
             ad::warm_rev(Bb, P.C[k], o[0], o[1], o[2], P.f, P.iMA, P.iMB, P.IA, P.IB, jb[k][0], jb[k][1], jb[k][2],
                          cb[k], IAb, IBb);
@@ -290,7 +312,9 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
         s3(wbar, i, g3(bw, j));
     }
     if (p >= 0) {
-        for (int k = 0; k < P.cnt; ++k) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k >= P.cnt) continue;
             size_t c = P.c0 + k;
             for (int m = 0; m < 3; ++m) A.tape.jbi[3 * c + m] = jb[k][m];
             float *o = A.tape.cbar + 20 * c;

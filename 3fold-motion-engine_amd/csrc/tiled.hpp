@@ -41,6 +41,7 @@ struct TiledBuffers {
     int *srec;                   // big tiles: 4 ints per colour-sorted pair (p, a, b, first contact)
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
     long long *bigw;             // big-tile work since the last reset: tiles, pairs, dynamic pair ends, contacts
+    int *biglist;                // tiles beyond one small tile (IS_NBIG of them)
 };
 
 // Recording of one substep for its adjoint (all device pointers into one caller buffer; V, P =
@@ -111,6 +112,7 @@ inline void tiled_layout(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
     t.srec = B.template take<int>(4 * p);
     t.dbg = B.template take<long long>(16);
     t.bigw = B.template take<long long>(4);
+    t.biglist = B.template take<int>(n);
 }
 
 // IS_MISMATCH after k_t_mode: 0 = ColorCache hit, 1 = colour this substep, 2 = no contacts
@@ -125,6 +127,7 @@ __global__ void k_t_init(int N, TiledBuffers T, int *istat) {
     if (i == 0) {
         istat[IS_NCOLJP] = 0; istat[IS_ISL] = 0; istat[IS_TILES] = 0; istat[IS_MAXTB] = 0;
         istat[IS_MISMATCH] = 0;
+        istat[IS_NBIG] = 0;
         T.dbg[7] = 0;
     }
 }
@@ -776,6 +779,14 @@ inline int tiled_scan(const TiledArgs &A, const int *in, int *out, int n, int *t
 
 inline int tblk(long long n) { return (int)((n + ME_THREADS - 1) / ME_THREADS); }
 
+// list of the tiles that exceed one small tile (the big-tile kernel's work)
+__global__ void k_t_biglist(int ntile, int *istat, TiledBuffers T) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntile || T.tb0[t] == INT_MAX) return;
+    int nb = T.tb1[t] - T.tb0[t], np = T.tp1[t] - T.tp0[t];
+    if (np > T_SMALL_P || nb > T_SMALL_B) T.biglist[atomicAdd(&istat[IS_NBIG], 1)] = t;
+}
+
 // dynamic-LDS limits of the tile kernels beyond the 64 KB default; set on the current device before
 // the first launch (at engine creation, so that no attribute call falls inside a stream capture)
 inline int tiled_set_attributes() {
@@ -803,7 +814,7 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
         hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>), dim3(ntile), dim3(T_SMALL_P), 0, s, A,
                            T);
     if (A.pev0 && hipEventRecord((hipEvent_t)A.pev0, s2) != hipSuccess) return ME_ERUNTIME;
-    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile), dim3(T_BIG_WG), dyn, s2, A, T);
+    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(T_BIG_WG), dyn, s2, A, T);
     if (A.pev1 && hipEventRecord((hipEvent_t)A.pev1, s2) != hipSuccess) return ME_ERUNTIME;
     if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return ME_ERUNTIME;
     return ME_OK;
@@ -850,6 +861,7 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T, A.b.x, A.b.invM, A.invIw);
     hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
     hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
+    hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile)), dim3(ME_THREADS), 0, s, ntile, A.istat, T);
     hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile), dim3(1024), T_COL_DYN, s, A, T);
     hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
     hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);

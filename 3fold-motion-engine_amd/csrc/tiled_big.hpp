@@ -150,18 +150,16 @@ __device__ __forceinline__ void big_sweeps_stream(const TiledArgs &A, const Tile
         }
 }
 
+// one big tile t by the whole workgroup (LDS: dsm dynamic, the colour tables and scalars static)
 template <int SOLVER>
-__global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T) {
+__device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers &T, int t, float *dsm, int *cst,
+                                         int *cfl, int *cfe, int &maxc, int &wdyn, int &wcon) {
     constexpr bool FIXED = SOLVER == ME_SOLVER_COLOR_CACHE;
-    int t = blockIdx.x;
     if (T.tb0[t] == INT_MAX) return;
     int b0, nb, p0, np;
     tile_bounds(T, t, b0, nb, p0, np);
     if (is_small_tile(nb, np)) return;
     if (A.taped && threadIdx.x == 0) atomicOr(&A.istat[IS_OVF], OVF_TAPE);  // no recording for big tiles
-    extern __shared__ float dsm[];
-    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
-    __shared__ int maxc, wdyn, wcon;
     const int tid = threadIdx.x;
     const bool lds = nb <= T_BIG_B && np <= T_BIG_P;
     // phase stamps of the largest big tile (>= 1000 bodies) of the launch
@@ -330,4 +328,18 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
     }
     __syncthreads();
     stamp(6);
+}
+
+// the big tiles listed by k_t_biglist, T_BIG_GRID workgroups taking them in turn (a launch over
+// every tile would schedule one 152 KB-LDS workgroup per tile, mostly to find a small tile and exit)
+#define T_BIG_GRID 128
+
+template <int SOLVER>
+__global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T) {
+    extern __shared__ float dsm[];
+    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
+    __shared__ int maxc, wdyn, wcon;
+    const int nbig = A.istat[IS_NBIG];
+    for (int i = blockIdx.x; i < nbig; i += gridDim.x)
+        big_tile<SOLVER>(A, T, T.biglist[i], dsm, cst, cfl, cfe, maxc, wdyn, wcon);
 }
