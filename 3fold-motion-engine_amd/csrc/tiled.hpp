@@ -169,6 +169,40 @@ __device__ __forceinline__ int uf_find(const int *parent, int x) {
     return x;
 }
 
+// find with path halving: a node's parent is replaced by its grandparent, an ancestor, so every
+// chain still ends at the same root whatever other threads read or write meanwhile
+__device__ __forceinline__ int uf_find_halve(int *parent, int x) {
+    volatile int *par = parent;
+    int p = par[x];
+    while (p != x) {
+        int g = par[p];
+        if (g != p) par[x] = g;
+        x = p;
+        p = par[x];
+    }
+    return x;
+}
+
+// Atomic add of 1 per lane to cnt[key] with the wave's most common case folded: the lanes that share
+// the first active lane's key add their count once (one atomic instead of up to 64 on one address,
+// the case of an island that holds most of the scene); the others add their own. Returns this
+// lane's old-value slot (base + rank among the lanes of its key).
+__device__ __forceinline__ int wave_add1(int *cnt, int key, bool active) {
+    const unsigned long long act = __ballot(active);
+    if (!active) return 0;
+    const int lane = __lane_id();
+    const int first = __ffsll((long long)act) - 1;
+    const int k0 = __shfl(key, first);
+    const unsigned long long same = __ballot(active && key == k0);
+    if (key == k0) {
+        int base = 0;
+        if (lane == first) base = atomicAdd(&cnt[k0], __popcll(same));
+        base = __shfl(base, first);
+        return base + __popcll(same & ((1ull << lane) - 1ull));
+    }
+    return atomicAdd(&cnt[key], 1);
+}
+
 // lock-free union: link the larger root under the smaller one (compare-and-swap on the root)
 __global__ void k_t_union(int capP, int *istat, const int *pbi, const int *pbj, const float *invM, int *parent) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -176,7 +210,7 @@ __global__ void k_t_union(int capP, int *istat, const int *pbi, const int *pbj, 
     int a = pbi[p], b = pbj[p];
     if (b < 0 || !(invM[a] > 0.0f) || !(invM[b] > 0.0f)) return;
     for (int it = 0; it < (1 << 20); ++it) {
-        int ra = uf_find(parent, a), rb = uf_find(parent, b);
+        int ra = uf_find_halve(parent, a), rb = uf_find_halve(parent, b);
         if (ra == rb) return;
         int hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
         if (atomicCAS(&parent[hi], hi, lo) == hi) return;
@@ -186,32 +220,35 @@ __global__ void k_t_union(int capP, int *istat, const int *pbi, const int *pbj, 
 
 __global__ void k_t_flatten(int N, const float *invM, TiledBuffers T) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    if (invM[i] > 0.0f) {
-        int r = uf_find(T.parent, i);
+    bool dyn = i < N && invM[i] > 0.0f;
+    int r = -1;
+    if (dyn) {
+        r = uf_find_halve(T.parent, i);
         T.root[i] = r;
-        atomicAdd(&T.cnt_b[r], 1);
-    } else {
+    } else if (i < N) {
         T.root[i] = -1;
     }
+    wave_add1(T.cnt_b, r, dyn);
 }
 
 __global__ void k_t_pair_root(int capP, const int *istat, const int *pbi, const int *pbj, const float *invM,
                               TiledBuffers T) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= istat[IS_P] || p >= capP) return;
-    int a = pbi[p], b = pbj[p];
-    int r = invM[a] > 0.0f ? T.root[a] : ((b >= 0 && invM[b] > 0.0f) ? T.root[b] : -1);
-    T.proot[p] = r;
-    if (r >= 0) atomicAdd(&T.cnt_p[r], 1);
+    int r = -1;
+    if (p < istat[IS_P] && p < capP) {
+        int a = pbi[p], b = pbj[p];
+        r = invM[a] > 0.0f ? T.root[a] : ((b >= 0 && invM[b] > 0.0f) ? T.root[b] : -1);
+        T.proot[p] = r;
+    }
+    wave_add1(T.cnt_p, r, r >= 0);
 }
 
 __global__ void k_t_scatter_b(int N, TiledBuffers T, const float *x, const float *invM, const float *invIw) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int r = T.root[i];
+    int r = i < N ? T.root[i] : -1;
+    int slot = wave_add1(T.fill_b, r, r >= 0);
     if (r < 0) return;
-    int pos = T.boff[r] + atomicAdd(&T.fill_b[r], 1);
+    int pos = T.boff[r] + slot;
     T.bsorted[pos] = i;
     T.bslot[i] = pos;
 #pragma unroll
@@ -224,10 +261,9 @@ __global__ void k_t_scatter_b(int N, TiledBuffers T, const float *x, const float
 
 __global__ void k_t_scatter_p(int capP, const int *istat, TiledBuffers T) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= istat[IS_P] || p >= capP) return;
-    int r = T.proot[p];
-    if (r < 0) return;
-    T.psorted[T.poff[r] + atomicAdd(&T.fill_p[r], 1)] = p;
+    int r = (p < istat[IS_P] && p < capP) ? T.proot[p] : -1;
+    int slot = wave_add1(T.fill_p, r, r >= 0);
+    if (r >= 0) T.psorted[T.poff[r] + slot] = p;
 }
 
 __global__ void k_t_tiles(int N, int TB, int *istat, TiledBuffers T) {
