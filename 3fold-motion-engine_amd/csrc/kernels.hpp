@@ -141,40 +141,62 @@ struct Bodies {
     const float *rest;
 };
 
+// One global atomic per wave instead of one per lane on a single status word (every lane of a
+// large launch hitting one address serialises at the memory-side atomic unit). All lanes of the
+// wave must call these.
+__device__ __forceinline__ void wave_atomic_max(int *addr, int val, bool pred) {
+    int v = pred ? val : INT_MIN;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        int w = __shfl_xor(v, o);
+        v = v > w ? v : w;
+    }
+    if (__lane_id() == 0 && v != INT_MIN) atomicMax(addr, v);
+}
+__device__ __forceinline__ void wave_atomic_count(int *addr, bool pred) {
+    unsigned long long m = __ballot(pred);
+    if (__lane_id() == 0 && m) atomicAdd(addr, (int)__popcll(m));
+}
+
 __global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb, float *vsnap, int *istat,
                            float dt, int box_mode) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    f3 v = ld3(b.v, i);
-    if (b.invM[i] > 0.0f) {  // k_grav (contact_engine_gpu.py:41-44)
-        v = v + mk(0.0f, 0.0f, -9.81f) * dt;
-        st3(b.v, i, v);
+    float mext = 0.0f;
+    bool has = false;
+    if (i < N) {
+        f3 v = ld3(b.v, i);
+        if (b.invM[i] > 0.0f) {  // k_grav (contact_engine_gpu.py:41-44)
+            v = v + mk(0.0f, 0.0f, -9.81f) * dt;
+            st3(b.v, i, v);
+        }
+        st3(vsnap, i, v);
+        m3 R = quat_to_matrix(ld4(b.q, i));  // k_invIw (contact_engine_gpu.py:47-53)
+        stm(Rw, i, R);
+        if (b.kin[i] == 1) {
+            m3 z;
+    #pragma unroll
+            for (int k = 0; k < 9; ++k) z.m[k / 3][k % 3] = 0.0f;
+            stm(invIw, i, z);
+        } else {
+            m3 Ib = ldm(b.IbInv, i);
+            m3 t = mm(R, Ib);
+            m3 r = mm(t, transpose(R));
+            stm(invIw, i, r);
+        }
+        if (box_mode) {
+            f3 h = ld3(b.half, i), c = ld3(b.x, i);
+            float e0 = fabsf(R.m[0][0]) * h.x + fabsf(R.m[0][1]) * h.y + fabsf(R.m[0][2]) * h.z;
+            float e1 = fabsf(R.m[1][0]) * h.x + fabsf(R.m[1][1]) * h.y + fabsf(R.m[1][2]) * h.z;
+            float e2 = fabsf(R.m[2][0]) * h.x + fabsf(R.m[2][1]) * h.y + fabsf(R.m[2][2]) * h.z;
+            aabb[6 * i + 0] = c.x - e0; aabb[6 * i + 1] = c.y - e1; aabb[6 * i + 2] = c.z - e2;
+            aabb[6 * i + 3] = c.x + e0; aabb[6 * i + 4] = c.y + e1; aabb[6 * i + 5] = c.z + e2;
+            // largest AABB half extent actually stored (hi - centre), as the oracle measures it
+            mext = fmaxf(fmaxf((c.x + e0) - c.x, (c.y + e1) - c.y), (c.z + e2) - c.z);
+            has = true;
+        }
     }
-    st3(vsnap, i, v);
-    m3 R = quat_to_matrix(ld4(b.q, i));  // k_invIw (contact_engine_gpu.py:47-53)
-    stm(Rw, i, R);
-    if (b.kin[i] == 1) {
-        m3 z;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) z.m[k / 3][k % 3] = 0.0f;
-        stm(invIw, i, z);
-    } else {
-        m3 Ib = ldm(b.IbInv, i);
-        m3 t = mm(R, Ib);
-        m3 r = mm(t, transpose(R));
-        stm(invIw, i, r);
-    }
-    if (box_mode) {
-        f3 h = ld3(b.half, i), c = ld3(b.x, i);
-        float e0 = fabsf(R.m[0][0]) * h.x + fabsf(R.m[0][1]) * h.y + fabsf(R.m[0][2]) * h.z;
-        float e1 = fabsf(R.m[1][0]) * h.x + fabsf(R.m[1][1]) * h.y + fabsf(R.m[1][2]) * h.z;
-        float e2 = fabsf(R.m[2][0]) * h.x + fabsf(R.m[2][1]) * h.y + fabsf(R.m[2][2]) * h.z;
-        aabb[6 * i + 0] = c.x - e0; aabb[6 * i + 1] = c.y - e1; aabb[6 * i + 2] = c.z - e2;
-        aabb[6 * i + 3] = c.x + e0; aabb[6 * i + 4] = c.y + e1; aabb[6 * i + 5] = c.z + e2;
-        // largest AABB half extent actually stored (hi - centre), as the oracle measures it
-        float m = fmaxf(fmaxf((c.x + e0) - c.x, (c.y + e1) - c.y), (c.z + e2) - c.z);
-        atomicMax(&istat[IS_MAXEXT], __float_as_int(m));
-    }
+    // non-negative floats order as their bit patterns
+    wave_atomic_max(&istat[IS_MAXEXT], __float_as_int(mext), has);
 }
 
 // cell size of the body grid: (2 * max half extent + margin + slack) * 1.001, so that any two
@@ -732,8 +754,8 @@ __global__ void k_cc_compare(int N, int *istat, const int *pbi, const int *pbj, 
         if (i == 0) istat[IS_MISMATCH] = 1;
         return;
     }
-    if (i < P && (pbi[i] != ca[i] || pbj[i] != cb[i])) atomicMax(&istat[IS_MISMATCH], 1);
-    if (i < N && invM[i] != cinvM[i]) atomicMax(&istat[IS_MISMATCH], 1);
+    bool diff = (i < P && (pbi[i] != ca[i] || pbj[i] != cb[i])) || (i < N && invM[i] != cinvM[i]);
+    if (__ballot(diff) && __lane_id() == 0) atomicMax(&istat[IS_MISMATCH], 1);
 }
 
 // PCG hash (Jarzynski & Olano 2020), behind rand_init(seed, offset) / randi

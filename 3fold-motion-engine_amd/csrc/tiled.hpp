@@ -268,13 +268,14 @@ __global__ void k_t_scatter_p(int capP, const int *istat, TiledBuffers T) {
 
 __global__ void k_t_tiles(int N, int TB, int *istat, TiledBuffers T) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || T.cnt_b[i] == 0) return;
+    bool root = i < N && T.cnt_b[i] != 0;
+    wave_atomic_count(&istat[IS_ISL], root);
+    if (!root) return;
     int t = T.boff[i] / TB;
     atomicMin(&T.tb0[t], T.boff[i]);
     atomicMax(&T.tb1[t], T.boff[i] + T.cnt_b[i]);
     atomicMin(&T.tp0[t], T.poff[i]);
     atomicMax(&T.tp1[t], T.poff[i] + T.cnt_p[i]);
-    atomicAdd(&istat[IS_ISL], 1);
 }
 
 // JP rounds of one tile (k_claim / k_win of contact_engine_gpu_colored.py:151-185, restricted to the
@@ -284,7 +285,7 @@ __global__ void k_t_tiles(int N, int TB, int *istat, TiledBuffers T) {
 // instructions); otherwise the global claim array and the pair colours.
 template <int WG, bool LDSM>
 __device__ __forceinline__ void t_color_tile(const TiledArgs &A, const TiledBuffers &T, int b0, int nb, int p0, int np,
-                                             unsigned long long *claim, int *colL, int &left) {
+                                             unsigned long long *claim, int *colL, int &left, int *hist) {
     constexpr bool lds = LDSM;
     const float *invM = A.b.invM;
     int *gcol = A.Pp.col;
@@ -328,32 +329,44 @@ __device__ __forceinline__ void t_color_tile(const TiledArgs &A, const TiledBuff
         __syncthreads();
         if (l == 0) { nround = cur + 1; break; }
     }
+    // class sizes: a histogram in LDS, then one global add per class and tile
+    for (int c = threadIdx.x; c < nround && c < ME_NCOLMAX; c += WG) hist[c] = 0;
+    __syncthreads();
     bool ovf = false;
     for (int k = threadIdx.x; k < np; k += WG) {
         int p = T.psorted[p0 + k];
         int c = getc(k, p);
         if (lds) gcol[p] = c;
-        if (c >= 0 && c < ME_NCOLMAX) atomicAdd(&T.ccount_jp[c], 1);
+        if (c >= 0 && c < ME_NCOLMAX) atomicAdd(&hist[c], 1);
         else ovf = true;
     }
+    __syncthreads();
+    for (int c = threadIdx.x; c < nround && c < ME_NCOLMAX; c += WG)
+        if (hist[c]) atomicAdd(&T.ccount_jp[c], hist[c]);
     if (ovf) atomicOr(&A.istat[IS_OVF], OVF_COLORS);
     if (threadIdx.x == 0) atomicMax(&A.istat[IS_NCOLJP], nround);
 }
 
 template <int WG>
+// the big tiles (k_t_biglist), taken in turn by the launch's workgroups; small tiles go to
+// k_t_color_small, many workgroups per CU instead of one 128 KB-LDS workgroup per tile
 __global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T) {
     if (A.istat[IS_MISMATCH] != 1) return;
-    int t = blockIdx.x;
-    int b0 = T.tb0[t];
-    if (b0 == INT_MAX) return;
-    int nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - T.tp0[t];
-    if (np <= 0) return;
     extern __shared__ unsigned long long cdsm[];
-    __shared__ int left;
-    if (nb <= T_COLB && np <= T_COLP)
-        t_color_tile<WG, true>(A, T, b0, nb, p0, np, cdsm, reinterpret_cast<int *>(cdsm + T_COLB), left);
-    else
-        t_color_tile<WG, false>(A, T, b0, nb, p0, np, A.claim, nullptr, left);
+    __shared__ int left, hist[ME_NCOLMAX];
+    const int nbig = A.istat[IS_NBIG];
+    for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
+        int t = T.biglist[i];
+        int b0 = T.tb0[t];
+        int nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - T.tp0[t];
+        if (np > 0) {
+            if (nb <= T_COLB && np <= T_COLP)
+                t_color_tile<WG, true>(A, T, b0, nb, p0, np, cdsm, reinterpret_cast<int *>(cdsm + T_COLB), left, hist);
+            else
+                t_color_tile<WG, false>(A, T, b0, nb, p0, np, A.claim, nullptr, left, hist);
+        }
+        __syncthreads();
+    }
 }
 
 // split of the JP classes (same rule as k_color_split_plan)
@@ -815,6 +828,63 @@ inline int tiled_scan(const TiledArgs &A, const int *in, int *out, int n, int *t
 
 inline int tblk(long long n) { return (int)((n + ME_THREADS - 1) / ME_THREADS); }
 
+// JP rounds of one small tile (the rounds of t_color_tile): one pair per thread, its key and body
+// slots kept in registers across the rounds, claims in static LDS
+__global__ void __launch_bounds__(T_SMALL_P) k_t_color_small(TiledArgs A, TiledBuffers T) {
+    if (A.istat[IS_MISMATCH] != 1) return;
+    int t = blockIdx.x;
+    int b0 = T.tb0[t];
+    if (b0 == INT_MAX) return;
+    int nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - T.tp0[t];
+    if (np <= 0 || !is_small_tile(nb, np)) return;
+    __shared__ unsigned long long sclaim[T_SMALL_B];
+    __shared__ int left;
+    const int tid = threadIdx.x;
+    const float *invM = A.b.invM;
+    int p = -1, la = -1, lb = -1, col = -1;
+    unsigned long long key = 0;
+    if (tid < np) {
+        p = T.psorted[p0 + tid];
+        key = (unsigned long long)jp_key(p);
+        int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
+        if (invM[a] > 0.0f) la = T.bslot[a] - b0;
+        if (b >= 0 && invM[b] > 0.0f) lb = T.bslot[b] - b0;
+    }
+    int nround = 0;
+    for (int cur = 0; cur <= np; ++cur) {
+        for (int j = tid; j < nb; j += T_SMALL_P) sclaim[j] = ~0ull;
+        if (tid == 0) left = 0;
+        __syncthreads();
+        if (p >= 0 && col < 0) {
+            if (la >= 0) atomicMin(&sclaim[la], key);
+            if (lb >= 0) atomicMin(&sclaim[lb], key);
+        }
+        __syncthreads();
+        if (p >= 0 && col < 0) {
+            bool ok = (la < 0 || sclaim[la] == key) && (lb < 0 || sclaim[lb] == key);
+            if (ok) col = cur;
+            else atomicAdd(&left, 1);
+        }
+        __syncthreads();
+        int l = left;
+        __syncthreads();
+        if (l == 0) { nround = cur + 1; break; }
+    }
+    // class sizes: a histogram in LDS, then one global add per class and tile
+    __shared__ int hist[ME_NCOLMAX];
+    for (int c = tid; c < nround && c < ME_NCOLMAX; c += T_SMALL_P) hist[c] = 0;
+    __syncthreads();
+    if (p >= 0) {
+        A.Pp.col[p] = col;
+        if (col >= 0 && col < ME_NCOLMAX) atomicAdd(&hist[col], 1);
+        else atomicOr(&A.istat[IS_OVF], OVF_COLORS);
+    }
+    __syncthreads();
+    for (int c = tid; c < nround && c < ME_NCOLMAX; c += T_SMALL_P)
+        if (hist[c]) atomicAdd(&T.ccount_jp[c], hist[c]);
+    if (tid == 0) atomicMax(&A.istat[IS_NCOLJP], nround);
+}
+
 // list of the tiles that exceed one small tile (the big-tile kernel's work)
 __global__ void k_t_biglist(int ntile, int *istat, TiledBuffers T) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -898,7 +968,8 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
     hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
     hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile)), dim3(ME_THREADS), 0, s, ntile, A.istat, T);
-    hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile), dim3(1024), T_COL_DYN, s, A, T);
+    hipLaunchKernelGGL(k_t_color_small, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
+    hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(1024), T_COL_DYN, s, A, T);
     hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
     hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);
     return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;
