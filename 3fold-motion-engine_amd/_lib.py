@@ -61,6 +61,7 @@ def lib():
         L.me_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.me_tape_size.argtypes = [vp, C.POINTER(C.c_size_t)]
         L.me_adopt.argtypes = [vp]
+        L.me_state_size.argtypes = [vp, C.POINTER(C.c_size_t)]
         L.me_step_taped.argtypes = [vp, C.c_double, vp, C.c_size_t, vp]
         L.me_adjoint_substep.argtypes = [vp, C.c_double, vp, C.c_size_t] + [vp] * 7 + [vp]
         L.me_last_error.restype = C.c_char_p
@@ -82,5 +83,5 @@ def check(rc: int, what: str) -> None:
 
 EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
             "me_buffer", "me_read_counters", "me_profile", "me_profile_read", "me_last_error",
-            "me_tape_size", "me_step_taped", "me_adjoint_substep", "me_adopt")
+            "me_tape_size", "me_step_taped", "me_adjoint_substep", "me_adopt", "me_state_size")
 K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL, K_TILE_BIG = range(6)

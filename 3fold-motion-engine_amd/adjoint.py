@@ -63,8 +63,11 @@ class Rollout:
         self.eng._build()
         e = self.eng
         self.N, self.capC = N, None
-        self._init = self._snap()
         L = _lib.lib()
+        sb = C.c_size_t()
+        _lib.check(L.me_state_size(e._h, C.byref(sb)), "me_state_size")
+        self._state = e._ws[e._ws_off:e._ws_off + int(sb.value)]  # the workspace's state block
+        self._init = self._snap()
         nb = C.c_size_t()
         _lib.check(L.me_tape_size(e._h, C.byref(nb)), "me_tape_size")
         self.tape_bytes = int(nb.value)
@@ -77,14 +80,15 @@ class Rollout:
         self._ckpt = []
         self.substeps_total = self.T * self.S
 
-    # engine state: body tensors + workspace (warm store, colour cache and counters live there)
+    # engine state: body tensors + the workspace's state block (status words, warm store, colour
+    # cache, and what the reverse of a recorded substep reads); the rest of the workspace is scratch
     def _snap(self):
         e = self.eng
-        return tuple(t.clone() for t in (e.x, e.q, e.v, e.w, e._ws))
+        return tuple(t.clone() for t in (e.x, e.q, e.v, e.w, self._state))
 
     def _restore(self, s):
         e = self.eng
-        for dst, src in zip((e.x, e.q, e.v, e.w, e._ws), s):
+        for dst, src in zip((e.x, e.q, e.v, e.w, self._state), s):
             dst.copy_(src)
 
     def _check(self):

@@ -93,6 +93,7 @@ struct me_engine {
     int nbsum;
     int *istat;
     int *hstat;  // pinned host mirror
+    size_t state_bytes = 0;  // leading workspace bytes that carry state between substeps
     hipStream_t side = nullptr;  // second stream for the big-tile sweeps (default priority)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // whole-step graphs (ME_FLAG_GRAPH): captured and launched on gs, keyed by (dt, substeps)
@@ -146,43 +147,24 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->cap_cand = c.max_pairs > 0 ? c.max_pairs : (P > 0 ? 1 : 16 * N + 1024);
     e->capC = c.max_contacts > 0 ? c.max_contacts : (P > 0 ? 400000 : 4 * e->cap_cand);
     e->capP = (P > 0 || (c.flags & ME_FLAG_PER_CONTACT)) ? e->capC : e->cap_cand;
-    Bump B{base};
-    e->invIw = B.take<float>(9 * (size_t)N);
-    e->Rw = B.take<float>(9 * (size_t)N);
-    e->aabb = B.take<float>(6 * (size_t)N);
-    e->pv = B.take<float>(3 * (size_t)N);
-    e->po = B.take<float>(3 * (size_t)N);
-    e->vsnap = B.take<float>(3 * (size_t)N);
-    e->allp = B.take<float>(3 * (size_t)(P > 0 ? NP : 1));
-    e->acc = B.take<int64_t>(6 * (size_t)N);
-    e->claim = B.take<unsigned long long>((size_t)N);
-    e->cellc = B.take<int>(4 * (size_t)items);
-    e->bucket = B.take<int>((size_t)items);
-    e->bcount = B.take<int>((size_t)e->T);
-    e->bstart = B.take<int>((size_t)e->T);
-    e->bcursor = B.take<int>((size_t)e->T);
-    e->items = B.take<int>((size_t)items);
     long long scan_n = items > e->cap_cand ? items : e->cap_cand;
     if (e->T > scan_n) scan_n = e->T;
     if (e->capC > scan_n) scan_n = e->capC;
-    e->cand_n = B.take<int>((size_t)items);
-    e->cand_off = B.take<int>((size_t)items);
-    e->cand_tmp = B.take<int>(P > 0 ? 1 : (size_t)N * ME_MAXCAND);
-    e->cc_n = B.take<int>(P > 0 ? 1 : 27 * (size_t)N);
-    e->cc_list = B.take<int>(P > 0 ? 1 : 27 * (size_t)ME_CELLCAND * N);
+    size_t C = (size_t)e->capC, Pc = (size_t)e->capP;
+    size_t ns = P > 0 ? 1 : 4 * (size_t)e->cap_cand;
+    Bump B{base};
+    // ---- state: what a substep leaves for the next one (status words, warm store, colour cache)
+    // and what the adjoint of a recorded substep reads (contacts, pairs, candidates, tiles,
+    // per-contact constants, world inertia, pseudo-velocities); a snapshot of these leading bytes
+    // plus the body arrays restores the engine exactly ----
+    e->istat = B.take<int>(IS_COUNT);
+    e->invIw = B.take<float>(9 * (size_t)N);
+    e->pv = B.take<float>(3 * (size_t)N);
+    e->po = B.take<float>(3 * (size_t)N);
     e->cand_a = B.take<int>((size_t)e->cap_cand);
     e->cand_b = B.take<int>((size_t)e->cap_cand);
-    size_t ns = P > 0 ? 1 : 4 * (size_t)e->cap_cand;
     e->S.n = B.take<int>((size_t)e->cap_cand);
-    e->S.key = B.take<int64_t>(ns);
-    e->S.p = B.take<float>(3 * ns);
-    e->S.nrm = B.take<float>(3 * ns);
-    e->S.pen = B.take<float>(ns);
-    e->S.mu = B.take<float>(ns);
     e->slot_off = B.take<int>((size_t)e->cap_cand);
-    e->pair_flag = B.take<int>((size_t)e->cap_cand);
-    e->pair_off = B.take<int>((size_t)e->cap_cand);
-    size_t C = (size_t)e->capC, Pc = (size_t)e->capP;
     e->Cc.key = B.take<int64_t>(C);
     e->Cc.bi = B.take<int>(C);
     e->Cc.bj = B.take<int>(C);
@@ -214,10 +196,36 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->wjn = B.take<float>(C);
     e->wjt1 = B.take<float>(C);
     e->wjt2 = B.take<float>(C);
-    tiled_layout(e->tb, B, N, (int)Pc, (int)C);
+    tiled_layout_state(e->tb, B, N, (int)Pc, (int)C);
+    e->state_bytes = (B.off + 255) & ~size_t(255);
+    // ---- scratch: rewritten by every substep before it is read ----
+    e->Rw = B.take<float>(9 * (size_t)N);
+    e->aabb = B.take<float>(6 * (size_t)N);
+    e->vsnap = B.take<float>(3 * (size_t)N);
+    e->allp = B.take<float>(3 * (size_t)(P > 0 ? NP : 1));
+    e->acc = B.take<int64_t>(6 * (size_t)N);
+    e->claim = B.take<unsigned long long>((size_t)N);
+    e->cellc = B.take<int>(4 * (size_t)items);
+    e->bucket = B.take<int>((size_t)items);
+    e->bcount = B.take<int>((size_t)e->T);
+    e->bstart = B.take<int>((size_t)e->T);
+    e->bcursor = B.take<int>((size_t)e->T);
+    e->items = B.take<int>((size_t)items);
+    e->cand_n = B.take<int>((size_t)items);
+    e->cand_off = B.take<int>((size_t)items);
+    e->cand_tmp = B.take<int>(P > 0 ? 1 : (size_t)N * ME_MAXCAND);
+    e->cc_n = B.take<int>(P > 0 ? 1 : 27 * (size_t)N);
+    e->cc_list = B.take<int>(P > 0 ? 1 : 27 * (size_t)ME_CELLCAND * N);
+    e->S.key = B.take<int64_t>(ns);
+    e->S.p = B.take<float>(3 * ns);
+    e->S.nrm = B.take<float>(3 * ns);
+    e->S.pen = B.take<float>(ns);
+    e->S.mu = B.take<float>(ns);
+    e->pair_flag = B.take<int>((size_t)e->cap_cand);
+    e->pair_off = B.take<int>((size_t)e->cap_cand);
+    tiled_layout_scratch(e->tb, B, N, (int)Pc, (int)C);
     e->nbsum = (int)((scan_n + SCAN_BLOCK - 1) / SCAN_BLOCK) + 1;
     e->bsum = B.take<int>((size_t)e->nbsum);
-    e->istat = B.take<int>(IS_COUNT);
     *bytes = (B.off + 255) & ~size_t(255);
 }
 
@@ -812,6 +820,12 @@ int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launche
     }
     *total_ms = tot;
     *launches = n;
+    return ME_OK;
+}
+
+int me_state_size(me_engine *e, size_t *bytes) {
+    if (!e || !bytes) return fail(ME_EARG, "NULL argument");
+    *bytes = e->state_bytes;
     return ME_OK;
 }
 

@@ -80,8 +80,23 @@ struct TiledArgs {
     AdjTape tape;
 };
 
+// tile arrays the adjoint of a recorded substep reads (bounds, body and pair order, constants)
 template <class BumpT>
-inline void tiled_layout(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
+inline void tiled_layout_state(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
+    size_t n = (size_t)N, p = (size_t)capP;
+    t.bsorted = B.template take<int>(n);
+    t.bslot = B.template take<int>(n);
+    t.psorted = B.template take<int>(p);
+    t.tb0 = B.template take<int>(n);
+    t.tb1 = B.template take<int>(n);
+    t.tp0 = B.template take<int>(n);
+    t.tp1 = B.template take<int>(n);
+    t.prep = B.template take<float>(20 * (size_t)capC);
+}
+
+// tile arrays rebuilt within every substep
+template <class BumpT>
+inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
     size_t n = (size_t)N, p = (size_t)capP;
     t.parent = B.template take<int>(n);
     t.root = B.template take<int>(n);
@@ -92,13 +107,6 @@ inline void tiled_layout(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
     t.fill_p = B.template take<int>(n);
     t.boff = B.template take<int>(n);
     t.poff = B.template take<int>(n);
-    t.bsorted = B.template take<int>(n);
-    t.bslot = B.template take<int>(n);
-    t.psorted = B.template take<int>(p);
-    t.tb0 = B.template take<int>(n);
-    t.tb1 = B.template take<int>(n);
-    t.tp0 = B.template take<int>(n);
-    t.tp1 = B.template take<int>(n);
     t.ccount_jp = B.template take<int>(ME_NCOLMAX + 1);
     t.csub = B.template take<int>(ME_NCOLMAX + 1);
     t.cbase = B.template take<int>(ME_NCOLMAX + 2);
@@ -107,7 +115,6 @@ inline void tiled_layout(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
     t.invIws = B.template take<float>(12 * n);
     t.psorted2 = B.template take<int>(p);
     t.rec = B.template take<int>(8 * p);
-    t.prep = B.template take<float>(20 * (size_t)capC);
     t.sprep = B.template take<float>(96 * p);
     t.srec = B.template take<int>(4 * p);
     t.dbg = B.template take<long long>(16);

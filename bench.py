@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large", "adjoint"])
     ap.add_argument("--rollouts", type=int, default=3, help="adjoint workload: timed forward+backward rollouts")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="pile workload: also measure the 256-scene batch and the config-4 adjoint and attach "
+                         "their results to the JSON line")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--seq-rounds", type=int, default=3)
     ap.add_argument("--autotune", action="store_true")
@@ -89,9 +92,11 @@ def main():
         sys.exit(2)
 
     if a.workload == "batched":
-        return bench_batched(a)
+        print(json.dumps(bench_batched(a)), flush=True)
+        return
     if a.workload == "adjoint":
-        return bench_adjoint(a)
+        print(json.dumps(bench_adjoint(a)), flush=True)
+        return
     if a.workload == "large":
         a.bodies = 100000 if a.bodies == 10000 else a.bodies
         a.tiled = 1
@@ -219,9 +224,28 @@ def main():
     traffic = _traffic_from_profile(roofline["kernel"] if roofline else None, out["config"]["workload"])
     if roofline and traffic:
         roofline["traffic"] = traffic
+    if a.extras and a.workload == "pile" and a.bodies == 10000:
+        del e
+        torch.cuda.empty_cache()
+        out["batched_256"] = _extra(bench_batched, a, warmup=100, steps=100, seq_rounds=1, scenes=256,
+                                    keys=("value", "ms_per_step", "speedup_vs_sequential", "sequential_plain_launches",
+                                          "config"))
+        torch.cuda.empty_cache()
+        out["adjoint"] = _extra(bench_adjoint, a, rollouts=2,
+                                keys=("value", "ms_per_step", "adjoint", "config"))
     print(json.dumps(out), flush=True)
     if a.calibrate_copy:
         _copy_kernel(a.calibrate_copy)
+
+
+def _extra(fn, a, keys, **over):
+    """Run another workload with its own settings and keep the named fields of its result."""
+    b = argparse.Namespace(**vars(a))
+    for k, v in over.items():
+        setattr(b, k, v)
+    print(f"[bench] extra workload {fn.__name__}", file=sys.stderr, flush=True)
+    r = fn(b)
+    return {k: r[k] for k in keys if k in r}
 
 
 def _copy_kernel(mib):
@@ -298,7 +322,7 @@ def bench_adjoint(a):
                        "adjoint_ms_per_step": round(ms_per_step, 4), "rollouts_timed": len(times),
                        "grad_norms_v0_w0_mu": gnorm},
            "roofline": None, "cpu_baseline": None}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_batched(a):
@@ -374,7 +398,7 @@ def bench_batched(a):
     if tune is not None:
         out["autotune"] = {"colours": tune.colours, "substeps": tune.substeps, "vit": tune.vit, "pit": tune.pit,
                            "max_pen": tune.max_pen}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -183,6 +183,11 @@ int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launche
  * me_step_taped, then me_adjoint_substep turns the adjoint of the state after the substep into the
  * adjoint of the state before it. */
 
+/* Leading bytes of the workspace that carry the engine's state from one substep to the next and
+ * what me_adjoint_substep reads of a recorded substep; the rest is scratch every substep rewrites
+ * before reading. A copy of these bytes plus the body arrays is a complete checkpoint. */
+int me_state_size(me_engine *e, size_t *bytes);
+
 /* Bytes of the caller-owned tape block (256-byte aligned device memory) for this engine. */
 int me_tape_size(me_engine *e, size_t *bytes);
 
