@@ -130,9 +130,13 @@ struct or_world {
     int64_t *key;
     int *cbi, *cbj;
     real *cpA, *cpB, *cn, *cpen, *cmu, *jn, *jt1, *jt2, *jp;
-    /* pairs */
+    int Craw;  /* contacts generated, before a manifold reduction */
+    /* bead manifold stage: float64 warm normal impulse, redistributed one, keep flags, pair order */
+    double *jnd, *jnr;
+    int *keep, *gord;
+    /* pairs: the k-th contact of pair p is pidx[pstart[p] + k] */
     int Pp, pcap;
-    int *pbi, *pbj, *pstart, *pcount, *col, *porder, *cstart, *ccount;
+    int *pbi, *pbj, *pstart, *pcount, *col, *porder, *cstart, *ccount, *pidx;
     int ncol;
     /* warm store */
     int wn;
@@ -171,6 +175,8 @@ or_world *or_create(int N, int P, const or_params *prm) {
     w->pcap = cap;
     w->pbi = xc(cap, 4); w->pbj = xc(cap, 4); w->pstart = xc(cap, 4); w->pcount = xc(cap, 4);
     w->col = xc(cap, 4); w->porder = xc(cap, 4); w->cstart = xc(cap + 1, 4); w->ccount = xc(cap + 1, 4);
+    w->pidx = xc(cap, 4); w->keep = xc(cap, 4); w->gord = xc(cap, 4);
+    w->jnd = xc(cap, 8); w->jnr = xc(cap, 8);
     w->wkey = xc(cap, 8); w->wjn = xc(cap, sizeof(real)); w->wjt1 = xc(cap, sizeof(real)); w->wjt2 = xc(cap, sizeof(real));
     w->cache_a = xc(cap, 4); w->cache_b = xc(cap, 4); w->cache_col = xc(cap, 4);
     w->cache_invM = xc(N, sizeof(real));
@@ -189,7 +195,8 @@ void or_destroy(or_world *w) {
                     w->force, w->key, w->cbi, w->cbj, w->cpA, w->cpB, w->cn, w->cpen, w->cmu,
                     w->jn, w->jt1, w->jt2, w->jp, w->pbi, w->pbj, w->pstart, w->pcount, w->col,
                     w->porder, w->cstart, w->ccount, w->wkey, w->wjn, w->wjt1, w->wjt2,
-                    w->cache_a, w->cache_b, w->cache_col, w->cache_invM, w->acc, w->tmpk, w->tmpi};
+                    w->cache_a, w->cache_b, w->cache_col, w->cache_invM, w->acc, w->tmpk, w->tmpi,
+                    w->pidx, w->keep, w->gord, w->jnd, w->jnr};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(w);
 }
@@ -811,21 +818,36 @@ static void jac_apply(or_world *w, real *vv, real *ww, real relax, real scale) {
 }
 
 /* ───────────────────────── pairs, warm start, colouring ───────────────────────── */
-/* A16 pair build (contact_engine_gpu_colored.py:342-389): contacts are key-sorted, so a pair is a
- * contiguous run of equal (bi, bj). */
+/* A16 pair build (contact_engine_gpu_colored.py:342-389; device form :1138-1157): a stable sort of the
+ * contacts by the pair key a*(N+1) + b + 1, then one pair per run of equal keys. Box contacts are
+ * generated pair-major, so the sort is the identity there; bead contacts are in feature-key order. */
 static void build_pairs(or_world *w) {
     w->Pp = 0;
     if (w->prm.per_contact) { /* contact_engine_gpu_colored.py:1105-1136: the contact graph */
         for (int c = 0; c < w->C; ++c) {
-            w->pstart[c] = c; w->pcount[c] = 1; w->pbi[c] = w->cbi[c]; w->pbj[c] = w->cbj[c];
+            w->pstart[c] = c; w->pcount[c] = 1; w->pbi[c] = w->cbi[c]; w->pbj[c] = w->cbj[c]; w->pidx[c] = c;
         }
         w->Pp = w->C;
         return;
     }
-    for (int c = 0; c < w->C; ++c) {
-        if (c == 0 || w->cbi[c] != w->cbi[c - 1] || w->cbj[c] != w->cbj[c - 1]) {
+    int C = w->C;
+    if (w->P > 0) {
+        cellrec *r = (cellrec *)malloc(sizeof(cellrec) * (C > 0 ? (size_t)C : (size_t)1));
+        for (int c = 0; c < C; ++c) {
+            r[c].cell = (int64_t)w->cbi[c] * (int64_t)(w->N + 1) + (int64_t)w->cbj[c] + 1;
+            r[c].idx = c;
+        }
+        qsort(r, C, sizeof(cellrec), cmp_cell); /* (key, index): the stable order */
+        for (int c = 0; c < C; ++c) w->pidx[c] = r[c].idx;
+        free(r);
+    } else {
+        for (int c = 0; c < C; ++c) w->pidx[c] = c;
+    }
+    for (int t = 0; t < C; ++t) {
+        int c = w->pidx[t];
+        if (t == 0 || w->cbi[c] != w->cbi[w->pidx[t - 1]] || w->cbj[c] != w->cbj[w->pidx[t - 1]]) {
             int p = w->Pp++;
-            w->pstart[p] = c; w->pcount[p] = 0; w->pbi[p] = w->cbi[c]; w->pbj[p] = w->cbj[c];
+            w->pstart[p] = t; w->pcount[p] = 0; w->pbi[p] = w->cbi[c]; w->pbj[p] = w->cbj[c];
         }
         w->pcount[w->Pp - 1]++;
     }
@@ -857,6 +879,148 @@ static void warm_store(or_world *w) {
     memcpy(w->wjt2, w->jt2, sizeof(real) * (size_t)w->C);
 }
 
+/* ── A15 manifold reduction of bead contacts, the device stage (contact_engine_gpu_colored.py:
+ * 1050-1087 with the kernels k_fkey / k_warm / k_pkey_pair / k_runflag / k_select / k_compact at
+ * :587-740): warm start of every generated contact by binary search of its feature key in the
+ * previous step's kept keys (normal impulse promoted to float64), grouping by body pair (stable sort
+ * of the pair key), and per pair with more than mx contacts the E-optimal choice of mx of them; the
+ * normal impulse of every contact of the pair goes to the kept point nearest to it in the contact
+ * plane, summed in float64. The kept contacts stay in feature-key order. */
+
+/* planar coordinates about ref in the (u, w) basis: float32 difference, then float64
+ * (contact_engine_gpu_colored.py:585-590) */
+static inline void planar2(v3 p, v3 ref, const double u[3], const double wv[3], double r[2]) {
+    real dx = p.x - ref.x, dy = p.y - ref.y, dz = p.z - ref.z;
+    double ax = (double)dx, ay = (double)dy, az = (double)dz;
+    r[0] = ax * u[0] + ay * u[1] + az * u[2];
+    r[1] = ax * wv[0] + ay * wv[1] + az * wv[2];
+}
+
+/* k_select for one pair (contact_engine_gpu_colored.py:647-720); g[0..L) = the pair's contacts in
+ * feature order. Returns 0, or 1 when the segment exceeds the kernel's fixed sizes. */
+static int select_segment(or_world *w, const int *g, int L, int mx) {
+    if (L <= mx) {
+        for (int t = 0; t < L; ++t) { w->keep[g[t]] = 1; w->jnr[g[t]] = w->jnd[g[t]]; }
+        return 0;
+    }
+    if (L > 1024 || mx > 8) {
+        for (int t = 0; t < L; ++t) { w->keep[g[t]] = 1; w->jnr[g[t]] = w->jnd[g[t]]; }
+        return 1;
+    }
+    int deep = 0;
+    real dbest = w->cpen[g[0]];
+    for (int t = 1; t < L; ++t)
+#ifdef OR_PLANT_DEEP_LAST /* planted mistake for a test: last index wins ties */
+        if (w->cpen[g[t]] >= dbest) { dbest = w->cpen[g[t]]; deep = t; }
+#else
+        if (w->cpen[g[t]] > dbest) { dbest = w->cpen[g[t]]; deep = t; }
+#endif
+    int gd = g[deep];
+    v3 nr = ld3(w->cn, gd);
+    double ax = 1.0, ay = 0.0, az = 0.0;
+    if (RFABS(nr.x) >= 0.9f) { ax = 0.0; ay = 1.0; az = 0.0; }
+    double n0 = (double)nr.x, n1 = (double)nr.y, n2 = (double)nr.z;
+    double dn = ax * n0 + ay * n1 + az * n2;
+    double u0 = ax - dn * n0, u1 = ay - dn * n1, u2 = az - dn * n2;
+    double ln = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    double u[3] = {u0 / ln, u1 / ln, u2 / ln};
+    double wv[3] = {n1 * u[2] - n2 * u[1], n2 * u[0] - n0 * u[2], n0 * u[1] - n1 * u[0]};
+    v3 ref = ld3(w->x, w->cbi[gd]);
+    double rd[2];
+    planar2(ld3(w->cpA, gd), ref, u, wv, rd);
+    double S00 = rd[0] * rd[0], S01 = rd[0] * rd[1], S10 = rd[1] * rd[0], S11 = rd[1] * rd[1];
+    int pick[8];
+    pick[0] = deep;
+    for (int k = 1; k < mx; ++k) {
+        double sbest = -1.0e300;
+        int tbest = -1;
+        double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+        for (int t = 0; t < L; ++t) {
+            int taken = 0;
+            for (int q = 0; q < k; ++q)
+                if (pick[q] == t) taken = 1;
+            if (taken) continue;
+            double rt[2];
+            planar2(ld3(w->cpA, g[t]), ref, u, wv, rt);
+            double a00 = S00 + rt[0] * rt[0], a01 = S01 + rt[0] * rt[1];
+            double a10 = S10 + rt[1] * rt[0], a11 = S11 + rt[1] * rt[1];
+            double tr = a00 + a11;
+            double det = a00 * a11 - a01 * a10;
+            double disc = tr * tr / 4.0 - det;
+            if (!(disc > 0.0)) disc = 0.0; /* max(., 0) */
+            double lam = tr / 2.0 - sqrt(disc);
+            double kk = (double)(k + 1);
+            double sc = lam < kk ? lam : kk; /* min(k + 1, lambda_min) */
+            if (sc > sbest) { sbest = sc; tbest = t; c00 = a00; c01 = a01; c10 = a10; c11 = a11; }
+        }
+        if (tbest < 0) return 1;
+        pick[k] = tbest;
+        S00 = c00; S01 = c01; S10 = c10; S11 = c11;
+    }
+    for (int k = 0; k < mx; ++k) { w->keep[g[pick[k]]] = 1; w->jnr[g[pick[k]]] = 0.0; }
+    for (int t = 0; t < L; ++t) {
+        double rt[2];
+        planar2(ld3(w->cpA, g[t]), ref, u, wv, rt);
+        double dbest2 = 1.0e300;
+        int kbest = 0;
+        for (int k = 0; k < mx; ++k) {
+            double rk[2];
+            planar2(ld3(w->cpA, g[pick[k]]), ref, u, wv, rk);
+            double e0 = rt[0] - rk[0], e1 = rt[1] - rk[1];
+            double d2 = e0 * e0 + e1 * e1;
+            if (d2 < dbest2) { dbest2 = d2; kbest = k; }
+        }
+        int gk = g[pick[kbest]];
+        w->jnr[gk] = w->jnr[gk] + w->jnd[g[t]];
+    }
+    return 0;
+}
+
+static int bead_manifold(or_world *w) {
+    int C = w->C, mx = w->prm.manifold;
+    for (int c = 0; c < C; ++c) {
+        w->jnd[c] = 0.0; w->jt1[c] = 0.0f; w->jt2[c] = 0.0f; w->keep[c] = 0;
+        if (!w->prm.warm_start || w->wn == 0) continue;
+        int64_t k = w->key[c];
+        int lo = 0, hi = w->wn - 1, hit = -1;
+        while (lo <= hi) {
+            int mid = (lo + hi) / 2;
+            int64_t km = w->wkey[mid];
+            if (km == k) { hit = mid; lo = hi + 1; }
+            else if (km < k) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        if (hit >= 0) { w->jnd[c] = (double)w->wjn[hit]; w->jt1[c] = w->wjt1[hit]; w->jt2[c] = w->wjt2[hit]; }
+    }
+    cellrec *r = (cellrec *)malloc(sizeof(cellrec) * (C > 0 ? (size_t)C : (size_t)1));
+    for (int c = 0; c < C; ++c) {
+        r[c].cell = (int64_t)w->cbi[c] * (int64_t)(w->N + 1) + (int64_t)w->cbj[c] + 1;
+        r[c].idx = c;
+    }
+    qsort(r, C, sizeof(cellrec), cmp_cell);
+    for (int c = 0; c < C; ++c) w->gord[c] = r[c].idx;
+    int bad = 0;
+    for (int i = 0; i < C;) {
+        int L = 1;
+        while (i + L < C && r[i + L].cell == r[i].cell) ++L;
+        bad += select_segment(w, w->gord + i, L, mx);
+        i += L;
+    }
+    free(r);
+    if (bad) return 5;
+    int n = 0;
+    for (int c = 0; c < C; ++c) {
+        if (!w->keep[c]) continue;
+        w->key[n] = w->key[c]; w->cbi[n] = w->cbi[c]; w->cbj[n] = w->cbj[c];
+        st3(w->cpA, n, ld3(w->cpA, c)); st3(w->cpB, n, ld3(w->cpB, c)); st3(w->cn, n, ld3(w->cn, c));
+        w->cpen[n] = w->cpen[c]; w->cmu[n] = w->cmu[c];
+        w->jn[n] = (real)w->jnr[c]; w->jt1[n] = w->jt1[c]; w->jt2[n] = w->jt2[c]; w->jp[n] = 0.0f;
+        ++n;
+    }
+    w->C = n;
+    return 0;
+}
+
 /* PCG hash (Jarzynski & Olano 2020, "Hash Functions for GPU Rendering"), the generator behind the
  * reference's rand_init(seed, offset) / randi (contact_engine_gpu_colored.py:145). */
 uint32_t or_pcg(uint32_t s) {
@@ -864,9 +1028,14 @@ uint32_t or_pcg(uint32_t s) {
     uint32_t c = ((b >> ((b >> 28u) + 4u)) ^ b) * 277803737u;
     return (c >> 22u) ^ c;
 }
-/* k_prio (contact_engine_gpu_colored.py:137-148): |randi(rand_init(7919, c))| * 2^32 + c */
+/* k_prio (contact_engine_gpu_colored.py:137-148): |randi(rand_init(7919, c))| * 2^32 + c.
+ * OR_JP_SEED exists so that a test can build the oracle with a planted wrong seed and show that
+ * the reference's archived trajectories reject it. */
+#ifndef OR_JP_SEED
+#define OR_JP_SEED 7919
+#endif
 int64_t or_jp_key(int rank) {
-    uint32_t st = or_pcg((uint32_t)7919 + or_pcg((uint32_t)rank));
+    uint32_t st = or_pcg((uint32_t)OR_JP_SEED + or_pcg((uint32_t)rank));
     st = or_pcg(st);
     int64_t p = (int64_t)(int32_t)st;
     if (p < 0) p = -p;
@@ -983,7 +1152,7 @@ static void apply_warm_pair(or_world *w, int p) {
     m33 IA = ldm(w->invIw, a), IB;
     if (b >= 0) IB = ldm(w->invIw, b);
     for (int k = 0; k < w->pcount[p]; ++k) {
-        int c = w->pstart[p] + k;
+        int c = w->pidx[w->pstart[p] + k];
         v3 n = ld3(w->cn, c);
         v3 t1 = fixed_t1(n);
         v3 t2 = vcross(n, t1);
@@ -1013,7 +1182,7 @@ static void chunk_vel(or_world *w, int p, int fixed, real t2d) {
     v3 vB = V3(0.0f, 0.0f, 0.0f), wB = V3(0.0f, 0.0f, 0.0f);
     if (bj >= 0) { iMB = w->invM[bj]; IB = ldm(w->invIw, bj); vB = ld3(w->v, bj); wB = ld3(w->w, bj); }
     for (int k = 0; k < w->pcount[p]; ++k) {
-        int c = w->pstart[p] + k;
+        int c = w->pidx[w->pstart[p] + k];
         v3 nrm = ld3(w->cn, c), rA = vsub(ld3(w->cpA, c), ld3(w->x, bi));
         v3 va = vadd(vA, vcross(wA, rA));
         v3 vb = V3(0.0f, 0.0f, 0.0f), rB = V3(0.0f, 0.0f, 0.0f);
@@ -1085,7 +1254,7 @@ static void chunk_pos(or_world *w, int p, real dt) {
     v3 pvB = V3(0.0f, 0.0f, 0.0f), poB = V3(0.0f, 0.0f, 0.0f);
     if (bj >= 0) { iMB = w->invM[bj]; IB = ldm(w->invIw, bj); pvB = ld3(w->pv, bj); poB = ld3(w->po, bj); }
     for (int k = 0; k < w->pcount[p]; ++k) {
-        int c = w->pstart[p] + k;
+        int c = w->pidx[w->pstart[p] + k];
         v3 nrm = ld3(w->cn, c), rA = vsub(ld3(w->cpA, c), ld3(w->x, bi));
         v3 rel = vadd(pvA, vcross(poA, rA));
         v3 rB = V3(0.0f, 0.0f, 0.0f);
@@ -1131,6 +1300,7 @@ static int step_jacobi(or_world *w, double sdt64) {
     grav(w, sdt);
     inv_inertia_world(w);
     generate(w);
+    w->Craw = w->C;
     memcpy(w->vsnap, w->v, 3 * sizeof(real) * (size_t)w->N);
     int C = w->C;
     if (C > 0) {
@@ -1162,9 +1332,15 @@ static int step_colored(or_world *w, double sdt64, int colorcache) {
     generate(w);
     memcpy(w->vsnap, w->v, 3 * sizeof(real) * (size_t)w->N);
     int C = w->C;
+    w->Craw = C;
     memset(w->pv, 0, 3 * sizeof(real) * (size_t)w->N); memset(w->po, 0, 3 * sizeof(real) * (size_t)w->N);
     if (C > 0) {
-        warm_lookup(w);
+        if (w->P > 0 && w->prm.manifold > 0) {
+            int rc = bead_manifold(w);
+            if (rc) return rc;
+        } else {
+            warm_lookup(w);
+        }
         build_pairs(w);
         int nc = colorcache ? color_cached(w) : color_pairs(w);
         if (nc < 0) return 2;
@@ -1226,7 +1402,7 @@ void or_get_aux(const or_world *w, real *pv, real *po, real *invIw) {
 void or_contact_forces(const or_world *w, double *F) { memcpy(F, w->force, 24 * (size_t)w->N); }
 void or_counters(const or_world *w, int64_t *out) {
     out[0] = w->C; out[1] = w->Pp; out[2] = w->ncol; out[3] = w->hits; out[4] = w->misses;
-    out[5] = w->overflow;
+    out[5] = w->overflow; out[6] = w->Craw;
 }
 int or_get_contacts(const or_world *w, int64_t *key, int *bi, int *bj, real *pA, real *pB, real *n,
                     real *pen, real *jn, real *jt1, real *jt2, real *jp) {

@@ -16,9 +16,10 @@
  *     closed forms (resting box -> 4 corners at known depth; aligned stacked boxes -> the
  *     overlap rectangle), brute-force SAT on tiny inputs;
  *   - pair build, JP colouring: invariants (proper colouring, sortedness) + brute force;
- *   - coloured chunk solvers: physics checks (resting force = Mg, stick/slide at atan mu).
- *     The PCG hash behind the colouring priorities is "parity unpinned" (the reference's
- *     ColorCache archive needs the bead manifold reduction, not restated here).
+ *   - bead manifold reduction, pair build, JP colouring (PCG hash), ColorCache and pair-chunk
+ *     solvers: the same archive's pair_chunks and color_cache trajectories (stack8 x120,
+ *     lattice1000 x40); invariants (proper colouring, sortedness) + brute force;
+ *   - coloured chunk solvers on boxes: physics checks (resting force = Mg, stick/slide at atan mu).
  */
 #ifndef MOTION_ORACLE_H
 #define MOTION_ORACLE_H
@@ -57,6 +58,8 @@ typedef struct or_params {
     int cap;           /* contact capacity */
     int n_colors;      /* 0 = JP colouring as is; K > 0 = split into max(K, JP colours) classes */
     int per_contact;   /* coloured solver without pair chunks: every contact its own chunk */
+    int manifold;      /* bead geometry, coloured solvers: keep <= manifold points per body pair
+                        * (E-optimal reduction); 0 = no reduction */
 } or_params;
 
 typedef struct or_world or_world;
@@ -75,7 +78,8 @@ int or_step(or_world *w, double dt, int substeps);   /* 0 ok; 4 capacity; 2 colo
 void or_get_state(const or_world *w, real *x, real *q, real *v, real *om);
 void or_get_aux(const or_world *w, real *pv, real *po, real *invIw);
 void or_contact_forces(const or_world *w, double *F);
-/* counters: [n_contacts, n_pairs, n_colors, cache_hits, cache_misses, overflow] */
+/* counters: [n_contacts, n_pairs, n_colors, cache_hits, cache_misses, overflow, n_generated]:
+ * n_contacts = contacts solved (after a manifold reduction), n_generated = before it */
 void or_counters(const or_world *w, int64_t *out);
 int or_get_contacts(const or_world *w, int64_t *key, int *bi, int *bj, real *pA, real *pB,
                     real *n, real *pen, real *jn, real *jt1, real *jt2, real *jp);

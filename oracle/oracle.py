@@ -21,7 +21,7 @@ class Params(C.Structure):
     _fields_ = [("solver", C.c_int), ("vit", C.c_int), ("pit", C.c_int), ("relax", C.c_float),
                 ("t2d", C.c_float), ("warm_start", C.c_int), ("R", C.c_float),
                 ("margin", C.c_float), ("cap", C.c_int), ("n_colors", C.c_int),
-                ("per_contact", C.c_int)]
+                ("per_contact", C.c_int), ("manifold", C.c_int)]
 
 
 def build() -> Path:
@@ -86,7 +86,8 @@ class OracleWorld:
     """One simulated world on the CPU restatement (fp64=True: the float64 build)."""
 
     def __init__(self, bodies, solver="jacobi", vit=40, pit=20, relax=0.2, t2d=1.0,
-                 warm_start=True, R=0.05, margin=0.0, cap=None, n_colors=0, per_contact=False, fp64=False):
+                 warm_start=True, R=0.05, margin=0.0, cap=None, n_colors=0, per_contact=False, fp64=False,
+                 manifold=None):
         self.fp64 = bool(fp64)
         self.dtype = np.float64 if fp64 else np.float32
         L = lib(self.fp64)
@@ -95,8 +96,10 @@ class OracleWorld:
         P = int(len(bodies.rest))
         if cap is None:
             cap = max(1024, 4 * N * 8) if P == 0 else max(1024, N * P * 8)
+        if manifold is None:  # the coloured bead engines reduce to 4 points per pair
+            manifold = 4 if (P > 0 and solver != "jacobi" and not per_contact) else 0
         prm = Params(SOLVERS[solver], vit, pit, relax, t2d, int(warm_start), R, margin, cap,
-                     n_colors, int(per_contact))
+                     n_colors, int(per_contact), int(manifold))
         self._prm = prm
         self.N, self.P, self.cap = N, P, cap
         self.h = L.or_create(N, P, C.byref(prm))
@@ -154,10 +157,11 @@ class OracleWorld:
         return F
 
     def counters(self):
-        out = np.zeros(6, np.int64)
+        out = np.zeros(7, np.int64)
         lib(self.fp64).or_counters(self.h, _p(out))
         return dict(n_contacts=int(out[0]), n_pairs=int(out[1]), n_colors=int(out[2]),
-                    cache_hits=int(out[3]), cache_misses=int(out[4]), overflow=int(out[5]))
+                    cache_hits=int(out[3]), cache_misses=int(out[4]), overflow=int(out[5]),
+                    n_generated=int(out[6]))
 
     def contacts(self):
         C_ = self.counters()["n_contacts"]
