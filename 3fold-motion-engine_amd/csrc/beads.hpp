@@ -1,0 +1,274 @@
+// Bead geometry for the coloured solvers: per-body-pair bead contacts (A7) and the manifold
+// reduction to at most mx points per body pair (A15), both on the device.
+//
+// Layout. The broad phase lists candidate body pairs in ascending pair key a*(N+1) + b + 1 (the
+// ground pair (a, -1) first for every body). One wave per candidate pair tests its P x P bead
+// pairs (P ground tests for a ground pair) and a ballot gives one bit per hit in slot order
+// la*P + lb, so the generated contacts come out pair-major and, within a pair, in the reference's
+// feature-key order (fa = a*P + la, fb = b*P + lb). A contact's key is the pair-major composite
+// key pairkey*P*(P+1) + la*(P+1) + lb + 1: it orders like (pair key, feature key) and names the same
+// feature as the reference's fa*(N*P+1) + fb + 1, so warm-start matches by key equality are the
+// reference's matches. The kept contacts stay pair-major, so every pair is a contiguous run of at
+// most mx contacts and the solvers read them without an index list.
+#pragma once
+#include "kernels.hpp"
+
+namespace me {
+
+struct BeadRaw {
+    int64_t *key;
+    float *pA, *pB, *n, *pen;
+    double *jnd, *jnr;  // warm normal impulse (float64, as the reference's jn_in) and the redistributed one
+    float *jt1, *jt2;
+    int *keep;
+};
+
+// L-infinity prefilter of the reference's pair-bitmask generator: the float32 successor of 0.1f,
+// wider than any hit (contact_generation_pair_bitmask.py:30)
+#define ME_BEAD_LINF 0.10000000894069672f
+#define ME_BEAD_MAXSEG 1024
+#define ME_BEAD_MAXKEEP 8
+
+// bead half extents of every body: the AABB of its bead spheres in the body frame
+__global__ void k_fill_half(int N, float hx, float hy, float hz, float *half) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    half[3 * i] = hx; half[3 * i + 1] = hy; half[3 * i + 2] = hz;
+}
+
+__device__ __forceinline__ bool bead_hit(const float *allp, int a, int b, int slot, int P, float R) {
+    if (b < 0) {
+        if (slot >= P) return false;
+        return allp[3 * (a * P + slot) + 2] - R < 0.0f;
+    }
+    if (slot >= P * P) return false;
+    f3 d = ld3(allp, a * P + slot / P) - ld3(allp, b * P + slot % P);
+    if (!(fabsf(d.x) <= ME_BEAD_LINF && fabsf(d.y) <= ME_BEAD_LINF && fabsf(d.z) <= ME_BEAD_LINF)) return false;
+    float dist = length(d);
+    return dist < 2.0f * R && dist > 1e-9f;
+}
+
+// A7 count (contact_generation_pair_bitmask.py:16-38): one wave per candidate pair, one 64-bit
+// ballot per word of 64 slots; the word masks are kept for the emit pass
+__global__ void __launch_bounds__(ME_THREADS) k_bp_count(int cap, const int *istat, const int *cand_a,
+                                                         const int *cand_b, const float *allp, int P, float R,
+                                                         int words, uint64_t *mask, int *cnt) {
+    int pair = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    int lane = threadIdx.x & 63;
+    int n = istat[IS_CAND];
+    if (n > cap) n = cap;
+    if (pair >= n) return;  // uniform across the wave
+    int a = cand_a[pair], b = cand_b[pair];
+    int total = 0;
+    for (int word = 0; word < words; ++word) {
+        uint64_t m = __ballot(bead_hit(allp, a, b, word * 64 + lane, P, R));
+        if (lane == 0) mask[(size_t)pair * words + word] = m;
+        total += __popcll(m);
+    }
+    if (lane == 0) cnt[pair] = total;
+}
+
+// A7 emit (contact_generation_pair_bitmask.py:40-64): every hit goes to offset + its rank among the
+// pair's hits; ground contacts: normal +z, pen R - z; bead pairs: normal (xi - xj)/|xi - xj|,
+// pen 2R - |xi - xj| (k_gen_fid, contact_engine_gpu_colored.py:103-122)
+__global__ void __launch_bounds__(ME_THREADS) k_bp_emit(int cap, const int *istat, const int *cand_a,
+                                                        const int *cand_b, const float *allp, int N, int P, float R,
+                                                        int words, const uint64_t *mask, const int *off, int capR,
+                                                        BeadRaw raw, int *istat_w) {
+    int pair = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    int lane = threadIdx.x & 63;
+    int n = istat[IS_CAND];
+    if (n > cap) n = cap;
+    if (pair >= n) return;
+    int a = cand_a[pair], b = cand_b[pair];
+    int base = off[pair];
+    int64_t pk = (int64_t)a * (int64_t)(N + 1) + (int64_t)b + 1;
+    int64_t kstride = (int64_t)P * (int64_t)(P + 1);
+    uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    bool over = false;
+    for (int word = 0; word < words; ++word) {
+        uint64_t m = mask[(size_t)pair * words + word];
+        if ((m >> lane) & 1ull) {
+            int t = base + __popcll(m & below);
+            if (t >= capR) {
+                over = true;
+            } else {
+                int slot = word * 64 + lane;
+                int la = b < 0 ? slot : slot / P, lb = b < 0 ? -1 : slot % P;
+                f3 xi = ld3(allp, a * P + la);
+                raw.key[t] = pk * kstride + (int64_t)la * (P + 1) + (int64_t)(lb + 1);
+                st3(raw.pA, t, xi);
+                if (b < 0) {
+                    st3(raw.pB, t, mk(0.0f, 0.0f, 0.0f));
+                    st3(raw.n, t, mk(0.0f, 0.0f, 1.0f));
+                    raw.pen[t] = -(xi.z - R);
+                } else {
+                    f3 xj = ld3(allp, b * P + lb);
+                    f3 d = xi - xj;
+                    float dist = length(d);
+                    st3(raw.pB, t, xj);
+                    st3(raw.n, t, d / dist);
+                    raw.pen[t] = 2.0f * R - dist;
+                }
+            }
+        }
+        base += __popcll(m);
+    }
+    if (over) atomicOr(&istat_w[IS_OVF], OVF_CONTACTS);
+}
+
+// planar coordinates about ref in the (u, w) basis: float32 difference, then float64
+// (contact_engine_gpu_colored.py:585-590)
+__device__ __forceinline__ void bead_planar(f3 p, f3 ref, const double *u, const double *w, double &r0, double &r1) {
+    float dx = p.x - ref.x, dy = p.y - ref.y, dz = p.z - ref.z;
+    double ax = (double)dx, ay = (double)dy, az = (double)dz;
+    r0 = ax * u[0] + ay * u[1] + az * u[2];
+    r1 = ax * w[0] + ay * w[1] + az * w[2];
+}
+
+// A14 + A15 for one candidate pair (contact_engine_gpu_colored.py:595-620 k_warm, :647-720 k_select):
+// warm start of each generated contact by binary search of its key in the previous step's kept keys,
+// then, above mx contacts, the E-optimal choice of mx of them (deepest first, then the greedy max of
+// min(k+1, lambda_min) of the planar second moment about body A's centre, first index on ties) and
+// the float64 redistribution of every contact's normal impulse onto the kept point nearest to it in
+// the contact plane. One thread per pair; returns the kept count, or -1 past the fixed sizes.
+__device__ int bead_select(int s0, int L, int mx, f3 ref, BeadRaw raw) {
+    if (L <= mx) {
+        for (int t = 0; t < L; ++t) { raw.keep[s0 + t] = 1; raw.jnr[s0 + t] = raw.jnd[s0 + t]; }
+        return L;
+    }
+    if (L > ME_BEAD_MAXSEG || mx > ME_BEAD_MAXKEEP) {
+        for (int t = 0; t < L; ++t) { raw.keep[s0 + t] = 1; raw.jnr[s0 + t] = raw.jnd[s0 + t]; }
+        return -1;
+    }
+    for (int t = 0; t < L; ++t) raw.keep[s0 + t] = 0;
+    int deep = 0;
+    float dbest = raw.pen[s0];
+    for (int t = 1; t < L; ++t)
+        if (raw.pen[s0 + t] > dbest) { dbest = raw.pen[s0 + t]; deep = t; }
+    f3 nr = ld3(raw.n, s0 + deep);
+    double ax = 1.0, ay = 0.0, az = 0.0;
+    if (fabsf(nr.x) >= 0.9f) { ax = 0.0; ay = 1.0; az = 0.0; }
+    double n0 = (double)nr.x, n1 = (double)nr.y, n2 = (double)nr.z;
+    double dn = ax * n0 + ay * n1 + az * n2;
+    double u0 = ax - dn * n0, u1 = ay - dn * n1, u2 = az - dn * n2;
+    double ln = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    double u[3] = {u0 / ln, u1 / ln, u2 / ln};
+    double w[3] = {n1 * u[2] - n2 * u[1], n2 * u[0] - n0 * u[2], n0 * u[1] - n1 * u[0]};
+    double r0, r1;
+    bead_planar(ld3(raw.pA, s0 + deep), ref, u, w, r0, r1);
+    double S00 = r0 * r0, S01 = r0 * r1, S10 = r1 * r0, S11 = r1 * r1;
+    int pick[ME_BEAD_MAXKEEP];
+    pick[0] = deep;
+    for (int k = 1; k < mx; ++k) {
+        double sbest = -1.0e300;
+        int tbest = -1;
+        double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+        for (int t = 0; t < L; ++t) {
+            bool taken = false;
+            for (int q = 0; q < k; ++q) taken = taken || pick[q] == t;
+            if (taken) continue;
+            double t0, t1;
+            bead_planar(ld3(raw.pA, s0 + t), ref, u, w, t0, t1);
+            double a00 = S00 + t0 * t0, a01 = S01 + t0 * t1;
+            double a10 = S10 + t1 * t0, a11 = S11 + t1 * t1;
+            double tr = a00 + a11;
+            double det = a00 * a11 - a01 * a10;
+            double disc = tr * tr / 4.0 - det;
+            if (!(disc > 0.0)) disc = 0.0;
+            double lam = tr / 2.0 - sqrt(disc);
+            double kk = (double)(k + 1);
+            double sc = lam < kk ? lam : kk;
+            if (sc > sbest) { sbest = sc; tbest = t; c00 = a00; c01 = a01; c10 = a10; c11 = a11; }
+        }
+        if (tbest < 0) return -1;
+        pick[k] = tbest;
+        S00 = c00; S01 = c01; S10 = c10; S11 = c11;
+    }
+    for (int k = 0; k < mx; ++k) { raw.keep[s0 + pick[k]] = 1; raw.jnr[s0 + pick[k]] = 0.0; }
+    for (int t = 0; t < L; ++t) {
+        double t0, t1;
+        bead_planar(ld3(raw.pA, s0 + t), ref, u, w, t0, t1);
+        double dbest2 = 1.0e300;
+        int kbest = 0;
+        for (int k = 0; k < mx; ++k) {
+            double k0, k1;
+            bead_planar(ld3(raw.pA, s0 + pick[k]), ref, u, w, k0, k1);
+            double e0 = t0 - k0, e1 = t1 - k1;
+            double d2 = e0 * e0 + e1 * e1;
+            if (d2 < dbest2) { dbest2 = d2; kbest = k; }
+        }
+        int gk = s0 + pick[kbest];
+        raw.jnr[gk] = raw.jnr[gk] + raw.jnd[s0 + t];
+    }
+    return mx;
+}
+
+__global__ void k_bp_select(int cap, int *istat, const int *cand_a, const float *x, const int *cnt, const int *off,
+                            int capR, int mx, int warm, const int64_t *wkey, const float *wjn, const float *wjt1,
+                            const float *wjt2, BeadRaw raw, int *kcnt, int *pflag) {
+    int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = istat[IS_CAND];
+    if (n > cap) n = cap;
+    if (pair >= n) return;
+    int L = cnt[pair], s0 = off[pair];
+    if (L > 0 && s0 + L > capR) L = 0;  // overflow already flagged by the emit pass
+    pflag[pair] = L > 0 ? 1 : 0;
+    if (L == 0) { kcnt[pair] = 0; return; }
+    int wn = warm ? istat[IS_WN] : 0;
+    for (int t = 0; t < L; ++t) {
+        int c = s0 + t;
+        raw.jnd[c] = 0.0; raw.jt1[c] = 0.0f; raw.jt2[c] = 0.0f;
+        if (wn == 0) continue;
+        int64_t k = raw.key[c];
+        int lo = 0, hi = wn - 1, hit = -1;
+        while (lo <= hi) {
+            int mid = (lo + hi) / 2;
+            int64_t km = wkey[mid];
+            if (km == k) { hit = mid; lo = hi + 1; }
+            else if (km < k) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        if (hit >= 0) { raw.jnd[c] = (double)wjn[hit]; raw.jt1[c] = wjt1[hit]; raw.jt2[c] = wjt2[hit]; }
+    }
+    int k = bead_select(s0, L, mx, ld3(x, cand_a[pair]), raw);
+    if (k < 0) {
+        atomicOr(&istat[IS_OVF], OVF_CONTACTS);
+        k = L;
+    }
+    kcnt[pair] = k;
+}
+
+// compaction of the kept contacts, pair-major, and one pair per candidate pair with contacts
+// (k_compact + the pair build, contact_engine_gpu_colored.py:723-740, 1138-1157)
+__global__ void k_bp_compact(int cap, const int *istat, const int *cand_a, const int *cand_b, const float *mu,
+                             const int *cnt, const int *off, const int *kcnt, const int *koff, const int *pflag,
+                             const int *pidx, int capC, int capP, BeadRaw raw, Contacts Cc, Pairs Pp,
+                             int *istat_w) {
+    int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = istat[IS_CAND];
+    if (n > cap) n = cap;
+    if (pair >= n || !pflag[pair]) return;
+    int a = cand_a[pair], b = cand_b[pair];
+    int p = pidx[pair], o = koff[pair], kn = kcnt[pair], L = cnt[pair], s0 = off[pair];
+    if (p >= capP || o + kn > capC) {
+        atomicOr(&istat_w[IS_OVF], OVF_CONTACTS);
+        return;
+    }
+    Pp.pbi[p] = a; Pp.pbj[p] = b; Pp.pstart[p] = o; Pp.pcount[p] = kn;
+    float m = pair_mu(mu, a, b);
+    int d = o;
+    for (int t = 0; t < L; ++t) {
+        int c = s0 + t;
+        if (!raw.keep[c]) continue;
+        Cc.key[d] = raw.key[c]; Cc.bi[d] = a; Cc.bj[d] = b;
+        st3(Cc.pA, d, ld3(raw.pA, c)); st3(Cc.pB, d, ld3(raw.pB, c)); st3(Cc.n, d, ld3(raw.n, c));
+        Cc.pen[d] = raw.pen[c]; Cc.mu[d] = m;
+        Cc.jn[d] = (float)raw.jnr[c]; Cc.jt1[d] = raw.jt1[c]; Cc.jt2[d] = raw.jt2[c]; Cc.jp[d] = 0.0f;
+        ++d;
+    }
+}
+
+__global__ void k_copy_stat(int *istat, int dst, int src) { istat[dst] = istat[src]; }
+
+}  // namespace me

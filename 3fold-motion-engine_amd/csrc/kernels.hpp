@@ -10,7 +10,7 @@ namespace me {
 enum {
     IS_C = 0, IS_P, IS_CAND, IS_NCOL, IS_HITS, IS_MISSES, IS_OVF, IS_ISL, IS_TILES, IS_MAXTB,
     IS_MAXEXT, IS_LEFT, IS_WN, IS_CACHED_P, IS_CACHED_NCOL, IS_MISMATCH, IS_CELL, IS_NBODYSORT,
-    IS_NPAIRSORT, IS_NCOLJP, IS_NBIG, IS_COUNT = 32
+    IS_NPAIRSORT, IS_NCOLJP, IS_NBIG, IS_CRAW, IS_COUNT = 32
 };
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,
        OVF_TAPE = 32 /* a recorded (differentiated) substep met a tile beyond one small tile */ };
@@ -240,51 +240,14 @@ __device__ __forceinline__ bool aabb_overlap(const float *a, const float *b, flo
 }
 
 // ── A6 body pairs: ground candidate + AABB overlaps with higher bodies of the same scene, in key order ──
-__global__ void k_box_cand(int N, Bodies b, const float *aabb, const int *cellc, const int *bstart, const int *bcount,
-                           const int *items, int T, float margin, int *cand_n, int *cand_tmp, int *istat) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= N) return;
-    int list[ME_MAXCAND];
-    int nc = 0;
-    bool over = false;
-    float m = margin + ME_BP_SLACK;
-    const float *A = aabb + 6 * a;
-    bool dynA = b.invM[a] > 0.0f;
-    if (dynA && A[2] < m) list[nc++] = -1;
-    int cx = cellc[4 * a], cy = cellc[4 * a + 1], cz = cellc[4 * a + 2], s = cellc[4 * a + 3];
-    int first = nc;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                int X = cx + dx, Y = cy + dy, Z = cz + dz;
-                int bk = (int)(cell_hash(X, Y, Z, s) & (uint32_t)(T - 1));
-                int st = bstart[bk], cnt = bcount[bk];
-                for (int t = 0; t < cnt; ++t) {
-                    int j = items[st + t];
-                    if (j <= a) continue;
-                    if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z || cellc[4 * j + 3] != s)
-                        continue;
-                    if (!dynA && !(b.invM[j] > 0.0f)) continue;
-                    if (!aabb_overlap(A, aabb + 6 * j, m)) continue;
-                    if (nc < ME_MAXCAND) {
-                        int k = nc++;  // insertion keeps ascending body order
-                        while (k > first && list[k - 1] > j) { list[k] = list[k - 1]; --k; }
-                        list[k] = j;
-                    } else {
-                        over = true;
-                    }
-                }
-            }
-    if (over) atomicOr(&istat[IS_OVF], OVF_CAND);
-    cand_n[a] = nc;
-    for (int k = 0; k < nc; ++k) cand_tmp[a * ME_MAXCAND + k] = list[k];
-}
 // broad phase in two parallel passes: one thread per (body, neighbour cell) collects the higher
 // bodies of that cell whose AABB overlaps (k_box_cell); one thread per body merges its 27 lists
-// in ascending order behind its ground candidate (k_box_merge)
+// in ascending order behind its ground candidate (k_box_merge). all_pairs (bead geometry): pairs of
+// two static bodies and ground pairs of static bodies are kept, as the reference's bead contact
+// generation emits their contacts too (contact_engine_gpu_colored.py:103-122)
 #define ME_CELLCAND 16
 __global__ void k_box_cell(int N, Bodies b, const float *aabb, const int *cellc, const int *bstart, const int *bcount,
-                           const int *items, int T, float margin, int *cc_n, int *cc_list, int *istat) {
+                           const int *items, int T, float margin, int all_pairs, int *cc_n, int *cc_list, int *istat) {
     long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= 27LL * N) return;
     int a = (int)(gid / 27), cl = (int)(gid % 27);
@@ -300,7 +263,7 @@ __global__ void k_box_cell(int N, Bodies b, const float *aabb, const int *cellc,
         int j = items[st + t];
         if (j <= a) continue;
         if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z || cellc[4 * j + 3] != s) continue;
-        if (!dynA && !(b.invM[j] > 0.0f)) continue;
+        if (!all_pairs && !dynA && !(b.invM[j] > 0.0f)) continue;
         if (!aabb_overlap(A, aabb + 6 * j, m)) continue;
         if (n < ME_CELLCAND) cc_list[gid * ME_CELLCAND + n] = j;
         else atomicOr(&istat[IS_OVF], OVF_CAND);
@@ -308,14 +271,14 @@ __global__ void k_box_cell(int N, Bodies b, const float *aabb, const int *cellc,
     }
     cc_n[gid] = n < ME_CELLCAND ? n : ME_CELLCAND;
 }
-__global__ void k_box_merge(int N, Bodies b, const float *aabb, float margin, const int *cc_n, const int *cc_list,
-                            int *cand_n, int *cand_tmp, int *istat) {
+__global__ void k_box_merge(int N, Bodies b, const float *aabb, float margin, int all_pairs, const int *cc_n,
+                            const int *cc_list, int *cand_n, int *cand_tmp, int *istat) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= N) return;
     int list[ME_MAXCAND];
     int nc = 0;
     bool over = false;
-    if (b.invM[a] > 0.0f && aabb[6 * a + 2] < margin + ME_BP_SLACK) list[nc++] = -1;
+    if ((all_pairs || b.invM[a] > 0.0f) && aabb[6 * a + 2] < margin + ME_BP_SLACK) list[nc++] = -1;
     int first = nc;
     for (int cl = 0; cl < 27; ++cl) {
         long long g = 27LL * a + cl;

@@ -12,6 +12,7 @@
 
 #include "kernels.hpp"
 #include "tiled.hpp"
+#include "beads.hpp"
 #include "adjoint_kernels.hpp"
 
 using namespace me;
@@ -86,6 +87,14 @@ struct me_engine {
     // warm store
     int64_t *wkey;
     float *wjn, *wjt1, *wjt2;
+    // bead geometry under a coloured solver: body-pair broad phase, per-pair bead tests, manifold stage
+    bool bead_pairs = false;
+    int bp_words = 0;      // 64-slot ballot words per candidate pair
+    float bhalf[3] = {0.0f, 0.0f, 0.0f};  // half extents of the bead spheres' box in the body frame
+    float *bp_half;        // [N,3] filled with bhalf every substep
+    uint64_t *bp_mask;     // [cap_cand * bp_words]
+    int *bp_kn, *bp_koff;  // kept contacts per candidate pair, their offsets
+    BeadRaw raw;           // generated contacts, pair-major, before the reduction
     // tiled solve
     TiledBuffers tb;
     // scan scratch + status
@@ -142,9 +151,15 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     int N = c.n_bodies, P = c.n_beads > 0 ? c.n_beads : 0;
     long long NP = (long long)N * (P > 0 ? P : 1);
     e->N = N; e->P = P; e->NP = (int)NP;
-    long long items = P > 0 ? NP : N;
+    // bead geometry: the Jacobi solver takes contacts from a grid of bead points; the coloured
+    // solvers take them per body pair from the body broad phase (pair-major, then reduced)
+    bool bp = P > 0 && c.solver != ME_SOLVER_JACOBI;
+    bool point_grid = P > 0 && !bp;
+    e->bead_pairs = bp;
+    e->bp_words = bp ? (P * P + 63) / 64 : 0;
+    long long items = point_grid ? NP : N;
     e->T = pow2_at_least(2 * items > 64 ? 2 * items : 64);
-    e->cap_cand = c.max_pairs > 0 ? c.max_pairs : (P > 0 ? 1 : 16 * N + 1024);
+    e->cap_cand = c.max_pairs > 0 ? c.max_pairs : (point_grid ? 1 : 16 * N + 1024);
     e->capC = c.max_contacts > 0 ? c.max_contacts : (P > 0 ? 400000 : 4 * e->cap_cand);
     e->capP = (P > 0 || (c.flags & ME_FLAG_PER_CONTACT)) ? e->capC : e->cap_cand;
     long long scan_n = items > e->cap_cand ? items : e->cap_cand;
@@ -213,9 +228,9 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->items = B.take<int>((size_t)items);
     e->cand_n = B.take<int>((size_t)items);
     e->cand_off = B.take<int>((size_t)items);
-    e->cand_tmp = B.take<int>(P > 0 ? 1 : (size_t)N * ME_MAXCAND);
-    e->cc_n = B.take<int>(P > 0 ? 1 : 27 * (size_t)N);
-    e->cc_list = B.take<int>(P > 0 ? 1 : 27 * (size_t)ME_CELLCAND * N);
+    e->cand_tmp = B.take<int>(point_grid ? 1 : (size_t)N * ME_MAXCAND);
+    e->cc_n = B.take<int>(point_grid ? 1 : 27 * (size_t)N);
+    e->cc_list = B.take<int>(point_grid ? 1 : 27 * (size_t)ME_CELLCAND * N);
     e->S.key = B.take<int64_t>(ns);
     e->S.p = B.take<float>(3 * ns);
     e->S.nrm = B.take<float>(3 * ns);
@@ -223,6 +238,23 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->S.mu = B.take<float>(ns);
     e->pair_flag = B.take<int>((size_t)e->cap_cand);
     e->pair_off = B.take<int>((size_t)e->cap_cand);
+    {
+        size_t R = bp ? C : 1, cc = bp ? (size_t)e->cap_cand : 1;
+        e->bp_half = B.take<float>(bp ? 3 * (size_t)N : 1);
+        e->bp_mask = B.take<uint64_t>(cc * (size_t)(bp ? e->bp_words : 1));
+        e->bp_kn = B.take<int>(cc);
+        e->bp_koff = B.take<int>(cc);
+        e->raw.key = B.take<int64_t>(R);
+        e->raw.pA = B.take<float>(3 * R);
+        e->raw.pB = B.take<float>(3 * R);
+        e->raw.n = B.take<float>(3 * R);
+        e->raw.pen = B.take<float>(R);
+        e->raw.jnd = B.take<double>(R);
+        e->raw.jnr = B.take<double>(R);
+        e->raw.jt1 = B.take<float>(R);
+        e->raw.jt2 = B.take<float>(R);
+        e->raw.keep = B.take<int>(R);
+    }
     tiled_layout_scratch(e->tb, B, N, (int)Pc, (int)C);
     e->nbsum = (int)((scan_n + SCAN_BLOCK - 1) / SCAN_BLOCK) + 1;
     e->bsum = B.take<int>((size_t)e->nbsum);
@@ -235,8 +267,10 @@ int validate(const me_config *c) {
     if (c->solver < 0 || c->solver > 2) return fail(ME_EARG, "solver must be 0 (jacobi), 1 (pair chunks) or 2 (color cache)");
     if (c->vit < 0 || c->pit < 0) return fail(ME_EARG, "iteration counts must be >= 0");
     if (c->n_beads < 0) return fail(ME_EARG, "n_beads must be >= 0");
-    if (c->n_beads > 0 && c->solver != ME_SOLVER_JACOBI)
-        return fail(ME_EARG, "bead geometry is supported by the Jacobi solver only (the bead manifold reduction is not built)");
+    if (c->n_beads > 0 && c->solver != ME_SOLVER_JACOBI && (c->flags & ME_FLAG_PER_CONTACT))
+        return fail(ME_EARG, "bead geometry under a coloured solver needs pair chunks (the manifold reduction to 4 "
+                             "points per body pair); per-contact colouring of raw bead contacts is not built");
+    if (c->n_beads > 64) return fail(ME_EARG, "n_beads must be <= 64");
     if (c->n_colors < 0 || c->n_colors > ME_NCOLMAX) return fail(ME_EARG, "n_colors must be in [0, 512]");
     if ((long long)c->n_bodies * (c->n_beads > 0 ? c->n_beads : 1) > (1LL << 30))
         return fail(ME_EARG, "too many bodies/beads");
@@ -263,7 +297,8 @@ int scan(me_engine *e, const int *in, int *out, const int *n_dev, int n_host, in
 Bodies bodies_of(me_engine *e) {
     Bodies b;
     b.x = e->b.x; b.q = e->b.q; b.v = e->b.v; b.w = e->b.w;
-    b.invM = e->b.inv_mass; b.IbInv = e->b.inv_inertia; b.half = e->b.half; b.mu = e->b.mu;
+    b.invM = e->b.inv_mass; b.IbInv = e->b.inv_inertia; b.half = e->bead_pairs ? e->bp_half : e->b.half;
+    b.mu = e->b.mu;
     b.kin = e->b.kinematic; b.scene = e->b.scene; b.rest = e->b.rest;
     return b;
 }
@@ -288,10 +323,67 @@ int build_grid(me_engine *e, int n, const float *pos, const int *scene, int div,
     return ME_OK;
 }
 
-// contact generation: box geometry (A6 body pairs + narrow phase) or bead geometry (A4 + A5)
+// body pairs (A6): grid over body centres, AABB overlaps in pair-key order, ground pair first;
+// all_pairs keeps pairs of static bodies (bead geometry)
+int body_pairs(me_engine *e, int all_pairs, hipStream_t s) {
+    Bodies b = bodies_of(e);
+    int N = e->N;
+    hipLaunchKernelGGL(k_grid_cell, dim3(1), dim3(1), 0, s, e->istat, e->cfg.margin);
+    int rc = build_grid(e, N, e->b.x, e->b.scene, 1, -1.0f, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_box_cell, dim3(nblk(27LL * N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cellc, e->bstart,
+                       e->bcount, e->items, e->T, e->cfg.margin, all_pairs, e->cc_n, e->cc_list, e->istat);
+    hipLaunchKernelGGL(k_box_merge, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cfg.margin, all_pairs,
+                       e->cc_n, e->cc_list, e->cand_n, e->cand_tmp, e->istat);
+    rc = scan(e, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_box_cand_emit, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->cand_n, e->cand_off,
+                       e->cand_tmp, e->cap_cand, e->cand_a, e->cand_b, e->istat);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+#define ME_BEAD_KEEP 4  // manifold_max_points of the reference's coloured engines
+
+// bead geometry under a coloured solver: A4 world points, A6 body pairs over the boxes of the bead
+// spheres, A7 per-pair bead tests (pair-major), A14 warm start and A15 manifold reduction per pair,
+// compaction into contiguous pairs of at most ME_BEAD_KEEP contacts. Leaves the contacts with their
+// warm-start impulses set.
+int generate_bead_pairs(me_engine *e, hipStream_t s) {
+    Bodies b = bodies_of(e);
+    int N = e->N, P = e->P, NP = e->NP, cc = e->cap_cand, W = e->bp_words;
+    float R = e->cfg.bead_radius;
+    hipLaunchKernelGGL(k_bead_points, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, N, P, b, e->allp);
+    int rc = body_pairs(e, 1, s);
+    if (rc) return rc;
+    int wg = nblk(64LL * cc);
+    hipLaunchKernelGGL(k_bp_count, dim3(wg), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->cand_b, e->allp, P,
+                       R, W, e->bp_mask, e->S.n);
+    rc = scan(e, e->S.n, e->slot_off, e->istat + IS_CAND, 0, cc, e->istat + IS_CRAW, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bp_emit, dim3(wg), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->cand_b, e->allp, N, P,
+                       R, W, e->bp_mask, e->slot_off, e->capC, e->raw, e->istat);
+    hipLaunchKernelGGL(k_bp_select, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->b.x, e->S.n,
+                       e->slot_off, e->capC, ME_BEAD_KEEP, e->cfg.warm_start, e->wkey, e->wjn, e->wjt1, e->wjt2,
+                       e->raw, e->bp_kn, e->pair_flag);
+    rc = scan(e, e->bp_kn, e->bp_koff, e->istat + IS_CAND, 0, cc, e->istat + IS_C, s);
+    if (rc) return rc;
+    rc = scan(e, e->pair_flag, e->pair_off, e->istat + IS_CAND, 0, cc, e->istat + IS_P, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bp_compact, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->cand_b,
+                       e->b.mu, e->S.n, e->slot_off, e->bp_kn, e->bp_koff, e->pair_flag, e->pair_off, e->capC,
+                       e->capP, e->raw, e->Cc, e->Pp, e->istat);
+    hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+// contact generation: box geometry (A6 body pairs + narrow phase) or bead geometry (A4 + A5, or
+// A4 + A6 + A7 + A15 under a coloured solver)
 int generate(me_engine *e, hipStream_t s) {
     Bodies b = bodies_of(e);
     int N = e->N;
+    if (e->bead_pairs) return generate_bead_pairs(e, s);
     if (e->P > 0) {
         int NP = e->NP, P = e->P;
         float R = e->cfg.bead_radius;
@@ -307,20 +399,12 @@ int generate(me_engine *e, hipStream_t s) {
                            e->bstart, e->bcount, e->items, e->T, R, e->b.mu, e->cand_n, e->cand_off, e->capC, e->Cc,
                            e->istat);
         hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP);
+        hipLaunchKernelGGL(k_copy_stat, dim3(1), dim3(1), 0, s, e->istat, IS_CRAW, IS_C);
         LAUNCH_CHECK();
         return ME_OK;
     }
-    hipLaunchKernelGGL(k_grid_cell, dim3(1), dim3(1), 0, s, e->istat, e->cfg.margin);
-    int rc = build_grid(e, N, e->b.x, e->b.scene, 1, -1.0f, s);
+    int rc = body_pairs(e, 0, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_box_cell, dim3(nblk(27LL * N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cellc, e->bstart,
-                       e->bcount, e->items, e->T, e->cfg.margin, e->cc_n, e->cc_list, e->istat);
-    hipLaunchKernelGGL(k_box_merge, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cfg.margin, e->cc_n,
-                       e->cc_list, e->cand_n, e->cand_tmp, e->istat);
-    rc = scan(e, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_box_cand_emit, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->cand_n, e->cand_off,
-                       e->cand_tmp, e->cap_cand, e->cand_a, e->cand_b, e->istat);
     int cc = e->cap_cand;
     prof_mark(e, ME_K_NARROW, s);
     hipLaunchKernelGGL(k_narrow, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, b, e->Rw, e->cand_a, e->cand_b,
@@ -334,6 +418,7 @@ int generate(me_engine *e, hipStream_t s) {
     hipLaunchKernelGGL(k_compact, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->capC, e->capP, e->istat, e->cand_a,
                        e->cand_b, e->S, e->slot_off, e->pair_off, e->Cc, e->Pp);
     hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP);
+    hipLaunchKernelGGL(k_copy_stat, dim3(1), dim3(1), 0, s, e->istat, IS_CRAW, IS_C);
     if (e->cfg.flags & ME_FLAG_PER_CONTACT)
         hipLaunchKernelGGL(k_pairs_per_contact, dim3(nblk(e->capC)), dim3(ME_THREADS), 0, s, e->capC, e->istat,
                            e->Cc.bi, e->Cc.bj, e->Pp);
@@ -431,8 +516,9 @@ int solve_colored(me_engine *e, float sdt, hipStream_t s) {
         LAUNCH_CHECK();
         return ME_OK;
     }
-    hipLaunchKernelGGL(k_warm_lookup, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->cfg.warm_start,
-                       e->Cc.key, e->wkey, e->wjn, e->wjt1, e->wjt2, e->Cc.jn, e->Cc.jt1, e->Cc.jt2, e->Cc.jp);
+    if (!e->bead_pairs)  // bead contacts carry their warm start from the manifold stage
+        hipLaunchKernelGGL(k_warm_lookup, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->cfg.warm_start,
+                           e->Cc.key, e->wkey, e->wjn, e->wjt1, e->wjt2, e->Cc.jn, e->Cc.jt1, e->Cc.jt2, e->Cc.jp);
     bool hit = false;
     if (cc_mode) {
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_MISMATCH, 1, 0);
@@ -495,8 +581,11 @@ int substep(me_engine *e, double sdt64, hipStream_t s) {
     int N = e->N;
     Bodies b = bodies_of(e);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_MAXEXT, 1, 0);
+    if (e->bead_pairs)
+        hipLaunchKernelGGL(k_fill_half, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->bhalf[0], e->bhalf[1], e->bhalf[2],
+                           e->bp_half);
     hipLaunchKernelGGL(k_body_pre, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->invIw, e->Rw, e->aabb, e->vsnap,
-                       e->istat, sdt, e->P == 0 ? 1 : 0);
+                       e->istat, sdt, (e->P == 0 || e->bead_pairs) ? 1 : 0);
     LAUNCH_CHECK();
     int rc = generate(e, s);
     if (rc) return rc;
@@ -623,6 +712,7 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.N = e->N; a.capC = e->capC; a.capP = e->capP;
     a.solver = e->cfg.solver; a.vit = e->cfg.vit; a.pit = e->cfg.pit; a.t2d = e->cfg.t2d;
     a.warm = e->cfg.warm_start; a.n_colors = e->cfg.n_colors; a.tile_bodies = e->cfg.tile_bodies;
+    a.warm_set = e->bead_pairs ? 1 : 0;
     a.dt = sdt;
     a.b = bodies_of(e);
     a.invIw = e->invIw; a.pv = e->pv; a.po = e->po;
@@ -637,6 +727,23 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.taped = e->taping ? 1 : 0;
     a.tape = e->tape;
     return a;
+}
+
+// bead geometry under a coloured solver: half extents of the box around the bead spheres (body
+// frame), the extent the body-pair broad phase bounds each body by
+int bead_half_extents(me_engine *e, hipStream_t s) {
+    if (!e->bead_pairs) return ME_OK;
+    std::vector<float> rest(3 * (size_t)e->P);
+    HIP_TRY(hipMemcpyAsync(rest.data(), e->b.rest, sizeof(float) * rest.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float R = e->cfg.bead_radius;
+    for (int k = 0; k < 3; ++k) {
+        float m = 0.0f;
+        for (int i = 0; i < e->P; ++i) m = fmaxf(m, fabsf(rest[3 * i + k]));
+        if (!(m == m) || !(R >= 0.0f)) return fail(ME_ENONFINITE, "bead rest positions or radius not finite");
+        e->bhalf[k] = m + R;
+    }
+    return ME_OK;
 }
 
 extern "C" {
@@ -704,6 +811,11 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
         release(e);
         return fail(ME_ERUNTIME, "workspace initialisation failed");
     }
+    rc = bead_half_extents(e, s);
+    if (rc) {
+        release(e);
+        return rc;
+    }
     *out = e;
     return ME_OK;
 }
@@ -723,9 +835,10 @@ int me_adopt(me_engine *e) {
 int me_set_bodies(me_engine *e, const me_bodies *bodies) {
     if (!e || !bodies) return fail(ME_EARG, "NULL argument");
     if (std::this_thread::get_id() != e->owner) return fail(ME_ETHREAD, "engine used from a thread that does not own it");
+    if (e->P > 0 && !bodies->rest) return fail(ME_EARG, "bead geometry needs rest positions");
     e->b = *bodies;
     drop_graphs(e);  // captured kernels hold the old body pointers
-    return ME_OK;
+    return bead_half_extents(e, nullptr);
 }
 
 int me_step(me_engine *e, double dt, int substeps, void *stream) {
@@ -780,7 +893,8 @@ int me_read_counters(me_engine *e, int64_t *out, void *stream) {
     int rc = read_stat(e, (hipStream_t)stream);
     if (rc) return rc;
     const int *h = e->hstat;
-    out[ME_CNT_CONTACTS] = h[IS_C];
+    out[ME_CNT_CONTACTS] = h[IS_CRAW];
+    out[ME_CNT_SOLVED] = h[IS_C];
     out[ME_CNT_PAIRS] = h[IS_P];
     out[ME_CNT_COLORS] = h[IS_NCOL];
     out[ME_CNT_CACHE_HITS] = h[IS_HITS];

@@ -62,6 +62,7 @@ struct AdjTape {
 
 struct TiledArgs {
     int N, capC, capP, solver, vit, pit, warm, n_colors, tile_bodies;
+    int warm_set;  // the contacts already carry their warm-start impulses (bead manifold stage)
     float t2d, dt;
     Bodies b;
     float *invIw, *pv, *po;
@@ -952,8 +953,9 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     if (A.taped)  // before the lookup: the warm store still holds the previous substep's contacts
         hipLaunchKernelGGL(k_tape_widx, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
                            A.wkey, A.tape.widx);
-    hipLaunchKernelGGL(k_warm_lookup, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
-                       A.wkey, A.wjn, A.wjt1, A.wjt2, A.Cc.jn, A.Cc.jt1, A.Cc.jt2, A.Cc.jp);
+    if (!A.warm_set)
+        hipLaunchKernelGGL(k_warm_lookup, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
+                           A.wkey, A.wjn, A.wjt1, A.wjt2, A.Cc.jn, A.Cc.jt1, A.Cc.jt2, A.Cc.jp);
     hipLaunchKernelGGL(k_t_init, dim3(tblk(N > ME_NCOLMAX + 1 ? N : ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, N, T,
                        A.istat);
     if (cc)

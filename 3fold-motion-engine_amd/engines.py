@@ -34,6 +34,19 @@ def _torch():
     return torch
 
 
+def bead_feature_keys(keys, n_bodies, n_beads):
+    """The reference's feature keys fa*(N*P+1) + fb + 1 (contact_engine_gpu_colored.py:903-917) of
+    bead contacts keyed pair-major (pairkey*P*(P+1) + la*(P+1) + lb + 1, pairkey = a*(N+1) + b + 1)."""
+    keys = np.asarray(keys, np.int64)
+    N, P = int(n_bodies), int(n_beads)
+    pk, rem = np.divmod(keys, np.int64(P * (P + 1)))
+    la, lb1 = np.divmod(rem, np.int64(P + 1))
+    a, b1 = np.divmod(pk, np.int64(N + 1))
+    fa = a * P + la
+    fb = np.where(b1 == 0, -1, (b1 - 1) * P + (lb1 - 1))
+    return fa * np.int64(N * P + 1) + fb + 1
+
+
 class _HipContactEngine:
     name = "hip_contact"
     _solver = _lib.SOLVER_JACOBI
@@ -229,7 +242,7 @@ class _HipContactEngine:
 
     # harness-visible counters (contact_engine_gpu_colored.py:824-829)
     n_contacts = property(lambda s: s._counters["n_contacts"])
-    n_contacts_solved = property(lambda s: s._counters["n_contacts"])
+    n_contacts_solved = property(lambda s: s._counters["n_solved"])
     n_pairs = property(lambda s: s._counters["n_pairs"])
     n_colors = property(lambda s: s._counters["n_colors"])
     color_cache_hits = property(lambda s: s._counters["cache_hits"])
@@ -278,8 +291,10 @@ class _HipContactEngine:
         return s.buffer_tensor(bid, dtype, cnt).cpu().numpy().reshape(shape)
 
     def contacts(s):
-        """Contact list of the last substep (sorted by feature key) with its impulses."""
-        Cn = s._counters["n_contacts"]
+        """Contact list of the last substep with its impulses: pair-major, sorted by key (box geometry:
+        pair key * 8192 + feature; bead geometry under a coloured solver: the pair-major composite key
+        of the bead features, see bead_feature_keys)."""
+        Cn = s._counters["n_solved"]
         b = lambda bid, dt, shape: s.buffer(bid, dt, shape)
         return dict(key=b(_lib.BUF_KEY, np.int64, (Cn,)), bi=b(_lib.BUF_BI, np.int32, (Cn,)),
                     bj=b(_lib.BUF_BJ, np.int32, (Cn,)), pA=b(_lib.BUF_PA, np.float32, (Cn, 3)),

@@ -54,7 +54,9 @@ extern "C" {
 typedef struct me_config {
     int32_t solver;        /* ME_SOLVER_* */
     int32_t n_bodies;      /* N */
-    int32_t n_beads;       /* 0 = box geometry; P > 0 = bead geometry (the reference's body model) */
+    int32_t n_beads;       /* 0 = box geometry; 0 < P <= 64 = bead geometry (the reference's body model).
+                            * Under the coloured solvers bead contacts are generated per body pair and
+                            * reduced to at most 4 per pair (E-optimal); needs pair chunks. */
     int32_t vit, pit;      /* velocity / position iterations */
     float relax;           /* Jacobi relaxation (reference 0.2) */
     float t2d;             /* pair-chunk friction t2 factor (reference 1.0) */
@@ -116,7 +118,7 @@ enum {
 
 /* counters for me_read_counters */
 enum {
-    ME_CNT_CONTACTS = 0, /* contacts generated in the last substep */
+    ME_CNT_CONTACTS = 0, /* contacts generated in the last substep (before a manifold reduction) */
     ME_CNT_PAIRS,        /* body pairs with contacts */
     ME_CNT_COLORS,       /* colours of the last solve */
     ME_CNT_CACHE_HITS,
@@ -127,6 +129,7 @@ enum {
     ME_CNT_TILES,        /* tiles of the last tiled solve */
     ME_CNT_MAX_TILE_BODIES,
     ME_CNT_GRAPH_CAPTURES, /* whole-step graphs captured so far (ME_FLAG_GRAPH) */
+    ME_CNT_SOLVED,       /* contacts solved in the last substep (after the bead manifold reduction) */
     ME_CNT_COUNT_
 };
 

@@ -1,0 +1,168 @@
+"""Bead geometry under the coloured solvers on the HIP path against the CPU oracle, through the C ABI:
+bead contacts per body pair (ballot over the P x P bead pairs), warm start and the E-optimal manifold
+reduction to 4 points per pair on the device, then the pair-chunk and ColorCache solves.
+
+Bar: bit-exact states every step, equal counters (generated and solved contacts, pairs, colours),
+and equal contact lists once the device's pair-major keys are mapped back to the reference's
+feature keys. The generated-contact counts are also checked against the reference's archived
+receipts. Every test prints its case before the GPU work starts."""
+import importlib
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def _pkg():
+    return importlib.import_module("3fold-motion-engine_amd")
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _bead_engine(kind, bodyset, dens, **kw):
+    pkg = _pkg()
+    cls = {"pair_chunks": pkg.GraphColoredContactEngine, "color_cache": pkg.ColorCacheContactEngine}[kind]
+    e = cls(vit=40, pit=10, **kw)
+    for c, d, k in zip(bodyset.x, dens, bodyset.kin):
+        e.add_body([float(t) for t in c], density=d, kin=bool(k))
+    e._build()
+    import torch
+    with torch.no_grad():
+        e.q.copy_(torch.from_numpy(np.ascontiguousarray(bodyset.q)))
+        e.v.copy_(torch.from_numpy(np.ascontiguousarray(bodyset.v)))
+        e.w.copy_(torch.from_numpy(np.ascontiguousarray(bodyset.w)))
+    return e
+
+
+def _assert_state_equal(e, o, tag):
+    xg, qg, vg, wg = e.state_numpy()
+    xo, qo, vo, wo = o.state()
+    for name, a, b in (("x", xg, xo), ("q", qg, qo), ("v", vg, vo), ("w", wg, wo)):
+        if not np.array_equal(a, b):
+            idx = np.argwhere(a != b)[0]
+            raise AssertionError(f"{tag}: {name} differs first at {idx.tolist()}: gpu {a[tuple(idx)]!r} "
+                                 f"oracle {b[tuple(idx)]!r} (max |d| {np.max(np.abs(a - b)):.3e})")
+
+
+def _assert_counters_equal(e, o, tag):
+    co = o.counters()
+    got = (e.n_contacts, e.n_contacts_solved, e.n_pairs, e.n_colors)
+    want = (co["n_generated"], co["n_contacts"], co["n_pairs"], co["n_colors"])
+    assert got == want, f"{tag}: gpu (generated, solved, pairs, colours) {got} oracle {want}"
+
+
+def _assert_bead_contacts_equal(e, o, tag):
+    pkg = _pkg()
+    cg, co = e.contacts(), o.contacts()
+    assert len(cg["key"]) == len(co["key"]), f"{tag}: contact count {len(cg['key'])} vs {len(co['key'])}"
+    fk = pkg.engines.bead_feature_keys(cg["key"], e.N, e.P)
+    assert np.all(np.diff(cg["key"]) > 0), f"{tag}: device keys not strictly ascending"
+    order = np.argsort(fk, kind="stable")
+    np.testing.assert_array_equal(fk[order], co["key"])
+    for k in ("bi", "bj", "pA", "pB", "n", "pen", "jn", "jt1", "jt2", "jp"):
+        assert np.array_equal(cg[k][order], co[k]), f"{tag}: contact field {k} differs"
+
+
+@pytest.mark.parametrize("kind,tiled", [("pair_chunks", False), ("color_cache", False), ("color_cache", True)])
+def test_bead_stack8_matches_oracle_and_archive_counts(oracle, kind, tiled):
+    _say(f"{kind} bead stack8 x120 tiled={tiled}")
+    pkg = _pkg()
+    meta = json.loads((GOLD / "reference_trajectories.json").read_text())
+    counts = meta["records"][f"{kind}__stack8"]["contacts"]
+    agree = {"pair_chunks": 60, "color_cache": 30}[kind]  # steps over which the oracle matches the archive
+    bs = pkg.bodies.bead_bodies(pkg.scenes.stack(8), [700.0] * 8, [0] * 8)
+    e = _bead_engine(kind, bs, [700.0] * 8, tiled=tiled)
+    o = oracle.OracleWorld(bs, kind, vit=40, pit=10)
+    for s in range(120):
+        e.step(1 / 240)
+        o.step(1 / 240)
+        _assert_state_equal(e, o, f"step {s}")
+        _assert_counters_equal(e, o, f"step {s}")
+        if s < agree:
+            assert e.n_contacts == counts[s], f"step {s}: generated {e.n_contacts} archive {counts[s]}"
+        if s in (3, 8, 40, 119):
+            _assert_bead_contacts_equal(e, o, f"step {s}")
+    np.testing.assert_array_equal(e.contact_forces(), o.forces())
+
+
+@pytest.mark.parametrize("kind,tiled", [("pair_chunks", False), ("color_cache", True)])
+def test_bead_lattice1000_matches_oracle(oracle, kind, tiled):
+    _say(f"{kind} bead lattice1000 x40 tiled={tiled}")
+    pkg = _pkg()
+    meta = json.loads((GOLD / "reference_trajectories.json").read_text())
+    counts = meta["records"][f"{kind}__lattice1000"]["contacts"]
+    agree = {"pair_chunks": 40, "color_cache": 26}[kind]
+    cen = pkg.scenes.lattice(1000)
+    bs = pkg.bodies.bead_bodies(cen, [700.0] * 1000, [0] * 1000)
+    e = _bead_engine(kind, bs, [700.0] * 1000, tiled=tiled)
+    o = oracle.OracleWorld(bs, kind, vit=40, pit=10)
+    for s in range(40):
+        e.step(1 / 240)
+        o.step(1 / 240)
+        _assert_state_equal(e, o, f"step {s}")
+        _assert_counters_equal(e, o, f"step {s}")
+        if s < agree:
+            assert e.n_contacts == counts[s], f"step {s}"
+    _assert_bead_contacts_equal(e, o, "final")
+
+
+def _jumble(seed=3):
+    """Two layers of bead boxes at random yaw and small tilt, touching sideways, with two kinematic
+    boxes among them (static-static and static-ground pairs stay in the pair list)."""
+    pkg = _pkg()
+    rng = np.random.default_rng(seed)
+    cen, kin = [], []
+    for lz in range(2):
+        for iy in range(4):
+            for ix in range(4):
+                cen.append([ix * 0.31 + rng.uniform(-0.02, 0.02), iy * 0.31 + rng.uniform(-0.02, 0.02),
+                            0.1 + lz * 0.215 + rng.uniform(0.0, 0.01)])
+                kin.append(0)
+    kin[5] = 1
+    kin[6] = 1
+    n = len(cen)
+    dens = list(rng.uniform(300.0, 1500.0, n))
+    bs = pkg.bodies.bead_bodies(cen, dens, kin)
+    yaw = rng.uniform(-0.3, 0.3, n)
+    tilt = rng.uniform(-0.05, 0.05, n)
+    q = np.zeros((n, 4))
+    q[:, 0] = np.sin(tilt / 2) * np.cos(yaw / 2)
+    q[:, 1] = np.sin(tilt / 2) * np.sin(yaw / 2)
+    q[:, 2] = np.cos(tilt / 2) * np.sin(yaw / 2)
+    q[:, 3] = np.cos(tilt / 2) * np.cos(yaw / 2)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    bs.q[:] = q.astype(np.float32)
+    kinmask = np.asarray(kin) == 1
+    v = rng.normal(0.0, 0.2, (n, 3)).astype(np.float32)
+    w = rng.normal(0.0, 0.5, (n, 3)).astype(np.float32)
+    v[kinmask] = 0.0
+    w[kinmask] = 0.0
+    bs.v[:] = v
+    bs.w[:] = w
+    return bs, dens
+
+
+@pytest.mark.parametrize("kind,tiled", [("pair_chunks", False), ("color_cache", False), ("color_cache", True)])
+def test_bead_jumble_with_kinematic_bodies_matches_oracle(oracle, kind, tiled):
+    _say(f"{kind} bead jumble (32 bodies, 2 kinematic) x80 tiled={tiled}")
+    bs, dens = _jumble()
+    e = _bead_engine(kind, bs, dens, tiled=tiled)
+    o = oracle.OracleWorld(bs, kind, vit=40, pit=10)
+    reduced = 0
+    for s in range(80):
+        e.step(1 / 240)
+        o.step(1 / 240)
+        _assert_state_equal(e, o, f"step {s}")
+        _assert_counters_equal(e, o, f"step {s}")
+        reduced += e.n_contacts > e.n_contacts_solved
+        if s % 20 == 19:
+            _assert_bead_contacts_equal(e, o, f"step {s}")
+    assert reduced > 0  # the reduction ran
+    np.testing.assert_array_equal(e.contact_forces(), o.forces())
