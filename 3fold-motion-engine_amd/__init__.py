@@ -2,7 +2,7 @@
 reference ContactEngine protocol (antonbj3/3fold-motion-engine, src/motion_engine/contact_engine.py)."""
 from .protocol import BodyState, ContactEngine
 from .engines import ColorCacheContactEngine, GraphColoredContactEngine, RelaxedJacobiContactEngine
-from . import adjoint, bodies, scenes
+from . import adjoint, bodies, ncp, scenes
 
 __all__ = ["BodyState", "ContactEngine", "RelaxedJacobiContactEngine", "GraphColoredContactEngine",
-           "ColorCacheContactEngine", "adjoint", "bodies", "scenes"]
+           "ColorCacheContactEngine", "adjoint", "bodies", "ncp", "scenes"]

@@ -16,7 +16,7 @@ FLAG_TILED, FLAG_PER_CONTACT, FLAG_GRAPH = 1, 2, 4
 
 (BUF_KEY, BUF_BI, BUF_BJ, BUF_PA, BUF_PB, BUF_N, BUF_PEN, BUF_JN, BUF_JT1, BUF_JT2, BUF_JP,
  BUF_PBI, BUF_PBJ, BUF_PSTART, BUF_PCOUNT, BUF_COLOR, BUF_VSNAP, BUF_PV, BUF_PO, BUF_INVIW,
- BUF_ISLAND, BUF_DEBUG, BUF_BIG_WORK) = range(23)
+ BUF_ISLAND, BUF_DEBUG, BUF_BIG_WORK, BUF_MU, BUF_XSNAP, BUF_WSNAP) = range(26)
 
 COUNTERS = ("n_contacts", "n_pairs", "n_colors", "cache_hits", "cache_misses", "overflow",
             "n_candidates", "n_islands", "n_tiles", "max_tile_bodies", "graph_captures", "n_solved")
@@ -62,6 +62,9 @@ def lib():
         L.me_tape_size.argtypes = [vp, C.POINTER(C.c_size_t)]
         L.me_adopt.argtypes = [vp]
         L.me_state_size.argtypes = [vp, C.POINTER(C.c_size_t)]
+        L.me_ncp_workspace_size.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_size_t)]
+        L.me_ncp_sensitivity.argtypes = [C.c_int32, C.c_int32, C.c_int32] + [vp] * 6 + [C.c_double] * 3 + \
+            [vp] * 5 + [C.c_size_t, vp]
         L.me_step_taped.argtypes = [vp, C.c_double, vp, C.c_size_t, vp]
         L.me_adjoint_substep.argtypes = [vp, C.c_double, vp, C.c_size_t] + [vp] * 7 + [vp]
         L.me_last_error.restype = C.c_char_p
@@ -83,5 +86,6 @@ def check(rc: int, what: str) -> None:
 
 EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
             "me_buffer", "me_read_counters", "me_profile", "me_profile_read", "me_last_error",
-            "me_tape_size", "me_step_taped", "me_adjoint_substep", "me_adopt", "me_state_size")
+            "me_tape_size", "me_step_taped", "me_adjoint_substep", "me_adopt", "me_state_size",
+            "me_ncp_workspace_size", "me_ncp_sensitivity")
 K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL, K_TILE_BIG = range(6)

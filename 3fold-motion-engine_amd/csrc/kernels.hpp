@@ -158,8 +158,8 @@ __device__ __forceinline__ void wave_atomic_count(int *addr, bool pred) {
     if (__lane_id() == 0 && m) atomicAdd(addr, (int)__popcll(m));
 }
 
-__global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb, float *vsnap, int *istat,
-                           float dt, int box_mode) {
+__global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb, float *vsnap, float *xsnap,
+                           float *wsnap, int *istat, float dt, int box_mode) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     float mext = 0.0f;
     bool has = false;
@@ -170,6 +170,8 @@ __global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb
             st3(b.v, i, v);
         }
         st3(vsnap, i, v);
+        st3(xsnap, i, ld3(b.x, i));
+        st3(wsnap, i, ld3(b.w, i));
         m3 R = quat_to_matrix(ld4(b.q, i));  // k_invIw (contact_engine_gpu.py:47-53)
         stm(Rw, i, R);
         if (b.kin[i] == 1) {

@@ -13,6 +13,7 @@
 #include "kernels.hpp"
 #include "tiled.hpp"
 #include "beads.hpp"
+#include "ncp.hpp"
 #include "adjoint_kernels.hpp"
 
 using namespace me;
@@ -68,7 +69,7 @@ struct me_engine {
     int cap_cand, capC, capP;
     std::thread::id owner;
     // per body
-    float *invIw, *Rw, *aabb, *pv, *po, *vsnap, *allp;
+    float *invIw, *Rw, *aabb, *pv, *po, *vsnap, *xsnap, *wsnap, *allp;
     int64_t *acc;
     unsigned long long *claim;
     // grid
@@ -217,6 +218,8 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->Rw = B.take<float>(9 * (size_t)N);
     e->aabb = B.take<float>(6 * (size_t)N);
     e->vsnap = B.take<float>(3 * (size_t)N);
+    e->xsnap = B.take<float>(3 * (size_t)N);
+    e->wsnap = B.take<float>(3 * (size_t)N);
     e->allp = B.take<float>(3 * (size_t)(P > 0 ? NP : 1));
     e->acc = B.take<int64_t>(6 * (size_t)N);
     e->claim = B.take<unsigned long long>((size_t)N);
@@ -585,7 +588,7 @@ int substep(me_engine *e, double sdt64, hipStream_t s) {
         hipLaunchKernelGGL(k_fill_half, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->bhalf[0], e->bhalf[1], e->bhalf[2],
                            e->bp_half);
     hipLaunchKernelGGL(k_body_pre, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->invIw, e->Rw, e->aabb, e->vsnap,
-                       e->istat, sdt, (e->P == 0 || e->bead_pairs) ? 1 : 0);
+                       e->xsnap, e->wsnap, e->istat, sdt, (e->P == 0 || e->bead_pairs) ? 1 : 0);
     LAUNCH_CHECK();
     int rc = generate(e, s);
     if (rc) return rc;
@@ -883,6 +886,9 @@ int me_buffer(me_engine *e, int id, void **ptr, int64_t *count) {
         case ME_BUF_ISLAND: *ptr = e->tb.root; *count = N; break;
         case ME_BUF_DEBUG: *ptr = e->tb.dbg; *count = 16; break;
         case ME_BUF_BIG_WORK: *ptr = e->tb.bigw; *count = 4; break;
+        case ME_BUF_MU: *ptr = e->Cc.mu; *count = C; break;
+        case ME_BUF_XSNAP: *ptr = e->xsnap; *count = 3 * N; break;
+        case ME_BUF_WSNAP: *ptr = e->wsnap; *count = 3 * N; break;
         default: return fail(ME_EARG, "unknown buffer id");
     }
     return ME_OK;
@@ -934,6 +940,47 @@ int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launche
     }
     *total_ms = tot;
     *launches = n;
+    return ME_OK;
+}
+
+// NCP sensitivity: LDS holds the augmented matrix up to this size, else the caller's scratch does
+static const size_t kNcpLds = 64 * 1024;
+
+static size_t ncp_matrix_bytes(int n_c, int n_dir) {
+    size_t n3 = 3 * (size_t)n_c;
+    return n3 * (n3 + (size_t)n_c + (size_t)n_dir) * sizeof(double);
+}
+
+int me_ncp_workspace_size(int32_t n_env, int32_t n_c, int32_t n_dir, size_t *bytes) {
+    if (!bytes) return fail(ME_EARG, "bytes is NULL");
+    if (n_env < 1 || n_c < 1 || n_c > NCP_MAXC || n_dir < 0)
+        return fail(ME_EARG, "need n_env >= 1, 1 <= n_c <= 64, n_dir >= 0");
+    size_t m = ncp_matrix_bytes(n_c, n_dir);
+    *bytes = m <= kNcpLds ? 0 : m * (size_t)n_env;
+    return ME_OK;
+}
+
+int me_ncp_sensitivity(int32_t n_env, int32_t n_c, int32_t n_dir, const double *G, const double *b,
+                       const double *mu, const double *lam, const double *dirs, const int32_t *labels_in,
+                       double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
+                       int32_t *labels, int32_t *status, void *workspace, size_t bytes, void *stream) {
+    if (n_env < 1 || n_c < 1 || n_c > NCP_MAXC || n_dir < 0)
+        return fail(ME_EARG, "need n_env >= 1, 1 <= n_c <= 64, n_dir >= 0");
+    if (!dirs && n_dir != 3 * n_c) return fail(ME_EARG, "without directions n_dir must be 3 * n_c (the identity)");
+    if (!G || !b || !mu || !lam || !dlam_dmu || (n_dir > 0 && !dlam_db) || !labels || !status)
+        return fail(ME_EARG, "NULL argument");
+    if (!(tol_lam >= 0.0) || !(tol_rel >= 0.0) || !(rcond > 0.0 && rcond < 1.0))
+        return fail(ME_EARG, "need tol_lam >= 0, tol_rel >= 0, 0 < rcond < 1");
+    size_t m = ncp_matrix_bytes(n_c, n_dir), need = 0;
+    (void)me_ncp_workspace_size(n_env, n_c, n_dir, &need);
+    if (need > 0 && (!workspace || bytes < need))
+        return fail(ME_EARG, "workspace too small: need " + std::to_string(need) + " bytes");
+    int lds = m <= kNcpLds ? 1 : 0;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_ncp_sensitivity, dim3(n_env), dim3(NCP_THREADS), lds ? m : 0, s, n_env, n_c, n_dir, G, b, mu,
+                       lam, dirs, labels_in, tol_lam, tol_rel, rcond, dlam_dmu, dlam_db, labels, status,
+                       (double *)workspace, lds);
+    LAUNCH_CHECK();
     return ME_OK;
 }
 

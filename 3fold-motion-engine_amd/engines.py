@@ -301,7 +301,7 @@ class _HipContactEngine:
                     pB=b(_lib.BUF_PB, np.float32, (Cn, 3)), n=b(_lib.BUF_N, np.float32, (Cn, 3)),
                     pen=b(_lib.BUF_PEN, np.float32, (Cn,)), jn=b(_lib.BUF_JN, np.float32, (Cn,)),
                     jt1=b(_lib.BUF_JT1, np.float32, (Cn,)), jt2=b(_lib.BUF_JT2, np.float32, (Cn,)),
-                    jp=b(_lib.BUF_JP, np.float32, (Cn,)))
+                    jp=b(_lib.BUF_JP, np.float32, (Cn,)), mu=b(_lib.BUF_MU, np.float32, (Cn,)))
 
     def pairs(s):
         P = s._counters["n_pairs"]

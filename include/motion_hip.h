@@ -113,6 +113,9 @@ enum {
     ME_BUF_DEBUG,      /* int64 [16]  phase timestamps of the largest big tile (wall clock) */
     ME_BUF_BIG_WORK,   /* int64 [4]   big-tile work since me_profile(e, 1): tiles, pairs, dynamic pair
                                       ends, contacts (for the algorithmic bytes of ME_K_TILE_BIG) */
+    ME_BUF_MU,         /* float [C]   friction coefficient of the contact's pair */
+    ME_BUF_XSNAP,      /* float [N,3] centres at the start of the last substep */
+    ME_BUF_WSNAP,      /* float [N,3] angular velocity before the contact solve */
     ME_BUF_COUNT_
 };
 
@@ -190,6 +193,25 @@ int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launche
  * what me_adjoint_substep reads of a recorded substep; the rest is scratch every substep rewrites
  * before reading. A copy of these bytes plus the body arrays is a complete checkpoint. */
 int me_state_size(me_engine *e, size_t *bytes);
+
+/* NCP-level sensitivity, batched, float64 (replaces the reference's BatchProxADMMGPU.build_active +
+ * grad_affine, ncp/ncp_batch_gpu.py:1160-1187 with kernels k_active :555-597 and k_solvesig :716-767,
+ * and its CPU form ncp/ncp_ref.py: classify :251, _active_system :786, sensitivity :907).
+ * n_env problems of n_c <= 64 contacts each, row-major per environment:
+ *   G [3n_c, 3n_c] Delassus operator, b [3n_c] free velocity, mu [n_c], lam [3n_c] solved impulse
+ *   (n, t1, t2 per contact), dirs [3n_c, n_dir] directions of b (NULL: the identity, n_dir = 3n_c),
+ *   labels_in [n_c] (NULL: classify with tol_lam, tol_rel as ncp_ref.classify).
+ * Outputs dlam_dmu [3n_c, n_c], dlam_db [3n_c, n_dir] = (dlam/db) dirs, labels [n_c] (bits 0-1:
+ * 0 open, 1 stick, 2 slip; bits 4-7: branch-boundary reasons), status (bit 0 singular: a pivot below
+ * rcond * max|A|; bit 1 a branch boundary; bit 2 non-finite input). Derivatives are NaN where status
+ * is non-zero, as the reference returns them without a minimum-norm selection. No engine and no
+ * thread ownership; everything on `stream`. workspace: me_ncp_workspace_size bytes (0 when the
+ * per-environment system fits LDS). */
+int me_ncp_workspace_size(int32_t n_env, int32_t n_c, int32_t n_dir, size_t *bytes);
+int me_ncp_sensitivity(int32_t n_env, int32_t n_c, int32_t n_dir, const double *G, const double *b,
+                       const double *mu, const double *lam, const double *dirs, const int32_t *labels_in,
+                       double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
+                       int32_t *labels, int32_t *status, void *workspace, size_t bytes, void *stream);
 
 /* Bytes of the caller-owned tape block (256-byte aligned device memory) for this engine. */
 int me_tape_size(me_engine *e, size_t *bytes);
