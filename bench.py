@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large", "adjoint"])
+    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large", "adjoint", "ncp"])
     ap.add_argument("--rollouts", type=int, default=3, help="adjoint workload: timed forward+backward rollouts")
     ap.add_argument("--extras", type=int, default=1,
                     help="pile workload: also measure the 256-scene batch and the config-4 adjoint and attach "
@@ -96,6 +96,9 @@ def main():
         return
     if a.workload == "adjoint":
         print(json.dumps(bench_adjoint(a)), flush=True)
+        return
+    if a.workload == "ncp":
+        print(json.dumps(bench_ncp(a)), flush=True)
         return
     if a.workload == "large":
         a.bodies = 100000 if a.bodies == 10000 else a.bodies
@@ -233,6 +236,8 @@ def main():
         torch.cuda.empty_cache()
         out["adjoint"] = _extra(bench_adjoint, a, rollouts=2,
                                 keys=("value", "ms_per_step", "adjoint", "config"))
+        out["ncp_sensitivity"] = _extra(bench_ncp, a, keys=("value", "unit", "ms_per_call", "vs_reference",
+                                                            "config", "cpu_baseline"))
     print(json.dumps(out), flush=True)
     if a.calibrate_copy:
         _copy_kernel(a.calibrate_copy)
@@ -323,6 +328,64 @@ def bench_adjoint(a):
                        "grad_norms_v0_w0_mu": gnorm},
            "roofline": None, "cpu_baseline": None}
     return out
+
+
+def bench_ncp(a):
+    """NCP-level sensitivity, the reference's batched adjoint workload: 4096 stance problems of 4 point
+    contacts on 18 generalised coordinates (synthetic G, b, mu with exactly known solutions, every
+    contact drawn open / stick / slip), dlam/db along 2 directions per problem plus dlam/dmu of every
+    contact, float64, inputs resident on the GPU; one me_ncp_sensitivity launch per call. The
+    reference's number is 119.6 ms per 2 directions at B = 4096 (68 488 env-grads/s, RTX 5070)."""
+    import ctypes as C
+    import torch
+    ncp = importlib.import_module("3fold-motion-engine_amd.ncp")
+    L = importlib.import_module("3fold-motion-engine_amd._lib")
+    E, n_c, m = 4096, 4, 2
+    G, b, mu, lam, labels = ncp.synthetic_problems(E, n_c=n_c, seed=a.seed)
+    D = np.random.default_rng(a.seed + 1).standard_normal((E, 3 * n_c, m))
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.tensor(np.asarray(x), dtype=torch.float64, device=dev).contiguous()
+    Gd, bd, md, ld, Dd = t(G), t(b), t(mu), t(lam), t(D)
+    dmu = torch.empty((E, 3 * n_c, n_c), dtype=torch.float64, device=dev)
+    dlb = torch.empty((E, 3 * n_c, m), dtype=torch.float64, device=dev)
+    lab = torch.empty((E, n_c), dtype=torch.int32, device=dev)
+    st = torch.empty(E, dtype=torch.int32, device=dev)
+    lib = L.lib()
+    nb = C.c_size_t()
+    L.check(lib.me_ncp_workspace_size(E, n_c, m, C.byref(nb)), "me_ncp_workspace_size")
+    ws = torch.empty(max(int(nb.value), 8), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    p = lambda x: C.c_void_p(x.data_ptr())
+    args = (E, n_c, m, p(Gd), p(bd), p(md), p(ld), p(Dd), None, 1e-10, 1e-6, 1e-12, p(dmu), p(dlb), p(lab),
+            p(st), p(ws), int(nb.value), C.c_void_p(stream))
+    for _ in range(20):
+        L.check(lib.me_ncp_sensitivity(*args), "me_ncp_sensitivity")
+    torch.cuda.synchronize()
+    reps = 200
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        L.check(lib.me_ncp_sensitivity(*args), "me_ncp_sensitivity")
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    assert np.array_equal(lab.cpu().numpy(), labels) and int(st.abs().sum()) == 0
+    value = E * m / (ms * 1e-3)
+    cpu = None
+    if not a.no_cpu:
+        from oracle import ncp_oracle as R
+        n_s, t0 = 256, time.perf_counter()
+        for e in range(n_s):
+            R.sensitivity(lam[e], G[e], b[e], mu[e], [R.LABELS[k] for k in labels[e]], db_dparam=D[e])
+        dt_cpu = time.perf_counter() - t0
+        cpu = {"value": round(n_s * m / dt_cpu, 1), "unit": "env-grads/s", "cores": 1, "kind": "port",
+               "sample": f"{n_s} of the {E} problems through the numpy oracle (SVD rank guard + LAPACK solve)"}
+    return {"metric": "NCP sensitivity env-grads/s (B=4096, n_c=4, 2 directions)", "value": round(value, 1),
+            "unit": "env-grads/s", "ms_per_call": round(ms, 5), "higher_is_better": True,
+            "vs_reference": round(value / 68488.0, 1), "dtype": "f64", "data": "synthetic (exactly solved problems)",
+            "config": {"workload": f"{E} stance problems x {n_c} contacts, nv 18, dlam/db along {m} directions + "
+                                   f"dlam/dmu, one launch", "envs": E, "contacts": n_c, "directions": m},
+            "cpu_baseline": cpu}
 
 
 def bench_batched(a):

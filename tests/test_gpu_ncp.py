@@ -116,3 +116,67 @@ def test_large_system_uses_global_scratch():
         dm_o, db_o, info_o = R.sensitivity(lam[e], G[e], b[e], mu[e], labs)
         _close(dmu[e], dm_o, 1e-8)
         _close(dlb[e], db_o, 1e-8)
+
+
+def _corner_boxes():
+    """Two boxes tilted about x and y so that one corner each touches the ground, 1 mm deep, with
+    small horizontal velocities and high friction: one sticking contact per box, a non-redundant
+    contact system."""
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    h = np.array([0.1, 0.08, 0.06])
+    ax, ay = 0.35, 0.28
+    qx = np.array([np.sin(ax / 2), 0, 0, np.cos(ax / 2)])
+    qy = np.array([0, np.sin(ay / 2), 0, np.cos(ay / 2)])
+    x1, y1, z1, w1 = qy
+    x2, y2, z2, w2 = qx
+    q = np.array([w1 * x2 + w2 * x1 + y1 * z2 - y2 * z1, w1 * y2 + w2 * y1 + z1 * x2 - z2 * x1,
+                  w1 * z2 + w2 * z1 + x1 * y2 - x2 * y1, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2])
+    Rm = pkg.bodies.quat_to_R_host(q[None].astype(np.float32))[0].astype(np.float64)
+    corners = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]) * h
+    zlow = np.sort((corners @ Rm.T)[:, 2])
+    assert zlow[1] - zlow[0] > 0.01  # one lowest corner
+    cz = -zlow[0] - 1e-3
+    return pkg.bodies.box_bodies([[0.0, 0.0, cz], [1.0, 0.0, cz]], h, 500.0, quats=[q, q], mu=2.0,
+                                 v=[[0.05, 0.02, 0.0], [-0.03, 0.04, 0.0]], w=[[0.1, 0.0, 0.0], [0.0, -0.2, 0.0]])
+
+
+def test_contact_step_problem_and_rollout_adjoint_match_the_ift():
+    """One ColorCache step of one substep (vit 300, no position sweeps) of two boxes on a corner each:
+    the extracted problem (G, b, mu, lam) satisfies the stick branch to float32 rounding, and the
+    hand-written rollout adjoint of x_T w.r.t. (v0, w0) equals the composition through the NCP
+    sensitivity, dt (I + J^T S^T J M^-1) xbar with S = dlam/db, to 1e-3 relative (float32 solve)."""
+    _say("ncp contact-step problem + rollout adjoint vs IFT (2 corner boxes)")
+    import torch
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    ncp = _ncp()
+    bs = _corner_boxes()
+    dt = 1.0 / 240
+    ro = pkg.adjoint.Rollout(bs, steps=1, dt=dt, substeps=1, vit=300, pit=0, n_colors=0, k=1)
+    dev = ro.eng.x.device
+    v0 = torch.from_numpy(bs.v).to(dev)
+    w0 = torch.from_numpy(bs.w).to(dev)
+    mu = torch.from_numpy(bs.mu).to(dev)
+    ro.run(v0, w0, mu)
+    prob = ncp.contact_problem(ro.eng)
+    G, b, mu_c, lam = prob["G"], prob["b"], prob["mu"], prob["lam"]
+    assert len(mu_c) == 2
+    labels = ncp.classify(lam, G, b, mu_c)
+    assert list(labels) == ["stick", "stick"]
+    u = G @ lam + b
+    assert np.max(np.abs(u)) <= 1e-5 * max(1.0, np.max(np.abs(b)))
+    dm, S, info = ncp.sensitivity(lam, G, b, mu_c, labels)
+    assert not info["singular"]
+    dm_o, S_o, _ = R.sensitivity(lam, G, b, mu_c, list(labels))
+    _close(S, S_o)
+    xbar = np.random.default_rng(1).standard_normal((2, 3))
+    vb, wb, mb = ro.grad(torch.from_numpy(xbar.astype(np.float32)).to(dev))
+    J, Minv, bodies = prob["J"], prob["Minv"], prob["bodies"]
+    X = np.zeros(J.shape[1])
+    for k, body in enumerate(bodies):
+        X[6 * k:6 * k + 3] = xbar[body]
+    g = dt * (X + J.T @ (S.T @ (J @ (Minv @ X))))
+    want = np.concatenate([np.concatenate([g[6 * k:6 * k + 3], g[6 * k + 3:6 * k + 6]]) for k in range(len(bodies))])
+    got = np.concatenate([np.concatenate([vb[body].cpu().numpy(), wb[body].cpu().numpy()]) for body in bodies])
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print(f"  rollout adjoint vs IFT composition: max rel err {err:.3e}", flush=True)
+    assert err <= 1e-3
