@@ -269,6 +269,4 @@ __global__ void k_bp_compact(int cap, const int *istat, const int *cand_a, const
     }
 }
 
-__global__ void k_copy_stat(int *istat, int dst, int src) { istat[dst] = istat[src]; }
-
 }  // namespace me

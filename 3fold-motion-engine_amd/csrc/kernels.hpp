@@ -158,9 +158,14 @@ __device__ __forceinline__ void wave_atomic_count(int *addr, bool pred) {
     if (__lane_id() == 0 && m) atomicAdd(addr, (int)__popcll(m));
 }
 
+// Also clears, for the grid built next, its T bucket counters and cursors (grid-stride), and for the
+// coloured solvers the pseudo-velocities pv, po (zeroed every substep, contact_engine_gpu_colored.py:1330).
 __global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb, float *vsnap, float *xsnap,
-                           float *wsnap, int *istat, float dt, int box_mode) {
+                           float *wsnap, int *istat, float dt, int box_mode, int T, int *bcount, int *bcursor,
+                           float *pv, float *po) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int k = i; k < T; k += gridDim.x * blockDim.x) { bcount[k] = 0; bcursor[k] = 0; }
+    if (pv && i < N) { st3(pv, i, mk(0.0f, 0.0f, 0.0f)); st3(po, i, mk(0.0f, 0.0f, 0.0f)); }
     float mext = 0.0f;
     bool has = false;
     if (i < N) {
@@ -203,11 +208,11 @@ __global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb
 
 // cell size of the body grid: (2 * max half extent + margin + slack) * 1.001, so that any two
 // overlapping AABBs have centre cells at most one apart even after rounding of x / cell
-__global__ void k_grid_cell(int *istat, float margin) {
+// the box grid's cell: twice the largest AABB half extent of the substep plus the margin, padded
+__device__ __forceinline__ float box_cell(const int *istat, float margin) {
     float hmax = __int_as_float(istat[IS_MAXEXT]);
     float cell = (2.0f * hmax + margin + ME_BP_SLACK) * 1.001f;
-    if (!(cell > 0.0f)) cell = 1.0f;
-    istat[IS_CELL] = __float_as_int(cell);
+    return cell > 0.0f ? cell : 1.0f;
 }
 
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, int s) {
@@ -217,10 +222,10 @@ __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, int s) {
 
 // hash grid: bucket per item (box centre or bead point), bucket counts
 __global__ void k_grid_count(int n, const float *pos, const int *scene_of, int div, const int *istat, float cell_fixed,
-                             int T, int *cellc, int *bucket, int *bcount) {
+                             float margin, int T, int *cellc, int *bucket, int *bcount) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float cell = cell_fixed > 0.0f ? cell_fixed : __int_as_float(istat[IS_CELL]);
+    float cell = cell_fixed > 0.0f ? cell_fixed : box_cell(istat, margin);
     f3 p = ld3(pos, i);
     int cx = (int)floorf(p.x / cell), cy = (int)floorf(p.y / cell), cz = (int)floorf(p.z / cell);
     int s = scene_of ? scene_of[i / div] : 0;
@@ -242,63 +247,66 @@ __device__ __forceinline__ bool aabb_overlap(const float *a, const float *b, flo
 }
 
 // ── A6 body pairs: ground candidate + AABB overlaps with higher bodies of the same scene, in key order ──
-// broad phase in two parallel passes: one thread per (body, neighbour cell) collects the higher
-// bodies of that cell whose AABB overlaps (k_box_cell); one thread per body merges its 27 lists
-// in ascending order behind its ground candidate (k_box_merge). all_pairs (bead geometry): pairs of
-// two static bodies and ground pairs of static bodies are kept, as the reference's bead contact
-// generation emits their contacts too (contact_engine_gpu_colored.py:103-122)
+// One wave per body a: lane l < 27 scans neighbour cell l's grid bucket for higher bodies of the same
+// scene whose AABB overlaps a's (at most ME_CELLCAND per cell); a wave prefix sum places the hits in
+// LDS and every lane ranks one hit among them, so the list comes out ascending behind the ground
+// candidate (the pair-key order a*(N+1) + b + 1). all_pairs (bead geometry): pairs of two static
+// bodies and ground pairs of static bodies are kept, as the reference's bead contact generation
+// emits their contacts too (contact_engine_gpu_colored.py:103-122).
 #define ME_CELLCAND 16
-__global__ void k_box_cell(int N, Bodies b, const float *aabb, const int *cellc, const int *bstart, const int *bcount,
-                           const int *items, int T, float margin, int all_pairs, int *cc_n, int *cc_list, int *istat) {
-    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= 27LL * N) return;
-    int a = (int)(gid / 27), cl = (int)(gid % 27);
-    int X = cellc[4 * a] + cl % 3 - 1, Y = cellc[4 * a + 1] + (cl / 3) % 3 - 1, Z = cellc[4 * a + 2] + cl / 9 - 1;
-    int s = cellc[4 * a + 3];
+__global__ void __launch_bounds__(64) k_box_pairs(int N, Bodies b, const float *aabb, const int *cellc,
+                                                  const int *bstart, const int *bcount, const int *items, int T,
+                                                  float margin, int all_pairs, int *cand_n, int *cand_tmp,
+                                                  int *istat) {
+    __shared__ int hits[ME_MAXCAND];
+    int a = blockIdx.x, lane = threadIdx.x;
+    if (a >= N) return;  // uniform: one body per block
     float m = margin + ME_BP_SLACK;
     const float *A = aabb + 6 * a;
     bool dynA = b.invM[a] > 0.0f;
-    int bk = (int)(cell_hash(X, Y, Z, s) & (uint32_t)(T - 1));
-    int st = bstart[bk], cnt = bcount[bk];
+    int mine[ME_CELLCAND];
     int n = 0;
-    for (int t = 0; t < cnt; ++t) {
-        int j = items[st + t];
-        if (j <= a) continue;
-        if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z || cellc[4 * j + 3] != s) continue;
-        if (!all_pairs && !dynA && !(b.invM[j] > 0.0f)) continue;
-        if (!aabb_overlap(A, aabb + 6 * j, m)) continue;
-        if (n < ME_CELLCAND) cc_list[gid * ME_CELLCAND + n] = j;
-        else atomicOr(&istat[IS_OVF], OVF_CAND);
-        n++;
-    }
-    cc_n[gid] = n < ME_CELLCAND ? n : ME_CELLCAND;
-}
-__global__ void k_box_merge(int N, Bodies b, const float *aabb, float margin, int all_pairs, const int *cc_n,
-                            const int *cc_list, int *cand_n, int *cand_tmp, int *istat) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= N) return;
-    int list[ME_MAXCAND];
-    int nc = 0;
     bool over = false;
-    if ((all_pairs || b.invM[a] > 0.0f) && aabb[6 * a + 2] < margin + ME_BP_SLACK) list[nc++] = -1;
-    int first = nc;
-    for (int cl = 0; cl < 27; ++cl) {
-        long long g = 27LL * a + cl;
-        int n = cc_n[g];
-        for (int t = 0; t < n; ++t) {
-            int j = cc_list[g * ME_CELLCAND + t];
-            if (nc < ME_MAXCAND) {
-                int k = nc++;
-                while (k > first && list[k - 1] > j) { list[k] = list[k - 1]; --k; }
-                list[k] = j;
-            } else {
-                over = true;
-            }
+    if (lane < 27) {
+        int X = cellc[4 * a] + lane % 3 - 1, Y = cellc[4 * a + 1] + (lane / 3) % 3 - 1, Z = cellc[4 * a + 2] + lane / 9 - 1;
+        int s = cellc[4 * a + 3];
+        int bk = (int)(cell_hash(X, Y, Z, s) & (uint32_t)(T - 1));
+        int st = bstart[bk], cnt = bcount[bk];
+        for (int t = 0; t < cnt; ++t) {
+            int j = items[st + t];
+            if (j <= a) continue;
+            if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z || cellc[4 * j + 3] != s) continue;
+            if (!all_pairs && !dynA && !(b.invM[j] > 0.0f)) continue;
+            if (!aabb_overlap(A, aabb + 6 * j, m)) continue;
+            if (n < ME_CELLCAND) mine[n++] = j;
+            else over = true;
         }
     }
-    if (over) atomicOr(&istat[IS_OVF], OVF_CAND);
-    cand_n[a] = nc;
-    for (int k = 0; k < nc; ++k) cand_tmp[a * ME_MAXCAND + k] = list[k];
+    int first = ((all_pairs || dynA) && A[2] < m) ? 1 : 0;
+    int incl = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    int total = __shfl(incl, 63);
+    int off = first + incl - n;
+    for (int k = 0; k < n; ++k)
+        if (off + k < ME_MAXCAND) hits[off + k] = mine[k];
+    over = over || first + total > ME_MAXCAND;
+    __syncthreads();
+    int nc = first + total < ME_MAXCAND ? first + total : ME_MAXCAND;
+    if (lane >= first && lane < nc) {
+        int c = hits[lane], rank = first;
+        for (int t = first; t < nc; ++t) rank += hits[t] < c;
+        cand_tmp[a * ME_MAXCAND + rank] = c;
+    }
+    if (lane == 0) {
+        if (first) cand_tmp[a * ME_MAXCAND] = -1;
+        cand_n[a] = nc;
+        if (a == 0) istat[IS_MAXEXT] = 0;  // read by this substep's grid only; the next k_body_pre maxes into it
+    }
+    if (__any(over) && lane == 0) atomicOr(&istat[IS_OVF], OVF_CAND);
 }
 
 __global__ void k_box_cand_emit(int N, const int *cand_n, const int *cand_off, const int *cand_tmp, int cap,
@@ -418,7 +426,9 @@ __global__ void k_pairs_per_contact(int capC, int *istat, const int *bi, const i
 }
 
 // clamp counts after compaction so later kernels stay in bounds
-__global__ void k_clamp_counts(int *istat, int capC, int capP) {
+// generated = solved unless a reduction ran (copy_raw)
+__global__ void k_clamp_counts(int *istat, int capC, int capP, int copy_raw) {
+    if (copy_raw) istat[IS_CRAW] = istat[IS_C];
     if (istat[IS_C] > capC) { istat[IS_C] = capC; atomicOr(&istat[IS_OVF], OVF_CONTACTS); }
     if (istat[IS_P] > capP) { istat[IS_P] = capP; atomicOr(&istat[IS_OVF], OVF_PAIRS); }
 }
@@ -849,12 +859,6 @@ __global__ void k_sweep_pos(SweepArgs S, int ci, float dt) {
     chunk_pos(S.B, S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p], dt);
 }
 
-__global__ void k_zero_pvpo(int N, float *pv, float *po) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    st3(pv, i, mk(0.0f, 0.0f, 0.0f));
-    st3(po, i, mk(0.0f, 0.0f, 0.0f));
-}
 
 // A11 integrate (contact_engine_gpu.py:172-180)
 __global__ void k_integrate(int N, Bodies b, const float *pv, const float *po, float dt) {

@@ -76,7 +76,6 @@ struct me_engine {
     int *cellc, *bucket, *bcount, *bstart, *bcursor, *items;
     // candidates and narrow-phase slots
     int *cand_n, *cand_off, *cand_tmp, *cand_a, *cand_b;
-    int *cc_n, *cc_list;  // per (body, neighbour cell) candidate lists
     Slots S;
     int *slot_off, *pair_flag, *pair_off;
     // contacts, pairs
@@ -232,8 +231,6 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->cand_n = B.take<int>((size_t)items);
     e->cand_off = B.take<int>((size_t)items);
     e->cand_tmp = B.take<int>(point_grid ? 1 : (size_t)N * ME_MAXCAND);
-    e->cc_n = B.take<int>(point_grid ? 1 : 27 * (size_t)N);
-    e->cc_list = B.take<int>(point_grid ? 1 : 27 * (size_t)ME_CELLCAND * N);
     e->S.key = B.take<int64_t>(ns);
     e->S.p = B.take<float>(3 * ns);
     e->S.nrm = B.take<float>(3 * ns);
@@ -314,10 +311,8 @@ int read_stat(me_engine *e, hipStream_t s) {
 
 // grid over n items with positions pos; cell from IS_CELL (cell_fixed <= 0) or fixed
 int build_grid(me_engine *e, int n, const float *pos, const int *scene, int div, float cell_fixed, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(e->T)), dim3(ME_THREADS), 0, s, e->bcount, e->T, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(e->T)), dim3(ME_THREADS), 0, s, e->bcursor, e->T, 0);
     hipLaunchKernelGGL(k_grid_count, dim3(nblk(n)), dim3(ME_THREADS), 0, s, n, pos, scene, div, e->istat, cell_fixed,
-                       e->T, e->cellc, e->bucket, e->bcount);
+                       e->cfg.margin, e->T, e->cellc, e->bucket, e->bcount);
     int rc = scan(e, e->bcount, e->bstart, nullptr, e->T, e->T, nullptr, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n)), dim3(ME_THREADS), 0, s, n, e->bucket, e->bstart, e->bcursor,
@@ -331,13 +326,10 @@ int build_grid(me_engine *e, int n, const float *pos, const int *scene, int div,
 int body_pairs(me_engine *e, int all_pairs, hipStream_t s) {
     Bodies b = bodies_of(e);
     int N = e->N;
-    hipLaunchKernelGGL(k_grid_cell, dim3(1), dim3(1), 0, s, e->istat, e->cfg.margin);
     int rc = build_grid(e, N, e->b.x, e->b.scene, 1, -1.0f, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_box_cell, dim3(nblk(27LL * N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cellc, e->bstart,
-                       e->bcount, e->items, e->T, e->cfg.margin, all_pairs, e->cc_n, e->cc_list, e->istat);
-    hipLaunchKernelGGL(k_box_merge, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->aabb, e->cfg.margin, all_pairs,
-                       e->cc_n, e->cc_list, e->cand_n, e->cand_tmp, e->istat);
+    hipLaunchKernelGGL(k_box_pairs, dim3(N), dim3(64), 0, s, N, b, e->aabb, e->cellc, e->bstart, e->bcount, e->items,
+                       e->T, e->cfg.margin, all_pairs, e->cand_n, e->cand_tmp, e->istat);
     rc = scan(e, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_box_cand_emit, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->cand_n, e->cand_off,
@@ -376,7 +368,7 @@ int generate_bead_pairs(me_engine *e, hipStream_t s) {
     hipLaunchKernelGGL(k_bp_compact, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->cand_b,
                        e->b.mu, e->S.n, e->slot_off, e->bp_kn, e->bp_koff, e->pair_flag, e->pair_off, e->capC,
                        e->capP, e->raw, e->Cc, e->Pp, e->istat);
-    hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP);
+    hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP, 0);
     LAUNCH_CHECK();
     return ME_OK;
 }
@@ -401,8 +393,7 @@ int generate(me_engine *e, hipStream_t s) {
         hipLaunchKernelGGL(k_bead_contacts<true>, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, NP, P, e->allp, e->cellc,
                            e->bstart, e->bcount, e->items, e->T, R, e->b.mu, e->cand_n, e->cand_off, e->capC, e->Cc,
                            e->istat);
-        hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP);
-        hipLaunchKernelGGL(k_copy_stat, dim3(1), dim3(1), 0, s, e->istat, IS_CRAW, IS_C);
+        hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP, 1);
         LAUNCH_CHECK();
         return ME_OK;
     }
@@ -420,8 +411,7 @@ int generate(me_engine *e, hipStream_t s) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_compact, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->capC, e->capP, e->istat, e->cand_a,
                        e->cand_b, e->S, e->slot_off, e->pair_off, e->Cc, e->Pp);
-    hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP);
-    hipLaunchKernelGGL(k_copy_stat, dim3(1), dim3(1), 0, s, e->istat, IS_CRAW, IS_C);
+    hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP, 1);
     if (e->cfg.flags & ME_FLAG_PER_CONTACT)
         hipLaunchKernelGGL(k_pairs_per_contact, dim3(nblk(e->capC)), dim3(ME_THREADS), 0, s, e->capC, e->istat,
                            e->Cc.bi, e->Cc.bj, e->Pp);
@@ -509,8 +499,7 @@ int color_jp(me_engine *e, int P, hipStream_t s) {
 int solve_colored(me_engine *e, float sdt, hipStream_t s) {
     bool cc_mode = e->cfg.solver == ME_SOLVER_COLOR_CACHE;
     int capC = e->capC, capP = e->capP, N = e->N;
-    hipLaunchKernelGGL(k_zero_pvpo, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->pv, e->po);
-    int rc = read_stat(e, s);
+    int rc = read_stat(e, s);  // pv, po were zeroed by k_body_pre
     if (rc) return rc;
     int C = e->hstat[IS_C], P = e->hstat[IS_P];
     if (C <= 0) {  // no contacts: warm store emptied (contact_engine_gpu_deferred_force.py:227-235)
@@ -583,12 +572,13 @@ int substep(me_engine *e, double sdt64, hipStream_t s) {
     float sdt = (float)sdt64;
     int N = e->N;
     Bodies b = bodies_of(e);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, e->istat + IS_MAXEXT, 1, 0);
     if (e->bead_pairs)
         hipLaunchKernelGGL(k_fill_half, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->bhalf[0], e->bhalf[1], e->bhalf[2],
                            e->bp_half);
+    bool colored = e->cfg.solver != ME_SOLVER_JACOBI;
     hipLaunchKernelGGL(k_body_pre, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, b, e->invIw, e->Rw, e->aabb, e->vsnap,
-                       e->xsnap, e->wsnap, e->istat, sdt, (e->P == 0 || e->bead_pairs) ? 1 : 0);
+                       e->xsnap, e->wsnap, e->istat, sdt, (e->P == 0 || e->bead_pairs) ? 1 : 0, e->T, e->bcount,
+                       e->bcursor, colored ? e->pv : nullptr, colored ? e->po : nullptr);
     LAUNCH_CHECK();
     int rc = generate(e, s);
     if (rc) return rc;

@@ -159,14 +159,29 @@ __global__ void k_t_cc_restore(int capP, const int *istat, const int *ccol, int 
     if (i < istat[IS_P] && i < capP) col[i] = ccol[i];
 }
 
-__global__ void k_t_cc_store(int N, int capP, int *istat, const int *pbi, const int *pbj, const int *col,
-                             const float *invM, int *ca, int *cb, int *ccol, float *cinvM) {
+
+// end of a tiled substep: the warm store (A23) and the colour cache's copy of the pair list, in one
+// launch (k_warm_store and k_t_cc_store side by side)
+__global__ void k_t_finish(int N, int capP, int capC, int warm, int cc, int *istat, const int64_t *key,
+                           const float *jn, const float *jt1, const float *jt2, int64_t *wkey, float *wjn, float *wjt1,
+                           float *wjt2, const int *pbi, const int *pbj, const int *col, const float *invM, int *ca,
+                           int *cb, int *ccol, float *cinvM) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (istat[IS_MISMATCH] != 1) return;
-    int P = istat[IS_P];
-    if (i < P && i < capP) { ca[i] = pbi[i]; cb[i] = pbj[i]; ccol[i] = col[i]; }
-    if (i < N) cinvM[i] = invM[i];
-    if (i == 0) { istat[IS_CACHED_P] = P; istat[IS_CACHED_NCOL] = istat[IS_NCOL]; }
+    if (warm) {
+        int C = istat[IS_C];
+        if (C > capC) C = capC;
+        if (i < C) { wkey[i] = key[i]; wjn[i] = jn[i]; wjt1[i] = jt1[i]; wjt2[i] = jt2[i]; }
+    }
+    if (cc && istat[IS_MISMATCH] == 1) {
+        int P = istat[IS_P];
+        if (i < P && i < capP) { ca[i] = pbi[i]; cb[i] = pbj[i]; ccol[i] = col[i]; }
+        if (i < N) cinvM[i] = invM[i];
+    }
+    if (i == 0) {  // no thread of this launch reads the words written here
+        int C = istat[IS_C];
+        if (warm) istat[IS_WN] = C > capC ? capC : C;
+        if (cc && istat[IS_MISMATCH] == 1) { istat[IS_CACHED_P] = istat[IS_P]; istat[IS_CACHED_NCOL] = istat[IS_NCOL]; }
+    }
 }
 
 // parent chains only ever point to smaller indices, so the walk ends within N steps
@@ -949,7 +964,7 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     int ntile = tiled_ntile(A);
     bool cc = A.solver == ME_SOLVER_COLOR_CACHE;
     int g = capP > N ? capP : N;
-    hipLaunchKernelGGL(k_zero_pvpo, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, A.pv, A.po);
+    // pv, po were zeroed by k_body_pre
     if (A.taped)  // before the lookup: the warm store still holds the previous substep's contacts
         hipLaunchKernelGGL(k_tape_widx, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
                            A.wkey, A.tape.widx);
@@ -993,12 +1008,13 @@ inline int tiled_sweeps(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
 inline int tiled_finish(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     int N = A.N, capP = A.capP, capC = A.capC;
     int g = capP > N ? capP : N;
-    if (A.warm)
-        hipLaunchKernelGGL(k_warm_store, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.Cc.key, A.Cc.jn,
-                           A.Cc.jt1, A.Cc.jt2, A.wkey, A.wjn, A.wjt1, A.wjt2);
-    if (A.solver == ME_SOLVER_COLOR_CACHE)
-        hipLaunchKernelGGL(k_t_cc_store, dim3(tblk(g)), dim3(ME_THREADS), 0, s, N, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
-                           A.Pp.col, A.b.invM, A.cache_a, A.cache_b, A.cache_col, A.cache_invM);
+    bool cc = A.solver == ME_SOLVER_COLOR_CACHE;
+    if (A.warm || cc) {
+        int gg = g > capC ? g : capC;
+        hipLaunchKernelGGL(k_t_finish, dim3(tblk(gg)), dim3(ME_THREADS), 0, s, N, capP, capC, A.warm, cc ? 1 : 0,
+                           A.istat, A.Cc.key, A.Cc.jn, A.Cc.jt1, A.Cc.jt2, A.wkey, A.wjn, A.wjt1, A.wjt2, A.Pp.pbi,
+                           A.Pp.pbj, A.Pp.col, A.b.invM, A.cache_a, A.cache_b, A.cache_col, A.cache_invM);
+    }
     return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;
 }
 
