@@ -859,7 +859,6 @@ __global__ void k_sweep_pos(SweepArgs S, int ci, float dt) {
     chunk_pos(S.B, S.B, S.Cv, S.pbi[p], S.pbj[p], S.pstart[p], S.pcount[p], dt);
 }
 
-
 // A11 integrate (contact_engine_gpu.py:172-180)
 __global__ void k_integrate(int N, Bodies b, const float *pv, const float *po, float dt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -159,9 +159,9 @@ __global__ void k_t_cc_restore(int capP, const int *istat, const int *ccol, int 
     if (i < istat[IS_P] && i < capP) col[i] = ccol[i];
 }
 
-
-// end of a tiled substep: the warm store (A23) and the colour cache's copy of the pair list, in one
-// launch (k_warm_store and k_t_cc_store side by side)
+// end of a tiled substep, one launch: the warm store (A23, contact_engine_gpu_colored.py:1089-1093)
+// and, after a colour-cache miss, the cache's copy of the pair list, colours and invM
+// (contact_engine_gpu_color_cache.py:48-53)
 __global__ void k_t_finish(int N, int capP, int capC, int warm, int cc, int *istat, const int64_t *key,
                            const float *jn, const float *jt1, const float *jt2, int64_t *wkey, float *wjn, float *wjt1,
                            float *wjt2, const int *pbi, const int *pbj, const int *col, const float *invM, int *ca,
