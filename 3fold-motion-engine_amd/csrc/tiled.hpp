@@ -124,11 +124,13 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
 }
 
 // IS_MISMATCH after k_t_mode: 0 = ColorCache hit, 1 = colour this substep, 2 = no contacts
-__global__ void k_t_init(int N, TiledBuffers T, int *istat) {
+__global__ void k_t_init(int N, TiledBuffers T, int *istat, int lds_islands) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) {
-        T.parent[i] = i;
-        T.cnt_b[i] = 0; T.cnt_p[i] = 0; T.fill_b[i] = 0; T.fill_p[i] = 0;
+        if (!lds_islands) {
+            T.parent[i] = i;
+            T.cnt_b[i] = 0; T.cnt_p[i] = 0; T.fill_b[i] = 0; T.fill_p[i] = 0;
+        }
         T.tb0[i] = INT_MAX; T.tb1[i] = -1; T.tp0[i] = INT_MAX; T.tp1[i] = -1;
     }
     if (i <= ME_NCOLMAX) T.ccount_jp[i] = 0;
@@ -299,6 +301,159 @@ __global__ void k_t_tiles(int N, int TB, int *istat, TiledBuffers T) {
     atomicMax(&T.tb1[t], T.boff[i] + T.cnt_b[i]);
     atomicMin(&T.tp0[t], T.poff[i]);
     atomicMax(&T.tp1[t], T.poff[i] + T.cnt_p[i]);
+}
+
+// ── the island build of a scene of at most T_ISL_MAXN bodies in one workgroup ──
+// The same result as k_t_union .. k_t_tiles (islands of the dynamic-dynamic pair graph, their bodies
+// and pairs laid out island-major by root, tiles of about TB bodies), with the union-find forest,
+// the per-root counts and their offsets in LDS and no kernel boundary between the phases. Within an
+// island the order of bodies and pairs is the order of LDS atomics, which no result depends on (a
+// tile's pairs are coloured by key and a colour's pairs touch disjoint bodies).
+#define T_ISL_THREADS 1024
+#ifndef T_ISL_MAXN
+#define T_ISL_MAXN 12288  // three int arrays of N in LDS
+#endif
+#define T_ISL_DYN (3 * T_ISL_MAXN * 4)
+
+__device__ __forceinline__ int lds_find_halve(volatile int *par, int x) {
+    int p = par[x];
+    while (p != x) {
+        int g = par[p];
+        if (g != p) par[x] = g;
+        x = p;
+        p = par[x];
+    }
+    return x;
+}
+
+// exclusive scan of a[0..n) in place by T_ISL_THREADS threads (a contiguous chunk each); returns the total
+__device__ int lds_exclusive_scan(int *a, int n, int *part) {
+    int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int chunk = (n + T_ISL_THREADS - 1) / T_ISL_THREADS;
+    int lo = t * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    int sum = 0;
+    for (int i = lo; i < hi; ++i) sum += a[i];
+    int incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) part[wv] = incl;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int w = 0; w < T_ISL_THREADS / 64; ++w) { int v = part[w]; part[w] = run; run += v; }
+        part[T_ISL_THREADS / 64] = run;
+    }
+    __syncthreads();
+    int base = part[wv] + incl - sum;
+    for (int i = lo; i < hi; ++i) { int v = a[i]; a[i] = base; base += v; }
+    int total = part[T_ISL_THREADS / 64];
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(T_ISL_THREADS) k_t_islands_lds(int N, int capP, int TB, int *istat, const int *pbi,
+                                                                 const int *pbj, const float *invM, const float *x,
+                                                                 const float *invIw, TiledBuffers T) {
+    extern __shared__ int sh[];
+    __shared__ int part[T_ISL_THREADS / 64 + 1];
+    int *par = sh, *cb = sh + N, *cp = sh + 2 * N;
+    volatile int *vpar = par;
+    const int t = threadIdx.x;
+    int P = istat[IS_P];
+    if (P > capP) P = capP;
+    for (int i = t; i < N; i += T_ISL_THREADS) { par[i] = i; cb[i] = 0; cp[i] = 0; }
+    __syncthreads();
+    // union: link the larger root under the smaller (compare-and-swap on the root)
+    bool stuck = false;
+    for (int p = t; p < P; p += T_ISL_THREADS) {
+        int a = pbi[p], b = pbj[p];
+        if (b < 0 || !(invM[a] > 0.0f) || !(invM[b] > 0.0f)) continue;
+        int it = 0;
+        for (; it < (1 << 20); ++it) {
+            int ra = lds_find_halve(vpar, a), rb = lds_find_halve(vpar, b);
+            if (ra == rb) break;
+            int hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+            if (atomicCAS(&par[hi], hi, lo) == hi) break;
+        }
+        stuck = stuck || it == (1 << 20);
+    }
+    if (stuck) atomicOr(&istat[IS_OVF], OVF_TILE);
+    __syncthreads();
+    // flatten: every dynamic body points at its root; bodies per root (loops of uniform trip count:
+    // wave_add1 folds the lanes that share a root into one LDS atomic)
+    for (int i0 = 0; i0 < N; i0 += T_ISL_THREADS) {
+        int i = i0 + t;
+        bool dyn = i < N && invM[i] > 0.0f;
+        int r = dyn ? lds_find_halve(vpar, i) : -1;
+        if (i < N) T.root[i] = r;
+        wave_add1(cb, r, dyn);
+    }
+    __syncthreads();
+    // the forest now points every body at its root; only after every find has ended (a path-halving
+    // write of a concurrent find could replace a root written earlier by an older ancestor)
+    for (int i = t; i < N; i += T_ISL_THREADS)
+        if (invM[i] > 0.0f) par[i] = T.root[i];
+    __syncthreads();
+    // pairs per root (a pair belongs to the island of its dynamic end)
+    for (int p0 = 0; p0 < P; p0 += T_ISL_THREADS) {
+        int p = p0 + t, r = -1;
+        if (p < P) {
+            int a = pbi[p], b = pbj[p];
+            r = invM[a] > 0.0f ? par[a] : ((b >= 0 && invM[b] > 0.0f) ? par[b] : -1);
+        }
+        wave_add1(cp, r, r >= 0);
+    }
+    __syncthreads();
+    int nbs = lds_exclusive_scan(cb, N, part);
+    int nps = lds_exclusive_scan(cp, N, part);
+    // tiles: an island goes to tile boff / TB; tile bounds over its islands
+    int nroot = 0;
+    for (int r = t; r < N; r += T_ISL_THREADS) {
+        int b1 = r + 1 < N ? cb[r + 1] : nbs;
+        if (b1 == cb[r]) continue;
+        ++nroot;
+        int p1 = r + 1 < N ? cp[r + 1] : nps;
+        int tt = cb[r] / TB;
+        atomicMin(&T.tb0[tt], cb[r]);
+        atomicMax(&T.tb1[tt], b1);
+        atomicMin(&T.tp0[tt], cp[r]);
+        atomicMax(&T.tp1[tt], p1);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nroot += __shfl_xor(nroot, o);
+    if (t == 0) part[0] = 0;
+    __syncthreads();  // every thread has read the offsets before they become cursors
+    if ((t & 63) == 0 && nroot) atomicAdd(&part[0], nroot);
+    // scatter: bodies, then pairs, island-major (the offsets are the cursors)
+    for (int i0 = 0; i0 < N; i0 += T_ISL_THREADS) {
+        int i = i0 + t;
+        bool dyn = i < N && invM[i] > 0.0f;
+        int r = dyn ? par[i] : -1;
+        int pos = wave_add1(cb, r, dyn);
+        if (!dyn) continue;
+        T.bsorted[pos] = i;
+        T.bslot[i] = pos;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) T.xs[3 * pos + k] = x[3 * i + k];
+        T.invMs[pos] = invM[i];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) T.invIws[12 * pos + k] = invIw[9 * i + k];
+        T.invIws[12 * pos + 9] = 0.0f; T.invIws[12 * pos + 10] = 0.0f; T.invIws[12 * pos + 11] = 0.0f;
+    }
+    for (int p0 = 0; p0 < P; p0 += T_ISL_THREADS) {
+        int p = p0 + t, r = -1;
+        if (p < P) {
+            int a = pbi[p], b = pbj[p];
+            r = invM[a] > 0.0f ? par[a] : ((b >= 0 && invM[b] > 0.0f) ? par[b] : -1);
+        }
+        int pos = wave_add1(cp, r, r >= 0);
+        if (r >= 0) T.psorted[pos] = p;
+    }
+    __syncthreads();
+    if (t == 0) { istat[IS_NBODYSORT] = nbs; istat[IS_NPAIRSORT] = nps; istat[IS_ISL] += part[0]; }
 }
 
 // JP rounds of one tile (k_claim / k_win of contact_engine_gpu_colored.py:151-185, restricted to the
@@ -924,7 +1079,9 @@ inline int tiled_set_attributes() {
         hipFuncSetAttribute((const void *)k_t_big<ME_SOLVER_COLOR_CACHE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)T_BIG_DYN) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_t_color<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)T_COL_DYN) != hipSuccess)
+                            (int)T_COL_DYN) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_t_islands_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)T_ISL_DYN) != hipSuccess)
         return ME_ERUNTIME;
     return ME_OK;
 }
@@ -971,26 +1128,32 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     if (!A.warm_set)
         hipLaunchKernelGGL(k_warm_lookup, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
                            A.wkey, A.wjn, A.wjt1, A.wjt2, A.Cc.jn, A.Cc.jt1, A.Cc.jt2, A.Cc.jp);
+    bool lds_isl = N <= T_ISL_MAXN;
     hipLaunchKernelGGL(k_t_init, dim3(tblk(N > ME_NCOLMAX + 1 ? N : ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, N, T,
-                       A.istat);
+                       A.istat, lds_isl ? 1 : 0);
     if (cc)
         hipLaunchKernelGGL(k_cc_compare, dim3(tblk(g)), dim3(ME_THREADS), 0, s, N, A.istat, A.Pp.pbi, A.Pp.pbj,
                            A.cache_a, A.cache_b, A.b.invM, A.cache_invM);
     hipLaunchKernelGGL(k_t_mode, dim3(1), dim3(1), 0, s, A.istat, cc ? 1 : 0);
     if (cc) hipLaunchKernelGGL(k_t_cc_restore, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.cache_col,
                                A.Pp.col);
-    hipLaunchKernelGGL(k_t_union, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj, A.b.invM,
-                       T.parent);
-    hipLaunchKernelGGL(k_t_flatten, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, A.b.invM, T);
-    hipLaunchKernelGGL(k_t_pair_root, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
-                       A.b.invM, T);
-    int rc = tiled_scan(A, T.cnt_b, T.boff, N, A.istat + IS_NBODYSORT, s);
-    if (rc) return rc;
-    rc = tiled_scan(A, T.cnt_p, T.poff, N, A.istat + IS_NPAIRSORT, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T, A.b.x, A.b.invM, A.invIw);
-    hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
-    hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
+    if (lds_isl) {
+        hipLaunchKernelGGL(k_t_islands_lds, dim3(1), dim3(T_ISL_THREADS), 3 * N * sizeof(int), s, N, capP, TB, A.istat,
+                           A.Pp.pbi, A.Pp.pbj, A.b.invM, A.b.x, A.invIw, T);
+    } else {
+        hipLaunchKernelGGL(k_t_union, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
+                           A.b.invM, T.parent);
+        hipLaunchKernelGGL(k_t_flatten, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, A.b.invM, T);
+        hipLaunchKernelGGL(k_t_pair_root, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
+                           A.b.invM, T);
+        int rc = tiled_scan(A, T.cnt_b, T.boff, N, A.istat + IS_NBODYSORT, s);
+        if (rc) return rc;
+        rc = tiled_scan(A, T.cnt_p, T.poff, N, A.istat + IS_NPAIRSORT, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T, A.b.x, A.b.invM, A.invIw);
+        hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
+        hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
+    }
     hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile)), dim3(ME_THREADS), 0, s, ntile, A.istat, T);
     hipLaunchKernelGGL(k_t_color_small, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
     hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(1024), T_COL_DYN, s, A, T);

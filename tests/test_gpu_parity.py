@@ -111,6 +111,24 @@ def test_box_pile_matches_oracle(oracle, kind, n_colors, per_contact, tiled):
         assert np.array_equal(pg[k], po[k]), k
 
 
+def test_tiled_pile_beyond_the_lds_island_build(oracle):
+    """A scene of more bodies than the one-workgroup island build holds (12 288) takes the multi-kernel
+    island build; same bar as every tiled case."""
+    _say("color_cache box pile 12800 x40 S2 V4 P2 tiled (multi-kernel island build)")
+    pkg = _pkg()
+    b = pkg.scenes.box_pile(12800, seed=4)
+    kw = dict(vit=4, pit=2, n_colors=16)
+    e = _engine("color_cache", tiled=True, tile_bodies=128, **kw)
+    e.add_boxes(b)
+    o = oracle.OracleWorld(b, "color_cache", **kw)
+    for s in range(40):
+        e.step(1 / 240, 2)
+        o.step(1 / 240, 2)
+        _assert_state_equal(e, o, f"step {s}")
+        assert e.n_contacts == o.counters()["n_contacts"]
+    assert e.n_contacts > 10000
+
+
 def test_colouring_is_proper_and_runs_repeat():
     _say("color_cache box pile 1000 x30: proper colouring, two runs bit-identical")
     pkg = _pkg()
