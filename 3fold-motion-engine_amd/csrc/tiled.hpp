@@ -33,7 +33,7 @@ struct TiledBuffers {
     int *bsorted, *bslot, *psorted;
     int *tb0, *tb1, *tp0, *tp1;
     int *ccount_jp, *csub, *cbase;
-    float *xs, *invMs, *invIws;  // body constants in island order (index = position in bsorted)
+    float *invMs, *invIws;       // body constants in island order (index = position in bsorted)
     int *psorted2;               // big tiles: pairs counting-sorted by colour
     int *rec;                    // big tiles: 8 ints per colour-sorted pair (p, a, b, la, lb, st, cnt, flags)
     float *prep;                 // 20 floats per contact: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu
@@ -111,7 +111,6 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.ccount_jp = B.template take<int>(ME_NCOLMAX + 1);
     t.csub = B.template take<int>(ME_NCOLMAX + 1);
     t.cbase = B.template take<int>(ME_NCOLMAX + 2);
-    t.xs = B.template take<float>(3 * n);
     t.invMs = B.template take<float>(n);
     t.invIws = B.template take<float>(12 * n);
     t.psorted2 = B.template take<int>(p);
@@ -268,7 +267,7 @@ __global__ void k_t_pair_root(int capP, const int *istat, const int *pbi, const 
     wave_add1(T.cnt_p, r, r >= 0);
 }
 
-__global__ void k_t_scatter_b(int N, TiledBuffers T, const float *x, const float *invM, const float *invIw) {
+__global__ void k_t_scatter_b(int N, TiledBuffers T) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int r = i < N ? T.root[i] : -1;
     int slot = wave_add1(T.fill_b, r, r >= 0);
@@ -276,12 +275,6 @@ __global__ void k_t_scatter_b(int N, TiledBuffers T, const float *x, const float
     int pos = T.boff[r] + slot;
     T.bsorted[pos] = i;
     T.bslot[i] = pos;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) T.xs[3 * pos + k] = x[3 * i + k];
-    T.invMs[pos] = invM[i];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) T.invIws[12 * pos + k] = invIw[9 * i + k];
-    T.invIws[12 * pos + 9] = 0.0f; T.invIws[12 * pos + 10] = 0.0f; T.invIws[12 * pos + 11] = 0.0f;
 }
 
 __global__ void k_t_scatter_p(int capP, const int *istat, TiledBuffers T) {
@@ -354,9 +347,17 @@ __device__ int lds_exclusive_scan(int *a, int n, int *part) {
     return total;
 }
 
+#define T_ISL_PK 16  // pairs per thread held in registers (the first 16 384 pairs; the rest re-read)
+#define T_ISL_BK ((T_ISL_MAXN + T_ISL_THREADS - 1) / T_ISL_THREADS)
+
+// root of pair p's island (-1: no dynamic end) from its endpoints a, b (b < 0: ground) and their
+// dynamic flags, once the forest points every body at its root
+__device__ __forceinline__ int isl_pair_root(const int *par, int a, int b, bool da, bool db) {
+    return da ? par[a] : (db ? par[b] : -1);
+}
+
 __global__ void __launch_bounds__(T_ISL_THREADS) k_t_islands_lds(int N, int capP, int TB, int *istat, const int *pbi,
-                                                                 const int *pbj, const float *invM, const float *x,
-                                                                 const float *invIw, TiledBuffers T) {
+                                                                 const int *pbj, const float *invM, TiledBuffers T) {
     extern __shared__ int sh[];
     __shared__ int part[T_ISL_THREADS / 64 + 1];
     int *par = sh, *cb = sh + N, *cp = sh + 2 * N;
@@ -364,13 +365,32 @@ __global__ void __launch_bounds__(T_ISL_THREADS) k_t_islands_lds(int N, int capP
     const int t = threadIdx.x;
     int P = istat[IS_P];
     if (P > capP) P = capP;
+    // this thread's pairs p = t + k * T_ISL_THREADS and bodies i = t + k * T_ISL_THREADS: endpoints and
+    // dynamic flags loaded up front (all loads of a kind in flight together, not one per iteration)
+    int pa[T_ISL_PK], pb[T_ISL_PK];
+    unsigned dA = 0, dB = 0, dBody = 0;
+#pragma unroll
+    for (int k = 0; k < T_ISL_PK; ++k) {
+        int p = t + k * T_ISL_THREADS;
+        pa[k] = p < P ? pbi[p] : 0;
+        pb[k] = p < P ? pbj[p] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < T_ISL_PK; ++k) {
+        int p = t + k * T_ISL_THREADS;
+        if (p < P && invM[pa[k]] > 0.0f) dA |= 1u << k;
+        if (p < P && pb[k] >= 0 && invM[pb[k]] > 0.0f) dB |= 1u << k;
+    }
+#pragma unroll
+    for (int k = 0; k < T_ISL_BK; ++k) {
+        int i = t + k * T_ISL_THREADS;
+        if (i < N && invM[i] > 0.0f) dBody |= 1u << k;
+    }
     for (int i = t; i < N; i += T_ISL_THREADS) { par[i] = i; cb[i] = 0; cp[i] = 0; }
     __syncthreads();
     // union: link the larger root under the smaller (compare-and-swap on the root)
     bool stuck = false;
-    for (int p = t; p < P; p += T_ISL_THREADS) {
-        int a = pbi[p], b = pbj[p];
-        if (b < 0 || !(invM[a] > 0.0f) || !(invM[b] > 0.0f)) continue;
+    auto unite = [&](int a, int b) {
         int it = 0;
         for (; it < (1 << 20); ++it) {
             int ra = lds_find_halve(vpar, a), rb = lds_find_halve(vpar, b);
@@ -379,30 +399,45 @@ __global__ void __launch_bounds__(T_ISL_THREADS) k_t_islands_lds(int N, int capP
             if (atomicCAS(&par[hi], hi, lo) == hi) break;
         }
         stuck = stuck || it == (1 << 20);
+    };
+#pragma unroll
+    for (int k = 0; k < T_ISL_PK; ++k)
+        if (((dA & dB) >> k) & 1u) unite(pa[k], pb[k]);
+    for (int p = t + T_ISL_PK * T_ISL_THREADS; p < P; p += T_ISL_THREADS) {
+        int a = pbi[p], b = pbj[p];
+        if (b >= 0 && invM[a] > 0.0f && invM[b] > 0.0f) unite(a, b);
     }
     if (stuck) atomicOr(&istat[IS_OVF], OVF_TILE);
     __syncthreads();
-    // flatten: every dynamic body points at its root; bodies per root (loops of uniform trip count:
-    // wave_add1 folds the lanes that share a root into one LDS atomic)
-    for (int i0 = 0; i0 < N; i0 += T_ISL_THREADS) {
-        int i = i0 + t;
-        bool dyn = i < N && invM[i] > 0.0f;
+    // roots and bodies per root (wave_add1 folds the lanes that share a root into one LDS atomic)
+    int rootk[T_ISL_BK];
+#pragma unroll
+    for (int k = 0; k < T_ISL_BK; ++k) {
+        int i = t + k * T_ISL_THREADS;
+        bool dyn = (dBody >> k) & 1u;
         int r = dyn ? lds_find_halve(vpar, i) : -1;
+        rootk[k] = r;
         if (i < N) T.root[i] = r;
         wave_add1(cb, r, dyn);
     }
     __syncthreads();
     // the forest now points every body at its root; only after every find has ended (a path-halving
     // write of a concurrent find could replace a root written earlier by an older ancestor)
-    for (int i = t; i < N; i += T_ISL_THREADS)
-        if (invM[i] > 0.0f) par[i] = T.root[i];
+#pragma unroll
+    for (int k = 0; k < T_ISL_BK; ++k)
+        if ((dBody >> k) & 1u) par[t + k * T_ISL_THREADS] = rootk[k];
     __syncthreads();
     // pairs per root (a pair belongs to the island of its dynamic end)
-    for (int p0 = 0; p0 < P; p0 += T_ISL_THREADS) {
+#pragma unroll
+    for (int k = 0; k < T_ISL_PK; ++k) {
+        int r = isl_pair_root(par, pa[k], pb[k], (dA >> k) & 1u, (dB >> k) & 1u);
+        wave_add1(cp, r, r >= 0);
+    }
+    for (int p0 = T_ISL_PK * T_ISL_THREADS; p0 < P; p0 += T_ISL_THREADS) {
         int p = p0 + t, r = -1;
         if (p < P) {
             int a = pbi[p], b = pbj[p];
-            r = invM[a] > 0.0f ? par[a] : ((b >= 0 && invM[b] > 0.0f) ? par[b] : -1);
+            r = isl_pair_root(par, a, b, invM[a] > 0.0f, b >= 0 && invM[b] > 0.0f);
         }
         wave_add1(cp, r, r >= 0);
     }
@@ -428,26 +463,27 @@ __global__ void __launch_bounds__(T_ISL_THREADS) k_t_islands_lds(int N, int capP
     __syncthreads();  // every thread has read the offsets before they become cursors
     if ((t & 63) == 0 && nroot) atomicAdd(&part[0], nroot);
     // scatter: bodies, then pairs, island-major (the offsets are the cursors)
-    for (int i0 = 0; i0 < N; i0 += T_ISL_THREADS) {
-        int i = i0 + t;
-        bool dyn = i < N && invM[i] > 0.0f;
-        int r = dyn ? par[i] : -1;
-        int pos = wave_add1(cb, r, dyn);
-        if (!dyn) continue;
+    int posk[T_ISL_BK];
+#pragma unroll
+    for (int k = 0; k < T_ISL_BK; ++k) posk[k] = wave_add1(cb, rootk[k], (dBody >> k) & 1u);
+#pragma unroll
+    for (int k = 0; k < T_ISL_BK; ++k) {
+        if (!((dBody >> k) & 1u)) continue;
+        int i = t + k * T_ISL_THREADS, pos = posk[k];
         T.bsorted[pos] = i;
         T.bslot[i] = pos;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) T.xs[3 * pos + k] = x[3 * i + k];
-        T.invMs[pos] = invM[i];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) T.invIws[12 * pos + k] = invIw[9 * i + k];
-        T.invIws[12 * pos + 9] = 0.0f; T.invIws[12 * pos + 10] = 0.0f; T.invIws[12 * pos + 11] = 0.0f;
     }
-    for (int p0 = 0; p0 < P; p0 += T_ISL_THREADS) {
+#pragma unroll
+    for (int k = 0; k < T_ISL_PK; ++k) {
+        int r = isl_pair_root(par, pa[k], pb[k], (dA >> k) & 1u, (dB >> k) & 1u);
+        int pos = wave_add1(cp, r, r >= 0);
+        if (r >= 0) T.psorted[pos] = t + k * T_ISL_THREADS;
+    }
+    for (int p0 = T_ISL_PK * T_ISL_THREADS; p0 < P; p0 += T_ISL_THREADS) {
         int p = p0 + t, r = -1;
         if (p < P) {
             int a = pbi[p], b = pbj[p];
-            r = invM[a] > 0.0f ? par[a] : ((b >= 0 && invM[b] > 0.0f) ? par[b] : -1);
+            r = isl_pair_root(par, a, b, invM[a] > 0.0f, b >= 0 && invM[b] > 0.0f);
         }
         int pos = wave_add1(cp, r, r >= 0);
         if (r >= 0) T.psorted[pos] = p;
@@ -1063,9 +1099,22 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_color_small(TiledArgs A, TiledB
     if (tid == 0) atomicMax(&A.istat[IS_NCOLJP], nround);
 }
 
-// list of the tiles that exceed one small tile (the big-tile kernel's work)
-__global__ void k_t_biglist(int ntile, int *istat, TiledBuffers T) {
+// list of the tiles that exceed one small tile (the big-tile kernel's work); and the big tiles' body
+// constants in island order (position j <- body bsorted[j]: coalesced writes, gathered reads), 48-byte
+// records of invIw for three 16-byte loads
+__global__ void k_t_biglist(int ntile, int *istat, const float *invM, const float *invIw, TiledBuffers T) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < istat[IS_NBODYSORT]) {
+        int i = T.bsorted[t];
+        T.invMs[t] = invM[i];
+        float I[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) I[q] = invIw[9 * i + q];
+        float4 *d = reinterpret_cast<float4 *>(T.invIws + 12 * (size_t)t);
+        d[0] = make_float4(I[0], I[1], I[2], I[3]);
+        d[1] = make_float4(I[4], I[5], I[6], I[7]);
+        d[2] = make_float4(I[8], 0.0f, 0.0f, 0.0f);
+    }
     if (t >= ntile || T.tb0[t] == INT_MAX) return;
     int nb = T.tb1[t] - T.tb0[t], np = T.tp1[t] - T.tp0[t];
     if (np > T_SMALL_P || nb > T_SMALL_B) T.biglist[atomicAdd(&istat[IS_NBIG], 1)] = t;
@@ -1139,7 +1188,7 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
                                A.Pp.col);
     if (lds_isl) {
         hipLaunchKernelGGL(k_t_islands_lds, dim3(1), dim3(T_ISL_THREADS), 3 * N * sizeof(int), s, N, capP, TB, A.istat,
-                           A.Pp.pbi, A.Pp.pbj, A.b.invM, A.b.x, A.invIw, T);
+                           A.Pp.pbi, A.Pp.pbj, A.b.invM, T);
     } else {
         hipLaunchKernelGGL(k_t_union, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
                            A.b.invM, T.parent);
@@ -1150,11 +1199,12 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
         if (rc) return rc;
         rc = tiled_scan(A, T.cnt_p, T.poff, N, A.istat + IS_NPAIRSORT, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T, A.b.x, A.b.invM, A.invIw);
+        hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T);
         hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
         hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
     }
-    hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile)), dim3(ME_THREADS), 0, s, ntile, A.istat, T);
+    hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile > N ? ntile : N)), dim3(ME_THREADS), 0, s, ntile, A.istat, A.b.invM,
+                       A.invIw, T);
     hipLaunchKernelGGL(k_t_color_small, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
     hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(1024), T_COL_DYN, s, A, T);
     hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
