@@ -36,6 +36,16 @@ class Bodies(C.Structure):
                                            "mu", "kinematic", "scene", "rest")]
 
 
+class PgsScene(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("Ja", "Jb", "Minv", "v_free", "mu", "bias", "sep", "body_pairs",
+                                           "feature")]
+
+
+PGS_DESAXCE, PGS_COLOR_CACHE = 1, 2
+PGS_LAM, PGS_V, PGS_VBIAS, PGS_RHO, PGS_RES, PGS_U, PGS_COLOR = range(7)
+PGS_INFO = ("n_colors", "cache_hits", "cache_misses", "n_islands", "n_tiles", "overflow")
+
+
 class MotionError(RuntimeError):
     pass
 
@@ -65,6 +75,15 @@ def lib():
         L.me_ncp_workspace_size.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_size_t)]
         L.me_ncp_sensitivity.argtypes = [C.c_int32, C.c_int32, C.c_int32] + [vp] * 6 + [C.c_double] * 3 + \
             [vp] * 5 + [C.c_size_t, vp]
+        L.me_pgs_workspace_size.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_size_t)]
+        L.me_pgs_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_size_t, C.POINTER(vp)]
+        L.me_pgs_destroy.argtypes = [vp]
+        L.me_pgs_prepare.argtypes = [vp, C.POINTER(PgsScene), C.c_int32, vp, vp, C.POINTER(C.c_int32), vp]
+        L.me_pgs_iterate.argtypes = [vp, C.c_int32, C.c_int32, vp]
+        L.me_pgs_snapshot.argtypes = [vp, vp]
+        L.me_pgs_residual.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), vp]
+        L.me_pgs_buffer.argtypes = [vp, C.c_int32, C.POINTER(vp), C.POINTER(C.c_int64)]
+        L.me_pgs_info.argtypes = [vp, vp]
         L.me_step_taped.argtypes = [vp, C.c_double, vp, C.c_size_t, vp]
         L.me_adjoint_substep.argtypes = [vp, C.c_double, vp, C.c_size_t] + [vp] * 7 + [vp]
         L.me_last_error.restype = C.c_char_p
@@ -87,5 +106,7 @@ def check(rc: int, what: str) -> None:
 EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
             "me_buffer", "me_read_counters", "me_profile", "me_profile_read", "me_last_error",
             "me_tape_size", "me_step_taped", "me_adjoint_substep", "me_adopt", "me_state_size",
-            "me_ncp_workspace_size", "me_ncp_sensitivity")
+            "me_ncp_workspace_size", "me_ncp_sensitivity", "me_pgs_workspace_size", "me_pgs_create",
+            "me_pgs_destroy", "me_pgs_prepare", "me_pgs_iterate", "me_pgs_snapshot", "me_pgs_residual",
+            "me_pgs_buffer", "me_pgs_info")
 K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL, K_TILE_BIG = range(6)

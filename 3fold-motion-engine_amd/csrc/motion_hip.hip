@@ -14,6 +14,7 @@
 #include "tiled.hpp"
 #include "beads.hpp"
 #include "ncp.hpp"
+#include "pgs.hpp"
 #include "adjoint_kernels.hpp"
 
 using namespace me;
@@ -739,6 +740,229 @@ int bead_half_extents(me_engine *e, hipStream_t s) {
     return ME_OK;
 }
 
+// ── exact-cone PGS solver (pgs.hpp) ──
+struct me_pgs {
+    int NB = 0, MC = 0, rb = 4, flags = 0;
+    int C = 0;
+    int cached_C = -1;
+    int64_t hits = 0, misses = 0;
+    me_pgs_scene sc{};
+    double soft[5] = {0, 0, 0, 0, 0};
+    int soft_on = 0;
+    void *rho = nullptr, *lam = nullptr, *v = nullptr, *vbias = nullptr, *res = nullptr, *uout = nullptr,
+         *cMinv = nullptr;
+    int64_t *acc = nullptr, *key = nullptr, *prio = nullptr;
+    long long *claim = nullptr;
+    unsigned long long *rmax = nullptr;
+    int *dyn = nullptr, *bucket = nullptr, *bcnt = nullptr, *boff = nullptr, *bfill = nullptr, *items = nullptr,
+        *par = nullptr, *isl = nullptr, *icnt = nullptr, *ioff = nullptr, *ifill = nullptr, *iorder = nullptr,
+        *col = nullptr, *order = nullptr, *tlo = nullptr, *thi = nullptr, *tncol = nullptr, *seg = nullptr,
+        *cpab = nullptr, *cfeat = nullptr, *ps = nullptr, *bsum = nullptr;
+    int nbsum = 0;
+    int *hps = nullptr;  // pinned host mirror of ps
+    unsigned long long *hrmax = nullptr;
+};
+
+namespace {
+
+int pgs_ntile(int C) { return C > 0 ? (C + PGS_TILE - 1) / PGS_TILE : 0; }
+
+size_t pgs_layout(me_pgs *s, char *base) {
+    const size_t NB = (size_t)s->NB, MC = (size_t)s->MC, R = (size_t)s->rb;
+    const size_t NI = NB + MC, NT = (size_t)pgs_ntile(s->MC) + 1;
+    Bump B{base};
+    s->rho = B.take<char>(R * MC);
+    s->lam = B.take<char>(R * 3 * MC);
+    s->v = B.take<char>(R * 6 * NB);
+    s->vbias = B.take<char>(R * 6 * NB);
+    s->res = B.take<char>(R * MC);
+    s->uout = B.take<char>(R * 3 * MC);
+    s->cMinv = B.take<char>(R * 36 * NB);
+    s->acc = B.take<int64_t>(6 * NB);
+    s->key = B.take<int64_t>(MC);
+    s->prio = B.take<int64_t>(MC);
+    s->claim = B.take<long long>(NB);
+    s->rmax = B.take<unsigned long long>(1);
+    s->dyn = B.take<int>(NB);
+    s->bucket = B.take<int>(MC);
+    s->bcnt = B.take<int>(NB + 1);
+    s->boff = B.take<int>(NB + 1);
+    s->bfill = B.take<int>(NB + 1);
+    s->items = B.take<int>(MC);
+    s->par = B.take<int>(NB);
+    s->isl = B.take<int>(MC);
+    s->icnt = B.take<int>(NI);
+    s->ioff = B.take<int>(NI);
+    s->ifill = B.take<int>(NI);
+    s->iorder = B.take<int>(MC);
+    s->col = B.take<int>(MC);
+    s->order = B.take<int>(MC);
+    s->tlo = B.take<int>(NT);
+    s->thi = B.take<int>(NT);
+    s->tncol = B.take<int>(NT);
+    s->seg = B.take<int>(NT * (PGS_NCOLMAX + 1));
+    s->cpab = B.take<int>(2 * MC);
+    s->cfeat = B.take<int>(MC);
+    s->ps = B.take<int>(PS_COUNT);
+    size_t scan_n = NI > NB + 1 ? NI : NB + 1;
+    s->nbsum = (int)((scan_n + SCAN_BLOCK - 1) / SCAN_BLOCK);
+    if (s->nbsum < 1) s->nbsum = 1;
+    s->bsum = B.take<int>((size_t)s->nbsum);
+    return B.off;
+}
+
+int pgs_scan(me_pgs *p, const int *in, int *out, int n, hipStream_t s) {
+    int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    if (nb < 1) nb = 1;
+    if (nb > p->nbsum) return fail(ME_ERUNTIME, "scan scratch too small");
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(ME_THREADS), 0, s, in, nullptr, n, n, p->bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, p->bsum, nb, nullptr);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(ME_THREADS), 0, s, in, out, nullptr, n, n, p->bsum);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+void pgs_fill(int *p, long long n, int val, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(ME_THREADS), 0, s, p, (int)n, val);
+}
+
+template <class T>
+PgsScene<T> pgs_scene_of(const me_pgs *p) {
+    PgsScene<T> S;
+    S.Ja = (const T *)p->sc.Ja; S.Jb = (const T *)p->sc.Jb; S.Minv = (const T *)p->sc.Minv;
+    S.vfree = (const T *)p->sc.v_free; S.mu = (const T *)p->sc.mu; S.bias = (const T *)p->sc.bias;
+    S.sep = (const T *)p->sc.sep; S.pab = p->sc.body_pairs; S.feat = p->sc.feature;
+    return S;
+}
+
+template <class T>
+PgsSoft<T> pgs_soft_of(const me_pgs *p, int use_bias) {
+    PgsSoft<T> sp;
+    sp.p0 = (T)p->soft[0]; sp.p1 = (T)p->soft[1]; sp.p2 = (T)p->soft[2]; sp.p3 = (T)p->soft[3];
+    sp.p4 = (T)p->soft[4];
+    sp.on = p->soft_on;
+    sp.use_bias = use_bias ? 1 : 0;
+    return sp;
+}
+
+// deterministic colouring of the current scene (NCPSolverGPU._color, ncp_gpu.py:873-926), as tiles
+int pgs_color(me_pgs *p, hipStream_t s) {
+    const int C = p->C, NB = p->NB, NI = NB + C, nt = pgs_ntile(C);
+    const int *pab = p->sc.body_pairs;
+    pgs_fill(p->ps + PS_NCOL, 4, 0, s);  // PS_NCOL, PS_OVF, PS_NISL, PS_NTILE
+    pgs_fill(p->bcnt, NB + 1, 0, s);
+    pgs_fill(p->bfill, NB + 1, 0, s);
+    pgs_fill(p->icnt, NI, 0, s);
+    pgs_fill(p->ifill, NI, 0, s);
+    pgs_fill(p->tlo, nt, INT_MAX, s);
+    pgs_fill(p->thi, nt, INT_MIN, s);
+    hipLaunchKernelGGL(k_pgs_keys, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, NB, pab, p->sc.feature, p->key, p->bucket,
+                       p->bcnt);
+    int rc = pgs_scan(p, p->bcnt, p->boff, NB + 1, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pgs_bucket_fill, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, p->bucket, p->boff, p->bfill,
+                       p->items);
+    hipLaunchKernelGGL(k_pgs_rank, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, p->key, p->bucket, p->boff, p->bcnt,
+                       p->items, p->prio);
+    hipLaunchKernelGGL(k_pgs_uf_init, dim3(nblk(NB)), dim3(ME_THREADS), 0, s, NB, p->par);
+    hipLaunchKernelGGL(k_pgs_union, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, pab, p->dyn, p->par);
+    hipLaunchKernelGGL(k_pgs_island, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, NB, pab, p->dyn, p->par, p->isl,
+                       p->icnt);
+    rc = pgs_scan(p, p->icnt, p->ioff, NI, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pgs_tiles, dim3(nblk(NI)), dim3(ME_THREADS), 0, s, NI, p->icnt, p->ioff, p->tlo, p->thi,
+                       p->ps);
+    hipLaunchKernelGGL(k_pgs_island_fill, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, p->isl, p->ioff, p->ifill,
+                       p->iorder);
+    hipLaunchKernelGGL(k_pgs_color, dim3(nt), dim3(PGS_THREADS), 0, s, pab, p->dyn, p->prio, p->tlo, p->thi,
+                       p->iorder, p->claim, p->col, p->order, p->seg, p->tncol, p->ps);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+template <class T>
+int pgs_prepare_t(me_pgs *p, const void *warm, int *hit_out, hipStream_t s) {
+    const int C = p->C, NB = p->NB;
+    PgsScene<T> S = pgs_scene_of<T>(p);
+    T *lam = (T *)p->lam, *v = (T *)p->v, *vb = (T *)p->vbias;
+    if (C > 0) {
+        pgs_fill(p->ps + PS_BAD, 1, 0, s);
+        hipLaunchKernelGGL(k_pgs_validate, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, NB, S.pab, p->ps);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(p->hps, p->ps, sizeof(int) * PS_COUNT, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (p->hps[PS_BAD]) return fail(ME_EARG, "body_pairs holds an index outside [-1, n_bodies)");
+    }
+    hipLaunchKernelGGL(k_pgs_dyn<T>, dim3(nblk(NB)), dim3(ME_THREADS), 0, s, NB, S.Minv, p->dyn);
+    if (C > 0) hipLaunchKernelGGL(k_pgs_rho<T>, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, S, (T *)p->rho);
+    long long n0 = 3LL * C > 6LL * NB ? 3LL * C : 6LL * NB;
+    hipLaunchKernelGGL(k_pgs_lam_init<T>, dim3(nblk(n0)), dim3(ME_THREADS), 0, s, C, NB, (const T *)warm, lam, p->acc);
+    if (warm && C > 0)
+        hipLaunchKernelGGL(k_pgs_scatter<T>, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, S, p->dyn, lam, p->acc);
+    hipLaunchKernelGGL(k_pgs_apply<T>, dim3(nblk(6LL * NB)), dim3(ME_THREADS), 0, s, NB, S.vfree, p->acc, v, vb);
+    LAUNCH_CHECK();
+    if (C == 0) {
+        if (hit_out) *hit_out = 0;
+        return ME_OK;
+    }
+    bool hit = false;
+    if ((p->flags & ME_PGS_COLOR_CACHE) && C == p->cached_C) {
+        pgs_fill(p->ps + PS_MISMATCH, 1, 0, s);
+        int n = C > NB ? C : NB;
+        hipLaunchKernelGGL(k_pgs_cache_cmp<T>, dim3(nblk(n)), dim3(ME_THREADS), 0, s, C, NB, S, p->cpab, p->cfeat,
+                           (const T *)p->cMinv, p->ps);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(p->hps, p->ps, sizeof(int) * PS_COUNT, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        hit = p->hps[PS_MISMATCH] == 0;
+    }
+    if (hit) {
+        p->hits += 1;
+    } else {
+        int rc = pgs_color(p, s);
+        if (rc) return rc;
+        long long n = 36LL * NB > C ? 36LL * NB : C;
+        hipLaunchKernelGGL(k_pgs_cache_store<T>, dim3(nblk(n)), dim3(ME_THREADS), 0, s, C, NB, S, p->cpab, p->cfeat,
+                           (T *)p->cMinv);
+        LAUNCH_CHECK();
+        p->cached_C = C;
+        p->misses += 1;
+    }
+    if (hit_out) *hit_out = hit ? 1 : 0;
+    return ME_OK;
+}
+
+template <class T>
+int pgs_iterate_t(me_pgs *p, int iters, int use_bias, hipStream_t s) {
+    if (p->C == 0 || iters <= 0) return ME_OK;
+    PgsScene<T> S = pgs_scene_of<T>(p);
+    hipLaunchKernelGGL(k_pgs_sweeps<T>, dim3(pgs_ntile(p->C)), dim3(PGS_THREADS), 0, s, iters, S,
+                       pgs_soft_of<T>(p, use_bias), (p->flags & ME_PGS_DESAXCE) ? 1 : 0, (const T *)p->rho, p->dyn,
+                       p->tlo, p->thi, p->tncol, p->seg, p->order, (T *)p->lam, (T *)p->v, p->acc);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+template <class T>
+int pgs_residual_t(me_pgs *p, int use_bias, double *out, hipStream_t s) {
+    HIP_TRY(hipMemsetAsync(p->rmax, 0, sizeof(unsigned long long), s));
+    if (p->C > 0) {
+        PgsScene<T> S = pgs_scene_of<T>(p);
+        hipLaunchKernelGGL(k_pgs_residual<T>, dim3(nblk(p->C)), dim3(ME_THREADS), 0, s, p->C, S,
+                           pgs_soft_of<T>(p, use_bias), (p->flags & ME_PGS_DESAXCE) ? 1 : 0, (const T *)p->rho,
+                           (const T *)p->lam, (const T *)p->v, (T *)p->res, (T *)p->uout, p->rmax);
+        LAUNCH_CHECK();
+    }
+    HIP_TRY(hipMemcpyAsync(p->hrmax, p->rmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double r;
+    memcpy(&r, p->hrmax, sizeof(double));
+    *out = r;
+    return ME_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int me_workspace_size(const me_config *cfg, size_t *bytes) {
@@ -1031,6 +1255,127 @@ int me_adjoint_substep(me_engine *e, double dt, void *tape, size_t bytes, float 
                        e->cand_b, e->S.n, e->slot_off, e->cfg.margin, xbar, qbar);
     hipLaunchKernelGGL(k_adj_body, dim3(nblk(N)), dim3(ME_THREADS), 0, s, A, qbar);
     LAUNCH_CHECK();
+    return ME_OK;
+}
+
+// ── exact-cone PGS ──
+
+int me_pgs_workspace_size(int32_t n_bodies, int32_t max_contacts, int32_t real_bytes, size_t *bytes) {
+    if (!bytes) return fail(ME_EARG, "bytes is NULL");
+    if (n_bodies < 1 || max_contacts < 1) return fail(ME_EARG, "need n_bodies >= 1 and max_contacts >= 1");
+    if (real_bytes != 4 && real_bytes != 8) return fail(ME_EARG, "real_bytes must be 4 (float32) or 8 (float64)");
+    if ((long long)n_bodies + max_contacts > (1LL << 30)) return fail(ME_EARG, "n_bodies + max_contacts too large");
+    me_pgs tmp;
+    tmp.NB = n_bodies; tmp.MC = max_contacts; tmp.rb = real_bytes;
+    *bytes = pgs_layout(&tmp, nullptr);
+    return ME_OK;
+}
+
+int me_pgs_create(int32_t n_bodies, int32_t max_contacts, int32_t real_bytes, int32_t flags, void *workspace,
+                  size_t bytes, me_pgs **out) {
+    if (!out) return fail(ME_EARG, "out is NULL");
+    *out = nullptr;
+    size_t need = 0;
+    int rc = me_pgs_workspace_size(n_bodies, max_contacts, real_bytes, &need);
+    if (rc) return rc;
+    if (!workspace || bytes < need) return fail(ME_EARG, "workspace too small: need " + std::to_string(need) + " bytes");
+    if ((reinterpret_cast<uintptr_t>(workspace) & 255) != 0) return fail(ME_EARG, "workspace must be 256-byte aligned");
+    me_pgs *p = new me_pgs();
+    p->NB = n_bodies; p->MC = max_contacts; p->rb = real_bytes;
+    p->flags = flags & (ME_PGS_DESAXCE | ME_PGS_COLOR_CACHE);
+    pgs_layout(p, (char *)workspace);
+    if (hipHostMalloc((void **)&p->hps, sizeof(int) * PS_COUNT + sizeof(unsigned long long)) != hipSuccess) {
+        delete p;
+        return fail(ME_ERUNTIME, "hipHostMalloc failed");
+    }
+    p->hrmax = reinterpret_cast<unsigned long long *>(p->hps + PS_COUNT);
+    memset(p->hps, 0, sizeof(int) * PS_COUNT);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(ME_THREADS), 0, nullptr, p->ps, PS_COUNT, 0);
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipHostFree(p->hps);
+        delete p;
+        return fail(ME_ERUNTIME, std::string("me_pgs_create: ") + hipGetErrorString(e));
+    }
+    *out = p;
+    return ME_OK;
+}
+
+int me_pgs_destroy(me_pgs *p) {
+    if (!p) return ME_OK;
+    if (p->hps) (void)hipHostFree(p->hps);
+    delete p;
+    return ME_OK;
+}
+
+int me_pgs_prepare(me_pgs *p, const me_pgs_scene *sc, int32_t n_c, const void *warm, const double *soft,
+                   int32_t *cache_hit, void *stream) {
+    if (!p || !sc) return fail(ME_EARG, "NULL argument");
+    if (n_c < 0 || n_c > p->MC)
+        return fail(ME_EARG, std::to_string(n_c) + " contacts exceeds capacity " + std::to_string(p->MC));
+    if (!sc->Minv || !sc->v_free) return fail(ME_EARG, "scene Minv / v_free is NULL");
+    if (n_c > 0 && (!sc->Ja || !sc->Jb || !sc->mu || !sc->bias || !sc->body_pairs || !sc->feature))
+        return fail(ME_EARG, "scene contact array is NULL");
+    if (soft && n_c > 0 && !sc->sep) return fail(ME_EARG, "soft-step mode needs the separations (sep)");
+    p->sc = *sc;
+    p->C = n_c;
+    p->soft_on = soft ? 1 : 0;
+    for (int k = 0; k < 5; ++k) p->soft[k] = soft ? soft[k] : 0.0;
+    int hit = 0;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = p->rb == 8 ? pgs_prepare_t<double>(p, warm, &hit, s) : pgs_prepare_t<float>(p, warm, &hit, s);
+    if (cache_hit) *cache_hit = hit;
+    return rc;
+}
+
+int me_pgs_iterate(me_pgs *p, int32_t iters, int32_t use_bias, void *stream) {
+    if (!p) return fail(ME_EARG, "NULL solver");
+    if (iters < 0) return fail(ME_EARG, "iters must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    return p->rb == 8 ? pgs_iterate_t<double>(p, iters, use_bias, s) : pgs_iterate_t<float>(p, iters, use_bias, s);
+}
+
+int me_pgs_snapshot(me_pgs *p, void *stream) {
+    if (!p) return fail(ME_EARG, "NULL solver");
+    int n = 6 * p->NB * (p->rb / 4);
+    hipLaunchKernelGGL(k_pgs_copy<float>, dim3(nblk(n)), dim3(ME_THREADS), 0, (hipStream_t)stream, n,
+                       (const float *)p->v, (float *)p->vbias);
+    LAUNCH_CHECK();
+    return ME_OK;
+}
+
+int me_pgs_residual(me_pgs *p, int32_t use_bias, double *residual, void *stream) {
+    if (!p || !residual) return fail(ME_EARG, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    return p->rb == 8 ? pgs_residual_t<double>(p, use_bias, residual, s)
+                      : pgs_residual_t<float>(p, use_bias, residual, s);
+}
+
+int me_pgs_buffer(me_pgs *p, int32_t which, void **ptr, int64_t *count) {
+    if (!p || !ptr || !count) return fail(ME_EARG, "NULL argument");
+    const int64_t C = p->C, NB = p->NB;
+    switch (which) {
+        case ME_PGS_LAM: *ptr = p->lam; *count = 3 * C; break;
+        case ME_PGS_V: *ptr = p->v; *count = 6 * NB; break;
+        case ME_PGS_VBIAS: *ptr = p->vbias; *count = 6 * NB; break;
+        case ME_PGS_RHO: *ptr = p->rho; *count = C; break;
+        case ME_PGS_RES: *ptr = p->res; *count = C; break;
+        case ME_PGS_U: *ptr = p->uout; *count = 3 * C; break;
+        case ME_PGS_COLOR: *ptr = p->col; *count = C; break;
+        default: return fail(ME_EARG, "unknown buffer id " + std::to_string(which));
+    }
+    return ME_OK;
+}
+
+int me_pgs_info(me_pgs *p, int64_t *out) {
+    if (!p || !out) return fail(ME_EARG, "NULL argument");
+    HIP_TRY(hipMemcpy(p->hps, p->ps, sizeof(int) * PS_COUNT, hipMemcpyDeviceToHost));
+    out[ME_PGS_NCOLORS] = p->hps[PS_NCOL];
+    out[ME_PGS_CACHE_HITS] = p->hits;
+    out[ME_PGS_CACHE_MISSES] = p->misses;
+    out[ME_PGS_ISLANDS] = p->hps[PS_NISL];
+    out[ME_PGS_TILES] = p->hps[PS_NTILE];
+    out[ME_PGS_OVERFLOW] = p->hps[PS_OVF];
     return ME_OK;
 }
 

@@ -213,6 +213,72 @@ int me_ncp_sensitivity(int32_t n_env, int32_t n_c, int32_t n_dir, const double *
                        double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
                        int32_t *labels, int32_t *status, void *workspace, size_t bytes, void *stream);
 
+/* Exact-cone NCP projected Gauss-Seidel, the reference's NCPSolverGPU (ncp/ncp_gpu.py:783-1069):
+ * lambda_c <- proj_K(lambda_c - rho_c (u_c + Gamma(u_c))) over a deterministic colouring, de Saxce
+ * correction on or off, optional TGS soft-step normal row (ncp_gpu.py:491-557), per-body
+ * v = v_free + M^-1 J^T lambda from an int64 fixed-point total (scale 1e10). Reals are float
+ * (real_bytes 4) or double (8). The solver keeps its colouring between solves (colour cache:
+ * ME_PGS_COLOR_CACHE) and lives in one caller-owned workspace block. */
+typedef struct me_pgs me_pgs;
+
+#define ME_PGS_DESAXCE 1      /* Gamma(u) = (mu |u_t|, 0, 0) on; off = the convex relaxation */
+#define ME_PGS_COLOR_CACHE 2  /* reuse the colouring while pairs, features and masses are unchanged */
+
+/* A BlockScene (ncp_gpu.py:185-205) in device memory; row-major, reals of the solver's width. */
+typedef struct me_pgs_scene {
+    const void *Ja, *Jb;         /* [n_c,3,6] Jacobian blocks; Jb carries the minus of the pair */
+    const void *Minv;            /* [n_b,6,6] inverse mass, world frame */
+    const void *v_free;          /* [n_b,6] */
+    const void *mu;              /* [n_c] */
+    const void *bias;            /* [n_c,3] added to u */
+    const void *sep;             /* [n_c] separation; read only in soft-step mode (may be NULL else) */
+    const int32_t *body_pairs;   /* [n_c,2], -1 = static */
+    const int32_t *feature;      /* [n_c] stable feature ids (the colouring's canonical key) */
+} me_pgs_scene;
+
+/* status slots of me_pgs_info */
+enum { ME_PGS_NCOLORS = 0, ME_PGS_CACHE_HITS, ME_PGS_CACHE_MISSES, ME_PGS_ISLANDS, ME_PGS_TILES,
+       ME_PGS_OVERFLOW, ME_PGS_INFO_COUNT_ };
+/* buffers of me_pgs_buffer (device, solver width unless noted) */
+enum { ME_PGS_LAM = 0,  /* [n_c,3] impulses (n, t1, t2) */
+       ME_PGS_V,        /* [n_b,6] velocity after the iterations so far */
+       ME_PGS_VBIAS,    /* [n_b,6] velocity after the biased pass (me_pgs_snapshot) */
+       ME_PGS_RHO,      /* [n_c] 1 / lambda_max of the contact's Delassus block */
+       ME_PGS_RES,      /* [n_c] residual per contact (me_pgs_residual) */
+       ME_PGS_U,        /* [n_c,3] contact velocity (me_pgs_residual) */
+       ME_PGS_COLOR,    /* int32 [n_c] colour of each contact */
+       ME_PGS_BUF_COUNT_ };
+
+/* Workspace bytes for n_bodies bodies and at most max_contacts contacts. */
+int me_pgs_workspace_size(int32_t n_bodies, int32_t max_contacts, int32_t real_bytes, size_t *bytes);
+/* NCPSolverGPU.__init__ (ncp_gpu.py:790-840): capacity fixed at construction. */
+int me_pgs_create(int32_t n_bodies, int32_t max_contacts, int32_t real_bytes, int32_t flags, void *workspace,
+                  size_t bytes, me_pgs **out);
+int me_pgs_destroy(me_pgs *s);
+/* The set-up block of NCPSolverGPU.solve (ncp_gpu.py:980-998) and _color_cached (:928-947): rho per
+ * contact, lambda = warm ([n_c,3] device, NULL = zero) with its warm scatter, v = v_free + M^-1 J^T
+ * lambda, and the colouring (reused on a cache hit). soft: 5 host doubles (bias_rate, mass_scale,
+ * impulse_scale, inv_h, contact_speed) of SoftStep.params, NULL = the Baumgarte path of `bias`.
+ * *cache_hit (may be NULL) receives 1 on a colour-cache hit. The scene arrays are read again by
+ * me_pgs_iterate and me_pgs_residual: keep them unchanged until the last call for this scene.
+ * Fails with ME_EARG when body_pairs holds an index outside [-1, n_bodies). Synchronises `stream`
+ * (the index check, the cache test). */
+int me_pgs_prepare(me_pgs *s, const me_pgs_scene *scene, int32_t n_c, const void *warm, const double *soft,
+                   int32_t *cache_hit, void *stream);
+/* `iters` full sweeps (every colour of every tile), soft-step use_bias flag as given
+ * (the graph replays of _iter_launches, ncp_gpu.py:949-962, 1001-1036). One launch. */
+int me_pgs_iterate(me_pgs *s, int32_t iters, int32_t use_bias, void *stream);
+/* Copy the current velocity into ME_PGS_VBIAS (SolveResult.v_bias: the biased pass's velocity). */
+int me_pgs_snapshot(me_pgs *s, void *stream);
+/* Natural-map residual max_c ||lambda_c - proj_K(z_c)||_inf of the map the last pass iterated
+ * (use_bias as given), per-contact residuals and contact velocities (k_resid, ncp_gpu.py:601-646).
+ * Synchronises `stream`. */
+int me_pgs_residual(me_pgs *s, int32_t use_bias, double *residual, void *stream);
+int me_pgs_buffer(me_pgs *s, int32_t which, void **ptr, int64_t *count);
+/* out[ME_PGS_INFO_COUNT_]: colours of the current colouring, cache hits and misses, islands and tiles
+ * of the current colouring, overflow (1: more than 1024 colours). Synchronous. */
+int me_pgs_info(me_pgs *s, int64_t *out);
+
 /* Bytes of the caller-owned tape block (256-byte aligned device memory) for this engine. */
 int me_tape_size(me_engine *e, size_t *bytes);
 

@@ -1,0 +1,292 @@
+"""The exact-cone PGS on the HIP path (me_pgs_*) against the CPU oracle (oracle/pgs_oracle.c), and the
+reference's own gates for its NCPSolverGPU (tests/test_ncp_gpu_cone.py) run on it.
+
+Bar: bit identity with the oracle, float32 and float64 - impulses, velocities, the biased-pass
+velocity, rho, colours and the residual - on scenes that exercise one big island, many small islands
+packed into tiles, static bodies, warm starts, the soft step and the convex relaxation. Every test
+prints its case before the GPU work starts."""
+import importlib
+
+import numpy as np
+import pytest
+
+from oracle import pgs_oracle as PO
+
+pytestmark = pytest.mark.gpu
+
+
+def _pgs():
+    return importlib.import_module("3fold-motion-engine_amd.pgs")
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _gpu_solve(bs, dtype, iters, warm=None, soft=None, relax_iters=0, desaxce=True):
+    m = _pgs()
+    s = m.NCPSolver(bs.n_b, max(bs.n_c, 1), dtype=dtype, desaxce=desaxce)
+    r = s.solve(bs, iters=iters, warm=warm, soft=soft, relax_iters=relax_iters)
+    return s, r
+
+
+def _bits_equal(s, r, o, relax_iters):
+    t = np.float64 if s.dtype == "float64" else np.float32
+    lam = s.lam_device()[:bs_n(s)].cpu().numpy()
+    v = s.v_device().cpu().numpy()
+    assert lam.dtype == t
+    assert lam.tobytes() == o.lam.tobytes(), f"lambda differs: max {np.max(np.abs(lam - o.lam)):.3e}"
+    assert v.tobytes() == o.v.tobytes(), f"v differs: max {np.max(np.abs(v - o.v)):.3e}"
+    assert s.rho().astype(t).tobytes() == o.rho.tobytes(), "rho differs"
+    col = s.coloring()[0]
+    assert np.array_equal(col, o.col), "colours differ"
+    assert s.n_colors == o.n_colors
+    if relax_iters > 0:
+        assert r.v_bias.astype(t).tobytes() == o.v_bias.tobytes(), "v_bias differs"
+    assert r.residual == o.residual or (np.isnan(r.residual) and np.isnan(o.residual))
+
+
+def bs_n(s):
+    return s.C
+
+
+SCENES = ["wellposed", "synthetic256", "synthetic1024", "synthetic4096", "boxstack", "lattice2000"]
+
+
+def _scene(name):
+    m = _pgs()
+    if name == "wellposed":
+        return m.random_wellposed_scene(n_c=8, n_b=8, seed=1)
+    if name == "synthetic256":
+        return m.synthetic_scene(256, n_b=64, seed=7)
+    if name == "synthetic1024":
+        return m.synthetic_scene(1024, n_b=128, seed=2)
+    if name == "synthetic4096":
+        return m.synthetic_scene(4096, n_b=512, seed=3)
+    if name == "boxstack":   # 3-box column plus a loose box: ground contacts, box-box manifolds
+        w = m.BoxWorld(dt=1.0 / 240.0, mu=0.5, iters=1)
+        for k in range(3):
+            w.add([0.1, 0.1, 0.1], 1.0 + k, [0.0, 0.0, 0.1 + 0.2 * k - 1e-3])
+        w.add([0.1, 0.1, 0.1], 2.0, [0.5, 0.0, 0.0999])
+        return w.build_scene()
+    if name == "lattice2000":
+        return _lattice_block_scene(2000)
+    raise KeyError(name)
+
+
+def _lattice_block_scene(n):
+    """The LatticeWorld scene (its build_scene) with a pushed fraction, without a GPU solver."""
+    m = _pgs()
+    orig = m.NCPSolver
+    try:
+        m.NCPSolver = lambda *a, **k: None
+        w = m.LatticeWorld(n=n, iters=40)
+    finally:
+        m.NCPSolver = orig
+    w.v[:, 0] = 0.01 * np.sin(np.arange(n))
+    f = m.lattice_pushed_forces(w)
+    return w.build_scene(f_ext=f)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("name", SCENES)
+def test_bit_identical_to_oracle(name, dtype):
+    bs = _scene(name)
+    iters = 60
+    _say("pgs parity", name, dtype, f"n_c={bs.n_c} n_b={bs.n_b} iters={iters}")
+    s, r = _gpu_solve(bs, dtype, iters)
+    o = PO.solve(bs, iters=iters, dtype=dtype)
+    _bits_equal(s, r, o, 0)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_bit_identical_warm_soft_relax(dtype):
+    m = _pgs()
+    bs = _scene("boxstack")
+    rng = np.random.default_rng(5)
+    warm = np.abs(rng.normal(scale=0.05, size=(bs.n_c, 3)))
+    sp = m.SoftStep().params(1.0 / 240.0)
+    _say("pgs parity warm + soft + relax", dtype, bs.n_c)
+    s, r = _gpu_solve(bs, dtype, 30, warm=warm, soft=sp, relax_iters=11)
+    o = PO.solve(bs, iters=30, warm=warm, soft=sp, relax_iters=11, dtype=dtype)
+    _bits_equal(s, r, o, 11)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_bit_identical_convex_relaxation_warm(dtype):
+    bs = _scene("synthetic1024")
+    warm = np.zeros((bs.n_c, 3))
+    warm[:, 0] = 0.01
+    _say("pgs parity desaxce off, warm", dtype)
+    s, r = _gpu_solve(bs, dtype, 40, warm=warm, desaxce=False)
+    o = PO.solve(bs, iters=40, warm=warm, desaxce=False, dtype=dtype)
+    _bits_equal(s, r, o, 0)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_bit_identical_two_runs_and_permutation(dtype):
+    """Reference gates (a) and the permutation gate: two solvers, same bytes; shuffled input order,
+    same answer (the colouring is a function of the sorted keys)."""
+    m = _pgs()
+    bs = m.synthetic_scene(256, n_b=64, seed=11)
+    _say("pgs determinism + permutation", dtype)
+    outs = []
+    for _ in range(2):
+        s = m.NCPSolver(bs.n_b, bs.n_c, dtype=dtype)
+        s.solve(bs, iters=300)
+        outs.append(s.raw_bytes())
+    assert outs[0] == outs[1]
+    p = np.random.default_rng(5).permutation(bs.n_c)
+    bp = m.BlockScene(Ja=bs.Ja[p], Jb=bs.Jb[p], Minv=bs.Minv, v_free=bs.v_free, mu=bs.mu[p],
+                      body_pairs=bs.body_pairs[p], dt=bs.dt, bias=bs.bias[p], feature=bs.feature[p])
+    s1 = m.NCPSolver(bs.n_b, bs.n_c, dtype=dtype)
+    r1 = s1.solve(bs, iters=300)
+    s2 = m.NCPSolver(bs.n_b, bs.n_c, dtype=dtype)
+    r2 = s2.solve(bp, iters=300)
+    assert r1.lam[p].tobytes() == r2.lam.tobytes()
+    assert r1.v.tobytes() == r2.v.tobytes()
+
+
+def test_coloring_is_a_proper_coloring():
+    m = _pgs()
+    bs = m.synthetic_scene(1024, n_b=128, seed=2)
+    _say("pgs proper colouring")
+    s = m.NCPSolver(bs.n_b, bs.n_c)
+    s.solve(bs, iters=10, measure_residual=False)
+    col, order, cstart, ccount = s.coloring()
+    for ci in range(s.n_colors):
+        idx = order[cstart[ci]:cstart[ci] + ccount[ci]]
+        bodies = np.concatenate([bs.body_pairs[idx, 0], bs.body_pairs[idx, 1]])
+        bodies = bodies[bodies >= 0]
+        assert len(np.unique(bodies)) == len(bodies), f"colour {ci} reuses a body"
+    assert sum(ccount) == bs.n_c
+
+
+def test_color_cache_hit_and_invalidation():
+    m = _pgs()
+    bs = m.synthetic_scene(256, n_b=64, seed=4)
+    _say("pgs colour cache")
+    s = m.NCPSolver(bs.n_b, bs.n_c)
+    r0 = s.solve(bs, iters=50)
+    r1 = s.solve(bs, iters=50)
+    assert r0.cache_hit is False and r1.cache_hit is True
+    assert r0.lam.tobytes() == r1.lam.tobytes(), "cache hit changed the answer"
+    bs2 = m.BlockScene(Ja=bs.Ja, Jb=bs.Jb, Minv=bs.Minv, v_free=bs.v_free, mu=bs.mu,
+                       body_pairs=np.roll(bs.body_pairs, 1, axis=0), dt=bs.dt, bias=bs.bias, feature=bs.feature)
+    r2 = s.solve(bs2, iters=50)
+    assert r2.cache_hit is False, "changed body pairs did not invalidate the colour cache"
+    assert s.cache_hits == 1 and s.cache_misses == 2
+
+
+def test_velocity_is_a_pure_function_of_lambda():
+    """1 x 1200 iterations == 3 x 400 warm-restarted, byte for byte (the fixed point holds the total)."""
+    m = _pgs()
+    bs = m.synthetic_scene(256, n_b=64, seed=7)
+    _say("pgs warm restart identity")
+    ra = m.NCPSolver(bs.n_b, bs.n_c, dtype="float64").solve(bs, iters=1200)
+    s2 = m.NCPSolver(bs.n_b, bs.n_c, dtype="float64")
+    rb = s2.solve(bs, iters=400)
+    for _ in range(2):
+        rb = s2.solve(bs, iters=400, warm=rb.lam)
+    assert ra.lam.tobytes() == rb.lam.tobytes()
+    assert ra.v.tobytes() == rb.v.tobytes()
+
+
+def test_fixed_point_drift_does_not_compound():
+    m = _pgs()
+    w = m.BoxWorld(dt=0.004, mu=0.5, iters=1)
+    w.add([0.1, 0.1, 0.1], 15.0, [0.0, 0.0, 0.1])
+    bs = w.build_scene(f_ext=lambda i: np.array([100.0, 0.0, 0.0]))
+    _say("pgs fixed-point drift")
+    out = []
+    for it in (500, 5000, 20000):
+        r = m.solve_scene_gpu(bs, iters=it, dtype="float64")
+        v_exact = bs.v_free.copy()
+        for c in range(bs.n_c):
+            a = bs.body_pairs[c, 0]
+            v_exact[a] += bs.Minv[a] @ (bs.Ja[c].T @ r.lam[c])
+        out.append(float(np.max(np.abs(r.v - v_exact))))
+    d = np.array(out)
+    assert d.max() < bs.n_c / 1e10
+    assert d[-1] <= 4.0 * max(d[0], 1e-10)
+
+
+def test_matches_cpu_ncp_conditions():
+    """Well-posed scene (G SPD, unique lambda), driven to convergence: float64 satisfies the NCP
+    conditions on the dense G of the host; float32 within 1e-5 of float64."""
+    m = _pgs()
+    bs = m.random_wellposed_scene(n_c=8, n_b=8, seed=1)
+    _say("pgs converged NCP conditions")
+    r64 = m.solve_scene_gpu(bs, iters=5000, dtype="float64")
+    r32 = m.solve_scene_gpu(bs, iters=5000, dtype="float32")
+    G, b = bs.delassus(), bs.rhs()
+    lam = r64.lam.reshape(-1)
+    u = (G @ lam + b).reshape(-1, 3)
+    L = lam.reshape(-1, 3)
+    mu = bs.mu
+    ut = np.hypot(u[:, 1], u[:, 2])
+    s = np.stack([u[:, 0] + mu * ut, u[:, 1], u[:, 2]], axis=1)
+    assert np.all(np.hypot(L[:, 1], L[:, 2]) <= mu * L[:, 0] + 1e-12)      # lambda in K
+    assert np.all(mu * np.hypot(s[:, 1], s[:, 2]) <= s[:, 0] + 1e-9)       # s in K*
+    assert np.max(np.abs(np.sum(L * s, axis=1))) < 1e-10                    # complementarity
+    assert np.max(np.abs(r32.lam - r64.lam)) < 1e-5
+
+
+def test_scene_sliding_box_gliding_number():
+    """(ii) the sliding box: exact cone stays on the ground, the convex relaxation glides - the
+    reference's recorded gaps (data/ncp/gpu_scene_reference.json) to 7 digits."""
+    m = _pgs()
+    _say("pgs sliding box")
+    _, g1, _ = m.scene_sliding_box(steps=200, iters=400, desaxce=True)
+    _, g2, _ = m.scene_sliding_box(steps=200, iters=400, desaxce=False)
+    assert abs(g1.max()) < 1e-8
+    assert abs(g1.max() - 1.552204e-11) < 1e-17
+    assert abs(g2.max() - 0.004352523) < 1e-9
+    assert abs(g2[0] - 0.001132727) < 1e-9
+
+
+def test_scene_pushed_cube():
+    m = _pgs()
+    _say("pgs pushed cube")
+    w, traj, pen, res = m.scene_pushed_cube(steps=250, iters=400)
+    assert w.n_contacts == 4 and w.n_colors == 4
+    assert pen.max() < 1e-4 and res.max() < 1e-4
+    assert abs(traj[-1, 2] - 0.1) < 1e-4
+
+
+def test_world_trajectory_bit_identical_to_oracle_world():
+    """A tower world stepped on the GPU and on the oracle: identical positions after 40 steps (the
+    host geometry is shared, so any solver difference would show)."""
+    m = _pgs()
+
+    class OW(m.BoxWorld):
+        def _make_solver(self, n, cap):
+            return PO.OracleNCPSolver(n, cap, dtype=self.dtype, desaxce=self.desaxce)
+
+    _say("pgs tower world vs oracle world")
+    wg, _ = m.scene_tower(4, steps=40, iters=50)
+    wo, _ = m.scene_tower(4, steps=40, iters=50, world_cls=OW)
+    xg = np.array([b.x for b in wg.bodies])
+    xo = np.array([b.x for b in wo.bodies])
+    assert xg.tobytes() == xo.tobytes(), f"max |dx| {np.max(np.abs(xg - xo)):.3e}"
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_soft_bit_identical_two_runs(dtype):
+    m = _pgs()
+    _say("pgs soft two runs", dtype)
+    b = []
+    for _ in range(2):
+        w, _pen = m.scene_tower(4, dtype=dtype, steps=25, iters=8, substeps=4, soft=m.SoftStep(hertz=240.0))
+        b.append(np.array([bd.x for bd in w.bodies]).tobytes())
+    assert b[0] == b[1]
+
+
+def test_bad_body_index_is_an_argument_error():
+    m = _pgs()
+    bs = m.synthetic_scene(16, n_b=8, seed=1)
+    bs.body_pairs[3, 1] = 8
+    _say("pgs bad body index")
+    s = m.NCPSolver(bs.n_b, bs.n_c)
+    with pytest.raises(ValueError, match="outside"):
+        s.solve(bs, iters=5)
