@@ -757,7 +757,8 @@ struct me_pgs {
     int *dyn = nullptr, *bucket = nullptr, *bcnt = nullptr, *boff = nullptr, *bfill = nullptr, *items = nullptr,
         *par = nullptr, *isl = nullptr, *icnt = nullptr, *ioff = nullptr, *ifill = nullptr, *iorder = nullptr,
         *col = nullptr, *order = nullptr, *tlo = nullptr, *thi = nullptr, *tncol = nullptr, *seg = nullptr,
-        *cpab = nullptr, *cfeat = nullptr, *ps = nullptr, *bsum = nullptr;
+        *cpab = nullptr, *cfeat = nullptr, *ps = nullptr, *bsum = nullptr, *tbcnt = nullptr, *tbstart = nullptr,
+        *tbody = nullptr, *bslot = nullptr, *btile = nullptr;
     int nbsum = 0;
     int *hps = nullptr;  // pinned host mirror of ps
     unsigned long long *hrmax = nullptr;
@@ -801,6 +802,11 @@ size_t pgs_layout(me_pgs *s, char *base) {
     s->thi = B.take<int>(NT);
     s->tncol = B.take<int>(NT);
     s->seg = B.take<int>(NT * (PGS_NCOLMAX + 1));
+    s->tbcnt = B.take<int>(NT);
+    s->tbstart = B.take<int>(NT);
+    s->tbody = B.take<int>(NB);
+    s->bslot = B.take<int>(NB);
+    s->btile = B.take<int>(NB);
     s->cpab = B.take<int>(2 * MC);
     s->cfeat = B.take<int>(MC);
     s->ps = B.take<int>(PS_COUNT);
@@ -856,6 +862,7 @@ int pgs_color(me_pgs *p, hipStream_t s) {
     pgs_fill(p->ifill, NI, 0, s);
     pgs_fill(p->tlo, nt, INT_MAX, s);
     pgs_fill(p->thi, nt, INT_MIN, s);
+    pgs_fill(p->tbcnt, nt, 0, s);
     hipLaunchKernelGGL(k_pgs_keys, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, NB, pab, p->sc.feature, p->key, p->bucket,
                        p->bcnt);
     int rc = pgs_scan(p, p->bcnt, p->boff, NB + 1, s);
@@ -874,6 +881,12 @@ int pgs_color(me_pgs *p, hipStream_t s) {
                        p->ps);
     hipLaunchKernelGGL(k_pgs_island_fill, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, p->isl, p->ioff, p->ifill,
                        p->iorder);
+    hipLaunchKernelGGL(k_pgs_body_slot, dim3(nblk(NB)), dim3(ME_THREADS), 0, s, NB, p->dyn, p->par, p->icnt, p->ioff,
+                       p->tbcnt, p->bslot, p->btile);
+    rc = pgs_scan(p, p->tbcnt, p->tbstart, nt, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pgs_body_fill, dim3(nblk(NB)), dim3(ME_THREADS), 0, s, NB, p->bslot, p->btile, p->tbstart,
+                       p->tbody);
     hipLaunchKernelGGL(k_pgs_color, dim3(nt), dim3(PGS_THREADS), 0, s, pab, p->dyn, p->prio, p->tlo, p->thi,
                        p->iorder, p->claim, p->col, p->order, p->seg, p->tncol, p->ps);
     LAUNCH_CHECK();
@@ -936,9 +949,11 @@ template <class T>
 int pgs_iterate_t(me_pgs *p, int iters, int use_bias, hipStream_t s) {
     if (p->C == 0 || iters <= 0) return ME_OK;
     PgsScene<T> S = pgs_scene_of<T>(p);
-    hipLaunchKernelGGL(k_pgs_sweeps<T>, dim3(pgs_ntile(p->C)), dim3(PGS_THREADS), 0, s, iters, S,
+    const size_t lds = (size_t)6 * PGS_LDS_BODIES * (sizeof(int64_t) + sizeof(T));
+    hipLaunchKernelGGL(k_pgs_sweeps<T>, dim3(pgs_ntile(p->C)), dim3(PGS_THREADS), lds, s, iters, S,
                        pgs_soft_of<T>(p, use_bias), (p->flags & ME_PGS_DESAXCE) ? 1 : 0, (const T *)p->rho, p->dyn,
-                       p->tlo, p->thi, p->tncol, p->seg, p->order, (T *)p->lam, (T *)p->v, p->acc);
+                       p->tlo, p->thi, p->tncol, p->seg, p->order, p->tbstart, p->tbcnt, p->tbody, p->bslot,
+                       (T *)p->lam, (T *)p->v, p->acc);
     LAUNCH_CHECK();
     return ME_OK;
 }

@@ -216,38 +216,6 @@ __device__ __forceinline__ void pgs_u(const PgsScene<T> &S, const T *v, int c, i
     if (b >= 0) pgs_Jv<T>(S.Jb + 18 * (size_t)c, v + 6 * (size_t)b, u0, u1, u2);
 }
 
-// replace body i's share of contact c in the fixed-point total, then rebuild v_i (f_swap + k_apply_c)
-template <class T>
-__device__ __forceinline__ void pgs_swap(const PgsScene<T> &S, const T *J, int i, T n0, T n1, T n2, T o0, T o1, T o2,
-                                         T *v, int64_t *acc) {
-    const T *M = S.Minv + 36 * (size_t)i;
-    T dn[6], dd[6];
-    pgs_MJtz<T>(M, J, n0, n1, n2, dn);
-    pgs_MJtz<T>(M, J, o0, o1, o2, dd);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        int64_t q = pgs_q((double)dn[k]) - pgs_q((double)dd[k]);
-        int64_t t = acc[6 * (size_t)i + k] + q;
-        acc[6 * (size_t)i + k] = t;
-        v[6 * (size_t)i + k] = S.vfree[6 * (size_t)i + k] + (T)((double)t / PGS_SCALE);
-    }
-}
-
-// one contact of a colour sweep (k_sweep, ncp_gpu.py:491-566, with k_apply_c fused)
-template <class T>
-__device__ __forceinline__ void pgs_contact(const PgsScene<T> &S, const PgsSoft<T> &sp, int desaxce, const T *rho,
-                                            const int *dyn, T *lam, T *v, int64_t *acc, int c) {
-    int a = S.pab[2 * c], b = S.pab[2 * c + 1];
-    T u0, u1, u2;
-    pgs_u<T>(S, v, c, a, b, u0, u1, u2);
-    T lo0 = lam[3 * c], lo1 = lam[3 * c + 1], lo2 = lam[3 * c + 2];
-    T z0, z1, z2;
-    pgs_z<T>(S, sp, desaxce, c, rho[c], u0, u1, u2, lo0, lo1, lo2, z0, z1, z2);
-    lam[3 * c] = z0; lam[3 * c + 1] = z1; lam[3 * c + 2] = z2;
-    if (a >= 0 && dyn[a]) pgs_swap<T>(S, S.Ja + 18 * (size_t)c, a, z0, z1, z2, lo0, lo1, lo2, v, acc);
-    if (b >= 0 && dyn[b]) pgs_swap<T>(S, S.Jb + 18 * (size_t)c, b, z0, z1, z2, lo0, lo1, lo2, v, acc);
-}
-
 // ── set-up ──
 
 // body indices in [-1, NB): every later kernel indexes bodies by them
@@ -572,22 +540,149 @@ __global__ void __launch_bounds__(PGS_THREADS) k_pgs_color(const int *pab, const
     }
 }
 
-// ── the sweeps: one workgroup per tile, every iteration, one barrier per colour ──
+// ── the sweeps ──
+
+// dynamic bodies of each tile: their slot in the tile (LDS index) and the tile's body list
+__global__ void k_pgs_body_slot(int NB, const int *dyn, int *par, const int *icnt, const int *ioff, int *tbcnt,
+                                int *bslot, int *btile) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NB) return;
+    int sl = -1, t = -1;
+    if (dyn[i]) {
+        int r = pgs_find(par, i);
+        if (icnt[r] > 0) {
+            t = ioff[r] / PGS_TILE;
+            sl = atomicAdd(&tbcnt[t], 1);
+        }
+    }
+    bslot[i] = sl;
+    btile[i] = t;
+}
+__global__ void k_pgs_body_fill(int NB, const int *bslot, const int *btile, const int *tbstart, int *tbody) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NB || bslot[i] < 0) return;
+    tbody[tbstart[btile[i]] + bslot[i]] = i;
+}
+
+#define PGS_LDS_BODIES 640  // dynamic bodies of a tile kept in LDS (velocity and int64 total)
+
+// One contact by the four lanes of a quad (lane r: side r >> 1 = body a / body b, half r & 1 = rows
+// 0-2 / 3-5 of that body's six components). Each side forms its J v; the quad exchanges the two
+// partial products and every lane forms u and the new lambda the same way; each lane then replaces
+// its three components of its body's total and rebuilds them. Every output is computed by the same
+// operations, in the same order, as when one thread updates the whole contact.
+template <class T>
+__device__ __forceinline__ void pgs_contact_quad(const PgsScene<T> &S, const PgsSoft<T> &sp, int desaxce,
+                                                 const T *rho, const int *dyn, const int *bslot, T *lam, T *gv,
+                                                 int64_t *gacc, T *lv, int64_t *lacc, bool lds, int c, int r) {
+    const int s = r >> 1, h = r & 1;
+    const int a = S.pab[2 * c], b = S.pab[2 * c + 1];
+    const int i = s ? b : a;
+    const T *J = (s ? S.Jb : S.Ja) + 18 * (size_t)c;
+    const bool has = i >= 0;
+    const bool dy = has && dyn[i];
+    T *vi = nullptr;
+    int64_t *ai = nullptr;
+    if (has) {
+        if (lds && dy) {
+            int sl = bslot[i];
+            vi = lv + 6 * sl;
+            ai = lacc + 6 * sl;
+        } else {
+            vi = gv + 6 * (size_t)i;
+            ai = gacc + 6 * (size_t)i;
+        }
+    }
+    T P[3] = {(T)0.0, (T)0.0, (T)0.0};
+    if (has) {
+        T x[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) x[k] = vi[k];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            T t = J[6 * q] * x[0];
+#pragma unroll
+            for (int k = 1; k < 6; ++k) t = t + J[6 * q + k] * x[k];
+            P[q] = t;
+        }
+    }
+    const int base = __lane_id() & ~3;
+    T pa[3], pb[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        pa[q] = __shfl(P[q], base);
+        pb[q] = __shfl(P[q], base + 2);
+    }
+    T u0 = S.bias[3 * c], u1 = S.bias[3 * c + 1], u2 = S.bias[3 * c + 2];
+    if (a >= 0) { u0 = u0 + pa[0]; u1 = u1 + pa[1]; u2 = u2 + pa[2]; }
+    if (b >= 0) { u0 = u0 + pb[0]; u1 = u1 + pb[1]; u2 = u2 + pb[2]; }
+    const T lo0 = lam[3 * c], lo1 = lam[3 * c + 1], lo2 = lam[3 * c + 2];
+    T z0, z1, z2;
+    pgs_z<T>(S, sp, desaxce, c, rho[c], u0, u1, u2, lo0, lo1, lo2, z0, z1, z2);
+    if (r == 0) { lam[3 * c] = z0; lam[3 * c + 1] = z1; lam[3 * c + 2] = z2; }
+    if (!dy) return;
+    T g[6], go[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        g[k] = (J[k] * z0 + J[6 + k] * z1) + J[12 + k] * z2;
+        go[k] = (J[k] * lo0 + J[6 + k] * lo1) + J[12 + k] * lo2;
+    }
+    const T *M = S.Minv + 36 * (size_t)i;
+    const T *vf = S.vfree + 6 * (size_t)i;
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+        const int row = 3 * h + kk;
+        T dn = M[6 * row] * g[0], dd = M[6 * row] * go[0];
+#pragma unroll
+        for (int j = 1; j < 6; ++j) {
+            dn = dn + M[6 * row + j] * g[j];
+            dd = dd + M[6 * row + j] * go[j];
+        }
+        int64_t t = ai[row] + (pgs_q((double)dn) - pgs_q((double)dd));
+        ai[row] = t;
+        vi[row] = vf[row] + (T)((double)t / PGS_SCALE);
+    }
+}
+
+// One workgroup per tile, every iteration, one barrier per colour; a quad per contact. The tile's
+// dynamic bodies (velocity, int64 total) sit in LDS for the launch when they fit, else in global
+// memory; static bodies are read from global memory (never written).
 template <class T>
 __global__ void __launch_bounds__(PGS_THREADS) k_pgs_sweeps(int iters, PgsScene<T> S, PgsSoft<T> sp, int desaxce,
                                                            const T *rho, const int *dyn, const int *tlo,
                                                            const int *thi, const int *tncol, const int *seg,
-                                                           const int *order, T *lam, T *v, int64_t *acc) {
+                                                           const int *order, const int *tbstart, const int *tbcnt,
+                                                           const int *tbody, const int *bslot, T *lam, T *v,
+                                                           int64_t *acc) {
+    extern __shared__ __align__(16) unsigned char pgs_smem[];
     const int t = blockIdx.x, tid = threadIdx.x;
     if (tlo[t] >= thi[t]) return;
     const int nc = tncol[t];
     const int *sg = seg + (size_t)t * (PGS_NCOLMAX + 1);
+    const int nb = tbcnt[t], b0 = tbstart[t];
+    const bool lds = nb <= PGS_LDS_BODIES;
+    int64_t *lacc = reinterpret_cast<int64_t *>(pgs_smem);
+    T *lv = reinterpret_cast<T *>(pgs_smem + sizeof(int64_t) * 6 * PGS_LDS_BODIES);
+    if (lds)
+        for (int k = tid; k < 6 * nb; k += PGS_THREADS) {
+            size_t g = 6 * (size_t)tbody[b0 + k / 6] + k % 6;
+            lacc[k] = acc[g];
+            lv[k] = v[g];
+        }
+    __syncthreads();
+    const int quad = tid >> 2, r = tid & 3;
     for (int it = 0; it < iters; ++it)
         for (int k = 0; k < nc; ++k) {
             const int e = sg[k + 1];
-            for (int j = sg[k] + tid; j < e; j += PGS_THREADS)
-                pgs_contact<T>(S, sp, desaxce, rho, dyn, lam, v, acc, order[j]);
+            for (int j = sg[k] + quad; j < e; j += PGS_THREADS / 4)
+                pgs_contact_quad<T>(S, sp, desaxce, rho, dyn, bslot, lam, v, acc, lv, lacc, lds, order[j], r);
             __syncthreads();
+        }
+    if (lds)
+        for (int k = tid; k < 6 * nb; k += PGS_THREADS) {
+            size_t g = 6 * (size_t)tbody[b0 + k / 6] + k % 6;
+            acc[g] = lacc[k];
+            v[g] = lv[k];
         }
 }
 

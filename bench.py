@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large", "adjoint", "ncp"])
+    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large", "adjoint", "ncp", "pgs"])
     ap.add_argument("--rollouts", type=int, default=3, help="adjoint workload: timed forward+backward rollouts")
     ap.add_argument("--extras", type=int, default=1,
                     help="pile workload: also measure the 256-scene batch and the config-4 adjoint and attach "
@@ -99,6 +99,9 @@ def main():
         return
     if a.workload == "ncp":
         print(json.dumps(bench_ncp(a)), flush=True)
+        return
+    if a.workload == "pgs":
+        print(json.dumps(bench_pgs(a)), flush=True)
         return
     if a.workload == "large":
         a.bodies = 100000 if a.bodies == 10000 else a.bodies
@@ -238,6 +241,8 @@ def main():
                                 keys=("value", "ms_per_step", "adjoint", "config"))
         out["ncp_sensitivity"] = _extra(bench_ncp, a, keys=("value", "unit", "ms_per_call", "vs_reference",
                                                             "config", "cpu_baseline"))
+        out["exact_cone_pgs"] = _extra(bench_pgs, a, keys=("value", "unit", "vs_reference", "solves", "kernel",
+                                                           "lattice_10000", "cpu_baseline"))
     print(json.dumps(out), flush=True)
     if a.calibrate_copy:
         _copy_kernel(a.calibrate_copy)
@@ -385,6 +390,101 @@ def bench_ncp(a):
             "vs_reference": round(value / 68488.0, 1), "dtype": "f64", "data": "synthetic (exactly solved problems)",
             "config": {"workload": f"{E} stance problems x {n_c} contacts, nv 18, dlam/db along {m} directions + "
                                    f"dlam/dmu, one launch", "envs": E, "contacts": n_c, "directions": m},
+            "cpu_baseline": cpu}
+
+
+def pgs_algorithmic_bytes(bs, real_bytes):
+    """Compulsory traffic of one sweep over every contact: per contact its order slot and body pair
+    (12 B), rho, mu, bias and lambda read + write (9 reals); per present side its 3x6 block and the
+    body's velocity (24 reals); per dynamic side the inverse mass, v_free, the written velocity
+    (48 reals) and the int64 total read + written (96 B)."""
+    R = real_bytes
+    pab = np.asarray(bs.body_pairs)
+    dyn = np.abs(bs.Minv).reshape(bs.n_b, 36).max(axis=1) > 0
+    side = (pab >= 0)
+    dside = side & dyn[np.maximum(pab, 0)]
+    return int(bs.n_c * (12 + 9 * R) + side.sum() * 24 * R + dside.sum() * (48 * R + 96))
+
+
+def bench_pgs(a):
+    """The exact-cone PGS (NCPSolverGPU's selftest workload, ncp_gpu.py:1872-1880): a full solve() of
+    synthetic_scene(n_c) over 512 bodies with 50 iterations, colour-cache hit, inputs uploaded from the
+    host each call as in the reference, at n_c = 4, 32, 256, 4096; the device-resident 50-iteration
+    launch at n_c = 4096 timed with HIP events (its roofline); and the N = 10 000 two-layer lattice
+    step (host-side scene build + solve, 120 iterations, float32). The reference's numbers are
+    0.73 / 1.44 / 4.32 / 47.5 ms per solve and 49.8 ms per lattice step (gpu_scene_reference.json)."""
+    import torch
+    m = importlib.import_module("3fold-motion-engine_amd.pgs")
+    ref_ms = {4: 0.73, 32: 1.44, 256: 4.32, 4096: 47.5}
+    solves = {}
+    for n in (4, 32, 256, 4096):
+        sc = m.synthetic_scene(n, seed=a.seed)
+        s = m.NCPSolver(sc.n_b, n)
+        for _ in range(3):
+            s.solve(sc, iters=50, measure_residual=False)
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = s.solve(sc, iters=50, measure_residual=False)
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        assert r.cache_hit
+        solves[n] = {"ms": round(ms, 4), "reference_ms": ref_ms[n], "speedup": round(ref_ms[n] / ms, 1),
+                     "colours": s.n_colors}
+    # device-resident launch of 50 sweeps at n_c = 4096 (the dominant kernel, k_pgs_sweeps)
+    sc = m.synthetic_scene(4096, seed=a.seed)
+    s = m.NCPSolver(sc.n_b, sc.n_c)
+    s.solve(sc, iters=1, measure_residual=False)
+    for _ in range(3):
+        s._iterate(50, 1)
+    torch.cuda.synchronize()
+    reps = 20
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        s._iterate(50, 1)
+    ev1.record()
+    torch.cuda.synchronize()
+    k_ms = ev0.elapsed_time(ev1) / reps
+    ab = 50 * pgs_algorithmic_bytes(sc, 4)
+    gbs = ab / (k_ms * 1e-3) / 1e9
+    # the lattice step
+    w = m.LatticeWorld(n=10000, iters=120)
+    f = m.lattice_pushed_forces(w)
+    for _ in range(4):
+        w.step(f_ext=f)
+    torch.cuda.synchronize()
+    ts, tsolve = [], []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        r = w.step(f_ext=f)
+        ts.append((time.perf_counter() - t0) * 1e3)
+        tsolve.append(sum(r.ms.values()))
+    lat_ms = float(np.median(ts))
+    cpu = None
+    if not a.no_cpu:
+        from oracle import pgs_oracle as PO
+        t0 = time.perf_counter()
+        nrep = 0
+        while time.perf_counter() - t0 < 10.0:
+            PO.solve(sc, iters=50, dtype="float32")
+            nrep += 1
+        cpu_ms = (time.perf_counter() - t0) / nrep * 1e3
+        cpu = {"value": round(cpu_ms, 3), "unit": "ms per 50-iteration solve (n_c = 4096)", "cores": 1,
+               "kind": "port", "sample": f"{nrep} solves of the n_c = 4096 scene through the C oracle"}
+    return {"metric": "exact-cone PGS ms per 50-iteration solve (n_c = 4096, 512 bodies, colour-cache hit)",
+            "value": solves[4096]["ms"], "unit": "ms", "higher_is_better": False,
+            "vs_reference": round(ref_ms[4096] / solves[4096]["ms"], 1), "dtype": "f32",
+            "data": "synthetic (the reference's synthetic_scene)", "solves": {str(k): v for k, v in solves.items()},
+            "kernel": {"name": "k_pgs_sweeps", "ms_per_50_iterations": round(k_ms, 4),
+                       "contact_updates_per_s": round(4096 * 50 / (k_ms * 1e-3), 1),
+                       "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                                    "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                                    "algorithmic_bytes_per_launch": ab}},
+            "lattice_10000": {"ms_per_step": round(lat_ms, 3), "solver_ms": round(float(np.median(tsolve)), 3),
+                              "reference_ms": 49.8, "contacts": w.n_contacts, "colours": w.n_colors,
+                              "iterations": 120},
+            "config": {"workload": "synthetic_scene(n_c) over 512 bodies, 50 iterations; two-layer lattice N=10000",
+                       "iters": 50},
             "cpu_baseline": cpu}
 
 
