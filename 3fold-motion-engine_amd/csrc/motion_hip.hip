@@ -750,7 +750,8 @@ struct me_pgs {
     double soft[5] = {0, 0, 0, 0, 0};
     int soft_on = 0;
     void *rho = nullptr, *lam = nullptr, *v = nullptr, *vbias = nullptr, *res = nullptr, *uout = nullptr,
-         *cMinv = nullptr;
+         *cMinv = nullptr, *lamo = nullptr, *pscal = nullptr, *plane = nullptr;
+    int *pint = nullptr;
     int64_t *acc = nullptr, *key = nullptr, *prio = nullptr;
     long long *claim = nullptr;
     unsigned long long *rmax = nullptr;
@@ -779,6 +780,10 @@ size_t pgs_layout(me_pgs *s, char *base) {
     s->res = B.take<char>(R * MC);
     s->uout = B.take<char>(R * 3 * MC);
     s->cMinv = B.take<char>(R * 36 * NB);
+    s->lamo = B.take<char>(R * 3 * MC);
+    s->pscal = B.take<char>(R * 8 * MC);
+    s->plane = B.take<char>(R * 4 * PGS_LANE * MC);
+    s->pint = B.take<int>(8 * MC);
     s->acc = B.take<int64_t>(6 * NB);
     s->key = B.take<int64_t>(MC);
     s->prio = B.take<int64_t>(MC);
@@ -941,6 +946,9 @@ int pgs_prepare_t(me_pgs *p, const void *warm, int *hit_out, hipStream_t s) {
         p->cached_C = C;
         p->misses += 1;
     }
+    hipLaunchKernelGGL(k_pgs_pack<T>, dim3(nblk(C)), dim3(ME_THREADS), 0, s, C, S, p->dyn, p->bslot, p->order,
+                       (const T *)p->rho, p->pint, (T *)p->pscal, (T *)p->plane);
+    LAUNCH_CHECK();
     if (hit_out) *hit_out = hit ? 1 : 0;
     return ME_OK;
 }
@@ -948,12 +956,11 @@ int pgs_prepare_t(me_pgs *p, const void *warm, int *hit_out, hipStream_t s) {
 template <class T>
 int pgs_iterate_t(me_pgs *p, int iters, int use_bias, hipStream_t s) {
     if (p->C == 0 || iters <= 0) return ME_OK;
-    PgsScene<T> S = pgs_scene_of<T>(p);
     const size_t lds = (size_t)6 * PGS_LDS_BODIES * (sizeof(int64_t) + sizeof(T));
-    hipLaunchKernelGGL(k_pgs_sweeps<T>, dim3(pgs_ntile(p->C)), dim3(PGS_THREADS), lds, s, iters, S,
-                       pgs_soft_of<T>(p, use_bias), (p->flags & ME_PGS_DESAXCE) ? 1 : 0, (const T *)p->rho, p->dyn,
-                       p->tlo, p->thi, p->tncol, p->seg, p->order, p->tbstart, p->tbcnt, p->tbody, p->bslot,
-                       (T *)p->lam, (T *)p->v, p->acc);
+    hipLaunchKernelGGL(k_pgs_sweeps<T>, dim3(pgs_ntile(p->C)), dim3(PGS_SWEEP_THREADS), lds, s, iters,
+                       pgs_soft_of<T>(p, use_bias), (p->flags & ME_PGS_DESAXCE) ? 1 : 0, p->tlo, p->thi, p->tncol,
+                       p->seg, p->order, p->tbstart, p->tbcnt, p->tbody, p->pint, (const T *)p->pscal,
+                       (const T *)p->plane, (T *)p->lam, (T *)p->lamo, (T *)p->v, p->acc);
     LAUNCH_CHECK();
     return ME_OK;
 }

@@ -178,11 +178,10 @@ __device__ __forceinline__ int64_t pgs_q(double x) { return (int64_t)round(x * P
 
 // the fixed-point map's argument z for contact c (Baumgarte or soft-step row), from u and lambda
 template <class T>
-__device__ __forceinline__ void pgs_z(const PgsScene<T> &S, const PgsSoft<T> &sp, int desaxce, int c, T rh, T u0, T u1,
-                                     T u2, T lo0, T lo1, T lo2, T &z0, T &z1, T &z2) {
-    T m = S.mu[c];
+__device__ __forceinline__ void pgs_z(const PgsSoft<T> &sp, int desaxce, T m, T sep, T rh, T u0, T u1, T u2, T lo0,
+                                     T lo1, T lo2, T &z0, T &z1, T &z2) {
     if (sp.on) {
-        T s = S.sep[c], vb = (T)0.0, ms = (T)1.0, isc = (T)0.0;
+        T s = sep, vb = (T)0.0, ms = (T)1.0, isc = (T)0.0;
         if (s > (T)0.0) {
             vb = s * sp.p3;
         } else if (sp.use_bias) {
@@ -564,35 +563,95 @@ __global__ void k_pgs_body_fill(int NB, const int *bslot, const int *btile, cons
     tbody[tbstart[btile[i]] + bslot[i]] = i;
 }
 
-#define PGS_LDS_BODIES 640  // dynamic bodies of a tile kept in LDS (velocity and int64 total)
+#define PGS_LANE 40         // reals of a lane record: J block (18), 3 inverse-mass rows (18), 3 of v_free, pad
+
+// Per-solve contact records in tile-position order j (the order the sweeps visit): the ints
+// (contact, bodies, their LDS slots, dynamic flags), the scalars (bias, mu, rho, separation) and, per
+// lane r of the contact's quad (side r >> 1, rows 3 (r & 1) .. +2), the side's J block, the three
+// inverse-mass rows and v_free entries that lane updates. A colour step then reads its contacts at
+// addresses known from j alone, in one round of loads.
+template <class T>
+__global__ void k_pgs_pack(int C, PgsScene<T> S, const int *dyn, const int *bslot, const int *order, const T *rho,
+                           int *pint, T *pscal, T *plane) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= C) return;
+    int c = order[j];
+    int a = S.pab[2 * c], b = S.pab[2 * c + 1];
+    bool da = a >= 0 && dyn[a], db = b >= 0 && dyn[b];
+    int *I = pint + 8 * (size_t)j;
+    I[0] = c; I[1] = a; I[2] = b;
+    I[3] = da ? bslot[a] : -1;
+    I[4] = db ? bslot[b] : -1;
+    I[5] = (da ? 1 : 0) | (db ? 2 : 0);
+    I[6] = 0; I[7] = 0;
+    T *P = pscal + 8 * (size_t)j;
+    P[0] = S.bias[3 * c]; P[1] = S.bias[3 * c + 1]; P[2] = S.bias[3 * c + 2];
+    P[3] = S.mu[c]; P[4] = rho[c]; P[5] = S.sep ? S.sep[c] : (T)0.0; P[6] = (T)0.0; P[7] = (T)0.0;
+    for (int r = 0; r < 4; ++r) {
+        int side = r >> 1, h = r & 1, i = side ? b : a;
+        T *L = plane + (4 * (size_t)j + r) * PGS_LANE;
+        const T *J = (side ? S.Jb : S.Ja) + 18 * (size_t)c;
+        for (int k = 0; k < 18; ++k) L[k] = i >= 0 ? J[k] : (T)0.0;
+        bool d = side ? db : da;
+        for (int k = 0; k < 18; ++k) L[18 + k] = d ? S.Minv[36 * (size_t)i + 18 * h + k] : (T)0.0;
+        for (int k = 0; k < 3; ++k) L[36 + k] = d ? S.vfree[6 * (size_t)i + 3 * h + k] : (T)0.0;
+        L[39] = (T)0.0;
+    }
+}
+
+// the record of one contact as one lane of its quad needs it
+template <class T>
+struct PgsRec {
+    int a, b, i, sl, dy;
+    T sc[6];   // bias (3), mu, rho, separation
+    T L[39];   // J block of the lane's side, its three inverse-mass rows, their v_free entries
+    T lo[3];   // lambda at the start of the update
+};
+
+template <class T>
+__device__ __forceinline__ void pgs_load_rec(PgsRec<T> &R, const int *pint, const T *pscal, const T *plane,
+                                             const T *lamo, int j, int r) {
+    const int s = r >> 1;
+    const int *I = pint + 8 * (size_t)j;
+    R.a = I[1];
+    R.b = I[2];
+    R.i = s ? R.b : R.a;
+    R.sl = s ? I[4] : I[3];
+    R.dy = (I[5] >> s) & 1;
+    const T *Sc = pscal + 8 * (size_t)j;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) R.sc[k] = Sc[k];
+    const T *L = plane + (4 * (size_t)j + r) * PGS_LANE;
+#pragma unroll
+    for (int k = 0; k < 39; ++k) R.L[k] = L[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) R.lo[k] = lamo[3 * (size_t)j + k];
+}
 
 // One contact by the four lanes of a quad (lane r: side r >> 1 = body a / body b, half r & 1 = rows
 // 0-2 / 3-5 of that body's six components). Each side forms its J v; the quad exchanges the two
 // partial products and every lane forms u and the new lambda the same way; each lane then replaces
 // its three components of its body's total and rebuilds them. Every output is computed by the same
-// operations, in the same order, as when one thread updates the whole contact.
+// operations, in the same order, as when one thread updates the whole contact. The new lambda is
+// returned in z and stored at tile position j by lane 0.
 template <class T>
-__device__ __forceinline__ void pgs_contact_quad(const PgsScene<T> &S, const PgsSoft<T> &sp, int desaxce,
-                                                 const T *rho, const int *dyn, const int *bslot, T *lam, T *gv,
-                                                 int64_t *gacc, T *lv, int64_t *lacc, bool lds, int c, int r) {
-    const int s = r >> 1, h = r & 1;
-    const int a = S.pab[2 * c], b = S.pab[2 * c + 1];
-    const int i = s ? b : a;
-    const T *J = (s ? S.Jb : S.Ja) + 18 * (size_t)c;
-    const bool has = i >= 0;
-    const bool dy = has && dyn[i];
+__device__ __forceinline__ void pgs_contact_quad(const PgsSoft<T> &sp, int desaxce, const PgsRec<T> &R, T *lamo,
+                                                 T *gv, int64_t *gacc, T *lv, int64_t *lacc, bool lds, int j, int r,
+                                                 T z[3]) {
+    const int h = r & 1;
+    const bool has = R.i >= 0;
     T *vi = nullptr;
     int64_t *ai = nullptr;
     if (has) {
-        if (lds && dy) {
-            int sl = bslot[i];
-            vi = lv + 6 * sl;
-            ai = lacc + 6 * sl;
+        if (lds && R.dy) {
+            vi = lv + 6 * R.sl;
+            ai = lacc + 6 * R.sl;
         } else {
-            vi = gv + 6 * (size_t)i;
-            ai = gacc + 6 * (size_t)i;
+            vi = gv + 6 * (size_t)R.i;
+            ai = gacc + 6 * (size_t)R.i;
         }
     }
+    const T *J = R.L;
     T P[3] = {(T)0.0, (T)0.0, (T)0.0};
     if (has) {
         T x[6];
@@ -613,50 +672,55 @@ __device__ __forceinline__ void pgs_contact_quad(const PgsScene<T> &S, const Pgs
         pa[q] = __shfl(P[q], base);
         pb[q] = __shfl(P[q], base + 2);
     }
-    T u0 = S.bias[3 * c], u1 = S.bias[3 * c + 1], u2 = S.bias[3 * c + 2];
-    if (a >= 0) { u0 = u0 + pa[0]; u1 = u1 + pa[1]; u2 = u2 + pa[2]; }
-    if (b >= 0) { u0 = u0 + pb[0]; u1 = u1 + pb[1]; u2 = u2 + pb[2]; }
-    const T lo0 = lam[3 * c], lo1 = lam[3 * c + 1], lo2 = lam[3 * c + 2];
+    T u0 = R.sc[0], u1 = R.sc[1], u2 = R.sc[2];
+    if (R.a >= 0) { u0 = u0 + pa[0]; u1 = u1 + pa[1]; u2 = u2 + pa[2]; }
+    if (R.b >= 0) { u0 = u0 + pb[0]; u1 = u1 + pb[1]; u2 = u2 + pb[2]; }
+    const T lo0 = R.lo[0], lo1 = R.lo[1], lo2 = R.lo[2];
     T z0, z1, z2;
-    pgs_z<T>(S, sp, desaxce, c, rho[c], u0, u1, u2, lo0, lo1, lo2, z0, z1, z2);
-    if (r == 0) { lam[3 * c] = z0; lam[3 * c + 1] = z1; lam[3 * c + 2] = z2; }
-    if (!dy) return;
+    pgs_z<T>(sp, desaxce, R.sc[3], R.sc[5], R.sc[4], u0, u1, u2, lo0, lo1, lo2, z0, z1, z2);
+    z[0] = z0; z[1] = z1; z[2] = z2;
+    if (r == 0) { lamo[3 * (size_t)j] = z0; lamo[3 * (size_t)j + 1] = z1; lamo[3 * (size_t)j + 2] = z2; }
+    if (!R.dy) return;
     T g[6], go[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         g[k] = (J[k] * z0 + J[6 + k] * z1) + J[12 + k] * z2;
         go[k] = (J[k] * lo0 + J[6 + k] * lo1) + J[12 + k] * lo2;
     }
-    const T *M = S.Minv + 36 * (size_t)i;
-    const T *vf = S.vfree + 6 * (size_t)i;
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
-        const int row = 3 * h + kk;
-        T dn = M[6 * row] * g[0], dd = M[6 * row] * go[0];
+        const T *M = R.L + 18 + 6 * kk;
+        T dn = M[0] * g[0], dd = M[0] * go[0];
 #pragma unroll
-        for (int j = 1; j < 6; ++j) {
-            dn = dn + M[6 * row + j] * g[j];
-            dd = dd + M[6 * row + j] * go[j];
+        for (int q = 1; q < 6; ++q) {
+            dn = dn + M[q] * g[q];
+            dd = dd + M[q] * go[q];
         }
+        const int row = 3 * h + kk;
         int64_t t = ai[row] + (pgs_q((double)dn) - pgs_q((double)dd));
         ai[row] = t;
-        vi[row] = vf[row] + (T)((double)t / PGS_SCALE);
+        vi[row] = R.L[36 + kk] + (T)((double)t / PGS_SCALE);
     }
 }
 
+#define PGS_LDS_BODIES 600    // dynamic bodies of a tile kept in LDS (velocity and int64 total)
+#define PGS_SWEEP_THREADS 512  // a quad per contact: 128 contacts of a colour at a time
+
 // One workgroup per tile, every iteration, one barrier per colour; a quad per contact. The tile's
 // dynamic bodies (velocity, int64 total) sit in LDS for the launch when they fit, else in global
-// memory; static bodies are read from global memory (never written).
+// memory; static bodies are read from global memory (never written). lambda is gathered into tile
+// positions for the launch (lamo) and scattered back at its end.
 template <class T>
-__global__ void __launch_bounds__(PGS_THREADS) k_pgs_sweeps(int iters, PgsScene<T> S, PgsSoft<T> sp, int desaxce,
-                                                           const T *rho, const int *dyn, const int *tlo,
-                                                           const int *thi, const int *tncol, const int *seg,
-                                                           const int *order, const int *tbstart, const int *tbcnt,
-                                                           const int *tbody, const int *bslot, T *lam, T *v,
-                                                           int64_t *acc) {
+__global__ void __launch_bounds__(PGS_SWEEP_THREADS) k_pgs_sweeps(int iters, PgsSoft<T> sp, int desaxce,
+                                                                 const int *tlo, const int *thi, const int *tncol,
+                                                                 const int *seg, const int *order, const int *tbstart,
+                                                                 const int *tbcnt, const int *tbody, const int *pint,
+                                                                 const T *pscal, const T *plane, T *lam, T *lamo, T *v,
+                                                                 int64_t *acc) {
     extern __shared__ __align__(16) unsigned char pgs_smem[];
     const int t = blockIdx.x, tid = threadIdx.x;
-    if (tlo[t] >= thi[t]) return;
+    const int lo = tlo[t], hi = thi[t];
+    if (lo >= hi) return;
     const int nc = tncol[t];
     const int *sg = seg + (size_t)t * (PGS_NCOLMAX + 1);
     const int nb = tbcnt[t], b0 = tbstart[t];
@@ -664,26 +728,32 @@ __global__ void __launch_bounds__(PGS_THREADS) k_pgs_sweeps(int iters, PgsScene<
     int64_t *lacc = reinterpret_cast<int64_t *>(pgs_smem);
     T *lv = reinterpret_cast<T *>(pgs_smem + sizeof(int64_t) * 6 * PGS_LDS_BODIES);
     if (lds)
-        for (int k = tid; k < 6 * nb; k += PGS_THREADS) {
+        for (int k = tid; k < 6 * nb; k += PGS_SWEEP_THREADS) {
             size_t g = 6 * (size_t)tbody[b0 + k / 6] + k % 6;
             lacc[k] = acc[g];
             lv[k] = v[g];
         }
+    for (int k = lo * 3 + tid; k < hi * 3; k += PGS_SWEEP_THREADS) lamo[k] = lam[3 * (size_t)order[k / 3] + k % 3];
     __syncthreads();
     const int quad = tid >> 2, r = tid & 3;
     for (int it = 0; it < iters; ++it)
         for (int k = 0; k < nc; ++k) {
             const int e = sg[k + 1];
-            for (int j = sg[k] + quad; j < e; j += PGS_THREADS / 4)
-                pgs_contact_quad<T>(S, sp, desaxce, rho, dyn, bslot, lam, v, acc, lv, lacc, lds, order[j], r);
+            for (int j = sg[k] + quad; j < e; j += PGS_SWEEP_THREADS / 4) {
+                PgsRec<T> R;
+                pgs_load_rec<T>(R, pint, pscal, plane, lamo, j, r);
+                T z[3];
+                pgs_contact_quad<T>(sp, desaxce, R, lamo, v, acc, lv, lacc, lds, j, r, z);
+            }
             __syncthreads();
         }
     if (lds)
-        for (int k = tid; k < 6 * nb; k += PGS_THREADS) {
+        for (int k = tid; k < 6 * nb; k += PGS_SWEEP_THREADS) {
             size_t g = 6 * (size_t)tbody[b0 + k / 6] + k % 6;
             acc[g] = lacc[k];
             v[g] = lv[k];
         }
+    for (int k = lo * 3 + tid; k < hi * 3; k += PGS_SWEEP_THREADS) lam[3 * (size_t)order[k / 3] + k % 3] = lamo[k];
 }
 
 // natural-map residual ||lambda - proj_K(z)||_inf per contact and the contact velocity u
@@ -700,7 +770,7 @@ __global__ void k_pgs_residual(int C, PgsScene<T> S, PgsSoft<T> sp, int desaxce,
         uout[3 * c] = u0; uout[3 * c + 1] = u1; uout[3 * c + 2] = u2;
         T l0 = lam[3 * c], l1 = lam[3 * c + 1], l2 = lam[3 * c + 2];
         T z0, z1, z2;
-        pgs_z<T>(S, sp, desaxce, c, rho[c], u0, u1, u2, l0, l1, l2, z0, z1, z2);
+        pgs_z<T>(sp, desaxce, S.mu[c], sp.on ? S.sep[c] : (T)0.0, rho[c], u0, u1, u2, l0, l1, l2, z0, z1, z2);
         T d0 = fabs(l0 - z0), d1 = fabs(l1 - z1), d2 = fabs(l2 - z2);
         T m12 = d1 > d2 ? d1 : d2;
         T rr = d0 > m12 ? d0 : m12;
