@@ -2,7 +2,8 @@
 reference ContactEngine protocol (antonbj3/3fold-motion-engine, src/motion_engine/contact_engine.py)."""
 from .protocol import BodyState, ContactEngine
 from .engines import ColorCacheContactEngine, GraphColoredContactEngine, RelaxedJacobiContactEngine
-from . import adjoint, bodies, ncp, scenes
+from . import adjoint, bodies, ncp, pgs, scenes
+from .pgs import ExactConeContactEngine
 
 __all__ = ["BodyState", "ContactEngine", "RelaxedJacobiContactEngine", "GraphColoredContactEngine",
-           "ColorCacheContactEngine", "adjoint", "bodies", "ncp", "scenes"]
+           "ColorCacheContactEngine", "ExactConeContactEngine", "adjoint", "bodies", "ncp", "pgs", "scenes"]

@@ -290,3 +290,23 @@ def test_bad_body_index_is_an_argument_error():
     s = m.NCPSolver(bs.n_b, bs.n_c)
     with pytest.raises(ValueError, match="outside"):
         s.solve(bs, iters=5)
+
+
+def test_exact_cone_engine_behind_the_protocol():
+    """The fourth ContactEngine member: a box resting on the ground holds its height, the contact force
+    balances gravity, and a kinematic box keeps the pose set_kinematic gives it."""
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    _say("pgs ContactEngine member")
+    e = pkg.ExactConeContactEngine(mu=0.5, iters=200)
+    assert isinstance(e, pkg.ContactEngine)
+    e.add_box([0.0, 0.0, 0.1], [0.1, 0.1, 0.1], 2.0)
+    e.add_box([1.0, 0.0, 0.1], [0.1, 0.1, 0.1], 1.0)
+    e.set_kinematic(1, [1.0, 0.0, 0.5])
+    for _ in range(60):
+        e.step(1.0 / 240.0)
+    st = e.get_state()
+    assert abs(st.xc[0, 2] - 0.1) < 1e-3
+    assert np.allclose(st.xc[1], [1.0, 0.0, 0.5]) and np.allclose(st.vc[1], 0.0)
+    f = e.contact_forces()
+    assert abs(f[0, 2] - 2.0 * 9.81) < 0.05 * 2.0 * 9.81
+    assert np.allclose(f[1], 0.0)

@@ -418,3 +418,22 @@ def test_pins_reject_planted_mistakes(tmp_path, monkeypatch):
     monkeypatch.setattr(PO, "_libs", {})
     with pytest.raises(AssertionError):
         test_velocity_is_the_rebuilt_total("float64")
+
+
+def test_exact_cone_engine_protocol_on_the_oracle():
+    """The ContactEngine member's host logic (kinematic bodies, forces) with the oracle as its solver."""
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    e = pkg.ExactConeContactEngine(mu=0.5, iters=200)
+    e.world._make_solver = lambda n, cap: PO.OracleNCPSolver(n, cap, dtype=e.world.dtype, desaxce=e.world.desaxce)
+    assert isinstance(e, pkg.ContactEngine)
+    e.add_box([0.0, 0.0, 0.1], [0.1, 0.1, 0.1], 2.0)
+    e.add_box([1.0, 0.0, 0.1], [0.1, 0.1, 0.1], 1.0)
+    e.set_kinematic(1, [1.0, 0.0, 0.5])
+    for _ in range(60):
+        e.step(1.0 / 240.0)
+    st = e.get_state()
+    assert abs(st.xc[0, 2] - 0.1) < 1e-3
+    assert np.allclose(st.xc[1], [1.0, 0.0, 0.5]) and np.allclose(st.vc[1], 0.0)
+    f = e.contact_forces()
+    assert abs(f[0, 2] - 2.0 * 9.81) < 0.05 * 2.0 * 9.81
+    assert np.allclose(f[1], 0.0)
