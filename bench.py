@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large", "adjoint", "ncp", "pgs"])
+    ap.add_argument("--workload", default="pile", choices=["pile", "batched", "large", "adjoint", "ncp", "pgs", "lattice"])
     ap.add_argument("--rollouts", type=int, default=3, help="adjoint workload: timed forward+backward rollouts")
     ap.add_argument("--extras", type=int, default=1,
                     help="pile workload: also measure the 256-scene batch and the config-4 adjoint and attach "
@@ -102,6 +102,9 @@ def main():
         return
     if a.workload == "pgs":
         print(json.dumps(bench_pgs(a)), flush=True)
+        return
+    if a.workload == "lattice":
+        print(json.dumps(bench_lattice(a)), flush=True)
         return
     if a.workload == "large":
         a.bodies = 100000 if a.bodies == 10000 else a.bodies
@@ -241,6 +244,8 @@ def main():
                                 keys=("value", "ms_per_step", "adjoint", "config"))
         out["ncp_sensitivity"] = _extra(bench_ncp, a, keys=("value", "unit", "ms_per_call", "vs_reference",
                                                             "config", "cpu_baseline"))
+        out["reference_lattice_10000"] = _extra(bench_lattice, a, keys=("value", "unit", "vs_reference",
+                                                                        "reference", "runs"))
         out["exact_cone_pgs"] = _extra(bench_pgs, a, keys=("value", "unit", "vs_reference", "solves", "kernel",
                                                            "lattice_10000", "cpu_baseline"))
     print(json.dumps(out), flush=True)
@@ -486,6 +491,54 @@ def bench_pgs(a):
             "config": {"workload": "synthetic_scene(n_c) over 512 bodies, 50 iterations; two-layer lattice N=10000",
                        "iters": 50},
             "cpu_baseline": cpu}
+
+
+# published wall-clock ms/step of the reference engines on this fixture (all RTX 5070): README.md:37
+# (ColorCache), docs/RUNNING.md:738 (pair chunks and Jacobi, first of two runs)
+LATTICE_REF_MS = {"color_cache": 3.7, "pair_chunks": 4.931, "jacobi": 5.591}
+
+
+def bench_lattice(a):
+    """The reference's own headline fixture (scripts/colored_vs_jacobi.py:104-135): N = 10 000 bead
+    boxes 0.3 x 0.3 x 0.2 m in a two-layer lattice 0.35 m apart, dropped onto the ground; each engine
+    built with its default arguments (vit 40, pit 20, mu 0.5) and stepped with step(1/240, substeps=1),
+    10 warm-up steps then 30 timed, wall clock per step. The default engines read their counters back
+    after every step, as the reference's do; 'tiled' is the island-tiled solve of the same engine."""
+    import torch
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    cen = pkg.scenes.lattice(10000)
+    cls = {"color_cache": pkg.ColorCacheContactEngine, "pair_chunks": pkg.GraphColoredContactEngine,
+           "jacobi": pkg.RelaxedJacobiContactEngine}
+    runs = []
+    for solver, tiled in (("color_cache", True), ("color_cache", False), ("pair_chunks", True),
+                          ("pair_chunks", False), ("jacobi", False)):
+        kw = {"tiled": True} if tiled else {}
+        e = cls[solver](**kw)
+        for c in cen:
+            e.add_body(c)
+        for _ in range(10):
+            e.step(1 / 240, substeps=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(30):
+            e.step(1 / 240, substeps=1)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / 30 * 1e3
+        cnt = e.refresh_counters()
+        r = {"solver": solver, "tiled": tiled, "ms_per_step": round(ms, 4), "reference_ms": LATTICE_REF_MS[solver],
+             "speedup": round(LATTICE_REF_MS[solver] / ms, 2), "contacts": cnt["n_contacts"],
+             "contacts_solved": cnt["n_solved"], "pairs": cnt["n_pairs"], "colours": cnt["n_colors"]}
+        print(f"[bench] lattice {r}", file=sys.stderr, flush=True)
+        runs.append(r)
+        del e
+        torch.cuda.empty_cache()
+    best = min((r for r in runs if r["solver"] == "color_cache"), key=lambda r: r["ms_per_step"])
+    return {"metric": "ms/step, reference lattice fixture N=10000 (ColorCache, vit 40, pit 20, S1)",
+            "value": best["ms_per_step"], "unit": "ms/step", "higher_is_better": False,
+            "vs_reference": best["speedup"], "reference": "RTX 5070, README.md:37 / docs/RUNNING.md:738",
+            "dtype": "f32", "data": "the reference's lattice(10000) fixture", "runs": runs,
+            "config": {"workload": "two-layer bead lattice, N=10000, dt 1/240, substeps 1, 10 warm-up + 30 timed "
+                                   "steps", "bodies": 10000}}
 
 
 def bench_batched(a):
