@@ -10,7 +10,7 @@ namespace me {
 enum {
     IS_C = 0, IS_P, IS_CAND, IS_NCOL, IS_HITS, IS_MISSES, IS_OVF, IS_ISL, IS_TILES, IS_MAXTB,
     IS_MAXEXT, IS_LEFT, IS_WN, IS_CACHED_P, IS_CACHED_NCOL, IS_MISMATCH, IS_CELL, IS_NBODYSORT,
-    IS_NPAIRSORT, IS_NCOLJP, IS_NBIG, IS_CRAW, IS_COUNT = 32
+    IS_NPAIRSORT, IS_NCOLJP, IS_NBIG, IS_CRAW, IS_JTOT, IS_COUNT = 32
 };
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,
        OVF_TAPE = 32 /* a recorded (differentiated) substep met a tile beyond one small tile */ };
@@ -506,12 +506,66 @@ __global__ void k_bead_contacts(int NP, int P, const float *allp, const int *cel
 }
 
 // ── A8-A10 relaxed Jacobi with int64 fixed-point accumulation ──
-// _acc (contact_engine_gpu.py:32-38): int64(round(f64(d) * f64(sign * scale))) with atomic add
-__device__ __forceinline__ void acc3(int64_t *acc, int N, int b, int off, f3 d, float sign, float scale) {
+// The reference adds every contact's quantised deltas into six int64 accumulators per body with
+// atomics. Here each contact owns one six-int64 entry per body end, in a body-major list built once
+// per substep, and stores its deltas there with plain stores; the apply pass sums a body's entries.
+// Integer addition is associative, so the sums -- and everything after them -- are the same bits as
+// the atomic form in any order, without a device-wide atomic in the iterations.
+
+// _acc (contact_engine_gpu.py:32-38): int64(round(f64(d) * f64(sign * scale))), added to the entry
+__device__ __forceinline__ void q3(int64_t *o, f3 d, float sign, float scale) {
     double sc = (double)(sign * scale);
-    atomicAdd((unsigned long long *)&acc[(size_t)(off + 0) * N + b], (unsigned long long)(int64_t)round((double)d.x * sc));
-    atomicAdd((unsigned long long *)&acc[(size_t)(off + 1) * N + b], (unsigned long long)(int64_t)round((double)d.y * sc));
-    atomicAdd((unsigned long long *)&acc[(size_t)(off + 2) * N + b], (unsigned long long)(int64_t)round((double)d.z * sc));
+    o[0] += (int64_t)round((double)d.x * sc);
+    o[1] += (int64_t)round((double)d.y * sc);
+    o[2] += (int64_t)round((double)d.z * sc);
+}
+
+// entry e = 6 int64 (linear xyz, angular xyz), 48 bytes: three 16-byte stores
+__device__ __forceinline__ void st_entry(int64_t *d, int e, const int64_t *q) {
+    longlong2 *p = reinterpret_cast<longlong2 *>(d + 6 * (size_t)e);
+    p[0] = make_longlong2(q[0], q[1]);
+    p[1] = make_longlong2(q[2], q[3]);
+    p[2] = make_longlong2(q[4], q[5]);
+}
+
+// Slot of this lane in cnt[key] when lanes with equal keys sit in runs of consecutive lanes (contacts
+// are grouped by body): one atomic per run. An inactive lane ends a run. Every lane of the wave must
+// call it (no lane may have left the kernel).
+__device__ __forceinline__ int wave_run_slot(int *cnt, int key, bool active) {
+    const int lane = __lane_id();
+    const int kk = active ? key : INT_MIN;
+    const int prev = __shfl_up(kk, 1);
+    const bool head = lane == 0 || prev != kk;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int h = 63 - __clzll(heads & upto);
+    const unsigned long long above = heads & ~upto;
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    int base = 0;
+    if (head && active) base = atomicAdd(&cnt[key], next - lane);
+    base = __shfl(base, h);
+    return base + lane - h;
+}
+
+// entry counts and each contact's slot among its bodies' entries (A end, then B end unless ground)
+__global__ void k_jac_slots(int capC, const int *istat, const int *bi, const int *bj, int *cnt, int *posA, int *posB) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int C = istat[IS_C] < capC ? istat[IS_C] : capC;
+    bool live = c < C;  // no early exit: every lane takes part in the run ballots
+    int a = live ? bi[c] : 0, b = live ? bj[c] : -1;
+    int sa = wave_run_slot(cnt, a, live);
+    int sb = wave_run_slot(cnt, b, live && b >= 0);
+    if (live) { posA[c] = sa; posB[c] = b >= 0 ? sb : -1; }
+}
+
+// slots -> entry positions (off = exclusive scan of the counts)
+__global__ void k_jac_place(int capC, const int *istat, const int *bi, const int *bj, const int *off, int *posA,
+                            int *posB) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= istat[IS_C] || c >= capC) return;
+    posA[c] += off[bi[c]];
+    int b = bj[c];
+    if (b >= 0) posB[c] += off[b];
 }
 
 struct JacArgs {
@@ -520,7 +574,8 @@ struct JacArgs {
     float *jn, *jt1, *jt2, *jp;
     const float *x, *invM, *invIw;
     const float *v, *w, *pv, *po;
-    int64_t *acc;
+    const int *posA, *posB;  // entry of each contact at its A and B body
+    int64_t *d;              // entries, 6 int64 each
     int N;
 };
 
@@ -529,6 +584,8 @@ __global__ void k_jac_vel(const int *istat, JacArgs A, float scale) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= istat[IS_C]) return;
     int bi = A.bi[c], bj = A.bj[c];
+    int ea = A.posA[c], eb = A.posB[c];
+    int64_t qa[6] = {0, 0, 0, 0, 0, 0}, qb[6] = {0, 0, 0, 0, 0, 0};
     f3 nrm = ld3(A.n, c), rA = ld3(A.pA, c) - ld3(A.x, bi);
     float iMA = A.invM[bi];
     m3 IA = ldm(A.invIw, bi), IB = IA;
@@ -554,19 +611,22 @@ __global__ void k_jac_vel(const int *istat, JacArgs A, float scale) {
         f3 rnB = cross(rB, nrm);
         meff = meff + iMB + dot(rnB, mv(IB, rnB));
     }
-    if (meff < 1e-12f) return;
+    if (meff < 1e-12f) {
+        st_entry(A.d, ea, qa);
+        if (bj >= 0) st_entry(A.d, eb, qb);
+        return;
+    }
     float jo = A.jn[c];
     float dj = -vn / meff;
     float nw = wmax(0.0f, jo + dj);
     dj = nw - jo;
     A.jn[c] = nw;
     f3 Jn = nrm * dj;
-    int N = A.N;
-    acc3(A.acc, N, bi, 0, Jn * iMA, 1.0f, scale);
-    acc3(A.acc, N, bi, 3, mv(IA, cross(rA, Jn)), 1.0f, scale);
+    q3(qa, Jn * iMA, 1.0f, scale);
+    q3(qa + 3, mv(IA, cross(rA, Jn)), 1.0f, scale);
     if (bj >= 0) {
-        acc3(A.acc, N, bj, 0, Jn * iMB, -1.0f, scale);
-        acc3(A.acc, N, bj, 3, mv(IB, cross(rB, Jn)), -1.0f, scale);
+        q3(qb, Jn * iMB, -1.0f, scale);
+        q3(qb + 3, mv(IB, cross(rB, Jn)), -1.0f, scale);
     }
     f3 rtA1 = cross(rA, t1);
     float meft1 = iMA + dot(rtA1, mv(IA, rtA1));
@@ -587,11 +647,13 @@ __global__ void k_jac_vel(const int *istat, JacArgs A, float scale) {
     A.jt1[c] = a1;
     A.jt2[c] = a2;
     f3 Jt = t1 * (a1 - o1) + t2 * (0.5f * (a2 - o2));
-    acc3(A.acc, N, bi, 0, Jt * iMA, 1.0f, scale);
-    acc3(A.acc, N, bi, 3, mv(IA, cross(rA, Jt)), 1.0f, scale);
+    q3(qa, Jt * iMA, 1.0f, scale);
+    q3(qa + 3, mv(IA, cross(rA, Jt)), 1.0f, scale);
+    st_entry(A.d, ea, qa);
     if (bj >= 0) {
-        acc3(A.acc, N, bj, 0, Jt * iMB, -1.0f, scale);
-        acc3(A.acc, N, bj, 3, mv(IB, cross(rB, Jt)), -1.0f, scale);
+        q3(qb, Jt * iMB, -1.0f, scale);
+        q3(qb + 3, mv(IB, cross(rB, Jt)), -1.0f, scale);
+        st_entry(A.d, eb, qb);
     }
 }
 
@@ -600,6 +662,8 @@ __global__ void k_jac_pos(const int *istat, JacArgs A, float dt, float scale) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= istat[IS_C]) return;
     int bi = A.bi[c], bj = A.bj[c];
+    int ea = A.posA[c], eb = A.posB[c];
+    int64_t qa[6] = {0, 0, 0, 0, 0, 0}, qb[6] = {0, 0, 0, 0, 0, 0};
     f3 nrm = ld3(A.n, c), rA = ld3(A.pA, c) - ld3(A.x, bi);
     float iMA = A.invM[bi];
     m3 IA = ldm(A.invIw, bi), IB = IA;
@@ -618,7 +682,11 @@ __global__ void k_jac_pos(const int *istat, JacArgs A, float dt, float scale) {
         f3 rnB = cross(rB, nrm);
         meff = meff + iMB + dot(rnB, mv(IB, rnB));
     }
-    if (meff < 1e-12f) return;
+    if (meff < 1e-12f) {
+        st_entry(A.d, ea, qa);
+        if (bj >= 0) st_entry(A.d, eb, qb);
+        return;
+    }
     float bias = 0.2f * wmax(A.pen[c] - 1e-4f, 0.0f) / dt;
     float dj = (bias - dot(rel, nrm)) / meff;
     float jo = A.jp[c];
@@ -626,34 +694,45 @@ __global__ void k_jac_pos(const int *istat, JacArgs A, float dt, float scale) {
     dj = nw - jo;
     A.jp[c] = nw;
     f3 Jn = nrm * dj;
-    int N = A.N;
-    acc3(A.acc, N, bi, 0, Jn * iMA, 1.0f, scale);
-    acc3(A.acc, N, bi, 3, mv(IA, cross(rA, Jn)), 1.0f, scale);
+    q3(qa, Jn * iMA, 1.0f, scale);
+    q3(qa + 3, mv(IA, cross(rA, Jn)), 1.0f, scale);
+    st_entry(A.d, ea, qa);
     if (bj >= 0) {
-        acc3(A.acc, N, bj, 0, Jn * iMB, -1.0f, scale);
-        acc3(A.acc, N, bj, 3, mv(IB, cross(rB, Jn)), -1.0f, scale);
+        q3(qb, Jn * iMB, -1.0f, scale);
+        q3(qb + 3, mv(IB, cross(rB, Jn)), -1.0f, scale);
+        st_entry(A.d, eb, qb);
     }
 }
 
-// k_apply (contact_engine_gpu.py:134-143)
-__global__ void k_jac_apply(int N, float *v, float *w, int64_t *acc, float relax, float scale) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+// k_apply (contact_engine_gpu.py:134-143) over the entries: thread (body b, component k) sums b's
+// entries in component k, dv = f32(f64(sum) * f64(f32(1/scale))), then v += relax * dv. The
+// reference zeroes its accumulators here; every entry is rewritten by the next pass instead.
+__global__ void k_jac_gather(int N, const int *off, const int *total, const int64_t *d, float *v, float *w,
+                             float relax, float scale) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * N) return;
+    int b = t / 6, k = t - 6 * b;
+    int e0 = off[b], e1 = b + 1 < N ? off[b + 1] : *total;
+    const int64_t *p = d + k;
+    int64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int e = e0;
+    for (; e + 4 <= e1; e += 4) {
+        s0 += p[6 * (size_t)e];
+        s1 += p[6 * (size_t)(e + 1)];
+        s2 += p[6 * (size_t)(e + 2)];
+        s3 += p[6 * (size_t)(e + 3)];
+    }
+    for (; e < e1; ++e) s0 += p[6 * (size_t)e];
+    int64_t sum = (s0 + s1) + (s2 + s3);
     float inv = 1.0f / scale;
-    double dinv = (double)inv;
-    f3 dv = mk((float)((double)acc[i] * dinv), (float)((double)acc[(size_t)N + i] * dinv),
-               (float)((double)acc[(size_t)2 * N + i] * dinv));
-    f3 dw = mk((float)((double)acc[(size_t)3 * N + i] * dinv), (float)((double)acc[(size_t)4 * N + i] * dinv),
-               (float)((double)acc[(size_t)5 * N + i] * dinv));
-    st3(v, i, ld3(v, i) + relax * dv);
-    st3(w, i, ld3(w, i) + relax * dw);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) acc[(size_t)k * N + i] = 0;
+    float dv = (float)((double)sum * (double)inv);
+    float *dst = k < 3 ? v + 3 * (size_t)b + k : w + 3 * (size_t)b + (k - 3);
+    *dst = *dst + relax * dv;
 }
 
-// Jacobi substep set-up: impulses to zero; pv, po, accumulators to zero only when there are
+// Jacobi substep set-up: impulses and entry counts to zero; pv, po to zero only when there are
 // contacts (the reference zeroes pv/po inside `if C > 0`, contact_engine_gpu.py:266-269)
-__global__ void k_jac_reset(int N, int capC, const int *istat, float *pv, float *po, int64_t *acc, float *jn, float *jt1,
+__global__ void k_jac_reset(int N, int capC, const int *istat, float *pv, float *po, int *cnt, float *jn, float *jt1,
                             float *jt2, float *jp) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int C = istat[IS_C];
@@ -661,8 +740,7 @@ __global__ void k_jac_reset(int N, int capC, const int *istat, float *pv, float 
     if (i < N) {
         st3(pv, i, mk(0.0f, 0.0f, 0.0f));
         st3(po, i, mk(0.0f, 0.0f, 0.0f));
-#pragma unroll
-        for (int k = 0; k < 6; ++k) acc[(size_t)k * N + i] = 0;
+        cnt[i] = 0;
     }
     if (i < C && i < capC) { jn[i] = 0.0f; jt1[i] = 0.0f; jt2[i] = 0.0f; jp[i] = 0.0f; }
 }

@@ -71,7 +71,9 @@ struct me_engine {
     std::thread::id owner;
     // per body
     float *invIw, *Rw, *aabb, *pv, *po, *vsnap, *xsnap, *wsnap, *allp;
-    int64_t *acc;
+    // Jacobi: per-body entry counts and offsets, each contact's entry at its two bodies, the entries
+    int *jcnt, *joff, *jposA, *jposB;
+    int64_t *jd;
     unsigned long long *claim;
     // grid
     int *cellc, *bucket, *bcount, *bstart, *bcursor, *items;
@@ -221,7 +223,15 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->xsnap = B.take<float>(3 * (size_t)N);
     e->wsnap = B.take<float>(3 * (size_t)N);
     e->allp = B.take<float>(3 * (size_t)(P > 0 ? NP : 1));
-    e->acc = B.take<int64_t>(6 * (size_t)N);
+    {
+        bool jac = c.solver == ME_SOLVER_JACOBI;
+        size_t jc = jac ? C : 1;
+        e->jcnt = B.take<int>(jac ? (size_t)N : 1);
+        e->joff = B.take<int>(jac ? (size_t)N : 1);
+        e->jposA = B.take<int>(jc);
+        e->jposB = B.take<int>(jc);
+        e->jd = B.take<int64_t>(jac ? 12 * C : 1);
+    }
     e->claim = B.take<unsigned long long>((size_t)N);
     e->cellc = B.take<int>(4 * (size_t)items);
     e->bucket = B.take<int>((size_t)items);
@@ -426,26 +436,35 @@ int solve_jacobi(me_engine *e, float sdt, hipStream_t s) {
     if (rc) return rc;
     int C = e->hstat[IS_C];
     if (C <= 0) return ME_OK;
-    int N = e->N;
+    int N = e->N, capC = e->capC;
     int gridN = nblk(N > C ? N : C);
-    hipLaunchKernelGGL(k_jac_reset, dim3(gridN), dim3(ME_THREADS), 0, s, N, e->capC, e->istat, e->pv, e->po, e->acc,
+    hipLaunchKernelGGL(k_jac_reset, dim3(gridN), dim3(ME_THREADS), 0, s, N, capC, e->istat, e->pv, e->po, e->jcnt,
                        e->Cc.jn, e->Cc.jt1, e->Cc.jt2, e->Cc.jp);
+    // the body-major entry list of this substep's contacts
+    hipLaunchKernelGGL(k_jac_slots, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->Cc.bi, e->Cc.bj, e->jcnt,
+                       e->jposA, e->jposB);
+    rc = scan(e, e->jcnt, e->joff, nullptr, N, N, e->istat + IS_JTOT, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_jac_place, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->Cc.bi, e->Cc.bj, e->joff,
+                       e->jposA, e->jposB);
     JacArgs A;
     A.bi = e->Cc.bi; A.bj = e->Cc.bj; A.pA = e->Cc.pA; A.pB = e->Cc.pB; A.n = e->Cc.n; A.pen = e->Cc.pen;
     A.mu = e->Cc.mu; A.jn = e->Cc.jn; A.jt1 = e->Cc.jt1; A.jt2 = e->Cc.jt2; A.jp = e->Cc.jp;
     A.x = e->b.x; A.invM = e->b.inv_mass; A.invIw = e->invIw; A.v = e->b.v; A.w = e->b.w; A.pv = e->pv; A.po = e->po;
-    A.acc = e->acc; A.N = N;
+    A.posA = e->jposA; A.posB = e->jposB; A.d = e->jd; A.N = N;
     const float S = 1.0e10f;
+    const int *tot = e->istat + IS_JTOT;
     for (int it = 0; it < e->cfg.vit; ++it) {
         prof_mark(e, ME_K_JAC_VEL, s);
         hipLaunchKernelGGL(k_jac_vel, dim3(nblk(C)), dim3(ME_THREADS), 0, s, e->istat, A, S);
         prof_mark(e, ME_K_JAC_VEL, s);
-        hipLaunchKernelGGL(k_jac_apply, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->b.v, e->b.w, e->acc,
-                           e->cfg.relax, S);
+        hipLaunchKernelGGL(k_jac_gather, dim3(nblk(6 * N)), dim3(ME_THREADS), 0, s, N, e->joff, tot, e->jd, e->b.v,
+                           e->b.w, e->cfg.relax, S);
     }
     for (int it = 0; it < e->cfg.pit; ++it) {
         hipLaunchKernelGGL(k_jac_pos, dim3(nblk(C)), dim3(ME_THREADS), 0, s, e->istat, A, sdt, S);
-        hipLaunchKernelGGL(k_jac_apply, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->pv, e->po, e->acc, e->cfg.relax, S);
+        hipLaunchKernelGGL(k_jac_gather, dim3(nblk(6 * N)), dim3(ME_THREADS), 0, s, N, e->joff, tot, e->jd, e->pv,
+                           e->po, e->cfg.relax, S);
     }
     LAUNCH_CHECK();
     return ME_OK;

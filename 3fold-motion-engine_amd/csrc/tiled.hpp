@@ -284,16 +284,27 @@ __global__ void k_t_scatter_p(int capP, const int *istat, TiledBuffers T) {
     if (r >= 0) T.psorted[T.poff[r] + slot] = p;
 }
 
+// Tile bounds from one island: bodies [s, e) and pairs [ps, pe), the islands laid out back to back in
+// root order over nb bodies; tile of an island = s / TB. The next island starts at e, so an island
+// closes its tile when e / TB lies beyond it, and then opens the tile e / TB; the first island opens
+// tile 0. Every tile that holds an island is written exactly once, with plain stores (the tile's
+// first start, last end, first pair start, last pair end); the others keep their initial values.
+__device__ __forceinline__ void island_tile_bounds(const TiledBuffers &T, int TB, int nb, int s, int e, int ps, int pe) {
+    int t = s / TB;
+    if (s == 0) { T.tb0[0] = 0; T.tp0[0] = ps; }
+    if (e >= nb || e / TB > t) {
+        T.tb1[t] = e; T.tp1[t] = pe;
+        if (e < nb) { T.tb0[e / TB] = e; T.tp0[e / TB] = pe; }
+    }
+}
+
 __global__ void k_t_tiles(int N, int TB, int *istat, TiledBuffers T) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool root = i < N && T.cnt_b[i] != 0;
     wave_atomic_count(&istat[IS_ISL], root);
     if (!root) return;
-    int t = T.boff[i] / TB;
-    atomicMin(&T.tb0[t], T.boff[i]);
-    atomicMax(&T.tb1[t], T.boff[i] + T.cnt_b[i]);
-    atomicMin(&T.tp0[t], T.poff[i]);
-    atomicMax(&T.tp1[t], T.poff[i] + T.cnt_p[i]);
+    island_tile_bounds(T, TB, istat[IS_NBODYSORT], T.boff[i], T.boff[i] + T.cnt_b[i], T.poff[i],
+                       T.poff[i] + T.cnt_p[i]);
 }
 
 // ── the island build of a scene of at most T_ISL_MAXN bodies in one workgroup ──
@@ -451,11 +462,7 @@ __global__ void __launch_bounds__(T_ISL_THREADS) k_t_islands_lds(int N, int capP
         if (b1 == cb[r]) continue;
         ++nroot;
         int p1 = r + 1 < N ? cp[r + 1] : nps;
-        int tt = cb[r] / TB;
-        atomicMin(&T.tb0[tt], cb[r]);
-        atomicMax(&T.tb1[tt], b1);
-        atomicMin(&T.tp0[tt], cp[r]);
-        atomicMax(&T.tp1[tt], p1);
+        island_tile_bounds(T, TB, nbs, cb[r], b1, cp[r], p1);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nroot += __shfl_xor(nroot, o);
