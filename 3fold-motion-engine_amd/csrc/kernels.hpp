@@ -215,9 +215,11 @@ __device__ __forceinline__ float box_cell(const int *istat, float margin) {
     return cell > 0.0f ? cell : 1.0f;
 }
 
+// cell -> bucket (mod the power-of-two table size): linear in cx, so the three cells (cx-1..cx+1) of a
+// neighbour row take consecutive buckets -- their ranges share cache lines and their items lie back to
+// back; cy, cz and the scene enter with large odd factors (a lookup tests the item's own cell)
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, int s) {
-    return ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u) ^
-           ((uint32_t)s * 2654435761u);
+    return (uint32_t)cx + (uint32_t)cy * 2654435761u + (uint32_t)cz * 2246822519u + (uint32_t)s * 3266489917u;
 }
 
 // hash grid: bucket per item (box centre or bead point), bucket counts
@@ -234,11 +236,19 @@ __global__ void k_grid_count(int n, const float *pos, const int *scene_of, int d
     bucket[i] = bk;
     atomicAdd(&bcount[bk], 1);
 }
-__global__ void k_grid_scatter(int n, const int *bucket, const int *bstart, int *bcursor, int *items) {
+// items in bucket order; with spos, also each item's position and id in the same order (one 16-byte
+// load per visited item for the bead grid)
+__global__ void k_grid_scatter(int n, const int *bucket, const int *bstart, int *bcursor, int *items, const float *pos,
+                               float4 *spos) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int bk = bucket[i];
-    items[bstart[bk] + atomicAdd(&bcursor[bk], 1)] = i;
+    int at = bstart[bk] + atomicAdd(&bcursor[bk], 1);
+    items[at] = i;
+    if (spos) {
+        f3 p = ld3(pos, i);
+        spos[at] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+    }
 }
 
 __device__ __forceinline__ bool aabb_overlap(const float *a, const float *b, float m) {
@@ -441,66 +451,120 @@ __global__ void k_bead_points(int N, int P, Bodies b, float *allp) {
     st3(allp, g, ld3(b.x, bi) + quat_rotate(ld4(b.q, bi), ld3(b.rest, g % P)));
 }
 
+// Every bead j > g of another body in the 27 grid cells around bead g (cell coordinates c, cell size
+// h) with 1e-9 < |xi - xj| < 2R: fn(j), in an order no caller depends on. The grid holds each item's
+// position and id in bucket order (spos); an item is in cell (X, Y, Z) iff floor(p / h) gives
+// (X, Y, Z), the grid's own cell rule. The 27 cells form 9 rows along x whose three cells take
+// consecutive buckets, so a row is one range of items (two range loads, usually one cache line);
+// the 9 ranges and then the first item of each are loaded together. A row whose buckets wrap around
+// the end of the table is walked bucket by bucket.
+template <class F>
+__device__ __forceinline__ void bead_neighbours(int g, int P, f3 xi, int4 c, const float4 *spos, const int *bstart,
+                                                const int *bcount, int T, float R, float h, F &&fn) {
+    const int bi = g / P;
+    const uint32_t mask = (uint32_t)(T - 1);
+    int lo[9], hi[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        uint32_t b0 = cell_hash(c.x - 1, c.y + r % 3 - 1, c.z + r / 3 - 1, 0) & mask;
+        if (b0 + 2u <= mask) {
+            lo[r] = bstart[b0];
+            hi[r] = bstart[b0 + 2] + bcount[b0 + 2];
+        } else {
+            lo[r] = 0;
+            hi[r] = -1;
+        }
+    }
+    float4 first[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) first[r] = hi[r] > lo[r] ? spos[lo[r]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    auto test = [&](float4 q, int Y, int Z) {
+        int j = __float_as_int(q.w);
+        if (j <= g || j / P == bi) return;
+        int fx = (int)floorf(q.x / h);
+        if (fx < c.x - 1 || fx > c.x + 1 || (int)floorf(q.y / h) != Y || (int)floorf(q.z / h) != Z) return;
+        f3 dv = xi - mk(q.x, q.y, q.z);
+        float dist = length(dv);
+        if (!(dist < 2.0f * R && dist > 1e-9f)) return;
+        fn(j);
+    };
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int Y = c.y + r % 3 - 1, Z = c.z + r / 3 - 1;
+        if (hi[r] >= 0) {
+            for (int t = lo[r]; t < hi[r]; ++t) test(t == lo[r] ? first[r] : spos[t], Y, Z);
+        } else {
+            uint32_t b0 = cell_hash(c.x - 1, Y, Z, 0) & mask;
+            for (uint32_t k = 0; k < 3u; ++k) {
+                uint32_t b = (b0 + k) & mask;
+                for (int t = bstart[b], e = bstart[b] + bcount[b]; t < e; ++t) test(spos[t], Y, Z);
+            }
+        }
+    }
+}
+
+// A5 k_gen over the bead grid: ground contact (z - R < 0) first, then the bead pairs in ascending j
+// (key g * (NP+1) + j + 1), at most ME_BEADCAND per bead. The count pass counts; the emit pass
+// writes the keys of its matches into its slots, sorts them there (a handful per bead) and then
+// fills the contacts in key order.
 template <bool EMIT>
 __global__ void k_bead_contacts(int NP, int P, const float *allp, const int *cellc, const int *bstart, const int *bcount,
-                                const int *items, int T, float R, const float *mu, int *cnt, const int *off, int capC,
+                                const float4 *spos, int T, float R, const float *mu, int *cnt, const int *off, int capC,
                                 Contacts Cc, int *istat) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= NP) return;
     f3 xi = ld3(allp, g);
     int bi = g / P;
-    int list[ME_BEADCAND];
-    int nc = 0;
-    int cx = cellc[4 * g], cy = cellc[4 * g + 1], cz = cellc[4 * g + 2];
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                int X = cx + dx, Y = cy + dy, Z = cz + dz;
-                int bk = (int)(cell_hash(X, Y, Z, 0) & (uint32_t)(T - 1));
-                int st = bstart[bk], c = bcount[bk];
-                for (int t = 0; t < c; ++t) {
-                    int j = items[st + t];
-                    if (j <= g || j / P == bi) continue;
-                    if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z) continue;
-                    f3 dv = xi - ld3(allp, j);
-                    float dist = length(dv);
-                    if (!(dist < 2.0f * R && dist > 1e-9f)) continue;
-                    if (nc < ME_BEADCAND) {
-                        int k = nc++;
-                        while (k > 0 && list[k - 1] > j) { list[k] = list[k - 1]; --k; }
-                        list[k] = j;
-                    } else {
-                        atomicOr(&istat[IS_OVF], OVF_CAND);
-                    }
-                }
-            }
+    int4 cg = *reinterpret_cast<const int4 *>(cellc + 4 * (size_t)g);
     float s = xi.z - R;
     int ground = s < 0.0f ? 1 : 0;
     if (!EMIT) {
+        int nc = 0;
+        bead_neighbours(g, P, xi, cg, spos, bstart, bcount, T, R, 2.0f * R, [&](int) { ++nc; });
+        if (nc > ME_BEADCAND) {
+            atomicOr(&istat[IS_OVF], OVF_CAND);
+            nc = ME_BEADCAND;
+        }
         cnt[g] = ground + nc;
         return;
     }
     int o = off[g];
-    int64_t stride = (int64_t)NP + 1;
+    int64_t stride = (int64_t)NP + 1, kg = (int64_t)g * stride;
     float m = mu[0];
     int c = o;
     if (ground) {
         if (c < capC) {
-            Cc.key[c] = (int64_t)g * stride; Cc.bi[c] = bi; Cc.bj[c] = -1;
+            Cc.key[c] = kg; Cc.bi[c] = bi; Cc.bj[c] = -1;
             st3(Cc.pA, c, xi); st3(Cc.pB, c, mk(0.0f, 0.0f, 0.0f)); st3(Cc.n, c, mk(0.0f, 0.0f, 1.0f));
             Cc.pen[c] = -s; Cc.mu[c] = m;
         }
         c++;
     }
-    for (int k = 0; k < nc; ++k, ++c) {
-        if (c >= capC) break;
-        int j = list[k];
+    const int c0 = c;
+    int nc = 0;
+    bead_neighbours(g, P, xi, cg, spos, bstart, bcount, T, R, 2.0f * R, [&](int j) {
+        if (nc < ME_BEADCAND) {
+            if (c0 + nc < capC) Cc.key[c0 + nc] = kg + (int64_t)j + 1;
+            ++nc;
+        }
+    });
+    int nw = c0 + nc <= capC ? nc : (capC > c0 ? capC - c0 : 0);
+    int64_t *key = Cc.key + c0;
+    for (int a = 1; a < nw; ++a) {
+        int64_t k = key[a];
+        int b = a;
+        while (b > 0 && key[b - 1] > k) { key[b] = key[b - 1]; --b; }
+        key[b] = k;
+    }
+    for (int k = 0; k < nw; ++k) {
+        int cc = c0 + k;
+        int j = (int)(key[k] - kg - 1);
         f3 xj = ld3(allp, j);
         f3 dv = xi - xj;
         float dist = length(dv);
-        Cc.key[c] = (int64_t)g * stride + (int64_t)j + 1; Cc.bi[c] = bi; Cc.bj[c] = j / P;
-        st3(Cc.pA, c, xi); st3(Cc.pB, c, xj); st3(Cc.n, c, dv / dist);
-        Cc.pen[c] = 2.0f * R - dist; Cc.mu[c] = m;
+        Cc.bi[cc] = bi; Cc.bj[cc] = j / P;
+        st3(Cc.pA, cc, xi); st3(Cc.pB, cc, xj); st3(Cc.n, cc, dv / dist);
+        Cc.pen[cc] = 2.0f * R - dist; Cc.mu[cc] = m;
     }
     if (o + ground + nc > capC) atomicOr(&istat[IS_OVF], OVF_CONTACTS);
 }

@@ -343,6 +343,9 @@ class GraphColoredContactEngine(_HipContactEngine):
             raise ValueError("contact_engine_gpu_colored: chunk_mode is 'registers' or 'global'")
         for k in _COLORED_IGNORED:
             kw.pop(k, None)
+        # the island-tiled solve is the default schedule: bit-identical to the per-colour launches
+        # (tiled=False), which stay available
+        kw.setdefault("tiled", True)
         super().__init__(dims=dims, vit=40 if vit is None else int(vit), pit=pit, mu=mu,
                          warm_start=warm_start, t2_damp=t2_damp, per_contact=not pair_chunks, **kw)
 

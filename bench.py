@@ -502,8 +502,9 @@ def bench_lattice(a):
     """The reference's own headline fixture (scripts/colored_vs_jacobi.py:104-135): N = 10 000 bead
     boxes 0.3 x 0.3 x 0.2 m in a two-layer lattice 0.35 m apart, dropped onto the ground; each engine
     built with its default arguments (vit 40, pit 20, mu 0.5) and stepped with step(1/240, substeps=1),
-    10 warm-up steps then 30 timed, wall clock per step. The default engines read their counters back
-    after every step, as the reference's do; 'tiled' is the island-tiled solve of the same engine."""
+    10 warm-up steps then 30 timed, wall clock per step. The engines read their counters back after
+    every step, as the reference's do. The coloured engines run their default island-tiled solve
+    and, for comparison, the per-colour launches (tiled=False)."""
     import torch
     pkg = importlib.import_module("3fold-motion-engine_amd")
     cen = pkg.scenes.lattice(10000)
@@ -512,8 +513,7 @@ def bench_lattice(a):
     runs = []
     for solver, tiled in (("color_cache", True), ("color_cache", False), ("pair_chunks", True),
                           ("pair_chunks", False), ("jacobi", False)):
-        kw = {"tiled": True} if tiled else {}
-        e = cls[solver](**kw)
+        e = cls[solver](**({} if solver == "jacobi" else {"tiled": tiled}))
         for c in cen:
             e.add_body(c)
         for _ in range(10):
