@@ -700,6 +700,16 @@ __device__ __forceinline__ void pc_load(PairCache &P, const TiledArgs &A, const 
     }
 }
 
+__device__ __forceinline__ void pc_store(const PairCache &P, const TiledArgs &A, int p) {
+    int st = A.Pp.pstart[p];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= P.cnt) break;
+        int c = st + k;
+        A.Cc.jn[c] = P.c[k].jn; A.Cc.jt1[c] = P.c[k].jt1; A.Cc.jt2[c] = P.c[k].jt2; A.Cc.jp[c] = P.c[k].jp;
+    }
+}
+
 // one contact of apply_warm (contact_engine_gpu_applied_warm.py:7-54)
 __device__ __forceinline__ void cc_warm(const CCon &C, f3 &va, f3 &wa, f3 &vb, f3 &wb, bool hasB, float iMA,
                                         float iMB, const m3 &IA, const m3 &IB) {
@@ -794,209 +804,45 @@ __device__ __forceinline__ void cc_pos(CCon &C, f3 &pvA, f3 &poA, f3 &pvB, f3 &p
     if (hasB && iMB > 0.0f) { pvB = pvB - Jn * iMB; poB = poB - mv(IB, cross(rB, Jn)); }
 }
 
-// ── small tiles: two lanes per pair ──
-// Lanes 2q and 2q+1 serve pair q of the tile: side 0 holds body A, side 1 body B. Both lanes hold
-// the pair's contact constants and impulses and evaluate every impulse update the same way; each
-// lane reads, updates and writes only its own body. The two point velocities are swapped between
-// the lanes with a DPP move, so an update puts one body's work on the chain instead of two, with the
-// expressions of cc_warm / cc_vel / cc_pos unchanged (hence the same bits). A pair's two lanes take
-// every branch together (all conditions are pair-wide), so both are active at each swap.
-__device__ __forceinline__ float lane_swap(float x) {
-    // quad_perm [1,0,3,2]: lane 2q <-> lane 2q+1
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
-}
-__device__ __forceinline__ f3 lane_swap3(f3 a) { return mk(lane_swap(a.x), lane_swap(a.y), lane_swap(a.z)); }
-
-struct SCon {
-    f3 n, r, t1, t2;  // r: this side's lever arm
-    float meff, meft1, meft2, bias, mu;
-    float jn, jt1, jt2, jp;
-};
-
-struct SideCache {
-    int cnt, slot;
-    bool side, hasB, dyn;  // dyn: this side's body is present and dynamic
-    float iM, iMB;         // this side's inverse mass; the pair's iMB
-    m3 I;                  // this side's world inverse inertia
-    f3 vs, ws, pvs, pos;   // this side's constant state (a static body; zero for the ground)
-    SCon c[4];
-};
-
-template <bool FIXED>
-__device__ __forceinline__ void sc_load(SideCache &S, const TiledArgs &A, const TiledBuffers &T, int p, int b0,
-                                        bool side) {
-    PairCache P;
-    pc_load<FIXED>(P, A, T, p, b0);
-    S.cnt = P.cnt;
-    S.side = side;
-    S.hasB = P.hasB;
-    S.iMB = P.iMB;
-    S.iM = side ? P.iMB : P.iMA;
-    S.dyn = side ? P.dynB : P.dynA;
-    S.slot = side ? P.lb : P.la;
-    S.I = side ? P.IB : P.IA;
-    S.vs = side ? P.vBs : P.vAs; S.ws = side ? P.wBs : P.wAs;
-    S.pvs = side ? P.pvBs : P.pvAs; S.pos = side ? P.poBs : P.poAs;
+// the cached pair: bodies from LDS (dynamic) or the cached static state
+__device__ __forceinline__ void pc_warm(const PairCache &P, float *sv, float *sw) {
+    f3 va = P.dynA ? ld3(sv, P.la) : P.vAs, wa = P.dynA ? ld3(sw, P.la) : P.wAs;
+    f3 vb = mk(0.0f, 0.0f, 0.0f), wb = mk(0.0f, 0.0f, 0.0f);
+    if (P.hasB) { vb = P.dynB ? ld3(sv, P.lb) : P.vBs; wb = P.dynB ? ld3(sw, P.lb) : P.wBs; }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const CCon &C = P.c[k];
-        SCon &D = S.c[k];
-        D.n = C.n; D.r = side ? C.rB : C.rA; D.t1 = C.t1; D.t2 = C.t2;
-        D.meff = C.meff; D.meft1 = C.meft1; D.meft2 = C.meft2; D.bias = C.bias; D.mu = C.mu;
-        D.jn = C.jn; D.jt1 = C.jt1; D.jt2 = C.jt2; D.jp = C.jp;
+        if (k >= P.cnt) break;
+        cc_warm(P.c[k], va, wa, vb, wb, P.hasB, P.iMA, P.iMB, P.IA, P.IB);
     }
-}
-
-__device__ __forceinline__ void sc_store(const SideCache &S, const TiledArgs &A, int p) {
-    int st = A.Pp.pstart[p];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k >= S.cnt) break;
-        int c = st + k;
-        A.Cc.jn[c] = S.c[k].jn; A.Cc.jt1[c] = S.c[k].jt1; A.Cc.jt2[c] = S.c[k].jt2; A.Cc.jp[c] = S.c[k].jp;
-    }
-}
-
-// this side's point velocity v + w x r (zero for the missing body of a ground pair), and the pair's
-// vrel = va - vb from it and the partner's
-__device__ __forceinline__ f3 sc_vrel(const SideCache &S, f3 v, f3 w, f3 r) {
-    f3 own = mk(0.0f, 0.0f, 0.0f);
-    if (!S.side || S.hasB) own = v + cross(w, r);
-    f3 other = lane_swap3(own);
-    return S.side ? other - own : own - other;
-}
-
-// this side's share of an impulse J: A gains it, B loses it (cc_vel's "vA + J*iMA", "vB - J*iMB")
-__device__ __forceinline__ void sc_apply(const SideCache &S, f3 &v, f3 &w, f3 r, f3 J) {
-    if (!S.dyn) return;
-    f3 dv = J * S.iM, dw = mv(S.I, cross(r, J));
-    if (S.side) { v = v - dv; w = w - dw; }
-    else { v = v + dv; w = w + dw; }
-}
-
-// one contact of apply_warm (cc_warm), this side
-__device__ __forceinline__ void sc_warm1(const SCon &C, const SideCache &S, f3 &v, f3 &w) {
-    f3 imp = (C.jn * C.n + C.jt1 * C.t1) + C.jt2 * C.t2;
-    if (!S.dyn) return;
-    f3 dv = S.iM * imp, dw = mv(S.I, cross(C.r, imp));
-    if (S.side) { v = v - dv; w = w - dw; }
-    else { v = v + dv; w = w + dw; }
-}
-
-// one contact of the chunk velocity sweep (cc_vel), this side
-template <bool FIXED>
-__device__ __forceinline__ void sc_vel1(SCon &C, const SideCache &S, f3 &v, f3 &w, float t2d) {
-    const f3 nrm = C.n, r = C.r;
-    f3 vrel = sc_vrel(S, v, w, r);
-    float vn = dot(vrel, nrm);
-    f3 t1, t2;
-    float meft1, meft2;
-    if (FIXED) {
-        t1 = C.t1; t2 = C.t2;
-    } else {
-        f3 vt = vrel - nrm * vn;
-        float mt = length(vt);
-        t1 = vt / mt;
-        if (mt <= 5e-3f) t1 = C.t1;
-        t2 = cross(nrm, t1);
-    }
-    if (!(C.meff >= 1e-12f)) return;
-    float jo = C.jn;
-    float dj = -vn / C.meff;
-    float nw = wmax(0.0f, jo + dj);
-    dj = nw - jo;
-    C.jn = nw;
-    f3 Jn = nrm * dj;
-    sc_apply(S, v, w, r, Jn);
-    vrel = sc_vrel(S, v, w, r);
-    if (FIXED) {
-        meft1 = C.meft1; meft2 = C.meft2;
-    } else {
-        // side A: iMA + rtA.IA.rtA; side B: rtB.IB.rtB; meft = (A's sum + iMB) + B's term
-        f3 rt1 = cross(r, t1), rt2 = cross(r, t2);
-        float e1 = dot(rt1, mv(S.I, rt1)), e2 = dot(rt2, mv(S.I, rt2));
-        if (!S.side) { e1 = S.iM + e1; e2 = S.iM + e2; }
-        float o1 = lane_swap(e1), o2 = lane_swap(e2);
-        float a1 = S.side ? o1 : e1, a2 = S.side ? o2 : e2;  // side A's sums
-        float b1 = S.side ? e1 : o1, b2 = S.side ? e2 : o2;  // side B's terms
-        meft1 = a1;
-        meft2 = a2;
-        if (S.hasB) {
-            meft1 = meft1 + S.iMB + b1;
-            meft2 = meft2 + S.iMB + b2;
-        }
-    }
-    if (meft1 >= 1e-12f && meft2 >= 1e-12f) {
-        float o1 = C.jt1, o2 = C.jt2;
-        float a1 = o1 - dot(vrel, t1) / meft1;
-        float a2 = o2 - dot(vrel, t2) / meft2;
-        float mag = sqrtf(a1 * a1 + a2 * a2);
-        float lim = C.mu * nw;
-        if (mag > lim && mag > 1e-12f) { a1 = a1 * lim / mag; a2 = a2 * lim / mag; }
-        C.jt1 = a1;
-        C.jt2 = a2;
-        f3 Jt = t1 * (a1 - o1) + t2 * (t2d * (a2 - o2));
-        sc_apply(S, v, w, r, Jt);
-    }
-}
-
-// one contact of the chunk position sweep (cc_pos), this side
-__device__ __forceinline__ void sc_pos1(SCon &C, const SideCache &S, f3 &pv, f3 &po) {
-    const f3 nrm = C.n, r = C.r;
-    f3 own = mk(0.0f, 0.0f, 0.0f);
-    if (!S.side || S.hasB) own = pv + cross(po, r);
-    f3 other = lane_swap3(own);
-    f3 pa = S.side ? other : own, pb = S.side ? own : other;
-    f3 rel = pa;
-    if (S.hasB) rel = rel - pb;
-    if (!(C.meff >= 1e-12f)) return;
-    float dj = (C.bias - dot(rel, nrm)) / C.meff;
-    float jo = C.jp;
-    float nw = wmax(0.0f, jo + dj);
-    dj = nw - jo;
-    C.jp = nw;
-    sc_apply(S, pv, po, r, nrm * dj);
-}
-
-// this side's body: from LDS (dynamic) or its constant state
-__device__ __forceinline__ void sc_body(const SideCache &S, const float *s0, const float *s1, f3 c0, f3 c1, f3 &a,
-                                        f3 &b) {
-    a = S.dyn ? ld3(s0, S.slot) : c0;
-    b = S.dyn ? ld3(s1, S.slot) : c1;
-}
-
-__device__ __forceinline__ void sc_warm(const SideCache &S, float *sv, float *sw) {
-    f3 v, w;
-    sc_body(S, sv, sw, S.vs, S.ws, v, w);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k >= S.cnt) break;
-        sc_warm1(S.c[k], S, v, w);
-    }
-    if (S.dyn) { st3(sv, S.slot, v); st3(sw, S.slot, w); }
+    if (P.iMA > 0.0f) { st3(sv, P.la, va); st3(sw, P.la, wa); }
+    if (P.hasB && P.iMB > 0.0f) { st3(sv, P.lb, vb); st3(sw, P.lb, wb); }
 }
 
 template <bool FIXED>
-__device__ __forceinline__ void sc_vel(SideCache &S, float *sv, float *sw, float t2d) {
-    f3 v, w;
-    sc_body(S, sv, sw, S.vs, S.ws, v, w);
+__device__ __forceinline__ void pc_vel(PairCache &P, float *sv, float *sw, float t2d) {
+    f3 vA = P.dynA ? ld3(sv, P.la) : P.vAs, wA = P.dynA ? ld3(sw, P.la) : P.wAs;
+    f3 vB = mk(0.0f, 0.0f, 0.0f), wB = mk(0.0f, 0.0f, 0.0f);
+    if (P.hasB) { vB = P.dynB ? ld3(sv, P.lb) : P.vBs; wB = P.dynB ? ld3(sw, P.lb) : P.wBs; }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (k >= S.cnt) break;
-        sc_vel1<FIXED>(S.c[k], S, v, w, t2d);
+        if (k >= P.cnt) break;
+        cc_vel<FIXED>(P.c[k], vA, wA, vB, wB, P.hasB, P.iMA, P.iMB, P.IA, P.IB, t2d);
     }
-    if (S.dyn) { st3(sv, S.slot, v); st3(sw, S.slot, w); }
+    if (P.iMA > 0.0f) { st3(sv, P.la, vA); st3(sw, P.la, wA); }
+    if (P.hasB && P.iMB > 0.0f) { st3(sv, P.lb, vB); st3(sw, P.lb, wB); }
 }
 
-__device__ __forceinline__ void sc_pos(SideCache &S, float *spv, float *spo) {
-    f3 pv, po;
-    sc_body(S, spv, spo, S.pvs, S.pos, pv, po);
+__device__ __forceinline__ void pc_pos(PairCache &P, float *spv, float *spo) {
+    f3 pvA = P.dynA ? ld3(spv, P.la) : P.pvAs, poA = P.dynA ? ld3(spo, P.la) : P.poAs;
+    f3 pvB = mk(0.0f, 0.0f, 0.0f), poB = mk(0.0f, 0.0f, 0.0f);
+    if (P.hasB) { pvB = P.dynB ? ld3(spv, P.lb) : P.pvBs; poB = P.dynB ? ld3(spo, P.lb) : P.poBs; }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (k >= S.cnt) break;
-        sc_pos1(S.c[k], S, pv, po);
+        if (k >= P.cnt) break;
+        cc_pos(P.c[k], pvA, poA, pvB, poB, P.hasB, P.iMA, P.iMB, P.IA, P.IB);
     }
-    if (S.dyn) { st3(spv, S.slot, pv); st3(spo, S.slot, po); }
+    if (P.iMA > 0.0f) { st3(spv, P.la, pvA); st3(spo, P.la, poA); }
+    if (P.hasB && P.iMB > 0.0f) { st3(spv, P.lb, pvB); st3(spo, P.lb, poB); }
 }
 
 // workgroup barrier that orders LDS only: s_waitcnt lgkmcnt(0) + s_barrier, global stores stay in
@@ -1034,15 +880,13 @@ __device__ __forceinline__ void tile_writeback(const TiledArgs &A, const TiledBu
     }
 }
 
-// small tiles: lanes 2t, 2t+1 own the t-th pair of the tile in colour order (one body side each), its
-// constants and impulses in registers; body velocities in LDS; one workgroup barrier per colour step
-#define T_SMALL_WG (2 * T_SMALL_P)
-
-// sweep-start record of a small tile: its bodies' state (LDS) and the pair's impulses (side 0 writes)
+// small tiles: thread t owns the t-th pair of the tile in colour order, its constants and impulses
+// in registers; body velocities in LDS; one workgroup barrier per colour step
+// sweep-start record of a small tile: its bodies' state (LDS) and this thread's pair's impulses
 __device__ __forceinline__ void tape_sweep(const TiledArgs &A, const TiledBuffers &T, int b0, int nb, int sweep,
-                                           const float *s0, const float *s1, const SideCache &S, int p) {
+                                           const float *s0, const float *s1, const PairCache &P, int p) {
     float *st = A.tape.sst + (size_t)sweep * 6 * A.N;
-    for (int j = threadIdx.x; j < nb; j += T_SMALL_WG) {
+    for (int j = threadIdx.x; j < nb; j += T_SMALL_P) {
         int i = T.bsorted[b0 + j];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -1050,20 +894,20 @@ __device__ __forceinline__ void tape_sweep(const TiledArgs &A, const TiledBuffer
             st[6 * (size_t)i + 3 + k] = s1[3 * j + k];
         }
     }
-    if (p >= 0 && !S.side) {
+    if (p >= 0) {
         int c0 = A.Pp.pstart[p];
         float *si = A.tape.simp + (size_t)sweep * 4 * A.capC;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (k >= S.cnt) break;
+            if (k >= P.cnt) break;
             float *o = si + 4 * (size_t)(c0 + k);
-            o[0] = S.c[k].jn; o[1] = S.c[k].jt1; o[2] = S.c[k].jt2; o[3] = S.c[k].jp;
+            o[0] = P.c[k].jn; o[1] = P.c[k].jt1; o[2] = P.c[k].jt2; o[3] = P.c[k].jp;
         }
     }
 }
 
 template <bool FIXED, bool TAPE>
-__global__ void __launch_bounds__(T_SMALL_WG) k_t_small(TiledArgs A, TiledBuffers T) {
+__global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers T) {
     int t = blockIdx.x;
     if (T.tb0[t] == INT_MAX) return;
     int b0, nb, p0, np;
@@ -1080,7 +924,7 @@ __global__ void __launch_bounds__(T_SMALL_WG) k_t_small(TiledArgs A, TiledBuffer
     }
     int ncol = A.istat[IS_NCOL];
     if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
-    for (int j = tid; j < nb; j += T_SMALL_WG) {
+    for (int j = tid; j < nb; j += T_SMALL_P) {
         int i = T.bsorted[b0 + j];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -1090,7 +934,7 @@ __global__ void __launch_bounds__(T_SMALL_WG) k_t_small(TiledArgs A, TiledBuffer
             spo[3 * j + k] = 0.0f;
         }
     }
-    for (int c = tid; c <= ncol; c += T_SMALL_WG) { cnt[c] = 0; fill[c] = 0; }
+    for (int c = tid; c <= ncol; c += T_SMALL_P) { cnt[c] = 0; fill[c] = 0; }
     __syncthreads();
     int myc = -1;
     if (tid < np) {
@@ -1112,44 +956,42 @@ __global__ void __launch_bounds__(T_SMALL_WG) k_t_small(TiledArgs A, TiledBuffer
     }
     __syncthreads();
     const int nc = maxc;
-    const int q = tid >> 1;
-    SideCache S;
-    S.side = tid & 1;
+    PairCache P;
     int p = -1, c_own = -1;
-    if (q < np) {
-        p = plist[q];
-        c_own = plcol[q];
-        sc_load<FIXED>(S, A, T, p, b0, tid & 1);
+    if (tid < np) {
+        p = plist[tid];
+        c_own = plcol[tid];
+        pc_load<FIXED>(P, A, T, p, b0);
     }
     if (np > 0) {
         if (FIXED) {
             for (int c = 0; c < nc; ++c) {
-                if (c_own == c) sc_warm(S, sv, sw);
+                if (c_own == c) pc_warm(P, sv, sw);
                 lds_barrier();
             }
         }
         const float t2d = FIXED ? 1.0f : A.t2d;
         for (int it = 0; it < A.vit; ++it) {
-            if (TAPE) tape_sweep(A, T, b0, nb, it, sv, sw, S, p);
+            if (TAPE) tape_sweep(A, T, b0, nb, it, sv, sw, P, p);
             for (int c = 0; c < nc; ++c) {
-                if (c_own == c) sc_vel<FIXED>(S, sv, sw, t2d);
+                if (c_own == c) pc_vel<FIXED>(P, sv, sw, t2d);
                 lds_barrier();
             }
         }
         for (int it = 0; it < A.pit; ++it) {
-            if (TAPE) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, S, p);
+            if (TAPE) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, p);
             for (int c = 0; c < nc; ++c) {
-                if (c_own == c) sc_pos(S, spv, spo);
+                if (c_own == c) pc_pos(P, spv, spo);
                 lds_barrier();
             }
         }
     } else if (TAPE) {
-        for (int it = 0; it < A.vit; ++it) tape_sweep(A, T, b0, nb, it, sv, sw, S, -1);
-        for (int it = 0; it < A.pit; ++it) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, S, -1);
+        for (int it = 0; it < A.vit; ++it) tape_sweep(A, T, b0, nb, it, sv, sw, P, -1);
+        for (int it = 0; it < A.pit; ++it) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, -1);
     }
-    if (p >= 0 && !S.side) sc_store(S, A, p);
+    if (p >= 0) pc_store(P, A, p);
     __syncthreads();
-    tile_writeback<T_SMALL_WG>(A, T, b0, nb, true, sv, sw, spv, spo);
+    tile_writeback<T_SMALL_P>(A, T, b0, nb, true, sv, sw, spv, spo);
 }
 
 // per-contact constants of the substep (k_t_prep): the same expressions pc_load evaluates
@@ -1309,10 +1151,10 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
     if (A.taped)
-        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile), dim3(T_SMALL_WG), 0, s, A, T);
+        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
     else
-        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>), dim3(ntile), dim3(T_SMALL_WG), 0, s,
-                           A, T);
+        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>), dim3(ntile), dim3(T_SMALL_P), 0, s, A,
+                           T);
     if (A.pev0 && hipEventRecord((hipEvent_t)A.pev0, s2) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(T_BIG_WG), dyn, s2, A, T);
     if (A.pev1 && hipEventRecord((hipEvent_t)A.pev1, s2) != hipSuccess) return ME_ERUNTIME;
