@@ -1,7 +1,8 @@
 """Autotuner for the coloured solvers: colour classes, substeps, iterations and tile shape.
 
-Searches colours in {8, 16, 32}, substeps in {1, 2, 4} and velocity iterations in {8, 16, 32}
-(position iterations follow as ceil(5/8 * vit), the 16 -> 10 ratio of the pile workload) in order of
+Searches colours in {8, 16, 32}, substeps in {1, 2, 4}, velocity iterations in {8, 16, 32} and bodies
+per tile in {128, 192, 224} (position iterations follow as ceil(5/8 * vit), the 16 -> 10 ratio of the
+pile workload) in order of
 increasing solve cost (substeps * vit * colours), and returns the first configuration whose trial keeps
 the maximum penetration at or below the target (1e-3 m by default); among configurations of equal
 cost the faster measured one wins. Each trial steps a copy of the caller's scene for ``trial_steps``
@@ -78,7 +79,7 @@ def run_trial(make_engine, bodies: BodySet, colours, substeps, vit, pit, tile_bo
 
 
 def autotune(make_engine, bodies: BodySet, target_pen=1e-3, steps=40, dt=1 / 240, warm_state=None,
-             colours=COLOURS, substeps=SUBSTEPS, iters=ITERS, tile_bodies=(128,), log=None) -> TuneResult:
+             colours=COLOURS, substeps=SUBSTEPS, iters=ITERS, tile_bodies=(128, 192, 224), log=None) -> TuneResult:
     """make_engine(**kw) -> a fresh tiled coloured engine (kw: n_colors, vit, pit, tile_bodies)."""
     cands = []
     for c, s, v, tb in itertools.product(colours, substeps, iters, tile_bodies):

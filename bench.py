@@ -581,10 +581,10 @@ def bench_batched(a):
     torch.cuda.synchronize()
     t_b = (time.perf_counter() - t0) / a.steps
     cnt = e.refresh_counters()
-    # sequential plain launches: one engine per scene, same state
+    # sequential plain launches: one engine per scene, same state (skipped with --seq-rounds 0)
     offs = np.concatenate([[0], np.cumsum(sizes)])
     engines = []
-    for k in range(a.scenes):
+    for k in range(a.scenes if a.seq_rounds > 0 else 0):
         sl = slice(offs[k], offs[k + 1])
         part = pkg.bodies.BodySet(x=x0[sl].copy(), q=q0[sl].copy(), v=v0[sl].copy(), w=w0[sl].copy(),
                                   invM=bodies.invM[sl], IbInv=bodies.IbInv[sl], kin=bodies.kin[sl],
@@ -600,17 +600,18 @@ def bench_batched(a):
         for ek in engines:
             ek.step(dt, S)
     torch.cuda.synchronize()
-    t_s = (time.perf_counter() - t0) / a.seq_rounds
+    t_s = (time.perf_counter() - t0) / a.seq_rounds if a.seq_rounds > 0 else float("nan")
     out = {"metric": METRIC, "value": round(N / t_b, 1), "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": round(1e3 * t_b, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box piles)",
            "config": {"workload": f"{a.scenes} independent box piles, N_s log-uniform [8, 2048], one tiled launch "
                                   f"sequence per substep", "bodies": int(N), "scenes": a.scenes,
                       "colors": kw.get("n_colors"), "substeps": S, "vit": kw.get("vit"), "pit": kw.get("pit"),
-                      "autotuned": tune is not None, "graph": bool(a.graph), "pairs": cnt["n_pairs"], "islands": cnt["n_islands"]},
-           "sequential_plain_launches": {"value": round(N / t_s, 1), "unit": "body-steps/s",
-                                         "ms_per_round": round(1e3 * t_s, 3)},
-           "speedup_vs_sequential": round(t_s / t_b, 2)}
+                      "autotuned": tune is not None, "graph": bool(a.graph), "pairs": cnt["n_pairs"], "islands": cnt["n_islands"],
+                      "tile_bodies": a.tile_bodies, "tiles": cnt["n_tiles"], "max_tile_bodies": cnt["max_tile_bodies"]},
+           "sequential_plain_launches": ({"value": round(N / t_s, 1), "unit": "body-steps/s",
+                                          "ms_per_round": round(1e3 * t_s, 3)} if a.seq_rounds > 0 else None),
+           "speedup_vs_sequential": round(t_s / t_b, 2) if a.seq_rounds > 0 else None}
     if tune is not None:
         out["autotune"] = {"colours": tune.colours, "substeps": tune.substeps, "vit": tune.vit, "pit": tune.pit,
                            "max_pen": tune.max_pen}
