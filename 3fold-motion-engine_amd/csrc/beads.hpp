@@ -216,20 +216,26 @@ __global__ void k_bp_select(int cap, int *istat, const int *cand_a, const float 
     pflag[pair] = L > 0 ? 1 : 0;
     if (L == 0) { kcnt[pair] = 0; return; }
     int wn = warm ? istat[IS_WN] : 0;
+    // the pair's keys ascend, and so do the previous step's kept keys: one binary search for the
+    // first kept key >= the pair's smallest key, then a merge (the exact-match set of a search per key)
+    int j = 0;
+    if (wn > 0) {
+        int64_t k0 = raw.key[s0];
+        int lo = 0, hi = wn;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (wkey[mid] < k0) lo = mid + 1;
+            else hi = mid;
+        }
+        j = lo;
+    }
     for (int t = 0; t < L; ++t) {
         int c = s0 + t;
         raw.jnd[c] = 0.0; raw.jt1[c] = 0.0f; raw.jt2[c] = 0.0f;
         if (wn == 0) continue;
         int64_t k = raw.key[c];
-        int lo = 0, hi = wn - 1, hit = -1;
-        while (lo <= hi) {
-            int mid = (lo + hi) / 2;
-            int64_t km = wkey[mid];
-            if (km == k) { hit = mid; lo = hi + 1; }
-            else if (km < k) lo = mid + 1;
-            else hi = mid - 1;
-        }
-        if (hit >= 0) { raw.jnd[c] = (double)wjn[hit]; raw.jt1[c] = wjt1[hit]; raw.jt2[c] = wjt2[hit]; }
+        while (j < wn && wkey[j] < k) ++j;
+        if (j < wn && wkey[j] == k) { raw.jnd[c] = (double)wjn[j]; raw.jt1[c] = wjt1[j]; raw.jt2[c] = wjt2[j]; }
     }
     int k = bead_select(s0, L, mx, ld3(x, cand_a[pair]), raw);
     if (k < 0) {

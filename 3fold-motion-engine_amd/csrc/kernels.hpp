@@ -917,10 +917,17 @@ __global__ void k_jp_win(int capP, int *istat, int cur, int *col, const int *pbi
 }
 
 // colour class sizes; then (single thread) the split of classes into max(K, JP colours) classes
-__global__ void k_color_count(int capP, const int *istat, const int *col, int *ccount) {
+// colour class sizes: a histogram per workgroup in LDS, then one device atomic per non-empty class per
+// workgroup (there are few classes, so per-pair device atomics on their counters would contend)
+__global__ void __launch_bounds__(ME_THREADS) k_color_count(int capP, const int *istat, const int *col, int *ccount) {
+    __shared__ int h[ME_NCOLMAX + 1];
+    for (int c = threadIdx.x; c <= ME_NCOLMAX; c += blockDim.x) h[c] = 0;
+    __syncthreads();
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= istat[IS_P] || p >= capP) return;
-    atomicAdd(&ccount[col[p]], 1);
+    if (p < istat[IS_P] && p < capP) atomicAdd(&h[col[p]], 1);
+    __syncthreads();
+    for (int c = threadIdx.x; c <= ME_NCOLMAX; c += blockDim.x)
+        if (h[c]) atomicAdd(&ccount[c], h[c]);
 }
 // sub[c] colours for JP class c: the extra K - ncol classes go one at a time to the class with the
 // most pairs per sub-colour (lowest class first); base[c] = first final colour of class c
@@ -950,11 +957,25 @@ __global__ void k_color_split_apply(int capP, const int *istat, int *col, const 
     col[p] = base[c] + p % sub[c];
 }
 // group pairs by colour (order inside a colour is free: its pairs touch disjoint dynamic bodies)
-__global__ void k_color_scatter(int capP, const int *istat, const int *col, const int *cstart, int *cfill, int *porder) {
+// pairs grouped by colour (any order within a colour: its pairs share no dynamic body); ranks within
+// the workgroup from LDS atomics, one device atomic per class per workgroup for the group's base
+__global__ void __launch_bounds__(ME_THREADS) k_color_scatter(int capP, const int *istat, const int *col,
+                                                              const int *cstart, int *cfill, int *porder) {
+    __shared__ int h[ME_NCOLMAX + 1], base[ME_NCOLMAX + 1];
+    for (int c = threadIdx.x; c <= ME_NCOLMAX; c += blockDim.x) h[c] = 0;
+    __syncthreads();
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= istat[IS_P] || p >= capP) return;
-    int c = col[p];
-    porder[cstart[c] + atomicAdd(&cfill[c], 1)] = p;
+    bool live = p < istat[IS_P] && p < capP;
+    int c = 0, r = 0;
+    if (live) {
+        c = col[p];
+        r = atomicAdd(&h[c], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k <= ME_NCOLMAX; k += blockDim.x)
+        if (h[k]) base[k] = atomicAdd(&cfill[k], h[k]);
+    __syncthreads();
+    if (live) porder[cstart[c] + base[c] + r] = p;
 }
 __global__ void k_cc_store(int N, int capP, int *istat, const int *pbi, const int *pbj, const int *col, const float *invM,
                            int *ca, int *cb, int *ccol, float *cinvM, int hit) {
