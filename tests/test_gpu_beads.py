@@ -6,6 +6,7 @@ Bar: bit-exact states every step, equal counters (generated and solved contacts,
 and equal contact lists once the device's pair-major keys are mapped back to the reference's
 feature keys. The generated-contact counts are also checked against the reference's archived
 receipts. Every test prints its case before the GPU work starts."""
+import hashlib
 import importlib
 import json
 from pathlib import Path
@@ -28,7 +29,8 @@ def _say(*a):
 
 def _bead_engine(kind, bodyset, dens, **kw):
     pkg = _pkg()
-    cls = {"pair_chunks": pkg.GraphColoredContactEngine, "color_cache": pkg.ColorCacheContactEngine}[kind]
+    cls = {"pair_chunks": pkg.GraphColoredContactEngine, "color_cache": pkg.ColorCacheContactEngine,
+           "jacobi": pkg.RelaxedJacobiContactEngine}[kind]
     e = cls(vit=40, pit=10, **kw)
     for c, d, k in zip(bodyset.x, dens, bodyset.kin):
         e.add_body([float(t) for t in c], density=d, kin=bool(k))
@@ -166,3 +168,39 @@ def test_bead_jumble_with_kinematic_bodies_matches_oracle(oracle, kind, tiled):
             _assert_bead_contacts_equal(e, o, f"step {s}")
     assert reduced > 0  # the reduction ran
     np.testing.assert_array_equal(e.contact_forces(), o.forces())
+
+
+def test_jacobi_bead_lattice1000_matches_oracle_and_archive(oracle):
+    """Relaxed Jacobi on the bead grid (90 contacts per 10 bodies, several per body in every wave):
+    the per-body entry lists and the 27-cell grid walk against the oracle, and the positions against
+    the archive's per-step sha256 (jacobi__lattice1000, 40 steps)."""
+    _say("jacobi bead lattice1000 x40 vs oracle + archive digests")
+    pkg = _pkg()
+    rec = json.loads((GOLD / "reference_trajectories.json").read_text())["records"]["jacobi__lattice1000"]
+    cen = pkg.scenes.lattice(1000)
+    bs = pkg.bodies.bead_bodies(cen, [700.0] * 1000, [0] * 1000)
+    e = _bead_engine("jacobi", bs, [700.0] * 1000)
+    o = oracle.OracleWorld(bs, "jacobi", vit=40, pit=10)
+    for s in range(40):
+        e.step(1 / 240)
+        o.step(1 / 240)
+        _assert_state_equal(e, o, f"step {s}")
+        assert e.n_contacts == o.counters()["n_contacts"] == rec["contacts"][s], f"step {s}"
+        x = e.state_numpy()[0]
+        assert hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest() == rec["sha256_per_step"]["xc"][s], \
+            f"step {s}: positions differ from the archive"
+    np.testing.assert_array_equal(e.contact_forces(), o.forces())
+
+
+def test_jacobi_bead_jumble_with_kinematic_bodies_matches_oracle(oracle):
+    """Relaxed Jacobi on tilted, spinning bead boxes with two kinematic ones: entries of static
+    bodies, one-sided and two-sided contacts in the same waves."""
+    _say("jacobi bead jumble (32 bodies, 2 kinematic) x80")
+    bs, dens = _jumble()
+    e = _bead_engine("jacobi", bs, dens)
+    o = oracle.OracleWorld(bs, "jacobi", vit=40, pit=10)
+    for s in range(80):
+        e.step(1 / 240)
+        o.step(1 / 240)
+        _assert_state_equal(e, o, f"step {s}")
+        assert e.n_contacts == o.counters()["n_contacts"], f"step {s}"
