@@ -68,7 +68,9 @@ def main():
     ap.add_argument("--vit", type=int, default=16)
     ap.add_argument("--pit", type=int, default=10)
     ap.add_argument("--tiled", type=int, default=1)
-    ap.add_argument("--tile-bodies", type=int, default=128)
+    ap.add_argument("--tile-bodies", type=int, default=0,
+                    help="bodies per tile of the tiled solve (0: 128, and 224 for the 256-scene batch -- the "
+                         "sizes the autotuner's tile search picks for them)")
     ap.add_argument("--graph", type=int, default=0, help="replay each step as one captured hipGraph (tiled path)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -127,7 +129,7 @@ def main():
            "jacobi": pkg.RelaxedJacobiContactEngine}[a.solver]
     kw = dict(vit=a.vit, pit=a.pit, check_every_step=False)
     if a.solver != "jacobi":
-        kw.update(n_colors=a.colors, tiled=bool(a.tiled), tile_bodies=a.tile_bodies, graph_capture=bool(a.graph))
+        kw.update(n_colors=a.colors, tiled=bool(a.tiled), tile_bodies=a.tile_bodies or 128, graph_capture=bool(a.graph))
     e = cls(**kw)
     e.add_boxes(bodies)
     dt, S = 1 / 240, a.substeps
@@ -569,7 +571,7 @@ def bench_pile_tuned(a):
             worst = max(worst, at._max_penetration(e))
         return worst
 
-    warm = mk(n_colors=a.colors, vit=a.vit, pit=a.pit, tile_bodies=a.tile_bodies)
+    warm = mk(n_colors=a.colors, vit=a.vit, pit=a.pit, tile_bodies=a.tile_bodies or 128)
     warm.add_boxes(bodies)
     for _ in range(a.warmup):
         warm.step(dt, a.substeps)
@@ -610,7 +612,8 @@ def bench_batched(a):
     bodies, sizes = pkg.scenes.batched_piles(a.scenes, seed=a.seed)
     N = bodies.n
     dt = 1 / 240
-    kw = dict(n_colors=a.colors, vit=a.vit, pit=a.pit, tile_bodies=a.tile_bodies)
+    tb = a.tile_bodies or 224
+    kw = dict(n_colors=a.colors, vit=a.vit, pit=a.pit, tile_bodies=tb)
     S = a.substeps
     tune = None
     if a.autotune:
@@ -668,7 +671,8 @@ def bench_batched(a):
                                   f"sequence per substep", "bodies": int(N), "scenes": a.scenes,
                       "colors": kw.get("n_colors"), "substeps": S, "vit": kw.get("vit"), "pit": kw.get("pit"),
                       "autotuned": tune is not None, "graph": bool(a.graph), "pairs": cnt["n_pairs"], "islands": cnt["n_islands"],
-                      "tile_bodies": a.tile_bodies, "tiles": cnt["n_tiles"], "max_tile_bodies": cnt["max_tile_bodies"]},
+                      "tile_bodies": kw.get("tile_bodies"), "tiles": cnt["n_tiles"],
+                      "max_tile_bodies": cnt["max_tile_bodies"]},
            "sequential_plain_launches": ({"value": round(N / t_s, 1), "unit": "body-steps/s",
                                           "ms_per_round": round(1e3 * t_s, 3)} if a.seq_rounds > 0 else None),
            "speedup_vs_sequential": round(t_s / t_b, 2) if a.seq_rounds > 0 else None}
