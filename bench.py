@@ -307,7 +307,7 @@ def bench_adjoint(a):
     N = b.n
     dt, S = 1 / 240, a.substeps
     ro = pkg.adjoint.Rollout(b, steps=T, dt=dt, substeps=S, vit=a.vit, pit=a.pit, n_colors=a.colors, k=k,
-                             tile_bodies=64)
+                             tile_bodies=a.tile_bodies or 128)
     fn = pkg.adjoint.contact_rollout(ro)
     dev = ro.eng.x.device
     f = lambda x: torch.tensor(np.asarray(x), dtype=torch.float32, device=dev)
