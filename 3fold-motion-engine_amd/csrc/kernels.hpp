@@ -257,27 +257,31 @@ __device__ __forceinline__ bool aabb_overlap(const float *a, const float *b, flo
 }
 
 // ── A6 body pairs: ground candidate + AABB overlaps with higher bodies of the same scene, in key order ──
-// One wave per body a: lane l < 27 scans neighbour cell l's grid bucket for higher bodies of the same
-// scene whose AABB overlaps a's (at most ME_CELLCAND per cell); a wave prefix sum places the hits in
-// LDS and every lane ranks one hit among them, so the list comes out ascending behind the ground
-// candidate (the pair-key order a*(N+1) + b + 1). all_pairs (bead geometry): pairs of two static
-// bodies and ground pairs of static bodies are kept, as the reference's bead contact generation
-// emits their contacts too (contact_engine_gpu_colored.py:103-122).
+// Half a wave per body a (32 lanes, two bodies per wave, eight per workgroup): lane l < 27 scans
+// neighbour cell l's grid bucket for higher bodies of the same scene whose AABB overlaps a's (at most
+// ME_CELLCAND per cell); a prefix sum over the half wave places the hits in LDS and the lanes rank
+// them, so the list comes out ascending behind the ground candidate (the pair-key order
+// a*(N+1) + b + 1). all_pairs (bead geometry): pairs of two static bodies and ground pairs of static
+// bodies are kept, as the reference's bead contact generation emits their contacts too
+// (contact_engine_gpu_colored.py:103-122).
 #define ME_CELLCAND 16
-__global__ void __launch_bounds__(64) k_box_pairs(int N, Bodies b, const float *aabb, const int *cellc,
-                                                  const int *bstart, const int *bcount, const int *items, int T,
-                                                  float margin, int all_pairs, int *cand_n, int *cand_tmp,
-                                                  int *istat) {
-    __shared__ int hits[ME_MAXCAND];
-    int a = blockIdx.x, lane = threadIdx.x;
-    if (a >= N) return;  // uniform: one body per block
+#define ME_BP_PER_WG 8
+__global__ void __launch_bounds__(32 * ME_BP_PER_WG) k_box_pairs(int N, Bodies b, const float *aabb, const int *cellc,
+                                                                 const int *bstart, const int *bcount, const int *items,
+                                                                 int T, float margin, int all_pairs, int *cand_n,
+                                                                 int *cand_tmp, int *istat) {
+    __shared__ int hits_all[ME_BP_PER_WG][ME_MAXCAND];
+    const int half = threadIdx.x >> 5, lane = threadIdx.x & 31;
+    const int a = blockIdx.x * ME_BP_PER_WG + half;
+    const bool live = a < N;  // no early exit: the half waves shuffle and the block synchronises
+    int *hits = hits_all[half];
     float m = margin + ME_BP_SLACK;
-    const float *A = aabb + 6 * a;
-    bool dynA = b.invM[a] > 0.0f;
+    const float *A = aabb + 6 * (live ? a : 0);
+    bool dynA = live && b.invM[a] > 0.0f;
     int mine[ME_CELLCAND];
     int n = 0;
     bool over = false;
-    if (lane < 27) {
+    if (live && lane < 27) {
         int X = cellc[4 * a] + lane % 3 - 1, Y = cellc[4 * a + 1] + (lane / 3) % 3 - 1, Z = cellc[4 * a + 2] + lane / 9 - 1;
         int s = cellc[4 * a + 3];
         int bk = (int)(cell_hash(X, Y, Z, s) & (uint32_t)(T - 1));
@@ -292,31 +296,31 @@ __global__ void __launch_bounds__(64) k_box_pairs(int N, Bodies b, const float *
             else over = true;
         }
     }
-    int first = ((all_pairs || dynA) && A[2] < m) ? 1 : 0;
+    int first = (live && (all_pairs || dynA) && A[2] < m) ? 1 : 0;
     int incl = n;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int v = __shfl_up(incl, o);
+    for (int o = 1; o < 32; o <<= 1) {
+        int v = __shfl_up(incl, o, 32);
         if (lane >= o) incl += v;
     }
-    int total = __shfl(incl, 63);
+    int total = __shfl(incl, 31, 32);
     int off = first + incl - n;
     for (int k = 0; k < n; ++k)
         if (off + k < ME_MAXCAND) hits[off + k] = mine[k];
     over = over || first + total > ME_MAXCAND;
     __syncthreads();
     int nc = first + total < ME_MAXCAND ? first + total : ME_MAXCAND;
-    if (lane >= first && lane < nc) {
-        int c = hits[lane], rank = first;
+    for (int q = first + lane; live && q < nc; q += 32) {
+        int c = hits[q], rank = first;
         for (int t = first; t < nc; ++t) rank += hits[t] < c;
         cand_tmp[a * ME_MAXCAND + rank] = c;
     }
-    if (lane == 0) {
+    if (live && lane == 0) {
         if (first) cand_tmp[a * ME_MAXCAND] = -1;
         cand_n[a] = nc;
         if (a == 0) istat[IS_MAXEXT] = 0;  // read by this substep's grid only; the next k_body_pre maxes into it
     }
-    if (__any(over) && lane == 0) atomicOr(&istat[IS_OVF], OVF_CAND);
+    if (__any(over) && threadIdx.x % 64 == 0) atomicOr(&istat[IS_OVF], OVF_CAND);
 }
 
 __global__ void k_box_cand_emit(int N, const int *cand_n, const int *cand_off, const int *cand_tmp, int cap,

@@ -342,7 +342,8 @@ int body_pairs(me_engine *e, int all_pairs, hipStream_t s) {
     int N = e->N;
     int rc = build_grid(e, N, e->b.x, e->b.scene, 1, -1.0f, nullptr, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_box_pairs, dim3(N), dim3(64), 0, s, N, b, e->aabb, e->cellc, e->bstart, e->bcount, e->items,
+    hipLaunchKernelGGL(k_box_pairs, dim3((N + ME_BP_PER_WG - 1) / ME_BP_PER_WG), dim3(32 * ME_BP_PER_WG), 0, s, N, b,
+                       e->aabb, e->cellc, e->bstart, e->bcount, e->items,
                        e->T, e->cfg.margin, all_pairs, e->cand_n, e->cand_tmp, e->istat);
     rc = scan(e, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
     if (rc) return rc;
