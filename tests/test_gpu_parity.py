@@ -233,3 +233,29 @@ def test_graph_replay_matches_eager(oracle):
     assert eg._counters["max_tile_bodies"] >= 600
     _assert_state_equal(eg, o, "final")
     _assert_contacts_equal(eg, o, "final")
+
+
+def test_tile_size_does_not_change_the_bits():
+    """The island tiling is a schedule only: a batch of 24 piles stepped at 64, 128 and 224 bodies per
+    tile (224 overflows some tiles past 256 pairs into the big-tile kernel) gives the same state bits,
+    contact impulses and colours after 60 steps of the batched bench configuration."""
+    _say("color_cache batch of 24 piles x60 S2 V16 P10 at tile_bodies 64 / 128 / 224")
+    pkg = _pkg()
+    bodies, _ = pkg.scenes.batched_piles(24, lo=8, hi=2048, seed=5)
+    runs = []
+    for tb in (64, 128, 224):
+        e = _engine("color_cache", vit=16, pit=10, n_colors=16, tiled=True, tile_bodies=tb, check_every_step=False)
+        e.add_boxes(bodies)
+        for _ in range(60):
+            e.step(1 / 240, 2)
+        e.refresh_counters()
+        runs.append((e.state_numpy(), e.contacts(), e.pairs()["color"], e._counters["n_tiles"]))
+        del e
+    (s0, c0, col0, t0) = runs[0]
+    for (s1, c1, col1, t1), tb in zip(runs[1:], (128, 224)):
+        assert t1 < t0, f"tile_bodies {tb}: expected fewer tiles than at 64 ({t1} vs {t0})"
+        for a, b, name in zip(s0, s1, "xqvw"):
+            assert np.array_equal(a, b), f"tile_bodies {tb}: {name} differs from tile_bodies 64"
+        for k in ("key", "jn", "jt1", "jt2", "jp"):
+            assert np.array_equal(c0[k], c1[k]), f"tile_bodies {tb}: contact {k} differs"
+        assert np.array_equal(col0, col1), f"tile_bodies {tb}: colours differ"
