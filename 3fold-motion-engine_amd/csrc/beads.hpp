@@ -204,20 +204,11 @@ __device__ int bead_select(int s0, int L, int mx, f3 ref, BeadRaw raw) {
     return mx;
 }
 
-__global__ void k_bp_select(int cap, int *istat, const int *cand_a, const float *x, const int *cnt, const int *off,
-                            int capR, int mx, int warm, const int64_t *wkey, const float *wjn, const float *wjt1,
-                            const float *wjt2, BeadRaw raw, int *kcnt, int *pflag) {
-    int pair = blockIdx.x * blockDim.x + threadIdx.x;
-    int n = istat[IS_CAND];
-    if (n > cap) n = cap;
-    if (pair >= n) return;
-    int L = cnt[pair], s0 = off[pair];
-    if (L > 0 && s0 + L > capR) L = 0;  // overflow already flagged by the emit pass
-    pflag[pair] = L > 0 ? 1 : 0;
-    if (L == 0) { kcnt[pair] = 0; return; }
-    int wn = warm ? istat[IS_WN] : 0;
-    // the pair's keys ascend, and so do the previous step's kept keys: one binary search for the
-    // first kept key >= the pair's smallest key, then a merge (the exact-match set of a search per key)
+// warm start of one pair's generated contacts by one thread: the pair's keys ascend, and so do the
+// previous step's kept keys, so one binary search for the first kept key >= the pair's smallest key
+// and then a merge give the exact-match set of a search per key
+__device__ void bead_warm_serial(int s0, int L, int wn, const int64_t *wkey, const float *wjn, const float *wjt1,
+                                 const float *wjt2, BeadRaw raw) {
     int j = 0;
     if (wn > 0) {
         int64_t k0 = raw.key[s0];
@@ -237,12 +228,186 @@ __global__ void k_bp_select(int cap, int *istat, const int *cand_a, const float 
         while (j < wn && wkey[j] < k) ++j;
         if (j < wn && wkey[j] == k) { raw.jnd[c] = (double)wjn[j]; raw.jt1[c] = wjt1[j]; raw.jt2[c] = wjt2[j]; }
     }
-    int k = bead_select(s0, L, mx, ld3(x, cand_a[pair]), raw);
-    if (k < 0) {
-        atomicOr(&istat[IS_OVF], OVF_CONTACTS);
-        k = L;
+}
+
+// A14 + A15 of one candidate pair by a group of BP_G lanes, one generated contact per lane (segments
+// of at most BP_G contacts with finite inputs; otherwise the group's first lane runs the one-thread
+// form, bead_warm_serial + bead_select). Every step evaluates bead_select's expressions on the
+// lane's own contact; the arg-max / arg-min choices reduce over the group with bead_select's
+// tie-break (first index), and the float64 redistribution sums each kept point's share in contact
+// order, as bead_select does.
+#define BP_G 32
+__device__ __forceinline__ double bp_shfl(double v, int src) { return __shfl(v, src, BP_G); }
+
+__device__ __forceinline__ void bp_select_pair(int pair, int *istat, const int *cand_a, const float *x,
+                                               const int *cnt, const int *off, int capR, int mx, int warm,
+                                               const int64_t *wkey, const float *wjn, const float *wjt1,
+                                               const float *wjt2, BeadRaw raw, int *kcnt, int *pflag) {
+    const int lane = threadIdx.x & (BP_G - 1);
+    int L = cnt[pair], s0 = off[pair];
+    if (L > 0 && s0 + L > capR) L = 0;  // overflow already flagged by the emit pass
+    if (lane == 0) pflag[pair] = L > 0 ? 1 : 0;
+    if (L == 0) {
+        if (lane == 0) kcnt[pair] = 0;
+        return;
     }
-    kcnt[pair] = k;
+    const int wn = warm ? istat[IS_WN] : 0;
+    const f3 ref = ld3(x, cand_a[pair]);
+    const bool mine = lane < L;
+    // inputs of this lane's contact, and whether the whole segment takes the group form
+    float pen = 0.0f;
+    f3 pa = mk(0.0f, 0.0f, 0.0f), nr = mk(0.0f, 0.0f, 1.0f);
+    if (mine) {
+        pen = raw.pen[s0 + lane];
+        pa = ld3(raw.pA, s0 + lane);
+        nr = ld3(raw.n, s0 + lane);
+    }
+    bool fin = isfinite(pen) && isfinite(pa.x) && isfinite(pa.y) && isfinite(pa.z) && isfinite(nr.x) &&
+               isfinite(nr.y) && isfinite(nr.z);
+    const unsigned long long gmask = (BP_G == 64 ? ~0ull : ((1ull << BP_G) - 1ull)) << (threadIdx.x & 63 & ~(BP_G - 1));
+    const bool group = L <= BP_G && (__ballot(!fin) & gmask) == 0ull;
+    if (!group) {
+        if (lane == 0) {
+            bead_warm_serial(s0, L, wn, wkey, wjn, wjt1, wjt2, raw);
+            int k = bead_select(s0, L, mx, ref, raw);
+            if (k < 0) {
+                atomicOr(&istat[IS_OVF], OVF_CONTACTS);
+                k = L;
+            }
+            kcnt[pair] = k;
+        }
+        return;
+    }
+    // A14 warm start: the first kept key >= the pair's smallest key (one search), then each lane
+    // walks forward from it to its own key
+    double jnd = 0.0;
+    float jt1 = 0.0f, jt2 = 0.0f;
+    if (wn > 0) {
+        int j0 = 0;
+        if (lane == 0) {
+            int64_t k0 = raw.key[s0];
+            int lo = 0, hi = wn;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (wkey[mid] < k0) lo = mid + 1;
+                else hi = mid;
+            }
+            j0 = lo;
+        }
+        j0 = __shfl(j0, 0, BP_G);
+        if (mine) {
+            int64_t k = raw.key[s0 + lane];
+            int j = j0;
+            while (j < wn && wkey[j] < k) ++j;
+            if (j < wn && wkey[j] == k) { jnd = (double)wjn[j]; jt1 = wjt1[j]; jt2 = wjt2[j]; }
+        }
+    }
+    if (mine) { raw.jnd[s0 + lane] = jnd; raw.jt1[s0 + lane] = jt1; raw.jt2[s0 + lane] = jt2; }
+    if (L <= mx) {
+        if (mine) { raw.keep[s0 + lane] = 1; raw.jnr[s0 + lane] = jnd; }
+        if (lane == 0) kcnt[pair] = L;
+        return;
+    }
+    if (mx > ME_BEAD_MAXKEEP) {  // bead_select's fixed-size refusal
+        if (mine) { raw.keep[s0 + lane] = 1; raw.jnr[s0 + lane] = jnd; }
+        if (lane == 0) { atomicOr(&istat[IS_OVF], OVF_CONTACTS); kcnt[pair] = L; }
+        return;
+    }
+    // deepest contact, first index on ties
+    float bp = mine ? pen : -INFINITY;
+    int bt = mine ? lane : BP_G;
+#pragma unroll
+    for (int o = BP_G / 2; o > 0; o >>= 1) {
+        float op = __shfl_xor(bp, o, BP_G);
+        int ot = __shfl_xor(bt, o, BP_G);
+        if (op > bp || (op == bp && ot < bt)) { bp = op; bt = ot; }
+    }
+    const int deep = bt;
+    // plane basis from the deepest contact's normal (float64), as bead_select
+    f3 nd = mk(__shfl(nr.x, deep, BP_G), __shfl(nr.y, deep, BP_G), __shfl(nr.z, deep, BP_G));
+    double ax = 1.0, ay = 0.0, az = 0.0;
+    if (fabsf(nd.x) >= 0.9f) { ax = 0.0; ay = 1.0; az = 0.0; }
+    double n0 = (double)nd.x, n1 = (double)nd.y, n2 = (double)nd.z;
+    double dn = ax * n0 + ay * n1 + az * n2;
+    double u0 = ax - dn * n0, u1 = ay - dn * n1, u2 = az - dn * n2;
+    double ln = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    double u[3] = {u0 / ln, u1 / ln, u2 / ln};
+    double w[3] = {n1 * u[2] - n2 * u[1], n2 * u[0] - n0 * u[2], n0 * u[1] - n1 * u[0]};
+    double t0 = 0.0, t1 = 0.0;
+    bead_planar(pa, ref, u, w, t0, t1);
+    const bool pfin = !mine || (isfinite(t0) && isfinite(t1));
+    if ((__ballot(!pfin) & gmask) != 0ull) {  // a degenerate basis: the one-thread form decides
+        if (lane == 0) {
+            int k = bead_select(s0, L, mx, ref, raw);
+            if (k < 0) {
+                atomicOr(&istat[IS_OVF], OVF_CONTACTS);
+                k = L;
+            }
+            kcnt[pair] = k;
+        }
+        return;
+    }
+    double r0 = bp_shfl(t0, deep), r1 = bp_shfl(t1, deep);
+    double S00 = r0 * r0, S01 = r0 * r1, S10 = r1 * r0, S11 = r1 * r1;
+    unsigned long long taken = 1ull << deep;
+    int pick[ME_BEAD_MAXKEEP];
+    pick[0] = deep;
+    for (int k = 1; k < mx; ++k) {
+        const bool cand = mine && !((taken >> lane) & 1ull);
+        double a00 = S00 + t0 * t0, a01 = S01 + t0 * t1;
+        double a10 = S10 + t1 * t0, a11 = S11 + t1 * t1;
+        double tr = a00 + a11;
+        double det = a00 * a11 - a01 * a10;
+        double disc = tr * tr / 4.0 - det;
+        if (!(disc > 0.0)) disc = 0.0;
+        double lam = tr / 2.0 - sqrt(disc);
+        double kk = (double)(k + 1);
+        double sc = lam < kk ? lam : kk;
+        double bs = cand ? sc : -1.0e300;
+        int bi = cand ? lane : BP_G;
+#pragma unroll
+        for (int o = BP_G / 2; o > 0; o >>= 1) {
+            double os = __shfl_xor(bs, o, BP_G);
+            int oi = __shfl_xor(bi, o, BP_G);
+            if (os > bs || (os == bs && oi < bi)) { bs = os; bi = oi; }
+        }
+        pick[k] = bi;
+        taken |= 1ull << bi;
+        S00 = bp_shfl(a00, bi); S01 = bp_shfl(a01, bi); S10 = bp_shfl(a10, bi); S11 = bp_shfl(a11, bi);
+    }
+    if (mine) raw.keep[s0 + lane] = (int)((taken >> lane) & 1ull);
+    // nearest kept point in the plane (first index on ties), then each kept point sums its share of
+    // the warm normal impulses in contact order
+    double kx[ME_BEAD_MAXKEEP], ky[ME_BEAD_MAXKEEP];
+    for (int k = 0; k < mx; ++k) { kx[k] = bp_shfl(t0, pick[k]); ky[k] = bp_shfl(t1, pick[k]); }
+    int kbest = 0;
+    double dbest2 = 1.0e300;
+    for (int k = 0; k < mx; ++k) {
+        double e0 = t0 - kx[k], e1 = t1 - ky[k];
+        double d2 = e0 * e0 + e1 * e1;
+        if (d2 < dbest2) { dbest2 = d2; kbest = k; }
+    }
+    double acc = 0.0;
+    for (int t = 0; t < L; ++t) {
+        int kb = __shfl(kbest, t, BP_G);
+        double jd = bp_shfl(jnd, t);
+        if (lane < mx && kb == lane) acc = acc + jd;
+    }
+    if (lane < mx) raw.jnr[s0 + pick[lane]] = acc;
+    if (lane == 0) kcnt[pair] = mx;
+}
+
+// groups of BP_G lanes take the candidate pairs in turn (the grid is sized from the capacity; the
+// loop bound is the live count)
+__global__ void __launch_bounds__(ME_THREADS) k_bp_select(int cap, int *istat, const int *cand_a, const float *x,
+                                                          const int *cnt, const int *off, int capR, int mx, int warm,
+                                                          const int64_t *wkey, const float *wjn, const float *wjt1,
+                                                          const float *wjt2, BeadRaw raw, int *kcnt, int *pflag) {
+    int n = istat[IS_CAND];
+    if (n > cap) n = cap;
+    const int groups = gridDim.x * (blockDim.x / BP_G);
+    for (int pair = (blockIdx.x * blockDim.x + threadIdx.x) / BP_G; pair < n; pair += groups)  // uniform per group
+        bp_select_pair(pair, istat, cand_a, x, cnt, off, capR, mx, warm, wkey, wjn, wjt1, wjt2, raw, kcnt, pflag);
 }
 
 // compaction of the kept contacts, pair-major, and one pair per candidate pair with contacts

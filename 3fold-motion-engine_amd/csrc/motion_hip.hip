@@ -373,7 +373,9 @@ int generate_bead_pairs(me_engine *e, hipStream_t s) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_bp_emit, dim3(wg), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->cand_b, e->allp, N, P,
                        R, W, e->bp_mask, e->slot_off, e->capC, e->raw, e->istat);
-    hipLaunchKernelGGL(k_bp_select, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->b.x, e->S.n,
+    int sel_blocks = nblk((long long)cc * BP_G);
+    if (sel_blocks > 2048) sel_blocks = 2048;
+    hipLaunchKernelGGL(k_bp_select, dim3(sel_blocks), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->b.x, e->S.n,
                        e->slot_off, e->capC, ME_BEAD_KEEP, e->cfg.warm_start, e->wkey, e->wjn, e->wjt1, e->wjt2,
                        e->raw, e->bp_kn, e->pair_flag);
     rc = scan(e, e->bp_kn, e->bp_koff, e->istat + IS_CAND, 0, cc, e->istat + IS_C, s);
