@@ -56,7 +56,8 @@ class Rollout:
         self.T, self.dt, self.S, self.k = int(steps), float(dt), int(substeps), int(k)
         self.sdt = self.dt / self.S
         N = bodies.n
-        self.eng = ColorCacheContactEngine(vit=vit, pit=pit, n_colors=n_colors, tiled=True, tile_bodies=tile_bodies,
+        self.eng = ColorCacheContactEngine(vit=vit, pit=pit, n_colors=n_colors, tiled=True, graph_capture=False,
+                                           tile_bodies=tile_bodies,
                                            max_contacts=max_contacts or 8 * N, max_pairs=max_pairs,
                                            check_every_step=False, device=device)
         self.eng.add_boxes(bodies)

@@ -288,9 +288,8 @@ int validate(const me_config *c) {
     if ((long long)c->n_bodies * (c->n_beads > 0 ? c->n_beads : 1) > (1LL << 30))
         return fail(ME_EARG, "too many bodies/beads");
     if (!(c->margin >= 0.0f)) return fail(ME_EARG, "margin must be >= 0");
-    if ((c->flags & ME_FLAG_GRAPH) &&
-        (!(c->flags & ME_FLAG_TILED) || c->n_beads > 0 || c->solver == ME_SOLVER_JACOBI))
-        return fail(ME_EARG, "graph capture needs the tiled solve, box geometry and a coloured solver");
+    if ((c->flags & ME_FLAG_GRAPH) && (!(c->flags & ME_FLAG_TILED) || c->solver == ME_SOLVER_JACOBI))
+        return fail(ME_EARG, "graph capture needs the tiled solve and a coloured solver");
     return ME_OK;
 }
 

@@ -34,7 +34,7 @@ def test_workspace_size_and_argument_errors():
     bead_pc = lib_mod.Config(1, 10, 18, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 0, 256, lib_mod.FLAG_PER_CONTACT)
     assert L.me_workspace_size(ctypes.byref(bead_pc), ctypes.byref(n)) == lib_mod.ME_EARG
     assert b"per-contact" in L.me_last_error()
-    # whole-step graphs need the tiled coloured solve on box bodies
+    # whole-step graphs need the tiled coloured solve
     g_plain = lib_mod.Config(2, 100, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64, lib_mod.FLAG_GRAPH)
     assert L.me_workspace_size(ctypes.byref(g_plain), ctypes.byref(n)) == lib_mod.ME_EARG
     assert b"graph" in L.me_last_error()
@@ -44,3 +44,6 @@ def test_workspace_size_and_argument_errors():
     g_ok = lib_mod.Config(2, 100, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64,
                           lib_mod.FLAG_GRAPH | lib_mod.FLAG_TILED)
     assert L.me_workspace_size(ctypes.byref(g_ok), ctypes.byref(n)) == 0
+    g_bead = lib_mod.Config(2, 100, 18, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64,
+                            lib_mod.FLAG_GRAPH | lib_mod.FLAG_TILED)
+    assert L.me_workspace_size(ctypes.byref(g_bead), ctypes.byref(n)) == 0

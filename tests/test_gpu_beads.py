@@ -94,16 +94,19 @@ def test_bead_stack8_matches_oracle_and_archive_counts(oracle, kind, tiled):
     np.testing.assert_array_equal(e.contact_forces(), o.forces())
 
 
-@pytest.mark.parametrize("kind,tiled", [("pair_chunks", False), ("color_cache", True)])
-def test_bead_lattice1000_matches_oracle(oracle, kind, tiled):
-    _say(f"{kind} bead lattice1000 x40 tiled={tiled}")
+@pytest.mark.parametrize("kind,tiled,graph", [("pair_chunks", False, False), ("color_cache", True, True),
+                                              ("color_cache", True, False)])
+def test_bead_lattice1000_matches_oracle(oracle, kind, tiled, graph):
+    """ColorCache runs its whole step as a replayed hipGraph by default (graph=True), as the
+    reference's ColorCache captures its solve; graph=False is the eager launch sequence."""
+    _say(f"{kind} bead lattice1000 x40 tiled={tiled} graph={graph}")
     pkg = _pkg()
     meta = json.loads((GOLD / "reference_trajectories.json").read_text())
     counts = meta["records"][f"{kind}__lattice1000"]["contacts"]
     agree = {"pair_chunks": 40, "color_cache": 26}[kind]
     cen = pkg.scenes.lattice(1000)
     bs = pkg.bodies.bead_bodies(cen, [700.0] * 1000, [0] * 1000)
-    e = _bead_engine(kind, bs, [700.0] * 1000, tiled=tiled)
+    e = _bead_engine(kind, bs, [700.0] * 1000, tiled=tiled, graph_capture=graph)
     o = oracle.OracleWorld(bs, kind, vit=40, pit=10)
     for s in range(40):
         e.step(1 / 240)
@@ -113,6 +116,7 @@ def test_bead_lattice1000_matches_oracle(oracle, kind, tiled):
         if s < agree:
             assert e.n_contacts == counts[s], f"step {s}"
     _assert_bead_contacts_equal(e, o, "final")
+    assert e.n_graph_captures == (1 if graph else 0)
 
 
 def _jumble(seed=3):

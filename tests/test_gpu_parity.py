@@ -84,16 +84,17 @@ def test_box_stack16_matches_oracle(oracle, kind):
     np.testing.assert_array_equal(e.contact_forces(), o.forces())
 
 
-@pytest.mark.parametrize("kind,n_colors,per_contact,tiled", [
-    ("color_cache", 0, False, False), ("pair_chunks", 0, False, False), ("color_cache", 16, False, False),
-    ("pair_chunks", 0, True, False), ("color_cache", 16, False, True), ("color_cache", 0, False, True),
-    ("pair_chunks", 8, False, True), ("pair_chunks", 0, True, True)])
-def test_box_pile_matches_oracle(oracle, kind, n_colors, per_contact, tiled):
-    _say(f"{kind} box pile 400 x60 S2 V16 P10 colors={n_colors} per_contact={per_contact} tiled={tiled}")
+@pytest.mark.parametrize("kind,n_colors,per_contact,tiled,graph", [
+    ("color_cache", 0, False, False, False), ("pair_chunks", 0, False, False, False),
+    ("color_cache", 16, False, False, False), ("pair_chunks", 0, True, False, False),
+    ("color_cache", 16, False, True, True), ("color_cache", 16, False, True, False),
+    ("color_cache", 0, False, True, True), ("pair_chunks", 8, False, True, False), ("pair_chunks", 0, True, True, False)])
+def test_box_pile_matches_oracle(oracle, kind, n_colors, per_contact, tiled, graph):
+    _say(f"{kind} box pile 400 x60 S2 V16 P10 colors={n_colors} per_contact={per_contact} tiled={tiled} graph={graph}")
     pkg = _pkg()
     b = pkg.scenes.box_pile(400, seed=3)
     kw = dict(vit=16, pit=10, n_colors=n_colors)
-    ekw = dict(kw, tiled=tiled, tile_bodies=32)
+    ekw = dict(kw, tiled=tiled, tile_bodies=32, graph_capture=graph)
     e = _engine(kind, pair_chunks=not per_contact, **ekw) if kind == "pair_chunks" else _engine(kind, **ekw)
     e.add_boxes(b)
     o = oracle.OracleWorld(b, kind, per_contact=per_contact, **kw)
@@ -217,7 +218,7 @@ def test_graph_replay_matches_eager(oracle):
     b = pkg.scenes.concat([row, pkg.scenes.box_pile(200, seed=5, origin=(0.0, 3.0, 0.0))])
     kw = dict(vit=8, pit=4, n_colors=16, tiled=True, tile_bodies=64)
     eg = _engine("color_cache", graph_capture=True, **kw)
-    ee = _engine("color_cache", **kw)
+    ee = _engine("color_cache", graph_capture=False, **kw)
     eg.add_boxes(b)
     ee.add_boxes(b)
     o = oracle.OracleWorld(b, "color_cache", vit=8, pit=4, n_colors=16)
