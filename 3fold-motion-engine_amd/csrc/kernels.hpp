@@ -237,9 +237,12 @@ __global__ void k_grid_count(int n, const float *pos, const int *scene_of, int d
     atomicAdd(&bcount[bk], 1);
 }
 // items in bucket order; with spos, also each item's position and id in the same order (one 16-byte
-// load per visited item for the bead grid)
+// load per visited item for the bead grid); with scell / sbox (the body grid), each item's cell and
+// scene, and its AABB with its inverse mass and id (lo.xyz, invM | hi.xyz, id), so a lookup reads an
+// item's data at its bucket position in one round of independent loads
 __global__ void k_grid_scatter(int n, const int *bucket, const int *bstart, int *bcursor, int *items, const float *pos,
-                               float4 *spos) {
+                               float4 *spos, const int *cellc, const float *aabb, const float *invM, int4 *scell,
+                               float4 *sbox) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int bk = bucket[i];
@@ -248,6 +251,12 @@ __global__ void k_grid_scatter(int n, const int *bucket, const int *bstart, int 
     if (spos) {
         f3 p = ld3(pos, i);
         spos[at] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+    }
+    if (scell) {
+        scell[at] = make_int4(cellc[4 * i], cellc[4 * i + 1], cellc[4 * i + 2], cellc[4 * i + 3]);
+        const float *bx = aabb + 6 * (size_t)i;
+        sbox[2 * (size_t)at] = make_float4(bx[0], bx[1], bx[2], invM[i]);
+        sbox[2 * (size_t)at + 1] = make_float4(bx[3], bx[4], bx[5], __int_as_float(i));
     }
 }
 
@@ -267,9 +276,9 @@ __device__ __forceinline__ bool aabb_overlap(const float *a, const float *b, flo
 #define ME_CELLCAND 16
 #define ME_BP_PER_WG 8
 __global__ void __launch_bounds__(32 * ME_BP_PER_WG) k_box_pairs(int N, Bodies b, const float *aabb, const int *cellc,
-                                                                 const int *bstart, const int *bcount, const int *items,
-                                                                 int T, float margin, int all_pairs, int *cand_n,
-                                                                 int *cand_tmp, int *istat) {
+                                                                 const int *bstart, const int *bcount, const int4 *scell,
+                                                                 const float4 *sbox, int T, float margin, int all_pairs,
+                                                                 int *cand_n, int *cand_tmp, int *istat) {
     __shared__ int hits_all[ME_BP_PER_WG][ME_MAXCAND];
     const int half = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const int a = blockIdx.x * ME_BP_PER_WG + half;
@@ -287,11 +296,14 @@ __global__ void __launch_bounds__(32 * ME_BP_PER_WG) k_box_pairs(int N, Bodies b
         int bk = (int)(cell_hash(X, Y, Z, s) & (uint32_t)(T - 1));
         int st = bstart[bk], cnt = bcount[bk];
         for (int t = 0; t < cnt; ++t) {
-            int j = items[st + t];
+            int4 cj = scell[st + t];
+            float4 lo = sbox[2 * (size_t)(st + t)], hi = sbox[2 * (size_t)(st + t) + 1];
+            int j = __float_as_int(hi.w);
             if (j <= a) continue;
-            if (cellc[4 * j] != X || cellc[4 * j + 1] != Y || cellc[4 * j + 2] != Z || cellc[4 * j + 3] != s) continue;
-            if (!all_pairs && !dynA && !(b.invM[j] > 0.0f)) continue;
-            if (!aabb_overlap(A, aabb + 6 * j, m)) continue;
+            if (cj.x != X || cj.y != Y || cj.z != Z || cj.w != s) continue;
+            if (!all_pairs && !dynA && !(lo.w > 0.0f)) continue;
+            const float B6[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+            if (!aabb_overlap(A, B6, m)) continue;
             if (n < ME_CELLCAND) mine[n++] = j;
             else over = true;
         }

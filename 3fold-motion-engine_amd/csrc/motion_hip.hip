@@ -78,6 +78,8 @@ struct me_engine {
     // grid
     int *cellc, *bucket, *bcount, *bstart, *bcursor, *items;
     float4 *spos;  // bead grid: position and id of each item in bucket order
+    int4 *scell;   // body grid: cell + scene of each item in bucket order
+    float4 *sbox;  // body grid: AABB lo + invM, hi + id of each item in bucket order
     // candidates and narrow-phase slots
     int *cand_n, *cand_off, *cand_tmp, *cand_a, *cand_b;
     Slots S;
@@ -241,6 +243,8 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->bcursor = B.take<int>((size_t)e->T);
     e->items = B.take<int>((size_t)items);
     e->spos = B.take<float4>(point_grid ? (size_t)items : 1);
+    e->scell = B.take<int4>(point_grid ? 1 : (size_t)N);
+    e->sbox = B.take<float4>(point_grid ? 1 : 2 * (size_t)N);
     e->cand_n = B.take<int>((size_t)items);
     e->cand_off = B.take<int>((size_t)items);
     e->cand_tmp = B.take<int>(point_grid ? 1 : (size_t)N * ME_MAXCAND);
@@ -323,13 +327,14 @@ int read_stat(me_engine *e, hipStream_t s) {
 
 // grid over n items with positions pos; cell from IS_CELL (cell_fixed <= 0) or fixed
 int build_grid(me_engine *e, int n, const float *pos, const int *scene, int div, float cell_fixed, float4 *spos,
-               hipStream_t s) {
+               bool body_grid, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_count, dim3(nblk(n)), dim3(ME_THREADS), 0, s, n, pos, scene, div, e->istat, cell_fixed,
                        e->cfg.margin, e->T, e->cellc, e->bucket, e->bcount);
     int rc = scan(e, e->bcount, e->bstart, nullptr, e->T, e->T, nullptr, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n)), dim3(ME_THREADS), 0, s, n, e->bucket, e->bstart, e->bcursor,
-                       e->items, pos, spos);
+                       e->items, pos, spos, e->cellc, e->aabb, e->b.inv_mass, body_grid ? e->scell : nullptr,
+                       body_grid ? e->sbox : nullptr);
     LAUNCH_CHECK();
     return ME_OK;
 }
@@ -339,10 +344,10 @@ int build_grid(me_engine *e, int n, const float *pos, const int *scene, int div,
 int body_pairs(me_engine *e, int all_pairs, hipStream_t s) {
     Bodies b = bodies_of(e);
     int N = e->N;
-    int rc = build_grid(e, N, e->b.x, e->b.scene, 1, -1.0f, nullptr, s);
+    int rc = build_grid(e, N, e->b.x, e->b.scene, 1, -1.0f, nullptr, true, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_box_pairs, dim3((N + ME_BP_PER_WG - 1) / ME_BP_PER_WG), dim3(32 * ME_BP_PER_WG), 0, s, N, b,
-                       e->aabb, e->cellc, e->bstart, e->bcount, e->items,
+                       e->aabb, e->cellc, e->bstart, e->bcount, e->scell, e->sbox,
                        e->T, e->cfg.margin, all_pairs, e->cand_n, e->cand_tmp, e->istat);
     rc = scan(e, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
     if (rc) return rc;
@@ -399,7 +404,7 @@ int generate(me_engine *e, hipStream_t s) {
         int NP = e->NP, P = e->P;
         float R = e->cfg.bead_radius;
         hipLaunchKernelGGL(k_bead_points, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, N, P, b, e->allp);
-        int rc = build_grid(e, NP, e->allp, nullptr, 1, 2.0f * R, e->spos, s);
+        int rc = build_grid(e, NP, e->allp, nullptr, 1, 2.0f * R, e->spos, false, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_bead_contacts<false>, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, NP, P, e->allp, e->cellc,
                            e->bstart, e->bcount, e->spos, e->T, R, e->b.mu, e->cand_n, e->cand_off, e->capC, e->Cc,
