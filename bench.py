@@ -44,11 +44,11 @@ def algorithmic_bytes(pairs, invM, solver, V, P, S, N, C):
                 step=S * (pre + warm + V * vel + P * pos + integ))
 
 
-def cpu_baseline(x, q, v, w, bodies_set, solver, V, P, S, n_colors, budget_s):
+def cpu_baseline(x, q, v, w, bodies_set, solver, V, P, S, n_colors, budget_s, omp=False):
     from oracle import oracle as orc
     bs = bodies_set
     bs.x[:], bs.q[:], bs.v[:], bs.w[:] = x, q, v, w
-    o = orc.OracleWorld(bs, solver, vit=V, pit=P, n_colors=n_colors)
+    o = orc.OracleWorld(bs, solver, vit=V, pit=P, n_colors=n_colors, omp=omp)
     steps, secs = 0, 0.0
     while secs < budget_s and steps < 400:
         secs += o.time_steps(1 / 240, S, 2)
@@ -216,7 +216,7 @@ def main():
                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None,
                         "bytes_per_launch": int(per_launch[dom]), "avg_launch_us": round(avg_s * 1e6, 3),
                         "launches_timed": prof[dom][1]}
-    cpu = None
+    cpu = cpu_all = None
     if not a.no_cpu:
         steps_c, secs_c = cpu_baseline(x0, q0, v0, w0, bodies, a.solver, a.vit, a.pit, S,
                                        a.colors if a.solver != "jacobi" else 0, a.cpu_seconds)
@@ -225,6 +225,16 @@ def main():
                "sample": f"CPU oracle (C, -O2, contraction off, 1 thread) from the GPU state at the start "
                          f"of the timed region: {steps_c} steps of the same {a.bodies}-box pile in "
                          f"{secs_c:.1f} s on {os.cpu_count()} visible host cores"}
+        if a.solver != "jacobi":
+            threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            steps_m, secs_m = cpu_baseline(x0, q0, v0, w0, bodies, a.solver, a.vit, a.pit, S, a.colors,
+                                           a.cpu_seconds, omp=True)
+            cpu_all = {"value": round(a.bodies * steps_m / secs_m, 1), "unit": "body-steps/s", "cores": threads,
+                       "kind": "port",
+                       "sample": f"the same oracle built with OpenMP over the pairs of a colour ({threads} "
+                                 f"threads; contact generation, colouring and integration stay serial): "
+                                 f"{steps_m} steps in {secs_m:.1f} s"}
     out = {"metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box pile)",
@@ -233,7 +243,7 @@ def main():
                       "bodies": a.bodies, "solver": a.solver, "colors": a.colors, "substeps": S,
                       "vit": a.vit, "pit": a.pit, "tiled": bool(a.tiled), "graph": bool(a.graph),
                       "contacts": cnt["n_contacts"], "pairs": cnt["n_pairs"], "colors_used": cnt["n_colors"]},
-           "roofline": roofline, "cpu_baseline": cpu,
+           "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all,
            "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in prof.items()}}
     if eager_ms is not None:
         out["kernel_timing_window"] = {"mode": "eager, right after the timed graph replays", "steps": a.steps,

@@ -1346,16 +1346,30 @@ static int step_colored(or_world *w, double sdt64, int colorcache) {
         if (nc < 0) return 2;
         w->ncol = nc;
         group_by_color(w);
+        /* the pairs of one colour share no dynamic body: the OpenMP build (timing only) runs a
+           colour's pairs in parallel, with the same result as the serial loop */
         if (colorcache)
-            for (int ci = 0; ci < nc; ++ci)
+            for (int ci = 0; ci < nc; ++ci) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
                 for (int t = w->cstart[ci]; t < w->cstart[ci + 1]; ++t) apply_warm_pair(w, w->porder[t]);
+            }
         for (int it = 0; it < w->prm.vit; ++it)
-            for (int ci = 0; ci < nc; ++ci)
+            for (int ci = 0; ci < nc; ++ci) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
                 for (int t = w->cstart[ci]; t < w->cstart[ci + 1]; ++t)
                     chunk_vel(w, w->porder[t], colorcache, colorcache ? 1.0f : w->prm.t2d);
+            }
         for (int it = 0; it < w->prm.pit; ++it)
-            for (int ci = 0; ci < nc; ++ci)
+            for (int ci = 0; ci < nc; ++ci) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
                 for (int t = w->cstart[ci]; t < w->cstart[ci + 1]; ++t) chunk_pos(w, w->porder[t], sdt);
+            }
         if (w->prm.warm_start) warm_store(w);
     } else {
         w->Pp = 0; w->ncol = 0; w->wn = 0;

@@ -12,6 +12,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
 LIB64 = HERE / "_build" / "liboracle64.so"  # the same restatement in float64 (FD reference of the adjoint)
+LIBMP = HERE / "_build" / "liboracle_omp.so"  # float32 with OpenMP over a colour's pairs (timing only)
 
 JACOBI, PAIR_CHUNKS, COLOR_CACHE = 0, 1, 2
 SOLVERS = {"jacobi": JACOBI, "pair_chunks": PAIR_CHUNKS, "color_cache": COLOR_CACHE}
@@ -32,9 +33,10 @@ def build() -> Path:
 _libs = {}
 
 
-def lib(fp64=False):
-    if fp64 not in _libs:
-        path = LIB64 if fp64 else LIB
+def lib(fp64=False, omp=False):
+    key = (bool(fp64), bool(omp) and not fp64)
+    if key not in _libs:
+        path = LIB64 if fp64 else (LIBMP if omp else LIB)
         if not path.exists():
             build()
         L = C.CDLL(str(path))
@@ -70,8 +72,8 @@ def lib(fp64=False):
         L.or_jp_key.restype = C.c_int64
         L.or_time_steps.argtypes = [vp, C.c_double, C.c_int, C.c_int]
         L.or_time_steps.restype = C.c_double
-        _libs[fp64] = L
-    return _libs[fp64]
+        _libs[key] = L
+    return _libs[key]
 
 
 def _p(a):
@@ -87,10 +89,11 @@ class OracleWorld:
 
     def __init__(self, bodies, solver="jacobi", vit=40, pit=20, relax=0.2, t2d=1.0,
                  warm_start=True, R=0.05, margin=0.0, cap=None, n_colors=0, per_contact=False, fp64=False,
-                 manifold=None):
+                 manifold=None, omp=False):
         self.fp64 = bool(fp64)
+        self.omp = bool(omp) and not self.fp64
         self.dtype = np.float64 if fp64 else np.float32
-        L = lib(self.fp64)
+        L = lib(self.fp64, self.omp)
         _f32 = self._arr
         N = bodies.n
         P = int(len(bodies.rest))
@@ -123,42 +126,42 @@ class OracleWorld:
 
     def __del__(self):
         try:
-            lib(self.fp64).or_destroy(self.h)
+            lib(self.fp64, self.omp).or_destroy(self.h)
         except Exception:
             pass
 
     def step(self, dt, substeps=1):
-        rc = lib(self.fp64).or_step(self.h, float(dt), int(substeps))
+        rc = lib(self.fp64, self.omp).or_step(self.h, float(dt), int(substeps))
         if rc == 4:
             raise RuntimeError("oracle: contact capacity exceeded")
         if rc:
             raise RuntimeError(f"oracle: step failed ({rc})")
 
     def time_steps(self, dt, substeps, steps):
-        return lib(self.fp64).or_time_steps(self.h, float(dt), int(substeps), int(steps))
+        return lib(self.fp64, self.omp).or_time_steps(self.h, float(dt), int(substeps), int(steps))
 
     def state(self):
         N = self.N
         x = np.zeros((N, 3), self.dtype); q = np.zeros((N, 4), self.dtype)
         v = np.zeros((N, 3), self.dtype); w = np.zeros((N, 3), self.dtype)
-        lib(self.fp64).or_get_state(self.h, _p(x), _p(q), _p(v), _p(w))
+        lib(self.fp64, self.omp).or_get_state(self.h, _p(x), _p(q), _p(v), _p(w))
         return x, q, v, w
 
     def aux(self):
         N = self.N
         pv = np.zeros((N, 3), self.dtype); po = np.zeros((N, 3), self.dtype)
         I = np.zeros((N, 3, 3), self.dtype)
-        lib(self.fp64).or_get_aux(self.h, _p(pv), _p(po), _p(I))
+        lib(self.fp64, self.omp).or_get_aux(self.h, _p(pv), _p(po), _p(I))
         return pv, po, I
 
     def forces(self):
         F = np.zeros((self.N, 3), np.float64)
-        lib(self.fp64).or_contact_forces(self.h, _p(F))
+        lib(self.fp64, self.omp).or_contact_forces(self.h, _p(F))
         return F
 
     def counters(self):
         out = np.zeros(7, np.int64)
-        lib(self.fp64).or_counters(self.h, _p(out))
+        lib(self.fp64, self.omp).or_counters(self.h, _p(out))
         return dict(n_contacts=int(out[0]), n_pairs=int(out[1]), n_colors=int(out[2]),
                     cache_hits=int(out[3]), cache_misses=int(out[4]), overflow=int(out[5]),
                     n_generated=int(out[6]))
@@ -170,7 +173,7 @@ class OracleWorld:
         n = np.zeros((C_, 3), self.dtype); pen = np.zeros(C_, self.dtype)
         jn = np.zeros(C_, self.dtype); jt1 = np.zeros(C_, self.dtype)
         jt2 = np.zeros(C_, self.dtype); jp = np.zeros(C_, self.dtype)
-        lib(self.fp64).or_get_contacts(self.h, _p(key), _p(bi), _p(bj), _p(pA), _p(pB), _p(n), _p(pen),
+        lib(self.fp64, self.omp).or_get_contacts(self.h, _p(key), _p(bi), _p(bj), _p(pA), _p(pB), _p(n), _p(pen),
                               _p(jn), _p(jt1), _p(jt2), _p(jp))
         return dict(key=key, bi=bi, bj=bj, pA=pA, pB=pB, n=n, pen=pen, jn=jn, jt1=jt1, jt2=jt2,
                     jp=jp)
@@ -179,7 +182,7 @@ class OracleWorld:
         P = self.counters()["n_pairs"]
         a = np.zeros(P, np.int32); b = np.zeros(P, np.int32); s = np.zeros(P, np.int32)
         c = np.zeros(P, np.int32); col = np.zeros(P, np.int32)
-        lib(self.fp64).or_get_pairs(self.h, _p(a), _p(b), _p(s), _p(c), _p(col))
+        lib(self.fp64, self.omp).or_get_pairs(self.h, _p(a), _p(b), _p(s), _p(c), _p(col))
         return dict(pbi=a, pbj=b, pstart=s, pcount=c, color=col)
 
 
