@@ -13,7 +13,8 @@ enum {
     IS_NPAIRSORT, IS_NCOLJP, IS_NBIG, IS_CRAW, IS_JTOT, IS_COUNT = 32
 };
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,
-       OVF_TAPE = 32 /* a recorded (differentiated) substep met a tile beyond one small tile */ };
+       OVF_TAPE = 32 /* a recorded (differentiated) substep met a tile beyond one small tile */,
+       OVF_GIANT = 64 /* a grid barrier of the giant-tile sweep waited past its bound */ };
 
 #define ME_NCOLMAX 512
 #define ME_MAXCAND 32

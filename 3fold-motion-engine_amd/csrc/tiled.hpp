@@ -42,6 +42,9 @@ struct TiledBuffers {
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
     long long *bigw;             // big-tile work since the last reset: tiles, pairs, dynamic pair ends, contacts
     int *biglist;                // tiles beyond one small tile (IS_NBIG of them)
+    int *gcst;                   // giant tile: colour-class starts (ME_NCOLMAX + 1), class count at [ME_NCOLMAX + 1]
+    unsigned *gbar;              // giant tiles: grid-barrier counts of the sweep [0] and colouring [1]
+                                 // launches, and the colouring's round counters [2..4]
 };
 
 // Recording of one substep for its adjoint (all device pointers into one caller buffer; V, P =
@@ -120,6 +123,8 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.dbg = B.template take<long long>(16);
     t.bigw = B.template take<long long>(4);
     t.biglist = B.template take<int>(n);
+    t.gcst = B.template take<int>(ME_NCOLMAX + 2);
+    t.gbar = B.template take<unsigned>(8);
 }
 
 // IS_MISMATCH after k_t_mode: 0 = ColorCache hit, 1 = colour this substep, 2 = no contacts
@@ -571,7 +576,7 @@ __device__ __forceinline__ void t_color_tile(const TiledArgs &A, const TiledBuff
 template <int WG>
 // the big tiles (k_t_biglist), taken in turn by the launch's workgroups; small tiles go to
 // k_t_color_small, many workgroups per CU instead of one 128 KB-LDS workgroup per tile
-__global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T) {
+__global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T, int giant) {
     if (A.istat[IS_MISMATCH] != 1) return;
     extern __shared__ unsigned long long cdsm[];
     __shared__ int left, hist[ME_NCOLMAX];
@@ -583,7 +588,7 @@ __global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T) {
         if (np > 0) {
             if (nb <= T_COLB && np <= T_COLP)
                 t_color_tile<WG, true>(A, T, b0, nb, p0, np, cdsm, reinterpret_cast<int *>(cdsm + T_COLB), left, hist);
-            else
+            else if (!giant)  // else k_t_color_giant's grid colours it
                 t_color_tile<WG, false>(A, T, b0, nb, p0, np, A.claim, nullptr, left, hist);
         }
         __syncthreads();
@@ -1157,6 +1162,10 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
                            T);
     if (A.pev0 && hipEventRecord((hipEvent_t)A.pev0, s2) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(T_BIG_WG), dyn, s2, A, T);
+    if (A.N > T_BIG_B) {  // an island beyond one workgroup's LDS is possible: the grid sweep (after k_t_big)
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s2, reinterpret_cast<int *>(T.gbar), 1, 0);
+        hipLaunchKernelGGL(k_t_giant<SOLVER>, dim3(T_GIANT_GRID), dim3(T_BIG_WG), 0, s2, A, T);
+    }
     if (A.pev1 && hipEventRecord((hipEvent_t)A.pev1, s2) != hipSuccess) return ME_ERUNTIME;
     if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return ME_ERUNTIME;
     return ME_OK;
@@ -1213,7 +1222,13 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile > N ? ntile : N)), dim3(ME_THREADS), 0, s, ntile, A.istat, A.b.invM,
                        A.invIw, T);
     hipLaunchKernelGGL(k_t_color_small, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
-    hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(1024), T_COL_DYN, s, A, T);
+    const int giant = N > T_COLB;  // a tile beyond the LDS colouring limits: coloured by a grid of workgroups
+    hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(1024), T_COL_DYN, s, A, T,
+                       giant);
+    if (giant) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(ME_THREADS), 0, s, reinterpret_cast<int *>(T.gbar + 1), 4, 0);
+        hipLaunchKernelGGL(k_t_color_giant, dim3(T_GIANT_GRID), dim3(T_BIG_WG), 0, s, A, T);
+    }
     hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
     hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);
     return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;

@@ -159,9 +159,10 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     int b0, nb, p0, np;
     tile_bounds(T, t, b0, nb, p0, np);
     if (is_small_tile(nb, np)) return;
+    const bool lds = nb <= T_BIG_B && np <= T_BIG_P;
+    if (!lds) return;  // a giant tile: k_t_giant's grid sweeps it
     if (A.taped && threadIdx.x == 0) atomicOr(&A.istat[IS_OVF], OVF_TAPE);  // no recording for big tiles
     const int tid = threadIdx.x;
-    const bool lds = nb <= T_BIG_B && np <= T_BIG_P;
     // phase stamps of the largest big tile (>= 1000 bodies) of the launch
     if (tid == 0 && nb >= 1000 && atomicMax(&T.dbg[7], (long long)nb) <= nb) { T.dbg[0] = wall_clock64(); T.dbg[8] = np; }
     auto stamp = [&](int k) {
@@ -342,4 +343,246 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
     const int nbig = A.istat[IS_NBIG];
     for (int i = blockIdx.x; i < nbig; i += gridDim.x)
         big_tile<SOLVER>(A, T, T.biglist[i], dsm, cst, cfl, cfe, maxc, wdyn, wcon);
+}
+
+// ── giant tiles: an island beyond the LDS limits of one workgroup, swept by a grid of workgroups ──
+// Each colour step spreads its pairs over every workgroup of the grid; a grid-wide barrier separates
+// the colour steps (a colour's pairs share no dynamic body, and the next colour reads what it wrote).
+// Body state, impulses and pair records live in global memory, as in big_tile's global mode; the
+// per-pair work (big_pair) and the colour order are the same, so the result is too. Workgroup 0
+// builds the colour classes and records first.
+#ifndef T_GIANT_GRID
+#define T_GIANT_GRID 64
+#endif
+
+// Grid barrier. `target` = (barriers so far in this launch) x G arrivals, the count being cleared
+// before the launch. Every wave first waits for its own stores to be acknowledged by its XCD's L2
+// (vmcnt 0); after the workgroup barrier, thread 0 writes that L2 back (agent-scope release), arrives,
+// waits, and invalidates the CU's L1 and the L2's copies of other XCDs' lines (agent-scope acquire)
+// before the workgroup reads anything. One release and one acquire per workgroup: a fence in every
+// wave costs about 8x more per barrier (32 workgroups: 12 us against 1.5 us).
+// The wait is bounded: a workgroup that is not yet resident delays it (the grid's other kernels
+// run to completion on their own), and past the bound the launch flags OVF_GIANT and winds down.
+__device__ __forceinline__ bool giant_barrier(unsigned *count, unsigned target, int *istat, int &stop) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // gfx9 encoding: vmcnt 0, expcnt and lgkmcnt not waited
+    __syncthreads();
+    if (threadIdx.x == 0 && !stop) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1u << 26)) {
+                atomicOr(&istat[IS_OVF], OVF_GIANT);
+                stop = 1;
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    return !stop;
+}
+
+// JP colouring of the tiles beyond the LDS colouring limits by a grid of workgroups: the rounds of
+// t_color_tile (claims by body id in the global claim array) with a grid barrier after each of a
+// round's three passes (reset, claim, win). The colours are those of the one-workgroup rounds: a
+// round's claims are the minimum key over each body's uncoloured pairs, whatever the order.
+// Uncoloured pairs per round are counted in gbar[2 + round % 3]: the counter of round r + 1 is
+// cleared in round r, after every workgroup has read that of round r - 2.
+__global__ void __launch_bounds__(T_BIG_WG) k_t_color_giant(TiledArgs A, TiledBuffers T) {
+    if (A.istat[IS_MISMATCH] != 1) return;
+    __shared__ int stop, wleft, hist[ME_NCOLMAX];
+    const int tid = threadIdx.x, G = gridDim.x, gt = blockIdx.x * T_BIG_WG + tid, stride = G * T_BIG_WG;
+    if (tid == 0) stop = 0;
+    __syncthreads();
+    unsigned target = 0;
+    unsigned *gb = T.gbar + 1, *gleft = T.gbar + 2;
+    const float *invM = A.b.invM;
+    int *gcol = A.Pp.col;
+    unsigned long long *claim = A.claim;
+    const int nbig = A.istat[IS_NBIG];
+    for (int i = 0; i < nbig; ++i) {
+        const int t = T.biglist[i];
+        const int b0 = T.tb0[t], nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - p0;
+        if (np == 0 || (nb <= T_COLB && np <= T_COLP)) continue;  // k_t_color's
+        for (int k = gt; k < np; k += stride) gcol[T.psorted[p0 + k]] = -1;
+        int nround = 0;
+        for (int cur = 0; cur <= np; ++cur) {
+            for (int j = gt; j < nb; j += stride) claim[T.bsorted[b0 + j]] = ~0ull;
+            if (gt == 0) __hip_atomic_store(&gleft[(cur + 1) % 3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0) wleft = 0;
+            target += G;
+            if (!giant_barrier(gb, target, A.istat, stop)) return;
+            for (int k = gt; k < np; k += stride) {
+                int p = T.psorted[p0 + k];
+                if (gcol[p] >= 0) continue;
+                unsigned long long key = (unsigned long long)jp_key(p);
+                int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
+                if (invM[a] > 0.0f) atomicMin(&claim[a], key);
+                if (b >= 0 && invM[b] > 0.0f) atomicMin(&claim[b], key);
+            }
+            target += G;
+            if (!giant_barrier(gb, target, A.istat, stop)) return;
+            int nl = 0;
+            for (int k = gt; k < np; k += stride) {
+                int p = T.psorted[p0 + k];
+                if (gcol[p] >= 0) continue;
+                unsigned long long key = (unsigned long long)jp_key(p);
+                int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
+                bool ok = true;
+                if (invM[a] > 0.0f && claim[a] != key) ok = false;
+                if (b >= 0 && invM[b] > 0.0f && claim[b] != key) ok = false;
+                if (ok) gcol[p] = cur;
+                else ++nl;
+            }
+            if (nl) atomicAdd(&wleft, nl);
+            __syncthreads();
+            if (tid == 0 && wleft) atomicAdd(&gleft[cur % 3], (unsigned)wleft);
+            target += G;
+            if (!giant_barrier(gb, target, A.istat, stop)) return;
+            if (__hip_atomic_load(&gleft[cur % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                nround = cur + 1;
+                break;
+            }
+        }
+        target += G;  // every workgroup has read the last round's counter
+        if (!giant_barrier(gb, target, A.istat, stop)) return;
+        if (gt < 3) __hip_atomic_store(&gleft[gt], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // class sizes: a histogram in LDS, then one global add per class and workgroup
+        for (int c = tid; c < nround && c < ME_NCOLMAX; c += T_BIG_WG) hist[c] = 0;
+        __syncthreads();
+        bool ovf = false;
+        for (int k = gt; k < np; k += stride) {
+            int c = gcol[T.psorted[p0 + k]];
+            if (c >= 0 && c < ME_NCOLMAX) atomicAdd(&hist[c], 1);
+            else ovf = true;
+        }
+        __syncthreads();
+        for (int c = tid; c < nround && c < ME_NCOLMAX; c += T_BIG_WG)
+            if (hist[c]) atomicAdd(&T.ccount_jp[c], hist[c]);
+        if (ovf) atomicOr(&A.istat[IS_OVF], OVF_COLORS);
+        if (gt == 0) atomicMax(&A.istat[IS_NCOLJP], nround);
+        target += G;  // the next tile's round counters start from zero
+        if (!giant_barrier(gb, target, A.istat, stop)) return;
+    }
+}
+
+template <int SOLVER>
+__global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers T) {
+    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
+    __shared__ int maxc, wdyn, wcon, stop;
+    constexpr bool FIXED = SOLVER == ME_SOLVER_COLOR_CACHE;
+    const int tid = threadIdx.x, G = gridDim.x;
+    if (tid == 0) stop = 0;
+    __syncthreads();
+    unsigned target = 0;
+    const float t2d = FIXED ? 1.0f : A.t2d;
+    const int nbig = A.istat[IS_NBIG];
+    for (int i = 0; i < nbig; ++i) {
+        const int t = T.biglist[i];
+        if (T.tb0[t] == INT_MAX) continue;
+        int b0, nb, p0, np;
+        tile_bounds(T, t, b0, nb, p0, np);
+        if (is_small_tile(nb, np) || (nb <= T_BIG_B && np <= T_BIG_P)) continue;  // k_t_small's or k_t_big's
+        if (blockIdx.x == 0) {
+            // colour classes and colour-ordered pair records, as big_tile's global mode
+            if (A.taped && tid == 0) atomicOr(&A.istat[IS_OVF], OVF_TAPE);
+            if (tid == 0 && atomicMax(&T.dbg[7], (long long)nb) <= nb) { T.dbg[0] = wall_clock64(); T.dbg[8] = np; }
+            if (tid == 0) {
+                atomicAdd(&A.istat[IS_TILES], 1);
+                atomicMax(&A.istat[IS_MAXTB], nb);
+                maxc = 0;
+                wdyn = 0;
+                wcon = 0;
+            }
+            int ncol = A.istat[IS_NCOL];
+            if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
+            for (int c = tid; c <= ncol; c += T_BIG_WG) { cst[c] = 0; cfl[c] = 0; cfe[c] = 0; }
+            __syncthreads();
+            for (int k = tid; k < np; k += T_BIG_WG) {
+                int c = A.Pp.col[T.psorted[p0 + k]];
+                if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+                atomicAdd(&cst[c], 1);
+                atomicMax(&maxc, c + 1);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int run = 0;
+                for (int c = 0; c < maxc; ++c) { int n = cst[c]; cst[c] = run; run += n; }
+                cst[maxc] = run;
+            }
+            __syncthreads();
+            const float *invM = A.b.invM;
+            int mdyn = 0, mcon = 0;
+            for (int k = tid; k < np; k += T_BIG_WG) {
+                int p = T.psorted[p0 + k];
+                int c = A.Pp.col[p];
+                if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+                int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
+                bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
+                int pos = (dA && dB) ? cst[c] + atomicAdd(&cfl[c], 1) : cst[c + 1] - 1 - atomicAdd(&cfe[c], 1);
+                int *r = T.rec + 8 * (size_t)(p0 + pos);
+                int st = A.Pp.pstart[p], cnt = A.Pp.pcount[p];
+                mdyn += (int)dA + (int)dB;
+                mcon += cnt;
+                r[0] = p; r[1] = a; r[2] = b; r[3] = a; r[4] = b; r[5] = st; r[6] = cnt;
+                r[7] = (dA ? 1 : 0) | (dB ? 2 : 0);
+            }
+            if (mdyn) atomicAdd(&wdyn, mdyn);
+            if (mcon) atomicAdd(&wcon, mcon);
+            __syncthreads();
+            for (int c = tid; c <= maxc; c += T_BIG_WG) T.gcst[c] = cst[c];
+            if (tid == 0) {
+                T.gcst[ME_NCOLMAX + 1] = maxc;
+                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[0]), 1ull);
+                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[1]), (unsigned long long)np);
+                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[2]), (unsigned long long)wdyn);
+                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[3]), (unsigned long long)wcon);
+            }
+        }
+        target += G;
+        if (!giant_barrier(T.gbar, target, A.istat, stop)) return;
+        const int nc = T.gcst[ME_NCOLMAX + 1];
+        // a colour's pairs in G contiguous runs of whole waves, one run per workgroup
+        auto sweep = [&](int pass) -> bool {
+            for (int c = 0; c < nc; ++c) {
+                const int c0 = T.gcst[c], c1 = T.gcst[c + 1];
+                const int run = ((c1 - c0 + G - 1) / G + 63) & ~63;
+                const int r0 = c0 + blockIdx.x * run, r1 = min(r0 + run, c1);
+                for (int k = r0 + tid; k < r1; k += T_BIG_WG) {
+                    const int q = p0 + k;
+                    const int *r = T.rec + 8 * (size_t)q;
+                    if (pass == 0) big_pair<0, FIXED, false>(A, T, 0u, r, q, b0, A.b.v, A.b.w, t2d);
+                    else if (pass == 1) big_pair<1, FIXED, false>(A, T, 0u, r, q, b0, A.b.v, A.b.w, t2d);
+                    else big_pair<2, FIXED, false>(A, T, 0u, r, q, b0, A.pv, A.po, t2d);
+                }
+                target += G;
+                if (!giant_barrier(T.gbar, target, A.istat, stop)) return false;
+            }
+            return true;
+        };
+        if (np > 0) {
+            if (FIXED && !sweep(0)) return;
+            for (int it = 0; it < A.vit; ++it)
+                if (!sweep(1)) return;
+            if (blockIdx.x == 0 && tid == 0 &&
+                (long long)atomicAdd(reinterpret_cast<unsigned long long *>(&T.dbg[7]), 0ull) == nb)
+                T.dbg[4] = wall_clock64();
+            for (int it = 0; it < A.pit; ++it)  // pv, po start at zero (k_body_pre)
+                if (!sweep(2)) return;
+        }
+        for (int j = blockIdx.x * T_BIG_WG + tid; j < nb; j += G * T_BIG_WG) {  // k_integ of the tile's bodies
+            int b = T.bsorted[b0 + j];
+            if (A.b.invM[b] > 0.0f)
+                integrate_body(A.b.x, A.b.q, ld3(A.b.v, b), ld3(A.b.w, b), ld3(A.pv, b), ld3(A.po, b), b, A.dt);
+        }
+        target += G;  // the next giant tile's records reuse gcst
+        if (!giant_barrier(T.gbar, target, A.istat, stop)) return;
+        if (blockIdx.x == 0 && tid == 0 &&
+            (long long)atomicAdd(reinterpret_cast<unsigned long long *>(&T.dbg[7]), 0ull) == nb) {
+            T.dbg[1] = T.dbg[2] = T.dbg[3] = T.dbg[0];
+            T.dbg[5] = T.dbg[6] = wall_clock64();
+        }
+    }
 }

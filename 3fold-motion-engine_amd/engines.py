@@ -236,6 +236,9 @@ class _HipContactEngine:
             what = [n for bit, n in ((1, "contacts"), (2, "body pairs"), (4, "broad-phase candidates"),
                                      (8, "colours"), (16, "tile"), (32, "recorded tile for the adjoint"))
                     if ovf & bit]
+            if ovf & 64:
+                raise RuntimeError(f"{s.name}.step: the grid sweep of an island beyond one workgroup timed out "
+                                   f"waiting for its workgroups; the step's state is not valid")
             raise RuntimeError(f"{s.name}.step: capacity exceeded ({', '.join(what)}); raise max_pairs / "
                                f"max_contacts")
         return s._counters

@@ -153,12 +153,16 @@ def test_colouring_is_proper_and_runs_repeat():
         assert np.array_equal(a, b2)
 
 
-@pytest.mark.parametrize("tile_bodies,n,steps", [(8, 600, 30), (64, 600, 30), (256, 600, 30), (128, 6600, 6)])
-def test_tiled_chain_island_beyond_lds(oracle, tile_bodies, n, steps):
+@pytest.mark.parametrize("kind,tile_bodies,n,steps", [("color_cache", 8, 600, 30), ("color_cache", 64, 600, 30),
+                                                      ("color_cache", 256, 600, 30), ("color_cache", 128, 6600, 6),
+                                                      ("pair_chunks", 128, 8400, 6)])
+def test_tiled_chain_island_beyond_lds(oracle, kind, tile_bodies, n, steps):
     """A row of touching cubes is one island: 600 bodies exceed a small tile (big-tile kernel, LDS
-    state), 6600 bodies exceed the big tile's LDS capacity (global-memory state); small tiles make
-    islands straddle tile boundaries."""
-    _say(f"color_cache tiled chain {n} + pile 200, tile_bodies={tile_bodies}")
+    state), 6600 bodies exceed the big tile's LDS capacity (the grid of workgroups sweeping one
+    island in global memory, a grid barrier per colour step), 8400 bodies also exceed the LDS
+    colouring limits (JP rounds by the grid, recoloured every substep under pair chunks); small
+    tiles make islands straddle tile boundaries."""
+    _say(f"{kind} tiled chain {n} + pile 200, tile_bodies={tile_bodies}")
     pkg = _pkg()
     c = np.zeros((n, 3))
     c[:, 0] = np.arange(n) * 0.0995
@@ -166,15 +170,49 @@ def test_tiled_chain_island_beyond_lds(oracle, tile_bodies, n, steps):
     row = pkg.bodies.box_bodies(c, [0.05, 0.05, 0.05], 1000.0, mu=0.6)
     pile = pkg.scenes.box_pile(200, seed=5, origin=(0.0, 3.0, 0.0))
     b = pkg.scenes.concat([row, pile])
-    e = _engine("color_cache", vit=8, pit=4, n_colors=16, tiled=True, tile_bodies=tile_bodies)
+    e = _engine(kind, vit=8, pit=4, n_colors=16, tiled=True, tile_bodies=tile_bodies)
     e.add_boxes(b)
-    o = oracle.OracleWorld(b, "color_cache", vit=8, pit=4, n_colors=16)
+    o = oracle.OracleWorld(b, kind, vit=8, pit=4, n_colors=16)
     for s in range(steps):
         e.step(1 / 240, 2)
         o.step(1 / 240, 2)
         _assert_state_equal(e, o, f"step {s}")
     assert e._counters["max_tile_bodies"] >= n
     _assert_contacts_equal(e, o, "final")
+
+
+def test_two_giant_islands_graph_replay_matches_eager_and_oracle(oracle):
+    """Rows of 8400 and 5200 touching cubes: two islands beyond one workgroup's LDS, swept one after
+    the other by the same grid launch (colour classes rebuilt per island), the first also coloured
+    by the grid, next to a pile of small islands. The whole step replayed as a hipGraph (the grid
+    barrier and round counts are cleared by fill kernels inside the graph) equals the eager engine
+    after every replay and the oracle at the end."""
+    _say("color_cache tiled chains of 8400 and 5200 + pile 300 x5: graph vs eager vs oracle")
+    pkg = _pkg()
+    rows = []
+    for k, n in enumerate((8400, 5200)):
+        c = np.zeros((n, 3))
+        c[:, 0] = np.arange(n) * 0.0995
+        c[:, 1] = 1.5 * k
+        c[:, 2] = 0.0499
+        rows.append(pkg.bodies.box_bodies(c, [0.05, 0.05, 0.05], 1000.0, mu=0.6))
+    b = pkg.scenes.concat(rows + [pkg.scenes.box_pile(300, seed=9, origin=(0.0, 4.0, 0.0))])
+    kw = dict(vit=8, pit=4, n_colors=16, tiled=True, tile_bodies=128)
+    eg = _engine("color_cache", graph_capture=True, **kw)
+    ee = _engine("color_cache", graph_capture=False, **kw)
+    eg.add_boxes(b)
+    ee.add_boxes(b)
+    o = oracle.OracleWorld(b, "color_cache", vit=8, pit=4, n_colors=16)
+    for s in range(5):
+        eg.step(1 / 240, 2)
+        ee.step(1 / 240, 2)
+        o.step(1 / 240, 2)
+        for name, a, r in zip("xqvw", eg.state_numpy(), ee.state_numpy()):
+            assert np.array_equal(a, r), f"step {s}: graph replay differs from eager in {name}"
+    assert eg.n_graph_captures == 1
+    assert eg._counters["max_tile_bodies"] >= 8400
+    _assert_state_equal(eg, o, "final")
+    _assert_contacts_equal(eg, o, "final")
 
 
 def test_batched_scenes_match_oracle_and_stay_independent(oracle):
