@@ -1116,6 +1116,7 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_color_small(TiledArgs A, TiledB
 // records of invIw for three 16-byte loads
 __global__ void k_t_biglist(int ntile, int *istat, const float *invM, const float *invIw, TiledBuffers T) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 5) T.gbar[t] = 0u;  // the giant-tile grid barriers and round counters of this substep start at zero
     if (t < istat[IS_NBODYSORT]) {
         int i = T.bsorted[t];
         T.invMs[t] = invM[i];
@@ -1162,10 +1163,8 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
                            T);
     if (A.pev0 && hipEventRecord((hipEvent_t)A.pev0, s2) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(T_BIG_WG), dyn, s2, A, T);
-    if (A.N > T_BIG_B) {  // an island beyond one workgroup's LDS is possible: the grid sweep (after k_t_big)
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s2, reinterpret_cast<int *>(T.gbar), 1, 0);
+    if (A.N > T_BIG_B)  // an island beyond one workgroup's LDS is possible: the grid sweep (after k_t_big)
         hipLaunchKernelGGL(k_t_giant<SOLVER>, dim3(T_GIANT_GRID), dim3(T_BIG_WG), 0, s2, A, T);
-    }
     if (A.pev1 && hipEventRecord((hipEvent_t)A.pev1, s2) != hipSuccess) return ME_ERUNTIME;
     if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return ME_ERUNTIME;
     return ME_OK;
@@ -1225,10 +1224,7 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     const int giant = N > T_COLB;  // a tile beyond the LDS colouring limits: coloured by a grid of workgroups
     hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(1024), T_COL_DYN, s, A, T,
                        giant);
-    if (giant) {
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(ME_THREADS), 0, s, reinterpret_cast<int *>(T.gbar + 1), 4, 0);
-        hipLaunchKernelGGL(k_t_color_giant, dim3(T_GIANT_GRID), dim3(T_BIG_WG), 0, s, A, T);
-    }
+    if (giant) hipLaunchKernelGGL(k_t_color_giant, dim3(T_GIANT_GRID), dim3(T_BIG_WG), 0, s, A, T);
     hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
     hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);
     return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;

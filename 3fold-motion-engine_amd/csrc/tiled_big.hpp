@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
 #endif
 
 // Grid barrier. `target` = (barriers so far in this launch) x G arrivals, the count being cleared
-// before the launch. Every wave first waits for its own stores to be acknowledged by its XCD's L2
+// before the launch (by k_t_biglist, every substep). Every wave first waits for its own stores to be acknowledged by its XCD's L2
 // (vmcnt 0); after the workgroup barrier, thread 0 writes that L2 back (agent-scope release), arrives,
 // waits, and invalidates the CU's L1 and the L2's copies of other XCDs' lines (agent-scope acquire)
 // before the workgroup reads anything. One release and one acquire per workgroup: a fence in every
