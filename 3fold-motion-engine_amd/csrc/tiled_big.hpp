@@ -5,7 +5,8 @@
 // contact constants and impulses as 96-byte records in global memory in sorted order, so a colour
 // step is one LDS read plus one round of 16-byte vector loads per thread. Body velocities
 // (velocity passes) and then pseudo-velocities (position passes) sit in LDS.
-// Global mode (larger tiles): the same sweeps with body state in global memory.
+// Larger tiles (giant islands): k_t_giant, the same sweeps by a grid of workgroups with body state in
+// global memory.
 #pragma once
 
 #define T_BIG_P 8192  // pairs of an LDS-mode big tile (32-bit word each)
@@ -50,12 +51,21 @@ __device__ __forceinline__ void ld_prep20(const float *o, CCon &C) {
     C.meff = q3.w; C.meft1 = q4.x; C.meft2 = q4.y; C.bias = q4.z; C.mu = q4.w;
 }
 
+// a giant tile's pair record (8 ints: p, a, b, a, b, first contact, count, dynamic flags) as loaded
+struct GRec {
+    int a, b, st, cnt, fl;
+};
+__device__ __forceinline__ GRec grec_load(const int *r) {
+    int4 x = *reinterpret_cast<const int4 *>(r), y = *reinterpret_cast<const int4 *>(r + 4);
+    return GRec{x.y, x.z, y.y, y.z, y.w};
+}
+
 // One pair of a big tile for one colour step, contacts streamed one at a time (few live registers).
 // PASS 0 warm apply, 1 velocity, 2 position. LDS mode: body state s0/s1 in LDS by tile slot, the
-// pair word and its colour-ordered records; global mode: record r (8 ints), body state in global
-// memory by body id, constants from prep, impulses from the contact arrays.
+// pair word and its colour-ordered records; global mode: record g, body state in global memory by
+// body id, constants from prep, impulses from the contact arrays.
 template <int PASS, bool FIXED, bool LDSM>
-__device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers &T, unsigned w, const int *r, int q,
+__device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers &T, unsigned w, GRec g, int q,
                                          int b0, float *s0, float *s1, float t2d) {
     int la, lb, cnt, a = -1, b = -1, st = 0;
     bool dynA, dynB, hasB;
@@ -63,8 +73,8 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
         la = w & 8191; lb = (w >> 13) & 8191; cnt = (int)((w >> 26) & 3) + 1;
         dynA = (w >> 28) & 1; dynB = (w >> 29) & 1; hasB = (w >> 30) & 1;
     } else {
-        a = r[1]; b = r[2]; la = a; lb = b; st = r[5]; cnt = r[6];
-        dynA = r[7] & 1; dynB = (r[7] & 2) != 0; hasB = b >= 0;
+        a = g.a; b = g.b; la = a; lb = b; st = g.st; cnt = g.cnt;
+        dynA = g.fl & 1; dynB = (g.fl & 2) != 0; hasB = b >= 0;
     }
     const float *gs0 = PASS == 2 ? A.pv : A.b.v, *gs1 = PASS == 2 ? A.po : A.b.w;
     f3 z = mk(0.0f, 0.0f, 0.0f);
@@ -145,12 +155,13 @@ __device__ __forceinline__ void big_sweeps_stream(const TiledArgs &A, const Tile
     for (int it = 0; it < reps; ++it)
         for (int c = 0; c < nc; ++c) {
             for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIG_WG)
-                big_pair<PASS, FIXED, true>(A, T, pword[k], nullptr, p0 + k, b0, s0, s1, t2d);
+                big_pair<PASS, FIXED, true>(A, T, pword[k], GRec{}, p0 + k, b0, s0, s1, t2d);
             lds_barrier();
         }
 }
 
-// one big tile t by the whole workgroup (LDS: dsm dynamic, the colour tables and scalars static)
+// one big tile t by the whole workgroup (LDS: dsm dynamic, the colour tables and scalars static);
+// tiles beyond T_BIG_B bodies or T_BIG_P pairs are left to k_t_giant
 template <int SOLVER>
 __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers &T, int t, float *dsm, int *cst,
                                          int *cfl, int *cfe, int &maxc, int &wdyn, int &wcon) {
@@ -158,9 +169,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     if (T.tb0[t] == INT_MAX) return;
     int b0, nb, p0, np;
     tile_bounds(T, t, b0, nb, p0, np);
-    if (is_small_tile(nb, np)) return;
-    const bool lds = nb <= T_BIG_B && np <= T_BIG_P;
-    if (!lds) return;  // a giant tile: k_t_giant's grid sweeps it
+    if (is_small_tile(nb, np) || !(nb <= T_BIG_B && np <= T_BIG_P)) return;
     if (A.taped && threadIdx.x == 0) atomicOr(&A.istat[IS_OVF], OVF_TAPE);  // no recording for big tiles
     const int tid = threadIdx.x;
     // phase stamps of the largest big tile (>= 1000 bodies) of the launch
@@ -178,18 +187,17 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     }
     int ncol = A.istat[IS_NCOL];
     if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
-    // LDS-mode body state (velocities, then pseudo-velocities) by tile slot; kept as LDS pointers so
-    // every access compiles to ds_* instructions rather than generic (flat) ones
+    // body state (velocities, then pseudo-velocities) by tile slot; kept as LDS pointers so every
+    // access compiles to ds_* instructions rather than generic (flat) ones
     float *lv = dsm, *lw = dsm + 3 * T_BIG_B;
-    if (lds)
-        for (int j = tid; j < nb; j += T_BIG_WG) {
-            int i = T.bsorted[b0 + j];
+    for (int j = tid; j < nb; j += T_BIG_WG) {
+        int i = T.bsorted[b0 + j];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                lv[3 * j + k] = A.b.v[3 * i + k];
-                lw[3 * j + k] = A.b.w[3 * i + k];
-            }
+        for (int k = 0; k < 3; ++k) {
+            lv[3 * j + k] = A.b.v[3 * i + k];
+            lw[3 * j + k] = A.b.w[3 * i + k];
         }
+    }
     for (int c = tid; c <= ncol; c += T_BIG_WG) { cst[c] = 0; cfl[c] = 0; cfe[c] = 0; }
     __syncthreads();
     stamp(1);
@@ -222,26 +230,17 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         int st = A.Pp.pstart[p], cnt = A.Pp.pcount[p];
         mdyn += (int)dA + (int)dB;
         mcon += cnt;
-        if (lds) {
-            int la = dA ? T.bslot[a] - b0 : 0, lb = dB ? T.bslot[b] - b0 : 0;
-            pword[pos] = pw_make(la, lb, cnt, dA, dB, b >= 0);
-            int *sr = T.srec + 4 * (size_t)q;
-            sr[0] = p; sr[1] = a; sr[2] = b; sr[3] = st;
-            float4 *o = reinterpret_cast<float4 *>(T.sprep + 96 * (size_t)q);
-            for (int i = 0; i < cnt; ++i) {
-                const float4 *s4 = reinterpret_cast<const float4 *>(T.prep + 20 * (size_t)(st + i));
-                float4 v0 = s4[0], v1 = s4[1], v2 = s4[2], v3 = s4[3], v4 = s4[4];
-                float4 v5 = make_float4(A.Cc.jn[st + i], A.Cc.jt1[st + i], A.Cc.jt2[st + i], A.Cc.jp[st + i]);
-                float4 *d = o + 6 * i;
-                d[0] = v0; d[1] = v1; d[2] = v2; d[3] = v3; d[4] = v4; d[5] = v5;
-            }
-        } else {
-            int *r = T.rec + 8 * (size_t)q;
-            r[0] = p; r[1] = a; r[2] = b;
-            r[3] = a;
-            r[4] = b;
-            r[5] = st; r[6] = cnt;
-            r[7] = (dA ? 1 : 0) | (dB ? 2 : 0);
+        int la = dA ? T.bslot[a] - b0 : 0, lb = dB ? T.bslot[b] - b0 : 0;
+        pword[pos] = pw_make(la, lb, cnt, dA, dB, b >= 0);
+        int *sr = T.srec + 4 * (size_t)q;
+        sr[0] = p; sr[1] = a; sr[2] = b; sr[3] = st;
+        float4 *o = reinterpret_cast<float4 *>(T.sprep + 96 * (size_t)q);
+        for (int i = 0; i < cnt; ++i) {
+            const float4 *s4 = reinterpret_cast<const float4 *>(T.prep + 20 * (size_t)(st + i));
+            float4 v0 = s4[0], v1 = s4[1], v2 = s4[2], v3 = s4[3], v4 = s4[4];
+            float4 v5 = make_float4(A.Cc.jn[st + i], A.Cc.jt1[st + i], A.Cc.jt2[st + i], A.Cc.jp[st + i]);
+            float4 *d = o + 6 * i;
+            d[0] = v0; d[1] = v1; d[2] = v2; d[3] = v3; d[4] = v4; d[5] = v5;
         }
     }
     if (mdyn) atomicAdd(&wdyn, mdyn);
@@ -256,75 +255,44 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     stamp(3);
     const int nc = maxc;
     const float t2d = FIXED ? 1.0f : A.t2d;
-    // global mode: body state in global memory, a full barrier per colour step
-    auto sweep = [&](int pass) {
-        for (int c = 0; c < nc; ++c) {
-            for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIG_WG) {
-                int q = p0 + k;
-                const int *r = T.rec + 8 * (size_t)q;
-                if (pass == 0) big_pair<0, FIXED, false>(A, T, 0u, r, q, b0, A.b.v, A.b.w, t2d);
-                else if (pass == 1) big_pair<1, FIXED, false>(A, T, 0u, r, q, b0, A.b.v, A.b.w, t2d);
-                else big_pair<2, FIXED, false>(A, T, 0u, r, q, b0, A.pv, A.po, t2d);
-            }
-            __syncthreads();
-        }
-    };
     if (np > 0) {
-        if (lds) {
-            if (FIXED) big_sweeps_stream<0, FIXED>(A, T, cst, pword, nc, 1, p0, b0, lv, lw, t2d);
-            big_sweeps_stream<1, FIXED>(A, T, cst, pword, nc, A.vit, p0, b0, lv, lw, t2d);
-        } else {
-            if (FIXED) sweep(0);
-            for (int it = 0; it < A.vit; ++it) sweep(1);
-        }
+        if (FIXED) big_sweeps_stream<0, FIXED>(A, T, cst, pword, nc, 1, p0, b0, lv, lw, t2d);
+        big_sweeps_stream<1, FIXED>(A, T, cst, pword, nc, A.vit, p0, b0, lv, lw, t2d);
     }
     __syncthreads();
     stamp(4);
     // velocities back to global memory; the LDS region now holds the pseudo-velocities
-    if (lds)
-        for (int j = tid; j < nb; j += T_BIG_WG) {
-            int i = T.bsorted[b0 + j];
-            st3(A.b.v, i, ld3(lv, j));
-            st3(A.b.w, i, ld3(lw, j));
-        }
-    __syncthreads();
-    if (lds)
-        for (int j = tid; j < nb; j += T_BIG_WG) {
-            st3(lv, j, mk(0.0f, 0.0f, 0.0f));
-            st3(lw, j, mk(0.0f, 0.0f, 0.0f));
-        }
-    __syncthreads();
-    if (np > 0) {
-        if (lds) big_sweeps_stream<2, FIXED>(A, T, cst, pword, nc, A.pit, p0, b0, lv, lw, t2d);
-        else
-            for (int it = 0; it < A.pit; ++it) sweep(2);
-    }
-    __syncthreads();
-    stamp(5);
-    if (lds)  // impulses back to the contact arrays
-        for (int k = tid; k < np; k += T_BIG_WG) {
-            int q = p0 + k;
-            int st = T.srec[4 * (size_t)q + 3];
-            int cnt = (int)((pword[k] >> 26) & 3) + 1;
-            const float *o = T.sprep + 96 * (size_t)q;
-            for (int i = 0; i < cnt; ++i) {
-                A.Cc.jn[st + i] = o[24 * i + 20];
-                A.Cc.jt1[st + i] = o[24 * i + 21];
-                A.Cc.jt2[st + i] = o[24 * i + 22];
-                A.Cc.jp[st + i] = o[24 * i + 23];
-            }
-        }
     for (int j = tid; j < nb; j += T_BIG_WG) {
         int i = T.bsorted[b0 + j];
-        f3 pv, po;
-        if (lds) {
-            pv = ld3(lv, j);
-            po = ld3(lw, j);
-        } else {
-            pv = ld3(A.pv, i);
-            po = ld3(A.po, i);
+        st3(A.b.v, i, ld3(lv, j));
+        st3(A.b.w, i, ld3(lw, j));
+    }
+    __syncthreads();
+    for (int j = tid; j < nb; j += T_BIG_WG) {
+        st3(lv, j, mk(0.0f, 0.0f, 0.0f));
+        st3(lw, j, mk(0.0f, 0.0f, 0.0f));
+    }
+    __syncthreads();
+    if (np > 0) big_sweeps_stream<2, FIXED>(A, T, cst, pword, nc, A.pit, p0, b0, lv, lw, t2d);
+    __syncthreads();
+    stamp(5);
+    for (int k = tid; k < np; k += T_BIG_WG) {  // impulses back to the contact arrays
+        int q = p0 + k;
+        int st = T.srec[4 * (size_t)q + 3];
+        int cnt = (int)((pword[k] >> 26) & 3) + 1;
+        const float *o = T.sprep + 96 * (size_t)q;
+        for (int i = 0; i < cnt; ++i) {
+            A.Cc.jn[st + i] = o[24 * i + 20];
+            A.Cc.jt1[st + i] = o[24 * i + 21];
+            A.Cc.jt2[st + i] = o[24 * i + 22];
+            A.Cc.jp[st + i] = o[24 * i + 23];
         }
-        if (lds) { st3(A.pv, i, pv); st3(A.po, i, po); }
+    }
+    for (int j = tid; j < nb; j += T_BIG_WG) {
+        int i = T.bsorted[b0 + j];
+        f3 pv = ld3(lv, j), po = ld3(lw, j);
+        st3(A.pv, i, pv);
+        st3(A.po, i, po);
         if (A.b.invM[i] > 0.0f) integrate_body(A.b.x, A.b.q, ld3(A.b.v, i), ld3(A.b.w, i), pv, po, i, A.dt);
     }
     __syncthreads();
@@ -544,7 +512,8 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
         target += G;
         if (!giant_barrier(T.gbar, target, A.istat, stop)) return;
         const int nc = T.gcst[ME_NCOLMAX + 1];
-        // a colour's pairs in G contiguous runs of whole waves, one run per workgroup
+        // a colour's pairs in G contiguous runs of whole waves, one run per workgroup (loading the next
+        // colour step's records before the barrier, which they do not depend on, measured 4 % slower)
         auto sweep = [&](int pass) -> bool {
             for (int c = 0; c < nc; ++c) {
                 const int c0 = T.gcst[c], c1 = T.gcst[c + 1];
@@ -552,10 +521,10 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
                 const int r0 = c0 + blockIdx.x * run, r1 = min(r0 + run, c1);
                 for (int k = r0 + tid; k < r1; k += T_BIG_WG) {
                     const int q = p0 + k;
-                    const int *r = T.rec + 8 * (size_t)q;
-                    if (pass == 0) big_pair<0, FIXED, false>(A, T, 0u, r, q, b0, A.b.v, A.b.w, t2d);
-                    else if (pass == 1) big_pair<1, FIXED, false>(A, T, 0u, r, q, b0, A.b.v, A.b.w, t2d);
-                    else big_pair<2, FIXED, false>(A, T, 0u, r, q, b0, A.pv, A.po, t2d);
+                    const GRec g = grec_load(T.rec + 8 * (size_t)q);
+                    if (pass == 0) big_pair<0, FIXED, false>(A, T, 0u, g, q, b0, A.b.v, A.b.w, t2d);
+                    else if (pass == 1) big_pair<1, FIXED, false>(A, T, 0u, g, q, b0, A.b.v, A.b.w, t2d);
+                    else big_pair<2, FIXED, false>(A, T, 0u, g, q, b0, A.pv, A.po, t2d);
                 }
                 target += G;
                 if (!giant_barrier(T.gbar, target, A.istat, stop)) return false;
