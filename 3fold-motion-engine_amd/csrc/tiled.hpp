@@ -42,6 +42,8 @@ struct TiledBuffers {
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
     long long *bigw;             // big-tile work since the last reset: tiles, pairs, dynamic pair ends, contacts
     int *biglist;                // tiles beyond one small tile (IS_NBIG of them)
+    int *giantlist;              // tiles beyond the big tile's LDS (IS_NGIANT of them): k_t_giant's
+    int *gcollist;               // tiles beyond the LDS colouring limits (IS_NGCOL): k_t_color_giant's
     int *gcst;                   // giant tile: colour-class starts (ME_NCOLMAX + 1), class count at [ME_NCOLMAX + 1]
     unsigned *gbar;              // giant tiles: grid-barrier counts of the sweep [0] and colouring [1]
                                  // launches, and the colouring's round counters [2..4]
@@ -123,6 +125,8 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.dbg = B.template take<long long>(16);
     t.bigw = B.template take<long long>(4);
     t.biglist = B.template take<int>(n);
+    t.giantlist = B.template take<int>(n);
+    t.gcollist = B.template take<int>(n);
     t.gcst = B.template take<int>(ME_NCOLMAX + 2);
     t.gbar = B.template take<unsigned>(8);
 }
@@ -142,6 +146,8 @@ __global__ void k_t_init(int N, TiledBuffers T, int *istat, int lds_islands) {
         istat[IS_NCOLJP] = 0; istat[IS_ISL] = 0; istat[IS_TILES] = 0; istat[IS_MAXTB] = 0;
         istat[IS_MISMATCH] = 0;
         istat[IS_NBIG] = 0;
+        istat[IS_NGIANT] = 0;
+        istat[IS_NGCOL] = 0;
         T.dbg[7] = 0;
     }
 }
@@ -1111,7 +1117,8 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_color_small(TiledArgs A, TiledB
     if (tid == 0) atomicMax(&A.istat[IS_NCOLJP], nround);
 }
 
-// list of the tiles that exceed one small tile (the big-tile kernel's work); and the big tiles' body
+// lists of the tiles that exceed one small tile (the big-tile kernel's work), and of those beyond the
+// big tile's LDS and the LDS colouring (the grid kernels' work, usually none); and the big tiles' body
 // constants in island order (position j <- body bsorted[j]: coalesced writes, gathered reads), 48-byte
 // records of invIw for three 16-byte loads
 __global__ void k_t_biglist(int ntile, int *istat, const float *invM, const float *invIw, TiledBuffers T) {
@@ -1129,8 +1136,12 @@ __global__ void k_t_biglist(int ntile, int *istat, const float *invM, const floa
         d[2] = make_float4(I[8], 0.0f, 0.0f, 0.0f);
     }
     if (t >= ntile || T.tb0[t] == INT_MAX) return;
-    int nb = T.tb1[t] - T.tb0[t], np = T.tp1[t] - T.tp0[t];
-    if (np > T_SMALL_P || nb > T_SMALL_B) T.biglist[atomicAdd(&istat[IS_NBIG], 1)] = t;
+    int b0, nb, p0, np;
+    tile_bounds(T, t, b0, nb, p0, np);
+    if (is_small_tile(nb, np)) return;
+    T.biglist[atomicAdd(&istat[IS_NBIG], 1)] = t;
+    if (!(nb <= T_BIG_B && np <= T_BIG_P)) T.giantlist[atomicAdd(&istat[IS_NGIANT], 1)] = t;
+    if (np > 0 && !(nb <= T_COLB && np <= T_COLP)) T.gcollist[atomicAdd(&istat[IS_NGCOL], 1)] = t;
 }
 
 // dynamic-LDS limits of the tile kernels beyond the 64 KB default; set on the current device before

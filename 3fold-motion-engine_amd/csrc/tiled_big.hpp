@@ -369,11 +369,10 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_color_giant(TiledArgs A, TiledBu
     const float *invM = A.b.invM;
     int *gcol = A.Pp.col;
     unsigned long long *claim = A.claim;
-    const int nbig = A.istat[IS_NBIG];
-    for (int i = 0; i < nbig; ++i) {
-        const int t = T.biglist[i];
-        const int b0 = T.tb0[t], nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - p0;
-        if (np == 0 || (nb <= T_COLB && np <= T_COLP)) continue;  // k_t_color's
+    const int ng = A.istat[IS_NGCOL];
+    for (int i = 0; i < ng; ++i) {
+        int b0, nb, p0, np;
+        tile_bounds(T, T.gcollist[i], b0, nb, p0, np);
         for (int k = gt; k < np; k += stride) gcol[T.psorted[p0 + k]] = -1;
         int nround = 0;
         for (int cur = 0; cur <= np; ++cur) {
@@ -446,13 +445,10 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
     __syncthreads();
     unsigned target = 0;
     const float t2d = FIXED ? 1.0f : A.t2d;
-    const int nbig = A.istat[IS_NBIG];
-    for (int i = 0; i < nbig; ++i) {
-        const int t = T.biglist[i];
-        if (T.tb0[t] == INT_MAX) continue;
+    const int ng = A.istat[IS_NGIANT];
+    for (int i = 0; i < ng; ++i) {
         int b0, nb, p0, np;
-        tile_bounds(T, t, b0, nb, p0, np);
-        if (is_small_tile(nb, np) || (nb <= T_BIG_B && np <= T_BIG_P)) continue;  // k_t_small's or k_t_big's
+        tile_bounds(T, T.giantlist[i], b0, nb, p0, np);
         if (blockIdx.x == 0) {
             // colour classes and colour-ordered pair records, as big_tile's global mode
             if (A.taped && tid == 0) atomicOr(&A.istat[IS_OVF], OVF_TAPE);
