@@ -126,7 +126,9 @@ enum {
     ME_CNT_COLORS,       /* colours of the last solve */
     ME_CNT_CACHE_HITS,
     ME_CNT_CACHE_MISSES,
-    ME_CNT_OVERFLOW,     /* capacity overflow flags (bit 0 contacts, 1 pairs, 2 candidates) */
+    ME_CNT_OVERFLOW,     /* overflow flags: bit 0 contacts, 1 pairs, 2 candidates, 3 colours, 4 tile,
+                            5 recorded tile beyond one small tile, 6 a giant island's grid barrier
+                            waited past its bound (the step's state is not valid) */
     ME_CNT_CANDIDATES,   /* broad-phase candidate pairs */
     ME_CNT_ISLANDS,      /* islands of the last tiled solve */
     ME_CNT_TILES,        /* tiles of the last tiled solve */
