@@ -1174,7 +1174,9 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
                            T);
     if (A.pev0 && hipEventRecord((hipEvent_t)A.pev0, s2) != hipSuccess) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(T_BIG_WG), dyn, s2, A, T);
-    if (A.N > T_BIG_B)  // an island beyond one workgroup's LDS is possible: the grid sweep (after k_t_big)
+    // a tile beyond the big tile's LDS (more than T_BIG_B bodies or T_BIG_P pairs) is possible: the grid
+    // sweep, after k_t_big, which leaves such tiles to it
+    if (A.N > T_BIG_B || A.capP > T_BIG_P)
         hipLaunchKernelGGL(k_t_giant<SOLVER>, dim3(T_GIANT_GRID), dim3(T_BIG_WG), 0, s2, A, T);
     if (A.pev1 && hipEventRecord((hipEvent_t)A.pev1, s2) != hipSuccess) return ME_ERUNTIME;
     if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return ME_ERUNTIME;

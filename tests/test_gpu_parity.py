@@ -154,12 +154,13 @@ def test_colouring_is_proper_and_runs_repeat():
 
 
 @pytest.mark.parametrize("kind,tile_bodies,n,steps", [("color_cache", 8, 600, 30), ("color_cache", 64, 600, 30),
-                                                      ("color_cache", 256, 600, 30), ("color_cache", 128, 6600, 6),
-                                                      ("pair_chunks", 128, 8400, 6)])
+                                                      ("color_cache", 256, 600, 30), ("color_cache", 128, 4500, 6),
+                                                      ("color_cache", 128, 6600, 6), ("pair_chunks", 128, 8400, 6)])
 def test_tiled_chain_island_beyond_lds(oracle, kind, tile_bodies, n, steps):
     """A row of touching cubes is one island: 600 bodies exceed a small tile (big-tile kernel, LDS
-    state), 6600 bodies exceed the big tile's LDS capacity (the grid of workgroups sweeping one
-    island in global memory, a grid barrier per colour step), 8400 bodies also exceed the LDS
+    state), 4500 bodies (about 9000 pairs with the ground) and 6600 bodies exceed the big tile's LDS
+    capacity by pairs and by bodies (the grid of workgroups sweeping one island in global memory, a
+    grid barrier per colour step), 8400 bodies also exceed the LDS
     colouring limits (JP rounds by the grid, recoloured every substep under pair chunks); small
     tiles make islands straddle tile boundaries."""
     _say(f"{kind} tiled chain {n} + pile 200, tile_bodies={tile_bodies}")
