@@ -16,7 +16,7 @@ FLAG_TILED, FLAG_PER_CONTACT, FLAG_GRAPH = 1, 2, 4
 
 (BUF_KEY, BUF_BI, BUF_BJ, BUF_PA, BUF_PB, BUF_N, BUF_PEN, BUF_JN, BUF_JT1, BUF_JT2, BUF_JP,
  BUF_PBI, BUF_PBJ, BUF_PSTART, BUF_PCOUNT, BUF_COLOR, BUF_VSNAP, BUF_PV, BUF_PO, BUF_INVIW,
- BUF_ISLAND, BUF_DEBUG, BUF_BIG_WORK, BUF_MU, BUF_XSNAP, BUF_WSNAP) = range(26)
+ BUF_ISLAND, BUF_DEBUG, BUF_BIG_WORK, BUF_MU, BUF_XSNAP, BUF_WSNAP, BUF_TILE_WORK) = range(27)
 
 COUNTERS = ("n_contacts", "n_pairs", "n_colors", "cache_hits", "cache_misses", "overflow",
             "n_candidates", "n_islands", "n_tiles", "max_tile_bodies", "graph_captures", "n_solved")
@@ -109,4 +109,5 @@ EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me
             "me_ncp_workspace_size", "me_ncp_sensitivity", "me_pgs_workspace_size", "me_pgs_create",
             "me_pgs_destroy", "me_pgs_prepare", "me_pgs_iterate", "me_pgs_snapshot", "me_pgs_residual",
             "me_pgs_buffer", "me_pgs_info")
-K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL, K_TILE_BIG = range(6)
+K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL, K_TILE_BIG, K_TILE_SMALL, K_TILE_GIANT, K_ISLANDS, \
+    K_COLOR = range(10)

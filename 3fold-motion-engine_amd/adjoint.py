@@ -93,10 +93,10 @@ class Rollout:
             dst.copy_(src)
 
     def _check(self):
-        self.eng.refresh_counters()
-        ovf = self.eng._counters["overflow"]
+        ovf = self.eng.refresh_counters(raise_on_overflow=False)["overflow"]
         if ovf & _lib.OVF_TAPE:
             raise RuntimeError("adjoint: an island exceeds one small tile (256 pairs / 512 bodies)")
+        self.eng.raise_overflow(ovf)
 
     def run(self, v0, w0, mu):
         """Forward: set initial velocities (N,3), (N,3) and friction (N,), step T steps; returns x_T."""

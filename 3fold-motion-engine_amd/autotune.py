@@ -5,9 +5,13 @@ per tile in {128, 192, 224} (position iterations follow as ceil(5/8 * vit), the 
 pile workload) in order of
 increasing solve cost (substeps * vit * colours), and returns the first configuration whose trial keeps
 the maximum penetration at or below the target (1e-3 m by default); among configurations of equal
-cost the faster measured one wins. Each trial steps a copy of the caller's scene for ``trial_steps``
-steps from the same initial state, on the GPU, with the tiled solve; the wall time of the trial
-(synchronised) is recorded so the caller can see the cost of every candidate.
+cost the faster measured one wins. Each trial steps a copy of the caller's scene for ``steps`` steps
+from the same initial state, on the GPU, with the tiled solve: the first half is timed (synchronised
+wall clock, no read-back inside), the second half gives the maximum penetration, reduced on the
+device after every step.
+
+The result's ``as_kwargs()`` are the engine constructor's arguments; the chosen substep count is not
+a constructor argument (the protocol takes it in ``step(dt, substeps)``) and is ``result.substeps``.
 
 This component has no counterpart in the reference (its colour count is an output of the JP
 colouring, contact_engine_gpu_colored.py:1178-1208, and its iteration counts are constructor
@@ -47,13 +51,22 @@ class TuneResult:
     trials: list = field(default_factory=list)
 
     def as_kwargs(self):
+        """Constructor arguments of the chosen configuration; step it with ``substeps``."""
         b = self.best
         return dict(n_colors=b.colours, vit=b.vit, pit=b.pit, tile_bodies=b.tile_bodies) if b else {}
 
+    @property
+    def substeps(self):
+        return self.best.substeps if self.best else None
+
 
 def _max_penetration(engine) -> float:
-    c = engine.contacts()
-    return float(np.max(c["pen"])) if len(c["pen"]) else 0.0
+    """Deepest contact of the engine's last substep, reduced on the device."""
+    from . import _lib
+    n = engine.refresh_counters()["n_solved"]
+    if n <= 0:
+        return 0.0
+    return float(engine.buffer_tensor(_lib.BUF_PEN, np.float32, n).max().item())
 
 
 def run_trial(make_engine, bodies: BodySet, colours, substeps, vit, pit, tile_bodies, steps, dt, warm_state=None):
@@ -65,16 +78,17 @@ def run_trial(make_engine, bodies: BodySet, colours, substeps, vit, pit, tile_bo
         with torch.no_grad():
             for name, arr in zip(("x", "q", "v", "w"), warm_state):
                 getattr(e, name).copy_(torch.from_numpy(np.ascontiguousarray(arr)).to(e.x.device))
+    half = max(1, steps // 2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    worst = 0.0
-    for s in range(steps):
+    for _ in range(half):
         e.step(dt, substeps)
-        if s >= steps // 2:
-            e.refresh_counters()
-            worst = max(worst, _max_penetration(e))
     torch.cuda.synchronize()
-    ms = 1e3 * (time.perf_counter() - t0) / steps
+    ms = 1e3 * (time.perf_counter() - t0) / half
+    worst = 0.0
+    for _ in range(max(1, steps - half)):
+        e.step(dt, substeps)
+        worst = max(worst, _max_penetration(e))
     return worst, ms
 
 

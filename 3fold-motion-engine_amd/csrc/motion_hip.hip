@@ -16,6 +16,7 @@
 #include "ncp.hpp"
 #include "pgs.hpp"
 #include "adjoint_kernels.hpp"
+#include "giant_grid.hpp"
 
 using namespace me;
 
@@ -128,6 +129,9 @@ struct me_engine {
     bool prof_on = false;
     std::vector<hipEvent_t> prof_ev[ME_K_COUNT_];
     int prof_used[ME_K_COUNT_] = {0};
+    // giant-island grid kernels: workgroups (all resident together) and the barrier's wait bound
+    int giant_grid = 0;
+    long long giant_wait_ticks = 0;
 };
 
 namespace {
@@ -151,6 +155,9 @@ void prof_mark(me_engine *e, int id, hipStream_t s) {
     hipEvent_t ev = prof_next(e, id);
     if (ev) (void)hipEventRecord(ev, s);
 }
+
+// the tiled solve's hook for its own kernels (TiledArgs::prof_ev)
+void *prof_event(void *eng, int id) { return (void *)prof_next((me_engine *)eng, id); }
 
 void layout(me_engine *e, char *base, size_t *bytes) {
     const me_config &c = e->cfg;
@@ -616,11 +623,6 @@ int substep(me_engine *e, double sdt64, hipStream_t s) {
         rc = solve_jacobi(e, sdt, s);
     } else if (e->cfg.flags & ME_FLAG_TILED) {
         TiledArgs ta = tiled_args_of(e, sdt);
-        if (e->prof_on && e->prof_used[ME_K_TILE_BIG] + 2 <= 2 * kProfMax) {
-            ta.pev0 = prof_next(e, ME_K_TILE_BIG);
-            ta.pev1 = ta.pev0 ? prof_next(e, ME_K_TILE_BIG) : nullptr;
-            if (!ta.pev1) ta.pev0 = nullptr;
-        }
         rc = tiled_prepare(e->tb, ta, s);
         if (rc) return fail(rc, "tiled solve: launch failed");
         prof_mark(e, ME_K_TILED, s);
@@ -746,7 +748,10 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.bsum = e->bsum; a.nbsum = e->nbsum;
     a.claim = e->claim;
     a.side = e->side; a.ev_fork = e->ev_fork; a.ev_join = e->ev_join;
-    a.pev0 = nullptr; a.pev1 = nullptr;
+    a.prof_ev = e->prof_on ? prof_event : nullptr;
+    a.prof = e;
+    a.giant_grid = e->giant_grid;
+    a.giant_wait_ticks = e->giant_wait_ticks;
     a.taped = e->taping ? 1 : 0;
     a.tape = e->tape;
     return a;
@@ -766,6 +771,34 @@ int bead_half_extents(me_engine *e, hipStream_t s) {
         if (!(m == m) || !(R >= 0.0f)) return fail(ME_ENONFINITE, "bead rest positions or radius not finite");
         e->bhalf[k] = m + R;
     }
+    return ME_OK;
+}
+
+// the giant-island kernels' grid: the preferred 64 workgroups lowered to what can be resident at
+// once on this device (their workgroups wait for each other at grid barriers), and the barrier's
+// wait bound, 100 ms of the constant-rate wall clock
+int giant_setup(me_engine *e) {
+    int dev = 0, ncu = 0, khz = 0, occ[3] = {0, 0, 0};
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) {
+        (void)hipGetLastError();
+        khz = 100000;  // gfx9 constant-rate counter: 100 MHz
+    }
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[0], (const void *)k_t_giant<ME_SOLVER_PAIR_CHUNKS>,
+                                                         T_BIG_WG, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[1], (const void *)k_t_giant<ME_SOLVER_COLOR_CACHE>,
+                                                         T_BIG_WG, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[2], (const void *)k_t_color_giant, T_BIG_WG, 0));
+    int sweep = occ[0] < occ[1] ? occ[0] : occ[1];
+    e->giant_grid = giant_grid_size(T_GIANT_GRID, sweep, occ[2], ncu);
+    e->giant_wait_ticks = 100LL * khz;
+    bool may_launch = (e->cfg.flags & ME_FLAG_TILED) && e->cfg.solver != ME_SOLVER_JACOBI &&
+                      (e->N > T_BIG_B || e->capP > T_BIG_P || e->N > T_COLB);
+    if (may_launch && e->giant_grid < 1)
+        return fail(ME_ERUNTIME, "the giant-island grid kernels cannot hold one workgroup resident on this device "
+                                 "(occupancy " + std::to_string(sweep) + "/" + std::to_string(occ[2]) + " per CU, " +
+                                 std::to_string(ncu) + " CUs)");
     return ME_OK;
 }
 
@@ -1067,6 +1100,11 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
         release(e);
         return fail(ME_ERUNTIME, "setting the tile kernels' LDS limits failed");
     }
+    rc = giant_setup(e);
+    if (rc) {
+        release(e);
+        return rc;
+    }
     hipStream_t s = (hipStream_t)stream;
     // zero the workspace with a fill kernel (no memset nodes, so a caller may capture later steps)
     size_t words = need / 4;
@@ -1151,6 +1189,7 @@ int me_buffer(me_engine *e, int id, void **ptr, int64_t *count) {
         case ME_BUF_ISLAND: *ptr = e->tb.root; *count = N; break;
         case ME_BUF_DEBUG: *ptr = e->tb.dbg; *count = 16; break;
         case ME_BUF_BIG_WORK: *ptr = e->tb.bigw; *count = 4; break;
+        case ME_BUF_TILE_WORK: *ptr = e->tb.bigw; *count = 12; break;
         case ME_BUF_MU: *ptr = e->Cc.mu; *count = C; break;
         case ME_BUF_XSNAP: *ptr = e->xsnap; *count = 3 * N; break;
         case ME_BUF_WSNAP: *ptr = e->wsnap; *count = 3 * N; break;
@@ -1185,7 +1224,7 @@ int me_profile(me_engine *e, int enable) {
     if (e->prof_on) {  // a new measurement window; disabling keeps the recorded events for reading
         for (int k = 0; k < ME_K_COUNT_; ++k) e->prof_used[k] = 0;
         HIP_TRY(hipDeviceSynchronize());
-        HIP_TRY(hipMemset(e->tb.bigw, 0, 4 * sizeof(long long)));
+        HIP_TRY(hipMemset(e->tb.bigw, 0, 12 * sizeof(long long)));
         HIP_TRY(hipDeviceSynchronize());
     }
     return ME_OK;

@@ -40,7 +40,8 @@ struct TiledBuffers {
     float *sprep;                // big tiles: 4 x 24 floats per colour-sorted pair (prep + impulses)
     int *srec;                   // big tiles: 4 ints per colour-sorted pair (p, a, b, first contact)
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
-    long long *bigw;             // big-tile work since the last reset: tiles, pairs, dynamic pair ends, contacts
+    long long *bigw;             // tile work since the last reset, 4 counts (tiles, pairs, dynamic pair ends,
+                                 // contacts) for each of k_t_big [0..3], k_t_small [4..7], k_t_giant [8..11]
     int *biglist;                // tiles beyond one small tile (IS_NBIG of them)
     int *giantlist;              // tiles beyond the big tile's LDS (IS_NGIANT of them): k_t_giant's
     int *gcollist;               // tiles beyond the LDS colouring limits (IS_NGCOL): k_t_color_giant's
@@ -81,8 +82,12 @@ struct TiledArgs {
     int nbsum;
     unsigned long long *claim;
     void *side, *ev_fork, *ev_join;  // side stream and fork/join events of the sweep kernels
-    void *pev0, *pev1;               // timing events around the big-tile kernel (null: not timed)
+    // timing: next event of kernel `id` (ME_K_*) for the engine `prof`, null when not timed
+    void *(*prof_ev)(void *prof, int id);
+    void *prof;
     int taped;                       // record the substep into tape (k_t_small<.., true>)
+    int giant_grid;                  // workgroups of the giant-island grid kernels, all resident together
+    long long giant_wait_ticks;      // bound of a grid barrier's wait, in wall-clock counter ticks
     AdjTape tape;
 };
 
@@ -123,7 +128,7 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.sprep = B.template take<float>(96 * p);
     t.srec = B.template take<int>(4 * p);
     t.dbg = B.template take<long long>(16);
-    t.bigw = B.template take<long long>(4);
+    t.bigw = B.template take<long long>(12);
     t.biglist = B.template take<int>(n);
     t.giantlist = B.template take<int>(n);
     t.gcollist = B.template take<int>(n);
@@ -926,12 +931,14 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers
     if (!is_small_tile(nb, np)) return;
     __shared__ float sv[3 * T_SMALL_B], sw[3 * T_SMALL_B], spv[3 * T_SMALL_B], spo[3 * T_SMALL_B];
     __shared__ int cnt[ME_NCOLMAX + 1], fill[ME_NCOLMAX + 1], plist[T_SMALL_P], plcol[T_SMALL_P];
-    __shared__ int maxc;
+    __shared__ int maxc, wdyn, wcon;
     const int tid = threadIdx.x;
     if (tid == 0) {
         atomicAdd(&A.istat[IS_TILES], 1);
         atomicMax(&A.istat[IS_MAXTB], nb);
         maxc = 0;
+        wdyn = 0;
+        wcon = 0;
     }
     int ncol = A.istat[IS_NCOL];
     if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
@@ -1000,8 +1007,18 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers
         for (int it = 0; it < A.vit; ++it) tape_sweep(A, T, b0, nb, it, sv, sw, P, -1);
         for (int it = 0; it < A.pit; ++it) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, -1);
     }
-    if (p >= 0) pc_store(P, A, p);
+    if (p >= 0) {
+        pc_store(P, A, p);
+        atomicAdd(&wdyn, (int)P.dynA + (int)P.dynB);
+        atomicAdd(&wcon, P.cnt);
+    }
     __syncthreads();
+    if (tid == 0) {  // work of this tile for the algorithmic-byte count of the launch
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[4]), 1ull);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[5]), (unsigned long long)np);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[6]), (unsigned long long)wdyn);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[7]), (unsigned long long)wcon);
+    }
     tile_writeback<T_SMALL_P>(A, T, b0, nb, true, sv, sw, spv, spo);
 }
 
@@ -1159,6 +1176,13 @@ inline int tiled_set_attributes() {
     return ME_OK;
 }
 
+// timing event of kernel `id` on stream s (no-op when the engine is not timing); non-zero on failure
+inline int prof_rec(const TiledArgs &A, int id, hipStream_t s) {
+    if (!A.prof_ev) return 0;
+    hipEvent_t ev = (hipEvent_t)A.prof_ev(A.prof, id);
+    return ev && hipEventRecord(ev, s) != hipSuccess;
+}
+
 // small tiles on stream s, big tiles on the side stream s2 (disjoint islands), joined on s
 template <int SOLVER>
 inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStream_t s) {
@@ -1167,18 +1191,22 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     const size_t dyn = T_BIG_DYN;
     hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
+    if (prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
     if (A.taped)
         hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
     else
         hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>), dim3(ntile), dim3(T_SMALL_P), 0, s, A,
                            T);
-    if (A.pev0 && hipEventRecord((hipEvent_t)A.pev0, s2) != hipSuccess) return ME_ERUNTIME;
+    if (prof_rec(A, ME_K_TILE_SMALL, s) || prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(T_BIG_WG), dyn, s2, A, T);
+    if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
     // a tile beyond the big tile's LDS (more than T_BIG_B bodies or T_BIG_P pairs) is possible: the grid
     // sweep, after k_t_big, which leaves such tiles to it
-    if (A.N > T_BIG_B || A.capP > T_BIG_P)
-        hipLaunchKernelGGL(k_t_giant<SOLVER>, dim3(T_GIANT_GRID), dim3(T_BIG_WG), 0, s2, A, T);
-    if (A.pev1 && hipEventRecord((hipEvent_t)A.pev1, s2) != hipSuccess) return ME_ERUNTIME;
+    if (A.N > T_BIG_B || A.capP > T_BIG_P) {
+        if (prof_rec(A, ME_K_TILE_GIANT, s2)) return ME_ERUNTIME;
+        hipLaunchKernelGGL(k_t_giant<SOLVER>, dim3(A.giant_grid), dim3(T_BIG_WG), 0, s2, A, T);
+        if (prof_rec(A, ME_K_TILE_GIANT, s2)) return ME_ERUNTIME;
+    }
     if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return ME_ERUNTIME;
     return ME_OK;
 }
@@ -1215,8 +1243,10 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     if (cc) hipLaunchKernelGGL(k_t_cc_restore, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.cache_col,
                                A.Pp.col);
     if (lds_isl) {
+        if (prof_rec(A, ME_K_ISLANDS, s)) return ME_ERUNTIME;
         hipLaunchKernelGGL(k_t_islands_lds, dim3(1), dim3(T_ISL_THREADS), 3 * N * sizeof(int), s, N, capP, TB, A.istat,
                            A.Pp.pbi, A.Pp.pbj, A.b.invM, T);
+        if (prof_rec(A, ME_K_ISLANDS, s)) return ME_ERUNTIME;
     } else {
         hipLaunchKernelGGL(k_t_union, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
                            A.b.invM, T.parent);
@@ -1235,9 +1265,11 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
                        A.invIw, T);
     hipLaunchKernelGGL(k_t_color_small, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
     const int giant = N > T_COLB;  // a tile beyond the LDS colouring limits: coloured by a grid of workgroups
+    if (prof_rec(A, ME_K_COLOR, s)) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(1024), T_COL_DYN, s, A, T,
                        giant);
-    if (giant) hipLaunchKernelGGL(k_t_color_giant, dim3(T_GIANT_GRID), dim3(T_BIG_WG), 0, s, A, T);
+    if (prof_rec(A, ME_K_COLOR, s)) return ME_ERUNTIME;
+    if (giant) hipLaunchKernelGGL(k_t_color_giant, dim3(A.giant_grid), dim3(T_BIG_WG), 0, s, A, T);
     hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
     hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);
     return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;

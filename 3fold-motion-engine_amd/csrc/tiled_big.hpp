@@ -329,18 +329,22 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T)
 // waits, and invalidates the CU's L1 and the L2's copies of other XCDs' lines (agent-scope acquire)
 // before the workgroup reads anything. One release and one acquire per workgroup: a fence in every
 // wave costs about 8x more per barrier (32 workgroups: 12 us against 1.5 us).
-// The wait is bounded: a workgroup that is not yet resident delays it (the grid's other kernels
-// run to completion on their own), and past the bound the launch flags OVF_GIANT and winds down.
-__device__ __forceinline__ bool giant_barrier(unsigned *count, unsigned target, int *istat, int &stop) {
+// The host sizes the grid so that all of its workgroups can be resident at once on an idle GPU
+// (giant_grid_size); a workgroup that is not yet resident only delays the barrier while the kernels
+// beside it on the other stream run to completion on their own. The wait is still bounded, by the
+// wall clock (`wait_ticks` of the constant-rate counter, about 100 ms): past it the launch flags
+// OVF_GIANT and winds down, and the engine's step raises.
+__device__ __forceinline__ bool giant_barrier(unsigned *count, unsigned target, int *istat, int &stop,
+                                              long long wait_ticks) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // gfx9 encoding: vmcnt 0, expcnt and lgkmcnt not waited
     __syncthreads();
     if (threadIdx.x == 0 && !stop) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned spins = 0;
+        const long long t0 = wall_clock64();
         while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
-            if (++spins > (1u << 26)) {
+            if (wall_clock64() - t0 > wait_ticks) {
                 atomicOr(&istat[IS_OVF], OVF_GIANT);
                 stop = 1;
                 break;
@@ -380,7 +384,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_color_giant(TiledArgs A, TiledBu
             if (gt == 0) __hip_atomic_store(&gleft[(cur + 1) % 3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (tid == 0) wleft = 0;
             target += G;
-            if (!giant_barrier(gb, target, A.istat, stop)) return;
+            if (!giant_barrier(gb, target, A.istat, stop, A.giant_wait_ticks)) return;
             for (int k = gt; k < np; k += stride) {
                 int p = T.psorted[p0 + k];
                 if (gcol[p] >= 0) continue;
@@ -390,7 +394,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_color_giant(TiledArgs A, TiledBu
                 if (b >= 0 && invM[b] > 0.0f) atomicMin(&claim[b], key);
             }
             target += G;
-            if (!giant_barrier(gb, target, A.istat, stop)) return;
+            if (!giant_barrier(gb, target, A.istat, stop, A.giant_wait_ticks)) return;
             int nl = 0;
             for (int k = gt; k < np; k += stride) {
                 int p = T.psorted[p0 + k];
@@ -407,14 +411,14 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_color_giant(TiledArgs A, TiledBu
             __syncthreads();
             if (tid == 0 && wleft) atomicAdd(&gleft[cur % 3], (unsigned)wleft);
             target += G;
-            if (!giant_barrier(gb, target, A.istat, stop)) return;
+            if (!giant_barrier(gb, target, A.istat, stop, A.giant_wait_ticks)) return;
             if (__hip_atomic_load(&gleft[cur % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
                 nround = cur + 1;
                 break;
             }
         }
         target += G;  // every workgroup has read the last round's counter
-        if (!giant_barrier(gb, target, A.istat, stop)) return;
+        if (!giant_barrier(gb, target, A.istat, stop, A.giant_wait_ticks)) return;
         if (gt < 3) __hip_atomic_store(&gleft[gt], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // class sizes: a histogram in LDS, then one global add per class and workgroup
         for (int c = tid; c < nround && c < ME_NCOLMAX; c += T_BIG_WG) hist[c] = 0;
@@ -431,7 +435,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_color_giant(TiledArgs A, TiledBu
         if (ovf) atomicOr(&A.istat[IS_OVF], OVF_COLORS);
         if (gt == 0) atomicMax(&A.istat[IS_NCOLJP], nround);
         target += G;  // the next tile's round counters start from zero
-        if (!giant_barrier(gb, target, A.istat, stop)) return;
+        if (!giant_barrier(gb, target, A.istat, stop, A.giant_wait_ticks)) return;
     }
 }
 
@@ -499,14 +503,14 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
             for (int c = tid; c <= maxc; c += T_BIG_WG) T.gcst[c] = cst[c];
             if (tid == 0) {
                 T.gcst[ME_NCOLMAX + 1] = maxc;
-                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[0]), 1ull);
-                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[1]), (unsigned long long)np);
-                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[2]), (unsigned long long)wdyn);
-                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[3]), (unsigned long long)wcon);
+                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[8]), 1ull);
+                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[9]), (unsigned long long)np);
+                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[10]), (unsigned long long)wdyn);
+                atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[11]), (unsigned long long)wcon);
             }
         }
         target += G;
-        if (!giant_barrier(T.gbar, target, A.istat, stop)) return;
+        if (!giant_barrier(T.gbar, target, A.istat, stop, A.giant_wait_ticks)) return;
         const int nc = T.gcst[ME_NCOLMAX + 1];
         // a colour's pairs in G contiguous runs of whole waves, one run per workgroup (loading the next
         // colour step's records before the barrier, which they do not depend on, measured 4 % slower)
@@ -523,7 +527,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
                     else big_pair<2, FIXED, false>(A, T, 0u, g, q, b0, A.pv, A.po, t2d);
                 }
                 target += G;
-                if (!giant_barrier(T.gbar, target, A.istat, stop)) return false;
+                if (!giant_barrier(T.gbar, target, A.istat, stop, A.giant_wait_ticks)) return false;
             }
             return true;
         };
@@ -543,7 +547,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
                 integrate_body(A.b.x, A.b.q, ld3(A.b.v, b), ld3(A.b.w, b), ld3(A.pv, b), ld3(A.po, b), b, A.dt);
         }
         target += G;  // the next giant tile's records reuse gcst
-        if (!giant_barrier(T.gbar, target, A.istat, stop)) return;
+        if (!giant_barrier(T.gbar, target, A.istat, stop, A.giant_wait_ticks)) return;
         if (blockIdx.x == 0 && tid == 0 &&
             (long long)atomicAdd(reinterpret_cast<unsigned long long *>(&T.dbg[7]), 0ull) == nb) {
             T.dbg[1] = T.dbg[2] = T.dbg[3] = T.dbg[0];

@@ -227,11 +227,16 @@ class _HipContactEngine:
         if s.check_every_step:
             s.refresh_counters()
 
-    def refresh_counters(s):
+    def refresh_counters(s, raise_on_overflow=True):
         out = (C.c_int64 * len(_lib.COUNTERS))()
         _lib.check(_lib.lib().me_read_counters(s._h, out, C.c_void_p(s._stream())), "me_read_counters")
         s._counters = dict(zip(_lib.COUNTERS, (int(v) for v in out)))
         ovf = s._counters["overflow"]
+        if ovf and raise_on_overflow:
+            s.raise_overflow(ovf)
+        return s._counters
+
+    def raise_overflow(s, ovf):
         if ovf:
             what = [n for bit, n in ((1, "contacts"), (2, "body pairs"), (4, "broad-phase candidates"),
                                      (8, "colours"), (16, "tile"), (32, "recorded tile for the adjoint"))
@@ -241,7 +246,6 @@ class _HipContactEngine:
                                    f"waiting for its workgroups; the step's state is not valid")
             raise RuntimeError(f"{s.name}.step: capacity exceeded ({', '.join(what)}); raise max_pairs / "
                                f"max_contacts")
-        return s._counters
 
     # harness-visible counters (contact_engine_gpu_colored.py:824-829)
     n_contacts = property(lambda s: s._counters["n_contacts"])
@@ -326,45 +330,70 @@ class RelaxedJacobiContactEngine(_HipContactEngine):
         super().__init__(dims=dims, vit=vit, pit=pit, mu=mu, relax=relax, **kw)
 
 
-_COLORED_IGNORED = ("manifold_reduce", "manifold_max_points", "manifold_on_host", "graph_mode", "chunk_mode",
-                    "trim_unused_priorities")
-
-
 class GraphColoredContactEngine(_HipContactEngine):
-    """Graph-coloured Gauss-Seidel with body-pair chunks (contact_engine_gpu_colored.py:793-1410,
-    pair_chunks=True): JP colouring of the body-pair graph, one thread per pair, slip-direction
-    friction frame, warm start by feature key without applying it to the velocities.
-    ``pair_chunks=False`` selects the per-contact colouring (every contact its own chunk)."""
+    """Graph-coloured Gauss-Seidel (contact_engine_gpu_colored.py:793-1410), with the reference's
+    constructor and defaults (:798-823):
+
+    - ``pair_chunks=False`` (default): Jones-Plassmann colouring of the contact graph, every contact
+      its own chunk, contacts in feature-key order (:903-917, 1105-1136), ``vit`` 20;
+    - ``pair_chunks=True``: colouring of the body-pair graph, a pair's <= 4 contacts solved as one
+      chunk; needs the device manifold reduction (``manifold_reduce=True``,
+      ``manifold_on_host=False``), ``vit`` 40.
+
+    Slip-direction friction frame, warm start by feature key without applying it to the
+    velocities. ``graph_mode`` and ``chunk_mode`` choose a launch schedule in the reference and
+    leave the results unchanged; they are validated as there. Box bodies always carry at most 4
+    narrow-phase points per pair, so ``manifold_reduce`` selects no further reduction for them."""
     name = "graph_colored_gpu"
     _solver = _lib.SOLVER_PAIR_CHUNKS
 
     def __init__(s, dims=(0.3, 0.3, 0.2), vit=None, pit=20, mu=0.5, warm_start=True, t2_damp=1.0,
-                 pair_chunks=True, graph_mode="device", chunk_mode="registers", **kw):
+                 manifold_reduce=False, manifold_max_points=4, manifold_on_host=False, profile=False,
+                 graph_capture=False, graph_mode="device", pair_chunks=False, chunk_mode="registers", **kw):
         if graph_mode not in ("device", "spans"):
             raise ValueError("contact_engine_gpu_colored: graph_mode is 'device' or 'spans'")
-        if chunk_mode not in ("registers", "global"):
-            raise ValueError("contact_engine_gpu_colored: chunk_mode is 'registers' or 'global'")
-        for k in _COLORED_IGNORED:
-            kw.pop(k, None)
+        pair_chunks = bool(pair_chunks)
+        if pair_chunks:
+            if chunk_mode not in ("registers", "global"):
+                raise ValueError("contact_engine_gpu_colored: chunk_mode is 'registers' or 'global'")
+            if not (manifold_reduce and not manifold_on_host):
+                raise ValueError("contact_engine_gpu_colored: pair_chunks=True needs the device manifold "
+                                 "reduction (manifold_reduce=True, manifold_on_host=False): a chunk is the "
+                                 "<= manifold_max_points contacts of one body pair")
+        if manifold_reduce and int(manifold_max_points) != 4:
+            raise NotImplementedError("contact_engine_gpu_colored: only manifold_max_points=4 is built")
+        if manifold_reduce and not pair_chunks:
+            raise NotImplementedError("contact_engine_gpu_colored: per-contact colouring of manifold-reduced "
+                                      "contacts (pair_chunks=False with manifold_reduce=True) is not built; use "
+                                      "pair_chunks=True or manifold_reduce=False")
+        s.manifold_reduce = bool(manifold_reduce)
+        s.manifold_max_points = int(manifold_max_points)
+        s.manifold_on_host = bool(manifold_on_host)
+        s.pair_chunks = pair_chunks
+        s.graph_mode, s.chunk_mode = graph_mode, chunk_mode
         # the island-tiled solve is the default schedule: bit-identical to the per-colour launches
         # (tiled=False), which stay available
         kw.setdefault("tiled", True)
-        super().__init__(dims=dims, vit=40 if vit is None else int(vit), pit=pit, mu=mu,
-                         warm_start=warm_start, t2_damp=t2_damp, per_contact=not pair_chunks, **kw)
+        vit = (40 if manifold_reduce else 20) if vit is None else int(vit)
+        super().__init__(dims=dims, vit=vit, pit=pit, mu=mu, warm_start=warm_start, t2_damp=t2_damp,
+                         per_contact=not pair_chunks, profile=profile, graph_capture=graph_capture, **kw)
 
 
 class ColorCacheContactEngine(GraphColoredContactEngine):
     """ColorCache (contact_engine_gpu_color_cache.py:21-54 and its chain): normal-derived friction
     frame, cached impulses applied per colour before the sweeps, pair colouring reused while the
-    pair list and invM are unchanged, force from the post-gravity velocity snapshot."""
+    pair list and invM are unchanged, force from the post-gravity velocity snapshot. As in the
+    chain's FixedFrameContactEngine (contact_engine_gpu_fixed_frame.py:83-86) the manifold
+    reduction and pair chunks are forced on, so ``vit`` defaults to 40 and ``manifold_on_host=True``
+    is refused by the pair-chunk rule."""
     _solver = _lib.SOLVER_COLOR_CACHE
 
     def __init__(s, **kw):
+        kw.update(manifold_reduce=True, pair_chunks=True, chunk_mode="registers")
         if kw.get("t2_damp", 1.0) != 1.0:
             raise ValueError("Applied warm start requires full tangent updates")
-        kw["pair_chunks"] = True
-        # the reference's ColorCache chain always captures its solve (contact_engine_gpu_fixed_frame.py:84);
-        # here graph_capture=True captures the whole step once per (dt, substeps) and replays it, but the
+        # the reference's chain always captures its solve (contact_engine_gpu_fixed_frame.py:84); here
+        # graph_capture=True captures the whole step once per (dt, substeps) and replays it, but the
         # default stays eager: on MI355X the replay measured 5 % slower than the launch sequence (the
         # reference's N = 10 000 lattice: 0.573 against 0.547 ms per step)
         super().__init__(**kw)

@@ -59,8 +59,11 @@ def cpu_baseline(x, q, v, w, bodies_set, solver, V, P, S, n_colors, budget_s, om
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--settle", type=int, default=None,
+                    help="untimed steps before the warm-up, so that the timed window sees the settled pile "
+                         "(default 400 for the piles and the batch)")
     ap.add_argument("--bodies", type=int, default=10000)
     ap.add_argument("--solver", default="color_cache", choices=["color_cache", "pair_chunks", "jacobi"])
     ap.add_argument("--colors", type=int, default=16)
@@ -79,17 +82,28 @@ def main():
                                                                  "pile_tuned"])
     ap.add_argument("--rollouts", type=int, default=3, help="adjoint workload: timed forward+backward rollouts")
     ap.add_argument("--extras", type=int, default=1,
-                    help="pile workload: also measure the 256-scene batch and the config-4 adjoint and attach "
-                         "their results to the JSON line")
+                    help="pile workload: also measure the 256-scene batch, the 100 000-box pile, the config-4 "
+                         "adjoint, the NCP sensitivity, the reference's lattice fixture and the exact-cone PGS, "
+                         "and attach their results to the JSON line")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--lattice-runs", default="color_cache:1,color_cache:0,pair_chunks:1,pair_chunks:0,jacobi:0",
                     help="lattice workload: engines to run, solver:tiled pairs")
     ap.add_argument("--seq-rounds", type=int, default=3)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--tune-steps", type=int, default=40, help="autotuner: steps per trial")
+    ap.add_argument("--trace-summary", default=None, metavar="CSV",
+                    help="print a per-kernel table of a rocprofv3 kernel_trace.csv over the substeps given by "
+                         "--trace-window, and exit (no GPU work)")
+    ap.add_argument("--trace-window", default=None, metavar="FIRST:LAST",
+                    help="substep range [FIRST, LAST) of --trace-summary, counted by k_body_pre dispatches "
+                         "(two substeps per step)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, one plain device-to-device copy of MIB MiB (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     a = ap.parse_args()
+    if a.trace_summary:
+        trace_summary(a.trace_summary, a.trace_window)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
               f"(independent scenes and islands shard without communication; see DESIGN.md)",
@@ -115,148 +129,20 @@ def main():
         print(json.dumps(bench_pile_tuned(a)), flush=True)
         return
     if a.workload == "large":
-        a.bodies = 100000 if a.bodies == 10000 else a.bodies
-        a.tiled = 1
-
-    import torch
-    pkg = importlib.import_module("3fold-motion-engine_amd")
-    L = importlib.import_module("3fold-motion-engine_amd._lib")
-    import ctypes as C
-
-    bodies = (pkg.scenes.box_pile(a.bodies, seed=a.seed, mass_ratio=100.0) if a.workload == "large"
-              else pkg.scenes.box_pile(a.bodies, seed=a.seed))
-    cls = {"color_cache": pkg.ColorCacheContactEngine, "pair_chunks": pkg.GraphColoredContactEngine,
-           "jacobi": pkg.RelaxedJacobiContactEngine}[a.solver]
-    kw = dict(vit=a.vit, pit=a.pit, check_every_step=False)
-    if a.solver != "jacobi":
-        kw.update(n_colors=a.colors, tiled=bool(a.tiled), tile_bodies=a.tile_bodies or 128, graph_capture=bool(a.graph))
-    e = cls(**kw)
-    e.add_boxes(bodies)
-    dt, S = 1 / 240, a.substeps
-    t_w = time.perf_counter()
-    for i in range(a.warmup):
-        e.step(dt, S)
-        if (i + 1) % 100 == 0:
-            torch.cuda.synchronize()
-            print(f"[bench] warmup {i + 1}/{a.warmup} {time.perf_counter() - t_w:.1f}s", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    e.refresh_counters()
-    print(f"[bench] at timed start: {e._counters}", file=sys.stderr, flush=True)
-    x0, q0, v0, w0 = e.state_numpy()
-    Lib = L.lib()
-    # kernel timing needs per-launch events, so steps taken while profiling run eagerly; with --graph
-    # the timed region replays graphs and a second window of the same length is profiled eagerly
-    if not a.graph:
-        L.check(Lib.me_profile(e._h, 1), "me_profile")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        e.step(dt, S)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    eager_ms = None
-    if a.graph:
-        cnt = e.refresh_counters()
-        print(f"[bench] graph replays: {1e3 * elapsed / a.steps:.4f} ms/step, {cnt['graph_captures']} capture(s)",
-              file=sys.stderr, flush=True)
-        L.check(Lib.me_profile(e._h, 1), "me_profile")
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        for i in range(a.steps):
-            e.step(dt, S)
-        torch.cuda.synchronize()
-        eager_ms = 1e3 * (time.perf_counter() - t2) / a.steps
-        print(f"[bench] profiled eager window after it: {eager_ms:.4f} ms/step", file=sys.stderr, flush=True)
-    L.check(Lib.me_profile(e._h, 0), "me_profile")
-    cnt = e.refresh_counters()
-    print(f"[bench] at timed end: {cnt}", file=sys.stderr, flush=True)
-    ms_per_step = 1e3 * elapsed / a.steps
-    value = a.bodies * a.steps / elapsed
-
-    # dominant kernel: the profiled kernel with the largest summed time
-    names = {L.K_SWEEP_VEL: "k_sweep_vel", L.K_SWEEP_POS: "k_sweep_pos", L.K_NARROW: "k_narrow",
-             L.K_TILED: "k_tiled_solve", L.K_JAC_VEL: "k_jac_vel", L.K_TILE_BIG: "k_t_big"}
-    prof = {}
-    for kid, nm in names.items():
-        tot, n = C.c_double(), C.c_int64()
-        L.check(Lib.me_profile_read(e._h, kid, C.byref(tot), C.byref(n)), "me_profile_read")
-        if n.value:
-            prof[nm] = (tot.value, n.value)
-    dbg = e.buffer(L.BUF_DEBUG, np.int64, (16,))
-    if dbg[6] > dbg[0] > 0:
-        ph = np.diff(dbg[:7]).astype(np.float64)
-        fr = ph / ph.sum()
-        print(f"[bench] largest big tile ({dbg[7]} bodies, {dbg[8]} pairs), last substep, share of its time: "
-              f"load {fr[0]:.3f} count {fr[1]:.3f} records {fr[2]:.3f} vel {fr[3]:.3f} pos {fr[4]:.3f} "
-              f"writeback {fr[5]:.3f} ({int(ph.sum())} wall-clock ticks)", file=sys.stderr, flush=True)
-    pairs = e.pairs()
-    ab = algorithmic_bytes(pairs, bodies.invM, a.solver, a.vit, a.pit, S, a.bodies, cnt["n_contacts"])
-    ncol = max(cnt["n_colors"], 1)
-    per_launch = {"k_sweep_vel": ab["vel_sweep"] / ncol, "k_sweep_pos": ab["pos_sweep"] / ncol}
-    if "k_t_big" in prof:
-        # the big-tile kernel is the critical path of the tiled solve (the small tiles run beside it on
-        # a second stream); its algorithmic bytes come from the work it counted over the same window
-        tiles, bp, bd, bc = (int(v) for v in e.buffer(L.BUF_BIG_WORK, np.int64, (4,)))
-        nl = prof["k_t_big"][1]
-        vel_b = 20 * bp + 100 * bd + 64 * bc
-        pos_b = 20 * bp + 100 * bd + 52 * bc
-        warm_b = pos_b if a.solver == "color_cache" else 0
-        per_launch["k_t_big"] = (warm_b + a.vit * vel_b + a.pit * pos_b) / max(nl, 1)
-        big_work = {"tiles": tiles / max(nl, 1), "pairs": bp / max(nl, 1), "dyn_ends": bd / max(nl, 1),
-                    "contacts": bc / max(nl, 1)}
-        print(f"[bench] k_t_big per launch: {big_work}", file=sys.stderr, flush=True)
-    roofline = None
-    if prof:
-        dom = max((k for k in prof if k in per_launch), key=lambda k: prof[k][0], default=None)
-        if dom:
-            avg_s = prof[dom][0] / prof[dom][1] / 1e3
-            ach = per_launch[dom] / avg_s / 1e9
-            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 3), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None,
-                        "bytes_per_launch": int(per_launch[dom]), "avg_launch_us": round(avg_s * 1e6, 3),
-                        "launches_timed": prof[dom][1]}
-    cpu = cpu_all = None
-    if not a.no_cpu:
-        steps_c, secs_c = cpu_baseline(x0, q0, v0, w0, bodies, a.solver, a.vit, a.pit, S,
-                                       a.colors if a.solver != "jacobi" else 0, a.cpu_seconds)
-        cpu = {"value": round(a.bodies * steps_c / secs_c, 1), "unit": "body-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"CPU oracle (C, -O2, contraction off, 1 thread) from the GPU state at the start "
-                         f"of the timed region: {steps_c} steps of the same {a.bodies}-box pile in "
-                         f"{secs_c:.1f} s on {os.cpu_count()} visible host cores"}
-        if a.solver != "jacobi":
-            threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
-            os.environ["OMP_NUM_THREADS"] = str(threads)
-            steps_m, secs_m = cpu_baseline(x0, q0, v0, w0, bodies, a.solver, a.vit, a.pit, S, a.colors,
-                                           a.cpu_seconds, omp=True)
-            cpu_all = {"value": round(a.bodies * steps_m / secs_m, 1), "unit": "body-steps/s", "cores": threads,
-                       "kind": "port",
-                       "sample": f"the same oracle built with OpenMP over the pairs of a colour ({threads} "
-                                 f"threads; contact generation, colouring and integration stay serial): "
-                                 f"{steps_m} steps in {secs_m:.1f} s"}
-    out = {"metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box pile)",
-           "config": {"workload": f"{a.bodies}-box pile, {a.solver}, {a.colors} colours, S{S} V{a.vit} P{a.pit}, "
-                                  f"dt 1/240" + (", tiled" if a.tiled else ", plain launches"),
-                      "bodies": a.bodies, "solver": a.solver, "colors": a.colors, "substeps": S,
-                      "vit": a.vit, "pit": a.pit, "tiled": bool(a.tiled), "graph": bool(a.graph),
-                      "contacts": cnt["n_contacts"], "pairs": cnt["n_pairs"], "colors_used": cnt["n_colors"]},
-           "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all,
-           "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in prof.items()}}
-    if eager_ms is not None:
-        out["kernel_timing_window"] = {"mode": "eager, right after the timed graph replays", "steps": a.steps,
-                                       "ms_per_step": round(eager_ms, 4)}
-    traffic = _traffic_from_profile(roofline["kernel"] if roofline else None, out["config"]["workload"])
-    if roofline and traffic:
-        roofline["traffic"] = traffic
-    if a.extras and a.workload == "pile" and a.bodies == 10000:
-        del e
+        print(json.dumps(bench_pile(a, large=True)), flush=True)
+        return
+    out = bench_pile(a)
+    if a.extras and a.bodies == 10000:
+        import torch
         torch.cuda.empty_cache()
-        out["batched_256"] = _extra(bench_batched, a, warmup=100, steps=100, seq_rounds=1, scenes=256,
-                                    keys=("value", "ms_per_step", "speedup_vs_sequential", "sequential_plain_launches",
-                                          "config"))
+        out["batched_256"] = _extra(bench_batched, a, warmup=20, steps=100, settle=400, seq_rounds=1, scenes=256,
+                                    autotune=True, tune_steps=24, tile_bodies=0,
+                                    keys=("value", "ms_per_step", "speedup_vs_sequential_tiled",
+                                          "speedup_vs_sequential_plain", "sequential_tiled", "sequential_plain_launches",
+                                          "graph_replay", "autotune", "roofline", "config"))
+        torch.cuda.empty_cache()
+        out["large_100k"] = _extra(bench_pile, a, large=True, settle=200, warmup=5, steps=95, cpu_seconds=0.0,
+                                   keys=("value", "ms_per_step", "roofline", "run_300", "config", "kernel_ms"))
         torch.cuda.empty_cache()
         out["adjoint"] = _extra(bench_adjoint, a, rollouts=2,
                                 keys=("value", "ms_per_step", "adjoint", "config"))
@@ -269,6 +155,238 @@ def main():
     print(json.dumps(out), flush=True)
     if a.calibrate_copy:
         _copy_kernel(a.calibrate_copy)
+
+
+def trace_summary(path, window):
+    """Per-kernel calls, total and average duration of a rocprofv3 kernel trace (CSV), restricted to
+    the dispatches that start within substeps [first, last); substep k starts at the k-th dispatch of
+    k_body_pre, the first kernel of every substep."""
+    import csv
+    rows = []
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    starts = [t0 for t0, _, n in rows if "k_body_pre" in n]
+    first, last = (int(x) for x in window.split(":")) if window else (0, len(starts))
+    lo = starts[first]
+    hi = starts[last] if last < len(starts) else float("inf")
+    agg = {}
+    for t0, t1, n in rows:
+        if lo <= t0 < hi:
+            short = n.split("(")[0].replace("void ", "")
+            c, tot = agg.get(short, (0, 0))
+            agg[short] = (c + 1, tot + (t1 - t0))
+    span = (max(t1 for t0, t1, n in rows if lo <= t0 < hi) - lo) / 1e6
+    print(f"substeps [{first}, {last}) of {len(starts)}: {span:.3f} ms from the first dispatch to the last end\n")
+    print("| kernel | calls | total ms | avg us |")
+    print("|---|---|---|---|")
+    for n, (c, tot) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{n}` | {c} | {tot / 1e6:.3f} | {tot / c / 1e3:.2f} |")
+
+
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+# kernels timed with HIP events on the stream each is launched on; the tiled solve's region
+# (K_TILED) is not a kernel and never competes for "dominant"
+def _kernel_names(L):
+    return {L.K_SWEEP_VEL: "k_sweep_vel", L.K_SWEEP_POS: "k_sweep_pos", L.K_NARROW: "k_narrow",
+            L.K_JAC_VEL: "k_jac_vel", L.K_TILE_BIG: "k_t_big", L.K_TILE_SMALL: "k_t_small",
+            L.K_TILE_GIANT: "k_t_giant", L.K_ISLANDS: "k_t_islands_lds", L.K_COLOR: "k_t_color"}
+
+
+def read_tile_work(e, L):
+    """The tile kernels' device work counters since me_profile(e, 1): read right after the window,
+    since they keep counting while the events are off."""
+    return [int(v) for v in e.buffer(L.BUF_TILE_WORK, np.int64, (12,))]
+
+
+def kernel_roofline(e, L, solver, V, P, ncol_pairs=None, work=None):
+    """Per-kernel HIP-event times over the profiled window and the roofline of the dominant kernel
+    (largest summed time). Algorithmic bytes of the tile kernels come from the work they counted on
+    the device over the same window (ME_BUF_TILE_WORK): per launch warm + V * vel + P * pos with
+    vel = 20*pairs + 100*dynamic pair ends + 64*contacts, pos = 20*pairs + 100*ends + 52*contacts,
+    warm = pos under ColorCache. Kernels without a byte model report achieved = null."""
+    import ctypes as C
+    Lib = L.lib()
+    prof = {}
+    for kid, nm in _kernel_names(L).items():
+        tot, n = C.c_double(), C.c_int64()
+        L.check(Lib.me_profile_read(e._h, kid, C.byref(tot), C.byref(n)), "me_profile_read")
+        if n.value:
+            prof[nm] = (tot.value, n.value)
+    region = None
+    tot, n = C.c_double(), C.c_int64()
+    L.check(Lib.me_profile_read(e._h, L.K_TILED, C.byref(tot), C.byref(n)), "me_profile_read")
+    if n.value:
+        region = (tot.value, n.value)
+    if work is None:
+        work = read_tile_work(e, L)
+    per_launch, unit_work = {}, {}
+    for k, nm in enumerate(("k_t_big", "k_t_small", "k_t_giant")):
+        if nm not in prof:
+            continue
+        tiles, bp, bd, bc = work[4 * k: 4 * k + 4]
+        nl = max(prof[nm][1], 1)
+        vel = 20 * bp + 100 * bd + 64 * bc
+        pos = 20 * bp + 100 * bd + 52 * bc
+        warm = pos if solver == "color_cache" else 0
+        per_launch[nm] = (warm + V * vel + P * pos) / nl
+        unit_work[nm] = {"tiles": tiles / nl, "pairs": bp / nl, "dynamic_pair_ends": bd / nl, "contacts": bc / nl}
+    if ncol_pairs is not None:
+        vel_b, pos_b, ncol = ncol_pairs
+        per_launch["k_sweep_vel"] = vel_b / ncol
+        per_launch["k_sweep_pos"] = pos_b / ncol
+    roof = None
+    if prof:
+        dom = max(prof, key=lambda k: prof[k][0])
+        avg_s = prof[dom][0] / prof[dom][1] / 1e3
+        b = per_launch.get(dom)
+        ach = None if b is None else b / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": None if ach is None else round(ach, 3),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None if ach is None else round(ach / HBM_PEAK_GBS, 6),
+                "traffic": None, "bytes_per_launch": None if b is None else int(b),
+                "avg_launch_us": round(avg_s * 1e6, 3), "launches_timed": prof[dom][1],
+                "work_per_launch": unit_work.get(dom)}
+    kms = {k: {"ms_total": round(v[0], 4), "launches": v[1], "avg_us": round(1e3 * v[0] / v[1], 3)}
+           for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    if region:
+        kms["(tiled solve region on the step stream)"] = {"ms_total": round(region[0], 4), "launches": region[1],
+                                                          "avg_us": round(1e3 * region[0] / region[1], 3)}
+    return roof, kms
+
+
+SETTLE_PILE = 400
+
+
+def bench_pile(a, large=False):
+    """Config 2 (10 000-box pile) or, with large=True, config 5 (100 000 boxes, mass ratio
+    log-U[1, 100]). The pile first falls and settles for `settle` untimed steps (400 by default:
+    the 10 k pile carries about 32 000 contacts by then), then `warmup` untimed steps, then `steps`
+    timed steps with every kernel of interest timed by HIP events; the run then continues untimed
+    to the config's full length (600 steps; 300 for the large pile) so that the whole run's
+    average is reported beside the settled window."""
+    import torch
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    L = importlib.import_module("3fold-motion-engine_amd._lib")
+    large = large or bool(getattr(a, "large", False))
+    nb = 100000 if large and a.bodies == 10000 else a.bodies
+    full = 300 if large else 600
+    settle = a.settle if a.settle is not None else SETTLE_PILE
+    bodies = (pkg.scenes.box_pile(nb, seed=a.seed, mass_ratio=100.0) if large
+              else pkg.scenes.box_pile(nb, seed=a.seed))
+    cls = {"color_cache": pkg.ColorCacheContactEngine, "pair_chunks": pkg.GraphColoredContactEngine,
+           "jacobi": pkg.RelaxedJacobiContactEngine}[a.solver]
+    kw = dict(vit=a.vit, pit=a.pit, check_every_step=False)
+    if a.solver == "pair_chunks":
+        kw.update(pair_chunks=True, manifold_reduce=True)
+    if a.solver != "jacobi":
+        kw.update(n_colors=a.colors, tiled=bool(a.tiled or large), tile_bodies=a.tile_bodies or 128,
+                  graph_capture=bool(a.graph))
+    e = cls(**kw)
+    e.add_boxes(bodies)
+    dt, S = 1 / 240, a.substeps
+    Lib = L.lib()
+    seg = {}
+
+    def run(n, name):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for i in range(n):
+            e.step(dt, S)
+            if (i + 1) % 100 == 0:
+                torch.cuda.synchronize()
+                print(f"[bench] {name} {i + 1}/{n} {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        seg[name] = time.perf_counter() - t
+
+    run(settle, "settle")
+    run(a.warmup, "warmup")
+    start = e.refresh_counters()
+    print(f"[bench] at timed start (step {settle + a.warmup}): {start}", file=sys.stderr, flush=True)
+    x0, q0, v0, w0 = e.state_numpy()
+    # kernel timing needs per-launch events, so steps taken while profiling run eagerly; with --graph
+    # the timed region replays graphs and a second window of the same length is profiled eagerly
+    if not a.graph:
+        L.check(Lib.me_profile(e._h, 1), "me_profile")
+    run(a.steps, "timed")
+    elapsed = seg["timed"]
+    eager_ms = None
+    if a.graph:
+        L.check(Lib.me_profile(e._h, 1), "me_profile")
+        run(a.steps, "eager_profiled")
+        eager_ms = 1e3 * seg["eager_profiled"] / a.steps
+    L.check(Lib.me_profile(e._h, 0), "me_profile")
+    cnt = e.refresh_counters()
+    work = read_tile_work(e, L)
+    print(f"[bench] at timed end: {cnt}", file=sys.stderr, flush=True)
+    pairs = e.pairs()
+    done = settle + a.warmup + a.steps * (2 if a.graph else 1)
+    if done < full:
+        run(full - done, "rest")
+    ms_per_step = 1e3 * elapsed / a.steps
+    value = nb * a.steps / elapsed
+    ab = algorithmic_bytes(pairs, bodies.invM, a.solver, a.vit, a.pit, S, nb, cnt["n_contacts"])
+    roofline, kms = kernel_roofline(e, L, a.solver, a.vit, a.pit,
+                                    (ab["vel_sweep"], ab["pos_sweep"], max(cnt["n_colors"], 1)), work=work)
+    dbg = e.buffer(L.BUF_DEBUG, np.int64, (16,))
+    if dbg[6] > dbg[0] > 0:
+        ph = np.diff(dbg[:7]).astype(np.float64)
+        fr = ph / ph.sum()
+        print(f"[bench] largest big tile ({dbg[7]} bodies, {dbg[8]} pairs), last substep, share of its time: "
+              f"load {fr[0]:.3f} count {fr[1]:.3f} records {fr[2]:.3f} vel {fr[3]:.3f} pos {fr[4]:.3f} "
+              f"writeback {fr[5]:.3f} ({int(ph.sum())} wall-clock ticks)", file=sys.stderr, flush=True)
+    whole = sum(seg[k] for k in ("settle", "warmup", "timed", "rest") if k in seg)
+    n_whole = settle + a.warmup + a.steps + (full - done if done < full else 0)
+    cpu = cpu_all = None
+    if not a.no_cpu and a.cpu_seconds > 0:
+        model = cpu_model()
+        steps_c, secs_c = cpu_baseline(x0, q0, v0, w0, bodies, a.solver, a.vit, a.pit, S,
+                                       a.colors if a.solver != "jacobi" else 0, a.cpu_seconds)
+        cpu = {"value": round(nb * steps_c / secs_c, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle (C, -O3, contraction off, 1 thread) from the GPU state at the start "
+                         f"of the timed window (step {settle + a.warmup}): {steps_c} steps of the same {nb}-box "
+                         f"pile in {secs_c:.1f} s on {model} ({os.cpu_count()} visible cores)"}
+        if a.solver != "jacobi":
+            threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            steps_m, secs_m = cpu_baseline(x0, q0, v0, w0, bodies, a.solver, a.vit, a.pit, S, a.colors,
+                                           a.cpu_seconds, omp=True)
+            cpu_all = {"value": round(nb * steps_m / secs_m, 1), "unit": "body-steps/s", "cores": threads,
+                       "kind": "port",
+                       "sample": f"the same oracle built -O3 with OpenMP over the pairs of a colour ({threads} "
+                                 f"threads of {model}; contact generation, colouring and integration stay "
+                                 f"serial): {steps_m} steps in {secs_m:.1f} s"}
+    tag = (f"{nb}-box pile" + (", mass ratio log-U[1,100]" if large else "") +
+           f", {a.solver}, {a.colors} colours, S{S} V{a.vit} P{a.pit}, dt 1/240" +
+           (", tiled" if (a.tiled or large) else ", plain launches"))
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box pile)",
+           "config": {"workload": tag, "bodies": nb, "solver": a.solver, "colors": a.colors, "substeps": S,
+                      "vit": a.vit, "pit": a.pit, "tiled": bool(a.tiled or large), "graph": bool(a.graph),
+                      "settle_steps_untimed": settle, "timed_window_steps": [settle + a.warmup,
+                                                                            settle + a.warmup + a.steps],
+                      "contacts_at_timed_start": start["n_contacts"], "pairs_at_timed_start": start["n_pairs"],
+                      "contacts": cnt["n_contacts"], "pairs": cnt["n_pairs"], "colors_used": cnt["n_colors"],
+                      "islands": cnt["n_islands"], "tiles": cnt["n_tiles"]},
+           "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all,
+           f"run_{full}": {"steps": n_whole, "ms_per_step": round(1e3 * whole / n_whole, 4),
+                           "note": "every step from the initial state, timed window included (kernel events on)"},
+           "kernel_ms": kms}
+    if eager_ms is not None:
+        out["kernel_timing_window"] = {"mode": "eager, right after the timed graph replays", "steps": a.steps,
+                                       "ms_per_step": round(eager_ms, 4)}
+    del e
+    return out
 
 
 def _extra(fn, a, keys, **over):
@@ -290,18 +408,6 @@ def _copy_kernel(mib):
     dst.copy_(src)
     torch.cuda.synchronize()
     print(f"[bench] calibration copy: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
-
-
-def _traffic_from_profile(kernel, workload):
-    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE runs,
-    each scaled by the reading for a plain copy of known size) on this workload, if recorded."""
-    f = ROOT / "profiles" / "traffic.json"
-    if not kernel or not f.exists():
-        return None
-    rec = json.loads(f.read_text()).get(kernel)
-    if not rec or rec.get("workload") != workload:
-        return None
-    return rec.get("bytes_per_launch")
 
 
 def bench_adjoint(a):
@@ -531,7 +637,10 @@ def bench_lattice(a):
     runs = []
     runs_wanted = [(r.split(":")[0], r.split(":")[1] == "1") for r in a.lattice_runs.split(",")]
     for solver, tiled in runs_wanted:
-        e = cls[solver](**({} if solver == "jacobi" else {"tiled": tiled}))
+        ekw = {} if solver == "jacobi" else {"tiled": tiled}
+        if solver == "pair_chunks":
+            ekw.update(pair_chunks=True, manifold_reduce=True)
+        e = cls[solver](**ekw)
         for c in cen:
             e.add_body(c)
         for _ in range(10):
@@ -614,81 +723,138 @@ def bench_pile_tuned(a):
 
 
 def bench_batched(a):
-    """256 independent piles (N_s log-uniform in [8, 2048]) in one tiled engine -- one launch sequence
-    per substep for every scene -- against the same scenes stepped one engine after another with
-    plain per-colour launches. Both start from the batched state after the warm-up steps."""
+    """Config 3: 256 independent piles (N_s log-uniform in [8, 2048]) in one tiled engine -- one
+    launch sequence per substep serves every scene, tiles and big-tile lists being the work units --
+    after `settle` untimed steps (400 by default) and `warmup` more, `steps` timed steps with the
+    kernels timed by HIP events. With --autotune the autotuner first picks colours, substeps,
+    iterations and tile size under the 1e-3 m penetration target (trials start from the settled
+    state); when no configuration meets it, the line says so and the default configuration runs.
+    The same window is then replayed as one captured hipGraph per step, and the same scenes, from
+    the same settled state, are stepped one engine after another, once with the same tiled solve
+    and once with plain per-colour launches (the sequential baselines)."""
     import torch
     pkg = importlib.import_module("3fold-motion-engine_amd")
+    L = importlib.import_module("3fold-motion-engine_amd._lib")
     bodies, sizes = pkg.scenes.batched_piles(a.scenes, seed=a.seed)
     N = bodies.n
     dt = 1 / 240
     tb = a.tile_bodies or 224
+    settle = a.settle if a.settle is not None else SETTLE_PILE
     kw = dict(n_colors=a.colors, vit=a.vit, pit=a.pit, tile_bodies=tb)
     S = a.substeps
+    mk = lambda **k: pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, **k)
+
+    def steps(e, n, S_):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            e.step(dt, S_)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    base = mk(**kw)
+    base.add_boxes(bodies)
+    t_settle = steps(base, settle, S)
+    print(f"[bench] batch settled: {settle} steps in {t_settle:.1f}s", file=sys.stderr, flush=True)
+    st = base.state_numpy()
     tune = None
+    tune_out = None
     if a.autotune:
         at = importlib.import_module("3fold-motion-engine_amd.autotune")
-        mk = lambda **k: pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, **k)
-        warm = pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, **kw)
-        warm.add_boxes(bodies)
-        for _ in range(a.warmup):
-            warm.step(dt, S)
-        st = warm.state_numpy()
-        res = at.autotune(mk, bodies, warm_state=st, log=lambda m: print(m, file=sys.stderr, flush=True))
-        if res.best:
+        res = at.autotune(mk, bodies, warm_state=st, steps=a.tune_steps, tile_bodies=(tb,),
+                          log=lambda m: print(m, file=sys.stderr, flush=True))
+        best_seen = min(res.trials, key=lambda t: t.max_pen) if res.trials else None
+        tune_out = {"target_m": 1e-3, "trials": len(res.trials), "trial_steps": a.tune_steps,
+                    "met_target": res.best is not None}
+        if res.best is not None:
             tune = res.best
-            kw = res.as_kwargs()
-            S = tune.substeps
-        del warm
-    e = pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, graph_capture=bool(a.graph), **kw)
-    e.add_boxes(bodies)
-    for i in range(a.warmup):
-        e.step(dt, S)
-    torch.cuda.synchronize()
-    e.refresh_counters()
+            tune_out.update(chosen={"colours": tune.colours, "substeps": tune.substeps, "vit": tune.vit,
+                                    "pit": tune.pit, "tile_bodies": tune.tile_bodies},
+                            max_penetration_m=tune.max_pen)
+        else:
+            tune_out.update(note="no configuration of the search met the 1e-3 m target; the default "
+                                 "configuration runs", smallest_max_penetration_m=best_seen.max_pen if best_seen else None,
+                            at={"colours": best_seen.colours, "substeps": best_seen.substeps,
+                                "vit": best_seen.vit} if best_seen else None)
+    if tune is not None:
+        kw = res.as_kwargs()
+        S = res.substeps
+        del base
+        base = mk(**kw)
+        base.add_boxes(bodies)
+        steps(base, settle, S)
+    e = base
+    steps(e, a.warmup, S)
+    start = e.refresh_counters()
     x0, q0, v0, w0 = e.state_numpy()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        e.step(dt, S)
-    torch.cuda.synchronize()
-    t_b = (time.perf_counter() - t0) / a.steps
+    Lib = L.lib()
+    L.check(Lib.me_profile(e._h, 1), "me_profile")
+    t_b = steps(e, a.steps, S) / a.steps
+    L.check(Lib.me_profile(e._h, 0), "me_profile")
     cnt = e.refresh_counters()
-    # sequential plain launches: one engine per scene, same state (skipped with --seq-rounds 0)
+    roofline, kms = kernel_roofline(e, L, "color_cache", kw["vit"], kw["pit"], work=read_tile_work(e, L))
+    pen_now = float(e.buffer_tensor(L.BUF_PEN, np.float32, cnt["n_solved"]).max().item()) if cnt["n_solved"] else 0.0
+
+    def from_state(eng):
+        eng.add_boxes(bodies)
+        eng._build()
+        with torch.no_grad():
+            for name, arr in zip(("x", "q", "v", "w"), (x0, q0, v0, w0)):
+                getattr(eng, name).copy_(torch.from_numpy(np.ascontiguousarray(arr)).to(eng.x.device))
+        return eng
+
+    # the same window replayed as captured graphs (state restored, warm impulses rebuilt by a few steps)
+    g = from_state(pkg.ColorCacheContactEngine(tiled=True, check_every_step=False, graph_capture=True, **kw))
+    steps(g, 3, S)
+    t_g = steps(g, a.steps, S) / a.steps
+    del g
+    # sequential baselines: one engine per scene, from the settled state
     offs = np.concatenate([[0], np.cumsum(sizes)])
-    engines = []
-    for k in range(a.scenes if a.seq_rounds > 0 else 0):
-        sl = slice(offs[k], offs[k + 1])
-        part = pkg.bodies.BodySet(x=x0[sl].copy(), q=q0[sl].copy(), v=v0[sl].copy(), w=w0[sl].copy(),
-                                  invM=bodies.invM[sl], IbInv=bodies.IbInv[sl], kin=bodies.kin[sl],
-                                  half=bodies.half[sl], mu=bodies.mu[sl],
-                                  scene=np.zeros(offs[k + 1] - offs[k], np.int32), M=bodies.M[sl])
-        ek = pkg.ColorCacheContactEngine(tiled=False, check_every_step=False, **kw)
-        ek.add_boxes(part)
-        ek.step(dt, S)
-        engines.append(ek)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for r in range(a.seq_rounds):
-        for ek in engines:
+    seq = {}
+    for label, tiled in (("tiled", True), ("plain", False)):
+        if a.seq_rounds <= 0:
+            break
+        engines = []
+        for k in range(a.scenes):
+            sl = slice(offs[k], offs[k + 1])
+            part = pkg.bodies.BodySet(x=x0[sl].copy(), q=q0[sl].copy(), v=v0[sl].copy(), w=w0[sl].copy(),
+                                      invM=bodies.invM[sl], IbInv=bodies.IbInv[sl], kin=bodies.kin[sl],
+                                      half=bodies.half[sl], mu=bodies.mu[sl],
+                                      scene=np.zeros(offs[k + 1] - offs[k], np.int32), M=bodies.M[sl])
+            ek = pkg.ColorCacheContactEngine(tiled=tiled, check_every_step=False, **kw)
+            ek.add_boxes(part)
             ek.step(dt, S)
-    torch.cuda.synchronize()
-    t_s = (time.perf_counter() - t0) / a.seq_rounds if a.seq_rounds > 0 else float("nan")
+            engines.append(ek)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(a.seq_rounds):
+            for ek in engines:
+                ek.step(dt, S)
+        torch.cuda.synchronize()
+        seq[label] = (time.perf_counter() - t0) / a.seq_rounds
+        print(f"[bench] sequential {label}: {1e3 * seq[label]:.1f} ms per round of {a.scenes} scenes",
+              file=sys.stderr, flush=True)
+        del engines
+        torch.cuda.empty_cache()
+    sq = lambda k: ({"value": round(N / seq[k], 1), "unit": "body-steps/s", "ms_per_round": round(1e3 * seq[k], 3)}
+                    if k in seq else None)
     out = {"metric": METRIC, "value": round(N / t_b, 1), "unit": "body-steps/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": round(1e3 * t_b, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box piles)",
            "config": {"workload": f"{a.scenes} independent box piles, N_s log-uniform [8, 2048], one tiled launch "
                                   f"sequence per substep", "bodies": int(N), "scenes": a.scenes,
                       "colors": kw.get("n_colors"), "substeps": S, "vit": kw.get("vit"), "pit": kw.get("pit"),
-                      "autotuned": tune is not None, "graph": bool(a.graph), "pairs": cnt["n_pairs"], "islands": cnt["n_islands"],
-                      "tile_bodies": kw.get("tile_bodies"), "tiles": cnt["n_tiles"],
-                      "max_tile_bodies": cnt["max_tile_bodies"]},
-           "sequential_plain_launches": ({"value": round(N / t_s, 1), "unit": "body-steps/s",
-                                          "ms_per_round": round(1e3 * t_s, 3)} if a.seq_rounds > 0 else None),
-           "speedup_vs_sequential": round(t_s / t_b, 2) if a.seq_rounds > 0 else None}
-    if tune is not None:
-        out["autotune"] = {"colours": tune.colours, "substeps": tune.substeps, "vit": tune.vit, "pit": tune.pit,
-                           "max_pen": tune.max_pen}
+                      "autotuned": tune is not None, "settle_steps_untimed": settle,
+                      "contacts_at_timed_start": start["n_contacts"], "pairs": cnt["n_pairs"],
+                      "islands": cnt["n_islands"], "tile_bodies": kw.get("tile_bodies"), "tiles": cnt["n_tiles"],
+                      "max_tile_bodies": cnt["max_tile_bodies"], "max_penetration_m_at_window_end": pen_now},
+           "roofline": roofline, "kernel_ms": kms,
+           "graph_replay": {"ms_per_step": round(1e3 * t_g, 4), "value": round(N / t_g, 1),
+                            "note": "the same window, each step one captured hipGraph"},
+           "autotune": tune_out,
+           "sequential_tiled": sq("tiled"), "sequential_plain_launches": sq("plain"),
+           "speedup_vs_sequential_tiled": round(seq["tiled"] / t_b, 2) if "tiled" in seq else None,
+           "speedup_vs_sequential_plain": round(seq["plain"] / t_b, 2) if "plain" in seq else None}
     return out
 
 

@@ -116,6 +116,9 @@ enum {
     ME_BUF_MU,         /* float [C]   friction coefficient of the contact's pair */
     ME_BUF_XSNAP,      /* float [N,3] centres at the start of the last substep */
     ME_BUF_WSNAP,      /* float [N,3] angular velocity before the contact solve */
+    ME_BUF_TILE_WORK,  /* int64 [12]  tile work since me_profile(e, 1), four counts (tiles, pairs, dynamic
+                                      pair ends, contacts) for each of ME_K_TILE_BIG, ME_K_TILE_SMALL,
+                                      ME_K_TILE_GIANT, in that order (the first four are ME_BUF_BIG_WORK) */
     ME_BUF_COUNT_
 };
 
@@ -172,14 +175,19 @@ enum {
     ME_K_SWEEP_VEL = 0,  /* per-colour pair-chunk velocity sweep */
     ME_K_SWEEP_POS,      /* per-colour pair-chunk position sweep */
     ME_K_NARROW,         /* box narrow phase */
-    ME_K_TILED,          /* island-tiled solve (one launch per substep) */
+    ME_K_TILED,          /* the tiled solve's sweep region on the step's stream (k_t_prep, k_t_small and the
+                            wait for the side stream's big and giant tiles): a region, not one kernel */
     ME_K_JAC_VEL,        /* Jacobi velocity pass */
-    ME_K_TILE_BIG,       /* big-tile kernel of the tiled solve (islands beyond one small tile) */
+    ME_K_TILE_BIG,       /* k_t_big: big tiles (islands beyond one small tile), on the side stream */
+    ME_K_TILE_SMALL,     /* k_t_small: small tiles, on the step's stream */
+    ME_K_TILE_GIANT,     /* k_t_giant: islands beyond the big tile's LDS, on the side stream */
+    ME_K_ISLANDS,        /* k_t_islands_lds: island build of scenes up to 12 288 bodies */
+    ME_K_COLOR,          /* k_t_color: JP colouring of the tiles */
     ME_K_COUNT_
 };
 
-/* Time launches of the kernels above with HIP events on the step's stream (the first 8192
- * launches per kernel after enabling). Not part of the reference interface: measurement only. */
+/* Time launches of the kernels above with HIP events on the stream each is launched on (the first
+ * 8192 launches per kernel after enabling). Not part of the reference interface: measurement only. */
 int me_profile(me_engine *e, int enable);
 int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launches);
 

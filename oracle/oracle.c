@@ -391,7 +391,7 @@ static void face_poly(v3 c, const m33 *R, v3 h, int a, real s, v3 out[4]) {
     static const real S0[4] = {-1.0f, -1.0f, 1.0f, 1.0f};
     static const real S1[4] = {-1.0f, 1.0f, 1.0f, -1.0f};
     for (int k = 0; k < 4; ++k) {
-        real l[3];
+        real l[3] = {0, 0, 0};  /* a, o0, o1 are a permutation of 0, 1, 2 */
         l[a] = s * hget(h, a);
         l[o0] = S0[k] * hget(h, o0);
         l[o1] = S1[k] * hget(h, o1);

@@ -31,6 +31,9 @@ def _bead_engine(kind, bodyset, dens, **kw):
     pkg = _pkg()
     cls = {"pair_chunks": pkg.GraphColoredContactEngine, "color_cache": pkg.ColorCacheContactEngine,
            "jacobi": pkg.RelaxedJacobiContactEngine}[kind]
+    if kind == "pair_chunks":  # the pair-chunk solve unless a test asks for the per-contact one
+        pc = kw.setdefault("pair_chunks", True)
+        kw.setdefault("manifold_reduce", pc)
     e = cls(vit=40, pit=10, **kw)
     for c, d, k in zip(bodyset.x, dens, bodyset.kin):
         e.add_body([float(t) for t in c], density=d, kin=bool(k))

@@ -26,6 +26,9 @@ def _engine(kind, **kw):
     pkg = _pkg()
     cls = {"jacobi": pkg.RelaxedJacobiContactEngine, "pair_chunks": pkg.GraphColoredContactEngine,
            "color_cache": pkg.ColorCacheContactEngine}[kind]
+    if kind == "pair_chunks":  # the pair-chunk solve unless a test asks for the per-contact one
+        pc = kw.setdefault("pair_chunks", True)
+        kw.setdefault("manifold_reduce", pc)
     return cls(**kw)
 
 
