@@ -110,3 +110,21 @@ def concat(parts) -> BodySet:
     cat = lambda k: np.concatenate([getattr(b, k) for b in parts])
     return BodySet(**{k: cat(k) for k in ("x", "q", "v", "w", "invM", "IbInv", "kin", "half", "mu",
                                            "scene", "M")})
+
+
+def ramp_box(deg, mu=0.5, half=(0.15, 0.1, 0.1), slab_half=(2.0, 0.5, 0.1), gap=1e-4, density=1000.0) -> BodySet:
+    """A box resting on a kinematic slab tilted by `deg` about y, the physics check of the reference's
+    ramp (tests/test_contact_and_grasp.py:20-31: a 0.3 x 0.2 x 0.2 box, mu 0.5, downhill direction
+    (-cos deg, 0, -sin deg)). Body 0 is the box, body 1 the slab; both carry mu, so the pair's mu is
+    mu. The slab's centre is high enough that it clears the ground plane. No random draws."""
+    th = math.radians(deg)
+    n = np.array([-math.sin(th), 0.0, math.cos(th)])       # slab top normal
+    q = np.array([0.0, math.sin(-th / 2), 0.0, math.cos(-th / 2)])  # rotation about y by -deg
+    cs = np.array([0.0, 0.0, slab_half[0] * math.sin(th) + slab_half[2] + 0.2])
+    cb = cs + (slab_half[2] + half[2] + gap) * n
+    return box_bodies([cb, cs], [half, slab_half], density, kin=[0, 1], quats=[q, q], mu=mu)
+
+
+def ramp_downhill(deg):
+    th = math.radians(deg)
+    return np.array([-math.cos(th), 0.0, -math.sin(th)])

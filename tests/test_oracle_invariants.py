@@ -243,3 +243,33 @@ def test_contacts_sorted_and_pairs_grouped(oracle):
     assert np.all(pr["pcount"] >= 1) and np.all(pr["pcount"] <= 4)
     assert np.array_equal(pr["pstart"][1:], np.cumsum(pr["pcount"])[:-1])
     assert np.all(c["bi"] < np.where(c["bj"] < 0, np.iinfo(np.int32).max, c["bj"]))
+
+
+@pytest.mark.parametrize("solver,per_contact", [("jacobi", False), ("color_cache", False), ("pair_chunks", False),
+                                                ("pair_chunks", True)])
+def test_ramp_stick_and_slide_on_the_oracle(oracle, solver, per_contact):
+    """The ramp of tests/test_contact_and_grasp.py:20-31 (mu 0.5, 20 and 35 degrees) on the oracle: the
+    normal-derived friction frame (Jacobi, ColorCache) sticks below atan(0.5) and slides above it by
+    a t^2 / 2, a = g (sin - mu cos); the slip-direction Gauss-Seidel sweep of the reference's coloured
+    engine (contact_engine_gpu_colored.py:188-242) holds the box at both angles, because each
+    iteration re-aims the accumulated tangential impulse along the current slip."""
+    import importlib
+    import math
+    sc = importlib.import_module("3fold-motion-engine_amd.scenes")
+    moved = {}
+    for deg in (20, 35):
+        b = sc.ramp_box(deg, mu=0.5)
+        o = oracle.OracleWorld(b, solver, vit=40 if solver != "pair_chunks" or not per_contact else 20, pit=20,
+                               per_contact=per_contact)
+        x0 = o.state()[0][0].copy()
+        for _ in range(200):
+            o.step(1 / 240, 2)
+        moved[deg] = float((o.state()[0][0] - x0) @ sc.ramp_downhill(deg))
+    assert moved[20] < 0.02
+    if solver == "pair_chunks":
+        assert moved[35] < 0.02
+    else:
+        th = math.radians(35)
+        t = 200 / 240 - math.sqrt(2e-4 / 9.81)
+        pred = 0.5 * 9.81 * (math.sin(th) - 0.5 * math.cos(th)) * t * t
+        assert abs(moved[35] - pred) / pred < 0.05, (moved, pred)
