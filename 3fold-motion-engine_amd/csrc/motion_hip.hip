@@ -6,6 +6,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1264,10 +1265,9 @@ int me_ncp_workspace_size(int32_t n_env, int32_t n_c, int32_t n_dir, size_t *byt
     return ME_OK;
 }
 
-int me_ncp_sensitivity(int32_t n_env, int32_t n_c, int32_t n_dir, const double *G, const double *b,
-                       const double *mu, const double *lam, const double *dirs, const int32_t *labels_in,
-                       double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
-                       int32_t *labels, int32_t *status, void *workspace, size_t bytes, void *stream) {
+static int ncp_check(int32_t n_env, int32_t n_c, int32_t n_dir, const double *G, const double *b, const double *mu,
+                     const double *lam, const double *dirs, double tol_lam, double tol_rel, double rcond,
+                     double *dlam_dmu, double *dlam_db, int32_t *labels, int32_t *status) {
     if (n_env < 1 || n_c < 1 || n_c > NCP_MAXC || n_dir < 0)
         return fail(ME_EARG, "need n_env >= 1, 1 <= n_c <= 64, n_dir >= 0");
     if (!dirs && n_dir != 3 * n_c) return fail(ME_EARG, "without directions n_dir must be 3 * n_c (the identity)");
@@ -1275,17 +1275,86 @@ int me_ncp_sensitivity(int32_t n_env, int32_t n_c, int32_t n_dir, const double *
         return fail(ME_EARG, "NULL argument");
     if (!(tol_lam >= 0.0) || !(tol_rel >= 0.0) || !(rcond > 0.0 && rcond < 1.0))
         return fail(ME_EARG, "need tol_lam >= 0, tol_rel >= 0, 0 < rcond < 1");
-    size_t m = ncp_matrix_bytes(n_c, n_dir), need = 0;
-    (void)me_ncp_workspace_size(n_env, n_c, n_dir, &need);
-    if (need > 0 && (!workspace || bytes < need))
-        return fail(ME_EARG, "workspace too small: need " + std::to_string(need) + " bytes");
+    return ME_OK;
+}
+
+static int ncp_launch(int32_t n_env, int32_t n_c, int32_t n_dir, const double *G, const double *b, const double *mu,
+                      const double *lam, const double *dirs, const int32_t *labels_in, double tol_lam, double tol_rel,
+                      double rcond, double *dlam_dmu, double *dlam_db, int32_t *labels, int32_t *status,
+                      double *scratch_rows, void *stream) {
+    size_t m = ncp_matrix_bytes(n_c, n_dir);
     int lds = m <= kNcpLds ? 1 : 0;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_ncp_sensitivity, dim3(n_env), dim3(NCP_THREADS), lds ? m : 0, s, n_env, n_c, n_dir, G, b, mu,
-                       lam, dirs, labels_in, tol_lam, tol_rel, rcond, dlam_dmu, dlam_db, labels, status,
-                       (double *)workspace, lds);
+                       lam, dirs, labels_in, tol_lam, tol_rel, rcond, dlam_dmu, dlam_db, labels, status, scratch_rows,
+                       lds);
     LAUNCH_CHECK();
     return ME_OK;
+}
+
+int me_ncp_sensitivity(int32_t n_env, int32_t n_c, int32_t n_dir, const double *G, const double *b,
+                       const double *mu, const double *lam, const double *dirs, const int32_t *labels_in,
+                       double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
+                       int32_t *labels, int32_t *status, void *workspace, size_t bytes, void *stream) {
+    int rc = ncp_check(n_env, n_c, n_dir, G, b, mu, lam, dirs, tol_lam, tol_rel, rcond, dlam_dmu, dlam_db, labels,
+                       status);
+    if (rc) return rc;
+    size_t need = 0;
+    (void)me_ncp_workspace_size(n_env, n_c, n_dir, &need);
+    if (need > 0 && (!workspace || bytes < need))
+        return fail(ME_EARG, "workspace too small: need " + std::to_string(need) + " bytes");
+    return ncp_launch(n_env, n_c, n_dir, G, b, mu, lam, dirs, labels_in, tol_lam, tol_rel, rcond, dlam_dmu, dlam_db,
+                      labels, status, (double *)workspace, stream);
+}
+
+// The no-allocation variant's scratch: the augmented matrices of systems beyond the LDS limit, one
+// fixed array per GPU for the life of the process. Host code never passes the array's name to a
+// kernel (in host code it is only a stand-in symbol): its device address is looked up once per
+// device with hipGetSymbolAddress, on the entry point's first call there, and kept per device.
+static __device__ __align__(256) unsigned char scratch[ME_NCP_STATIC_BYTES];
+
+namespace {
+const int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ME_ERUNTIME, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !ptr)
+            return fail(ME_ERUNTIME, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return ME_OK;
+}
+}  // namespace
+
+int me_ncp_sensitivity_noalloc(int32_t n_env, int32_t n_c, int32_t n_dir, const double *G, const double *b,
+                               const double *mu, const double *lam, const double *dirs, const int32_t *labels_in,
+                               double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
+                               int32_t *labels, int32_t *status, void *stream) {
+    int rc = ncp_check(n_env, n_c, n_dir, G, b, mu, lam, dirs, tol_lam, tol_rel, rcond, dlam_dmu, dlam_db, labels,
+                       status);
+    if (rc) return rc;
+    size_t need = 0;
+    (void)me_ncp_workspace_size(n_env, n_c, n_dir, &need);
+    if (need > (size_t)ME_NCP_STATIC_BYTES)
+        return fail(ME_ECAPACITY, "me_ncp_sensitivity_noalloc: " + std::to_string(need) + " bytes of matrices exceed "
+                                  "the fixed scratch of " + std::to_string((size_t)ME_NCP_STATIC_BYTES) +
+                                  " bytes; use fewer environments per call or me_ncp_sensitivity");
+    void *ptr = nullptr;
+    if (need > 0) {
+        rc = scratch_address(&ptr);
+        if (rc) return rc;
+    }
+    return ncp_launch(n_env, n_c, n_dir, G, b, mu, lam, dirs, labels_in, tol_lam, tol_rel, rcond, dlam_dmu, dlam_db,
+                      labels, status, (double *)ptr, stream);
 }
 
 int me_state_size(me_engine *e, size_t *bytes) {

@@ -223,6 +223,19 @@ int me_ncp_sensitivity(int32_t n_env, int32_t n_c, int32_t n_dir, const double *
                        double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
                        int32_t *labels, int32_t *status, void *workspace, size_t bytes, void *stream);
 
+/* me_ncp_sensitivity without a workspace argument, for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart
+ * in the reference. The matrices of systems beyond the LDS limit go into one fixed device array of
+ * ME_NCP_STATIC_BYTES per GPU, held until the process ends; a call whose me_ncp_workspace_size exceeds
+ * it fails with ME_ECAPACITY before any launch. The call allocates nothing (no hipMalloc, hipFree or
+ * allocating library routine). Every call on a GPU shares that GPU's array, so calls that overlap on
+ * one GPU (other streams, other threads) race; serialise them. Same results as me_ncp_sensitivity. */
+#define ME_NCP_STATIC_BYTES (16u << 20)
+int me_ncp_sensitivity_noalloc(int32_t n_env, int32_t n_c, int32_t n_dir, const double *G, const double *b,
+                               const double *mu, const double *lam, const double *dirs, const int32_t *labels_in,
+                               double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
+                               int32_t *labels, int32_t *status, void *stream);
+
 /* Exact-cone NCP projected Gauss-Seidel, the reference's NCPSolverGPU (ncp/ncp_gpu.py:783-1069):
  * lambda_c <- proj_K(lambda_c - rho_c (u_c + Gamma(u_c))) over a deterministic colouring, de Saxce
  * correction on or off, optional TGS soft-step normal row (ncp_gpu.py:491-557), per-body
