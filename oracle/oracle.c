@@ -430,15 +430,20 @@ static void clip_poly(const poly_t *in, v3 sn, real d, int plane, poly_t *out) {
 }
 
 /* deepest first (stable by gap), then greedily the candidate farthest (squared distance) from
- * those kept; returns kept indices ordered by ascending feature code */
+ * those kept; returns kept indices ordered by ascending feature code.
+ * OR_PLANT_REDUCE_UNSORTED / OR_PLANT_REDUCE_NEAREST exist so that a test can build the oracle with a
+ * planted mistake (start from the first point instead of the deepest; take the nearest candidate
+ * instead of the farthest) and show that its closed-form case rejects it. */
 static int reduce4(int n, const v3 *p, const real *gap, const int *code, int *keep) {
     int ord[MAXPOLY];
     for (int i = 0; i < n; ++i) ord[i] = i;
+#ifndef OR_PLANT_REDUCE_UNSORTED
     for (int a = 1; a < n; ++a) {
         int t = ord[a], b = a - 1;
         while (b >= 0 && gap[ord[b]] > gap[t]) { ord[b + 1] = ord[b]; --b; }
         ord[b + 1] = t;
     }
+#endif
     int nk;
     if (n <= 4) {
         for (int i = 0; i < n; ++i) keep[i] = ord[i];
@@ -456,7 +461,11 @@ static int reduce4(int n, const v3 *p, const real *gap, const int *code, int *ke
                     real d2 = vdot(e, e);
                     if (d2 < md) md = d2;
                 }
+#ifdef OR_PLANT_REDUCE_NEAREST
+                if (best < 0 || md < bd) { bd = md; best = r; }
+#else
                 if (md > bd) { bd = md; best = r; }
+#endif
             }
             used[best] = 1; keep[nk++] = ord[best];
         }
@@ -590,10 +599,17 @@ int or_box_ground(const real *x, const real *q, const real *h, real margin, real
     return k;
 }
 
+/* friction of a body pair: the geometric mean of the two bodies' coefficients (equal ones kept as
+ * they are); the ground takes the body's own. OR_PLANT_MU_MEAN (the arithmetic mean) exists so that a
+ * test can show that the slide distance v0^2 / (2 mu g) rejects it. */
 static inline real pair_mu(const or_world *w, int a, int b) {
     if (b < 0) return w->mu[a];
     real ma = w->mu[a], mb = w->mu[b];
+#ifdef OR_PLANT_MU_MEAN
+    return 0.5f * (ma + mb);
+#else
     return ma == mb ? ma : RSQRT(ma * mb);
+#endif
 }
 
 /* body AABB half extents ext_k = sum_j |R_kj| h_j */
@@ -1084,7 +1100,11 @@ static int color_pairs(or_world *w) {
             /* class with the largest cnt/sub (compare cnt_a*sub_b > cnt_b*sub_a), lowest first */
             int bc = 0;
             for (int c = 1; c < cur; ++c)
+#ifdef OR_PLANT_SPLIT_BY_COUNT /* planted mistake for a test: the largest class, whatever its sub */
+                if (cnt[c] > cnt[bc]) bc = c;
+#else
                 if ((int64_t)cnt[c] * sub[bc] > (int64_t)cnt[bc] * sub[c]) bc = c;
+#endif
             sub[bc]++;
         }
         for (int c = 0; c < cur; ++c) base[c + 1] = base[c] + sub[c];
