@@ -165,9 +165,10 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     int N = c.n_bodies, P = c.n_beads > 0 ? c.n_beads : 0;
     long long NP = (long long)N * (P > 0 ? P : 1);
     e->N = N; e->P = P; e->NP = (int)NP;
-    // bead geometry: the Jacobi solver takes contacts from a grid of bead points; the coloured
-    // solvers take them per body pair from the body broad phase (pair-major, then reduced)
-    bool bp = P > 0 && c.solver != ME_SOLVER_JACOBI;
+    // bead geometry: the Jacobi solver and the per-contact coloured solver take contacts from a grid
+    // of bead points (feature-major, unreduced); the pair-chunk solvers take them per body pair from the
+    // body broad phase (pair-major, then reduced)
+    bool bp = P > 0 && c.solver != ME_SOLVER_JACOBI && !(c.flags & ME_FLAG_PER_CONTACT);
     bool point_grid = P > 0 && !bp;
     e->bead_pairs = bp;
     e->bp_words = bp ? (P * P + 63) / 64 : 0;
@@ -292,9 +293,9 @@ int validate(const me_config *c) {
     if (c->solver < 0 || c->solver > 2) return fail(ME_EARG, "solver must be 0 (jacobi), 1 (pair chunks) or 2 (color cache)");
     if (c->vit < 0 || c->pit < 0) return fail(ME_EARG, "iteration counts must be >= 0");
     if (c->n_beads < 0) return fail(ME_EARG, "n_beads must be >= 0");
-    if (c->n_beads > 0 && c->solver != ME_SOLVER_JACOBI && (c->flags & ME_FLAG_PER_CONTACT))
-        return fail(ME_EARG, "bead geometry under a coloured solver needs pair chunks (the manifold reduction to 4 "
-                             "points per body pair); per-contact colouring of raw bead contacts is not built");
+    if (c->solver == ME_SOLVER_COLOR_CACHE && (c->flags & ME_FLAG_PER_CONTACT))
+        return fail(ME_EARG, "ColorCache solves body-pair chunks; per-contact colouring is the pair-chunk solver's "
+                             "pair_chunks=False mode");
     if (c->n_beads > 64) return fail(ME_EARG, "n_beads must be <= 64");
     if (c->n_colors < 0 || c->n_colors > ME_NCOLMAX) return fail(ME_EARG, "n_colors must be in [0, 512]");
     if ((long long)c->n_bodies * (c->n_beads > 0 ? c->n_beads : 1) > (1LL << 30))
@@ -423,6 +424,11 @@ int generate(me_engine *e, hipStream_t s) {
                            e->bstart, e->bcount, e->spos, e->T, R, e->b.mu, e->cand_n, e->cand_off, e->capC, e->Cc,
                            e->istat);
         hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP, 1);
+        // per-contact colouring (GraphColoredContactEngine pair_chunks=False): every raw bead contact its
+        // own chunk, in the feature-key order the emit pass wrote
+        if (e->cfg.solver != ME_SOLVER_JACOBI)
+            hipLaunchKernelGGL(k_pairs_per_contact, dim3(nblk(e->capC)), dim3(ME_THREADS), 0, s, e->capC, e->istat,
+                               e->Cc.bi, e->Cc.bj, e->Pp);
         LAUNCH_CHECK();
         return ME_OK;
     }

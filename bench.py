@@ -86,7 +86,8 @@ def main():
                          "adjoint, the NCP sensitivity, the reference's lattice fixture and the exact-cone PGS, "
                          "and attach their results to the JSON line")
     ap.add_argument("--scenes", type=int, default=256)
-    ap.add_argument("--lattice-runs", default="color_cache:1,color_cache:0,pair_chunks:1,pair_chunks:0,jacobi:0",
+    ap.add_argument("--lattice-runs",
+                    default="color_cache:1,color_cache:0,pair_chunks:1,pair_chunks:0,jacobi:0,colored:1,colored:0",
                     help="lattice workload: engines to run, solver:tiled pairs")
     ap.add_argument("--seq-rounds", type=int, default=3)
     ap.add_argument("--autotune", action="store_true")
@@ -618,8 +619,9 @@ def bench_pgs(a):
 
 
 # published wall-clock ms/step of the reference engines on this fixture (all RTX 5070): README.md:37
-# (ColorCache), docs/RUNNING.md:738 (pair chunks and Jacobi, first of two runs)
-LATTICE_REF_MS = {"color_cache": 3.7, "pair_chunks": 4.931, "jacobi": 5.591}
+# (ColorCache), docs/RUNNING.md:738 (pair chunks and Jacobi, first of two runs), docs/RUNNING.md:639
+# (the default coloured engine: per-contact colouring of the unreduced contacts, uncaptured)
+LATTICE_REF_MS = {"color_cache": 3.7, "pair_chunks": 4.931, "jacobi": 5.591, "colored": 30.629}
 
 
 def bench_lattice(a):
@@ -633,7 +635,7 @@ def bench_lattice(a):
     pkg = importlib.import_module("3fold-motion-engine_amd")
     cen = pkg.scenes.lattice(10000)
     cls = {"color_cache": pkg.ColorCacheContactEngine, "pair_chunks": pkg.GraphColoredContactEngine,
-           "jacobi": pkg.RelaxedJacobiContactEngine}
+           "jacobi": pkg.RelaxedJacobiContactEngine, "colored": pkg.GraphColoredContactEngine}
     runs = []
     runs_wanted = [(r.split(":")[0], r.split(":")[1] == "1") for r in a.lattice_runs.split(",")]
     for solver, tiled in runs_wanted:

@@ -28,11 +28,14 @@ def test_workspace_size_and_argument_errors():
     bad = lib_mod.Config(7, 10, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 0, 256, 0)
     assert L.me_workspace_size(ctypes.byref(bad), ctypes.byref(n)) == lib_mod.ME_EARG
     assert b"solver" in L.me_last_error()
-    # bead geometry under a coloured solver: pair chunks with the manifold reduction, not per contact
+    # bead geometry under a coloured solver: pair chunks with the manifold reduction, or per contact
     bead_cc = lib_mod.Config(2, 10, 18, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 0, 256, 0)
     assert L.me_workspace_size(ctypes.byref(bead_cc), ctypes.byref(n)) == 0
     bead_pc = lib_mod.Config(1, 10, 18, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 0, 256, lib_mod.FLAG_PER_CONTACT)
-    assert L.me_workspace_size(ctypes.byref(bead_pc), ctypes.byref(n)) == lib_mod.ME_EARG
+    assert L.me_workspace_size(ctypes.byref(bead_pc), ctypes.byref(n)) == 0
+    # ColorCache always solves pair chunks
+    cc_pc = lib_mod.Config(2, 10, 18, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 0, 256, lib_mod.FLAG_PER_CONTACT)
+    assert L.me_workspace_size(ctypes.byref(cc_pc), ctypes.byref(n)) == lib_mod.ME_EARG
     assert b"per-contact" in L.me_last_error()
     # whole-step graphs need the tiled coloured solve
     g_plain = lib_mod.Config(2, 100, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64, lib_mod.FLAG_GRAPH)

@@ -211,3 +211,72 @@ def test_jacobi_bead_jumble_with_kinematic_bodies_matches_oracle(oracle):
         o.step(1 / 240)
         _assert_state_equal(e, o, f"step {s}")
         assert e.n_contacts == o.counters()["n_contacts"], f"step {s}"
+
+
+# ── the reference's default coloured engine: per-contact colouring of the unreduced bead contacts ──
+# GraphColoredContactEngine() (contact_engine_gpu_colored.py:798-809): pair_chunks=False,
+# manifold_reduce=False, vit 20, pit 20; contacts in feature-key order fa*(N*P+1) + fb + 1 (:903-917),
+# Jones-Plassmann over contacts (:1105-1136), one f_gs_vel / f_gs_pos per contact (:188-320).
+
+def _per_contact_engine(bodyset, dens, **kw):
+    pkg = _pkg()
+    e = pkg.GraphColoredContactEngine(**kw)
+    for c, d, k in zip(bodyset.x, dens, bodyset.kin):
+        e.add_body([float(t) for t in c], density=d, kin=bool(k))
+    assert e.vit == 20 and e.pit == 20 and e.per_contact
+    return e
+
+
+def _first_contact_counts(meta, name):
+    """The archive's generated counts up to and including its first step with contacts: until then
+    every engine has the same free-fall trajectory, so the same contacts are generated."""
+    counts = meta["records"][name]["contacts"]
+    first = next(i for i, c in enumerate(counts) if c > 0)
+    return counts[:first + 1]
+
+
+@pytest.mark.parametrize("scene,steps,tiled,graph", [("stack8", 120, False, False), ("stack8", 120, True, False),
+                                                     ("lattice1000", 40, True, False),
+                                                     ("lattice1000", 40, True, True),
+                                                     ("lattice1000", 40, False, False)])
+def test_per_contact_bead_engine_matches_oracle(oracle, scene, steps, tiled, graph):
+    _say(f"per-contact coloured bead {scene} x{steps} tiled={tiled} graph={graph}")
+    pkg = _pkg()
+    meta = json.loads((GOLD / "reference_trajectories.json").read_text())
+    early = _first_contact_counts(meta, f"jacobi__{scene}")
+    cen = pkg.scenes.stack(8) if scene == "stack8" else pkg.scenes.lattice(1000)
+    n = len(cen)
+    bs = pkg.bodies.bead_bodies(cen, [700.0] * n, [0] * n)
+    e = _per_contact_engine(bs, [700.0] * n, tiled=tiled, graph_capture=graph)
+    o = oracle.OracleWorld(bs, "pair_chunks", vit=20, pit=20, per_contact=True)
+    for s in range(steps):
+        e.step(1 / 240)
+        o.step(1 / 240)
+        _assert_state_equal(e, o, f"step {s}")
+        _assert_counters_equal(e, o, f"step {s}")
+        if s < len(early):
+            assert e.n_contacts == early[s], f"step {s}: generated {e.n_contacts} archive {early[s]}"
+        if s in (len(early) - 1, steps // 2, steps - 1):
+            cg, co = e.contacts(), o.contacts()
+            assert np.all(np.diff(cg["key"]) > 0), "feature keys not strictly ascending"
+            for k in ("key", "bi", "bj", "pA", "pB", "n", "pen", "jn", "jt1", "jt2", "jp"):
+                assert np.array_equal(cg[k], co[k]), f"step {s}: contact field {k} differs"
+    assert e.n_contacts == e.n_contacts_solved == e.n_pairs  # unreduced: every contact its own chunk
+    np.testing.assert_array_equal(e.contact_forces(), o.forces())
+
+
+def test_per_contact_bead_stack8_keeps_its_spacing():
+    """The reference's own gate for this engine (tests/test_colored_solver.py:38-49): K = 8 stack,
+    400 steps at the default iterations, kinetic energy < 0.5 J and every spacing in (0.15, 0.25)."""
+    _say("per-contact coloured bead stack8 x400, the reference's stability gate")
+    pkg = _pkg()
+    bs = pkg.bodies.bead_bodies(pkg.scenes.stack(8), [700.0] * 8, [0] * 8)
+    e = _per_contact_engine(bs, [700.0] * 8)
+    for _ in range(400):
+        e.step(1 / 240)
+    st = e.get_state()
+    ke = float(np.sum(0.5 * e._M[:, None] * st.vc.astype(np.float64) ** 2))
+    zs = sorted(st.xc[:, 2])
+    seps = [zs[i + 1] - zs[i] for i in range(7)]
+    assert ke < 0.5 and all(0.15 < sp < 0.25 for sp in seps), (ke, seps)
+    assert e.n_colors > 0 and e.n_contacts > 0
