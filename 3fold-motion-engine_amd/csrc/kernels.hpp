@@ -10,7 +10,10 @@ namespace me {
 enum {
     IS_C = 0, IS_P, IS_CAND, IS_NCOL, IS_HITS, IS_MISSES, IS_OVF, IS_ISL, IS_TILES, IS_MAXTB,
     IS_MAXEXT, IS_LEFT, IS_WN, IS_CACHED_P, IS_CACHED_NCOL, IS_MISMATCH, IS_CELL, IS_NBODYSORT,
-    IS_NPAIRSORT, IS_NCOLJP, IS_NBIG, IS_CRAW, IS_JTOT, IS_NGIANT, IS_NGCOL, IS_COUNT = 32
+    IS_NPAIRSORT, IS_NCOLJP, IS_NBIG, IS_CRAW, IS_JTOT, IS_NGIANT, IS_NGCOL,
+    IS_BIGQ,  // next entry of the largest-first big-tile queue taken by k_t_big
+    IS_COLQ,  // the same queue for the big tiles' colouring (k_t_color)
+    IS_COUNT = 32
 };
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,
        OVF_TAPE = 32 /* a recorded (differentiated) substep met a tile beyond one small tile */,

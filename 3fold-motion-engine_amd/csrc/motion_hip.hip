@@ -133,6 +133,7 @@ struct me_engine {
     // giant-island grid kernels: workgroups (all resident together) and the barrier's wait bound
     int giant_grid = 0;
     long long giant_wait_ticks = 0;
+    int big_grid = 0;  // workgroups of k_t_big / k_t_color: one per CU their LDS allows
 };
 
 namespace {
@@ -758,6 +759,7 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.prof_ev = e->prof_on ? prof_event : nullptr;
     a.prof = e;
     a.giant_grid = e->giant_grid;
+    a.big_grid = e->big_grid;
     a.giant_wait_ticks = e->giant_wait_ticks;
     a.taped = e->taping ? 1 : 0;
     a.tape = e->tape;
@@ -799,6 +801,18 @@ int giant_setup(me_engine *e) {
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[2], (const void *)k_t_color_giant, T_BIG_WG, 0));
     int sweep = occ[0] < occ[1] ? occ[0] : occ[1];
     e->giant_grid = giant_grid_size(T_GIANT_GRID, sweep, occ[2], ncu);
+    // the big-tile kernels take tiles from a queue: as many workgroups as can be resident, at most
+    // T_BIG_GRID_MAX (more would only find the queue empty)
+    int ob[3] = {0, 0, 0};
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob[0], (const void *)k_t_big<ME_SOLVER_PAIR_CHUNKS>,
+                                                         T_BIG_WG, T_BIG_DYN));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob[1], (const void *)k_t_big<ME_SOLVER_COLOR_CACHE>,
+                                                         T_BIG_WG, T_BIG_DYN));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob[2], (const void *)k_t_color<1024>, 1024, T_COL_DYN));
+    int obig = ob[0] < ob[1] ? ob[0] : ob[1];
+    if (ob[2] < obig) obig = ob[2];
+    e->big_grid = giant_grid_size(T_BIG_GRID_MAX, obig, obig, ncu);
+    if (e->big_grid < 1) e->big_grid = 1;  // a workgroup that fits nowhere fails at launch with an error
     e->giant_wait_ticks = 100LL * khz;
     bool may_launch = (e->cfg.flags & ME_FLAG_TILED) && e->cfg.solver != ME_SOLVER_JACOBI &&
                       (e->N > T_BIG_B || e->capP > T_BIG_P || e->N > T_COLB);

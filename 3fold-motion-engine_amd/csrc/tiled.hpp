@@ -42,7 +42,8 @@ struct TiledBuffers {
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
     long long *bigw;             // tile work since the last reset, 4 counts (tiles, pairs, dynamic pair ends,
                                  // contacts) for each of k_t_big [0..3], k_t_small [4..7], k_t_giant [8..11]
-    int *biglist;                // tiles beyond one small tile (IS_NBIG of them)
+    int *biglist;                // tiles beyond one small tile (IS_NBIG of them), in k_t_biglist's order
+    int *bigq;                   // the same tiles, most pairs first (ties: lower tile index): the work queue
     int *giantlist;              // tiles beyond the big tile's LDS (IS_NGIANT of them): k_t_giant's
     int *gcollist;               // tiles beyond the LDS colouring limits (IS_NGCOL): k_t_color_giant's
     int *gcst;                   // giant tile: colour-class starts (ME_NCOLMAX + 1), class count at [ME_NCOLMAX + 1]
@@ -87,6 +88,7 @@ struct TiledArgs {
     void *prof;
     int taped;                       // record the substep into tape (k_t_small<.., true>)
     int giant_grid;                  // workgroups of the giant-island grid kernels, all resident together
+    int big_grid;                    // workgroups of the big-tile kernels (one per CU the LDS allows)
     long long giant_wait_ticks;      // bound of a grid barrier's wait, in wall-clock counter ticks
     AdjTape tape;
 };
@@ -130,6 +132,7 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.dbg = B.template take<long long>(16);
     t.bigw = B.template take<long long>(12);
     t.biglist = B.template take<int>(n);
+    t.bigq = B.template take<int>(n);
     t.giantlist = B.template take<int>(n);
     t.gcollist = B.template take<int>(n);
     t.gcst = B.template take<int>(ME_NCOLMAX + 2);
@@ -151,6 +154,8 @@ __global__ void k_t_init(int N, TiledBuffers T, int *istat, int lds_islands) {
         istat[IS_NCOLJP] = 0; istat[IS_ISL] = 0; istat[IS_TILES] = 0; istat[IS_MAXTB] = 0;
         istat[IS_MISMATCH] = 0;
         istat[IS_NBIG] = 0;
+        istat[IS_BIGQ] = 0;
+        istat[IS_COLQ] = 0;
         istat[IS_NGIANT] = 0;
         istat[IS_NGCOL] = 0;
         T.dbg[7] = 0;
@@ -592,8 +597,14 @@ __global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T, int
     extern __shared__ unsigned long long cdsm[];
     __shared__ int left, hist[ME_NCOLMAX];
     const int nbig = A.istat[IS_NBIG];
-    for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
-        int t = T.biglist[i];
+    __shared__ int qi;
+    for (;;) {
+        if (threadIdx.x == 0) qi = atomicAdd(&A.istat[IS_COLQ], 1);
+        __syncthreads();
+        const int i = qi;
+        __syncthreads();
+        if (i >= nbig) break;
+        int t = T.bigq[i];
         int b0 = T.tb0[t];
         int nb = T.tb1[t] - b0, p0 = T.tp0[t], np = T.tp1[t] - T.tp0[t];
         if (np > 0) {
@@ -1161,6 +1172,25 @@ __global__ void k_t_biglist(int ntile, int *istat, const float *invM, const floa
     if (np > 0 && !(nb <= T_COLB && np <= T_COLP)) T.gcollist[atomicAdd(&istat[IS_NGCOL], 1)] = t;
 }
 
+// the big-tile queue: the listed tiles ranked by pair count, largest first (ties by tile index, so the
+// order does not depend on k_t_biglist's atomics). Workgroups that take tiles from the front in turn
+// finish the launch close to max(largest tile, total work / workgroups) -- the longest-processing-time
+// rule -- where a fixed stride could stack two large tiles on one workgroup. One thread per entry.
+__global__ void k_t_bigsort(int ntile, const int *istat, TiledBuffers T) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = istat[IS_NBIG];
+    if (i >= n || i >= ntile) return;
+    const int t = T.biglist[i];
+    const int w = T.tp1[t] - T.tp0[t];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+        int u = T.biglist[j];
+        int wu = T.tp1[u] - T.tp0[u];
+        rank += (wu > w || (wu == w && u < t)) ? 1 : 0;
+    }
+    T.bigq[rank] = t;
+}
+
 // dynamic-LDS limits of the tile kernels beyond the 64 KB default; set on the current device before
 // the first launch (at engine creation, so that no attribute call falls inside a stream capture)
 inline int tiled_set_attributes() {
@@ -1198,7 +1228,7 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
         hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>), dim3(ntile), dim3(T_SMALL_P), 0, s, A,
                            T);
     if (prof_rec(A, ME_K_TILE_SMALL, s) || prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
-    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(T_BIG_WG), dyn, s2, A, T);
+    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIG_WG), dyn, s2, A, T);
     if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
     // a tile beyond the big tile's LDS (more than T_BIG_B bodies or T_BIG_P pairs) is possible: the grid
     // sweep, after k_t_big, which leaves such tiles to it
@@ -1263,10 +1293,11 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile > N ? ntile : N)), dim3(ME_THREADS), 0, s, ntile, A.istat, A.b.invM,
                        A.invIw, T);
+    hipLaunchKernelGGL(k_t_bigsort, dim3(tblk(ntile)), dim3(ME_THREADS), 0, s, ntile, A.istat, T);
     hipLaunchKernelGGL(k_t_color_small, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
     const int giant = N > T_COLB;  // a tile beyond the LDS colouring limits: coloured by a grid of workgroups
     if (prof_rec(A, ME_K_COLOR, s)) return ME_ERUNTIME;
-    hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < T_BIG_GRID ? ntile : T_BIG_GRID), dim3(1024), T_COL_DYN, s, A, T,
+    hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(1024), T_COL_DYN, s, A, T,
                        giant);
     if (prof_rec(A, ME_K_COLOR, s)) return ME_ERUNTIME;
     if (giant) hipLaunchKernelGGL(k_t_color_giant, dim3(A.giant_grid), dim3(T_BIG_WG), 0, s, A, T);

@@ -299,18 +299,26 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     stamp(6);
 }
 
-// the big tiles listed by k_t_biglist, T_BIG_GRID workgroups taking them in turn (a launch over
-// every tile would schedule one 152 KB-LDS workgroup per tile, mostly to find a small tile and exit)
-#define T_BIG_GRID 128
+// the big tiles, taken from the largest-first queue (k_t_bigsort) by workgroups that each take the
+// next entry when done, one workgroup per CU (a launch over every tile would schedule one 152 KB-LDS
+// workgroup per tile, mostly to find a small tile and exit). Which workgroup solves a tile does not
+// change its result.
+#define T_BIG_GRID_MAX 256
 
 template <int SOLVER>
 __global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T) {
     extern __shared__ float dsm[];
     __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
-    __shared__ int maxc, wdyn, wcon;
+    __shared__ int maxc, wdyn, wcon, qi;
     const int nbig = A.istat[IS_NBIG];
-    for (int i = blockIdx.x; i < nbig; i += gridDim.x)
-        big_tile<SOLVER>(A, T, T.biglist[i], dsm, cst, cfl, cfe, maxc, wdyn, wcon);
+    for (;;) {
+        if (threadIdx.x == 0) qi = atomicAdd(&A.istat[IS_BIGQ], 1);
+        __syncthreads();
+        const int i = qi;
+        __syncthreads();
+        if (i >= nbig) break;
+        big_tile<SOLVER>(A, T, T.bigq[i], dsm, cst, cfl, cfe, maxc, wdyn, wcon);
+    }
 }
 
 // ── giant tiles: an island beyond the LDS limits of one workgroup, swept by a grid of workgroups ──
