@@ -60,6 +60,15 @@ __device__ __forceinline__ GRec grec_load(const int *r) {
     return GRec{x.y, x.z, y.y, y.z, y.w};
 }
 
+// A colour-sorted pair's record: a 40-float header -- both sides' inverse mass (and two pad floats),
+// inverse world inertia (12 floats each, row-major with 3 pad), and the velocity and angular velocity
+// of a static (not dynamic) side (12 floats: vA, wA, vB, wB) -- then per contact 24 floats (the 20
+// constants of k_t_prep and the impulses jn, jt1, jt2, jp). Everything a colour step reads from global
+// memory sits in the pair's own record, at an address known before the pair word is read, so the
+// step issues all of its global loads at once.
+#define T_REC_HDR 40
+#define T_REC (T_REC_HDR + 96)
+
 // One pair of a big tile for one colour step, contacts streamed one at a time (few live registers).
 // PASS 0 warm apply, 1 velocity, 2 position. LDS mode: body state s0/s1 in LDS by tile slot, the
 // pair word and its colour-ordered records; global mode: record g, body state in global memory by
@@ -81,30 +90,44 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
     float iMA, iMB = 0.0f;
     m3 IA, IB;
     f3 xA, yA, xB = z, yB = z;
-    if (LDSM && dynA) {
-        iMA = T.invMs[b0 + la];
-        IA = ld_m12(T.invIws + 12 * (size_t)(b0 + la));
-        xA = ld3(s0, la); yA = ld3(s1, la);
+    float *rec = T.sprep + T_REC * (size_t)q;
+    if (LDSM) {
+        // the record header: masses, inertias, a static side's velocities (its pseudo-velocities are
+        // zero: they are cleared every substep and a static body takes no impulse)
+        const float4 h0 = *reinterpret_cast<const float4 *>(rec);
+        iMA = h0.x;
+        iMB = h0.y;
+        IA = ld_m12(rec + 4);
+        IB = ld_m12(rec + 16);
+        const float4 h7 = *reinterpret_cast<const float4 *>(rec + 28);
+        const float4 h8 = *reinterpret_cast<const float4 *>(rec + 32);
+        const float4 h9 = *reinterpret_cast<const float4 *>(rec + 36);
+        if (dynA) {
+            xA = ld3(s0, la); yA = ld3(s1, la);
+        } else if (PASS == 2) {
+            xA = z; yA = z;
+        } else {
+            xA = mk(h7.x, h7.y, h7.z); yA = mk(h7.w, h8.x, h8.y);
+        }
+        if (dynB) {
+            xB = ld3(s0, lb); yB = ld3(s1, lb);
+        } else if (hasB && PASS != 2) {
+            xB = mk(h8.z, h8.w, h9.x); yB = mk(h9.y, h9.z, h9.w);
+        }
     } else {
-        if (LDSM) a = T.srec[4 * (size_t)q + 1];
         iMA = A.b.invM[a];
         IA = ldm(A.invIw, a);
-        xA = (!LDSM && dynA) ? ld3(s0, a) : ld3(gs0, a);
-        yA = (!LDSM && dynA) ? ld3(s1, a) : ld3(gs1, a);
+        xA = dynA ? ld3(s0, a) : ld3(gs0, a);
+        yA = dynA ? ld3(s1, a) : ld3(gs1, a);
+        IB = IA;
+        if (hasB) {
+            iMB = A.b.invM[b];
+            IB = ldm(A.invIw, b);
+            xB = dynB ? ld3(s0, b) : ld3(gs0, b);
+            yB = dynB ? ld3(s1, b) : ld3(gs1, b);
+        }
     }
-    IB = IA;
-    if (LDSM && dynB) {
-        iMB = T.invMs[b0 + lb];
-        IB = ld_m12(T.invIws + 12 * (size_t)(b0 + lb));
-        xB = ld3(s0, lb); yB = ld3(s1, lb);
-    } else if (hasB) {
-        if (LDSM) b = T.srec[4 * (size_t)q + 2];
-        iMB = A.b.invM[b];
-        IB = ldm(A.invIw, b);
-        xB = (!LDSM && dynB) ? ld3(s0, b) : ld3(gs0, b);
-        yB = (!LDSM && dynB) ? ld3(s1, b) : ld3(gs1, b);
-    }
-    float *o = T.sprep + 96 * (size_t)q;
+    float *o = rec + T_REC_HDR;
     // software pipeline: contact k+1 is loaded before contact k is solved and stored
     auto load = [&](int k, CCon &C) {
         if (LDSM) {
@@ -234,7 +257,32 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         pword[pos] = pw_make(la, lb, cnt, dA, dB, b >= 0);
         int *sr = T.srec + 4 * (size_t)q;
         sr[0] = p; sr[1] = a; sr[2] = b; sr[3] = st;
-        float4 *o = reinterpret_cast<float4 *>(T.sprep + 96 * (size_t)q);
+        float4 *hd = reinterpret_cast<float4 *>(T.sprep + T_REC * (size_t)q);
+        {
+            float sA[12], sB[12];
+            const float iA = dA ? T.invMs[b0 + la] : invM[a];
+            const float iB = dB ? T.invMs[b0 + lb] : (b >= 0 ? invM[b] : 0.0f);
+            for (int r = 0; r < 9; ++r) {
+                sA[r] = dA ? T.invIws[12 * (size_t)(b0 + la) + r] : A.invIw[9 * (size_t)a + r];
+                sB[r] = dB ? T.invIws[12 * (size_t)(b0 + lb) + r] : (b >= 0 ? A.invIw[9 * (size_t)b + r] : sA[r]);
+            }
+            sA[9] = sA[10] = sA[11] = 0.0f;
+            sB[9] = sB[10] = sB[11] = 0.0f;
+            f3 z3 = mk(0.0f, 0.0f, 0.0f);
+            f3 vA = dA ? z3 : ld3(A.b.v, a), wA = dA ? z3 : ld3(A.b.w, a);
+            f3 vB = (dB || b < 0) ? z3 : ld3(A.b.v, b), wB = (dB || b < 0) ? z3 : ld3(A.b.w, b);
+            hd[0] = make_float4(iA, iB, 0.0f, 0.0f);
+            hd[1] = make_float4(sA[0], sA[1], sA[2], sA[3]);
+            hd[2] = make_float4(sA[4], sA[5], sA[6], sA[7]);
+            hd[3] = make_float4(sA[8], sA[9], sA[10], sA[11]);
+            hd[4] = make_float4(sB[0], sB[1], sB[2], sB[3]);
+            hd[5] = make_float4(sB[4], sB[5], sB[6], sB[7]);
+            hd[6] = make_float4(sB[8], sB[9], sB[10], sB[11]);
+            hd[7] = make_float4(vA.x, vA.y, vA.z, wA.x);
+            hd[8] = make_float4(wA.y, wA.z, vB.x, vB.y);
+            hd[9] = make_float4(vB.z, wB.x, wB.y, wB.z);
+        }
+        float4 *o = reinterpret_cast<float4 *>(T.sprep + T_REC * (size_t)q + T_REC_HDR);
         for (int i = 0; i < cnt; ++i) {
             const float4 *s4 = reinterpret_cast<const float4 *>(T.prep + 20 * (size_t)(st + i));
             float4 v0 = s4[0], v1 = s4[1], v2 = s4[2], v3 = s4[3], v4 = s4[4];
@@ -280,7 +328,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         int q = p0 + k;
         int st = T.srec[4 * (size_t)q + 3];
         int cnt = (int)((pword[k] >> 26) & 3) + 1;
-        const float *o = T.sprep + 96 * (size_t)q;
+        const float *o = T.sprep + T_REC * (size_t)q + T_REC_HDR;
         for (int i = 0; i < cnt; ++i) {
             A.Cc.jn[st + i] = o[24 * i + 20];
             A.Cc.jt1[st + i] = o[24 * i + 21];
