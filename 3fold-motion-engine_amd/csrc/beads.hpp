@@ -44,7 +44,7 @@ __device__ __forceinline__ bool bead_hit(const float *allp, int a, int b, int sl
     if (slot >= P * P) return false;
     f3 d = ld3(allp, a * P + slot / P) - ld3(allp, b * P + slot % P);
     if (!(fabsf(d.x) <= ME_BEAD_LINF && fabsf(d.y) <= ME_BEAD_LINF && fabsf(d.z) <= ME_BEAD_LINF)) return false;
-    float dist = length(d);
+    float dist = clength(d);
     return dist < 2.0f * R && dist > 1e-9f;
 }
 
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(ME_THREADS) k_bp_emit(int cap, const int *ista
                 } else {
                     f3 xj = ld3(allp, b * P + lb);
                     f3 d = xi - xj;
-                    float dist = length(d);
+                    float dist = clength(d);
                     st3(raw.pB, t, xj);
                     st3(raw.n, t, d / dist);
                     raw.pen[t] = 2.0f * R - dist;

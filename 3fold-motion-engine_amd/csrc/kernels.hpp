@@ -175,13 +175,13 @@ __global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb
     if (i < N) {
         f3 v = ld3(b.v, i);
         if (b.invM[i] > 0.0f) {  // k_grav (contact_engine_gpu.py:41-44)
-            v = v + mk(0.0f, 0.0f, -9.81f) * dt;
+            v = cmadd(dt, mk(0.0f, 0.0f, -9.81f), v);
             st3(b.v, i, v);
         }
         st3(vsnap, i, v);
         st3(xsnap, i, ld3(b.x, i));
         st3(wsnap, i, ld3(b.w, i));
-        m3 R = quat_to_matrix(ld4(b.q, i));  // k_invIw (contact_engine_gpu.py:47-53)
+        m3 R = cquat_to_matrix(ld4(b.q, i));  // k_invIw (contact_engine_gpu.py:47-53)
         stm(Rw, i, R);
         if (b.kin[i] == 1) {
             m3 z;
@@ -190,8 +190,8 @@ __global__ void k_body_pre(int N, Bodies b, float *invIw, float *Rw, float *aabb
             stm(invIw, i, z);
         } else {
             m3 Ib = ldm(b.IbInv, i);
-            m3 t = mm(R, Ib);
-            m3 r = mm(t, transpose(R));
+            m3 t = cmm(R, Ib);
+            m3 r = cmm(t, transpose(R));
             stm(invIw, i, r);
         }
         if (box_mode) {
@@ -468,7 +468,7 @@ __global__ void k_bead_points(int N, int P, Bodies b, float *allp) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= N * P) return;
     int bi = g / P;
-    st3(allp, g, ld3(b.x, bi) + quat_rotate(ld4(b.q, bi), ld3(b.rest, g % P)));
+    st3(allp, g, ld3(b.x, bi) + cquat_rotate(ld4(b.q, bi), ld3(b.rest, g % P)));
 }
 
 // Every bead j > g of another body in the 27 grid cells around bead g (cell coordinates c, cell size
@@ -504,7 +504,7 @@ __device__ __forceinline__ void bead_neighbours(int g, int P, f3 xi, int4 c, con
         int fx = (int)floorf(q.x / h);
         if (fx < c.x - 1 || fx > c.x + 1 || (int)floorf(q.y / h) != Y || (int)floorf(q.z / h) != Z) return;
         f3 dv = xi - mk(q.x, q.y, q.z);
-        float dist = length(dv);
+        float dist = clength(dv);
         if (!(dist < 2.0f * R && dist > 1e-9f)) return;
         fn(j);
     };
@@ -581,7 +581,7 @@ __global__ void k_bead_contacts(int NP, int P, const float *allp, const int *cel
         int j = (int)(key[k] - kg - 1);
         f3 xj = ld3(allp, j);
         f3 dv = xi - xj;
-        float dist = length(dv);
+        float dist = clength(dv);
         Cc.bi[cc] = bi; Cc.bj[cc] = j / P;
         st3(Cc.pA, cc, xi); st3(Cc.pB, cc, xj); st3(Cc.n, cc, dv / dist);
         Cc.pen[cc] = 2.0f * R - dist; Cc.mu[cc] = m;
@@ -663,7 +663,7 @@ struct JacArgs {
     int N;
 };
 
-// k_jac_vel (contact_engine_gpu.py:82-131)
+// k_jac_vel (contact_engine_gpu.py:82-131), contracted forms throughout
 __global__ void k_jac_vel(const int *istat, JacArgs A, float scale) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= istat[IS_C]) return;
@@ -673,27 +673,27 @@ __global__ void k_jac_vel(const int *istat, JacArgs A, float scale) {
     f3 nrm = ld3(A.n, c), rA = ld3(A.pA, c) - ld3(A.x, bi);
     float iMA = A.invM[bi];
     m3 IA = ldm(A.invIw, bi), IB = IA;
-    f3 va = ld3(A.v, bi) + cross(ld3(A.w, bi), rA);
+    f3 va = ld3(A.v, bi) + ccross(ld3(A.w, bi), rA);
     f3 vb = mk(0.0f, 0.0f, 0.0f), rB = mk(0.0f, 0.0f, 0.0f);
     float iMB = 0.0f;
     if (bj >= 0) {
         rB = ld3(A.pB, c) - ld3(A.x, bj);
-        vb = ld3(A.v, bj) + cross(ld3(A.w, bj), rB);
+        vb = ld3(A.v, bj) + ccross(ld3(A.w, bj), rB);
         iMB = A.invM[bj];
         IB = ldm(A.invIw, bj);
     }
     f3 vrel = va - vb;
-    float vn = dot(vrel, nrm);
-    f3 vt = vrel - nrm * vn;
-    float mt = length(vt);
+    float vn = cdot(vrel, nrm);
+    f3 vt = cmadd(-vn, nrm, vrel);
+    float mt = clength(vt);
     f3 t1 = vt / mt;
-    if (mt <= 5e-3f) t1 = fixed_t1(nrm);
-    f3 t2 = cross(nrm, t1);
-    f3 rnA = cross(rA, nrm);
-    float meff = iMA + dot(rnA, mv(IA, rnA));
+    if (mt <= 5e-3f) t1 = cfixed_t1(nrm);
+    f3 t2 = ccross(nrm, t1);
+    f3 rnA = ccross(rA, nrm);
+    float meff = iMA + cdot(rnA, cmv(IA, rnA));
     if (bj >= 0) {
-        f3 rnB = cross(rB, nrm);
-        meff = meff + iMB + dot(rnB, mv(IB, rnB));
+        f3 rnB = ccross(rB, nrm);
+        meff = meff + iMB + cdot(rnB, cmv(IB, rnB));
     }
     if (meff < 1e-12f) {
         st_entry(A.d, ea, qa);
@@ -707,41 +707,41 @@ __global__ void k_jac_vel(const int *istat, JacArgs A, float scale) {
     A.jn[c] = nw;
     f3 Jn = nrm * dj;
     q3(qa, Jn * iMA, 1.0f, scale);
-    q3(qa + 3, mv(IA, cross(rA, Jn)), 1.0f, scale);
+    q3(qa + 3, cmv(IA, ccross(rA, Jn)), 1.0f, scale);
     if (bj >= 0) {
         q3(qb, Jn * iMB, -1.0f, scale);
-        q3(qb + 3, mv(IB, cross(rB, Jn)), -1.0f, scale);
+        q3(qb + 3, cmv(IB, ccross(rB, Jn)), -1.0f, scale);
     }
-    f3 rtA1 = cross(rA, t1);
-    float meft1 = iMA + dot(rtA1, mv(IA, rtA1));
-    f3 rtA2 = cross(rA, t2);
-    float meft2 = iMA + dot(rtA2, mv(IA, rtA2));
+    f3 rtA1 = ccross(rA, t1);
+    float meft1 = iMA + cdot(rtA1, cmv(IA, rtA1));
+    f3 rtA2 = ccross(rA, t2);
+    float meft2 = iMA + cdot(rtA2, cmv(IA, rtA2));
     if (bj >= 0) {
-        f3 rtB1 = cross(rB, t1);
-        meft1 = meft1 + iMB + dot(rtB1, mv(IB, rtB1));
-        f3 rtB2 = cross(rB, t2);
-        meft2 = meft2 + iMB + dot(rtB2, mv(IB, rtB2));
+        f3 rtB1 = ccross(rB, t1);
+        meft1 = meft1 + iMB + cdot(rtB1, cmv(IB, rtB1));
+        f3 rtB2 = ccross(rB, t2);
+        meft2 = meft2 + iMB + cdot(rtB2, cmv(IB, rtB2));
     }
     float o1 = A.jt1[c], o2 = A.jt2[c];
-    float a1 = o1 - dot(vrel, t1) / meft1;
-    float a2 = o2 - dot(vrel, t2) / meft2;
-    float mag = sqrtf(a1 * a1 + a2 * a2);
+    float a1 = o1 - cdot(vrel, t1) / meft1;
+    float a2 = o2 - cdot(vrel, t2) / meft2;
+    float mag = sqrtf(fmaf(a1, a1, a2 * a2));
     float lim = A.mu[c] * nw;
     if (mag > lim && mag > 1e-12f) { a1 = a1 * lim / mag; a2 = a2 * lim / mag; }
     A.jt1[c] = a1;
     A.jt2[c] = a2;
-    f3 Jt = t1 * (a1 - o1) + t2 * (0.5f * (a2 - o2));
+    f3 Jt = cmadd(a1 - o1, t1, t2 * (0.5f * (a2 - o2)));
     q3(qa, Jt * iMA, 1.0f, scale);
-    q3(qa + 3, mv(IA, cross(rA, Jt)), 1.0f, scale);
+    q3(qa + 3, cmv(IA, ccross(rA, Jt)), 1.0f, scale);
     st_entry(A.d, ea, qa);
     if (bj >= 0) {
         q3(qb, Jt * iMB, -1.0f, scale);
-        q3(qb + 3, mv(IB, cross(rB, Jt)), -1.0f, scale);
+        q3(qb + 3, cmv(IB, ccross(rB, Jt)), -1.0f, scale);
         st_entry(A.d, eb, qb);
     }
 }
 
-// k_jac_pos (contact_engine_gpu.py:146-169)
+// k_jac_pos (contact_engine_gpu.py:146-169), contracted forms throughout
 __global__ void k_jac_pos(const int *istat, JacArgs A, float dt, float scale) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= istat[IS_C]) return;
@@ -751,20 +751,20 @@ __global__ void k_jac_pos(const int *istat, JacArgs A, float dt, float scale) {
     f3 nrm = ld3(A.n, c), rA = ld3(A.pA, c) - ld3(A.x, bi);
     float iMA = A.invM[bi];
     m3 IA = ldm(A.invIw, bi), IB = IA;
-    f3 rel = ld3(A.pv, bi) + cross(ld3(A.po, bi), rA);
+    f3 rel = ld3(A.pv, bi) + ccross(ld3(A.po, bi), rA);
     f3 rB = mk(0.0f, 0.0f, 0.0f);
     float iMB = 0.0f;
     if (bj >= 0) {
         rB = ld3(A.pB, c) - ld3(A.x, bj);
-        rel = rel - (ld3(A.pv, bj) + cross(ld3(A.po, bj), rB));
+        rel = rel - (ld3(A.pv, bj) + ccross(ld3(A.po, bj), rB));
         iMB = A.invM[bj];
         IB = ldm(A.invIw, bj);
     }
-    f3 rnA = cross(rA, nrm);
-    float meff = iMA + dot(rnA, mv(IA, rnA));
+    f3 rnA = ccross(rA, nrm);
+    float meff = iMA + cdot(rnA, cmv(IA, rnA));
     if (bj >= 0) {
-        f3 rnB = cross(rB, nrm);
-        meff = meff + iMB + dot(rnB, mv(IB, rnB));
+        f3 rnB = ccross(rB, nrm);
+        meff = meff + iMB + cdot(rnB, cmv(IB, rnB));
     }
     if (meff < 1e-12f) {
         st_entry(A.d, ea, qa);
@@ -772,24 +772,24 @@ __global__ void k_jac_pos(const int *istat, JacArgs A, float dt, float scale) {
         return;
     }
     float bias = 0.2f * wmax(A.pen[c] - 1e-4f, 0.0f) / dt;
-    float dj = (bias - dot(rel, nrm)) / meff;
+    float dj = (bias - cdot(rel, nrm)) / meff;
     float jo = A.jp[c];
     float nw = wmax(0.0f, jo + dj);
     dj = nw - jo;
     A.jp[c] = nw;
     f3 Jn = nrm * dj;
     q3(qa, Jn * iMA, 1.0f, scale);
-    q3(qa + 3, mv(IA, cross(rA, Jn)), 1.0f, scale);
+    q3(qa + 3, cmv(IA, ccross(rA, Jn)), 1.0f, scale);
     st_entry(A.d, ea, qa);
     if (bj >= 0) {
         q3(qb, Jn * iMB, -1.0f, scale);
-        q3(qb + 3, mv(IB, cross(rB, Jn)), -1.0f, scale);
+        q3(qb + 3, cmv(IB, ccross(rB, Jn)), -1.0f, scale);
         st_entry(A.d, eb, qb);
     }
 }
 
 // k_apply (contact_engine_gpu.py:134-143) over the entries: thread (body b, component k) sums b's
-// entries in component k, dv = f32(f64(sum) * f64(f32(1/scale))), then v += relax * dv. The
+// entries in component k, dv = f32(f64(sum) * f64(f32(1/scale))), then v = fma(relax, dv, v). The
 // reference zeroes its accumulators here; every entry is rewritten by the next pass instead.
 __global__ void k_jac_gather(int N, const int *off, const int *total, const int64_t *d, float *v, float *w,
                              float relax, float scale) {
@@ -811,7 +811,7 @@ __global__ void k_jac_gather(int N, const int *off, const int *total, const int6
     float inv = 1.0f / scale;
     float dv = (float)((double)sum * (double)inv);
     float *dst = k < 3 ? v + 3 * (size_t)b + k : w + 3 * (size_t)b + (k - 3);
-    *dst = *dst + relax * dv;
+    *dst = fmaf(relax, dv, *dst);
 }
 
 // Jacobi substep set-up: impulses and entry counts to zero; pv, po to zero only when there are

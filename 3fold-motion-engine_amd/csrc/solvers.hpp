@@ -38,16 +38,16 @@ __device__ __forceinline__ void apply_warm_pair(const BodyView &BA, const BodyVi
     for (int k = 0; k < cnt; ++k) {
         int c = st + k;
         f3 n = ld3(Cv.n, c);
-        f3 t1 = fixed_t1(n);
-        f3 t2 = cross(n, t1);
-        f3 imp = (Cv.jn[c] * n + Cv.jt1[c] * t1) + Cv.jt2[c] * t2;
+        f3 t1 = cfixed_t1(n);
+        f3 t2 = ccross(n, t1);
+        f3 imp = cmadd(Cv.jt2[c], t2, cmadd(Cv.jn[c], n, Cv.jt1[c] * t1));
         if (iMA > 0.0f) {
-            va = va + iMA * imp;
-            wa = wa + mv(IA, cross(ld3(Cv.pA, c) - xa, imp));
+            va = cmadd(iMA, imp, va);
+            wa = wa + cmv(IA, ccross(ld3(Cv.pA, c) - xa, imp));
         }
         if (b >= 0 && iMB > 0.0f) {
-            vb = vb - iMB * imp;
-            wb = wb - mv(IB, cross(ld3(Cv.pB, c) - xb, imp));
+            vb = cmadd(-iMB, imp, vb);
+            wb = wb - cmv(IB, ccross(ld3(Cv.pB, c) - xb, imp));
         }
     }
     if (iMA > 0.0f) { st3(B.v, a, va); st3(B.w, a, wa); }
@@ -172,15 +172,15 @@ __device__ __forceinline__ void chunk_pos(const BodyView &B, const BodyView &BB,
 
 // integrate one body (k_integ, contact_engine_gpu.py:172-180)
 __device__ __forceinline__ void integrate_body(float *x, float *q, f3 v, f3 w, f3 pv, f3 po, int i, float dt) {
-    st3(x, i, ld3(x, i) + (v + pv) * dt);
+    st3(x, i, cmadd(dt, v + pv, ld3(x, i)));
     f3 ww = w + po;
     f4 h;
     h.x = 0.5f * ww.x; h.y = 0.5f * ww.y; h.z = 0.5f * ww.z; h.w = 0.5f * 0.0f;
     f4 qi = ld4(q, i);
-    f4 m = qmul(h, qi);
+    f4 m = cqmul(h, qi);
     f4 qn;
-    qn.x = qi.x + m.x * dt; qn.y = qi.y + m.y * dt; qn.z = qi.z + m.z * dt; qn.w = qi.w + m.w * dt;
-    f4 r = qnormalize(qn);
+    qn.x = fmaf(m.x, dt, qi.x); qn.y = fmaf(m.y, dt, qi.y); qn.z = fmaf(m.z, dt, qi.z); qn.w = fmaf(m.w, dt, qi.w);
+    f4 r = cqnormalize(qn);
     q[4 * i] = r.x; q[4 * i + 1] = r.y; q[4 * i + 2] = r.z; q[4 * i + 3] = r.w;
 }
 

@@ -741,14 +741,17 @@ __device__ __forceinline__ void pc_store(const PairCache &P, const TiledArgs &A,
 // one contact of apply_warm (contact_engine_gpu_applied_warm.py:7-54)
 __device__ __forceinline__ void cc_warm(const CCon &C, f3 &va, f3 &wa, f3 &vb, f3 &wb, bool hasB, float iMA,
                                         float iMB, const m3 &IA, const m3 &IB) {
-    f3 imp = (C.jn * C.n + C.jt1 * C.t1) + C.jt2 * C.t2;
+    // the warm apply's own frame (contracted), not the sweep's C.t1, C.t2
+    f3 t1 = cfixed_t1(C.n);
+    f3 t2 = ccross(C.n, t1);
+    f3 imp = cmadd(C.jt2, t2, cmadd(C.jn, C.n, C.jt1 * t1));
     if (iMA > 0.0f) {
-        va = va + iMA * imp;
-        wa = wa + mv(IA, cross(C.rA, imp));
+        va = cmadd(iMA, imp, va);
+        wa = wa + cmv(IA, ccross(C.rA, imp));
     }
     if (hasB && iMB > 0.0f) {
-        vb = vb - iMB * imp;
-        wb = wb - mv(IB, cross(C.rB, imp));
+        vb = cmadd(-iMB, imp, vb);
+        wb = wb - cmv(IB, ccross(C.rB, imp));
     }
 }
 

@@ -10,6 +10,22 @@
  *   quat_rotate(q,x) = x*(2w*w-1) + qv*(2*(qv.x)) + cross(qv,x)*w*2
  *   quat_to_matrix(q) = columns quat_rotate(q, e0), quat_rotate(q, e1), quat_rotate(q, e2)
  *   quat product a*b (Hamilton, xyzw storage), quat normalize = q * (1/length(q))
+ * Those are the uncontracted forms, used where the reference builds a kernel with fuse_fp=False: the
+ * manifold selection (contact_engine_gpu_colored.py:651) and the chunk sweeps of its unfused engines
+ * (contact_engine_gpu_unfused_solve.py:223-231, contact_engine_gpu_unfused_position.py). Every other
+ * kernel the restatement follows (gravity, world inertia, world points and bead contacts, warm apply,
+ * the relaxed Jacobi solve and apply, the integrator) is built with Warp's default contraction, so
+ * a product that feeds an add is rounded once, as a fused multiply-add; the c-prefixed forms below
+ * write those fusions out (RFMA) as the compiler makes them:
+ *   a*b + c*d -> fma(a, b, c*d); s + a*b and a*b + s -> fma(a, b, s); s - a*b -> fma(-a, b, s);
+ *   a*b - c*d -> fma(a, b, -(c*d)); a product by the literal 1 folds away before fusion; an add whose
+ *   operands are not products (a fused result plus a value) stays an add.
+ *   cdot(a,b) = fma(a.z,b.z, fma(a.x,b.x, a.y*b.y)); cmv row i = fma(M[i][2],v.z, fma(M[i][0],v.x, M[i][1]*v.y));
+ *   cmm element ij = fma(M[i][2],N[2][j], fma(M[i][1],N[1][j], fma(M[i][0],N[0][j], 0))).
+ * Pinned by the reference's archived trajectories (reports/position_solve_matrix/inputs/
+ * cross_hardware_l4.npz): with these forms the Jacobi, pair-chunk and ColorCache restatements
+ * reproduce every archived state, force and per-step hash bit for bit; with the uncontracted forms
+ * in their place they drift by ulps from the first rotating step.
  */
 #define _POSIX_C_SOURCE 199309L
 #include "oracle.h"
@@ -105,6 +121,84 @@ static inline qt qmul(qt a, qt b) {
 }
 static inline qt qnormalize(qt q) {
     real l = RSQRT(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    qt r;
+    if (l > 0.0f) {
+        real inv = 1.0f / l;
+        r.x = q.x * inv; r.y = q.y * inv; r.z = q.z * inv; r.w = q.w * inv;
+    } else {
+        r.x = 0.0f; r.y = 0.0f; r.z = 0.0f; r.w = 1.0f;
+    }
+    return r;
+}
+
+/* contracted forms (kernels the reference builds with FMA contraction; rules in the file header) */
+static inline real cdot(v3 a, v3 b) { return RFMA(a.z, b.z, RFMA(a.x, b.x, a.y * b.y)); }
+static inline v3 ccross(v3 a, v3 b) {
+    return V3(RFMA(a.y, b.z, -(a.z * b.y)), RFMA(a.z, b.x, -(a.x * b.z)), RFMA(a.x, b.y, -(a.y * b.x)));
+}
+static inline v3 cmv(const m33 *M, v3 b) {
+    return V3(RFMA(M->m[0][2], b.z, RFMA(M->m[0][0], b.x, M->m[0][1] * b.y)),
+              RFMA(M->m[1][2], b.z, RFMA(M->m[1][0], b.x, M->m[1][1] * b.y)),
+              RFMA(M->m[2][2], b.z, RFMA(M->m[2][0], b.x, M->m[2][1] * b.y)));
+}
+static inline m33 cmm(const m33 *A, const m33 *B) {
+    m33 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            real s = RFMA(A->m[i][0], B->m[0][j], 0.0f);
+            s = RFMA(A->m[i][1], B->m[1][j], s);
+            r.m[i][j] = RFMA(A->m[i][2], B->m[2][j], s);
+        }
+    return r;
+}
+/* s + a*u, per component */
+static inline v3 cmadd(real a, v3 u, v3 s) { return V3(RFMA(a, u.x, s.x), RFMA(a, u.y, s.y), RFMA(a, u.z, s.z)); }
+static inline v3 cnormalize(v3 a) {
+    real l = RSQRT(cdot(a, a));
+    if (l > 0.0f) return vdiv(a, l);
+    return V3(0.0f, 0.0f, 0.0f);
+}
+/* normalize(aa - dot(aa, n)*n) with aa = e_x, or e_y when |n.x| >= 0.9 */
+static inline v3 cfixed_t1(v3 n) {
+    v3 aa = V3(1.0f, 0.0f, 0.0f);
+    if (RFABS(n.x) >= 0.9f) aa = V3(0.0f, 1.0f, 0.0f);
+    return cnormalize(cmadd(-cdot(aa, n), n, aa));
+}
+/* quat_rotate: c = fma(2w, w, -1), d = 2*cdot(qv, x); x*c + qv*d + (cross(qv,x)*w)*2 */
+static inline v3 cquat_rotate(qt q, v3 x) {
+    real c = RFMA(2.0f * q.w, q.w, -1.0f);
+    real d = 2.0f * RFMA(q.z, x.z, RFMA(q.x, x.x, q.y * x.y));
+    v3 cr = ccross(V3(q.x, q.y, q.z), x);
+    return V3(RFMA(cr.x * q.w, 2.0f, RFMA(x.x, c, q.x * d)), RFMA(cr.y * q.w, 2.0f, RFMA(x.y, c, q.y * d)),
+              RFMA(cr.z * q.w, 2.0f, RFMA(x.z, c, q.z * d)));
+}
+/* quat_to_matrix: column k is quat_rotate(q, e_k) with the unit factors folded (d = 2*q_k, the
+ * diagonal fma(q_k, d, c)) */
+static inline m33 cquat_to_matrix(qt q) {
+    real c = RFMA(2.0f * q.w, q.w, -1.0f);
+    real qq[3] = {q.x, q.y, q.z};
+    m33 R;
+    for (int k = 0; k < 3; ++k) {
+        real d = 2.0f * qq[k];
+        v3 cr = vcross(V3(q.x, q.y, q.z), V3(k == 0, k == 1, k == 2));
+        real cj[3] = {cr.x, cr.y, cr.z};
+        for (int j = 0; j < 3; ++j) {
+            real inner = j == k ? RFMA(qq[j], d, c) : RFMA(qq[j], d, 0.0f * c);
+            R.m[j][k] = RFMA(cj[j] * q.w, 2.0f, inner);
+        }
+    }
+    return R;
+}
+static inline qt cqmul(qt a, qt b) {
+    qt r;
+    r.x = RFMA(-b.y, a.z, RFMA(a.y, b.z, RFMA(a.w, b.x, b.w * a.x)));
+    r.y = RFMA(-b.z, a.x, RFMA(a.z, b.x, RFMA(a.w, b.y, b.w * a.y)));
+    r.z = RFMA(-b.x, a.y, RFMA(a.x, b.y, RFMA(a.w, b.z, b.w * a.z)));
+    r.w = RFMA(-a.z, b.z, RFMA(-a.y, b.y, RFMA(a.w, b.w, -(a.x * b.x))));
+    return r;
+}
+static inline qt cqnormalize(qt q) {
+    real l = RSQRT(RFMA(q.w, q.w, RFMA(q.z, q.z, RFMA(q.x, q.x, q.y * q.y))));
     qt r;
     if (l > 0.0f) {
         real inv = 1.0f / l;
@@ -224,24 +318,23 @@ void or_set_velocity(or_world *w, const real *v, const real *om) {
 static void grav(or_world *w, real dt) {
     for (int i = 0; i < w->N; ++i)
         if (w->invM[i] > 0.0f) {
-            v3 g = vmul(V3(0.0f, 0.0f, -9.81f), dt);
-            st3(w->v, i, vadd(ld3(w->v, i), g));
+            st3(w->v, i, cmadd(dt, V3(0.0f, 0.0f, -9.81f), ld3(w->v, i)));
         }
 }
 
 /* ── A3 k_invIw (contact_engine_gpu.py:47-53): invIw = (R*IbInv)*R^T, zero if kinematic ── */
 static void inv_inertia_world(or_world *w) {
     for (int i = 0; i < w->N; ++i) {
-        m33 R = quat_to_matrix(ldq(w->q, i));
+        m33 R = cquat_to_matrix(ldq(w->q, i));
         stm(w->Rw, i, &R);
         if (w->kin[i] == 1) {
             m33 z; memset(&z, 0, sizeof z);
             stm(w->invIw, i, &z);
         } else {
             m33 Ib = ldm(w->IbInv, i);
-            m33 t = mm(&R, &Ib);
+            m33 t = cmm(&R, &Ib);
             m33 Rt = mtrans(&R);
-            m33 r = mm(&t, &Rt);
+            m33 r = cmm(&t, &Rt);
             stm(w->invIw, i, &r);
         }
     }
@@ -252,15 +345,18 @@ static void integrate_range(int N, real *x, real *q, const real *v, const real *
                             const real *pv, const real *po, const real *invM, real dt) {
     for (int i = 0; i < N; ++i) {
         if (invM[i] <= 0.0f) continue;
-        st3(x, i, vadd(ld3(x, i), vmul(vadd(ld3(v, i), ld3(pv, i)), dt)));
+        v3 s = vadd(ld3(v, i), ld3(pv, i)), xi = ld3(x, i);
+        st3(x, i, V3(RFMA(s.x, dt, xi.x), RFMA(s.y, dt, xi.y), RFMA(s.z, dt, xi.z)));
         v3 ww = vadd(ld3(om, i), ld3(po, i));
-        qt wq = {ww.x, ww.y, ww.z, 0.0f};
-        qt h = {0.5f * wq.x, 0.5f * wq.y, 0.5f * wq.z, 0.5f * wq.w};
+        qt h = {0.5f * ww.x, 0.5f * ww.y, 0.5f * ww.z, 0.5f * 0.0f};
         qt qi = ldq(q, i);
-        qt m = qmul(h, qi);
-        qt md = {m.x * dt, m.y * dt, m.z * dt, m.w * dt};
-        qt qn = {qi.x + md.x, qi.y + md.y, qi.z + md.z, qi.w + md.w};
-        qt r = qnormalize(qn);
+        qt m = cqmul(h, qi);
+#ifdef OR_PLANT_UNFUSED_INTEG
+        qt qn = {qi.x + m.x * dt, qi.y + m.y * dt, qi.z + m.z * dt, qi.w + m.w * dt};
+#else
+        qt qn = {RFMA(m.x, dt, qi.x), RFMA(m.y, dt, qi.y), RFMA(m.z, dt, qi.z), RFMA(m.w, dt, qi.w)};
+#endif
+        qt r = cqnormalize(qn);
         q[4 * i] = r.x; q[4 * i + 1] = r.y; q[4 * i + 2] = r.z; q[4 * i + 3] = r.w;
     }
 }
@@ -322,7 +418,7 @@ static void bead_contacts(or_world *w) {
     for (int g = 0; g < NP; ++g) {
         int b = g / P;
         qt q = ldq(w->q, b);
-        v3 p = vadd(ld3(w->x, b), quat_rotate(q, ld3(w->rest, g % P)));
+        v3 p = vadd(ld3(w->x, b), cquat_rotate(q, ld3(w->rest, g % P)));
         st3(w->allp, g, p);
     }
     real h = 2.0f * R;
@@ -364,7 +460,7 @@ static void bead_contacts(or_world *w) {
             int j = cand[a];
             v3 xj = ld3(w->allp, j);
             v3 dv = vsub(xi, xj);
-            real dist = vlen(dv);
+            real dist = RSQRT(cdot(dv, dv));
             if (dist < 2.0f * R && dist > 1e-9f)
                 push_contact(w, (int64_t)g * stride + (int64_t)j + 1, bi, j / P, xi, xj,
                              vdiv(dv, dist), 2.0f * R - dist, mu);
@@ -713,31 +809,29 @@ static void jac_vel(or_world *w, int c, real scale) {
     v3 nrm = ld3(w->cn, c), rA = vsub(ld3(w->cpA, c), ld3(w->x, bi));
     real iMA = w->invM[bi];
     m33 IA = ldm(w->invIw, bi), IB;
-    v3 va = vadd(ld3(w->v, bi), vcross(ld3(w->w, bi), rA));
+    v3 va = vadd(ld3(w->v, bi), ccross(ld3(w->w, bi), rA));
     v3 vb = V3(0.0f, 0.0f, 0.0f), rB = V3(0.0f, 0.0f, 0.0f);
     real iMB = 0.0f;
     if (bj >= 0) {
         rB = vsub(ld3(w->cpB, c), ld3(w->x, bj));
-        vb = vadd(ld3(w->v, bj), vcross(ld3(w->w, bj), rB));
+        vb = vadd(ld3(w->v, bj), ccross(ld3(w->w, bj), rB));
         iMB = w->invM[bj];
         IB = ldm(w->invIw, bj);
     }
     v3 vrel = vsub(va, vb);
-    real vn = vdot(vrel, nrm);
-    v3 vt = vsub(vrel, vmul(nrm, vn));
-    real mt = vlen(vt);
+    real vn = cdot(vrel, nrm);
+    v3 vt = cmadd(-vn, nrm, vrel);
+    real mt = RSQRT(cdot(vt, vt));
     v3 t1 = vdiv(vt, mt);
     if (mt <= 5e-3f) {
-        v3 aa = V3(1.0f, 0.0f, 0.0f);
-        if (RFABS(nrm.x) >= 0.9f) aa = V3(0.0f, 1.0f, 0.0f);
-        t1 = vnormalize(vsub(aa, vmul(nrm, vdot(aa, nrm))));
+        t1 = cfixed_t1(nrm);
     }
-    v3 t2 = vcross(nrm, t1);
-    v3 rnA = vcross(rA, nrm);
-    real meff = iMA + vdot(rnA, mv(&IA, rnA));
+    v3 t2 = ccross(nrm, t1);
+    v3 rnA = ccross(rA, nrm);
+    real meff = iMA + cdot(rnA, cmv(&IA, rnA));
     if (bj >= 0) {
-        v3 rnB = vcross(rB, nrm);
-        meff = meff + iMB + vdot(rnB, mv(&IB, rnB));
+        v3 rnB = ccross(rB, nrm);
+        meff = meff + iMB + cdot(rnB, cmv(&IB, rnB));
     }
     if (meff < 1e-12f) return;
     real dj = -vn / meff;
@@ -746,37 +840,37 @@ static void jac_vel(or_world *w, int c, real scale) {
     w->jn[c] = nw;
     v3 Jn = vmul(nrm, dj);
     acc3(w->acc, N, bi, 0, vmul(Jn, iMA), 1.0f, scale);
-    acc3(w->acc, N, bi, 3, mv(&IA, vcross(rA, Jn)), 1.0f, scale);
+    acc3(w->acc, N, bi, 3, cmv(&IA, ccross(rA, Jn)), 1.0f, scale);
     if (bj >= 0) {
         acc3(w->acc, N, bj, 0, vmul(Jn, iMB), -1.0f, scale);
-        acc3(w->acc, N, bj, 3, mv(&IB, vcross(rB, Jn)), -1.0f, scale);
+        acc3(w->acc, N, bj, 3, cmv(&IB, ccross(rB, Jn)), -1.0f, scale);
     }
-    v3 rtA1 = vcross(rA, t1);
-    real meft1 = iMA + vdot(rtA1, mv(&IA, rtA1));
-    v3 rtA2 = vcross(rA, t2);
-    real meft2 = iMA + vdot(rtA2, mv(&IA, rtA2));
+    v3 rtA1 = ccross(rA, t1);
+    real meft1 = iMA + cdot(rtA1, cmv(&IA, rtA1));
+    v3 rtA2 = ccross(rA, t2);
+    real meft2 = iMA + cdot(rtA2, cmv(&IA, rtA2));
     if (bj >= 0) {
-        v3 rtB1 = vcross(rB, t1);
-        meft1 = meft1 + iMB + vdot(rtB1, mv(&IB, rtB1));
-        v3 rtB2 = vcross(rB, t2);
-        meft2 = meft2 + iMB + vdot(rtB2, mv(&IB, rtB2));
+        v3 rtB1 = ccross(rB, t1);
+        meft1 = meft1 + iMB + cdot(rtB1, cmv(&IB, rtB1));
+        v3 rtB2 = ccross(rB, t2);
+        meft2 = meft2 + iMB + cdot(rtB2, cmv(&IB, rtB2));
     }
     real o1 = w->jt1[c], o2 = w->jt2[c];
-    real a1 = o1 - vdot(vrel, t1) / meft1;
-    real a2 = o2 - vdot(vrel, t2) / meft2;
-    real mag = RSQRT(a1 * a1 + a2 * a2);
+    real a1 = o1 - cdot(vrel, t1) / meft1;
+    real a2 = o2 - cdot(vrel, t2) / meft2;
+    real mag = RSQRT(RFMA(a1, a1, a2 * a2));
     real lim = w->cmu[c] * w->jn[c];
     if (mag > lim && mag > 1e-12f) {
         a1 = a1 * lim / mag;
         a2 = a2 * lim / mag;
     }
     w->jt1[c] = a1; w->jt2[c] = a2;
-    v3 Jt = vadd(vmul(t1, a1 - o1), vmul(t2, 0.5f * (a2 - o2)));
+    v3 Jt = cmadd(a1 - o1, t1, vmul(t2, 0.5f * (a2 - o2)));
     acc3(w->acc, N, bi, 0, vmul(Jt, iMA), 1.0f, scale);
-    acc3(w->acc, N, bi, 3, mv(&IA, vcross(rA, Jt)), 1.0f, scale);
+    acc3(w->acc, N, bi, 3, cmv(&IA, ccross(rA, Jt)), 1.0f, scale);
     if (bj >= 0) {
         acc3(w->acc, N, bj, 0, vmul(Jt, iMB), -1.0f, scale);
-        acc3(w->acc, N, bj, 3, mv(&IB, vcross(rB, Jt)), -1.0f, scale);
+        acc3(w->acc, N, bj, 3, cmv(&IB, ccross(rB, Jt)), -1.0f, scale);
     }
 }
 
@@ -786,37 +880,37 @@ static void jac_pos(or_world *w, int c, real dt, real scale) {
     v3 nrm = ld3(w->cn, c), rA = vsub(ld3(w->cpA, c), ld3(w->x, bi));
     real iMA = w->invM[bi];
     m33 IA = ldm(w->invIw, bi), IB;
-    v3 rel = vadd(ld3(w->pv, bi), vcross(ld3(w->po, bi), rA));
+    v3 rel = vadd(ld3(w->pv, bi), ccross(ld3(w->po, bi), rA));
     v3 rB = V3(0.0f, 0.0f, 0.0f);
     real iMB = 0.0f;
     if (bj >= 0) {
         rB = vsub(ld3(w->cpB, c), ld3(w->x, bj));
-        rel = vsub(rel, vadd(ld3(w->pv, bj), vcross(ld3(w->po, bj), rB)));
+        rel = vsub(rel, vadd(ld3(w->pv, bj), ccross(ld3(w->po, bj), rB)));
         iMB = w->invM[bj];
         IB = ldm(w->invIw, bj);
     }
-    v3 rnA = vcross(rA, nrm);
-    real meff = iMA + vdot(rnA, mv(&IA, rnA));
+    v3 rnA = ccross(rA, nrm);
+    real meff = iMA + cdot(rnA, cmv(&IA, rnA));
     if (bj >= 0) {
-        v3 rnB = vcross(rB, nrm);
-        meff = meff + iMB + vdot(rnB, mv(&IB, rnB));
+        v3 rnB = ccross(rB, nrm);
+        meff = meff + iMB + cdot(rnB, cmv(&IB, rnB));
     }
     if (meff < 1e-12f) return;
     real bias = 0.2f * wmax(w->cpen[c] - 1e-4f, 0.0f) / dt;
-    real dj = (bias - vdot(rel, nrm)) / meff;
+    real dj = (bias - cdot(rel, nrm)) / meff;
     real nw = wmax(0.0f, w->jp[c] + dj);
     dj = nw - w->jp[c];
     w->jp[c] = nw;
     v3 Jn = vmul(nrm, dj);
     acc3(w->acc, N, bi, 0, vmul(Jn, iMA), 1.0f, scale);
-    acc3(w->acc, N, bi, 3, mv(&IA, vcross(rA, Jn)), 1.0f, scale);
+    acc3(w->acc, N, bi, 3, cmv(&IA, ccross(rA, Jn)), 1.0f, scale);
     if (bj >= 0) {
         acc3(w->acc, N, bj, 0, vmul(Jn, iMB), -1.0f, scale);
-        acc3(w->acc, N, bj, 3, mv(&IB, vcross(rB, Jn)), -1.0f, scale);
+        acc3(w->acc, N, bj, 3, cmv(&IB, ccross(rB, Jn)), -1.0f, scale);
     }
 }
 
-/* k_apply (contact_engine_gpu.py:134-143): dv = f32(f64(acc) * f64(f32(1/scale))), v += relax*dv */
+/* k_apply (contact_engine_gpu.py:134-143): dv = f32(f64(acc) * f64(f32(1/scale))), v = fma(relax, dv, v) */
 static void jac_apply(or_world *w, real *vv, real *ww, real relax, real scale) {
     int N = w->N;
     real inv = 1.0f / scale;
@@ -827,8 +921,8 @@ static void jac_apply(or_world *w, real *vv, real *ww, real relax, real scale) {
         v3 dw = V3((real)((double)w->acc[3 * (size_t)N + i] * (double)inv),
                    (real)((double)w->acc[4 * (size_t)N + i] * (double)inv),
                    (real)((double)w->acc[5 * (size_t)N + i] * (double)inv));
-        st3(vv, i, vadd(ld3(vv, i), smul(relax, dv)));
-        st3(ww, i, vadd(ld3(ww, i), smul(relax, dw)));
+        st3(vv, i, cmadd(relax, dv, ld3(vv, i)));
+        st3(ww, i, cmadd(relax, dw, ld3(ww, i)));
         for (int k = 0; k < 6; ++k) w->acc[k * (size_t)N + i] = 0;
     }
 }
@@ -1174,16 +1268,16 @@ static void apply_warm_pair(or_world *w, int p) {
     for (int k = 0; k < w->pcount[p]; ++k) {
         int c = w->pidx[w->pstart[p] + k];
         v3 n = ld3(w->cn, c);
-        v3 t1 = fixed_t1(n);
-        v3 t2 = vcross(n, t1);
-        v3 imp = vadd(vadd(smul(w->jn[c], n), smul(w->jt1[c], t1)), smul(w->jt2[c], t2));
+        v3 t1 = cfixed_t1(n);
+        v3 t2 = ccross(n, t1);
+        v3 imp = cmadd(w->jt2[c], t2, cmadd(w->jn[c], n, smul(w->jt1[c], t1)));
         if (w->invM[a] > 0.0f) {
-            va = vadd(va, smul(w->invM[a], imp));
-            wa = vadd(wa, mv(&IA, vcross(vsub(ld3(w->cpA, c), ld3(w->x, a)), imp)));
+            va = cmadd(w->invM[a], imp, va);
+            wa = vadd(wa, cmv(&IA, ccross(vsub(ld3(w->cpA, c), ld3(w->x, a)), imp)));
         }
         if (b >= 0 && w->invM[b] > 0.0f) {
-            vb = vsub(vb, smul(w->invM[b], imp));
-            wb = vsub(wb, mv(&IB, vcross(vsub(ld3(w->cpB, c), ld3(w->x, b)), imp)));
+            vb = cmadd(-w->invM[b], imp, vb);
+            wb = vsub(wb, cmv(&IB, ccross(vsub(ld3(w->cpB, c), ld3(w->x, b)), imp)));
         }
     }
     if (w->invM[a] > 0.0f) { st3(w->v, a, va); st3(w->w, a, wa); }

@@ -34,11 +34,13 @@ typedef double real;
 #define RSQRT sqrt
 #define RFABS fabs
 #define RFLOOR floor
+#define RFMA fma
 #else
 typedef float real;
 #define RSQRT sqrtf
 #define RFABS fabsf
 #define RFLOOR floorf
+#define RFMA fmaf
 #endif
 
 #ifdef __cplusplus

@@ -4,8 +4,11 @@ reduction to 4 points per pair on the device, then the pair-chunk and ColorCache
 
 Bar: bit-exact states every step, equal counters (generated and solved contacts, pairs, colours),
 and equal contact lists once the device's pair-major keys are mapped back to the reference's
-feature keys. The generated-contact counts are also checked against the reference's archived
-receipts. Every test prints its case before the GPU work starts."""
+feature keys. On the archived scenes the HIP states are also compared with the reference's own
+archived trajectories: every archived position, rotation, velocity and force (stack8, 120 steps)
+and every per-step sha256 of positions, velocities and angular velocities (lattice1000, 40 steps)
+bit for bit, with the generated-contact count of every step. Every test prints its case before
+the GPU work starts."""
 import hashlib
 import importlib
 import json
@@ -56,6 +59,30 @@ def _assert_state_equal(e, o, tag):
                                  f"oracle {b[tuple(idx)]!r} (max |d| {np.max(np.abs(a - b)):.3e})")
 
 
+def _archive():
+    meta = json.loads((GOLD / "reference_trajectories.json").read_text())
+    with np.load(GOLD / "reference_trajectories.npz") as d:
+        arr = {k: d[k] for k in d.files}
+    return meta, arr
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _assert_stack8_archive(e, arr, kind, s):
+    pkg = _pkg()
+    x, q, v, w = e.state_numpy()
+    for name, a in (("xc", x), ("Rm", pkg.bodies.quat_to_R_host(q)), ("vc", v), ("om", w)):
+        assert np.array_equal(a, arr[f"{kind}__stack8__{name}"][s]), f"step {s}: {name} differs from the archive"
+
+
+def _assert_lattice_archive(e, rec, s):
+    x, _, v, w = e.state_numpy()
+    for name, a in (("xc", x), ("vc", v), ("om", w)):
+        assert _sha(a) == rec["sha256_per_step"][name][s], f"step {s}: {name} differs from the archive"
+
+
 def _assert_counters_equal(e, o, tag):
     co = o.counters()
     got = (e.n_contacts, e.n_contacts_solved, e.n_pairs, e.n_colors)
@@ -79,9 +106,8 @@ def _assert_bead_contacts_equal(e, o, tag):
 def test_bead_stack8_matches_oracle_and_archive_counts(oracle, kind, tiled):
     _say(f"{kind} bead stack8 x120 tiled={tiled}")
     pkg = _pkg()
-    meta = json.loads((GOLD / "reference_trajectories.json").read_text())
+    meta, arr = _archive()
     counts = meta["records"][f"{kind}__stack8"]["contacts"]
-    agree = {"pair_chunks": 60, "color_cache": 30}[kind]  # steps over which the oracle matches the archive
     bs = pkg.bodies.bead_bodies(pkg.scenes.stack(8), [700.0] * 8, [0] * 8)
     e = _bead_engine(kind, bs, [700.0] * 8, tiled=tiled)
     o = oracle.OracleWorld(bs, kind, vit=40, pit=10)
@@ -90,11 +116,12 @@ def test_bead_stack8_matches_oracle_and_archive_counts(oracle, kind, tiled):
         o.step(1 / 240)
         _assert_state_equal(e, o, f"step {s}")
         _assert_counters_equal(e, o, f"step {s}")
-        if s < agree:
-            assert e.n_contacts == counts[s], f"step {s}: generated {e.n_contacts} archive {counts[s]}"
+        assert e.n_contacts == counts[s], f"step {s}: generated {e.n_contacts} archive {counts[s]}"
+        _assert_stack8_archive(e, arr, kind, s)
         if s in (3, 8, 40, 119):
             _assert_bead_contacts_equal(e, o, f"step {s}")
     np.testing.assert_array_equal(e.contact_forces(), o.forces())
+    np.testing.assert_array_equal(e.contact_forces(), arr[f"{kind}__stack8__force"][119])
 
 
 @pytest.mark.parametrize("kind,tiled,graph", [("pair_chunks", False, False), ("color_cache", True, True),
@@ -104,9 +131,8 @@ def test_bead_lattice1000_matches_oracle(oracle, kind, tiled, graph):
     reference's ColorCache captures its solve; graph=False is the eager launch sequence."""
     _say(f"{kind} bead lattice1000 x40 tiled={tiled} graph={graph}")
     pkg = _pkg()
-    meta = json.loads((GOLD / "reference_trajectories.json").read_text())
-    counts = meta["records"][f"{kind}__lattice1000"]["contacts"]
-    agree = {"pair_chunks": 40, "color_cache": 26}[kind]
+    meta, _ = _archive()
+    rec = meta["records"][f"{kind}__lattice1000"]
     cen = pkg.scenes.lattice(1000)
     bs = pkg.bodies.bead_bodies(cen, [700.0] * 1000, [0] * 1000)
     e = _bead_engine(kind, bs, [700.0] * 1000, tiled=tiled, graph_capture=graph)
@@ -116,8 +142,8 @@ def test_bead_lattice1000_matches_oracle(oracle, kind, tiled, graph):
         o.step(1 / 240)
         _assert_state_equal(e, o, f"step {s}")
         _assert_counters_equal(e, o, f"step {s}")
-        if s < agree:
-            assert e.n_contacts == counts[s], f"step {s}"
+        assert e.n_contacts == rec["contacts"][s], f"step {s}"
+        _assert_lattice_archive(e, rec, s)
     _assert_bead_contacts_equal(e, o, "final")
     assert e.n_graph_captures == (1 if graph else 0)
 
@@ -179,8 +205,8 @@ def test_bead_jumble_with_kinematic_bodies_matches_oracle(oracle, kind, tiled):
 
 def test_jacobi_bead_lattice1000_matches_oracle_and_archive(oracle):
     """Relaxed Jacobi on the bead grid (90 contacts per 10 bodies, several per body in every wave):
-    the per-body entry lists and the 27-cell grid walk against the oracle, and the positions against
-    the archive's per-step sha256 (jacobi__lattice1000, 40 steps)."""
+    the per-body entry lists and the 27-cell grid walk against the oracle, and positions, velocities
+    and angular velocities against the archive's per-step sha256 (jacobi__lattice1000, 40 steps)."""
     _say("jacobi bead lattice1000 x40 vs oracle + archive digests")
     pkg = _pkg()
     rec = json.loads((GOLD / "reference_trajectories.json").read_text())["records"]["jacobi__lattice1000"]
@@ -193,9 +219,7 @@ def test_jacobi_bead_lattice1000_matches_oracle_and_archive(oracle):
         o.step(1 / 240)
         _assert_state_equal(e, o, f"step {s}")
         assert e.n_contacts == o.counters()["n_contacts"] == rec["contacts"][s], f"step {s}"
-        x = e.state_numpy()[0]
-        assert hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest() == rec["sha256_per_step"]["xc"][s], \
-            f"step {s}: positions differ from the archive"
+        _assert_lattice_archive(e, rec, s)
     np.testing.assert_array_equal(e.contact_forces(), o.forces())
 
 

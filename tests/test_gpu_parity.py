@@ -53,7 +53,7 @@ def test_jacobi_bead_stack8_matches_oracle_and_archive(oracle):
     _say("jacobi bead stack8 x120 vs oracle + archive")
     pkg = _pkg()
     with np.load(GOLD / "reference_trajectories.npz") as d:
-        gold_x = d["jacobi__stack8__xc"]
+        gold = {k: d[f"jacobi__stack8__{k}"] for k in ("xc", "Rm", "vc", "om", "force")}
     meta = json.loads((GOLD / "reference_trajectories.json").read_text())
     counts = meta["records"]["jacobi__stack8"]["contacts"]
     e = _engine("jacobi", vit=40, pit=10)
@@ -65,7 +65,10 @@ def test_jacobi_bead_stack8_matches_oracle_and_archive(oracle):
         o.step(1 / 240)
         _assert_state_equal(e, o, f"step {s}")
         assert e.n_contacts == counts[s]
-        np.testing.assert_array_equal(e.get_state().xc, gold_x[s])
+        st = e.get_state()  # every archived array of every step, bit for bit
+        for name in ("xc", "Rm", "vc", "om"):
+            np.testing.assert_array_equal(getattr(st, name), gold[name][s], err_msg=f"step {s}: {name}")
+        np.testing.assert_array_equal(e.contact_forces(), gold["force"][s], err_msg=f"step {s}: force")
     _assert_contacts_equal(e, o, "final")
     np.testing.assert_array_equal(e.contact_forces(), o.forces())
 

@@ -1,22 +1,12 @@
-"""The CPU oracle's coloured bead path against the reference's archived trajectories.
+"""The CPU oracle's coloured bead path: the colouring of the reference's archived lattice and
+planted mistakes the archive rejects.
 
 The archive (reports/position_solve_matrix/inputs/cross_hardware_l4.npz, fixtures extracted by
-tests/golden/make_golden.py) holds the reference's pair-chunk engine
-(UnfusedPositionPairContactEngine: slip-frame chunks, no warm apply, colours recomputed every step)
-and its ColorCache engine (UnfusedPositionColorCacheContactEngine: warm apply, fixed-frame chunks,
-colour cache), both with the device manifold reduction to 4 points per body pair, vit 40, pit 10,
-dt 1/240, one substep (probes/cross_position_state_probe.py:19-64).
-
-What this pins: bead contact generation in feature-key order, the warm start by feature key, the
-E-optimal manifold reduction with float64 impulse redistribution, the pair build, the Jones-Plassmann
-colouring with the PCG-hash priority, the colour cache, warm apply and both chunk sweeps. Through the
-first contact step the states are bit-identical. After it they agree to ulps and then drift: the
-reference compiled its integrator and world-inertia kernels with FMA contraction (Warp's default
-fuse_fp) and the restatement keeps contraction off, so the quaternion update differs in the last bit
-from the first rotating step on, and stacked contacts amplify that. The tolerances below sit one to
-two orders above the measured drift; the planted-mistake test shows that a wrong hash seed or a
-wrong tie-break in the reduction exceeds them by orders of magnitude."""
-import hashlib
+tests/golden/make_golden.py) is matched bit for bit by the oracle's pair-chunk and ColorCache
+engines over every archived step (test_oracle_golden.py). What this file adds: the lattice's
+colouring (every column is a 2-pair chain whose colour order the PCG-hash priority decides), and
+that a wrong hash seed or a wrong tie-break for the deepest point of the manifold reduction moves
+the archived velocities by orders of magnitude more than any rounding."""
 import importlib
 import json
 import subprocess
@@ -59,53 +49,18 @@ def _world(oracle, centres, engine):
     return oracle.OracleWorld(b, engine, vit=40, pit=10)
 
 
-@pytest.mark.parametrize("engine,count_steps", [("pair_chunks", 60), ("color_cache", 30)])
-def test_bead_colored_stack8_against_archive(pkg, oracle, gold, engine, count_steps):
-    meta, arr = gold
-    bodies = importlib.import_module("3fold-motion-engine_amd.bodies")
-    o = _world(oracle, stack(8), engine)
-    rec = meta["records"][f"{engine}__stack8"]
-    for s in range(count_steps):
+def test_lattice1000_colouring(pkg, oracle, gold):
+    """At step 8 all 1000 bodies are in contact pairs, two per column, and the columns need two
+    colours; the pair count and colour count equal those of the reference's colouring."""
+    meta, _ = gold
+    o = _world(oracle, lattice(1000), "pair_chunks")
+    rec = meta["records"]["pair_chunks__lattice1000"]
+    for s in range(9):
         o.step(1 / 240, 1)
-        x, q, v, w = o.state()
-        c = o.counters()
-        assert c["n_generated"] == rec["contacts"][s], f"step {s}"
-        assert c["n_contacts"] <= 4 * c["n_pairs"]
-        if s <= 3:  # up to and including the first contact step: bit for bit
-            np.testing.assert_array_equal(x, arr[f"{engine}__stack8__xc"][s])
-            np.testing.assert_array_equal(bodies.quat_to_R_host(q), arr[f"{engine}__stack8__Rm"][s])
-            np.testing.assert_array_equal(v, arr[f"{engine}__stack8__vc"][s])
-            np.testing.assert_array_equal(w, arr[f"{engine}__stack8__om"][s])
-        elif s < 16:
-            assert np.max(np.abs(x - arr[f"{engine}__stack8__xc"][s])) <= 1e-7
-            assert np.max(np.abs(v - arr[f"{engine}__stack8__vc"][s])) <= 1e-6
-            assert np.max(np.abs(w - arr[f"{engine}__stack8__om"][s])) <= 1e-5
-            assert np.max(np.abs(o.forces() - arr[f"{engine}__stack8__force"][s])) <= 1e-2
-
-
-@pytest.mark.parametrize("engine,count_steps", [("pair_chunks", 40), ("color_cache", 26)])
-def test_bead_colored_lattice1000_against_archive(pkg, oracle, gold, engine, count_steps):
-    """Generated-contact counts equal the archive's receipts over count_steps steps (ColorCache
-    gains one near-threshold contact from step 26 on)."""
-    meta, arr = gold
-    o = _world(oracle, lattice(1000), engine)
-    rec = meta["records"][f"{engine}__lattice1000"]
-    steps = list(arr[f"{engine}__lattice1000__steps"])
-    for s in range(count_steps):
-        o.step(1 / 240, 1)
-        c = o.counters()
-        assert c["n_generated"] == rec["contacts"][s], f"step {s}"
-        x, q, v, w = o.state()
-        if s <= 3:
-            assert hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest() == rec["sha256_per_step"]["xc"][s]
-            assert hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest() == rec["sha256_per_step"]["vc"][s]
-        if s in steps and s <= 9:
-            k = steps.index(s)
-            assert np.max(np.abs(x - arr[f"{engine}__lattice1000__xc"][k])) <= 1e-8
-            assert np.max(np.abs(v - arr[f"{engine}__lattice1000__vc"][k])) <= 1e-6
-            assert np.max(np.abs(w - arr[f"{engine}__lattice1000__om"][k])) <= 1e-5
-        if s == 8:  # every column of the lattice is a 2-pair chain: the hash decides its colour order
-            assert c["n_pairs"] == 1000 and c["n_colors"] == 2
+        assert o.counters()["n_generated"] == rec["contacts"][s]
+    c = o.counters()
+    assert c["n_pairs"] == 1000 and c["n_colors"] == 2
+    assert c["n_contacts"] <= 4 * c["n_pairs"]
 
 
 def _planted(tmp_path, flag):
