@@ -353,7 +353,7 @@ def bench_pile(a, large=False):
         steps_c, secs_c = cpu_baseline(x0, q0, v0, w0, bodies, a.solver, a.vit, a.pit, S,
                                        a.colors if a.solver != "jacobi" else 0, a.cpu_seconds)
         cpu = {"value": round(nb * steps_c / secs_c, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle (C, -O3, contraction off, 1 thread) from the GPU state at the start "
+               "sample": f"CPU oracle (C, -O3, 1 thread) from the GPU state at the start "
                          f"of the timed window (step {settle + a.warmup}): {steps_c} steps of the same {nb}-box "
                          f"pile in {secs_c:.1f} s on {model} ({os.cpu_count()} visible cores)"}
         if a.solver != "jacobi":

@@ -3,22 +3,23 @@
  * This library is the parity checker for the HIP path. Only tests/, the smoke check and the
  * benchmark's CPU-baseline leg may load it; the product never links or calls it.
  *
- * Every routine follows the reference kernel it restates in operation order, with
- * floating-point contraction off (built with -ffp-contract=off), float32 state, the int64
- * fixed-point scale 1e10 and float64 quantisation. Citations are to files of the reference
+ * Every routine follows the reference kernel it restates in operation order, float32 state, the
+ * int64 fixed-point scale 1e10 and float64 quantisation. The build keeps the compiler's
+ * contraction off (-ffp-contract=off); where the reference kernel is built with FMA contraction,
+ * its fused multiply-adds are written out explicitly (RFMA; the rules are in oracle.c's header). Citations are to files of the reference
  * repository antonbj3/3fold-motion-engine (src/motion_engine/...).
  *
  * Parity pins (see tests/test_oracle_golden.py and tests/test_oracle_invariants.py):
  *   - body build, world points, bead contacts, relaxed Jacobi, integrate: the reference's
  *     archived trajectories reports/position_solve_matrix/inputs/cross_hardware_l4.npz
- *     (jacobi stack8 x120, lattice1000 x40) and per-step contact counts;
+ *     (jacobi stack8 x120, lattice1000 x40), bit for bit, and per-step contact counts;
  *   - box narrow phase (an extension; the reference's bead model cannot express boxes):
  *     closed forms (resting box -> 4 corners at known depth; aligned stacked boxes -> the
  *     overlap rectangle), brute-force SAT on tiny inputs;
  *   - pair build, JP colouring: invariants (proper colouring, sortedness) + brute force;
  *   - bead manifold reduction, pair build, JP colouring (PCG hash), ColorCache and pair-chunk
  *     solvers: the same archive's pair_chunks and color_cache trajectories (stack8 x120,
- *     lattice1000 x40); invariants (proper colouring, sortedness) + brute force;
+ *     lattice1000 x40), bit for bit; invariants (proper colouring, sortedness) + brute force;
  *   - coloured chunk solvers on boxes: physics checks (resting force = Mg, stick/slide at atan mu).
  */
 #ifndef MOTION_ORACLE_H

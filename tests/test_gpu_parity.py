@@ -1,7 +1,8 @@
 """HIP path against the CPU oracle, through the C ABI (libmotion_hip.so).
 
-Bar: bit-exact (integer, index and float32 state alike; both sides compute with contraction off in
-the reference's operation order). Every test prints its case before the GPU work starts."""
+Bar: bit-exact (integer, index and float32 state alike; both sides compute in the reference's
+operation order, with its fused multiply-adds written out where it builds a kernel with contraction).
+Every test prints its case before the GPU work starts."""
 import importlib
 import json
 from pathlib import Path
