@@ -15,7 +15,8 @@ CSRC = HERE / "csrc"
 OUT = HERE / "libmotion_hip.so"
 SOURCES = ["motion_hip.hip"]
 
-# -ffp-contract=off: no fused multiply-add anywhere (bit parity with the CPU oracle);
+# -ffp-contract=off: no fused multiply-add but those the source writes out (fmaf, as the CPU oracle
+# writes the same ones) -- bit parity with the oracle;
 # IEEE fp32 div/sqrt and denormals kept; packed fp32 ops disabled (some packed forms have
 # returned wrong results on these machines).
 HIPFLAGS = [

@@ -37,8 +37,10 @@ struct TiledBuffers {
     int *psorted2;               // big tiles: pairs counting-sorted by colour
     int *rec;                    // big tiles: 8 ints per colour-sorted pair (p, a, b, la, lb, st, cnt, flags)
     float *prep;                 // 20 floats per contact: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu
-    float *sprep;                // big tiles: a 136-float record per colour-sorted pair (body header, then
-                                 // 4 x 24 floats: prep + impulses; T_REC in tiled_big.hpp)
+    float *sprep;                // big tiles: a T_REC-float record per colour-sorted pair (read only in the
+                                 // sweeps: body header, then 4 x 20 floats of prep; T_REC in tiled_big.hpp)
+    float *simp;                 // big tiles: the impulses (jn, jt1, jt2, jp) of a colour-sorted pair's
+                                 // contacts, 16 floats per pair
     int *srec;                   // big tiles: 4 ints per colour-sorted pair (p, a, b, first contact)
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
     long long *bigw;             // tile work since the last reset, 4 counts (tiles, pairs, dynamic pair ends,
@@ -128,7 +130,8 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.invIws = B.template take<float>(12 * n);
     t.psorted2 = B.template take<int>(p);
     t.rec = B.template take<int>(8 * p);
-    t.sprep = B.template take<float>(136 * p);
+    t.sprep = B.template take<float>(120 * p);
+    t.simp = B.template take<float>(16 * p);
     t.srec = B.template take<int>(4 * p);
     t.dbg = B.template take<long long>(16);
     t.bigw = B.template take<long long>(12);

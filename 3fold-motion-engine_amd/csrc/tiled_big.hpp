@@ -1,10 +1,10 @@
 // Big tiles (more pairs than one small tile holds): one 1024-thread workgroup per tile.
 //
 // LDS mode (tile within T_BIG_B bodies and T_BIG_P pairs): the tile's pairs are counting-sorted by
-// colour; each sorted pair keeps a 32-bit word in LDS (body slots, contact count, flags) and its
-// contact constants and impulses as 96-byte records in global memory in sorted order, so a colour
-// step is one LDS read plus one round of 16-byte vector loads per thread. Body velocities
-// (velocity passes) and then pseudo-velocities (position passes) sit in LDS.
+// colour; each sorted pair keeps a 32-bit word in LDS (body slots, contact count, flags), its contact
+// constants as a record in global memory in sorted order (T_REC), and its impulses in 64 bytes beside
+// (simp), so a colour step is one LDS read plus one round of 16-byte vector loads per thread. Body
+// velocities (velocity passes) and then pseudo-velocities (position passes) sit in LDS.
 // Larger tiles (giant islands): k_t_giant, the same sweeps by a grid of workgroups with body state in
 // global memory.
 #pragma once
@@ -17,13 +17,13 @@ __device__ __forceinline__ unsigned pw_make(int la, int lb, int cnt, bool dA, bo
            ((unsigned)dA << 28) | ((unsigned)dB << 29) | ((unsigned)hB << 30);
 }
 
-__device__ __forceinline__ void ld_sprep(const float *o, CCon &C) {
+__device__ __forceinline__ void ld_sprep(const float *o, const float *oi, CCon &C) {
     float4 q0 = *reinterpret_cast<const float4 *>(o);
     float4 q1 = *reinterpret_cast<const float4 *>(o + 4);
     float4 q2 = *reinterpret_cast<const float4 *>(o + 8);
     float4 q3 = *reinterpret_cast<const float4 *>(o + 12);
     float4 q4 = *reinterpret_cast<const float4 *>(o + 16);
-    float4 q5 = *reinterpret_cast<const float4 *>(o + 20);
+    float4 q5 = *reinterpret_cast<const float4 *>(oi);
     C.n = mk(q0.x, q0.y, q0.z); C.rA = mk(q0.w, q1.x, q1.y); C.rB = mk(q1.z, q1.w, q2.x);
     C.t1 = mk(q2.y, q2.z, q2.w); C.t2 = mk(q3.x, q3.y, q3.z);
     C.meff = q3.w; C.meft1 = q4.x; C.meft2 = q4.y; C.bias = q4.z; C.mu = q4.w;
@@ -62,12 +62,13 @@ __device__ __forceinline__ GRec grec_load(const int *r) {
 
 // A colour-sorted pair's record: a 40-float header -- both sides' inverse mass (and two pad floats),
 // inverse world inertia (12 floats each, row-major with 3 pad), and the velocity and angular velocity
-// of a static (not dynamic) side (12 floats: vA, wA, vB, wB) -- then per contact 24 floats (the 20
-// constants of k_t_prep and the impulses jn, jt1, jt2, jp). Everything a colour step reads from global
-// memory sits in the pair's own record, at an address known before the pair word is read, so the
-// step issues all of its global loads at once.
+// of a static (not dynamic) side (12 floats: vA, wA, vB, wB) -- then per contact the 20 constants of
+// k_t_prep. The record is written once per substep and only read by the sweeps; the impulses, which
+// every colour step rewrites, sit apart in 64 bytes per pair (simp), so a step dirties half a cache
+// line per pair instead of one line per contact. Everything a colour step reads from global memory
+// sits at addresses known before the pair word is read, so the step issues all of its loads at once.
 #define T_REC_HDR 40
-#define T_REC (T_REC_HDR + 96)
+#define T_REC (T_REC_HDR + 80)
 
 // One pair of a big tile for one colour step, contacts streamed one at a time (few live registers).
 // PASS 0 warm apply, 1 velocity, 2 position. LDS mode: body state s0/s1 in LDS by tile slot, the
@@ -131,7 +132,7 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
     // software pipeline: contact k+1 is loaded before contact k is solved and stored
     auto load = [&](int k, CCon &C) {
         if (LDSM) {
-            ld_sprep(o + 24 * k, C);
+            ld_sprep(o + 20 * k, T.simp + 16 * (size_t)q + 4 * k, C);
         } else {
             int c = st + k;
             ld_prep20(T.prep + 20 * (size_t)c, C);
@@ -151,7 +152,7 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
             if (PASS == 1) cc_vel<FIXED>(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB, t2d);
             else cc_pos(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
             if (LDSM) {
-                *reinterpret_cast<float4 *>(o + 24 * k + 20) = make_float4(C.jn, C.jt1, C.jt2, C.jp);
+                *reinterpret_cast<float4 *>(T.simp + 16 * (size_t)q + 4 * k) = make_float4(C.jn, C.jt1, C.jt2, C.jp);
             } else {
                 int c = st + k;
                 A.Cc.jn[c] = C.jn; A.Cc.jt1[c] = C.jt1; A.Cc.jt2[c] = C.jt2; A.Cc.jp[c] = C.jp;
@@ -286,9 +287,10 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         for (int i = 0; i < cnt; ++i) {
             const float4 *s4 = reinterpret_cast<const float4 *>(T.prep + 20 * (size_t)(st + i));
             float4 v0 = s4[0], v1 = s4[1], v2 = s4[2], v3 = s4[3], v4 = s4[4];
-            float4 v5 = make_float4(A.Cc.jn[st + i], A.Cc.jt1[st + i], A.Cc.jt2[st + i], A.Cc.jp[st + i]);
-            float4 *d = o + 6 * i;
-            d[0] = v0; d[1] = v1; d[2] = v2; d[3] = v3; d[4] = v4; d[5] = v5;
+            float4 *d = o + 5 * i;
+            d[0] = v0; d[1] = v1; d[2] = v2; d[3] = v3; d[4] = v4;
+            reinterpret_cast<float4 *>(T.simp + 16 * (size_t)q)[i] =
+                make_float4(A.Cc.jn[st + i], A.Cc.jt1[st + i], A.Cc.jt2[st + i], A.Cc.jp[st + i]);
         }
     }
     if (mdyn) atomicAdd(&wdyn, mdyn);
@@ -328,12 +330,12 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         int q = p0 + k;
         int st = T.srec[4 * (size_t)q + 3];
         int cnt = (int)((pword[k] >> 26) & 3) + 1;
-        const float *o = T.sprep + T_REC * (size_t)q + T_REC_HDR;
+        const float *o = T.simp + 16 * (size_t)q;
         for (int i = 0; i < cnt; ++i) {
-            A.Cc.jn[st + i] = o[24 * i + 20];
-            A.Cc.jt1[st + i] = o[24 * i + 21];
-            A.Cc.jt2[st + i] = o[24 * i + 22];
-            A.Cc.jp[st + i] = o[24 * i + 23];
+            A.Cc.jn[st + i] = o[4 * i];
+            A.Cc.jt1[st + i] = o[4 * i + 1];
+            A.Cc.jt2[st + i] = o[4 * i + 2];
+            A.Cc.jp[st + i] = o[4 * i + 3];
         }
     }
     for (int j = tid; j < nb; j += T_BIG_WG) {
