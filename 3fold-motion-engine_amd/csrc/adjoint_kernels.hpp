@@ -1,8 +1,8 @@
 // Reverse pass of one recorded ColorCache substep (tiled solve, small tiles, box geometry).
 //
 // Order (the forward order reversed): integrate -> position sweeps -> velocity sweeps -> warm apply
-// (k_adj_tiles, one workgroup per tile, replaying each sweep from its recorded start and reversing
-// its colour steps) -> per-contact constants and the warm-start carry (k_adj_contacts) -> narrow
+// (k_adj_tiles, one workgroup per tile, replaying each sweep from its recorded start -- each pair
+// keeping its contacts' pre-states in registers -- and reversing its colour steps) -> per-contact constants and the warm-start carry (k_adj_contacts) -> narrow
 // phase (k_adj_narrow, dual numbers) -> world inverse inertia (k_adj_body). Gravity adds a constant,
 // so its adjoint is the identity. Body-level sums from several contacts use float atomics.
 #pragma once
@@ -182,14 +182,15 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
                 if (k < P.cnt) jp0[k] = si[4 * (size_t)(P.c0 + k) + 3];
         }
         __syncthreads();
-        PairBodies<float> pre;
+        // the forward replay keeps each contact's pre-state, which the reverse step then reads
+        PairBodies<float> chain[4];
         for (int c = 0; c < nc; ++c) {
             if (c_own == c) {
-                pre = adj_load(P, sv, sw);
-                PairBodies<float> B = pre;
+                PairBodies<float> B = adj_load(P, sv, sw);
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     if (k >= P.cnt) continue;
+                    chain[k] = B;
                     float jp = jp0[k];
                     ad::pos_fwd(B, P.C[k], jp, P.f, P.iMA, P.iMB, P.IA, P.IB);
                 }
@@ -200,21 +201,10 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
         for (int c = nc - 1; c >= 0; --c) {
             if (c_own == c) {
                 PairBodies<float> Bb = adj_load_bar(P, bv, bw);
-                PairBodies<float> chain[4];
-                float jpre[4];
-                PairBodies<float> B = pre;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    if (k >= P.cnt) continue;
-                    chain[k] = B;
-                    jpre[k] = jp0[k];
-                    float jp = jp0[k];
-                    ad::pos_fwd(B, P.C[k], jp, P.f, P.iMA, P.iMB, P.IA, P.IB);
-                }
 #pragma unroll
                 for (int k = 3; k >= 0; --k)
                     if (k < P.cnt)
-                        ad::pos_rev(chain[k], P.C[k], jpre[k], P.f, P.iMA, P.iMB, P.IA, P.IB, Bb, jpb[k], cb[k], IAb,
+                        ad::pos_rev(chain[k], P.C[k], jp0[k], P.f, P.iMA, P.iMB, P.IA, P.IB, Bb, jpb[k], cb[k], IAb,
                                     IBb);
                 adj_store(P, bv, bw, Bb);
             }
@@ -255,14 +245,14 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
                     for (int m = 0; m < 3; ++m) j0[k][m] = si[4 * (size_t)(P.c0 + k) + m];
         }
         __syncthreads();
-        PairBodies<float> pre;
+        PairBodies<float> chain[4];
         for (int c = 0; c < nc; ++c) {
             if (c_own == c) {
-                pre = adj_load(P, sv, sw);
-                PairBodies<float> B = pre;
+                PairBodies<float> B = adj_load(P, sv, sw);
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     if (k >= P.cnt) continue;
+                    chain[k] = B;
                     float jn = j0[k][0], jt1 = j0[k][1], jt2 = j0[k][2];
                     ad::vel_fwd(B, P.C[k], jn, jt1, jt2, P.f, P.iMA, P.iMB, P.IA, P.IB);
                 }
@@ -273,15 +263,6 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
         for (int c = nc - 1; c >= 0; --c) {
             if (c_own == c) {
                 PairBodies<float> Bb = adj_load_bar(P, bv, bw);
-                PairBodies<float> chain[4];
-                PairBodies<float> B = pre;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    if (k >= P.cnt) continue;
-                    chain[k] = B;
-                    float jn = j0[k][0], jt1 = j0[k][1], jt2 = j0[k][2];
-                    ad::vel_fwd(B, P.C[k], jn, jt1, jt2, P.f, P.iMA, P.iMB, P.IA, P.IB);
-                }
 #pragma unroll
                 for (int k = 3; k >= 0; --k)
                     if (k < P.cnt)

@@ -654,7 +654,9 @@ __global__ void k_t_apply_split(int capP, const int *istat, int *col, TiledBuffe
 
 #define T_SMALL_P 256   // pairs of a small tile: one per thread, constants in registers
 #define T_SMALL_B 512   // bodies of a small tile (velocities in LDS)
+#ifndef T_BIG_B
 #define T_BIG_B 5000    // bodies of a big tile whose v, w (then pv, po) sit in LDS (120 KB)
+#endif
 
 // per-contact constants of one substep's solve (identical to recomputing them every sweep: their
 // inputs -- normal, contact points, centres, world inverse inertia -- do not change in the solve)
