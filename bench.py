@@ -72,8 +72,9 @@ def main():
     ap.add_argument("--pit", type=int, default=10)
     ap.add_argument("--tiled", type=int, default=1)
     ap.add_argument("--tile-bodies", type=int, default=0,
-                    help="bodies per tile of the tiled solve (0: 128, and 224 for the 256-scene batch -- the "
-                         "sizes the autotuner's tile search picks for them)")
+                    help="bodies per tile of the tiled solve (0: 128; 224 for the 256-scene batch, the size the "
+                         "autotuner's tile search picks for it; 256 for the adjoint's 64-box piles, four per tile, "
+                         "so that the reverse tile kernel's 256 tiles run in one round)")
     ap.add_argument("--graph", type=int, default=0, help="replay each step as one captured hipGraph (tiled path)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -424,7 +425,7 @@ def bench_adjoint(a):
     N = b.n
     dt, S = 1 / 240, a.substeps
     ro = pkg.adjoint.Rollout(b, steps=T, dt=dt, substeps=S, vit=a.vit, pit=a.pit, n_colors=a.colors, k=k,
-                             tile_bodies=a.tile_bodies or 128)
+                             tile_bodies=a.tile_bodies or 256)
     fn = pkg.adjoint.contact_rollout(ro)
     dev = ro.eng.x.device
     f = lambda x: torch.tensor(np.asarray(x), dtype=torch.float32, device=dev)
@@ -457,8 +458,10 @@ def bench_adjoint(a):
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded box piles)",
            "config": {"workload": f"{n_sc} x {n}-box piles, {T} steps, checkpoint k={k}, ColorCache S{S} V{a.vit} "
                                   f"P{a.pit}, forward + hand-written adjoint (grad of v0, w0, mu)",
-                      "bodies": N, "scenes": n_sc, "steps_per_rollout": T, "checkpoint_k": k},
+                      "bodies": N, "scenes": n_sc, "steps_per_rollout": T, "checkpoint_k": k,
+                      "tile_bodies": a.tile_bodies or 256},
            "adjoint": {"forward_ms_per_step": round(1e3 * fw / T, 4), "backward_ms_per_step": round(1e3 * bw / T, 4),
+                       "backward_over_forward": round(bw / fw, 2),
                        "adjoint_ms_per_step": round(ms_per_step, 4), "rollouts_timed": len(times),
                        "grad_norms_v0_w0_mu": gnorm},
            "roofline": None, "cpu_baseline": None}
