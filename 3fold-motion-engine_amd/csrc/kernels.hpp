@@ -13,6 +13,8 @@ enum {
     IS_NPAIRSORT, IS_NCOLJP, IS_NBIG, IS_CRAW, IS_JTOT, IS_NGIANT, IS_NGCOL,
     IS_BIGQ,  // next entry of the largest-first big-tile queue taken by k_t_big
     IS_COLQ,  // the same queue for the big tiles' colouring (k_t_color)
+    IS_NSMALL,  // small tiles of the substep (k_t_biglist)
+    IS_SMALLQ,  // next entry of the largest-first small-tile queue taken by k_t_small
     IS_COUNT = 32
 };
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,

@@ -134,6 +134,7 @@ struct me_engine {
     int giant_grid = 0;
     long long giant_wait_ticks = 0;
     int big_grid = 0;  // workgroups of k_t_big / k_t_color: one per CU their LDS allows
+    int small_grid = 0;  // workgroups of the persistent k_t_small: as many as can be resident
 };
 
 namespace {
@@ -760,6 +761,7 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.prof = e;
     a.giant_grid = e->giant_grid;
     a.big_grid = e->big_grid;
+    a.small_grid = e->small_grid;
     a.giant_wait_ticks = e->giant_wait_ticks;
     a.taped = e->taping ? 1 : 0;
     a.tape = e->tape;
@@ -813,6 +815,15 @@ int giant_setup(me_engine *e) {
     if (ob[2] < obig) obig = ob[2];
     e->big_grid = giant_grid_size(T_BIG_GRID_MAX, obig, obig, ncu);
     if (e->big_grid < 1) e->big_grid = 1;  // a workgroup that fits nowhere fails at launch with an error
+    // the small-tile kernel takes tiles from a queue too: every workgroup that can be resident
+    int os[4] = {0, 0, 0, 0};
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&os[0], (const void *)k_t_small<false, false>, T_SMALL_P, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&os[1], (const void *)k_t_small<true, false>, T_SMALL_P, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&os[2], (const void *)k_t_small<false, true>, T_SMALL_P, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&os[3], (const void *)k_t_small<true, true>, T_SMALL_P, 0));
+    int osm = os[0];
+    for (int k = 1; k < 4; ++k) osm = os[k] < osm ? os[k] : osm;
+    e->small_grid = (osm > 0 ? osm : 1) * (ncu > 0 ? ncu : 1);
     e->giant_wait_ticks = 100LL * khz;
     bool may_launch = (e->cfg.flags & ME_FLAG_TILED) && e->cfg.solver != ME_SOLVER_JACOBI &&
                       (e->N > T_BIG_B || e->capP > T_BIG_P || e->N > T_COLB);

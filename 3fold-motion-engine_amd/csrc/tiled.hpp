@@ -47,6 +47,8 @@ struct TiledBuffers {
                                  // contacts) for each of k_t_big [0..3], k_t_small [4..7], k_t_giant [8..11]
     int *biglist;                // tiles beyond one small tile (IS_NBIG of them), in k_t_biglist's order
     int *bigq;                   // the same tiles, most pairs first (ties: lower tile index): the work queue
+    int *smalllist;              // tiles within one small tile (IS_NSMALL of them), in k_t_biglist's order
+    int *smallq;                 // the same, most pairs first (ties: lower tile index): k_t_small's queue
     int *giantlist;              // tiles beyond the big tile's LDS (IS_NGIANT of them): k_t_giant's
     int *gcollist;               // tiles beyond the LDS colouring limits (IS_NGCOL): k_t_color_giant's
     int *gcst;                   // giant tile: colour-class starts (ME_NCOLMAX + 1), class count at [ME_NCOLMAX + 1]
@@ -92,6 +94,7 @@ struct TiledArgs {
     int taped;                       // record the substep into tape (k_t_small<.., true>)
     int giant_grid;                  // workgroups of the giant-island grid kernels, all resident together
     int big_grid;                    // workgroups of the big-tile kernels (one per CU the LDS allows)
+    int small_grid;                  // workgroups of the persistent small-tile kernel (all resident ones)
     long long giant_wait_ticks;      // bound of a grid barrier's wait, in wall-clock counter ticks
     AdjTape tape;
 };
@@ -137,6 +140,8 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.bigw = B.template take<long long>(12);
     t.biglist = B.template take<int>(n);
     t.bigq = B.template take<int>(n);
+    t.smalllist = B.template take<int>(n);
+    t.smallq = B.template take<int>(n);
     t.giantlist = B.template take<int>(n);
     t.gcollist = B.template take<int>(n);
     t.gcst = B.template take<int>(ME_NCOLMAX + 2);
@@ -160,6 +165,8 @@ __global__ void k_t_init(int N, TiledBuffers T, int *istat, int lds_islands) {
         istat[IS_NBIG] = 0;
         istat[IS_BIGQ] = 0;
         istat[IS_COLQ] = 0;
+        istat[IS_NSMALL] = 0;
+        istat[IS_SMALLQ] = 0;
         istat[IS_NGIANT] = 0;
         istat[IS_NGCOL] = 0;
         T.dbg[7] = 0;
@@ -942,104 +949,115 @@ __device__ __forceinline__ void tape_sweep(const TiledArgs &A, const TiledBuffer
     }
 }
 
+// Small tiles by a persistent grid: each workgroup takes the next tile of the small-tile queue
+// (k_t_bigsort: most pairs first) until the queue is empty, so a batch of uneven scenes finishes close
+// to max(largest tile, total work / workgroups) whatever the number of tiles.
 template <bool FIXED, bool TAPE>
 __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers T) {
-    int t = blockIdx.x;
-    if (T.tb0[t] == INT_MAX) return;
-    int b0, nb, p0, np;
-    tile_bounds(T, t, b0, nb, p0, np);
-    if (!is_small_tile(nb, np)) return;
     __shared__ float sv[3 * T_SMALL_B], sw[3 * T_SMALL_B], spv[3 * T_SMALL_B], spo[3 * T_SMALL_B];
     __shared__ int cnt[ME_NCOLMAX + 1], fill[ME_NCOLMAX + 1], plist[T_SMALL_P], plcol[T_SMALL_P];
-    __shared__ int maxc, wdyn, wcon;
+    __shared__ int maxc, wdyn, wcon, qi;
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        atomicAdd(&A.istat[IS_TILES], 1);
-        atomicMax(&A.istat[IS_MAXTB], nb);
-        maxc = 0;
-        wdyn = 0;
-        wcon = 0;
-    }
-    int ncol = A.istat[IS_NCOL];
-    if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
-    for (int j = tid; j < nb; j += T_SMALL_P) {
-        int i = T.bsorted[b0 + j];
+    const int nq = A.istat[IS_NSMALL];
+    for (;;) {
+        if (tid == 0) qi = atomicAdd(&A.istat[IS_SMALLQ], 1);
+        __syncthreads();
+        const int iq = qi;
+        __syncthreads();  // every thread has read qi before thread 0 takes the next entry
+        if (iq >= nq) break;
+        const int t = T.smallq[iq];
+        int b0, nb, p0, np;
+        tile_bounds(T, t, b0, nb, p0, np);
+        if (tid == 0) {
+            atomicAdd(&A.istat[IS_TILES], 1);
+            atomicMax(&A.istat[IS_MAXTB], nb);
+            maxc = 0;
+            wdyn = 0;
+            wcon = 0;
+        }
+        int ncol = A.istat[IS_NCOL];
+        if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
+        for (int j = tid; j < nb; j += T_SMALL_P) {
+            int i = T.bsorted[b0 + j];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            sv[3 * j + k] = A.b.v[3 * i + k];
-            sw[3 * j + k] = A.b.w[3 * i + k];
-            spv[3 * j + k] = 0.0f;
-            spo[3 * j + k] = 0.0f;
-        }
-    }
-    for (int c = tid; c <= ncol; c += T_SMALL_P) { cnt[c] = 0; fill[c] = 0; }
-    __syncthreads();
-    int myc = -1;
-    if (tid < np) {
-        myc = A.Pp.col[T.psorted[p0 + tid]];
-        if (myc < 0 || myc >= ncol) myc = ncol > 0 ? ncol - 1 : 0;  // only after a flagged colour overflow
-        atomicAdd(&cnt[myc], 1);
-        atomicMax(&maxc, myc + 1);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int c = 0; c < maxc; ++c) { int n = cnt[c]; cnt[c] = run; run += n; }
-    }
-    __syncthreads();
-    if (tid < np) {
-        int pos = cnt[myc] + atomicAdd(&fill[myc], 1);
-        plist[pos] = T.psorted[p0 + tid];
-        plcol[pos] = myc;
-    }
-    __syncthreads();
-    const int nc = maxc;
-    PairCache P;
-    int p = -1, c_own = -1;
-    if (tid < np) {
-        p = plist[tid];
-        c_own = plcol[tid];
-        pc_load<FIXED>(P, A, T, p, b0);
-    }
-    if (np > 0) {
-        if (FIXED) {
-            for (int c = 0; c < nc; ++c) {
-                if (c_own == c) pc_warm(P, sv, sw);
-                lds_barrier();
+            for (int k = 0; k < 3; ++k) {
+                sv[3 * j + k] = A.b.v[3 * i + k];
+                sw[3 * j + k] = A.b.w[3 * i + k];
+                spv[3 * j + k] = 0.0f;
+                spo[3 * j + k] = 0.0f;
             }
         }
-        const float t2d = FIXED ? 1.0f : A.t2d;
-        for (int it = 0; it < A.vit; ++it) {
-            if (TAPE) tape_sweep(A, T, b0, nb, it, sv, sw, P, p);
-            for (int c = 0; c < nc; ++c) {
-                if (c_own == c) pc_vel<FIXED>(P, sv, sw, t2d);
-                lds_barrier();
-            }
+        for (int c = tid; c <= ncol; c += T_SMALL_P) { cnt[c] = 0; fill[c] = 0; }
+        __syncthreads();
+        int myc = -1;
+        if (tid < np) {
+            myc = A.Pp.col[T.psorted[p0 + tid]];
+            if (myc < 0 || myc >= ncol) myc = ncol > 0 ? ncol - 1 : 0;  // only after a flagged colour overflow
+            atomicAdd(&cnt[myc], 1);
+            atomicMax(&maxc, myc + 1);
         }
-        for (int it = 0; it < A.pit; ++it) {
-            if (TAPE) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, p);
-            for (int c = 0; c < nc; ++c) {
-                if (c_own == c) pc_pos(P, spv, spo);
-                lds_barrier();
-            }
+        __syncthreads();
+        if (tid == 0) {
+            int run = 0;
+            for (int c = 0; c < maxc; ++c) { int n = cnt[c]; cnt[c] = run; run += n; }
         }
-    } else if (TAPE) {
-        for (int it = 0; it < A.vit; ++it) tape_sweep(A, T, b0, nb, it, sv, sw, P, -1);
-        for (int it = 0; it < A.pit; ++it) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, -1);
+        __syncthreads();
+        if (tid < np) {
+            int pos = cnt[myc] + atomicAdd(&fill[myc], 1);
+            plist[pos] = T.psorted[p0 + tid];
+            plcol[pos] = myc;
+        }
+        __syncthreads();
+        const int nc = maxc;
+        PairCache P;
+        int p = -1, c_own = -1;
+        if (tid < np) {
+            p = plist[tid];
+            c_own = plcol[tid];
+            pc_load<FIXED>(P, A, T, p, b0);
+        }
+        if (np > 0) {
+            if (FIXED) {
+                for (int c = 0; c < nc; ++c) {
+                    if (c_own == c) pc_warm(P, sv, sw);
+                    lds_barrier();
+                }
+            }
+            const float t2d = FIXED ? 1.0f : A.t2d;
+            for (int it = 0; it < A.vit; ++it) {
+                if (TAPE) tape_sweep(A, T, b0, nb, it, sv, sw, P, p);
+                for (int c = 0; c < nc; ++c) {
+                    if (c_own == c) pc_vel<FIXED>(P, sv, sw, t2d);
+                    lds_barrier();
+                }
+            }
+            for (int it = 0; it < A.pit; ++it) {
+                if (TAPE) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, p);
+                for (int c = 0; c < nc; ++c) {
+                    if (c_own == c) pc_pos(P, spv, spo);
+                    lds_barrier();
+                }
+            }
+        } else if (TAPE) {
+            for (int it = 0; it < A.vit; ++it) tape_sweep(A, T, b0, nb, it, sv, sw, P, -1);
+            for (int it = 0; it < A.pit; ++it) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, -1);
+        }
+        if (p >= 0) {
+            pc_store(P, A, p);
+            atomicAdd(&wdyn, (int)P.dynA + (int)P.dynB);
+            atomicAdd(&wcon, P.cnt);
+        }
+        __syncthreads();
+        if (tid == 0) {  // work of this tile for the algorithmic-byte count of the launch
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[4]), 1ull);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[5]), (unsigned long long)np);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[6]), (unsigned long long)wdyn);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[7]), (unsigned long long)wcon);
+        }
+
+        tile_writeback<T_SMALL_P>(A, T, b0, nb, true, sv, sw, spv, spo);
+        __syncthreads();  // the next tile reuses the LDS
     }
-    if (p >= 0) {
-        pc_store(P, A, p);
-        atomicAdd(&wdyn, (int)P.dynA + (int)P.dynB);
-        atomicAdd(&wcon, P.cnt);
-    }
-    __syncthreads();
-    if (tid == 0) {  // work of this tile for the algorithmic-byte count of the launch
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[4]), 1ull);
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[5]), (unsigned long long)np);
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[6]), (unsigned long long)wdyn);
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[7]), (unsigned long long)wcon);
-    }
-    tile_writeback<T_SMALL_P>(A, T, b0, nb, true, sv, sw, spv, spo);
 }
 
 // per-contact constants of the substep (k_t_prep): the same expressions pc_load evaluates
@@ -1175,29 +1193,37 @@ __global__ void k_t_biglist(int ntile, int *istat, const float *invM, const floa
     if (t >= ntile || T.tb0[t] == INT_MAX) return;
     int b0, nb, p0, np;
     tile_bounds(T, t, b0, nb, p0, np);
-    if (is_small_tile(nb, np)) return;
+    if (is_small_tile(nb, np)) {
+        T.smalllist[atomicAdd(&istat[IS_NSMALL], 1)] = t;
+        return;
+    }
     T.biglist[atomicAdd(&istat[IS_NBIG], 1)] = t;
     if (!(nb <= T_BIG_B && np <= T_BIG_P)) T.giantlist[atomicAdd(&istat[IS_NGIANT], 1)] = t;
     if (np > 0 && !(nb <= T_COLB && np <= T_COLP)) T.gcollist[atomicAdd(&istat[IS_NGCOL], 1)] = t;
 }
 
-// the big-tile queue: the listed tiles ranked by pair count, largest first (ties by tile index, so the
-// order does not depend on k_t_biglist's atomics). Workgroups that take tiles from the front in turn
-// finish the launch close to max(largest tile, total work / workgroups) -- the longest-processing-time
-// rule -- where a fixed stride could stack two large tiles on one workgroup. One thread per entry.
+// the big- and small-tile queues: each list's tiles ranked by pair count, largest first (ties by tile
+// index, so the order does not depend on k_t_biglist's atomics). Workgroups that take tiles from the
+// front in turn finish the launch close to max(largest tile, total work / workgroups) -- the
+// longest-processing-time rule -- where a fixed stride could stack two large tiles on one workgroup.
+// One thread per entry: the big list's first, then the small list's.
 __global__ void k_t_bigsort(int ntile, const int *istat, TiledBuffers T) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = istat[IS_NBIG];
+    const int nbig = istat[IS_NBIG], nsmall = istat[IS_NSMALL];
+    const bool big = i < nbig;
+    const int n = big ? nbig : nsmall;
+    if (!big) i -= nbig;
     if (i >= n || i >= ntile) return;
-    const int t = T.biglist[i];
+    const int *list = big ? T.biglist : T.smalllist;
+    const int t = list[i];
     const int w = T.tp1[t] - T.tp0[t];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-        int u = T.biglist[j];
+        int u = list[j];
         int wu = T.tp1[u] - T.tp0[u];
         rank += (wu > w || (wu == w && u < t)) ? 1 : 0;
     }
-    T.bigq[rank] = t;
+    (big ? T.bigq : T.smallq)[rank] = t;
 }
 
 // dynamic-LDS limits of the tile kernels beyond the 64 KB default; set on the current device before
@@ -1232,10 +1258,11 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
     if (prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
     if (A.taped)
-        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
+        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile < A.small_grid ? ntile : A.small_grid),
+                           dim3(T_SMALL_P), 0, s, A, T);
     else
-        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>), dim3(ntile), dim3(T_SMALL_P), 0, s, A,
-                           T);
+        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>),
+                           dim3(ntile < A.small_grid ? ntile : A.small_grid), dim3(T_SMALL_P), 0, s, A, T);
     if (prof_rec(A, ME_K_TILE_SMALL, s) || prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIG_WG), dyn, s2, A, T);
     if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
