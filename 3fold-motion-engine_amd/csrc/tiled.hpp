@@ -1256,16 +1256,18 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     const size_t dyn = T_BIG_DYN;
     hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
-    if (prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
+    // the big tiles (the critical path when there are any) are submitted first, so that their
+    // workgroups find CUs with free LDS before the small tiles fill them
+    if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
+    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIG_WG), dyn, s2, A, T);
+    if (prof_rec(A, ME_K_TILE_BIG, s2) || prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
     if (A.taped)
         hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile < A.small_grid ? ntile : A.small_grid),
                            dim3(T_SMALL_P), 0, s, A, T);
     else
         hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>),
                            dim3(ntile < A.small_grid ? ntile : A.small_grid), dim3(T_SMALL_P), 0, s, A, T);
-    if (prof_rec(A, ME_K_TILE_SMALL, s) || prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
-    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIG_WG), dyn, s2, A, T);
-    if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
+    if (prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
     // a tile beyond the big tile's LDS (more than T_BIG_B bodies or T_BIG_P pairs) is possible: the grid
     // sweep, after k_t_big, which leaves such tiles to it
     if (A.N > T_BIG_B || A.capP > T_BIG_P) {
