@@ -250,6 +250,28 @@ def test_batched_scenes_match_oracle_and_stay_independent(oracle):
     assert np.all(np.isfinite(xs))
 
 
+def test_tile_queue_with_more_tiles_than_resident_workgroups(oracle):
+    """The small tiles are taken from a largest-first queue by a persistent grid of as many workgroups
+    as can be resident. 700 scenes of 8 to 24 boxes at 8 bodies per tile make more tiles than that
+    grid has workgroups, so workgroups take several tiles each; the batch equals the oracle bit for
+    bit, and every tile is solved once (the tile count the kernels report equals the tiles built)."""
+    _say("color_cache tiled batch of 700 small scenes, 8 bodies per tile, x30 vs oracle")
+    pkg = _pkg()
+    bodies, sizes = pkg.scenes.batched_piles(700, lo=8, hi=24, seed=5)
+    e = _engine("color_cache", vit=8, pit=5, n_colors=16, tiled=True, tile_bodies=8)
+    e.add_boxes(bodies)
+    o = oracle.OracleWorld(bodies, "color_cache", vit=8, pit=5, n_colors=16, omp=True)
+    most = 0
+    for s in range(30):
+        e.step(1 / 240, 2)
+        o.step(1 / 240, 2)
+        _assert_state_equal(e, o, f"step {s}")
+        c = e.refresh_counters()
+        most = max(most, c["n_tiles"])
+    # k_t_small holds two workgroups per CU (its registers), 512 on the 256 CUs of an MI355X
+    assert most > 600, f"at most {most} tiles: the queue never held more tiles than resident workgroups"
+
+
 def test_graph_replay_matches_eager(oracle):
     """Whole-step hipGraphs (ME_FLAG_GRAPH): the engine replaying captured steps equals the eager
     engine bit for bit after every replay, with eager work between replays (state and counter
