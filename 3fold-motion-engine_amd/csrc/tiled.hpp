@@ -259,6 +259,41 @@ __device__ __forceinline__ int wave_add1(int *cnt, int key, bool active) {
     return atomicAdd(&cnt[key], 1);
 }
 
+// wave_add1 over a workgroup of NT threads: every lane whose key is that of the workgroup's first
+// active lane (in a large scene mostly the one island that holds most of it) gets its slot from one
+// atomic per workgroup, the others from wave_add1. The same-address atomics of an island of 90 000
+// bodies drop from one per wave to one per workgroup. Every thread of the workgroup calls it.
+template <int NT>
+__device__ __forceinline__ int block_add1(int *cnt, int key, bool active) {
+    constexpr int NW = NT / 64;
+    __shared__ int s_key[NW], s_has[NW], s_cnt[NW], s_base;
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
+    const unsigned long long act = __ballot(active);
+    const int first = act ? __ffsll((long long)act) - 1 : 0;
+    const int kw = __shfl(key, first);
+    if (lane == 0) { s_key[wv] = kw; s_has[wv] = act != 0ull; }
+    __syncthreads();
+    int k0 = 0, any = 0;
+    for (int w = 0; w < NW; ++w)
+        if (s_has[w]) { k0 = s_key[w]; any = 1; break; }
+    const bool mine = any && active && key == k0;
+    const unsigned long long same = __ballot(mine);
+    if (lane == 0) s_cnt[wv] = __popcll(same);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < NW; ++w) tot += s_cnt[w];
+        s_base = tot ? atomicAdd(&cnt[k0], tot) : 0;
+    }
+    __syncthreads();
+    int rank = s_base;
+    for (int w = 0; w < wv; ++w) rank += s_cnt[w];
+    rank += __popcll(same & ((1ull << lane) - 1ull));
+    const int other = wave_add1(cnt, key, active && !mine);
+    return mine ? rank : other;
+}
+#define T_AGG 1024  // workgroup of the island-build kernels that count or scatter by root
+
 // lock-free union: link the larger root under the smaller one (compare-and-swap on the root)
 __global__ void k_t_union(int capP, int *istat, const int *pbi, const int *pbj, const float *invM, int *parent) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -274,7 +309,7 @@ __global__ void k_t_union(int capP, int *istat, const int *pbi, const int *pbj, 
     atomicOr(&istat[IS_OVF], OVF_TILE);
 }
 
-__global__ void k_t_flatten(int N, const float *invM, TiledBuffers T) {
+__global__ void __launch_bounds__(T_AGG) k_t_flatten(int N, const float *invM, TiledBuffers T) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool dyn = i < N && invM[i] > 0.0f;
     int r = -1;
@@ -284,10 +319,10 @@ __global__ void k_t_flatten(int N, const float *invM, TiledBuffers T) {
     } else if (i < N) {
         T.root[i] = -1;
     }
-    wave_add1(T.cnt_b, r, dyn);
+    block_add1<T_AGG>(T.cnt_b, r, dyn);
 }
 
-__global__ void k_t_pair_root(int capP, const int *istat, const int *pbi, const int *pbj, const float *invM,
+__global__ void __launch_bounds__(T_AGG) k_t_pair_root(int capP, const int *istat, const int *pbi, const int *pbj, const float *invM,
                               TiledBuffers T) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int r = -1;
@@ -296,23 +331,23 @@ __global__ void k_t_pair_root(int capP, const int *istat, const int *pbi, const 
         r = invM[a] > 0.0f ? T.root[a] : ((b >= 0 && invM[b] > 0.0f) ? T.root[b] : -1);
         T.proot[p] = r;
     }
-    wave_add1(T.cnt_p, r, r >= 0);
+    block_add1<T_AGG>(T.cnt_p, r, r >= 0);
 }
 
-__global__ void k_t_scatter_b(int N, TiledBuffers T) {
+__global__ void __launch_bounds__(T_AGG) k_t_scatter_b(int N, TiledBuffers T) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int r = i < N ? T.root[i] : -1;
-    int slot = wave_add1(T.fill_b, r, r >= 0);
+    int slot = block_add1<T_AGG>(T.fill_b, r, r >= 0);
     if (r < 0) return;
     int pos = T.boff[r] + slot;
     T.bsorted[pos] = i;
     T.bslot[i] = pos;
 }
 
-__global__ void k_t_scatter_p(int capP, const int *istat, TiledBuffers T) {
+__global__ void __launch_bounds__(T_AGG) k_t_scatter_p(int capP, const int *istat, TiledBuffers T) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int r = (p < istat[IS_P] && p < capP) ? T.proot[p] : -1;
-    int slot = wave_add1(T.fill_p, r, r >= 0);
+    int slot = block_add1<T_AGG>(T.fill_p, r, r >= 0);
     if (r >= 0) T.psorted[T.poff[r] + slot] = p;
 }
 
@@ -1318,15 +1353,15 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     } else {
         hipLaunchKernelGGL(k_t_union, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
                            A.b.invM, T.parent);
-        hipLaunchKernelGGL(k_t_flatten, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, A.b.invM, T);
-        hipLaunchKernelGGL(k_t_pair_root, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
+        hipLaunchKernelGGL(k_t_flatten, dim3((N + T_AGG - 1) / T_AGG), dim3(T_AGG), 0, s, N, A.b.invM, T);
+        hipLaunchKernelGGL(k_t_pair_root, dim3((capP + T_AGG - 1) / T_AGG), dim3(T_AGG), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
                            A.b.invM, T);
         int rc = tiled_scan(A, T.cnt_b, T.boff, N, A.istat + IS_NBODYSORT, s);
         if (rc) return rc;
         rc = tiled_scan(A, T.cnt_p, T.poff, N, A.istat + IS_NPAIRSORT, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_t_scatter_b, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, T);
-        hipLaunchKernelGGL(k_t_scatter_p, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, T);
+        hipLaunchKernelGGL(k_t_scatter_b, dim3((N + T_AGG - 1) / T_AGG), dim3(T_AGG), 0, s, N, T);
+        hipLaunchKernelGGL(k_t_scatter_p, dim3((capP + T_AGG - 1) / T_AGG), dim3(T_AGG), 0, s, capP, A.istat, T);
         hipLaunchKernelGGL(k_t_tiles, dim3(tblk(N)), dim3(ME_THREADS), 0, s, N, TB, A.istat, T);
     }
     hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile > N ? ntile : N)), dim3(ME_THREADS), 0, s, ntile, A.istat, A.b.invM,
