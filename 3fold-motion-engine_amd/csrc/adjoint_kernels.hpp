@@ -382,7 +382,17 @@ __device__ __forceinline__ void adj_narrow_body(int a, int bb, int k, const floa
     qs.x.d[3] = 1.0f; qs.y.d[4] = 1.0f; qs.z.d[5] = 1.0f; qs.w.d[6] = 1.0f;
     V3<float> ha = g3(b.half, a);
     V3<D7> hA = ad::v3<D7>(D7(ha.x), D7(ha.y), D7(ha.z));
+    // values of the orientation as the forward step formed them (k_body_pre's contracted
+    // quat_to_matrix), so the dual narrow phase takes the forward's branches; derivatives from ad::
+    auto fwd_R = [](M3<D7> &R, Q4<float> q) {
+        f4 qf;
+        qf.x = q.x; qf.y = q.y; qf.z = q.z; qf.w = q.w;
+        const m3 Rf = cquat_to_matrix(qf);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) R.m[i][j].v = Rf.m[i][j];
+    };
     M3<D7> RA = ad::quat_to_matrix(qA);
+    fwd_R(RA, qa);
     V3<D7> pp[4], nn[4];
     D7 pen[4];
     int nk;
@@ -393,6 +403,7 @@ __device__ __forceinline__ void adj_narrow_body(int a, int bb, int k, const floa
         V3<float> hb = g3(b.half, bb);
         V3<D7> hB = ad::v3<D7>(D7(hb.x), D7(hb.y), D7(hb.z));
         M3<D7> RB = ad::quat_to_matrix(qB);
+        fwd_R(RB, g4(qpre, bb));
         nk = ad::box_box_g(xA, RA, hA, xB, RB, hB, margin, pp, nn, pen);
     }
     if (nk != k) { atomicOr(bad, OVF_TAPE); nk = nk < k ? nk : k; }
