@@ -807,9 +807,9 @@ int giant_setup(me_engine *e) {
     // T_BIG_GRID_MAX (more would only find the queue empty)
     int ob[3] = {0, 0, 0};
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob[0], (const void *)k_t_big<ME_SOLVER_PAIR_CHUNKS>,
-                                                         T_BIG_WG, T_BIG_DYN));
+                                                         T_BIGT_WG, T_BIG_DYN));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob[1], (const void *)k_t_big<ME_SOLVER_COLOR_CACHE>,
-                                                         T_BIG_WG, T_BIG_DYN));
+                                                         T_BIGT_WG, T_BIG_DYN));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob[2], (const void *)k_t_color<1024>, 1024, T_COL_DYN));
     int obig = ob[0] < ob[1] ? ob[0] : ob[1];
     if (ob[2] < obig) obig = ob[2];

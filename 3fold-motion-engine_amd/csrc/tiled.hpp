@@ -37,10 +37,10 @@ struct TiledBuffers {
     int *psorted2;               // big tiles: pairs counting-sorted by colour
     int *rec;                    // big tiles: 8 ints per colour-sorted pair (p, a, b, la, lb, st, cnt, flags)
     float *prep;                 // 20 floats per contact: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu
-    float *sprep;                // big tiles: a T_REC-float record per colour-sorted pair (read only in the
-                                 // sweeps: body header, then 4 x 20 floats of prep; T_REC in tiled_big.hpp)
+    float *sprep;                // big tiles: T_REC4 float4 fields per colour-sorted pair, field-major (read only
+                                 // in the sweeps: body header, then 4 x 5 fields of prep; tiled_big.hpp)
     float *simp;                 // big tiles: the impulses (jn, jt1, jt2, jp) of a colour-sorted pair's
-                                 // contacts, 16 floats per pair
+                                 // contacts, one float4 per contact slot, slot-major
     int *srec;                   // big tiles: 4 ints per colour-sorted pair (p, a, b, first contact)
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
     long long *bigw;             // tile work since the last reset, 4 counts (tiles, pairs, dynamic pair ends,
@@ -1294,7 +1294,7 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     // the big tiles (the critical path when there are any) are submitted first, so that their
     // workgroups find CUs with free LDS before the small tiles fill them
     if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
-    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIG_WG), dyn, s2, A, T);
+    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIGT_WG), dyn, s2, A, T);
     if (prof_rec(A, ME_K_TILE_BIG, s2) || prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
     if (A.taped)
         hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile < A.small_grid ? ntile : A.small_grid),

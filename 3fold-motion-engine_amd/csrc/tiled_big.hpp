@@ -2,9 +2,10 @@
 //
 // LDS mode (tile within T_BIG_B bodies and T_BIG_P pairs): the tile's pairs are counting-sorted by
 // colour; each sorted pair keeps a 32-bit word in LDS (body slots, contact count, flags), its contact
-// constants as a record in global memory in sorted order (T_REC), and its impulses in 64 bytes beside
-// (simp), so a colour step is one LDS read plus one round of 16-byte vector loads per thread. Body
-// velocities (velocity passes) and then pseudo-velocities (position passes) sit in LDS.
+// constants as a lane-coalesced record in global memory in sorted order (T_REC4 float4 fields), and
+// its impulses beside (simp), so a colour step is one LDS read plus one round of 16-byte vector loads
+// per thread. Body velocities (velocity passes) and then pseudo-velocities (position passes) sit in
+// LDS.
 // Larger tiles (giant islands): k_t_giant, the same sweeps by a grid of workgroups with body state in
 // global memory.
 #pragma once
@@ -15,29 +16,6 @@
 __device__ __forceinline__ unsigned pw_make(int la, int lb, int cnt, bool dA, bool dB, bool hB) {
     return (unsigned)(la & 8191) | ((unsigned)(lb & 8191) << 13) | ((unsigned)(cnt - 1) << 26) |
            ((unsigned)dA << 28) | ((unsigned)dB << 29) | ((unsigned)hB << 30);
-}
-
-__device__ __forceinline__ void ld_sprep(const float *o, const float *oi, CCon &C) {
-    float4 q0 = *reinterpret_cast<const float4 *>(o);
-    float4 q1 = *reinterpret_cast<const float4 *>(o + 4);
-    float4 q2 = *reinterpret_cast<const float4 *>(o + 8);
-    float4 q3 = *reinterpret_cast<const float4 *>(o + 12);
-    float4 q4 = *reinterpret_cast<const float4 *>(o + 16);
-    float4 q5 = *reinterpret_cast<const float4 *>(oi);
-    C.n = mk(q0.x, q0.y, q0.z); C.rA = mk(q0.w, q1.x, q1.y); C.rB = mk(q1.z, q1.w, q2.x);
-    C.t1 = mk(q2.y, q2.z, q2.w); C.t2 = mk(q3.x, q3.y, q3.z);
-    C.meff = q3.w; C.meft1 = q4.x; C.meft2 = q4.y; C.bias = q4.z; C.mu = q4.w;
-    C.jn = q5.x; C.jt1 = q5.y; C.jt2 = q5.z; C.jp = q5.w;
-}
-
-__device__ __forceinline__ m3 ld_m12(const float *o) {
-    float4 a = *reinterpret_cast<const float4 *>(o);
-    float4 b = *reinterpret_cast<const float4 *>(o + 4);
-    float4 c = *reinterpret_cast<const float4 *>(o + 8);
-    m3 M;
-    M.m[0][0] = a.x; M.m[0][1] = a.y; M.m[0][2] = a.z; M.m[1][0] = a.w; M.m[1][1] = b.x; M.m[1][2] = b.y;
-    M.m[2][0] = b.z; M.m[2][1] = b.w; M.m[2][2] = c.x;
-    return M;
 }
 
 __device__ __forceinline__ void ld_prep20(const float *o, CCon &C) {
@@ -60,15 +38,48 @@ __device__ __forceinline__ GRec grec_load(const int *r) {
     return GRec{x.y, x.z, y.y, y.z, y.w};
 }
 
-// A colour-sorted pair's record: a 40-float header -- both sides' inverse mass (and two pad floats),
-// inverse world inertia (12 floats each, row-major with 3 pad), and the velocity and angular velocity
-// of a static (not dynamic) side (12 floats: vA, wA, vB, wB) -- then per contact the 20 constants of
-// k_t_prep. The record is written once per substep and only read by the sweeps; the impulses, which
-// every colour step rewrites, sit apart in 64 bytes per pair (simp), so a step dirties half a cache
-// line per pair instead of one line per contact. Everything a colour step reads from global memory
+// A colour-sorted pair's record: a header of 10 float4 -- both sides' inverse mass (and two pad
+// floats), inverse world inertia (3 float4 each, row-major with 3 pad), and the velocity and angular
+// velocity of a static (not dynamic) side (3 float4: vA, wA, vB, wB) -- then per contact 5 float4 of
+// k_t_prep's constants, regrouped so that the warm and position passes read only the first three:
+// (n, meff), (rA, bias), (rB, mu), (t1, meft1), (t2, meft2). The records are lane-coalesced: float4
+// field j of sorted pair q sits at sprep[j * capP + q], so the 64 lanes of a wave, which hold
+// consecutive sorted pairs of a colour, read 1 KB of consecutive memory per field (with one record
+// after another, each field load touched 64 cache lines). The record is written once per substep and
+// only read by the sweeps; the impulses (jn, jt1, jt2, jp), which every colour step rewrites, sit apart
+// in simp, contact k of pair q at simp[k * capP + q]. Everything a colour step reads from global memory
 // sits at addresses known before the pair word is read, so the step issues all of its loads at once.
-#define T_REC_HDR 40
-#define T_REC (T_REC_HDR + 80)
+#define T_REC_HDR4 10
+#define T_REC4 (T_REC_HDR4 + 20)
+
+// contact k of sorted pair q from the lane-coalesced records (S = capP float4 per field); the
+// friction frame and its effective masses only for the velocity pass
+template <int PASS>
+__device__ __forceinline__ void ld_srec(const float4 *R, const float4 *I, size_t S, int q, int k, CCon &C) {
+    const float4 *c = R + (T_REC_HDR4 + 5 * k) * S + q;
+    const float4 a = c[0], b = c[S], d = c[2 * S];
+    C.n = mk(a.x, a.y, a.z); C.meff = a.w;
+    C.rA = mk(b.x, b.y, b.z); C.bias = b.w;
+    C.rB = mk(d.x, d.y, d.z); C.mu = d.w;
+    if (PASS == 1) {
+        const float4 e = c[3 * S], f = c[4 * S];
+        C.t1 = mk(e.x, e.y, e.z); C.meft1 = e.w;
+        C.t2 = mk(f.x, f.y, f.z); C.meft2 = f.w;
+    } else {
+        C.t1 = C.t2 = mk(0.0f, 0.0f, 0.0f);
+        C.meft1 = C.meft2 = 0.0f;
+    }
+    const float4 i = I[k * S + q];
+    C.jn = i.x; C.jt1 = i.y; C.jt2 = i.z; C.jp = i.w;
+}
+
+__device__ __forceinline__ m3 ld_m12s(const float4 *o, size_t S) {
+    const float4 a = o[0], b = o[S], c = o[2 * S];
+    m3 M;
+    M.m[0][0] = a.x; M.m[0][1] = a.y; M.m[0][2] = a.z; M.m[1][0] = a.w; M.m[1][1] = b.x; M.m[1][2] = b.y;
+    M.m[2][0] = b.z; M.m[2][1] = b.w; M.m[2][2] = c.x;
+    return M;
+}
 
 // One pair of a big tile for one colour step, contacts streamed one at a time (few live registers).
 // PASS 0 warm apply, 1 velocity, 2 position. LDS mode: body state s0/s1 in LDS by tile slot, the
@@ -91,18 +102,19 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
     float iMA, iMB = 0.0f;
     m3 IA, IB;
     f3 xA, yA, xB = z, yB = z;
-    float *rec = T.sprep + T_REC * (size_t)q;
+    const size_t S = (size_t)A.capP;
+    const float4 *R = reinterpret_cast<const float4 *>(T.sprep), *I = reinterpret_cast<const float4 *>(T.simp);
     if (LDSM) {
         // the record header: masses, inertias, a static side's velocities (its pseudo-velocities are
         // zero: they are cleared every substep and a static body takes no impulse)
-        const float4 h0 = *reinterpret_cast<const float4 *>(rec);
+        const float4 *H = R + q;
+        const float4 h0 = H[0];
         iMA = h0.x;
         iMB = h0.y;
-        IA = ld_m12(rec + 4);
-        IB = ld_m12(rec + 16);
-        const float4 h7 = *reinterpret_cast<const float4 *>(rec + 28);
-        const float4 h8 = *reinterpret_cast<const float4 *>(rec + 32);
-        const float4 h9 = *reinterpret_cast<const float4 *>(rec + 36);
+        IA = ld_m12s(H + S, S);
+        IB = ld_m12s(H + 4 * S, S);
+        float4 h7 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), h8 = h7, h9 = h7;
+        if (PASS != 2) { h7 = H[7 * S]; h8 = H[8 * S]; h9 = H[9 * S]; }
         if (dynA) {
             xA = ld3(s0, la); yA = ld3(s1, la);
         } else if (PASS == 2) {
@@ -128,11 +140,25 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
             yB = dynB ? ld3(s1, b) : ld3(gs1, b);
         }
     }
-    float *o = rec + T_REC_HDR;
-    // software pipeline: contact k+1 is loaded before contact k is solved and stored
+    auto solve = [&](int k, CCon &C) {
+        if (PASS == 0) {
+            cc_warm(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
+        } else {
+            if (PASS == 1) cc_vel<FIXED>(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB, t2d);
+            else cc_pos(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
+            if (LDSM) {
+                reinterpret_cast<float4 *>(T.simp)[k * S + q] = make_float4(C.jn, C.jt1, C.jt2, C.jp);
+            } else {
+                int c = st + k;
+                A.Cc.jn[c] = C.jn; A.Cc.jt1[c] = C.jt1; A.Cc.jt2[c] = C.jt2; A.Cc.jp[c] = C.jp;
+            }
+        }
+    };
+    // software pipeline: contact k+1 is loaded before contact k is solved and stored (loading every
+    // contact at once, in 512-thread workgroups for the registers, measured slower)
     auto load = [&](int k, CCon &C) {
         if (LDSM) {
-            ld_sprep(o + 20 * k, T.simp + 16 * (size_t)q + 4 * k, C);
+            ld_srec<PASS>(R, I, S, q, k, C);
         } else {
             int c = st + k;
             ld_prep20(T.prep + 20 * (size_t)c, C);
@@ -146,31 +172,23 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
         if (k >= cnt) break;
         CCon C = Cnext;
         if (k + 1 < cnt) load(k + 1, Cnext);
-        if (PASS == 0) {
-            cc_warm(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
-        } else {
-            if (PASS == 1) cc_vel<FIXED>(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB, t2d);
-            else cc_pos(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
-            if (LDSM) {
-                *reinterpret_cast<float4 *>(T.simp + 16 * (size_t)q + 4 * k) = make_float4(C.jn, C.jt1, C.jt2, C.jp);
-            } else {
-                int c = st + k;
-                A.Cc.jn[c] = C.jn; A.Cc.jt1[c] = C.jt1; A.Cc.jt2[c] = C.jt2; A.Cc.jp[c] = C.jp;
-            }
-        }
+        solve(k, C);
     }
     if (iMA > 0.0f) { st3(s0, la, xA); st3(s1, la, yA); }
     if (hasB && iMB > 0.0f) { st3(s0, lb, xB); st3(s1, lb, yB); }
 }
 
-#define T_BIG_WG 1024
+#define T_BIG_WG 1024   // threads of a giant-grid workgroup
+#define T_BIGT_WG 1024  // threads of a k_t_big workgroup
 #define T_BIG_DYN (6 * sizeof(float) * T_BIG_B + sizeof(unsigned) * T_BIG_P)
 
 // `reps` sweeps of one pass over an LDS-mode big tile, contacts streamed (big_pair). A colour step
-// of a big island is bound by VALU issue: every active wave runs the whole per-pair instruction chain,
-// and a colour of more than 256 pairs puts two waves on a SIMD. Prefetching the next pair across
-// colour steps (to keep loads clear of the step's impulse stores, which gfx9 counts in vmcnt) costs
-// more instructions than the latency it hides, since the second wave on each SIMD covers it.
+// of a big island is bound by the latency of one pair's dependent chain: a colour of up to 256 pairs
+// leaves one wave on each SIMD, with nothing to hide the chain behind. Measured with cycle stamps on
+// the settled 10 000-box pile (2.25 GHz): a velocity step takes about 10 000 cycles, of which about
+// 1 800 wait for the pair's first records, and a position step about 5 000 (900). Prefetching the next
+// pair across colour steps (to keep loads clear of the step's impulse stores, which gfx9 counts in
+// vmcnt) costs more instructions than the latency it hides.
 template <int PASS, bool FIXED>
 __device__ __forceinline__ void big_sweeps_stream(const TiledArgs &A, const TiledBuffers &T, const int *cst,
                                                   const unsigned *pword, int nc, int reps, int p0, int b0, float *s0,
@@ -178,7 +196,7 @@ __device__ __forceinline__ void big_sweeps_stream(const TiledArgs &A, const Tile
     const int tid = threadIdx.x;
     for (int it = 0; it < reps; ++it)
         for (int c = 0; c < nc; ++c) {
-            for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIG_WG)
+            for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIGT_WG)
                 big_pair<PASS, FIXED, true>(A, T, pword[k], GRec{}, p0 + k, b0, s0, s1, t2d);
             lds_barrier();
         }
@@ -214,7 +232,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     // body state (velocities, then pseudo-velocities) by tile slot; kept as LDS pointers so every
     // access compiles to ds_* instructions rather than generic (flat) ones
     float *lv = dsm, *lw = dsm + 3 * T_BIG_B;
-    for (int j = tid; j < nb; j += T_BIG_WG) {
+    for (int j = tid; j < nb; j += T_BIGT_WG) {
         int i = T.bsorted[b0 + j];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -222,10 +240,10 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
             lw[3 * j + k] = A.b.w[3 * i + k];
         }
     }
-    for (int c = tid; c <= ncol; c += T_BIG_WG) { cst[c] = 0; cfl[c] = 0; cfe[c] = 0; }
+    for (int c = tid; c <= ncol; c += T_BIGT_WG) { cst[c] = 0; cfl[c] = 0; cfe[c] = 0; }
     __syncthreads();
     stamp(1);
-    for (int k = tid; k < np; k += T_BIG_WG) {
+    for (int k = tid; k < np; k += T_BIGT_WG) {
         int c = A.Pp.col[T.psorted[p0 + k]];
         if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
         atomicAdd(&cst[c], 1);
@@ -241,7 +259,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     stamp(2);
     const float *invM = A.b.invM;
     int mdyn = 0, mcon = 0;
-    for (int k = tid; k < np; k += T_BIG_WG) {
+    for (int k = tid; k < np; k += T_BIGT_WG) {
         int p = T.psorted[p0 + k];
         int c = A.Pp.col[p];
         if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
@@ -258,7 +276,8 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         pword[pos] = pw_make(la, lb, cnt, dA, dB, b >= 0);
         int *sr = T.srec + 4 * (size_t)q;
         sr[0] = p; sr[1] = a; sr[2] = b; sr[3] = st;
-        float4 *hd = reinterpret_cast<float4 *>(T.sprep + T_REC * (size_t)q);
+        const size_t S = (size_t)A.capP;
+        float4 *hd = reinterpret_cast<float4 *>(T.sprep) + q;
         {
             float sA[12], sB[12];
             const float iA = dA ? T.invMs[b0 + la] : invM[a];
@@ -273,24 +292,28 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
             f3 vA = dA ? z3 : ld3(A.b.v, a), wA = dA ? z3 : ld3(A.b.w, a);
             f3 vB = (dB || b < 0) ? z3 : ld3(A.b.v, b), wB = (dB || b < 0) ? z3 : ld3(A.b.w, b);
             hd[0] = make_float4(iA, iB, 0.0f, 0.0f);
-            hd[1] = make_float4(sA[0], sA[1], sA[2], sA[3]);
-            hd[2] = make_float4(sA[4], sA[5], sA[6], sA[7]);
-            hd[3] = make_float4(sA[8], sA[9], sA[10], sA[11]);
-            hd[4] = make_float4(sB[0], sB[1], sB[2], sB[3]);
-            hd[5] = make_float4(sB[4], sB[5], sB[6], sB[7]);
-            hd[6] = make_float4(sB[8], sB[9], sB[10], sB[11]);
-            hd[7] = make_float4(vA.x, vA.y, vA.z, wA.x);
-            hd[8] = make_float4(wA.y, wA.z, vB.x, vB.y);
-            hd[9] = make_float4(vB.z, wB.x, wB.y, wB.z);
+            hd[S] = make_float4(sA[0], sA[1], sA[2], sA[3]);
+            hd[2 * S] = make_float4(sA[4], sA[5], sA[6], sA[7]);
+            hd[3 * S] = make_float4(sA[8], sA[9], sA[10], sA[11]);
+            hd[4 * S] = make_float4(sB[0], sB[1], sB[2], sB[3]);
+            hd[5 * S] = make_float4(sB[4], sB[5], sB[6], sB[7]);
+            hd[6 * S] = make_float4(sB[8], sB[9], sB[10], sB[11]);
+            hd[7 * S] = make_float4(vA.x, vA.y, vA.z, wA.x);
+            hd[8 * S] = make_float4(wA.y, wA.z, vB.x, vB.y);
+            hd[9 * S] = make_float4(vB.z, wB.x, wB.y, wB.z);
         }
-        float4 *o = reinterpret_cast<float4 *>(T.sprep + T_REC * (size_t)q + T_REC_HDR);
+        float4 *imp = reinterpret_cast<float4 *>(T.simp) + q;
         for (int i = 0; i < cnt; ++i) {
+            // prep: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu (k_t_prep)
             const float4 *s4 = reinterpret_cast<const float4 *>(T.prep + 20 * (size_t)(st + i));
-            float4 v0 = s4[0], v1 = s4[1], v2 = s4[2], v3 = s4[3], v4 = s4[4];
-            float4 *d = o + 5 * i;
-            d[0] = v0; d[1] = v1; d[2] = v2; d[3] = v3; d[4] = v4;
-            reinterpret_cast<float4 *>(T.simp + 16 * (size_t)q)[i] =
-                make_float4(A.Cc.jn[st + i], A.Cc.jt1[st + i], A.Cc.jt2[st + i], A.Cc.jp[st + i]);
+            const float4 v0 = s4[0], v1 = s4[1], v2 = s4[2], v3 = s4[3], v4 = s4[4];
+            float4 *d = hd + (T_REC_HDR4 + 5 * i) * S;
+            d[0] = make_float4(v0.x, v0.y, v0.z, v3.w);      // n, meff
+            d[S] = make_float4(v0.w, v1.x, v1.y, v4.z);      // rA, bias
+            d[2 * S] = make_float4(v1.z, v1.w, v2.x, v4.w);  // rB, mu
+            d[3 * S] = make_float4(v2.y, v2.z, v2.w, v4.x);  // t1, meft1
+            d[4 * S] = make_float4(v3.x, v3.y, v3.z, v4.y);  // t2, meft2
+            imp[i * S] = make_float4(A.Cc.jn[st + i], A.Cc.jt1[st + i], A.Cc.jt2[st + i], A.Cc.jp[st + i]);
         }
     }
     if (mdyn) atomicAdd(&wdyn, mdyn);
@@ -312,13 +335,13 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     __syncthreads();
     stamp(4);
     // velocities back to global memory; the LDS region now holds the pseudo-velocities
-    for (int j = tid; j < nb; j += T_BIG_WG) {
+    for (int j = tid; j < nb; j += T_BIGT_WG) {
         int i = T.bsorted[b0 + j];
         st3(A.b.v, i, ld3(lv, j));
         st3(A.b.w, i, ld3(lw, j));
     }
     __syncthreads();
-    for (int j = tid; j < nb; j += T_BIG_WG) {
+    for (int j = tid; j < nb; j += T_BIGT_WG) {
         st3(lv, j, mk(0.0f, 0.0f, 0.0f));
         st3(lw, j, mk(0.0f, 0.0f, 0.0f));
     }
@@ -326,19 +349,20 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     if (np > 0) big_sweeps_stream<2, FIXED>(A, T, cst, pword, nc, A.pit, p0, b0, lv, lw, t2d);
     __syncthreads();
     stamp(5);
-    for (int k = tid; k < np; k += T_BIG_WG) {  // impulses back to the contact arrays
+    for (int k = tid; k < np; k += T_BIGT_WG) {  // impulses back to the contact arrays
         int q = p0 + k;
         int st = T.srec[4 * (size_t)q + 3];
         int cnt = (int)((pword[k] >> 26) & 3) + 1;
-        const float *o = T.simp + 16 * (size_t)q;
+        const float4 *o = reinterpret_cast<const float4 *>(T.simp) + q;
         for (int i = 0; i < cnt; ++i) {
-            A.Cc.jn[st + i] = o[4 * i];
-            A.Cc.jt1[st + i] = o[4 * i + 1];
-            A.Cc.jt2[st + i] = o[4 * i + 2];
-            A.Cc.jp[st + i] = o[4 * i + 3];
+            const float4 j = o[i * (size_t)A.capP];
+            A.Cc.jn[st + i] = j.x;
+            A.Cc.jt1[st + i] = j.y;
+            A.Cc.jt2[st + i] = j.z;
+            A.Cc.jp[st + i] = j.w;
         }
     }
-    for (int j = tid; j < nb; j += T_BIG_WG) {
+    for (int j = tid; j < nb; j += T_BIGT_WG) {
         int i = T.bsorted[b0 + j];
         f3 pv = ld3(lv, j), po = ld3(lw, j);
         st3(A.pv, i, pv);
@@ -356,7 +380,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
 #define T_BIG_GRID_MAX 256
 
 template <int SOLVER>
-__global__ void __launch_bounds__(T_BIG_WG) k_t_big(TiledArgs A, TiledBuffers T) {
+__global__ void __launch_bounds__(T_BIGT_WG) k_t_big(TiledArgs A, TiledBuffers T) {
     extern __shared__ float dsm[];
     __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
     __shared__ int maxc, wdyn, wcon, qi;
