@@ -46,6 +46,11 @@ PGS_LAM, PGS_V, PGS_VBIAS, PGS_RHO, PGS_RES, PGS_U, PGS_COLOR = range(7)
 PGS_INFO = ("n_colors", "cache_hits", "cache_misses", "n_islands", "n_tiles", "overflow")
 
 
+class NcpSvdOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("X", "S", "V", "U", "rank", "compat", "finite", "resid", "Y", "valid",
+                                           "invariant", "leak", "labels", "status")]
+
+
 class MotionError(RuntimeError):
     pass
 
@@ -77,6 +82,9 @@ def lib():
             [vp] * 5 + [C.c_size_t, vp]
         L.me_ncp_sensitivity_noalloc.argtypes = [C.c_int32, C.c_int32, C.c_int32] + [vp] * 6 + [C.c_double] * 3 + \
             [vp] * 4 + [vp]
+        L.me_ncp_svd_workspace_size.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_size_t)]
+        L.me_ncp_svd.argtypes = [C.c_int32] * 4 + [vp] * 7 + [C.c_int32] + [C.c_double] * 4 + \
+            [C.c_int32, C.POINTER(NcpSvdOut), vp, C.c_size_t, vp]
         L.me_pgs_workspace_size.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_size_t)]
         L.me_pgs_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_size_t, C.POINTER(vp)]
         L.me_pgs_destroy.argtypes = [vp]
@@ -110,6 +118,7 @@ EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me
             "me_tape_size", "me_step_taped", "me_adjoint_substep", "me_adopt", "me_state_size",
             "me_ncp_workspace_size", "me_ncp_sensitivity", "me_pgs_workspace_size", "me_pgs_create",
             "me_pgs_destroy", "me_pgs_prepare", "me_pgs_iterate", "me_pgs_snapshot", "me_pgs_residual",
-            "me_pgs_buffer", "me_pgs_info", "me_ncp_sensitivity_noalloc")
+            "me_pgs_buffer", "me_pgs_info", "me_ncp_sensitivity_noalloc", "me_ncp_svd_workspace_size",
+            "me_ncp_svd")
 K_SWEEP_VEL, K_SWEEP_POS, K_NARROW, K_TILED, K_JAC_VEL, K_TILE_BIG, K_TILE_SMALL, K_TILE_GIANT, K_ISLANDS, \
     K_COLOR = range(10)

@@ -15,6 +15,7 @@
 #include "tiled.hpp"
 #include "beads.hpp"
 #include "ncp.hpp"
+#include "ncp_svd.hpp"
 #include "pgs.hpp"
 #include "adjoint_kernels.hpp"
 #include "giant_grid.hpp"
@@ -1386,6 +1387,52 @@ int me_ncp_sensitivity_noalloc(int32_t n_env, int32_t n_c, int32_t n_dir, const 
     }
     return ncp_launch(n_env, n_c, n_dir, G, b, mu, lam, dirs, labels_in, tol_lam, tol_rel, rcond, dlam_dmu, dlam_db,
                       labels, status, (double *)ptr, stream);
+}
+
+int me_ncp_svd_workspace_size(int32_t n_env, int32_t n_c, int32_t n_dir, size_t *bytes) {
+    if (!bytes) return fail(ME_EARG, "bytes is NULL");
+    if (n_env < 1 || n_c < 1 || n_c > NCPS_MAXC || n_dir < 0)
+        return fail(ME_EARG, "need n_env >= 1, 1 <= n_c <= 256, n_dir >= 0");
+    *bytes = ncp_svd_scratch(n_c, n_dir) * sizeof(double) * (size_t)n_env;
+    return ME_OK;
+}
+
+int me_ncp_svd(int32_t n_env, int32_t n_c, int32_t n_dir, int32_t n_obs, const double *G, const double *b,
+               const double *mu, const double *lam, const double *dirs, const int32_t *labels_in, const double *Q,
+               int32_t q_shared, double tol_lam, double tol_rel, double rcond, double tol, int32_t minimum_norm,
+               const me_ncp_svd_out *out, void *workspace, size_t bytes, void *stream) {
+    if (n_env < 1 || n_c < 1 || n_c > NCPS_MAXC || n_dir < 0 || n_obs < 0)
+        return fail(ME_EARG, "need n_env >= 1, 1 <= n_c <= 256, n_dir >= 0, n_obs >= 0");
+    if (!dirs && n_dir != 3 * n_c && n_dir != 0)
+        return fail(ME_EARG, "without directions n_dir must be 3 * n_c (the identity) or 0 (no b columns)");
+    if (!G || !b || !mu || !lam || !out || (n_obs > 0 && !Q)) return fail(ME_EARG, "NULL argument");
+    const me_ncp_svd_out &o = *out;
+    if (!o.X || !o.S || !o.V || !o.U || !o.rank || !o.compat || !o.finite || !o.resid || !o.labels || !o.status ||
+        (n_obs > 0 && (!o.Y || !o.valid || !o.invariant || !o.leak)))
+        return fail(ME_EARG, "NULL output");
+    if (!(tol_lam >= 0.0) || !(tol_rel >= 0.0) || !(rcond > 0.0 && rcond < 1.0) || !(tol > 0.0 && tol < 1.0))
+        return fail(ME_EARG, "need tol_lam >= 0, tol_rel >= 0, 0 < rcond < 1, 0 < tol < 1");
+    size_t need = 0;
+    (void)me_ncp_svd_workspace_size(n_env, n_c, n_dir, &need);
+    if (!workspace || bytes < need) return fail(ME_EARG, "workspace too small: need " + std::to_string(need) + " bytes");
+    NcpSvdOut k{o.X, o.S, o.V, o.U, o.rank, o.compat, o.finite, o.resid, o.Y, o.valid, o.invariant, o.leak,
+                o.labels, o.status};
+    // no b columns: a non-null direction pointer of zero columns (never read) keeps the identity out
+    const double *D = (!dirs && n_dir == 0) ? b : dirs;
+    hipStream_t s = (hipStream_t)stream;
+    const int n3 = 3 * n_c;
+    double *ws = (double *)workspace;
+    if (n3 <= 24)
+        hipLaunchKernelGGL(k_ncp_svd<64>, dim3(n_env), dim3(64), 0, s, n_env, n_c, n_dir, n_obs, G, b, mu, lam, D,
+                           labels_in, Q, q_shared, tol_lam, tol_rel, rcond, tol, minimum_norm, k, ws);
+    else if (n3 <= 192)
+        hipLaunchKernelGGL(k_ncp_svd<256>, dim3(n_env), dim3(256), 0, s, n_env, n_c, n_dir, n_obs, G, b, mu, lam, D,
+                           labels_in, Q, q_shared, tol_lam, tol_rel, rcond, tol, minimum_norm, k, ws);
+    else
+        hipLaunchKernelGGL(k_ncp_svd<1024>, dim3(n_env), dim3(1024), 0, s, n_env, n_c, n_dir, n_obs, G, b, mu, lam, D,
+                           labels_in, Q, q_shared, tol_lam, tol_rel, rcond, tol, minimum_norm, k, ws);
+    LAUNCH_CHECK();
+    return ME_OK;
 }
 
 int me_state_size(me_engine *e, size_t *bytes) {

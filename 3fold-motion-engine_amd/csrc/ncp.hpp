@@ -73,27 +73,34 @@ __device__ bool ncp_lu(double *M, int n3, int W, double thresh) {
     return true;
 }
 
-__global__ void __launch_bounds__(NCP_THREADS)
-k_ncp_sensitivity(int E, int n, int nd, const double *G, const double *b, const double *mu, const double *lam,
-                  const double *dirs, const int *lab_in, double tol_lam, double tol_rel, double rcond,
-                  double *dmu, double *dlb, int *lab_out, int *status, double *gscratch, int use_lds) {
-    extern __shared__ double sm[];
-    int e = blockIdx.x, t = threadIdx.x;
-    if (e >= E) return;
-    const int n3 = 3 * n, W = n3 + n + nd;
-    const double *Ge = G + (size_t)e * n3 * n3;
-    const double *be = b + (size_t)e * n3;
-    const double *me = mu + (size_t)e * n;
-    const double *le = lam + (size_t)e * n3;
-    const double *De = dirs ? dirs + (size_t)e * n3 * nd : nullptr;
-    __shared__ double u[3 * NCP_MAXC];
-    __shared__ int lab[NCP_MAXC];
-    __shared__ int st;
-    double *M = use_lds ? sm : gscratch + (size_t)e * n3 * W;
-    if (t == 0) st = 0;
+// maximum of v over the NT threads of the workgroup (every thread gets it); red: NT/64 doubles of LDS
+template <int NT>
+__device__ __forceinline__ double ncp_block_max(double v, double *red) {
+    v = ncp_wave_max(v);
+    if (NT == 64) return v;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int w = 1; w < NT / 64; ++w) m = fmax(m, red[w]);
+    __syncthreads();
+    return m;
+}
+
+// Set-up of one environment by its NT threads: u = G lam + b, finiteness of every input, labels
+// (classify, ncp_ref.py:251-266) with their boundary reasons (ncp_ref.py:838-842), and the row-major
+// augmented matrix M = [A | B_mu | B_b D] (n3 rows, leading dimension W = n3 + n + nd; D = identity
+// when dirs is null) of the fixed-branch residual (ncp_ref.py:786-836). u (n3), lab (n), st and red
+// (NT/64) are LDS; st collects NCP_ST_NONFINITE and NCP_ST_BOUNDARY.
+template <int NT>
+__device__ void ncp_setup(int n, int nd, const double *Ge, const double *be, const double *me, const double *le,
+                          const double *De, const int *lab_in, double tol_lam, double tol_rel, double *M, int W,
+                          double *u, int *lab, int *st, double *red) {
+    const int t = threadIdx.x, n3 = 3 * n;
+    if (t == 0) *st = 0;
     // u = G lam + b; finiteness of every input
     bool bad = false;
-    for (int r = t; r < n3; r += NCP_THREADS) {
+    for (int r = t; r < n3; r += NT) {
         double s = 0.0;
         for (int j = 0; j < n3; ++j) {
             double g = Ge[(size_t)r * n3 + j];
@@ -103,20 +110,20 @@ k_ncp_sensitivity(int E, int n, int nd, const double *G, const double *b, const 
         u[r] = s + be[r];
         bad = bad || !isfinite(be[r]) || !isfinite(le[r]);
     }
-    for (int c = t; c < n; c += NCP_THREADS) bad = bad || !isfinite(me[c]);
+    for (int c = t; c < n; c += NT) bad = bad || !isfinite(me[c]);
     if (De)
-        for (int idx = t; idx < n3 * nd; idx += NCP_THREADS) bad = bad || !isfinite(De[idx]);
+        for (int idx = t; idx < n3 * nd; idx += NT) bad = bad || !isfinite(De[idx]);
     double sc = 0.0;
-    for (int r = t; r < n3; r += NCP_THREADS) sc = fmax(sc, fabs(le[r]));
-    sc = fmax(ncp_wave_max(sc), 1.0e-30);
-    if (__any(bad) && t == 0) st = NCP_ST_NONFINITE;
+    for (int r = t; r < n3; r += NT) sc = fmax(sc, fabs(le[r]));
+    sc = fmax(ncp_block_max<NT>(sc, red), 1.0e-30);
+    if (bad) atomicOr(st, NCP_ST_NONFINITE);
     __syncthreads();
     // labels (classify, ncp_ref.py:251-266) and boundary reasons (ncp_ref.py:838-842)
-    for (int c = t; c < n; c += NCP_THREADS) {
+    for (int c = t; c < n; c += NT) {
         double ln = le[3 * c], lt = sqrt(le[3 * c + 1] * le[3 * c + 1] + le[3 * c + 2] * le[3 * c + 2]);
         int L;
         if (lab_in) {
-            L = lab_in[(size_t)e * n + c] & 3;
+            L = lab_in[c] & 3;
         } else if (ln <= tol_lam + tol_rel * sc) {
             L = NCP_OPEN;
         } else if (lt >= me[c] * ln * (1.0 - tol_rel) - tol_lam) {
@@ -132,11 +139,11 @@ k_ncp_sensitivity(int E, int n, int nd, const double *G, const double *b, const 
         if (L == NCP_STICK && fabs(lt - me[c] * ln) < 1e-7 * me[c] * scl) flags |= NCP_B_STICK_ON_CONE;
         if (L != NCP_OPEN && ln < 1e-9) flags |= NCP_B_NEAR_ZERO_N;
         lab[c] = L | flags;
-        if (flags) atomicOr(&st, NCP_ST_BOUNDARY);
+        if (flags) atomicOr(st, NCP_ST_BOUNDARY);
     }
     __syncthreads();
     // assemble [A | B_mu | B_b D] row by row
-    for (int r = t; r < n3; r += NCP_THREADS) {
+    for (int r = t; r < n3; r += NT) {
         double *row = M + (size_t)r * W;
         for (int j = 0; j < W; ++j) row[j] = 0.0;
         int c = r / 3, k = r % 3, i = 3 * c, L = lab[c] & 3;
@@ -173,6 +180,24 @@ k_ncp_sensitivity(int E, int n, int nd, const double *G, const double *b, const 
         }
     }
     __syncthreads();
+}
+
+__global__ void __launch_bounds__(NCP_THREADS)
+k_ncp_sensitivity(int E, int n, int nd, const double *G, const double *b, const double *mu, const double *lam,
+                  const double *dirs, const int *lab_in, double tol_lam, double tol_rel, double rcond,
+                  double *dmu, double *dlb, int *lab_out, int *status, double *gscratch, int use_lds) {
+    extern __shared__ double sm[];
+    int e = blockIdx.x, t = threadIdx.x;
+    if (e >= E) return;
+    const int n3 = 3 * n, W = n3 + n + nd;
+    __shared__ double u[3 * NCP_MAXC];
+    __shared__ int lab[NCP_MAXC];
+    __shared__ int st;
+    __shared__ double red[1];
+    double *M = use_lds ? sm : gscratch + (size_t)e * n3 * W;
+    ncp_setup<NCP_THREADS>(n, nd, G + (size_t)e * n3 * n3, b + (size_t)e * n3, mu + (size_t)e * n,
+                           lam + (size_t)e * n3, dirs ? dirs + (size_t)e * n3 * nd : nullptr,
+                           lab_in ? lab_in + (size_t)e * n : nullptr, tol_lam, tol_rel, M, W, u, lab, &st, red);
     // singularity threshold from max|A|
     double amax = 0.0;
     for (int idx = t; idx < n3 * n3; idx += NCP_THREADS) amax = fmax(amax, fabs(M[(size_t)(idx / n3) * W + idx % n3]));

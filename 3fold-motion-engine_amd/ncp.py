@@ -5,11 +5,16 @@ batched GPU form ``BatchProxADMMGPU.build_active`` + ``grad_affine`` (ncp/ncp_ba
 At a solved impulse lam of the contact problem u = G lam + b, lam in the Coulomb cones, every
 contact is labelled open / stick / slip and the fixed-branch residual
     open: lam_c = 0;  stick: u_c = 0;  slip: u_n = 0, lam_t + mu lam_n u_t / |u_t| = 0
-is differentiated by the implicit function theorem: A dlam = -(B_mu dmu + B_b db). The library
-solves it per environment with LU and partial pivoting in LDS (``me_ncp_sensitivity``), for many
-environments in one launch. Singular systems (redundant contacts) and branch-boundary cases give NaN
-derivatives, as the reference does without a minimum-norm selection; that selection and observable
-rows are not built.
+is differentiated by the implicit function theorem: A dlam = -(B_mu dmu + B_b db).
+
+* ``batch_sensitivity`` solves it for many environments in one launch, per environment by LU with
+  partial pivoting in LDS (``me_ncp_sensitivity``); singular systems (redundant contacts) and
+  branch-boundary cases give NaN derivatives.
+* ``sensitivity`` and ``contact_diagnostics`` (ncp_ref.py:845-866, 907-959) take the reference's SVD
+  rank guard (contact_rank.py:13-70) on the GPU (``me_ncp_svd``: one-sided Jacobi SVD of A and of
+  A^T): rank at rcond, right and left nullspaces, compatibility of each parameter column, the
+  minimum-norm linear response on request (``selection='minimum_norm'``) and observable rows Q whose
+  responses Q dlam are reported only where Q N = 0; with the same info dictionary as the reference.
 
 ``contact_problem(engine)`` builds (G, b, mu, lam) of the last substep of a ColorCache engine, so the
 sensitivity can be taken at the contact step's own solution.
@@ -96,34 +101,171 @@ def classify(lam, G, b, mu, tol_lam=1e-10, tol_rel=1e-6, device="cuda:0"):
     return np.array([LABELS[int(k)] for k in out["labels"][0].cpu().numpy()])
 
 
-def sensitivity(lam, G, b, mu, labels, rcond=1e-12, *, selection=None, observable=None, db_dparam=None,
-                device="cuda:0"):
-    """(dlam/dmu, dlam/db, info) of one problem, as ncp_ref.sensitivity without a minimum-norm
-    selection; db_dparam maps parameters to b (the result is dlam/db @ db_dparam)."""
-    if selection is not None:
-        raise ValueError("selection must be None: the minimum-norm response is not built on the GPU path")
+def svd_response(lam, G, b, mu, directions=None, labels=None, observable=None, minimum_norm=False, rcond=1e-12,
+                 tol=1e-10, tol_lam=1e-10, tol_rel=1e-6, device="cuda:0"):
+    """The SVD-guarded response of E problems of n contacts (``me_ncp_svd``). G (E,3n,3n), b (E,3n),
+    mu (E,n), lam (E,3n); directions (3n,m) or (E,3n,m), None for the identity, or a (3n,0) array for
+    no b columns; labels (E,n) of 0/1/2 or label strings, None to classify; observable (K,3n) shared
+    or (E,K,3n). Returns a dict of numpy arrays: X (E,3n,n+m), S, V, U, rank, compat, finite, resid,
+    Y, valid, invariant, leak, labels (label bits), boundary (reason bits), status."""
+    torch = _torch()
+    Gt = torch.as_tensor(G) if not isinstance(G, torch.Tensor) else G
+    E = int(Gt.shape[0]) if Gt.dim() == 3 else 1
+    n3 = int(Gt.shape[-1])
+    if n3 % 3:
+        raise ValueError("G must be (E, 3n, 3n)")
+    n = n3 // 3
+    dev = torch.device(device)
+    Gd, bd, md, ld = _dev(G, dev, (E, n3, n3)), _dev(b, dev, (E, n3)), _dev(mu, dev, (E, n)), _dev(lam, dev, (E, n3))
+    if directions is None:
+        m, Dd = n3, None
+    elif np.shape(directions)[-1] == 0:
+        m, Dd = 0, None
+    else:
+        D = _dev(directions, dev, (-1, n3, np.shape(directions)[-1]))
+        m = int(D.shape[-1])
+        Dd = (D.expand(E, n3, m) if D.shape[0] == 1 else D).contiguous()
+        if Dd.shape[0] != E:
+            raise ValueError("directions must be (3n, m) or (E, 3n, m)")
+    labs_in = None
+    if labels is not None:
+        la = np.asarray(labels)
+        if la.dtype.kind in "US":
+            la = np.vectorize(LABELS.index)(la)
+        labs_in = torch.as_tensor(la.astype(np.int32)).to(dev).reshape(E, n).contiguous()
+    K, Qd, q_shared = 0, None, 0
     if observable is not None:
-        raise ValueError("observable rows are not built on the GPU path")
+        Qa = np.asarray(observable, np.float64)
+        if Qa.ndim == 1:
+            Qa = Qa[None, :]
+        if Qa.shape[-1] != n3:
+            raise ValueError("observable must have 3 n_c columns")
+        q_shared = int(Qa.ndim == 2)
+        K = int(Qa.shape[-2])
+        Qd = _dev(Qa, dev, (-1, K, n3))
+    R = n + m
+    f64, i32 = dict(dtype=torch.float64, device=dev), dict(dtype=torch.int32, device=dev)
+    out = dict(X=torch.empty((E, n3, R), **f64), S=torch.empty((E, n3), **f64), V=torch.empty((E, n3, n3), **f64),
+               U=torch.empty((E, n3, n3), **f64), rank=torch.empty(E, **i32), compat=torch.empty((E, R), **i32),
+               finite=torch.empty((E, R), **i32), resid=torch.empty((E, R), **f64),
+               Y=torch.empty((E, max(K, 1), R), **f64), valid=torch.empty((E, max(K, 1), R), **i32),
+               invariant=torch.empty((E, max(K, 1)), **i32), leak=torch.empty((E, max(K, 1)), **f64),
+               labels=torch.empty((E, n), **i32), status=torch.empty(E, **i32))
+    p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    o = _lib.NcpSvdOut(*[p(out[k]) for k in ("X", "S", "V", "U", "rank", "compat", "finite", "resid", "Y", "valid",
+                                             "invariant", "leak", "labels", "status")])
+    L = _lib.lib()
+    nb = C.c_size_t()
+    _lib.check(L.me_ncp_svd_workspace_size(E, n, m, C.byref(nb)), "me_ncp_svd_workspace_size")
+    ws = torch.empty(max(int(nb.value), 8), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    _lib.check(L.me_ncp_svd(E, n, m, K, p(Gd), p(bd), p(md), p(ld), p(Dd), p(labs_in), p(Qd), q_shared,
+                            float(tol_lam), float(tol_rel), float(rcond), float(tol), int(bool(minimum_norm)),
+                            C.byref(o), p(ws), int(nb.value), C.c_void_p(stream)), "me_ncp_svd")
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    res["boundary"] = res["labels"] & ~3
+    res["labels"] = res["labels"] & 3
+    if K == 0:
+        for k in ("Y", "valid", "invariant", "leak"):
+            res[k] = res[k][:, :0]
+    return res
+
+
+def _check_inputs(lam, G, b, mu, labels):
     mu = np.asarray(mu, np.float64)
-    for name, a in (("lam", lam), ("G", G), ("b", b), ("mu", mu)):
-        if not np.all(np.isfinite(np.asarray(a, np.float64))):
-            raise ValueError("contact inputs must be finite")
     n = len(mu)
+    if np.shape(G) != (3 * n, 3 * n) or np.shape(b) != (3 * n,) or np.shape(lam) != (3 * n,) or len(labels) != n:
+        raise ValueError("contact arrays and labels have incompatible shapes")
     if any(lab not in LABELS for lab in labels):
         raise ValueError("labels must be open, stick or slip")
-    if len(labels) != n or np.shape(G) != (3 * n, 3 * n):
-        raise ValueError("contact arrays and labels have incompatible shapes")
-    out = batch_sensitivity(lam, G, b, mu, directions=db_dparam, labels=list(labels), rcond=rcond, device=device)
-    st = int(out["status"][0])
-    bnd = out["boundary"][0].cpu().numpy()
-    boundary = [(c, why) for c in range(n) for bit, why in _BOUNDARY if bnd[c] & bit]
-    singular = bool(st & SINGULAR)
-    info = {"status": "REDUNDANT_CONTACTS" if singular else "UNIQUE_LINEAR_SYSTEM", "singular": singular,
-            "boundary": boundary, "labels": list(labels), "residual_differentiable": not boundary,
-            "response_scope": "NONSMOOTH_BRANCH" if boundary else
-            ("LINEARIZED_RESPONSE_ONLY" if singular else "IMPLICIT_RESPONSE_UNDER_VALID_BRANCH_ASSUMPTIONS"),
-            "rcond": rcond, "selection": None}
-    return out["dlam_dmu"][0].cpu().numpy(), out["dlam_db"][0].cpu().numpy(), info
+    for a in (lam, G, b, mu):
+        if not np.all(np.isfinite(np.asarray(a, np.float64))):
+            raise ValueError("contact inputs must be finite")
+    return mu, n
+
+
+def _rank_info(r, n3, rcond, labels):
+    """rank_factor's info (contact_rank.py:13-30) and the branch fields of ncp_ref.py:845-866."""
+    rank = int(r["rank"][0])
+    s = r["S"][0]
+    V, U = r["V"][0], r["U"][0]
+    cond = float(s[0] / s[-1]) if s.size and s[-1] > 0 else (float("inf") if s.size else 0.0)
+    bnd = r["boundary"][0]
+    boundary = [(c, why) for c in range(len(labels)) for bit, why in _BOUNDARY if bnd[c] & bit]
+    singular = rank < n3
+    info = {"status": "REDUNDANT_CONTACTS" if singular else "UNIQUE_LINEAR_SYSTEM", "rank": rank,
+            "nullity": n3 - rank, "nullspace": V[:, rank:].copy(), "left_nullspace": U[:, rank:].copy(),
+            "singular_values": s.copy(), "rank_cutoff": rcond * s[0] if s.size else 0.0, "rcond": rcond,
+            "cond": cond, "singular": singular, "scope": "NUMERICAL_FIXED_BRANCH_AT_SUPPLIED_IMPULSE",
+            "selection": None}
+    info.update(boundary=boundary, labels=list(labels), residual_differentiable=not boundary,
+                response_scope="NONSMOOTH_BRANCH" if boundary else
+                ("LINEARIZED_RESPONSE_ONLY" if singular else "IMPLICIT_RESPONSE_UNDER_VALID_BRANCH_ASSUMPTIONS"))
+    return info, boundary
+
+
+def sensitivity(lam, G, b, mu, labels, rcond=1e-12, *, selection=None, observable=None, db_dparam=None,
+                device="cuda:0"):
+    """(dlam/dmu, dlam/db, info) of one problem, as ncp_ref.sensitivity (ncp_ref.py:907-959): the
+    SVD rank guard on the GPU, NaN for ambiguous derivatives unless selection='minimum_norm', NaN
+    for incompatible columns and on a branch boundary in either mode; observable=Q (K, 3n) reports
+    the responses of Q lam for rows with Q N = 0 (info['observable_dmu'], ['observable_db'], ...);
+    db_dparam maps parameters to b (the b columns are dlam/db @ db_dparam)."""
+    if selection not in (None, "minimum_norm"):
+        raise ValueError("selection must be None or 'minimum_norm'")
+    if not 0.0 < rcond < 1.0:
+        raise ValueError("rcond must be between zero and one")
+    mu, n = _check_inputs(lam, G, b, mu, labels)
+    n3 = 3 * n
+    if db_dparam is not None:
+        P = np.asarray(db_dparam, dtype=np.float64)
+        if P.ndim != 2 or P.shape[0] != n3 or not np.isfinite(P).all():
+            raise ValueError("db_dparam must be a finite matrix with 3*n_c rows")
+    else:
+        P = None
+    if observable is not None:
+        Q = np.atleast_2d(np.asarray(observable, dtype=np.float64))
+        if Q.shape[1] != n3 or not np.isfinite(Q).all():
+            raise ValueError("observable must be finite with A.shape[1] columns")
+    else:
+        Q = None
+    r = svd_response(lam, G, b, mu, directions=P, labels=list(labels), observable=Q,
+                     minimum_norm=selection == "minimum_norm", rcond=rcond, device=device)
+    info, boundary = _rank_info(r, n3, rcond, labels)
+    X = r["X"][0]
+    dm, db = X[:, :n].copy(), X[:, n:].copy()
+    comp, fin, res = r["compat"][0].astype(bool), r["finite"][0].astype(bool), r["resid"][0]
+    info.update(selection="MINIMUM_NORM_LINEAR_RESPONSE" if selection else None,
+                mu_compatible=comp[:n], b_compatible=comp[n:], mu_response_finite=fin[:n],
+                b_response_finite=fin[n:], mu_compatibility_residual=res[:n], b_compatibility_residual=res[n:])
+    if Q is not None:
+        Y, valid = r["Y"][0], r["valid"][0].astype(bool)
+        info.update(observable_dmu=Y[:, :n].copy(), observable_db=Y[:, n:].copy(),
+                    observable_mu_valid=valid[:, :n].copy(), observable_b_valid=valid[:, n:].copy(),
+                    observable_invariant=r["invariant"][0].astype(bool), observable_nullspace_residual=r["leak"][0])
+    if boundary:
+        dm[:] = np.nan
+        db[:] = np.nan
+    return dm, db, info
+
+
+def contact_diagnostics(lam, G, b, mu, labels, rcond=1e-12, observable=None, device="cuda:0"):
+    """Numerical local rank, nullspaces and optional invariant observable rows at a fixed branch
+    (ncp_ref.contact_diagnostics, ncp_ref.py:845-866), on the GPU."""
+    if not 0.0 < rcond < 1.0:
+        raise ValueError("rcond must be between zero and one")
+    mu, n = _check_inputs(lam, G, b, mu, labels)
+    Q = None
+    if observable is not None:
+        Q = np.atleast_2d(np.asarray(observable, dtype=np.float64))
+        if Q.shape[1] != 3 * n or not np.isfinite(Q).all():
+            raise ValueError("observable must be finite with A.shape[1] columns")
+    r = svd_response(lam, G, b, mu, directions=np.zeros((3 * n, 0)), labels=list(labels), observable=Q,
+                     rcond=rcond, device=device)
+    info, _ = _rank_info(r, 3 * n, rcond, labels)
+    if Q is not None:
+        info.update(observable_invariant=r["invariant"][0].astype(bool), observable_nullspace_residual=r["leak"][0])
+    return info
 
 
 def synthetic_problems(E, n_c=4, nv=18, seed=0, p_open=0.1, p_slip=0.4, mu_range=(0.3, 0.9)):

@@ -197,6 +197,36 @@ class SolveResult:
     force_selection: str = "COLORED_PGS_ORDER_WARM_START"
     contact_model: str = "NCP_RIGID"
     bias_applied: bool = True
+    contact_info: dict = field(default_factory=lambda: {"status": "UNASSESSED_CONTACT_UNIQUENESS"})
+
+    def diagnose_contacts(self, scene, rcond=1e-12, max_contacts=256):
+        """Opt-in bounded diagnostic of the solved impulses (SolveResult.diagnose_contacts,
+        ncp_gpu.py:311-347): at the labels of this result, the numerical rank and nullspaces of the
+        fixed-branch residual Jacobian and whether the body wrenches J^T lam are invariant observables
+        (ncp.contact_diagnostics on the GPU, observable = J^T). `scene` is the scene solved for this
+        result (a BlockScene or a dense Scene). A numerical fixed-branch statement, not a global
+        uniqueness proof; soft-step and relaxed-cone results stay unassessed. The GPU diagnostic holds
+        at most 256 contacts: larger scenes, or more than max_contacts, are UNASSESSED_SIZE_LIMIT."""
+        bs = scene.blocks() if hasattr(scene, "blocks") else scene
+        limit = min(int(max_contacts), 256)
+        if self.contact_model != "NCP_RIGID":
+            self.contact_info = {"status": "UNASSESSED_CONTACT_MODEL", "scope": self.contact_model}
+        elif bs.n_c > limit:
+            self.contact_info = {"status": "UNASSESSED_SIZE_LIMIT", "max_contacts": limit}
+        else:
+            if np.shape(self.lam) != (bs.n_c, 3):
+                raise ValueError("scene and result contact counts do not agree")
+            from . import ncp
+            J = bs._J()
+            G, b = bs.delassus(), bs.rhs()
+            if not self.bias_applied:
+                b = b - np.asarray(bs.bias, np.float64).reshape(-1)
+            lam = np.asarray(self.lam, np.float64).reshape(-1)
+            mu = np.asarray(bs.mu, np.float64)
+            labels = ncp.classify(lam, G, b, mu)
+            self.contact_info = ncp.contact_diagnostics(lam, G, b, mu, list(labels), rcond=rcond, observable=J.T)
+        self.contact_info["force_selection"] = self.force_selection
+        return self.contact_info
 
 
 # ── the solver ──

@@ -236,6 +236,42 @@ int me_ncp_sensitivity_noalloc(int32_t n_env, int32_t n_c, int32_t n_dir, const 
                                double tol_lam, double tol_rel, double rcond, double *dlam_dmu, double *dlam_db,
                                int32_t *labels, int32_t *status, void *stream);
 
+/* NCP-level sensitivity with the reference's SVD rank guard, batched, float64: ncp_ref.sensitivity
+ * with selection='minimum_norm' and observable rows (ncp/ncp_ref.py:907-959) and contact_diagnostics
+ * (:845-866), over rank_factor and guarded_response (ncp/contact_rank.py:13-30, 33-70). Same inputs
+ * as me_ncp_sensitivity for n_env problems of n_c <= 256 contacts (dirs NULL: the identity with
+ * n_dir = 3n_c, or no b columns with n_dir = 0), plus n_obs observable rows Q
+ * [n_obs, 3n_c] per environment (q_shared: one Q for all). rcond is the rank cutoff relative to the
+ * largest singular value; tol the compatibility and invariance tolerance (the reference's 1e-10).
+ * Per environment (R = n_c + n_dir response columns, the mu columns first):
+ *   X [3n_c, R] the guarded response: -A^+ B (A^+ the pseudo-inverse); NaN in every column when the
+ *     branch is on a boundary, and, when A is rank-deficient, in every column unless minimum_norm is
+ *     set, and in incompatible columns in either mode; NaN where the response is not finite;
+ *   S [3n_c] singular values, descending; V [3n_c, 3n_c] right singular vectors (column k for S[k]),
+ *   U [3n_c, 3n_c] left singular vectors (from the same method on A^T, by its descending order);
+ *   rank: singular values above rcond * S[0]; the nullspace is V[:, rank:], the left nullspace
+ *   U[:, rank:];
+ *   compat, finite, resid [R]: column compatible (|L^T B_j| <= tol |B_j|, max-scaled), unguarded
+ *   response finite, and the unscaled residual |L^T B_j|;
+ *   Y, valid [n_obs, R]: Q X where the row is invariant (Q_k N = 0 at tol), the column compatible and
+ *   the value finite, else NaN; invariant, leak [n_obs]: the invariance flag and |Q_k N|;
+ *   labels [n_c] and status as me_ncp_sensitivity, with the singular bit set from the rank.
+ * workspace: me_ncp_svd_workspace_size bytes. No engine and no thread ownership; everything on
+ * `stream`. */
+typedef struct {
+    double *X, *S, *V, *U;
+    int32_t *rank, *compat, *finite;
+    double *resid, *Y;
+    int32_t *valid, *invariant;
+    double *leak;
+    int32_t *labels, *status;
+} me_ncp_svd_out;
+int me_ncp_svd_workspace_size(int32_t n_env, int32_t n_c, int32_t n_dir, size_t *bytes);
+int me_ncp_svd(int32_t n_env, int32_t n_c, int32_t n_dir, int32_t n_obs, const double *G, const double *b,
+               const double *mu, const double *lam, const double *dirs, const int32_t *labels_in, const double *Q,
+               int32_t q_shared, double tol_lam, double tol_rel, double rcond, double tol, int32_t minimum_norm,
+               const me_ncp_svd_out *out, void *workspace, size_t bytes, void *stream);
+
 /* Exact-cone NCP projected Gauss-Seidel, the reference's NCPSolverGPU (ncp/ncp_gpu.py:783-1069):
  * lambda_c <- proj_K(lambda_c - rho_c (u_c + Gamma(u_c))) over a deterministic colouring, de Saxce
  * correction on or off, optional TGS soft-step normal row (ncp_gpu.py:491-557), per-body

@@ -11,8 +11,9 @@ following src/motion_engine/ncp/ncp_ref.py of antonbj3/3fold-motion-engine:
   active_system       ncp_ref.py:786-842   fixed-branch residual Jacobian A, B_mu, B_b
   rank_factor         contact_rank.py:13-30 SVD rank at rcond, nullspaces
   guarded_response    contact_rank.py:33-70 A X = -B, NaN for ambiguous columns unless a
-                                             minimum-norm selection is asked for
+                                             minimum-norm selection is asked for; observable rows
   sensitivity         ncp_ref.py:907-959   (dlam/dmu, dlam/db, info)
+  contact_diagnostics ncp_ref.py:845-866   rank, nullspaces, observable invariance
   sensitivity_fd      ncp_ref.py:962-993   central differences of the PGS solution
 
 Pins (tests/test_ncp_oracle.py): the reference's all-slip analytic scenes
@@ -20,7 +21,12 @@ Pins (tests/test_ncp_oracle.py): the reference's all-slip analytic scenes
 active-set Jacobian recorded in data/ncp/adjoint_fix_table.csv; central finite differences of the
 PGS solution (the reference's own criterion, tests/test_ncp_reference.py:316-350); the diagonal
 slip case in closed form; the hyperstatic 4-point cube of data/ncp/scenes/00.json flagged singular
-with NaN derivatives."""
+with NaN derivatives; the redundant-corner contracts of tests/test_ncp_redundant_contacts.py
+(REDUNDANT_CONTACTS with an orthonormal nullspace and the alternating-normal left-null witness in all
+64 closed corner instances, the minimum-norm response orthogonal to the nullspace, observable rows
+invariant for the wrench and not for a spinning slip, the boundary and near-singular guards).
+
+corner_box_scene / algebraic_branch are the fixtures of those contracts."""
 from __future__ import annotations
 
 import numpy as np
@@ -241,43 +247,85 @@ def rank_factor(A, rcond=1e-12):
     cond = float(s[0] / s[-1]) if s.size and s[-1] > 0 else (float("inf") if s.size else 0.0)
     info = {"status": "REDUNDANT_CONTACTS" if rank < A.shape[1] else "UNIQUE_LINEAR_SYSTEM",
             "rank": rank, "nullity": A.shape[1] - rank, "nullspace": Vt[rank:].T.copy(),
-            "left_nullspace": U[:, rank:].copy(), "singular_values": s.copy(), "cond": cond,
-            "singular": rank < A.shape[1]}
+            "left_nullspace": U[:, rank:].copy(), "singular_values": s.copy(), "rank_cutoff": cutoff,
+            "cond": cond, "singular": rank < A.shape[1]}
     return U, s, Vt, info
 
 
-def guarded_response(A, B, factors, selection=None, tol=1e-10):
+def guarded_response(A, B, factors, selection=None, observable=None, tol=1e-10):
+    """contact_rank.py:33-70: X = -A^+ B with NaN for ambiguous (singular A, no selection),
+    incompatible and non-finite columns; observable rows Q reported where Q N = 0."""
     U, s, Vt, info = factors
     B = np.asarray(B, np.float64)
     rank = info["rank"]
     bscale = np.max(np.abs(B), axis=0, initial=0.0)
     sB = B / np.where(bscale > 0, bscale, 1.0)
-    compatible = np.linalg.norm(info["left_nullspace"].T @ sB, axis=0) <= tol * np.linalg.norm(sB, axis=0)
+    rel = np.linalg.norm(info["left_nullspace"].T @ sB, axis=0)
+    compatible = rel <= tol * np.linalg.norm(sB, axis=0)
     if info["singular"]:
-        X = -(Vt[:rank].T @ ((U[:, :rank].T @ B) / s[:rank, None]))
-        if selection != "minimum_norm":
-            X = np.full_like(X, np.nan)
+        projected = -(Vt[:rank].T @ ((U[:, :rank].T @ B) / s[:rank, None]))
+        X = projected.copy() if selection == "minimum_norm" else np.full_like(projected, np.nan)
         X[:, ~compatible] = np.nan
     else:
-        X = -np.linalg.solve(A, B)
-    X[:, ~np.isfinite(X).all(axis=0)] = np.nan
-    return X, compatible
+        projected = -np.linalg.solve(A, B)
+        X = projected.copy()
+    finite = np.isfinite(projected).all(axis=0)
+    X[:, ~finite] = np.nan
+    out = {"response_finite": finite, "compatible": compatible, "compatibility_residual": rel * bscale}
+    if observable is not None:
+        Q = np.atleast_2d(np.asarray(observable, np.float64))
+        qscale = np.max(np.abs(Q), axis=1, initial=0.0)
+        sQ = Q / np.where(qscale > 0, qscale, 1.0)[:, None]
+        leak = np.linalg.norm(sQ @ info["nullspace"], axis=1)
+        invariant = leak <= tol * np.linalg.norm(sQ, axis=1)
+        Y = Q @ projected
+        valid = invariant[:, None] & compatible[None, :] & np.isfinite(Y)
+        Y[~valid] = np.nan
+        out.update(observable=Y, observable_valid=valid, observable_invariant=invariant,
+                   observable_nullspace_residual=leak * qscale)
+    return X, out
 
 
-def sensitivity(lam, G, b, mu, labels, rcond=1e-12, *, selection=None, db_dparam=None):
+def sensitivity(lam, G, b, mu, labels, rcond=1e-12, *, selection=None, observable=None, db_dparam=None):
     """(dlam/dmu, dlam/db, info); db_dparam maps parameters to b (dlam/db @ db_dparam)."""
     A, Bmu, Bb, boundary = active_system(lam, G, b, mu, labels)
     if db_dparam is not None:
         Bb = Bb @ np.asarray(db_dparam, np.float64)
     factors = rank_factor(A, rcond)
-    dm, _ = guarded_response(A, Bmu, factors, selection)
-    db, _ = guarded_response(A, Bb, factors, selection)
+    dm, im = guarded_response(A, Bmu, factors, selection, observable)
+    db, ib = guarded_response(A, Bb, factors, selection, observable)
     info = factors[3]
-    info.update(boundary=boundary, labels=list(labels))
+    info.update(boundary=boundary, labels=list(labels), mu_compatible=im["compatible"],
+                b_compatible=ib["compatible"], mu_response_finite=im["response_finite"],
+                b_response_finite=ib["response_finite"], mu_compatibility_residual=im["compatibility_residual"],
+                b_compatibility_residual=ib["compatibility_residual"])
+    if observable is not None:
+        info.update(observable_dmu=im["observable"], observable_db=ib["observable"],
+                    observable_mu_valid=im["observable_valid"], observable_b_valid=ib["observable_valid"],
+                    observable_invariant=im["observable_invariant"],
+                    observable_nullspace_residual=im["observable_nullspace_residual"])
     if boundary:
         dm[:] = np.nan
         db[:] = np.nan
+        if observable is not None:
+            info["observable_dmu"][:] = np.nan
+            info["observable_db"][:] = np.nan
+            info["observable_mu_valid"][:] = False
+            info["observable_b_valid"][:] = False
     return dm, db, info
+
+
+def contact_diagnostics(lam, G, b, mu, labels, rcond=1e-12, observable=None):
+    """ncp_ref.py:845-866: rank, nullspaces and observable invariance at a fixed branch."""
+    A, _, _, boundary = active_system(lam, G, b, mu, labels)
+    factors = rank_factor(A, rcond)
+    info = factors[3]
+    info.update(boundary=boundary, labels=list(labels))
+    if observable is not None:
+        _, chk = guarded_response(A, np.zeros((A.shape[0], 0)), factors, observable=observable)
+        info.update(observable_invariant=chk["observable_invariant"],
+                    observable_nullspace_residual=chk["observable_nullspace_residual"])
+    return info
 
 
 def sensitivity_fd(G, b, mu, lam_star, h_mu=1e-6, h_b=1e-6, iters=20000, tol=1e-14):
@@ -301,6 +349,58 @@ def sensitivity_fd(G, b, mu, lam_star, h_mu=1e-6, h_b=1e-6, iters=20000, tol=1e-
         db[:, j] = (solve_ncp_pgs(G, bp, mu, iters, tol, lam_star)[0]
                     - solve_ncp_pgs(G, bm, mu, iters, tol, lam_star)[0]) / (2 * h_b)
     return dmu, db
+
+
+def corner_box_scene(half=(0.5, 0.5, 0.5), mass=1.0, angle=0.0, moving=True, mu=0.3, dt=0.01):
+    """One solid box standing on the ground on its four bottom corners, tilted by `angle` about y, the
+    fixture of the reference's redundant-contact tests (tests/test_ncp_redundant_contacts.py:15-28):
+    contact frame n = the box's z axis, t1 = n x a (a = x, or y when |n_x| >= 0.9), t2 = n x t1 (the
+    reference's _tangents, ncp/ncp_ref.py:1057-1062), J_c = [D, -D [r]x] per corner (r from the
+    centre), Minv = diag(1/m, I^-1 in world), v_free = the box's +x velocity (when moving) + dt g.
+    Returns (G, b, mu, J, Minv)."""
+    h = np.asarray(half, np.float64)
+    c, s_ = np.cos(angle), np.sin(angle)
+    rot = np.array([[c, 0.0, s_], [0.0, 1.0, 0.0], [-s_, 0.0, c]])
+    centre = rot @ np.array([0.0, 0.0, h[2]])
+    nrm = rot[:, 2]
+    a = np.array([1.0, 0.0, 0.0]) if abs(nrm[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    t1 = np.cross(nrm, a)
+    t1 /= np.linalg.norm(t1)
+    D = np.stack([nrm, t1, np.cross(nrm, t1)])
+    J = np.zeros((12, 6))
+    for k, (sx, sy) in enumerate(((-1, -1), (-1, 1), (1, -1), (1, 1))):
+        r = rot @ np.array([sx * h[0], sy * h[1], 0.0]) - centre
+        rx = np.array([[0.0, -r[2], r[1]], [r[2], 0.0, -r[0]], [-r[1], r[0], 0.0]])
+        J[3 * k:3 * k + 3, :3] = D
+        J[3 * k:3 * k + 3, 3:] = -D @ rx
+    Ib = mass / 3.0 * np.array([h[1] ** 2 + h[2] ** 2, h[0] ** 2 + h[2] ** 2, h[0] ** 2 + h[1] ** 2])
+    Minv = np.zeros((6, 6))
+    Minv[:3, :3] = np.eye(3) / mass
+    Minv[3:, 3:] = rot @ np.diag(1.0 / Ib) @ rot.T
+    vf = np.zeros(6)
+    vf[:3] = (rot @ np.array([1.0, 0.0, 0.0]) if moving else 0.0) + dt * np.array([0.0, 0.0, -9.81])
+    G = J @ Minv @ J.T
+    return G, J @ vf, np.full(4, mu), J, Minv
+
+
+def algebraic_branch(G, mu, labels, spin=False):
+    """A residual fixture with the given labels (the reference's algebraic_branch,
+    tests/test_ncp_redundant_contacts.py:31-42): unit normal impulses, slip impulses on the cone
+    against unit slip velocities (diagonal per corner with spin), open contacts separating; b = u -
+    G lam. Returns (lam, b)."""
+    n = 3 * len(mu)
+    lam, u = np.zeros(n), np.zeros(n)
+    for c, label in enumerate(labels):
+        i = 3 * c
+        lam[i] = 0.0 if label == "open" else 1.0
+        if label == "open":
+            u[i] = 0.5
+        if label == "slip":
+            sh = (np.array([1.0, 0.0]) if not spin
+                  else np.array([(c % 2) * 2 - 1.0, (c // 2) * 2 - 1.0]) / np.sqrt(2.0))
+            lam[i + 1:i + 3] = -mu[c] * sh
+            u[i + 1:i + 3] = sh
+    return lam, u - G @ lam
 
 
 def all_slip_scene(n_c):
