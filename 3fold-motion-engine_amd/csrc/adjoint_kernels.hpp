@@ -103,6 +103,7 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
     if (tid < np) {
         myc = A.Pp.col[T.psorted[p0 + tid]];
         if (myc < 0 || myc >= ncol) myc = ncol > 0 ? ncol - 1 : 0;
+        myc = tile_class(T, myc, ncol);
         atomicAdd(&cnt[myc], 1);
         atomicMax(&maxc, myc + 1);
     }

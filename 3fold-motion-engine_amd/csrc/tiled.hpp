@@ -33,6 +33,8 @@ struct TiledBuffers {
     int *bsorted, *bslot, *psorted;
     int *tb0, *tb1, *tp0, *tp1;
     int *ccount_jp, *csub, *cbase;
+    int *ccls;                   // JP class of every final colour [ME_NCOLMAX], and at [ME_NCOLMAX] the colour
+                                 // count it was written for (k_t_plan); kept with the colour cache
     float *invMs, *invIws;       // body constants in island order (index = position in bsorted)
     int *psorted2;               // big tiles: pairs counting-sorted by colour
     int *rec;                    // big tiles: 8 ints per colour-sorted pair (p, a, b, la, lb, st, cnt, flags)
@@ -111,6 +113,15 @@ inline void tiled_layout_state(TiledBuffers &t, BumpT &B, int N, int capP, int c
     t.tp0 = B.template take<int>(n);
     t.tp1 = B.template take<int>(n);
     t.prep = B.template take<float>(20 * (size_t)capC);
+    t.ccls = B.template take<int>(ME_NCOLMAX + 1);
+}
+
+// The JP class of final colour c (colours base[j] .. base[j+1]-1 split JP class j). Pairs of one JP
+// class share no dynamic body, so a colour step can take the whole class: the same result as its
+// sub-colours one after another, in fewer steps. The identity when the table was written for another
+// colour count.
+__device__ __forceinline__ int tile_class(const TiledBuffers &T, int c, int ncol) {
+    return (T.ccls[ME_NCOLMAX] == ncol && c >= 0 && c < ncol) ? T.ccls[c] : c;
 }
 
 // tile arrays rebuilt within every substep
@@ -683,6 +694,9 @@ __global__ void k_t_plan(int *istat, int K, TiledBuffers T) {
     }
     T.cbase[0] = 0;
     for (int c = 0; c < nc; ++c) T.cbase[c + 1] = T.cbase[c] + T.csub[c];
+    for (int c = 0; c < nc; ++c)
+        for (int s = T.cbase[c]; s < T.cbase[c + 1] && s < ME_NCOLMAX; ++s) T.ccls[s] = c;
+    T.ccls[ME_NCOLMAX] = tot;
     istat[IS_NCOL] = tot;
 }
 
@@ -1028,6 +1042,7 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers
         if (tid < np) {
             myc = A.Pp.col[T.psorted[p0 + tid]];
             if (myc < 0 || myc >= ncol) myc = ncol > 0 ? ncol - 1 : 0;  // only after a flagged colour overflow
+            myc = tile_class(T, myc, ncol);
             atomicAdd(&cnt[myc], 1);
             atomicMax(&maxc, myc + 1);
         }

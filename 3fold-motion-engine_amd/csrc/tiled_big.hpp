@@ -246,6 +246,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     for (int k = tid; k < np; k += T_BIGT_WG) {
         int c = A.Pp.col[T.psorted[p0 + k]];
         if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+        c = tile_class(T, c, ncol);
         atomicAdd(&cst[c], 1);
         atomicMax(&maxc, c + 1);
     }
@@ -263,6 +264,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         int p = T.psorted[p0 + k];
         int c = A.Pp.col[p];
         if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+        c = tile_class(T, c, ncol);
         int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
         bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
         // within a colour (order-free), pairs with two dynamic bodies first and one-sided pairs
@@ -553,6 +555,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
             for (int k = tid; k < np; k += T_BIG_WG) {
                 int c = A.Pp.col[T.psorted[p0 + k]];
                 if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+                c = tile_class(T, c, ncol);
                 atomicAdd(&cst[c], 1);
                 atomicMax(&maxc, c + 1);
             }
@@ -569,6 +572,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
                 int p = T.psorted[p0 + k];
                 int c = A.Pp.col[p];
                 if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+                c = tile_class(T, c, ncol);
                 int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
                 bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
                 int pos = (dA && dB) ? cst[c] + atomicAdd(&cfl[c], 1) : cst[c + 1] - 1 - atomicAdd(&cfe[c], 1);
