@@ -143,15 +143,17 @@ class Rollout:
                 t0 = seg * self.k
                 n_sub = (min(self.T, t0 + self.k) - t0) * self.S
                 tapes, after = [], []
-                for _ in range(n_sub):
+                for i in range(n_sub):
                     buf, ptr = self._tape()
                     _lib.check(L.me_step_taped(e._h, self.sdt, C.c_void_p(ptr), self.tape_bytes, stream),
                                "me_step_taped")
                     tapes.append((buf, ptr))
-                    after.append(self._snap())
+                    # the state after the segment's last substep is the engine's own: no copy
+                    after.append(self._snap() if i < n_sub - 1 else None)
                 self._check()
                 for i in range(n_sub - 1, -1, -1):
-                    self._restore(after[i])
+                    if after[i] is not None:
+                        self._restore(after[i])
                     _, ptr = tapes[i]
                     _lib.check(L.me_adjoint_substep(e._h, self.sdt, C.c_void_p(ptr), self.tape_bytes, _vp(xbar),
                                                     _vp(qbar), _vp(vbar), _vp(wbar), _vp(jend), _vp(jprev),

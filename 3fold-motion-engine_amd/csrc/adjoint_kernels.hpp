@@ -35,6 +35,24 @@ __device__ __forceinline__ void a33(float *p, size_t i, const M3<float> &M) {
     for (int k = 0; k < 9; ++k) atomicAdd(p + 9 * i + k, M.m[k / 3][k % 3]);
 }
 
+// zero fill of up to 8 arrays in one launch (the reverse pass's accumulators): a grid-stride loop
+// over the concatenated index space
+struct ZeroSpans {
+    float *p[8];
+    size_t n[8];
+    int k;
+};
+__global__ void k_zero_spans(ZeroSpans Z) {
+    size_t tot = 0;
+    for (int s = 0; s < Z.k; ++s) tot += Z.n[s];
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < tot; g += (size_t)gridDim.x * blockDim.x) {
+        size_t i = g;
+        int s = 0;
+        while (s < Z.k - 1 && i >= Z.n[s]) { i -= Z.n[s]; ++s; }
+        Z.p[s][i] = 0.0f;
+    }
+}
+
 // integrate reverse; vbar/wbar: adjoint of the step's output velocities in, of the solve's output out
 __global__ void k_adj_integ(TiledArgs A, float *xbar, float *qbar, float *vbar, float *wbar) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

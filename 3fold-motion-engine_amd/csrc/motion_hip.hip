@@ -1468,19 +1468,14 @@ int me_adjoint_substep(me_engine *e, double dt, void *tape, size_t bytes, float 
     hipStream_t s = (hipStream_t)stream;
     int N = e->N, capC = e->capC;
     const AdjTape &t = e->tape;
-    auto zero = [&](float *p, size_t n) {
-        for (size_t off = 0; off < n; off += (size_t)1 << 30) {
-            int m = (int)((n - off) < ((size_t)1 << 30) ? (n - off) : ((size_t)1 << 30));
-            hipLaunchKernelGGL(k_fill_i32, dim3(nblk(m)), dim3(ME_THREADS), 0, s, (int *)(p + off), m, 0);
-        }
-    };
-    zero(t.cbar, 20 * (size_t)capC);
-    zero(t.jbi, 3 * (size_t)capC);
-    zero(t.gbar, 7 * (size_t)capC);
-    zero(t.Ibar, 9 * (size_t)N);
-    zero(t.pvb, 3 * (size_t)N);
-    zero(t.pob, 3 * (size_t)N);
-    zero(jbar_prev, 3 * (size_t)capC);
+    // the reverse pass's accumulators, zeroed in one launch
+    ZeroSpans z{};
+    float *zp[7] = {t.cbar, t.jbi, t.gbar, t.Ibar, t.pvb, t.pob, jbar_prev};
+    size_t zn[7] = {20 * (size_t)capC, 3 * (size_t)capC, 7 * (size_t)capC, 9 * (size_t)N, 3 * (size_t)N, 3 * (size_t)N,
+                    3 * (size_t)capC};
+    for (int k = 0; k < 7; ++k) { z.p[k] = zp[k]; z.n[k] = zn[k]; }
+    z.k = 7;
+    hipLaunchKernelGGL(k_zero_spans, dim3(2048), dim3(ME_THREADS), 0, s, z);
     e->taping = true;
     TiledArgs A = tiled_args_of(e, (float)dt);
     e->taping = false;
