@@ -20,8 +20,30 @@ using ad::V3;
 
 __device__ __forceinline__ V3<float> g3(const float *p, size_t i) { return ad::v3<float>(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 __device__ __forceinline__ void s3(float *p, size_t i, V3<float> a) { p[3 * i] = a.x; p[3 * i + 1] = a.y; p[3 * i + 2] = a.z; }
-__device__ __forceinline__ void a3(float *p, size_t i, V3<float> a) {
-    atomicAdd(p + 3 * i, a.x); atomicAdd(p + 3 * i + 1, a.y); atomicAdd(p + 3 * i + 2, a.z);
+// Body-level adjoint contributions (several contacts, pairs and candidates per body) are summed in
+// int64 fixed point, 2^-32 per unit: integer addition is associative, so the sums, and the gradients,
+// are the same bits whatever order the atomics land in. Per body ADJ_ACC words: the centre (3),
+// the orientation (4), the world inverse inertia (9) and the friction coefficient (1). A contribution
+// of magnitude 2^30 or more flags OVF_ADJ and is dropped (the step then raises).
+#define ADJ_ACC 17
+#define ADJ_X 0
+#define ADJ_Q 3
+#define ADJ_I 7
+#define ADJ_MU 16
+__device__ __forceinline__ void fx_add(long long *acc, int f, size_t i, float v, int *istat) {
+    if (v == 0.0f) return;
+    if (!(fabsf(v) < 1073741824.0f)) {  // also NaN
+        atomicOr(&istat[IS_OVF], OVF_ADJ);
+        return;
+    }
+    const long long q = __double2ll_rn((double)v * 4294967296.0);
+    atomicAdd(reinterpret_cast<unsigned long long *>(acc + ADJ_ACC * i + f), (unsigned long long)q);
+}
+__device__ __forceinline__ float fx_get(const long long *acc, int f, size_t i) {
+    return (float)((double)acc[ADJ_ACC * i + f] * (1.0 / 4294967296.0));
+}
+__device__ __forceinline__ void a3(long long *acc, int f, size_t i, V3<float> a, int *istat) {
+    fx_add(acc, f, i, a.x, istat); fx_add(acc, f + 1, i, a.y, istat); fx_add(acc, f + 2, i, a.z, istat);
 }
 __device__ __forceinline__ Q4<float> g4(const float *p, size_t i) {
     return ad::q4<float>(p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]);
@@ -31,8 +53,8 @@ __device__ __forceinline__ M3<float> g33(const float *p, size_t i) {
     for (int k = 0; k < 9; ++k) r.m[k / 3][k % 3] = p[9 * i + k];
     return r;
 }
-__device__ __forceinline__ void a33(float *p, size_t i, const M3<float> &M) {
-    for (int k = 0; k < 9; ++k) atomicAdd(p + 9 * i + k, M.m[k / 3][k % 3]);
+__device__ __forceinline__ void a33(long long *acc, size_t i, const M3<float> &M, int *istat) {
+    for (int k = 0; k < 9; ++k) fx_add(acc, ADJ_I + k, i, M.m[k / 3][k % 3], istat);
 }
 
 // zero fill of up to 8 arrays in one launch (the reverse pass's accumulators): a grid-stride loop
@@ -324,13 +346,14 @@ __global__ void __launch_bounds__(T_SMALL_P) k_adj_tiles(TiledArgs A, TiledBuffe
             o[12] = C.t2.x; o[13] = C.t2.y; o[14] = C.t2.z;
             o[15] = C.meff; o[16] = C.meft1; o[17] = C.meft2; o[18] = C.bias; o[19] = C.mu;
         }
-        if (P.f.dynA) a33(A.tape.Ibar, P.a, IAb);
-        if (P.f.dynB) a33(A.tape.Ibar, P.b, IBb);
+        if (P.f.dynA) a33(A.tape.acc, P.a, IAb, A.istat);
+        if (P.f.dynB) a33(A.tape.acc, P.b, IBb, A.istat);
     }
 }
 
-// per contact: constants -> contact data and bodies; the warm-start carry; pair friction -> body mu
-__global__ void k_adj_contacts(TiledArgs A, float *xbar, float *mubar, float *jbar_prev) {
+// per contact: constants -> contact data and bodies (centres and inverse inertias into the body
+// sums); the warm-start carry; pair friction -> body mu
+__global__ void k_adj_contacts(TiledArgs A, float *jbar_prev) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     int C = A.istat[IS_C];
     if (C > A.capC) C = A.capC;
@@ -351,23 +374,23 @@ __global__ void k_adj_contacts(TiledArgs A, float *xbar, float *mubar, float *jb
                  hasB ? g3(A.tape.xpre, b) : ad::zero3<float>(), hasB, IA, IB, A.Cc.pen[c], A.dt, cb, nb, pAb, pBb,
                  xAb, xBb, IAb, IBb, penb, mub);
     bool dA = A.b.invM[a] > 0.0f, dB = hasB && A.b.invM[b] > 0.0f;
-    if (dA) { a3(xbar, a, xAb); a33(A.tape.Ibar, a, IAb); }
-    if (dB) { a3(xbar, b, xBb); a33(A.tape.Ibar, b, IBb); }
+    if (dA) { a3(A.tape.acc, ADJ_X, a, xAb, A.istat); a33(A.tape.acc, a, IAb, A.istat); }
+    if (dB) { a3(A.tape.acc, ADJ_X, b, xBb, A.istat); a33(A.tape.acc, b, IBb, A.istat); }
     float *g = A.tape.gbar + 7 * (size_t)c;
     V3<float> pb = pAb + pBb;  // box contacts carry one point for both bodies
     g[0] = pb.x; g[1] = pb.y; g[2] = pb.z; g[3] = nb.x; g[4] = nb.y; g[5] = nb.z; g[6] = penb;
     // pair friction coefficient (pair_mu): mu_a (ground), mu_a == mu_b, or sqrt(mu_a mu_b)
     if (!hasB) {
-        atomicAdd(mubar + a, mub);
+        fx_add(A.tape.acc, ADJ_MU, a, mub, A.istat);
     } else {
         float ma = A.b.mu[a], mb = A.b.mu[b];
         if (ma == mb) {
-            atomicAdd(mubar + a, 0.5f * mub);
-            atomicAdd(mubar + b, 0.5f * mub);
+            fx_add(A.tape.acc, ADJ_MU, a, 0.5f * mub, A.istat);
+            fx_add(A.tape.acc, ADJ_MU, b, 0.5f * mub, A.istat);
         } else {
             float m = sqrtf(ma * mb);
-            atomicAdd(mubar + a, mub * mb / (2.0f * m));
-            atomicAdd(mubar + b, mub * ma / (2.0f * m));
+            fx_add(A.tape.acc, ADJ_MU, a, mub * mb / (2.0f * m), A.istat);
+            fx_add(A.tape.acc, ADJ_MU, b, mub * ma / (2.0f * m), A.istat);
         }
     }
     int w = A.tape.widx[c];
@@ -436,7 +459,7 @@ __device__ __forceinline__ void adj_narrow_body(int a, int bb, int k, const floa
 }
 
 __global__ void k_adj_narrow(TiledArgs A, int cap, const int *cand_a, const int *cand_b, const int *slot_n,
-                             const int *slot_off, float margin, float *xbar, float *qbar) {
+                             const int *slot_off, float margin) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int n = A.istat[IS_CAND];
     if (n > cap) n = cap;
@@ -449,26 +472,29 @@ __global__ void k_adj_narrow(TiledArgs A, int cap, const int *cand_a, const int 
     float g[7];
     if (A.b.invM[a] > 0.0f) {
         adj_narrow_body(a, bb, k, gb, margin, A.b, A.tape.xpre, A.tape.qpre, true, g, A.istat + IS_OVF);
-        atomicAdd(xbar + 3 * (size_t)a, g[0]); atomicAdd(xbar + 3 * (size_t)a + 1, g[1]);
-        atomicAdd(xbar + 3 * (size_t)a + 2, g[2]);
-        for (int m = 0; m < 4; ++m) atomicAdd(qbar + 4 * (size_t)a + m, g[3 + m]);
+        for (int m = 0; m < 7; ++m) fx_add(A.tape.acc, ADJ_X + m, a, g[m], A.istat);
     }
     if (bb >= 0 && A.b.invM[bb] > 0.0f) {
         adj_narrow_body(a, bb, k, gb, margin, A.b, A.tape.xpre, A.tape.qpre, false, g, A.istat + IS_OVF);
-        atomicAdd(xbar + 3 * (size_t)bb, g[0]); atomicAdd(xbar + 3 * (size_t)bb + 1, g[1]);
-        atomicAdd(xbar + 3 * (size_t)bb + 2, g[2]);
-        for (int m = 0; m < 4; ++m) atomicAdd(qbar + 4 * (size_t)bb + m, g[3 + m]);
+        for (int m = 0; m < 7; ++m) fx_add(A.tape.acc, ADJ_X + m, bb, g[m], A.istat);
     }
 }
 
-// world inverse inertia -> orientation
-__global__ void k_adj_body(TiledArgs A, float *qbar) {
+// the body's fixed-point sums into its adjoints (centre, orientation, friction), then world inverse
+// inertia -> orientation
+__global__ void k_adj_body(TiledArgs A, float *xbar, float *qbar, float *mubar) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.N) return;
+    const long long *acc = A.tape.acc;
+    for (int k = 0; k < 3; ++k) xbar[3 * i + k] += fx_get(acc, ADJ_X + k, i);
+    for (int k = 0; k < 4; ++k) qbar[4 * i + k] += fx_get(acc, ADJ_Q + k, i);
+    mubar[i] += fx_get(acc, ADJ_MU, i);
     if (!(A.b.invM[i] > 0.0f) || A.b.kin[i] == 1) return;
+    M3<float> Ib;
+    for (int k = 0; k < 9; ++k) Ib.m[k / 3][k % 3] = fx_get(acc, ADJ_I + k, i);
     Q4<float> q = g4(A.tape.qpre, i);
     M3<float> R = ad::quat_to_matrix(q), Rb = ad::zero33<float>();
-    ad::inv_inertia_rev(R, g33(A.b.IbInv, i), g33(A.tape.Ibar, i), Rb);
+    ad::inv_inertia_rev(R, g33(A.b.IbInv, i), Ib, Rb);
     Q4<float> qb = ad::q4<float>(0.0f, 0.0f, 0.0f, 0.0f);
     ad::quat_to_matrix_rev(q, Rb, qb);
     qbar[4 * i] += qb.x; qbar[4 * i + 1] += qb.y; qbar[4 * i + 2] += qb.z; qbar[4 * i + 3] += qb.w;

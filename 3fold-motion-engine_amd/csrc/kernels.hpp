@@ -19,7 +19,8 @@ enum {
 };
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,
        OVF_TAPE = 32 /* a recorded (differentiated) substep met a tile beyond one small tile */,
-       OVF_GIANT = 64 /* a grid barrier of the giant-tile sweep waited past its bound */ };
+       OVF_GIANT = 64 /* a grid barrier of the giant-tile sweep waited past its bound */,
+       OVF_ADJ = 128 /* an adjoint contribution beyond the fixed-point accumulators' range */ };
 
 #define ME_NCOLMAX 512
 #define ME_MAXCAND 32

@@ -719,9 +719,9 @@ size_t tape_layout(const me_engine *e, char *base, AdjTape *t) {
     x.cbar = B.take<float>(20 * C);
     x.jbi = B.take<float>(3 * C);
     x.gbar = B.take<float>(7 * C);
-    x.Ibar = B.take<float>(9 * N);
     x.pvb = B.take<float>(3 * N);
     x.pob = B.take<float>(3 * N);
+    x.acc = B.take<long long>(ADJ_ACC * N);
     if (t) *t = x;
     return B.off;
 }
@@ -1470,9 +1470,9 @@ int me_adjoint_substep(me_engine *e, double dt, void *tape, size_t bytes, float 
     const AdjTape &t = e->tape;
     // the reverse pass's accumulators, zeroed in one launch
     ZeroSpans z{};
-    float *zp[7] = {t.cbar, t.jbi, t.gbar, t.Ibar, t.pvb, t.pob, jbar_prev};
-    size_t zn[7] = {20 * (size_t)capC, 3 * (size_t)capC, 7 * (size_t)capC, 9 * (size_t)N, 3 * (size_t)N, 3 * (size_t)N,
-                    3 * (size_t)capC};
+    float *zp[7] = {t.cbar, t.jbi, t.gbar, t.pvb, t.pob, jbar_prev, (float *)t.acc};
+    size_t zn[7] = {20 * (size_t)capC, 3 * (size_t)capC, 7 * (size_t)capC, 3 * (size_t)N, 3 * (size_t)N,
+                    3 * (size_t)capC, 2 * ADJ_ACC * (size_t)N};
     for (int k = 0; k < 7; ++k) { z.p[k] = zp[k]; z.n[k] = zn[k]; }
     z.k = 7;
     hipLaunchKernelGGL(k_zero_spans, dim3(2048), dim3(ME_THREADS), 0, s, z);
@@ -1482,10 +1482,10 @@ int me_adjoint_substep(me_engine *e, double dt, void *tape, size_t bytes, float 
     int ntile = tiled_ntile(A);
     hipLaunchKernelGGL(k_adj_integ, dim3(nblk(N)), dim3(ME_THREADS), 0, s, A, xbar, qbar, vbar, wbar);
     hipLaunchKernelGGL(k_adj_tiles, dim3(ntile), dim3(T_SMALL_P), 0, s, A, e->tb, vbar, wbar, jbar_end);
-    hipLaunchKernelGGL(k_adj_contacts, dim3(nblk(capC)), dim3(ME_THREADS), 0, s, A, xbar, mubar, jbar_prev);
+    hipLaunchKernelGGL(k_adj_contacts, dim3(nblk(capC)), dim3(ME_THREADS), 0, s, A, jbar_prev);
     hipLaunchKernelGGL(k_adj_narrow, dim3(nblk(e->cap_cand)), dim3(ME_THREADS), 0, s, A, e->cap_cand, e->cand_a,
-                       e->cand_b, e->S.n, e->slot_off, e->cfg.margin, xbar, qbar);
-    hipLaunchKernelGGL(k_adj_body, dim3(nblk(N)), dim3(ME_THREADS), 0, s, A, qbar);
+                       e->cand_b, e->S.n, e->slot_off, e->cfg.margin);
+    hipLaunchKernelGGL(k_adj_body, dim3(nblk(N)), dim3(ME_THREADS), 0, s, A, xbar, qbar, mubar);
     LAUNCH_CHECK();
     return ME_OK;
 }

@@ -70,8 +70,8 @@ struct AdjTape {
     float *cbar;         // [20 * capC] adjoint of the per-contact constants (k_t_prep layout)
     float *jbi;          // [3 * capC] adjoint of the warm-started impulses
     float *gbar;         // [7 * capC] adjoint of contact point, normal, penetration
-    float *Ibar;         // [9N] adjoint of the world inverse inertia
     float *pvb, *pob;    // [3N] adjoint of the pseudo-velocities
+    long long *acc;      // [ADJ_ACC * N] fixed-point sums of the body-level adjoint contributions
 };
 
 struct TiledArgs {

@@ -244,6 +244,9 @@ class _HipContactEngine:
             if ovf & 64:
                 raise RuntimeError(f"{s.name}.step: the grid sweep of an island beyond one workgroup timed out "
                                    f"waiting for its workgroups; the step's state is not valid")
+            if ovf & 128:
+                raise RuntimeError(f"{s.name}: an adjoint contribution reached 2^30, beyond the range of the "
+                                   f"fixed-point body sums; the gradient is not valid")
             raise RuntimeError(f"{s.name}.step: capacity exceeded ({', '.join(what)}); raise max_pairs / "
                                f"max_contacts")
 
