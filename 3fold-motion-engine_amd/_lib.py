@@ -12,7 +12,7 @@ LIB_PATH = HERE / "libmotion_hip.so"
 
 ME_OK, ME_EARG, ME_ERUNTIME, ME_ETHREAD, ME_ECAPACITY, ME_ENONFINITE = 0, 1, 2, 3, 4, 5
 SOLVER_JACOBI, SOLVER_PAIR_CHUNKS, SOLVER_COLOR_CACHE = 0, 1, 2
-FLAG_TILED, FLAG_PER_CONTACT, FLAG_GRAPH = 1, 2, 4
+FLAG_TILED, FLAG_PER_CONTACT, FLAG_GRAPH, FLAG_FAST_MATH = 1, 2, 4, 8
 
 (BUF_KEY, BUF_BI, BUF_BJ, BUF_PA, BUF_PB, BUF_N, BUF_PEN, BUF_JN, BUF_JT1, BUF_JT2, BUF_JP,
  BUF_PBI, BUF_PBJ, BUF_PSTART, BUF_PCOUNT, BUF_COLOR, BUF_VSNAP, BUF_PV, BUF_PO, BUF_INVIW,

@@ -136,6 +136,7 @@ struct me_engine {
     long long giant_wait_ticks = 0;
     int big_grid = 0;  // workgroups of k_t_big / k_t_color: one per CU their LDS allows
     int small_grid = 0;  // workgroups of the persistent k_t_small: as many as can be resident
+    int small_grid_fast = 0;  // the same for its fast-arithmetic variant
 };
 
 namespace {
@@ -284,7 +285,7 @@ void layout(me_engine *e, char *base, size_t *bytes) {
         e->raw.jt2 = B.take<float>(R);
         e->raw.keep = B.take<int>(R);
     }
-    tiled_layout_scratch(e->tb, B, N, (int)Pc, (int)C);
+    tiled_layout_scratch(e->tb, B, N, (int)Pc, (int)C, (c.flags & ME_FLAG_FAST_MATH) != 0);
     e->nbsum = (int)((scan_n + SCAN_BLOCK - 1) / SCAN_BLOCK) + 1;
     e->bsum = B.take<int>((size_t)e->nbsum);
     *bytes = (B.off + 255) & ~size_t(255);
@@ -306,6 +307,8 @@ int validate(const me_config *c) {
     if (!(c->margin >= 0.0f)) return fail(ME_EARG, "margin must be >= 0");
     if ((c->flags & ME_FLAG_GRAPH) && (!(c->flags & ME_FLAG_TILED) || c->solver == ME_SOLVER_JACOBI))
         return fail(ME_EARG, "graph capture needs the tiled solve and a coloured solver");
+    if ((c->flags & ME_FLAG_FAST_MATH) && (!(c->flags & ME_FLAG_TILED) || c->solver != ME_SOLVER_COLOR_CACHE))
+        return fail(ME_EARG, "fast arithmetic needs the tiled ColorCache solver");
     return ME_OK;
 }
 
@@ -732,6 +735,8 @@ int tape_check(me_engine *e, void *tape, size_t bytes) {
     const me_config &c = e->cfg;
     if (c.solver != ME_SOLVER_COLOR_CACHE || !(c.flags & ME_FLAG_TILED) || c.n_beads > 0)
         return fail(ME_EARG, "the adjoint covers the tiled ColorCache solver on box bodies");
+    if (c.flags & ME_FLAG_FAST_MATH)
+        return fail(ME_EARG, "the adjoint differentiates the exact arithmetic: an engine without ME_FLAG_FAST_MATH");
     if (!tape || ((uintptr_t)tape & 255) != 0) return fail(ME_EARG, "tape must be a 256-byte aligned device block");
     size_t need = tape_layout(e, nullptr, nullptr);
     if (bytes < need) return fail(ME_EARG, "tape too small: need " + std::to_string(need) + " bytes");
@@ -762,7 +767,8 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.prof = e;
     a.giant_grid = e->giant_grid;
     a.big_grid = e->big_grid;
-    a.small_grid = e->small_grid;
+    a.fast = (e->cfg.flags & ME_FLAG_FAST_MATH) != 0;
+    a.small_grid = a.fast ? e->small_grid_fast : e->small_grid;
     a.giant_wait_ticks = e->giant_wait_ticks;
     a.taped = e->taping ? 1 : 0;
     a.tape = e->tape;
@@ -803,6 +809,12 @@ int giant_setup(me_engine *e) {
                                                          T_BIG_WG, 0));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[2], (const void *)k_t_color_giant, T_BIG_WG, 0));
     int sweep = occ[0] < occ[1] ? occ[0] : occ[1];
+    {
+        int of = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&of, (const void *)k_t_giant<ME_SOLVER_COLOR_CACHE, true>,
+                                                             T_BIG_WG, 0));
+        if (of < sweep) sweep = of;
+    }
     e->giant_grid = giant_grid_size(T_GIANT_GRID, sweep, occ[2], ncu);
     // the big-tile kernels take tiles from a queue: as many workgroups as can be resident, at most
     // T_BIG_GRID_MAX (more would only find the queue empty)
@@ -813,6 +825,12 @@ int giant_setup(me_engine *e) {
                                                          T_BIGT_WG, T_BIG_DYN));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ob[2], (const void *)k_t_color<1024>, 1024, T_COL_DYN));
     int obig = ob[0] < ob[1] ? ob[0] : ob[1];
+    {
+        int of = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&of, (const void *)k_t_big<ME_SOLVER_COLOR_CACHE, true>,
+                                                             T_BIGT_WG, T_BIG_DYN));
+        if (of < obig) obig = of;
+    }
     if (ob[2] < obig) obig = ob[2];
     e->big_grid = giant_grid_size(T_BIG_GRID_MAX, obig, obig, ncu);
     if (e->big_grid < 1) e->big_grid = 1;  // a workgroup that fits nowhere fails at launch with an error
@@ -825,6 +843,9 @@ int giant_setup(me_engine *e) {
     int osm = os[0];
     for (int k = 1; k < 4; ++k) osm = os[k] < osm ? os[k] : osm;
     e->small_grid = (osm > 0 ? osm : 1) * (ncu > 0 ? ncu : 1);
+    int osf = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&osf, (const void *)k_t_small<true, false, true>, T_SMALL_P, 0));
+    e->small_grid_fast = (osf > 0 ? osf : 1) * (ncu > 0 ? ncu : 1);
     e->giant_wait_ticks = 100LL * khz;
     bool may_launch = (e->cfg.flags & ME_FLAG_TILED) && e->cfg.solver != ME_SOLVER_JACOBI &&
                       (e->N > T_BIG_B || e->capP > T_BIG_P || e->N > T_COLB);

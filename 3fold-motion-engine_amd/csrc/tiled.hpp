@@ -26,6 +26,9 @@ namespace me {
 #define T_COLB 8192   // bodies of a tile whose JP claims sit in LDS
 #define T_COLP 16384  // pairs of a tile whose colours sit in LDS
 #define T_COL_DYN (8 * T_COLB + 4 * T_COLP)
+#define T_PREPF 40               // floats of a contact's fast-arithmetic constants (10 float4)
+#define T_REC_HDR4 10            // float4 of a big-tile pair record's header
+#define T_REC4 (T_REC_HDR4 + 20)   // float4 of a big-tile pair record: header + 4 x 5 (k_t_prep)
 
 struct TiledBuffers {
     int *parent, *root, *proot;
@@ -39,10 +42,12 @@ struct TiledBuffers {
     int *psorted2;               // big tiles: pairs counting-sorted by colour
     int *rec;                    // big tiles: 8 ints per colour-sorted pair (p, a, b, la, lb, st, cnt, flags)
     float *prep;                 // 20 floats per contact: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu
-    float *sprep;                // big tiles: T_REC4 float4 fields per colour-sorted pair, field-major (read only
-                                 // in the sweeps: body header, then 4 x 5 fields of prep; tiled_big.hpp)
+    float *prepf;                // fast arithmetic: T_PREPF floats per contact (k_t_prep<true, true>)
+    float *sprep;                // big tiles: T_REC4 float4 fields per colour-sorted pair, field-major in blocks
+                                 // of 8 pairs (read only in the sweeps: body header, then 4 x 5 fields of
+                                 // prep; tiled_big.hpp)
     float *simp;                 // big tiles: the impulses (jn, jt1, jt2, jp) of a colour-sorted pair's
-                                 // contacts, one float4 per contact slot, slot-major
+                                 // contacts, one float4 per contact slot, slot-major in blocks of 8 pairs
     int *srec;                   // big tiles: 4 ints per colour-sorted pair (p, a, b, first contact)
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
     long long *bigw;             // tile work since the last reset, 4 counts (tiles, pairs, dynamic pair ends,
@@ -97,6 +102,7 @@ struct TiledArgs {
     int giant_grid;                  // workgroups of the giant-island grid kernels, all resident together
     int big_grid;                    // workgroups of the big-tile kernels (one per CU the LDS allows)
     int small_grid;                  // workgroups of the persistent small-tile kernel (all resident ones)
+    int fast;                        // fast arithmetic (ME_FLAG_FAST_MATH): the *_f contact updates
     long long giant_wait_ticks;      // bound of a grid barrier's wait, in wall-clock counter ticks
     AdjTape tape;
 };
@@ -126,7 +132,7 @@ __device__ __forceinline__ int tile_class(const TiledBuffers &T, int c, int ncol
 
 // tile arrays rebuilt within every substep
 template <class BumpT>
-inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int capC) {
+inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int capC, bool fast) {
     size_t n = (size_t)N, p = (size_t)capP;
     t.parent = B.template take<int>(n);
     t.root = B.template take<int>(n);
@@ -144,8 +150,10 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.invIws = B.template take<float>(12 * n);
     t.psorted2 = B.template take<int>(p);
     t.rec = B.template take<int>(8 * p);
-    t.sprep = B.template take<float>(120 * p);
-    t.simp = B.template take<float>(16 * p);
+    t.prepf = B.template take<float>(fast ? T_PREPF * (size_t)capC : 1);
+    const size_t p8 = (p + 7) & ~size_t(7);  // whole blocks of 8 pairs (tiled_big.hpp: T_RB)
+    t.sprep = B.template take<float>(4 * T_REC4 * p8);
+    t.simp = B.template take<float>(16 * p8);
     t.srec = B.template take<int>(4 * p);
     t.dbg = B.template take<long long>(16);
     t.bigw = B.template take<long long>(12);
@@ -716,11 +724,65 @@ __global__ void k_t_apply_split(int capP, const int *istat, int *col, TiledBuffe
 
 // per-contact constants of one substep's solve (identical to recomputing them every sweep: their
 // inputs -- normal, contact points, centres, world inverse inertia -- do not change in the solve)
+// A contact's constants for the substep and its impulses. Under fast arithmetic meff, meft1 and meft2
+// hold the reciprocals of the effective masses (zero where the exact solve skips the update) and
+// kA, kB, k1A, k2A, k1B, k2B the bodies' angular responses to unit impulses along n, t1, t2.
 struct CCon {
     f3 n, rA, rB, t1, t2;
     float meff, meft1, meft2, bias, mu;
     float jn, jt1, jt2, jp;
+    f3 kA, kB, k1A, k2A, k1B, k2B;
 };
+
+// Per-contact constants of the substep under fast arithmetic (ColorCache's fixed friction frame), 10
+// float4 per contact in prepf, grouped so that the position pass reads the first five and the warm
+// apply the first and the last seven:
+//   (n, 1/meff) (rA, bias) (rB, mu) (kA, 1/meft1) (kB, 1/meft2) (t1, k1A.x) (t2, k1A.y)
+//   (k1A.z, k2A) (k1B, k2B.x) (k2B.y, k2B.z, 0, 0)
+// with kA = IA (rA x n), k1A = IA (rA x t1), k2A = IA (rA x t2) and the same for B (zero without B).
+// The reciprocals are zero where the exact sweep skips the update (meff < 1e-12; meft1 or meft2 <
+// 1e-12), and the effective masses are the exact sweep's own expressions.
+__device__ __forceinline__ void prepf_write(float4 *o, f3 n, f3 rA, f3 rB, f3 t1, f3 t2, float meff, float m1,
+                                            float m2, float bias, float mu, f3 kA, f3 kB, f3 k1A, f3 k2A, f3 k1B,
+                                            f3 k2B) {
+    const float rm = meff >= 1e-12f ? 1.0f / meff : 0.0f;
+    const bool tok = m1 >= 1e-12f && m2 >= 1e-12f;
+    const float r1 = tok ? 1.0f / m1 : 0.0f, r2 = tok ? 1.0f / m2 : 0.0f;
+    o[0] = make_float4(n.x, n.y, n.z, rm);
+    o[1] = make_float4(rA.x, rA.y, rA.z, bias);
+    o[2] = make_float4(rB.x, rB.y, rB.z, mu);
+    o[3] = make_float4(kA.x, kA.y, kA.z, r1);
+    o[4] = make_float4(kB.x, kB.y, kB.z, r2);
+    o[5] = make_float4(t1.x, t1.y, t1.z, k1A.x);
+    o[6] = make_float4(t2.x, t2.y, t2.z, k1A.y);
+    o[7] = make_float4(k1A.z, k2A.x, k2A.y, k2A.z);
+    o[8] = make_float4(k1B.x, k1B.y, k1B.z, k2B.x);
+    o[9] = make_float4(k2B.y, k2B.z, 0.0f, 0.0f);
+}
+
+// fields f0 .. f9 of prepf_write into a contact (impulses untouched)
+__device__ __forceinline__ void prepf_unpack(CCon &C, float4 f0, float4 f1, float4 f2, float4 f3, float4 f4,
+                                             float4 f5, float4 f6, float4 f7, float4 f8, float4 f9) {
+    C.n = mk(f0.x, f0.y, f0.z); C.meff = f0.w;
+    C.rA = mk(f1.x, f1.y, f1.z); C.bias = f1.w;
+    C.rB = mk(f2.x, f2.y, f2.z); C.mu = f2.w;
+    C.kA = mk(f3.x, f3.y, f3.z); C.meft1 = f3.w;
+    C.kB = mk(f4.x, f4.y, f4.z); C.meft2 = f4.w;
+    C.t1 = mk(f5.x, f5.y, f5.z); C.t2 = mk(f6.x, f6.y, f6.z);
+    C.k1A = mk(f5.w, f6.w, f7.x); C.k2A = mk(f7.y, f7.z, f7.w);
+    C.k1B = mk(f8.x, f8.y, f8.z); C.k2B = mk(f8.w, f9.x, f9.y);
+}
+
+// contact c of prepf, the fields pass PASS reads (0 warm: f0, f3..f9; 1 velocity: all; 2 position: f0..f4)
+template <int PASS>
+__device__ __forceinline__ void ld_prepf(const float4 *o, CCon &C) {
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 f0 = o[0], f3 = o[3], f4 = o[4];
+    const float4 f1 = PASS != 0 ? o[1] : z, f2 = PASS != 0 ? o[2] : z;
+    float4 f5 = z, f6 = z, f7 = z, f8 = z, f9 = z;
+    if (PASS != 2) { f5 = o[5]; f6 = o[6]; f7 = o[7]; f8 = o[8]; f9 = o[9]; }
+    prepf_unpack(C, f0, f1, f2, f3, f4, f5, f6, f7, f8, f9);
+}
 
 struct PairCache {
     int cnt, la, lb;
@@ -731,7 +793,7 @@ struct PairCache {
     CCon c[4];
 };
 
-template <bool FIXED>
+template <bool FIXED, bool FAST>
 __device__ __forceinline__ void pc_load(PairCache &P, const TiledArgs &A, const TiledBuffers &T, int p, int b0) {
     int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
     const float *invM = A.b.invM;
@@ -752,6 +814,17 @@ __device__ __forceinline__ void pc_load(PairCache &P, const TiledArgs &A, const 
     P.pvBs = (P.hasB && !P.dynB) ? ld3(A.pv, b) : z; P.poBs = (P.hasB && !P.dynB) ? ld3(A.po, b) : z;
     int st = A.Pp.pstart[p];
     P.cnt = A.Pp.pcount[p];
+    if (FAST) {  // the substep's constants from k_t_prep (the inertias are in the responses)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k >= P.cnt) break;
+            int c = st + k;
+            CCon &C = P.c[k];
+            ld_prepf<1>(reinterpret_cast<const float4 *>(T.prepf) + 10 * (size_t)c, C);
+            C.jn = A.Cc.jn[c]; C.jt1 = A.Cc.jt1[c]; C.jt2 = A.Cc.jt2[c]; C.jp = A.Cc.jp[c];
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= P.cnt) break;
@@ -896,7 +969,142 @@ __device__ __forceinline__ void cc_pos(CCon &C, f3 &pvA, f3 &poA, f3 &pvB, f3 &p
     if (hasB && iMB > 0.0f) { pvB = pvB - Jn * iMB; poB = poB - mv(IB, cross(rB, Jn)); }
 }
 
+// ── fast arithmetic (ME_FLAG_FAST_MATH; ColorCache's fixed friction frame) ──
+// The contact updates of cc_warm, cc_vel<true> and cc_pos in a shorter instruction chain: the bodies'
+// angular responses to unit impulses (kA = IA (rA x n), ...) and the reciprocals of the effective
+// masses come from the substep's constants (prepf_write), products are fused into FMAs, and the
+// friction cone is scaled through the hardware's reciprocal square root. Equal to the exact updates
+// in real arithmetic; the rounding differs, and the order of the updates (the Gauss-Seidel order) does
+// not, so the result is the same from run to run.
+
+// v + w x r
+__device__ __forceinline__ f3 f_vel(f3 v, f3 w, f3 r) {
+    return mk(fmaf(w.y, r.z, fmaf(-w.z, r.y, v.x)), fmaf(w.z, r.x, fmaf(-w.x, r.z, v.y)),
+              fmaf(w.x, r.y, fmaf(-w.y, r.x, v.z)));
+}
+__device__ __forceinline__ float f_dot(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+
+// warm apply: the impulse jn n + jt1 t1 + jt2 t2 in the sweep's frame
+__device__ __forceinline__ void cc_warm_f(const CCon &C, f3 &vA, f3 &wA, f3 &vB, f3 &wB, bool hasB, float iMA,
+                                          float iMB) {
+    if (iMA > 0.0f) {
+        vA = cmadd(C.jn * iMA, C.n, cmadd(C.jt1 * iMA, C.t1, cmadd(C.jt2 * iMA, C.t2, vA)));
+        wA = cmadd(C.jn, C.kA, cmadd(C.jt1, C.k1A, cmadd(C.jt2, C.k2A, wA)));
+    }
+    if (hasB && iMB > 0.0f) {
+        vB = cmadd(-C.jn * iMB, C.n, cmadd(-C.jt1 * iMB, C.t1, cmadd(-C.jt2 * iMB, C.t2, vB)));
+        wB = cmadd(-C.jn, C.kB, cmadd(-C.jt1, C.k1B, cmadd(-C.jt2, C.k2B, wB)));
+    }
+}
+
+// velocity pass: normal impulse, then the friction pair clamped to the cone mu * jn
+__device__ __forceinline__ void cc_vel_f(CCon &C, f3 &vA, f3 &wA, f3 &vB, f3 &wB, bool hasB, float iMA,
+                                         float iMB) {
+    if (!(C.meff > 0.0f)) return;
+    f3 vrel = f_vel(vA, wA, C.rA);
+    if (hasB) vrel = vrel - f_vel(vB, wB, C.rB);
+    const float jo = C.jn;
+    const float nw = wmax(0.0f, fmaf(-f_dot(vrel, C.n), C.meff, jo));
+    float dj = nw - jo;
+    C.jn = nw;
+    if (iMA > 0.0f) { vA = cmadd(dj * iMA, C.n, vA); wA = cmadd(dj, C.kA, wA); }
+    if (hasB && iMB > 0.0f) { vB = cmadd(-dj * iMB, C.n, vB); wB = cmadd(-dj, C.kB, wB); }
+    if (!(C.meft1 > 0.0f)) return;
+    vrel = f_vel(vA, wA, C.rA);
+    if (hasB) vrel = vrel - f_vel(vB, wB, C.rB);
+    const float o1 = C.jt1, o2 = C.jt2;
+    float a1 = fmaf(-f_dot(vrel, C.t1), C.meft1, o1);
+    float a2 = fmaf(-f_dot(vrel, C.t2), C.meft2, o2);
+    const float s2 = fmaf(a1, a1, a2 * a2), lim = C.mu * nw;
+    if (s2 > lim * lim && s2 > 1e-24f) {
+        const float f = lim * __builtin_amdgcn_rsqf(s2);
+        a1 = a1 * f;
+        a2 = a2 * f;
+    }
+    C.jt1 = a1;
+    C.jt2 = a2;
+    const float d1 = a1 - o1, d2 = a2 - o2;
+    if (iMA > 0.0f) {
+        vA = cmadd(d1 * iMA, C.t1, cmadd(d2 * iMA, C.t2, vA));
+        wA = cmadd(d1, C.k1A, cmadd(d2, C.k2A, wA));
+    }
+    if (hasB && iMB > 0.0f) {
+        vB = cmadd(-d1 * iMB, C.t1, cmadd(-d2 * iMB, C.t2, vB));
+        wB = cmadd(-d1, C.k1B, cmadd(-d2, C.k2B, wB));
+    }
+}
+
+// position pass
+__device__ __forceinline__ void cc_pos_f(CCon &C, f3 &pvA, f3 &poA, f3 &pvB, f3 &poB, bool hasB, float iMA,
+                                         float iMB) {
+    if (!(C.meff > 0.0f)) return;
+    f3 rel = f_vel(pvA, poA, C.rA);
+    if (hasB) rel = rel - f_vel(pvB, poB, C.rB);
+    const float jo = C.jp;
+    const float nw = wmax(0.0f, fmaf(C.bias - f_dot(rel, C.n), C.meff, jo));
+    const float dj = nw - jo;
+    C.jp = nw;
+    if (iMA > 0.0f) { pvA = cmadd(dj * iMA, C.n, pvA); poA = cmadd(dj, C.kA, poA); }
+    if (hasB && iMB > 0.0f) { pvB = cmadd(-dj * iMB, C.n, pvB); poB = cmadd(-dj, C.kB, poB); }
+}
+
+// The same fast updates with the angular responses formed from the bodies' inverse inertias at each
+// use (for the big and giant tiles, whose records stream from memory every colour step: four more
+// float4 per contact would not stay in the L2). C.meff, meft1, meft2 hold the reciprocals.
+__device__ __forceinline__ void cc_warm_c(const CCon &C, f3 &vA, f3 &wA, f3 &vB, f3 &wB, bool hasB, float iMA,
+                                          float iMB, const m3 &IA, const m3 &IB) {
+    // the warm apply's own frame, as cc_warm (the records carry the sweep's frame to the velocity pass only)
+    const f3 t1 = cfixed_t1(C.n), t2 = ccross(C.n, t1);
+    const f3 imp = cmadd(C.jn, C.n, cmadd(C.jt1, t1, C.jt2 * t2));
+    if (iMA > 0.0f) { vA = cmadd(iMA, imp, vA); wA = wA + cmv(IA, ccross(C.rA, imp)); }
+    if (hasB && iMB > 0.0f) { vB = cmadd(-iMB, imp, vB); wB = wB - cmv(IB, ccross(C.rB, imp)); }
+}
+
+__device__ __forceinline__ void cc_vel_c(CCon &C, f3 &vA, f3 &wA, f3 &vB, f3 &wB, bool hasB, float iMA,
+                                         float iMB, const m3 &IA, const m3 &IB) {
+    if (!(C.meff > 0.0f)) return;
+    f3 vrel = f_vel(vA, wA, C.rA);
+    if (hasB) vrel = vrel - f_vel(vB, wB, C.rB);
+    const float jo = C.jn;
+    const float nw = wmax(0.0f, fmaf(-f_dot(vrel, C.n), C.meff, jo));
+    const float dj = nw - jo;
+    C.jn = nw;
+    if (iMA > 0.0f) { vA = cmadd(dj * iMA, C.n, vA); wA = cmadd(dj, cmv(IA, ccross(C.rA, C.n)), wA); }
+    if (hasB && iMB > 0.0f) { vB = cmadd(-dj * iMB, C.n, vB); wB = cmadd(-dj, cmv(IB, ccross(C.rB, C.n)), wB); }
+    if (!(C.meft1 > 0.0f)) return;
+    vrel = f_vel(vA, wA, C.rA);
+    if (hasB) vrel = vrel - f_vel(vB, wB, C.rB);
+    const float o1 = C.jt1, o2 = C.jt2;
+    float a1 = fmaf(-f_dot(vrel, C.t1), C.meft1, o1);
+    float a2 = fmaf(-f_dot(vrel, C.t2), C.meft2, o2);
+    const float s2 = fmaf(a1, a1, a2 * a2), lim = C.mu * nw;
+    if (s2 > lim * lim && s2 > 1e-24f) {
+        const float f = lim * __builtin_amdgcn_rsqf(s2);
+        a1 = a1 * f;
+        a2 = a2 * f;
+    }
+    C.jt1 = a1;
+    C.jt2 = a2;
+    const f3 Jt = cmadd(a1 - o1, C.t1, (a2 - o2) * C.t2);
+    if (iMA > 0.0f) { vA = cmadd(iMA, Jt, vA); wA = wA + cmv(IA, ccross(C.rA, Jt)); }
+    if (hasB && iMB > 0.0f) { vB = cmadd(-iMB, Jt, vB); wB = wB - cmv(IB, ccross(C.rB, Jt)); }
+}
+
+__device__ __forceinline__ void cc_pos_c(CCon &C, f3 &pvA, f3 &poA, f3 &pvB, f3 &poB, bool hasB, float iMA,
+                                         float iMB, const m3 &IA, const m3 &IB) {
+    if (!(C.meff > 0.0f)) return;
+    f3 rel = f_vel(pvA, poA, C.rA);
+    if (hasB) rel = rel - f_vel(pvB, poB, C.rB);
+    const float jo = C.jp;
+    const float nw = wmax(0.0f, fmaf(C.bias - f_dot(rel, C.n), C.meff, jo));
+    const float dj = nw - jo;
+    C.jp = nw;
+    if (iMA > 0.0f) { pvA = cmadd(dj * iMA, C.n, pvA); poA = cmadd(dj, cmv(IA, ccross(C.rA, C.n)), poA); }
+    if (hasB && iMB > 0.0f) { pvB = cmadd(-dj * iMB, C.n, pvB); poB = cmadd(-dj, cmv(IB, ccross(C.rB, C.n)), poB); }
+}
+
 // the cached pair: bodies from LDS (dynamic) or the cached static state
+template <bool FAST>
 __device__ __forceinline__ void pc_warm(const PairCache &P, float *sv, float *sw) {
     f3 va = P.dynA ? ld3(sv, P.la) : P.vAs, wa = P.dynA ? ld3(sw, P.la) : P.wAs;
     f3 vb = mk(0.0f, 0.0f, 0.0f), wb = mk(0.0f, 0.0f, 0.0f);
@@ -904,13 +1112,14 @@ __device__ __forceinline__ void pc_warm(const PairCache &P, float *sv, float *sw
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= P.cnt) break;
-        cc_warm(P.c[k], va, wa, vb, wb, P.hasB, P.iMA, P.iMB, P.IA, P.IB);
+        if (FAST) cc_warm_f(P.c[k], va, wa, vb, wb, P.hasB, P.iMA, P.iMB);
+        else cc_warm(P.c[k], va, wa, vb, wb, P.hasB, P.iMA, P.iMB, P.IA, P.IB);
     }
     if (P.iMA > 0.0f) { st3(sv, P.la, va); st3(sw, P.la, wa); }
     if (P.hasB && P.iMB > 0.0f) { st3(sv, P.lb, vb); st3(sw, P.lb, wb); }
 }
 
-template <bool FIXED>
+template <bool FIXED, bool FAST>
 __device__ __forceinline__ void pc_vel(PairCache &P, float *sv, float *sw, float t2d) {
     f3 vA = P.dynA ? ld3(sv, P.la) : P.vAs, wA = P.dynA ? ld3(sw, P.la) : P.wAs;
     f3 vB = mk(0.0f, 0.0f, 0.0f), wB = mk(0.0f, 0.0f, 0.0f);
@@ -918,12 +1127,14 @@ __device__ __forceinline__ void pc_vel(PairCache &P, float *sv, float *sw, float
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= P.cnt) break;
-        cc_vel<FIXED>(P.c[k], vA, wA, vB, wB, P.hasB, P.iMA, P.iMB, P.IA, P.IB, t2d);
+        if (FAST) cc_vel_f(P.c[k], vA, wA, vB, wB, P.hasB, P.iMA, P.iMB);
+        else cc_vel<FIXED>(P.c[k], vA, wA, vB, wB, P.hasB, P.iMA, P.iMB, P.IA, P.IB, t2d);
     }
     if (P.iMA > 0.0f) { st3(sv, P.la, vA); st3(sw, P.la, wA); }
     if (P.hasB && P.iMB > 0.0f) { st3(sv, P.lb, vB); st3(sw, P.lb, wB); }
 }
 
+template <bool FAST>
 __device__ __forceinline__ void pc_pos(PairCache &P, float *spv, float *spo) {
     f3 pvA = P.dynA ? ld3(spv, P.la) : P.pvAs, poA = P.dynA ? ld3(spo, P.la) : P.poAs;
     f3 pvB = mk(0.0f, 0.0f, 0.0f), poB = mk(0.0f, 0.0f, 0.0f);
@@ -931,7 +1142,8 @@ __device__ __forceinline__ void pc_pos(PairCache &P, float *spv, float *spo) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= P.cnt) break;
-        cc_pos(P.c[k], pvA, poA, pvB, poB, P.hasB, P.iMA, P.iMB, P.IA, P.IB);
+        if (FAST) cc_pos_f(P.c[k], pvA, poA, pvB, poB, P.hasB, P.iMA, P.iMB);
+        else cc_pos(P.c[k], pvA, poA, pvB, poB, P.hasB, P.iMA, P.iMB, P.IA, P.IB);
     }
     if (P.iMA > 0.0f) { st3(spv, P.la, pvA); st3(spo, P.la, poA); }
     if (P.hasB && P.iMB > 0.0f) { st3(spv, P.lb, pvB); st3(spo, P.lb, poB); }
@@ -1001,7 +1213,7 @@ __device__ __forceinline__ void tape_sweep(const TiledArgs &A, const TiledBuffer
 // Small tiles by a persistent grid: each workgroup takes the next tile of the small-tile queue
 // (k_t_bigsort: most pairs first) until the queue is empty, so a batch of uneven scenes finishes close
 // to max(largest tile, total work / workgroups) whatever the number of tiles.
-template <bool FIXED, bool TAPE>
+template <bool FIXED, bool TAPE, bool FAST = false>
 __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers T) {
     __shared__ float sv[3 * T_SMALL_B], sw[3 * T_SMALL_B], spv[3 * T_SMALL_B], spo[3 * T_SMALL_B];
     __shared__ int cnt[ME_NCOLMAX + 1], fill[ME_NCOLMAX + 1], plist[T_SMALL_P], plcol[T_SMALL_P];
@@ -1064,12 +1276,12 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers
         if (tid < np) {
             p = plist[tid];
             c_own = plcol[tid];
-            pc_load<FIXED>(P, A, T, p, b0);
+            pc_load<FIXED, FAST>(P, A, T, p, b0);
         }
         if (np > 0) {
             if (FIXED) {
                 for (int c = 0; c < nc; ++c) {
-                    if (c_own == c) pc_warm(P, sv, sw);
+                    if (c_own == c) pc_warm<FAST>(P, sv, sw);
                     lds_barrier();
                 }
             }
@@ -1077,14 +1289,14 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers
             for (int it = 0; it < A.vit; ++it) {
                 if (TAPE) tape_sweep(A, T, b0, nb, it, sv, sw, P, p);
                 for (int c = 0; c < nc; ++c) {
-                    if (c_own == c) pc_vel<FIXED>(P, sv, sw, t2d);
+                    if (c_own == c) pc_vel<FIXED, FAST>(P, sv, sw, t2d);
                     lds_barrier();
                 }
             }
             for (int it = 0; it < A.pit; ++it) {
                 if (TAPE) tape_sweep(A, T, b0, nb, A.vit + it, spv, spo, P, p);
                 for (int c = 0; c < nc; ++c) {
-                    if (c_own == c) pc_pos(P, spv, spo);
+                    if (c_own == c) pc_pos<FAST>(P, spv, spo);
                     lds_barrier();
                 }
             }
@@ -1110,8 +1322,9 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_small(TiledArgs A, TiledBuffers
     }
 }
 
-// per-contact constants of the substep (k_t_prep): the same expressions pc_load evaluates
-template <bool FIXED>
+// per-contact constants of the substep (k_t_prep): the same expressions pc_load evaluates; under fast
+// arithmetic (FAST, with the fixed frame) the prepf record instead
+template <bool FIXED, bool FAST>
 __global__ void k_t_prep(TiledArgs A, TiledBuffers T) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     int C = A.istat[IS_C];
@@ -1123,6 +1336,32 @@ __global__ void k_t_prep(TiledArgs A, TiledBuffers T) {
     m3 IA = ldm(A.invIw, a), IB = b >= 0 ? ldm(A.invIw, b) : IA;
     f3 n = ld3(A.Cc.n, c), rA = ld3(A.Cc.pA, c) - ld3(A.b.x, a);
     f3 rB = b >= 0 ? ld3(A.Cc.pB, c) - ld3(A.b.x, b) : mk(0.0f, 0.0f, 0.0f);
+    if (FAST) {
+        const f3 z = mk(0.0f, 0.0f, 0.0f);
+        f3 rnA = cross(rA, n), kA = mv(IA, rnA), kB = z;
+        float meff = iMA + dot(rnA, kA);
+        f3 t1 = fixed_t1(n), t2 = cross(n, t1);
+        f3 r1A = cross(rA, t1), r2A = cross(rA, t2);
+        f3 k1A = mv(IA, r1A), k2A = mv(IA, r2A), k1B = z, k2B = z;
+        float m1 = iMA + dot(r1A, k1A), m2 = iMA + dot(r2A, k2A);
+        if (b >= 0) {
+            f3 rnB = cross(rB, n), r1B = cross(rB, t1), r2B = cross(rB, t2);
+            kB = mv(IB, rnB); k1B = mv(IB, r1B); k2B = mv(IB, r2B);
+            meff = meff + iMB + dot(rnB, kB);
+            m1 = m1 + iMB + dot(r1B, k1B);
+            m2 = m2 + iMB + dot(r2B, k2B);
+        }
+        const float bias = 0.2f * wmax(A.Cc.pen[c] - 1e-4f, 0.0f) / A.dt;
+        float4 *of = reinterpret_cast<float4 *>(T.prepf) + 10 * (size_t)c;
+        prepf_write(of, n, rA, rB, t1, t2, meff, m1, m2, bias, A.Cc.mu[c], kA, kB, k1A, k2A, k1B, k2B);
+        // the big and giant tiles' records: prep's layout with the reciprocals of prepf
+        const float4 r0 = of[0], r3 = of[3], r4 = of[4];
+        float *o = T.prep + 20 * (size_t)c;
+        o[0] = n.x; o[1] = n.y; o[2] = n.z; o[3] = rA.x; o[4] = rA.y; o[5] = rA.z; o[6] = rB.x; o[7] = rB.y;
+        o[8] = rB.z; o[9] = t1.x; o[10] = t1.y; o[11] = t1.z; o[12] = t2.x; o[13] = t2.y; o[14] = t2.z;
+        o[15] = r0.w; o[16] = r3.w; o[17] = r4.w; o[18] = bias; o[19] = A.Cc.mu[c];
+        return;
+    }
     f3 rnA = cross(rA, n);
     float meff = iMA + dot(rnA, mv(IA, rnA));
     if (b >= 0) {
@@ -1283,6 +1522,8 @@ inline int tiled_set_attributes() {
                             (int)T_BIG_DYN) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_t_big<ME_SOLVER_COLOR_CACHE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)T_BIG_DYN) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_t_big<ME_SOLVER_COLOR_CACHE, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)T_BIG_DYN) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_t_color<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)T_COL_DYN) != hipSuccess ||
         hipFuncSetAttribute((const void *)k_t_islands_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1298,31 +1539,36 @@ inline int prof_rec(const TiledArgs &A, int id, hipStream_t s) {
     return ev && hipEventRecord(ev, s) != hipSuccess;
 }
 
-// small tiles on stream s, big tiles on the side stream s2 (disjoint islands), joined on s
-template <int SOLVER>
+// small tiles on stream s, big tiles on the side stream s2 (disjoint islands), joined on s; FAST: fast
+// arithmetic (ColorCache only, never recorded)
+template <int SOLVER, bool FAST = false>
 inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStream_t s) {
+    static_assert(!FAST || SOLVER == ME_SOLVER_COLOR_CACHE, "fast arithmetic takes the fixed friction frame");
     hipStream_t s2 = (hipStream_t)A.side;
     hipEvent_t fork = (hipEvent_t)A.ev_fork, join = (hipEvent_t)A.ev_join;
     const size_t dyn = T_BIG_DYN;
-    hipLaunchKernelGGL(k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE>, dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s, A, T);
+    if (FAST && A.taped) return ME_ERUNTIME;
+    hipLaunchKernelGGL((k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE, FAST>), dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s,
+                       A, T);
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
     // the big tiles (the critical path when there are any) are submitted first, so that their
     // workgroups find CUs with free LDS before the small tiles fill them
     if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
-    hipLaunchKernelGGL(k_t_big<SOLVER>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIGT_WG), dyn, s2, A, T);
+    hipLaunchKernelGGL((k_t_big<SOLVER, FAST>), dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIGT_WG), dyn, s2,
+                       A, T);
     if (prof_rec(A, ME_K_TILE_BIG, s2) || prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
     if (A.taped)
         hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, true>), dim3(ntile < A.small_grid ? ntile : A.small_grid),
                            dim3(T_SMALL_P), 0, s, A, T);
     else
-        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false>),
+        hipLaunchKernelGGL((k_t_small<SOLVER == ME_SOLVER_COLOR_CACHE, false, FAST>),
                            dim3(ntile < A.small_grid ? ntile : A.small_grid), dim3(T_SMALL_P), 0, s, A, T);
     if (prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
     // a tile beyond the big tile's LDS (more than T_BIG_B bodies or T_BIG_P pairs) is possible: the grid
     // sweep, after k_t_big, which leaves such tiles to it
     if (A.N > T_BIG_B || A.capP > T_BIG_P) {
         if (prof_rec(A, ME_K_TILE_GIANT, s2)) return ME_ERUNTIME;
-        hipLaunchKernelGGL(k_t_giant<SOLVER>, dim3(A.giant_grid), dim3(T_BIG_WG), 0, s2, A, T);
+        hipLaunchKernelGGL((k_t_giant<SOLVER, FAST>), dim3(A.giant_grid), dim3(T_BIG_WG), 0, s2, A, T);
         if (prof_rec(A, ME_K_TILE_GIANT, s2)) return ME_ERUNTIME;
     }
     if (hipEventRecord(join, s2) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess) return ME_ERUNTIME;
@@ -1396,8 +1642,10 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
 
 inline int tiled_sweeps(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     int ntile = tiled_ntile(A);
-    return A.solver == ME_SOLVER_COLOR_CACHE ? launch_solve<ME_SOLVER_COLOR_CACHE>(A, T, ntile, s)
-                                             : launch_solve<ME_SOLVER_PAIR_CHUNKS>(A, T, ntile, s);
+    if (A.solver == ME_SOLVER_COLOR_CACHE)
+        return A.fast ? launch_solve<ME_SOLVER_COLOR_CACHE, true>(A, T, ntile, s)
+                      : launch_solve<ME_SOLVER_COLOR_CACHE>(A, T, ntile, s);
+    return launch_solve<ME_SOLVER_PAIR_CHUNKS>(A, T, ntile, s);
 }
 
 inline int tiled_finish(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {

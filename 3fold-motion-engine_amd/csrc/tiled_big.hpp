@@ -42,21 +42,27 @@ __device__ __forceinline__ GRec grec_load(const int *r) {
 // floats), inverse world inertia (3 float4 each, row-major with 3 pad), and the velocity and angular
 // velocity of a static (not dynamic) side (3 float4: vA, wA, vB, wB) -- then per contact 5 float4 of
 // k_t_prep's constants, regrouped so that the warm and position passes read only the first three:
-// (n, meff), (rA, bias), (rB, mu), (t1, meft1), (t2, meft2). The records are lane-coalesced: float4
-// field j of sorted pair q sits at sprep[j * capP + q], so the 64 lanes of a wave, which hold
-// consecutive sorted pairs of a colour, read 1 KB of consecutive memory per field (with one record
-// after another, each field load touched 64 cache lines). The record is written once per substep and
+// (n, meff), (rA, bias), (rB, mu), (t1, meft1), (t2, meft2). The records are lane-coalesced in blocks
+// of T_RB consecutive sorted pairs: float4 field j of sorted pair q sits at
+// sprep[(q / T_RB) * T_REC4 * T_RB + j * T_RB + q % T_RB], so the 64 lanes of a wave, which hold
+// consecutive sorted pairs of a colour, read whole 128-byte lines per field (with one record after
+// another, each field load touched 64 cache lines), and every field is an immediate offset (j * 128
+// bytes) from one per-lane address (a field-major layout over all pairs kept 30 64-bit field
+// addresses live, more than the scalar registers hold). The record is written once per substep and
 // only read by the sweeps; the impulses (jn, jt1, jt2, jp), which every colour step rewrites, sit apart
-// in simp, contact k of pair q at simp[k * capP + q]. Everything a colour step reads from global memory
-// sits at addresses known before the pair word is read, so the step issues all of its loads at once.
-#define T_REC_HDR4 10
-#define T_REC4 (T_REC_HDR4 + 20)
+// in simp, contact k of pair q at simp[(q / T_RB) * 4 * T_RB + k * T_RB + q % T_RB]. Everything a colour
+// step reads from global memory sits at addresses known before the pair word is read, so the step
+// issues all of its loads at once.
+#define T_RB 8
+__device__ __forceinline__ size_t rec_off(int q) { return (size_t)(q / T_RB) * (T_REC4 * T_RB) + q % T_RB; }
+__device__ __forceinline__ size_t imp_off(int q) { return (size_t)(q / T_RB) * (4 * T_RB) + q % T_RB; }
 
-// contact k of sorted pair q from the lane-coalesced records (S = capP float4 per field); the
-// friction frame and its effective masses only for the velocity pass
+// contact k of a sorted pair from its record (Rq = sprep + rec_off(q)) and impulses (Iq = simp +
+// imp_off(q)); the friction frame and its effective masses only for the velocity pass
 template <int PASS>
-__device__ __forceinline__ void ld_srec(const float4 *R, const float4 *I, size_t S, int q, int k, CCon &C) {
-    const float4 *c = R + (T_REC_HDR4 + 5 * k) * S + q;
+__device__ __forceinline__ void ld_srec(const float4 *Rq, const float4 *Iq, int k, CCon &C) {
+    constexpr int S = T_RB;
+    const float4 *c = Rq + (T_REC_HDR4 + 5 * k) * S;
     const float4 a = c[0], b = c[S], d = c[2 * S];
     C.n = mk(a.x, a.y, a.z); C.meff = a.w;
     C.rA = mk(b.x, b.y, b.z); C.bias = b.w;
@@ -69,11 +75,12 @@ __device__ __forceinline__ void ld_srec(const float4 *R, const float4 *I, size_t
         C.t1 = C.t2 = mk(0.0f, 0.0f, 0.0f);
         C.meft1 = C.meft2 = 0.0f;
     }
-    const float4 i = I[k * S + q];
+    const float4 i = Iq[k * S];
     C.jn = i.x; C.jt1 = i.y; C.jt2 = i.z; C.jp = i.w;
 }
 
-__device__ __forceinline__ m3 ld_m12s(const float4 *o, size_t S) {
+__device__ __forceinline__ m3 ld_m12s(const float4 *o) {
+    constexpr int S = T_RB;
     const float4 a = o[0], b = o[S], c = o[2 * S];
     m3 M;
     M.m[0][0] = a.x; M.m[0][1] = a.y; M.m[0][2] = a.z; M.m[1][0] = a.w; M.m[1][1] = b.x; M.m[1][2] = b.y;
@@ -84,8 +91,9 @@ __device__ __forceinline__ m3 ld_m12s(const float4 *o, size_t S) {
 // One pair of a big tile for one colour step, contacts streamed one at a time (few live registers).
 // PASS 0 warm apply, 1 velocity, 2 position. LDS mode: body state s0/s1 in LDS by tile slot, the
 // pair word and its colour-ordered records; global mode: record g, body state in global memory by
-// body id, constants from prep, impulses from the contact arrays.
-template <int PASS, bool FIXED, bool LDSM>
+// body id, constants from prep, impulses from the contact arrays. FAST: the fast-arithmetic contact
+// updates (cc_*_c), with the same records (k_t_prep then stores the reciprocal effective masses).
+template <int PASS, bool FIXED, bool LDSM, bool FAST = false>
 __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers &T, unsigned w, GRec g, int q,
                                          int b0, float *s0, float *s1, float t2d) {
     int la, lb, cnt, a = -1, b = -1, st = 0;
@@ -102,17 +110,18 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
     float iMA, iMB = 0.0f;
     m3 IA, IB;
     f3 xA, yA, xB = z, yB = z;
-    const size_t S = (size_t)A.capP;
-    const float4 *R = reinterpret_cast<const float4 *>(T.sprep), *I = reinterpret_cast<const float4 *>(T.simp);
+    constexpr int S = T_RB;
+    const float4 *Rq = LDSM ? reinterpret_cast<const float4 *>(T.sprep) + rec_off(q) : nullptr;
+    float4 *Iq = LDSM ? reinterpret_cast<float4 *>(T.simp) + imp_off(q) : nullptr;
     if (LDSM) {
         // the record header: masses, inertias, a static side's velocities (its pseudo-velocities are
         // zero: they are cleared every substep and a static body takes no impulse)
-        const float4 *H = R + q;
+        const float4 *H = Rq;
         const float4 h0 = H[0];
         iMA = h0.x;
         iMB = h0.y;
-        IA = ld_m12s(H + S, S);
-        IB = ld_m12s(H + 4 * S, S);
+        IA = ld_m12s(H + S);
+        IB = ld_m12s(H + 4 * S);
         float4 h7 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), h8 = h7, h9 = h7;
         if (PASS != 2) { h7 = H[7 * S]; h8 = H[8 * S]; h9 = H[9 * S]; }
         if (dynA) {
@@ -142,12 +151,15 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
     }
     auto solve = [&](int k, CCon &C) {
         if (PASS == 0) {
-            cc_warm(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
+            if (FAST) cc_warm_c(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
+            else cc_warm(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
         } else {
-            if (PASS == 1) cc_vel<FIXED>(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB, t2d);
+            if (PASS == 1 && FAST) cc_vel_c(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
+            else if (PASS == 1) cc_vel<FIXED>(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB, t2d);
+            else if (FAST) cc_pos_c(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
             else cc_pos(C, xA, yA, xB, yB, hasB, iMA, iMB, IA, IB);
             if (LDSM) {
-                reinterpret_cast<float4 *>(T.simp)[k * S + q] = make_float4(C.jn, C.jt1, C.jt2, C.jp);
+                Iq[k * S] = make_float4(C.jn, C.jt1, C.jt2, C.jp);
             } else {
                 int c = st + k;
                 A.Cc.jn[c] = C.jn; A.Cc.jt1[c] = C.jt1; A.Cc.jt2[c] = C.jt2; A.Cc.jp[c] = C.jp;
@@ -158,22 +170,23 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
     // contact at once, in 512-thread workgroups for the registers, measured slower)
     auto load = [&](int k, CCon &C) {
         if (LDSM) {
-            ld_srec<PASS>(R, I, S, q, k, C);
+            ld_srec<PASS>(Rq, Iq, k, C);
         } else {
             int c = st + k;
             ld_prep20(T.prep + 20 * (size_t)c, C);
             C.jn = A.Cc.jn[c]; C.jt1 = A.Cc.jt1[c]; C.jt2 = A.Cc.jt2[c]; C.jp = A.Cc.jp[c];
         }
     };
-    CCon Cnext;
-    load(0, Cnext);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (k >= cnt) break;
-        CCon C = Cnext;
-        if (k + 1 < cnt) load(k + 1, Cnext);
-        solve(k, C);
-    }
+    // four named contacts rather than one rotated buffer: no register copies between them
+    CCon c0, c1, c2, c3;
+    load(0, c0);
+    if (cnt > 1) load(1, c1);
+    solve(0, c0);
+    if (cnt > 2) load(2, c2);
+    if (cnt > 1) solve(1, c1);
+    if (cnt > 3) load(3, c3);
+    if (cnt > 2) solve(2, c2);
+    if (cnt > 3) solve(3, c3);
     if (iMA > 0.0f) { st3(s0, la, xA); st3(s1, la, yA); }
     if (hasB && iMB > 0.0f) { st3(s0, lb, xB); st3(s1, lb, yB); }
 }
@@ -189,7 +202,7 @@ __device__ __forceinline__ void big_pair(const TiledArgs &A, const TiledBuffers 
 // 1 800 wait for the pair's first records, and a position step about 5 000 (900). Prefetching the next
 // pair across colour steps (to keep loads clear of the step's impulse stores, which gfx9 counts in
 // vmcnt) costs more instructions than the latency it hides.
-template <int PASS, bool FIXED>
+template <int PASS, bool FIXED, bool FAST>
 __device__ __forceinline__ void big_sweeps_stream(const TiledArgs &A, const TiledBuffers &T, const int *cst,
                                                   const unsigned *pword, int nc, int reps, int p0, int b0, float *s0,
                                                   float *s1, float t2d) {
@@ -197,14 +210,14 @@ __device__ __forceinline__ void big_sweeps_stream(const TiledArgs &A, const Tile
     for (int it = 0; it < reps; ++it)
         for (int c = 0; c < nc; ++c) {
             for (int k = cst[c] + tid; k < cst[c + 1]; k += T_BIGT_WG)
-                big_pair<PASS, FIXED, true>(A, T, pword[k], GRec{}, p0 + k, b0, s0, s1, t2d);
+                big_pair<PASS, FIXED, true, FAST>(A, T, pword[k], GRec{}, p0 + k, b0, s0, s1, t2d);
             lds_barrier();
         }
 }
 
 // one big tile t by the whole workgroup (LDS: dsm dynamic, the colour tables and scalars static);
 // tiles beyond T_BIG_B bodies or T_BIG_P pairs are left to k_t_giant
-template <int SOLVER>
+template <int SOLVER, bool FAST>
 __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers &T, int t, float *dsm, int *cst,
                                          int *cfl, int *cfe, int &maxc, int &wdyn, int &wcon) {
     constexpr bool FIXED = SOLVER == ME_SOLVER_COLOR_CACHE;
@@ -278,8 +291,8 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         pword[pos] = pw_make(la, lb, cnt, dA, dB, b >= 0);
         int *sr = T.srec + 4 * (size_t)q;
         sr[0] = p; sr[1] = a; sr[2] = b; sr[3] = st;
-        const size_t S = (size_t)A.capP;
-        float4 *hd = reinterpret_cast<float4 *>(T.sprep) + q;
+        constexpr int S = T_RB;
+        float4 *hd = reinterpret_cast<float4 *>(T.sprep) + rec_off(q);
         {
             float sA[12], sB[12];
             const float iA = dA ? T.invMs[b0 + la] : invM[a];
@@ -304,7 +317,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
             hd[8 * S] = make_float4(wA.y, wA.z, vB.x, vB.y);
             hd[9 * S] = make_float4(vB.z, wB.x, wB.y, wB.z);
         }
-        float4 *imp = reinterpret_cast<float4 *>(T.simp) + q;
+        float4 *imp = reinterpret_cast<float4 *>(T.simp) + imp_off(q);
         for (int i = 0; i < cnt; ++i) {
             // prep: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu (k_t_prep)
             const float4 *s4 = reinterpret_cast<const float4 *>(T.prep + 20 * (size_t)(st + i));
@@ -331,8 +344,8 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
     const int nc = maxc;
     const float t2d = FIXED ? 1.0f : A.t2d;
     if (np > 0) {
-        if (FIXED) big_sweeps_stream<0, FIXED>(A, T, cst, pword, nc, 1, p0, b0, lv, lw, t2d);
-        big_sweeps_stream<1, FIXED>(A, T, cst, pword, nc, A.vit, p0, b0, lv, lw, t2d);
+        if (FIXED) big_sweeps_stream<0, FIXED, FAST>(A, T, cst, pword, nc, 1, p0, b0, lv, lw, t2d);
+        big_sweeps_stream<1, FIXED, FAST>(A, T, cst, pword, nc, A.vit, p0, b0, lv, lw, t2d);
     }
     __syncthreads();
     stamp(4);
@@ -348,16 +361,16 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
         st3(lw, j, mk(0.0f, 0.0f, 0.0f));
     }
     __syncthreads();
-    if (np > 0) big_sweeps_stream<2, FIXED>(A, T, cst, pword, nc, A.pit, p0, b0, lv, lw, t2d);
+    if (np > 0) big_sweeps_stream<2, FIXED, FAST>(A, T, cst, pword, nc, A.pit, p0, b0, lv, lw, t2d);
     __syncthreads();
     stamp(5);
     for (int k = tid; k < np; k += T_BIGT_WG) {  // impulses back to the contact arrays
         int q = p0 + k;
         int st = T.srec[4 * (size_t)q + 3];
         int cnt = (int)((pword[k] >> 26) & 3) + 1;
-        const float4 *o = reinterpret_cast<const float4 *>(T.simp) + q;
+        const float4 *o = reinterpret_cast<const float4 *>(T.simp) + imp_off(q);
         for (int i = 0; i < cnt; ++i) {
-            const float4 j = o[i * (size_t)A.capP];
+            const float4 j = o[i * T_RB];
             A.Cc.jn[st + i] = j.x;
             A.Cc.jt1[st + i] = j.y;
             A.Cc.jt2[st + i] = j.z;
@@ -381,7 +394,7 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
 // change its result.
 #define T_BIG_GRID_MAX 256
 
-template <int SOLVER>
+template <int SOLVER, bool FAST = false>
 __global__ void __launch_bounds__(T_BIGT_WG) k_t_big(TiledArgs A, TiledBuffers T) {
     extern __shared__ float dsm[];
     __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
@@ -393,7 +406,7 @@ __global__ void __launch_bounds__(T_BIGT_WG) k_t_big(TiledArgs A, TiledBuffers T
         const int i = qi;
         __syncthreads();
         if (i >= nbig) break;
-        big_tile<SOLVER>(A, T, T.bigq[i], dsm, cst, cfl, cfe, maxc, wdyn, wcon);
+        big_tile<SOLVER, FAST>(A, T, T.bigq[i], dsm, cst, cfl, cfe, maxc, wdyn, wcon);
     }
 }
 
@@ -523,7 +536,7 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_color_giant(TiledArgs A, TiledBu
     }
 }
 
-template <int SOLVER>
+template <int SOLVER, bool FAST = false>
 __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers T) {
     __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
     __shared__ int maxc, wdyn, wcon, stop;
@@ -608,9 +621,9 @@ __global__ void __launch_bounds__(T_BIG_WG) k_t_giant(TiledArgs A, TiledBuffers 
                 for (int k = r0 + tid; k < r1; k += T_BIG_WG) {
                     const int q = p0 + k;
                     const GRec g = grec_load(T.rec + 8 * (size_t)q);
-                    if (pass == 0) big_pair<0, FIXED, false>(A, T, 0u, g, q, b0, A.b.v, A.b.w, t2d);
-                    else if (pass == 1) big_pair<1, FIXED, false>(A, T, 0u, g, q, b0, A.b.v, A.b.w, t2d);
-                    else big_pair<2, FIXED, false>(A, T, 0u, g, q, b0, A.pv, A.po, t2d);
+                    if (pass == 0) big_pair<0, FIXED, false, FAST>(A, T, 0u, g, q, b0, A.b.v, A.b.w, t2d);
+                    else if (pass == 1) big_pair<1, FIXED, false, FAST>(A, T, 0u, g, q, b0, A.b.v, A.b.w, t2d);
+                    else big_pair<2, FIXED, false, FAST>(A, T, 0u, g, q, b0, A.pv, A.po, t2d);
                 }
                 target += G;
                 if (!giant_barrier(T.gbar, target, A.istat, stop, A.giant_wait_ticks)) return false;

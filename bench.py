@@ -76,6 +76,9 @@ def main():
                          "autotuner's tile search picks for it; 256 for the adjoint's 64-box piles, four per tile, "
                          "so that the reverse tile kernel's 256 tiles run in one round)")
     ap.add_argument("--graph", type=int, default=0, help="replay each step as one captured hipGraph (tiled path)")
+    ap.add_argument("--fast-math", type=int, default=1,
+                    help="pile workloads under ColorCache: also time the fast-arithmetic engine from the exact "
+                         "engine's state at the run's end (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
@@ -315,6 +318,9 @@ def bench_pile(a, large=False):
     start = e.refresh_counters()
     print(f"[bench] at timed start (step {settle + a.warmup}): {start}", file=sys.stderr, flush=True)
     x0, q0, v0, w0 = e.state_numpy()
+    fast = None
+    if a.fast_math and a.solver == "color_cache" and (a.tiled or large):
+        fast = fast_math_engines(e, bodies, a)  # copies of the state at the window's start
     # kernel timing needs per-launch events, so steps taken while profiling run eagerly; with --graph
     # the timed region replays graphs and a second window of the same length is profiled eagerly
     if not a.graph:
@@ -387,8 +393,88 @@ def bench_pile(a, large=False):
     if eager_ms is not None:
         out["kernel_timing_window"] = {"mode": "eager, right after the timed graph replays", "steps": a.steps,
                                        "ms_per_step": round(eager_ms, 4)}
+    if fast is not None:
+        out["fast_math"] = fast_math_leg(fast, a, L, S, settle + a.warmup, round(ms_per_step, 4), nb)
     del e
     return out
+
+
+def copy_engine_state(src, dst):
+    """dst (built, same configuration but the arithmetic flag) takes src's complete state: the
+    leading workspace bytes me_state_size names (warm-start store, colour cache, status) and the bodies."""
+    import ctypes as C
+    L = importlib.import_module("3fold-motion-engine_amd._lib")
+    sb = C.c_size_t()
+    L.check(L.lib().me_state_size(src._h, C.byref(sb)), "me_state_size")
+    n = int(sb.value)
+    dst._ws[dst._ws_off:dst._ws_off + n].copy_(src._ws[src._ws_off:src._ws_off + n])
+    for k in ("x", "q", "v", "w"):
+        getattr(dst, k).copy_(getattr(src, k))
+
+
+def fast_math_engines(e, bodies, a):
+    """A fast-arithmetic engine and a second exact one, both holding e's complete state (taken at the
+    start of the timed window)."""
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    kw = dict(vit=a.vit, pit=a.pit, check_every_step=False, n_colors=a.colors, tiled=True,
+              tile_bodies=a.tile_bodies or 128)
+    out = []
+    for fm in (True, False):
+        g = pkg.ColorCacheContactEngine(fast_math=fm, **kw)
+        g.add_boxes(bodies)
+        g._build()
+        copy_engine_state(e, g)
+        out.append(g)
+    return out
+
+
+def fast_math_leg(engines, a, L, S, at_step, exact_ms, nb):
+    """The pile under fast arithmetic (ColorCacheContactEngine(fast_math=True)) from the exact engine's
+    state at the timed window's start (step `at_step`): the fast engine and an exact copy take one step
+    (the fast step's deviation from the exact one: centres in m, orientations, impulses relative in the
+    2-norm and relative to the largest of their kind), then the fast engine steps `steps` more along its
+    own trajectory, timed with kernel events as the exact window was."""
+    import torch
+    f, g = engines
+    dt = 1 / 240
+    L.check(L.lib().me_profile(f._h, 1), "me_profile")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    f.step(dt, S)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t
+    g.step(dt, S)
+    f.refresh_counters()
+    g.refresh_counters()
+    xe, qe, _, _ = g.state_numpy()
+    xf, qf, _, _ = f.state_numpy()
+    ce, cf = g.contacts(), f.contacts()
+    dev = {"x_m": float(np.max(np.abs(xf.astype(np.float64) - xe))),
+           "q": float(np.max(np.abs(qf.astype(np.float64) - qe))), "same_contacts": bool(np.array_equal(ce["key"], cf["key"]))}
+    if dev["same_contacts"] and len(ce["key"]):
+        for k in ("jn", "jt1", "jt2", "jp"):
+            d = cf[k].astype(np.float64) - ce[k]
+            dev[k + "_rel_norm"] = float(np.linalg.norm(d)) / max(float(np.linalg.norm(ce[k])), 1e-30)
+            dev[k + "_max_over_max"] = float(np.max(np.abs(d))) / max(float(np.max(np.abs(ce[k]))), 1e-30)
+    del g
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(a.steps - 1):
+        f.step(dt, S)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t + t_first
+    L.check(L.lib().me_profile(f._h, 0), "me_profile")
+    _, kms = kernel_roofline(f, L, a.solver, a.vit, a.pit)
+    cnt = f.refresh_counters()
+    print(f"[bench] fast-math leg: {1e3 * el / a.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"note": "ColorCacheContactEngine(fast_math=True) from the exact engine's complete state at the "
+                    f"timed window's start (step {at_step}), {a.steps} steps along its own trajectory, kernel "
+                    "events on as in the exact window; deviation: its first step against an exact copy's",
+            "window_steps": [at_step, at_step + a.steps], "ms_per_step": round(1e3 * el / a.steps, 4),
+            "value": round(nb * a.steps / el, 1), "exact_ms_per_step": exact_ms,
+            "kernel_ms": {k: v for k, v in list(kms.items())[:6]},
+            "one_step_deviation_from_exact": dev, "contacts_at_end": cnt["n_contacts"],
+            "islands_at_end": cnt["n_islands"], "max_tile_bodies_at_end": cnt["max_tile_bodies"]}
 
 
 def _extra(fn, a, keys, **over):

@@ -5,6 +5,8 @@ import importlib
 import re
 from pathlib import Path
 
+import pytest
+
 ROOT = Path(__file__).resolve().parents[1]
 
 
@@ -50,3 +52,22 @@ def test_workspace_size_and_argument_errors():
     g_bead = lib_mod.Config(2, 100, 18, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64,
                             lib_mod.FLAG_GRAPH | lib_mod.FLAG_TILED)
     assert L.me_workspace_size(ctypes.byref(g_bead), ctypes.byref(n)) == 0
+
+
+def test_fast_math_flag_needs_the_tiled_color_cache_solve():
+    lib_mod = importlib.import_module("3fold-motion-engine_amd._lib")
+    L = lib_mod.lib()
+    n, m = ctypes.c_size_t(), ctypes.c_size_t()
+    F, T = lib_mod.FLAG_FAST_MATH, lib_mod.FLAG_TILED
+    for solver, flags in ((2, F), (1, F | T), (0, F | T)):
+        c = lib_mod.Config(solver, 100, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64, flags)
+        assert L.me_workspace_size(ctypes.byref(c), ctypes.byref(n)) == lib_mod.ME_EARG
+        assert b"fast arithmetic" in L.me_last_error()
+    exact = lib_mod.Config(2, 1000, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64, T)
+    fast = lib_mod.Config(2, 1000, 0, 16, 10, 0.2, 1.0, 1, 0.05, 0.0, 0, 0, 16, 64, F | T)
+    assert L.me_workspace_size(ctypes.byref(exact), ctypes.byref(n)) == 0
+    assert L.me_workspace_size(ctypes.byref(fast), ctypes.byref(m)) == 0
+    assert m.value > n.value  # the fast constants and records
+    pkg = importlib.import_module("3fold-motion-engine_amd")
+    with pytest.raises(ValueError, match="tiled"):
+        pkg.ColorCacheContactEngine(fast_math=True)
