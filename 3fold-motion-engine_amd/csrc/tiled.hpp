@@ -39,7 +39,6 @@ struct TiledBuffers {
     int *ccls;                   // JP class of every final colour [ME_NCOLMAX], and at [ME_NCOLMAX] the colour
                                  // count it was written for (k_t_plan); kept with the colour cache
     float *invMs, *invIws;       // body constants in island order (index = position in bsorted)
-    int *psorted2;               // big tiles: pairs counting-sorted by colour
     int *rec;                    // big tiles: 8 ints per colour-sorted pair (p, a, b, la, lb, st, cnt, flags)
     float *prep;                 // 20 floats per contact: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu
     float *prepf;                // fast arithmetic: T_PREPF floats per contact (k_t_prep<true, true>)
@@ -49,6 +48,8 @@ struct TiledBuffers {
     float *simp;                 // big tiles: the impulses (jn, jt1, jt2, jp) of a colour-sorted pair's
                                  // contacts, one float4 per contact slot, slot-major in blocks of 8 pairs
     int *srec;                   // big tiles: 4 ints per colour-sorted pair (p, a, b, first contact)
+    unsigned *pwordg;            // big tiles: the 32-bit word of each colour-sorted pair (k_t_bigord)
+    int *pclass;                 // big tiles: the colour class of each colour-sorted pair (k_t_bigord)
     long long *dbg;              // phase timestamps (wall clock) of the largest big tile of >= 1000 bodies
     long long *bigw;             // tile work since the last reset, 4 counts (tiles, pairs, dynamic pair ends,
                                  // contacts) for each of k_t_big [0..3], k_t_small [4..7], k_t_giant [8..11]
@@ -148,13 +149,14 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.cbase = B.template take<int>(ME_NCOLMAX + 2);
     t.invMs = B.template take<float>(n);
     t.invIws = B.template take<float>(12 * n);
-    t.psorted2 = B.template take<int>(p);
     t.rec = B.template take<int>(8 * p);
     t.prepf = B.template take<float>(fast ? T_PREPF * (size_t)capC : 1);
     const size_t p8 = (p + 7) & ~size_t(7);  // whole blocks of 8 pairs (tiled_big.hpp: T_RB)
     t.sprep = B.template take<float>(4 * T_REC4 * p8);
     t.simp = B.template take<float>(16 * p8);
     t.srec = B.template take<int>(4 * p);
+    t.pwordg = B.template take<unsigned>(p);
+    t.pclass = B.template take<int>(p);
     t.dbg = B.template take<long long>(16);
     t.bigw = B.template take<long long>(12);
     t.biglist = B.template take<int>(n);
@@ -1553,6 +1555,10 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
     // the big tiles (the critical path when there are any) are submitted first, so that their
     // workgroups find CUs with free LDS before the small tiles fill them
+    // the big tiles' colour order and records, spread over the GPU beside the small tiles' start
+    hipLaunchKernelGGL(k_t_bigord, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(1024), 0, s2, A, T);
+    hipLaunchKernelGGL(k_t_bigfill, dim3(ntile < A.big_grid ? ntile : A.big_grid, T_FILL_CHUNKS), dim3(T_FILL_WG), 0,
+                       s2, A, T);
     if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
     hipLaunchKernelGGL((k_t_big<SOLVER, FAST>), dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIGT_WG), dyn, s2,
                        A, T);

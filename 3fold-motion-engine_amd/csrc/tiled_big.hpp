@@ -215,82 +215,94 @@ __device__ __forceinline__ void big_sweeps_stream(const TiledArgs &A, const Tile
         }
 }
 
-// one big tile t by the whole workgroup (LDS: dsm dynamic, the colour tables and scalars static);
-// tiles beyond T_BIG_B bodies or T_BIG_P pairs are left to k_t_giant
-template <int SOLVER, bool FAST>
-__device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers &T, int t, float *dsm, int *cst,
-                                         int *cfl, int *cfe, int &maxc, int &wdyn, int &wcon) {
-    constexpr bool FIXED = SOLVER == ME_SOLVER_COLOR_CACHE;
-    if (T.tb0[t] == INT_MAX) return;
-    int b0, nb, p0, np;
+// The colour order and pair records of the big tiles (LDS mode), built before k_t_big by kernels that
+// spread over the GPU, so that the one workgroup that sweeps a large island starts its sweeps at once.
+// k_t_bigord: one workgroup per big tile counting-sorts its pairs by colour class (pairs with two
+// dynamic bodies first within a class, one-sided pairs last, so most waves skip one side's updates as
+// a whole; the order within a class is free, its pairs sharing no dynamic body) and stores each sorted
+// pair's word, class and (p, a, b, first contact). k_t_bigfill: one thread per sorted pair writes its
+// record (lane-coalesced blocks, above) and impulses.
+__device__ __forceinline__ bool lds_big_tile(const TiledBuffers &T, int t, int &b0, int &nb, int &p0, int &np) {
+    if (T.tb0[t] == INT_MAX) return false;
     tile_bounds(T, t, b0, nb, p0, np);
-    if (is_small_tile(nb, np) || !(nb <= T_BIG_B && np <= T_BIG_P)) return;
-    if (A.taped && threadIdx.x == 0) atomicOr(&A.istat[IS_OVF], OVF_TAPE);  // no recording for big tiles
-    const int tid = threadIdx.x;
-    // phase stamps of the largest big tile (>= 1000 bodies) of the launch
-    if (tid == 0 && nb >= 1000 && atomicMax(&T.dbg[7], (long long)nb) <= nb) { T.dbg[0] = wall_clock64(); T.dbg[8] = np; }
-    auto stamp = [&](int k) {
-        if (tid == 0 && nb >= 1000 && (long long)atomicAdd(reinterpret_cast<unsigned long long *>(&T.dbg[7]), 0ull) == nb) T.dbg[k] = wall_clock64();
-    };
-    unsigned *pword = reinterpret_cast<unsigned *>(dsm + 6 * T_BIG_B);
-    if (tid == 0) {
-        atomicAdd(&A.istat[IS_TILES], 1);
-        atomicMax(&A.istat[IS_MAXTB], nb);
-        maxc = 0;
-        wdyn = 0;
-        wcon = 0;
-    }
+    return !is_small_tile(nb, np) && nb <= T_BIG_B && np <= T_BIG_P;
+}
+
+__global__ void __launch_bounds__(1024) k_t_bigord(TiledArgs A, TiledBuffers T) {
+    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
+    __shared__ int maxc, wdyn, wcon;
+    const int tid = threadIdx.x, nbig = A.istat[IS_NBIG];
     int ncol = A.istat[IS_NCOL];
     if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
-    // body state (velocities, then pseudo-velocities) by tile slot; kept as LDS pointers so every
-    // access compiles to ds_* instructions rather than generic (flat) ones
-    float *lv = dsm, *lw = dsm + 3 * T_BIG_B;
-    for (int j = tid; j < nb; j += T_BIGT_WG) {
-        int i = T.bsorted[b0 + j];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            lv[3 * j + k] = A.b.v[3 * i + k];
-            lw[3 * j + k] = A.b.w[3 * i + k];
-        }
-    }
-    for (int c = tid; c <= ncol; c += T_BIGT_WG) { cst[c] = 0; cfl[c] = 0; cfe[c] = 0; }
-    __syncthreads();
-    stamp(1);
-    for (int k = tid; k < np; k += T_BIGT_WG) {
-        int c = A.Pp.col[T.psorted[p0 + k]];
-        if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
-        c = tile_class(T, c, ncol);
-        atomicAdd(&cst[c], 1);
-        atomicMax(&maxc, c + 1);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int c = 0; c < maxc; ++c) { int n = cst[c]; cst[c] = run; run += n; }
-        cst[maxc] = run;
-    }
-    __syncthreads();
-    stamp(2);
     const float *invM = A.b.invM;
-    int mdyn = 0, mcon = 0;
-    for (int k = tid; k < np; k += T_BIGT_WG) {
-        int p = T.psorted[p0 + k];
-        int c = A.Pp.col[p];
-        if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
-        c = tile_class(T, c, ncol);
-        int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
-        bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
-        // within a colour (order-free), pairs with two dynamic bodies first and one-sided pairs
-        // (ground, static body) last, so most waves skip one side's updates as a whole
-        int pos = (dA && dB) ? cst[c] + atomicAdd(&cfl[c], 1) : cst[c + 1] - 1 - atomicAdd(&cfe[c], 1);
-        int q = p0 + pos;
-        int st = A.Pp.pstart[p], cnt = A.Pp.pcount[p];
-        mdyn += (int)dA + (int)dB;
-        mcon += cnt;
-        int la = dA ? T.bslot[a] - b0 : 0, lb = dB ? T.bslot[b] - b0 : 0;
-        pword[pos] = pw_make(la, lb, cnt, dA, dB, b >= 0);
-        int *sr = T.srec + 4 * (size_t)q;
-        sr[0] = p; sr[1] = a; sr[2] = b; sr[3] = st;
+    for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
+        int b0, nb, p0, np;
+        if (!lds_big_tile(T, T.bigq[i], b0, nb, p0, np)) continue;
+        if (tid == 0) { maxc = 0; wdyn = 0; wcon = 0; }
+        for (int c = tid; c <= ncol; c += 1024) { cst[c] = 0; cfl[c] = 0; cfe[c] = 0; }
+        __syncthreads();
+        for (int k = tid; k < np; k += 1024) {
+            int c = A.Pp.col[T.psorted[p0 + k]];
+            if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+            c = tile_class(T, c, ncol);
+            atomicAdd(&cst[c], 1);
+            atomicMax(&maxc, c + 1);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int run = 0;
+            for (int c = 0; c < maxc; ++c) { int n = cst[c]; cst[c] = run; run += n; }
+            cst[maxc] = run;
+        }
+        __syncthreads();
+        int mdyn = 0, mcon = 0;
+        for (int k = tid; k < np; k += 1024) {
+            int p = T.psorted[p0 + k];
+            int c = A.Pp.col[p];
+            if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
+            c = tile_class(T, c, ncol);
+            int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
+            bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
+            int pos = (dA && dB) ? cst[c] + atomicAdd(&cfl[c], 1) : cst[c + 1] - 1 - atomicAdd(&cfe[c], 1);
+            int q = p0 + pos;
+            int st = A.Pp.pstart[p], cnt = A.Pp.pcount[p];
+            mdyn += (int)dA + (int)dB;
+            mcon += cnt;
+            int la = dA ? T.bslot[a] - b0 : 0, lb = dB ? T.bslot[b] - b0 : 0;
+            T.pwordg[q] = pw_make(la, lb, cnt, dA, dB, b >= 0);
+            T.pclass[q] = c;
+            *reinterpret_cast<int4 *>(T.srec + 4 * (size_t)q) = make_int4(p, a, b, st);
+        }
+        if (mdyn) atomicAdd(&wdyn, mdyn);
+        if (mcon) atomicAdd(&wcon, mcon);
+        __syncthreads();
+        if (tid == 0) {  // work of this tile for the algorithmic-byte count of k_t_big
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[0]), 1ull);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[1]), (unsigned long long)np);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[2]), (unsigned long long)wdyn);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[3]), (unsigned long long)wcon);
+        }
+        __syncthreads();  // the next tile reuses the tables
+    }
+}
+
+#define T_FILL_WG 256
+#define T_FILL_CHUNKS (T_BIG_P / T_FILL_WG)
+// grid (x, T_FILL_CHUNKS): workgroup (x, y) fills sorted pairs y * T_FILL_WG .. of big tiles x, x + gridDim.x, ...
+__global__ void __launch_bounds__(T_FILL_WG) k_t_bigfill(TiledArgs A, TiledBuffers T) {
+    const int nbig = A.istat[IS_NBIG];
+    const float *invM = A.b.invM;
+    for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
+        int b0, nb, p0, np;
+        if (!lds_big_tile(T, T.bigq[i], b0, nb, p0, np)) continue;
+        const int k = blockIdx.y * T_FILL_WG + threadIdx.x;
+        if (k >= np) continue;
+        const int q = p0 + k;
+        const int4 sr = *reinterpret_cast<const int4 *>(T.srec + 4 * (size_t)q);
+        const int a = sr.y, b = sr.z, st = sr.w;
+        const unsigned w = T.pwordg[q];
+        const int la = w & 8191, lb = (w >> 13) & 8191, cnt = (int)((w >> 26) & 3) + 1;
+        const bool dA = (w >> 28) & 1, dB = (w >> 29) & 1;
         constexpr int S = T_RB;
         float4 *hd = reinterpret_cast<float4 *>(T.sprep) + rec_off(q);
         {
@@ -318,28 +330,74 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
             hd[9 * S] = make_float4(vB.z, wB.x, wB.y, wB.z);
         }
         float4 *imp = reinterpret_cast<float4 *>(T.simp) + imp_off(q);
-        for (int i = 0; i < cnt; ++i) {
+        for (int c = 0; c < cnt; ++c) {
             // prep: n, rA, rB, t1, t2, meff, meft1, meft2, bias, mu (k_t_prep)
-            const float4 *s4 = reinterpret_cast<const float4 *>(T.prep + 20 * (size_t)(st + i));
+            const float4 *s4 = reinterpret_cast<const float4 *>(T.prep + 20 * (size_t)(st + c));
             const float4 v0 = s4[0], v1 = s4[1], v2 = s4[2], v3 = s4[3], v4 = s4[4];
-            float4 *d = hd + (T_REC_HDR4 + 5 * i) * S;
+            float4 *d = hd + (T_REC_HDR4 + 5 * c) * S;
             d[0] = make_float4(v0.x, v0.y, v0.z, v3.w);      // n, meff
             d[S] = make_float4(v0.w, v1.x, v1.y, v4.z);      // rA, bias
             d[2 * S] = make_float4(v1.z, v1.w, v2.x, v4.w);  // rB, mu
             d[3 * S] = make_float4(v2.y, v2.z, v2.w, v4.x);  // t1, meft1
             d[4 * S] = make_float4(v3.x, v3.y, v3.z, v4.y);  // t2, meft2
-            imp[i * S] = make_float4(A.Cc.jn[st + i], A.Cc.jt1[st + i], A.Cc.jt2[st + i], A.Cc.jp[st + i]);
+            imp[c * S] = make_float4(A.Cc.jn[st + c], A.Cc.jt1[st + c], A.Cc.jt2[st + c], A.Cc.jp[st + c]);
         }
     }
-    if (mdyn) atomicAdd(&wdyn, mdyn);
-    if (mcon) atomicAdd(&wcon, mcon);
-    __syncthreads();
-    if (tid == 0) {  // work of this tile for the algorithmic-byte count of the launch
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[0]), (unsigned long long)(1ll));
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[1]), (unsigned long long)((long long)np));
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[2]), (unsigned long long)((long long)wdyn));
-        atomicAdd(reinterpret_cast<unsigned long long *>(&T.bigw[3]), (unsigned long long)((long long)wcon));
+}
+
+// one big tile t by the whole workgroup (LDS: dsm dynamic, the colour tables and scalars static), its
+// colour order and records from k_t_bigord / k_t_bigfill; tiles beyond T_BIG_B bodies or T_BIG_P
+// pairs are left to k_t_giant
+template <int SOLVER, bool FAST>
+__device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers &T, int t, float *dsm, int *cst,
+                                         int &maxc) {
+    constexpr bool FIXED = SOLVER == ME_SOLVER_COLOR_CACHE;
+    int b0, nb, p0, np;
+    if (!lds_big_tile(T, t, b0, nb, p0, np)) return;
+    if (A.taped && threadIdx.x == 0) atomicOr(&A.istat[IS_OVF], OVF_TAPE);  // no recording for big tiles
+    const int tid = threadIdx.x;
+    // phase stamps of the largest big tile (>= 1000 bodies) of the launch
+    if (tid == 0 && nb >= 1000 && atomicMax(&T.dbg[7], (long long)nb) <= nb) { T.dbg[0] = wall_clock64(); T.dbg[8] = np; }
+    auto stamp = [&](int k) {
+        if (tid == 0 && nb >= 1000 && (long long)atomicAdd(reinterpret_cast<unsigned long long *>(&T.dbg[7]), 0ull) == nb) T.dbg[k] = wall_clock64();
+    };
+    unsigned *pword = reinterpret_cast<unsigned *>(dsm + 6 * T_BIG_B);
+    if (tid == 0) {
+        atomicAdd(&A.istat[IS_TILES], 1);
+        atomicMax(&A.istat[IS_MAXTB], nb);
     }
+    int ncol = A.istat[IS_NCOL];
+    if (ncol > ME_NCOLMAX) ncol = ME_NCOLMAX;
+    // body state (velocities, then pseudo-velocities) by tile slot; kept as LDS pointers so every
+    // access compiles to ds_* instructions rather than generic (flat) ones
+    float *lv = dsm, *lw = dsm + 3 * T_BIG_B;
+    for (int j = tid; j < nb; j += T_BIGT_WG) {
+        int i = T.bsorted[b0 + j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lv[3 * j + k] = A.b.v[3 * i + k];
+            lw[3 * j + k] = A.b.w[3 * i + k];
+        }
+    }
+    for (int c = tid; c <= ncol; c += T_BIGT_WG) cst[c] = -1;
+    __syncthreads();
+    stamp(1);
+    // the pair words, and the class starts from where the sorted classes change
+    for (int k = tid; k < np; k += T_BIGT_WG) {
+        pword[k] = T.pwordg[p0 + k];
+        const int c = T.pclass[p0 + k];
+        if (k == 0 || T.pclass[p0 + k - 1] != c) cst[c] = k;
+        if (k == np - 1) maxc = c + 1;
+    }
+    __syncthreads();
+    if (tid == 0) {  // empty classes start where the next one does
+        if (np == 0) maxc = 0;
+        cst[maxc] = np;
+        for (int c = maxc - 1; c >= 0; --c)
+            if (cst[c] < 0) cst[c] = cst[c + 1];
+    }
+    __syncthreads();
+    stamp(2);
     stamp(3);
     const int nc = maxc;
     const float t2d = FIXED ? 1.0f : A.t2d;
@@ -397,8 +455,8 @@ __device__ __forceinline__ void big_tile(const TiledArgs &A, const TiledBuffers 
 template <int SOLVER, bool FAST = false>
 __global__ void __launch_bounds__(T_BIGT_WG) k_t_big(TiledArgs A, TiledBuffers T) {
     extern __shared__ float dsm[];
-    __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
-    __shared__ int maxc, wdyn, wcon, qi;
+    __shared__ int cst[ME_NCOLMAX + 1];
+    __shared__ int maxc, qi;
     const int nbig = A.istat[IS_NBIG];
     for (;;) {
         if (threadIdx.x == 0) qi = atomicAdd(&A.istat[IS_BIGQ], 1);
@@ -406,7 +464,7 @@ __global__ void __launch_bounds__(T_BIGT_WG) k_t_big(TiledArgs A, TiledBuffers T
         const int i = qi;
         __syncthreads();
         if (i >= nbig) break;
-        big_tile<SOLVER, FAST>(A, T, T.bigq[i], dsm, cst, cfl, cfe, maxc, wdyn, wcon);
+        big_tile<SOLVER, FAST>(A, T, T.bigq[i], dsm, cst, maxc);
     }
 }
 
