@@ -21,6 +21,7 @@ FLAG_TILED, FLAG_PER_CONTACT, FLAG_GRAPH, FLAG_FAST_MATH = 1, 2, 4, 8
 COUNTERS = ("n_contacts", "n_pairs", "n_colors", "cache_hits", "cache_misses", "overflow",
             "n_candidates", "n_islands", "n_tiles", "max_tile_bodies", "graph_captures", "n_solved")
 OVF_CONTACTS, OVF_PAIRS, OVF_CAND, OVF_COLORS, OVF_TILE, OVF_TAPE, OVF_GIANT, OVF_ADJ = 1, 2, 4, 8, 16, 32, 64, 128
+OVF_SCAN = 256
 
 
 class Config(C.Structure):

@@ -20,7 +20,8 @@ enum {
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,
        OVF_TAPE = 32 /* a recorded (differentiated) substep met a tile beyond one small tile */,
        OVF_GIANT = 64 /* a grid barrier of the giant-tile sweep waited past its bound */,
-       OVF_ADJ = 128 /* an adjoint contribution beyond the fixed-point accumulators' range */ };
+       OVF_ADJ = 128 /* an adjoint contribution beyond the fixed-point accumulators' range */,
+       OVF_SCAN = 256 /* a one-pass scan's look-back waited past its bound */ };
 
 #define ME_NCOLMAX 512
 #define ME_MAXCAND 32
@@ -137,6 +138,143 @@ __global__ void __launch_bounds__(ME_THREADS) k_scan_down(const int *in, int *ou
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         if (base + k < n) out[base + k] = run;
         run += vals[k];
+    }
+}
+
+// ── exclusive scan in one pass (decoupled look-back) ──
+// One launch instead of three. Each block takes a ticket from its call site's counter, which only
+// grows: ticket / nb numbers the launch and ticket % nb is the block's place in the scan, so a block
+// only ever waits on blocks that started before it. Every block publishes its sum (AGG) as soon as it
+// has it and its inclusive prefix (INC) once known, in one 64-bit word per block and value: launch
+// number (30 bits) | state (2) | value (32); a word of another launch reads as not yet published, so
+// the words need no clearing between launches. The first wave of a block reads 64 predecessors' words
+// at a time and adds back to the nearest INC. Integer sums in a fixed order of blocks: the result is
+// the three-pass scan's. NV = 2 also scans the flags (in[i] > 0) into out2. The last block writes the
+// totals; with capC > 0 also the clamped contact and pair counts (IS_CRAW, IS_C, IS_P) of a
+// compaction, as k_clamp_counts does. The look-back's wait is bounded by the wall clock (OVF_SCAN).
+#define SCAN_SITES 12  // call sites with their own ticket counter and block words
+#define SC_TILED_B 10   // the island build's two scans of large scenes (tiled_scan)
+#define SC_TILED_P 11
+#define SC_AGG 1ull
+#define SC_INC 2ull
+
+__device__ __forceinline__ unsigned long long sc_word(unsigned launch, unsigned long long state, int v) {
+    return ((unsigned long long)(launch & 0x3fffffffu) << 34) | (state << 32) | (unsigned)v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// the exclusive prefix of block b (> 0) of launch L from its predecessors' words (wave 0 of the block)
+__device__ __forceinline__ int sc_lookback(unsigned long long *words, int b, unsigned L, int *istat,
+                                           long long wait_ticks) {
+    const int lane = threadIdx.x & 63;
+    const unsigned ep = L & 0x3fffffffu;
+    int acc = 0, j = b - 1;
+    const long long t0 = wall_clock64();
+    for (;;) {
+        const int idx = j - lane;
+        bool ready = true, inc = true;
+        int v = 0;
+        if (idx >= 0) {
+            const unsigned long long w = __hip_atomic_load(words + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned st = (unsigned)(w >> 32) & 3u;
+            ready = (unsigned)(w >> 34) == ep && st != 0;
+            inc = ready && st == (unsigned)SC_INC;
+            v = (int)(unsigned)w;
+        }
+        const unsigned long long im = __ballot(inc), rm = __ballot(ready);
+        const int fi = im ? __ffsll((long long)im) - 1 : 63;  // the nearest inclusive word, else the window's end
+        const unsigned long long need = fi >= 63 ? ~0ull : ((2ull << fi) - 1);
+        if ((rm & need) == need) {
+            acc += wave_sum(lane <= fi ? v : 0);
+            if (im) return acc;
+            j -= 64;
+            continue;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > wait_ticks) {
+            if (lane == 0) atomicOr(&istat[IS_OVF], OVF_SCAN);
+            return acc;
+        }
+    }
+}
+
+template <int NV>
+__global__ void __launch_bounds__(ME_THREADS) k_scan1(const int *in, int *out, int *out2, const int *n_dev, int n_host,
+                                                      int cap, int *total, int *total2, unsigned long long *site,
+                                                      int *istat, long long wait_ticks, int capC, int capP) {
+    __shared__ int sh[ME_THREADS / 64];
+    __shared__ unsigned long long tk;
+    __shared__ int pre[2];
+    const int tid = threadIdx.x, nb = (int)gridDim.x;
+    if (tid == 0) tk = __hip_atomic_fetch_add(site, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int b = (int)(tk % (unsigned long long)nb);
+    const unsigned L = (unsigned)(tk / (unsigned long long)nb);
+    unsigned long long *words = site + 1;
+    const int n = scan_len(n_dev, n_host, cap);
+    const int base = b * SCAN_BLOCK + tid * SCAN_ITEMS;
+    int vals[SCAN_ITEMS];
+    int s = 0, f = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        vals[k] = (base + k < n) ? in[base + k] : 0;
+        s += vals[k];
+        f += vals[k] > 0 ? 1 : 0;
+    }
+    int agg, agg2 = 0;
+    const int x = block_excl_scan(s, sh, &agg);
+    const int x2 = NV == 2 ? block_excl_scan(f, sh, &agg2) : 0;
+    if (tid < 64) {
+        int p0 = 0, p1 = 0;
+        if (b == 0) {
+            if (tid == 0) {
+                __hip_atomic_store(words, sc_word(L, SC_INC, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (NV == 2)
+                    __hip_atomic_store(words + nb, sc_word(L, SC_INC, agg2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+            if (tid == 0) {
+                __hip_atomic_store(words + b, sc_word(L, SC_AGG, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (NV == 2)
+                    __hip_atomic_store(words + nb + b, sc_word(L, SC_AGG, agg2), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+            p0 = sc_lookback(words, b, L, istat, wait_ticks);
+            if (NV == 2) p1 = sc_lookback(words + nb, b, L, istat, wait_ticks);
+            if (tid == 0) {
+                __hip_atomic_store(words + b, sc_word(L, SC_INC, p0 + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (NV == 2)
+                    __hip_atomic_store(words + nb + b, sc_word(L, SC_INC, p1 + agg2), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (tid == 0) { pre[0] = p0; pre[1] = p1; }
+    }
+    __syncthreads();
+    int run = pre[0] + x, run2 = pre[1] + x2;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) {
+            out[base + k] = run;
+            if (NV == 2) out2[base + k] = run2;
+        }
+        run += vals[k];
+        run2 += vals[k] > 0 ? 1 : 0;
+    }
+    if (b == nb - 1 && tid == 0) {
+        const int t1 = pre[0] + agg, t2 = pre[1] + agg2;
+        if (total) *total = t1;
+        if (NV == 2 && total2) *total2 = t2;
+        if (capC > 0) {  // the compaction's counts, clamped (k_clamp_counts with copy_raw)
+            istat[IS_CRAW] = t1;
+            if (t1 > capC) { istat[IS_C] = capC; atomicOr(&istat[IS_OVF], OVF_CONTACTS); }
+            if (NV == 2 && t2 > capP) { istat[IS_P] = capP; atomicOr(&istat[IS_OVF], OVF_PAIRS); }
+        }
     }
 }
 
@@ -403,13 +541,6 @@ __global__ void k_narrow(int cap, const int *istat, Bodies b, const float *Rw, c
     }
 }
 
-__global__ void k_pair_flag(int cap, const int *istat, const int *slot_n, int *flag) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    int n = istat[IS_CAND];
-    if (n > cap) n = cap;
-    if (p >= n) return;
-    flag[p] = slot_n[p] > 0 ? 1 : 0;
-}
 
 struct Contacts {
     int64_t *key;

@@ -109,6 +109,7 @@ struct me_engine {
     // scan scratch + status
     int *bsum;
     int nbsum;
+    unsigned long long *scan_words;  // the one-pass scans' per-site ticket counter and block words
     int *istat;
     int *hstat;  // pinned host mirror
     size_t state_bytes = 0;  // leading workspace bytes that carry state between substeps
@@ -288,6 +289,9 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     tiled_layout_scratch(e->tb, B, N, (int)Pc, (int)C, (c.flags & ME_FLAG_FAST_MATH) != 0);
     e->nbsum = (int)((scan_n + SCAN_BLOCK - 1) / SCAN_BLOCK) + 1;
     e->bsum = B.take<int>((size_t)e->nbsum);
+    // the one-pass scans' ticket counters and block words: zeroed with the workspace at creation and
+    // never cleared again (each launch's words carry its own launch number)
+    e->scan_words = B.take<unsigned long long>((size_t)SCAN_SITES * (1 + 2 * (size_t)e->nbsum));
     *bytes = (B.off + 255) & ~size_t(255);
 }
 
@@ -312,17 +316,33 @@ int validate(const me_config *c) {
     return ME_OK;
 }
 
-// exclusive scan in -> out over n elements (n_dev device count or n_host), total -> *total_dev
-int scan(me_engine *e, const int *in, int *out, const int *n_dev, int n_host, int cap, int *total_dev, hipStream_t s) {
+// call sites of the one-pass scan, each with its own ticket counter and block words (k_scan1)
+enum { SC_GRID = 0, SC_CAND, SC_BP_SLOT, SC_BP_KEEP, SC_BP_PAIR, SC_BEAD, SC_NARROW, SC_JAC, SC_COLOR };
+static_assert(SC_COLOR < SC_TILED_B, "scan sites");
+
+unsigned long long *scan_site(me_engine *e, int site) { return e->scan_words + (size_t)site * (1 + 2 * (size_t)e->nbsum); }
+
+// exclusive scan in -> out over n elements (n_dev device count or n_host), total -> *total_dev; one
+// launch (k_scan1). scan2 also scans the flags in[i] > 0 into out2 (total2) and, with capC > 0, leaves
+// the clamped contact and pair counts of a compaction in the status words.
+int scan2(me_engine *e, int site, const int *in, int *out, int *out2, const int *n_dev, int n_host, int cap,
+          int *total_dev, int *total2, int capC, int capP, hipStream_t s) {
     int nb = (cap + SCAN_BLOCK - 1) / SCAN_BLOCK;
     if (nb < 1) nb = 1;
     if (nb > e->nbsum) return fail(ME_ERUNTIME, "scan scratch too small");
-    if (nb > 1024 * 64) return fail(ME_ERUNTIME, "scan too large");
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(ME_THREADS), 0, s, in, n_dev, n_host, cap, e->bsum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, e->bsum, nb, total_dev);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(ME_THREADS), 0, s, in, out, n_dev, n_host, cap, e->bsum);
+    if (out2)
+        hipLaunchKernelGGL(k_scan1<2>, dim3(nb), dim3(ME_THREADS), 0, s, in, out, out2, n_dev, n_host, cap, total_dev,
+                           total2, scan_site(e, site), e->istat, e->giant_wait_ticks, capC, capP);
+    else
+        hipLaunchKernelGGL(k_scan1<1>, dim3(nb), dim3(ME_THREADS), 0, s, in, out, nullptr, n_dev, n_host, cap,
+                           total_dev, nullptr, scan_site(e, site), e->istat, e->giant_wait_ticks, 0, 0);
     LAUNCH_CHECK();
     return ME_OK;
+}
+
+int scan(me_engine *e, int site, const int *in, int *out, const int *n_dev, int n_host, int cap, int *total_dev,
+         hipStream_t s) {
+    return scan2(e, site, in, out, nullptr, n_dev, n_host, cap, total_dev, nullptr, 0, 0, s);
 }
 
 Bodies bodies_of(me_engine *e) {
@@ -345,7 +365,7 @@ int build_grid(me_engine *e, int n, const float *pos, const int *scene, int div,
                bool body_grid, hipStream_t s) {
     hipLaunchKernelGGL(k_grid_count, dim3(nblk(n)), dim3(ME_THREADS), 0, s, n, pos, scene, div, e->istat, cell_fixed,
                        e->cfg.margin, e->T, e->cellc, e->bucket, e->bcount);
-    int rc = scan(e, e->bcount, e->bstart, nullptr, e->T, e->T, nullptr, s);
+    int rc = scan(e, SC_GRID, e->bcount, e->bstart, nullptr, e->T, e->T, nullptr, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n)), dim3(ME_THREADS), 0, s, n, e->bucket, e->bstart, e->bcursor,
                        e->items, pos, spos, e->cellc, e->aabb, e->b.inv_mass, body_grid ? e->scell : nullptr,
@@ -364,7 +384,7 @@ int body_pairs(me_engine *e, int all_pairs, hipStream_t s) {
     hipLaunchKernelGGL(k_box_pairs, dim3((N + ME_BP_PER_WG - 1) / ME_BP_PER_WG), dim3(32 * ME_BP_PER_WG), 0, s, N, b,
                        e->aabb, e->cellc, e->bstart, e->bcount, e->scell, e->sbox,
                        e->T, e->cfg.margin, all_pairs, e->cand_n, e->cand_tmp, e->istat);
-    rc = scan(e, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
+    rc = scan(e, SC_CAND, e->cand_n, e->cand_off, nullptr, N, N, e->istat + IS_CAND, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_box_cand_emit, dim3(nblk(N)), dim3(ME_THREADS), 0, s, N, e->cand_n, e->cand_off,
                        e->cand_tmp, e->cap_cand, e->cand_a, e->cand_b, e->istat);
@@ -388,7 +408,7 @@ int generate_bead_pairs(me_engine *e, hipStream_t s) {
     int wg = nblk(64LL * cc);
     hipLaunchKernelGGL(k_bp_count, dim3(wg), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->cand_b, e->allp, P,
                        R, W, e->bp_mask, e->S.n);
-    rc = scan(e, e->S.n, e->slot_off, e->istat + IS_CAND, 0, cc, e->istat + IS_CRAW, s);
+    rc = scan(e, SC_BP_SLOT, e->S.n, e->slot_off, e->istat + IS_CAND, 0, cc, e->istat + IS_CRAW, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_bp_emit, dim3(wg), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->cand_b, e->allp, N, P,
                        R, W, e->bp_mask, e->slot_off, e->capC, e->raw, e->istat);
@@ -397,9 +417,9 @@ int generate_bead_pairs(me_engine *e, hipStream_t s) {
     hipLaunchKernelGGL(k_bp_select, dim3(sel_blocks), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->b.x, e->S.n,
                        e->slot_off, e->capC, ME_BEAD_KEEP, e->cfg.warm_start, e->wkey, e->wjn, e->wjt1, e->wjt2,
                        e->raw, e->bp_kn, e->pair_flag);
-    rc = scan(e, e->bp_kn, e->bp_koff, e->istat + IS_CAND, 0, cc, e->istat + IS_C, s);
+    rc = scan(e, SC_BP_KEEP, e->bp_kn, e->bp_koff, e->istat + IS_CAND, 0, cc, e->istat + IS_C, s);
     if (rc) return rc;
-    rc = scan(e, e->pair_flag, e->pair_off, e->istat + IS_CAND, 0, cc, e->istat + IS_P, s);
+    rc = scan(e, SC_BP_PAIR, e->pair_flag, e->pair_off, e->istat + IS_CAND, 0, cc, e->istat + IS_P, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_bp_compact, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, e->cand_a, e->cand_b,
                        e->b.mu, e->S.n, e->slot_off, e->bp_kn, e->bp_koff, e->pair_flag, e->pair_off, e->capC,
@@ -424,7 +444,7 @@ int generate(me_engine *e, hipStream_t s) {
         hipLaunchKernelGGL(k_bead_contacts<false>, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, NP, P, e->allp, e->cellc,
                            e->bstart, e->bcount, e->spos, e->T, R, e->b.mu, e->cand_n, e->cand_off, e->capC, e->Cc,
                            e->istat);
-        rc = scan(e, e->cand_n, e->cand_off, nullptr, NP, NP, e->istat + IS_C, s);
+        rc = scan(e, SC_BEAD, e->cand_n, e->cand_off, nullptr, NP, NP, e->istat + IS_C, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_bead_contacts<true>, dim3(nblk(NP)), dim3(ME_THREADS), 0, s, NP, P, e->allp, e->cellc,
                            e->bstart, e->bcount, e->spos, e->T, R, e->b.mu, e->cand_n, e->cand_off, e->capC, e->Cc,
@@ -445,14 +465,12 @@ int generate(me_engine *e, hipStream_t s) {
     hipLaunchKernelGGL(k_narrow, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, b, e->Rw, e->cand_a, e->cand_b,
                        e->cfg.margin, N, e->S);
     prof_mark(e, ME_K_NARROW, s);
-    rc = scan(e, e->S.n, e->slot_off, e->istat + IS_CAND, 0, cc, e->istat + IS_C, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_pair_flag, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, e->S.n, e->pair_flag);
-    rc = scan(e, e->pair_flag, e->pair_off, e->istat + IS_CAND, 0, cc, e->istat + IS_P, s);
+    // contact slots and non-empty pairs in one scan, which leaves the clamped counts (k_clamp_counts)
+    rc = scan2(e, SC_NARROW, e->S.n, e->slot_off, e->pair_off, e->istat + IS_CAND, 0, cc, e->istat + IS_C,
+               e->istat + IS_P, e->capC, e->capP, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_compact, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->capC, e->capP, e->istat, e->cand_a,
                        e->cand_b, e->S, e->slot_off, e->pair_off, e->Cc, e->Pp);
-    hipLaunchKernelGGL(k_clamp_counts, dim3(1), dim3(1), 0, s, e->istat, e->capC, e->capP, 1);
     if (e->cfg.flags & ME_FLAG_PER_CONTACT)
         hipLaunchKernelGGL(k_pairs_per_contact, dim3(nblk(e->capC)), dim3(ME_THREADS), 0, s, e->capC, e->istat,
                            e->Cc.bi, e->Cc.bj, e->Pp);
@@ -473,7 +491,7 @@ int solve_jacobi(me_engine *e, float sdt, hipStream_t s) {
     // the body-major entry list of this substep's contacts
     hipLaunchKernelGGL(k_jac_slots, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->Cc.bi, e->Cc.bj, e->jcnt,
                        e->jposA, e->jposB);
-    rc = scan(e, e->jcnt, e->joff, nullptr, N, N, e->istat + IS_JTOT, s);
+    rc = scan(e, SC_JAC, e->jcnt, e->joff, nullptr, N, N, e->istat + IS_JTOT, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_jac_place, dim3(nblk(C)), dim3(ME_THREADS), 0, s, capC, e->istat, e->Cc.bi, e->Cc.bj, e->joff,
                        e->jposA, e->jposB);
@@ -590,7 +608,7 @@ int solve_colored(me_engine *e, float sdt, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, e->ccount, ME_NCOLMAX + 1, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, e->cfill, ME_NCOLMAX + 1, 0);
     hipLaunchKernelGGL(k_color_count, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->Pp.col, e->ccount);
-    rc = scan(e, e->ccount, e->cstart, nullptr, ncol, ME_NCOLMAX + 1, nullptr, s);
+    rc = scan(e, SC_COLOR, e->ccount, e->cstart, nullptr, ncol, ME_NCOLMAX + 1, nullptr, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_color_scatter, dim3(nblk(P)), dim3(ME_THREADS), 0, s, capP, e->istat, e->Pp.col, e->cstart,
                        e->cfill, e->Pp.porder);
@@ -761,6 +779,7 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.wkey = e->wkey; a.wjn = e->wjn; a.wjt1 = e->wjt1; a.wjt2 = e->wjt2;
     a.istat = e->istat;
     a.bsum = e->bsum; a.nbsum = e->nbsum;
+    a.scan_words = e->scan_words;
     a.claim = e->claim;
     a.side = e->side; a.ev_fork = e->ev_fork; a.ev_join = e->ev_join;
     a.prof_ev = e->prof_on ? prof_event : nullptr;

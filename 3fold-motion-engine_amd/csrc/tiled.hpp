@@ -94,6 +94,7 @@ struct TiledArgs {
     float *wjn, *wjt1, *wjt2;
     int *istat, *bsum;
     int nbsum;
+    unsigned long long *scan_words;  // the one-pass scans' ticket counters and block words (k_scan1)
     unsigned long long *claim;
     void *side, *ev_fork, *ev_join;  // side stream and fork/join events of the sweep kernels
     // timing: next event of kernel `id` (ME_K_*) for the engine `prof`, null when not timed
@@ -1394,14 +1395,13 @@ __global__ void k_t_prep(TiledArgs A, TiledBuffers T) {
 
 #include "tiled_big.hpp"
 
-inline int tiled_scan(const TiledArgs &A, const int *in, int *out, int n, int *total, hipStream_t s) {
+inline int tiled_scan(const TiledArgs &A, int site, const int *in, int *out, int n, int *total, hipStream_t s) {
     int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
     if (nb < 1) nb = 1;
     if (nb > A.nbsum) return ME_ERUNTIME;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(ME_THREADS), 0, s, in, nullptr, n, n, A.bsum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, A.bsum, nb, total);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(ME_THREADS), 0, s, in, out, nullptr, n, n, A.bsum);
-    return ME_OK;
+    hipLaunchKernelGGL(k_scan1<1>, dim3(nb), dim3(ME_THREADS), 0, s, in, out, nullptr, nullptr, n, n, total, nullptr,
+                       A.scan_words + (size_t)site * (1 + 2 * (size_t)A.nbsum), A.istat, A.giant_wait_ticks, 0, 0);
+    return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;
 }
 
 inline int tblk(long long n) { return (int)((n + ME_THREADS - 1) / ME_THREADS); }
@@ -1623,9 +1623,9 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
         hipLaunchKernelGGL(k_t_flatten, dim3((N + T_AGG - 1) / T_AGG), dim3(T_AGG), 0, s, N, A.b.invM, T);
         hipLaunchKernelGGL(k_t_pair_root, dim3((capP + T_AGG - 1) / T_AGG), dim3(T_AGG), 0, s, capP, A.istat, A.Pp.pbi, A.Pp.pbj,
                            A.b.invM, T);
-        int rc = tiled_scan(A, T.cnt_b, T.boff, N, A.istat + IS_NBODYSORT, s);
+        int rc = tiled_scan(A, SC_TILED_B, T.cnt_b, T.boff, N, A.istat + IS_NBODYSORT, s);
         if (rc) return rc;
-        rc = tiled_scan(A, T.cnt_p, T.poff, N, A.istat + IS_NPAIRSORT, s);
+        rc = tiled_scan(A, SC_TILED_P, T.cnt_p, T.poff, N, A.istat + IS_NPAIRSORT, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_t_scatter_b, dim3((N + T_AGG - 1) / T_AGG), dim3(T_AGG), 0, s, N, T);
         hipLaunchKernelGGL(k_t_scatter_p, dim3((capP + T_AGG - 1) / T_AGG), dim3(T_AGG), 0, s, capP, A.istat, T);

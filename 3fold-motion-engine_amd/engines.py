@@ -243,6 +243,9 @@ class _HipContactEngine:
             what = [n for bit, n in ((1, "contacts"), (2, "body pairs"), (4, "broad-phase candidates"),
                                      (8, "colours"), (16, "tile"), (32, "recorded tile for the adjoint"))
                     if ovf & bit]
+            if ovf & 256:
+                raise RuntimeError(f"{s.name}.step: a one-pass scan waited past its bound for an earlier block; "
+                                   f"the step's state is not valid")
             if ovf & 64:
                 raise RuntimeError(f"{s.name}.step: the grid sweep of an island beyond one workgroup timed out "
                                    f"waiting for its workgroups; the step's state is not valid")

@@ -139,7 +139,9 @@ enum {
     ME_CNT_CACHE_MISSES,
     ME_CNT_OVERFLOW,     /* overflow flags: bit 0 contacts, 1 pairs, 2 candidates, 3 colours, 4 tile,
                             5 recorded tile beyond one small tile, 6 a giant island's grid barrier
-                            waited past its bound (the step's state is not valid) */
+                            waited past its bound (the step's state is not valid), 7 an adjoint
+                            contribution beyond the fixed-point range, 8 a one-pass scan waited past
+                            its bound for an earlier block (the step's state is not valid) */
     ME_CNT_CANDIDATES,   /* broad-phase candidate pairs */
     ME_CNT_ISLANDS,      /* islands of the last tiled solve */
     ME_CNT_TILES,        /* tiles of the last tiled solve */
