@@ -15,6 +15,7 @@ enum {
     IS_COLQ,  // the same queue for the big tiles' colouring (k_t_color)
     IS_NSMALL,  // small tiles of the substep (k_t_biglist)
     IS_SMALLQ,  // next entry of the largest-first small-tile queue taken by k_t_small
+    IS_CCDIFF,  // the tiled solve's pair list differs from the colour cache's (k_t_setup)
     IS_COUNT = 32
 };
 enum { OVF_CONTACTS = 1, OVF_PAIRS = 2, OVF_CAND = 4, OVF_COLORS = 8, OVF_TILE = 16,
@@ -965,10 +966,9 @@ __global__ void k_jac_reset(int N, int capC, const int *istat, float *pv, float 
 
 // ── coloured solvers ──
 // A14 warm start by binary search in the previous step's kept keys (contact_engine_gpu_colored.py:602-626)
-__global__ void k_warm_lookup(int capC, const int *istat, int warm, const int64_t *key, const int64_t *wkey,
-                              const float *wjn, const float *wjt1, const float *wjt2, float *jn, float *jt1, float *jt2,
-                              float *jp) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void warm_lookup_at(int c, int capC, const int *istat, int warm, const int64_t *key,
+                                               const int64_t *wkey, const float *wjn, const float *wjt1,
+                                               const float *wjt2, float *jn, float *jt1, float *jt2, float *jp) {
     int C = istat[IS_C];
     if (c >= C || c >= capC) return;
     jn[c] = 0.0f; jt1[c] = 0.0f; jt2[c] = 0.0f; jp[c] = 0.0f;
@@ -985,9 +985,15 @@ __global__ void k_warm_lookup(int capC, const int *istat, int warm, const int64_
     }
     if (hit >= 0) { jn[c] = wjn[hit]; jt1[c] = wjt1[hit]; jt2[c] = wjt2[hit]; }
 }
+__global__ void k_warm_lookup(int capC, const int *istat, int warm, const int64_t *key, const int64_t *wkey,
+                              const float *wjn, const float *wjt1, const float *wjt2, float *jn, float *jt1, float *jt2,
+                              float *jp) {
+    warm_lookup_at(blockIdx.x * blockDim.x + threadIdx.x, capC, istat, warm, key, wkey, wjn, wjt1, wjt2, jn, jt1, jt2,
+                   jp);
+}
 // the warm-start match of every contact (k_warm_lookup's binary search), kept for the adjoint
-__global__ void k_tape_widx(int capC, const int *istat, int warm, const int64_t *key, const int64_t *wkey, int *widx) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tape_widx_at(int c, int capC, const int *istat, int warm, const int64_t *key,
+                                             const int64_t *wkey, int *widx) {
     int C = istat[IS_C];
     if (c >= C || c >= capC) return;
     int wn = istat[IS_WN], hit = -1;
@@ -1004,6 +1010,9 @@ __global__ void k_tape_widx(int capC, const int *istat, int warm, const int64_t 
     }
     widx[c] = hit;
 }
+__global__ void k_tape_widx(int capC, const int *istat, int warm, const int64_t *key, const int64_t *wkey, int *widx) {
+    tape_widx_at(blockIdx.x * blockDim.x + threadIdx.x, capC, istat, warm, key, wkey, widx);
+}
 
 // A23 warm store (contact_engine_gpu_colored.py:1089-1093)
 __global__ void k_warm_store(int capC, int *istat, const int64_t *key, const float *jn, const float *jt1,
@@ -1017,16 +1026,21 @@ __global__ void k_warm_store(int capC, int *istat, const int64_t *key, const flo
 }
 
 // ColorCache compare (contact_engine_gpu_color_cache.py:7-18, 37-54)
-__global__ void k_cc_compare(int N, int *istat, const int *pbi, const int *pbj, const int *ca, const int *cb,
-                             const float *invM, const float *cinvM) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// status word `slot` := 1 when the pair list or a body's invM differs from the colour cache's (called
+// by every thread of the grid: a wave ballot)
+__device__ __forceinline__ void cc_compare_at(int i, int N, int *istat, int slot, const int *pbi, const int *pbj,
+                                              const int *ca, const int *cb, const float *invM, const float *cinvM) {
     int P = istat[IS_P];
     if (P != istat[IS_CACHED_P]) {
-        if (i == 0) istat[IS_MISMATCH] = 1;
+        if (i == 0) istat[slot] = 1;
         return;
     }
     bool diff = (i < P && (pbi[i] != ca[i] || pbj[i] != cb[i])) || (i < N && invM[i] != cinvM[i]);
-    if (__ballot(diff) && __lane_id() == 0) atomicMax(&istat[IS_MISMATCH], 1);
+    if (__ballot(diff) && __lane_id() == 0) atomicMax(&istat[slot], 1);
+}
+__global__ void k_cc_compare(int N, int *istat, const int *pbi, const int *pbj, const int *ca, const int *cb,
+                             const float *invM, const float *cinvM) {
+    cc_compare_at(blockIdx.x * blockDim.x + threadIdx.x, N, istat, IS_MISMATCH, pbi, pbj, ca, cb, invM, cinvM);
 }
 
 // PCG hash (Jarzynski & Olano 2020), behind rand_init(seed, offset) / randi

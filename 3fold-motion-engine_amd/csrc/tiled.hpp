@@ -171,8 +171,7 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
 }
 
 // IS_MISMATCH after k_t_mode: 0 = ColorCache hit, 1 = colour this substep, 2 = no contacts
-__global__ void k_t_init(int N, TiledBuffers T, int *istat, int lds_islands) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void t_init_at(int i, int N, const TiledBuffers &T, int *istat, int lds_islands) {
     if (i < N) {
         if (!lds_islands) {
             T.parent[i] = i;
@@ -195,23 +194,34 @@ __global__ void k_t_init(int N, TiledBuffers T, int *istat, int lds_islands) {
     }
 }
 
-__global__ void k_t_mode(int *istat, int cc_mode) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (istat[IS_C] <= 0 || istat[IS_P] <= 0) { istat[IS_MISMATCH] = 2; istat[IS_NCOL] = 0; return; }
-    if (!cc_mode) { istat[IS_MISMATCH] = 1; return; }
-    if (istat[IS_MISMATCH] == 0) {
-        istat[IS_HITS] += 1;
-        istat[IS_NCOL] = istat[IS_CACHED_NCOL];
-    } else {
-        istat[IS_MISMATCH] = 1;
-        istat[IS_MISSES] += 1;
-    }
+// The first launch of the tiled set-up: the adjoint's warm-start matches (recording), the contacts'
+// warm-start impulses (k_warm_lookup), the tile arrays' initial values and, under ColorCache, the
+// comparison of the pair list with the colour cache (into IS_CCDIFF, clear since the last substep's
+// k_t_biglist). One thread per contact, pair and body.
+__global__ void k_t_setup(TiledArgs A, TiledBuffers T, int lds_islands, int cc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (A.taped) tape_widx_at(i, A.capC, A.istat, A.warm, A.Cc.key, A.wkey, A.tape.widx);
+    if (!A.warm_set)
+        warm_lookup_at(i, A.capC, A.istat, A.warm, A.Cc.key, A.wkey, A.wjn, A.wjt1, A.wjt2, A.Cc.jn, A.Cc.jt1,
+                       A.Cc.jt2, A.Cc.jp);
+    t_init_at(i, A.N, T, A.istat, lds_islands);
+    if (cc) cc_compare_at(i, A.N, A.istat, IS_CCDIFF, A.Pp.pbi, A.Pp.pbj, A.cache_a, A.cache_b, A.b.invM, A.cache_invM);
 }
 
-__global__ void k_t_cc_restore(int capP, const int *istat, const int *ccol, int *col) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (istat[IS_MISMATCH] != 0) return;
-    if (i < istat[IS_P] && i < capP) col[i] = ccol[i];
+// The substep's colouring mode, IS_MISMATCH: 2 without contacts, 1 to colour (a cache miss, or no
+// cache), 0 to take the cached colours, which the launch copies back into the pairs. Every thread
+// derives the mode from the same words; the first writes it and the cache counters.
+__global__ void k_t_mode(int capP, int *istat, int cc_mode, const int *ccol, int *col) {
+    const int P = istat[IS_P];
+    const int mode = (istat[IS_C] <= 0 || P <= 0) ? 2 : (!cc_mode || istat[IS_CCDIFF] != 0) ? 1 : 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        istat[IS_MISMATCH] = mode;
+        if (mode == 2) istat[IS_NCOL] = 0;
+        else if (cc_mode && mode == 0) { istat[IS_HITS] += 1; istat[IS_NCOL] = istat[IS_CACHED_NCOL]; }
+        else if (cc_mode) istat[IS_MISSES] += 1;
+    }
+    if (mode == 0 && i < P && i < capP) col[i] = ccol[i];
 }
 
 // end of a tiled substep, one launch: the warm store (A23, contact_engine_gpu_colored.py:1089-1093)
@@ -686,29 +696,49 @@ __global__ void __launch_bounds__(WG) k_t_color(TiledArgs A, TiledBuffers T, int
 }
 
 // split of the JP classes (same rule as k_color_split_plan)
-__global__ void k_t_plan(int *istat, int K, TiledBuffers T) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave: the greedy split rule of k_color_split_plan (each of the K - nc extra colours goes to the
+// class with the most pairs per sub-colour, the lowest class index on ties) with the class counts in
+// LDS and an argmax over the wave per extra colour.
+__global__ void __launch_bounds__(64) k_t_plan(int *istat, int K, TiledBuffers T) {
+    __shared__ int cnt[ME_NCOLMAX], sub[ME_NCOLMAX], base[ME_NCOLMAX + 1];
     if (istat[IS_MISMATCH] != 1) return;
+    const int lane = threadIdx.x;
     int nc = istat[IS_NCOLJP];
-    if (nc > ME_NCOLMAX) { nc = ME_NCOLMAX; atomicOr(&istat[IS_OVF], OVF_COLORS); }
-    for (int c = 0; c < nc; ++c) T.csub[c] = 1;
-    int tot = nc;
-    if (K > nc && nc > 0) {
-        for (int e = nc; e < K && e < ME_NCOLMAX; ++e) {
-            int bc = 0;
-            for (int c = 1; c < nc; ++c)
-                if ((long long)T.ccount_jp[c] * T.csub[bc] > (long long)T.ccount_jp[bc] * T.csub[c]) bc = c;
-            T.csub[bc]++;
-        }
-        tot = 0;
-        for (int c = 0; c < nc; ++c) tot += T.csub[c];
+    if (nc > ME_NCOLMAX) {
+        nc = ME_NCOLMAX;
+        if (lane == 0) atomicOr(&istat[IS_OVF], OVF_COLORS);
     }
-    T.cbase[0] = 0;
-    for (int c = 0; c < nc; ++c) T.cbase[c + 1] = T.cbase[c] + T.csub[c];
-    for (int c = 0; c < nc; ++c)
-        for (int s = T.cbase[c]; s < T.cbase[c + 1] && s < ME_NCOLMAX; ++s) T.ccls[s] = c;
-    T.ccls[ME_NCOLMAX] = tot;
-    istat[IS_NCOL] = tot;
+    for (int c = lane; c < nc; c += 64) { cnt[c] = T.ccount_jp[c]; sub[c] = 1; }
+    __syncthreads();
+    for (int e = nc; K > nc && nc > 0 && e < K && e < ME_NCOLMAX; ++e) {
+        int bc = -1;
+        for (int c = lane; c < nc; c += 64)  // this lane's best, lowest index first
+            if (bc < 0 || (long long)cnt[c] * sub[bc] > (long long)cnt[bc] * sub[c]) bc = c;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const int oc = __shfl_xor(bc, o, 64);
+            if (oc >= 0 && (bc < 0 || (long long)cnt[oc] * sub[bc] > (long long)cnt[bc] * sub[oc] ||
+                            ((long long)cnt[oc] * sub[bc] == (long long)cnt[bc] * sub[oc] && oc < bc)))
+                bc = oc;
+        }
+        if (lane == 0) sub[bc]++;
+        __syncthreads();
+    }
+    if (lane == 0) {
+        base[0] = 0;
+        for (int c = 0; c < nc; ++c) base[c + 1] = base[c] + sub[c];
+    }
+    __syncthreads();
+    const int tot = nc > 0 ? base[nc] : 0;
+    for (int c = lane; c < nc; c += 64) {
+        T.csub[c] = sub[c];
+        for (int s = base[c]; s < base[c + 1] && s < ME_NCOLMAX; ++s) T.ccls[s] = c;
+    }
+    for (int c = lane; c <= nc; c += 64) T.cbase[c] = base[c];
+    if (lane == 0) {
+        T.ccls[ME_NCOLMAX] = tot;
+        istat[IS_NCOL] = tot;
+    }
 }
 
 // final colour of every pair: on a fresh colouring, JP class c becomes base[c] + rank mod sub[c]
@@ -1470,6 +1500,7 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_color_small(TiledArgs A, TiledB
 __global__ void k_t_biglist(int ntile, int *istat, const float *invM, const float *invIw, TiledBuffers T) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < 5) T.gbar[t] = 0u;  // the giant-tile grid barriers and round counters of this substep start at zero
+    if (t == 0) istat[IS_CCDIFF] = 0;  // read for the last time by k_t_mode: clear for the next substep
     if (t < istat[IS_NBODYSORT]) {
         int i = T.bsorted[t];
         T.invMs[t] = invM[i];
@@ -1596,22 +1627,13 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     int ntile = tiled_ntile(A);
     bool cc = A.solver == ME_SOLVER_COLOR_CACHE;
     int g = capP > N ? capP : N;
-    // pv, po were zeroed by k_body_pre
-    if (A.taped)  // before the lookup: the warm store still holds the previous substep's contacts
-        hipLaunchKernelGGL(k_tape_widx, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
-                           A.wkey, A.tape.widx);
-    if (!A.warm_set)
-        hipLaunchKernelGGL(k_warm_lookup, dim3(tblk(capC)), dim3(ME_THREADS), 0, s, capC, A.istat, A.warm, A.Cc.key,
-                           A.wkey, A.wjn, A.wjt1, A.wjt2, A.Cc.jn, A.Cc.jt1, A.Cc.jt2, A.Cc.jp);
+    // pv, po were zeroed by k_body_pre; the warm store still holds the previous substep's contacts
     bool lds_isl = N <= T_ISL_MAXN;
-    hipLaunchKernelGGL(k_t_init, dim3(tblk(N > ME_NCOLMAX + 1 ? N : ME_NCOLMAX + 1)), dim3(ME_THREADS), 0, s, N, T,
-                       A.istat, lds_isl ? 1 : 0);
-    if (cc)
-        hipLaunchKernelGGL(k_cc_compare, dim3(tblk(g)), dim3(ME_THREADS), 0, s, N, A.istat, A.Pp.pbi, A.Pp.pbj,
-                           A.cache_a, A.cache_b, A.b.invM, A.cache_invM);
-    hipLaunchKernelGGL(k_t_mode, dim3(1), dim3(1), 0, s, A.istat, cc ? 1 : 0);
-    if (cc) hipLaunchKernelGGL(k_t_cc_restore, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.cache_col,
-                               A.Pp.col);
+    int gs = g > capC ? g : capC;
+    if (gs < ME_NCOLMAX + 1) gs = ME_NCOLMAX + 1;
+    hipLaunchKernelGGL(k_t_setup, dim3(tblk(gs)), dim3(ME_THREADS), 0, s, A, T, lds_isl ? 1 : 0, cc ? 1 : 0);
+    hipLaunchKernelGGL(k_t_mode, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, cc ? 1 : 0, A.cache_col,
+                       A.Pp.col);
     if (lds_isl) {
         if (prof_rec(A, ME_K_ISLANDS, s)) return ME_ERUNTIME;
         hipLaunchKernelGGL(k_t_islands_lds, dim3(1), dim3(T_ISL_THREADS), 3 * N * sizeof(int), s, N, capP, TB, A.istat,
@@ -1641,7 +1663,7 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
                        giant);
     if (prof_rec(A, ME_K_COLOR, s)) return ME_ERUNTIME;
     if (giant) hipLaunchKernelGGL(k_t_color_giant, dim3(A.giant_grid), dim3(T_BIG_WG), 0, s, A, T);
-    hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(1), 0, s, A.istat, A.n_colors, T);
+    hipLaunchKernelGGL(k_t_plan, dim3(1), dim3(64), 0, s, A.istat, A.n_colors, T);
     hipLaunchKernelGGL(k_t_apply_split, dim3(tblk(capP)), dim3(ME_THREADS), 0, s, capP, A.istat, A.Pp.col, T);
     return hipGetLastError() == hipSuccess ? ME_OK : ME_ERUNTIME;
 }

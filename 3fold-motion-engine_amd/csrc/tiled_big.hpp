@@ -228,6 +228,7 @@ __device__ __forceinline__ bool lds_big_tile(const TiledBuffers &T, int t, int &
     return !is_small_tile(nb, np) && nb <= T_BIG_B && np <= T_BIG_P;
 }
 
+#define T_ORD_PER (T_BIG_P / 1024)  // pairs per thread of k_t_bigord (kept in registers between its passes)
 __global__ void __launch_bounds__(1024) k_t_bigord(TiledArgs A, TiledBuffers T) {
     __shared__ int cst[ME_NCOLMAX + 1], cfl[ME_NCOLMAX + 1], cfe[ME_NCOLMAX + 1];
     __shared__ int maxc, wdyn, wcon;
@@ -241,10 +242,27 @@ __global__ void __launch_bounds__(1024) k_t_bigord(TiledArgs A, TiledBuffers T) 
         if (tid == 0) { maxc = 0; wdyn = 0; wcon = 0; }
         for (int c = tid; c <= ncol; c += 1024) { cst[c] = 0; cfl[c] = 0; cfe[c] = 0; }
         __syncthreads();
-        for (int k = tid; k < np; k += 1024) {
-            int c = A.Pp.col[T.psorted[p0 + k]];
+        // pass 1: each thread's pairs (k = tid + 1024 j), their class, word and record entries
+        int pc[T_ORD_PER], pp[T_ORD_PER], pa[T_ORD_PER], pb[T_ORD_PER], pst[T_ORD_PER];
+        unsigned pw[T_ORD_PER];
+        int mdyn = 0, mcon = 0;
+#pragma unroll
+        for (int j = 0; j < T_ORD_PER; ++j) {
+            const int k = tid + 1024 * j;
+            pc[j] = -1;
+            if (k >= np) continue;
+            const int p = T.psorted[p0 + k];
+            int c = A.Pp.col[p];
             if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
             c = tile_class(T, c, ncol);
+            const int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
+            const bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
+            const int cnt = A.Pp.pcount[p];
+            const int la = dA ? T.bslot[a] - b0 : 0, lb = dB ? T.bslot[b] - b0 : 0;
+            pc[j] = c; pp[j] = p; pa[j] = a; pb[j] = b; pst[j] = A.Pp.pstart[p];
+            pw[j] = pw_make(la, lb, cnt, dA, dB, b >= 0);
+            mdyn += (int)dA + (int)dB;
+            mcon += cnt;
             atomicAdd(&cst[c], 1);
             atomicMax(&maxc, c + 1);
         }
@@ -255,23 +273,17 @@ __global__ void __launch_bounds__(1024) k_t_bigord(TiledArgs A, TiledBuffers T) 
             cst[maxc] = run;
         }
         __syncthreads();
-        int mdyn = 0, mcon = 0;
-        for (int k = tid; k < np; k += 1024) {
-            int p = T.psorted[p0 + k];
-            int c = A.Pp.col[p];
-            if (c < 0 || c >= ncol) c = ncol > 0 ? ncol - 1 : 0;
-            c = tile_class(T, c, ncol);
-            int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
-            bool dA = invM[a] > 0.0f, dB = b >= 0 && invM[b] > 0.0f;
-            int pos = (dA && dB) ? cst[c] + atomicAdd(&cfl[c], 1) : cst[c + 1] - 1 - atomicAdd(&cfe[c], 1);
-            int q = p0 + pos;
-            int st = A.Pp.pstart[p], cnt = A.Pp.pcount[p];
-            mdyn += (int)dA + (int)dB;
-            mcon += cnt;
-            int la = dA ? T.bslot[a] - b0 : 0, lb = dB ? T.bslot[b] - b0 : 0;
-            T.pwordg[q] = pw_make(la, lb, cnt, dA, dB, b >= 0);
+        // pass 2: the sorted places
+#pragma unroll
+        for (int j = 0; j < T_ORD_PER; ++j) {
+            if (pc[j] < 0) continue;
+            const int c = pc[j];
+            const bool two = ((pw[j] >> 28) & 3u) == 3u;  // both bodies dynamic
+            const int pos = two ? cst[c] + atomicAdd(&cfl[c], 1) : cst[c + 1] - 1 - atomicAdd(&cfe[c], 1);
+            const int q = p0 + pos;
+            T.pwordg[q] = pw[j];
             T.pclass[q] = c;
-            *reinterpret_cast<int4 *>(T.srec + 4 * (size_t)q) = make_int4(p, a, b, st);
+            *reinterpret_cast<int4 *>(T.srec + 4 * (size_t)q) = make_int4(pp[j], pa[j], pb[j], pst[j]);
         }
         if (mdyn) atomicAdd(&wdyn, mdyn);
         if (mcon) atomicAdd(&wcon, mcon);
