@@ -1583,13 +1583,14 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     if (FAST && A.taped) return ME_ERUNTIME;
     hipLaunchKernelGGL((k_t_prep<SOLVER == ME_SOLVER_COLOR_CACHE, FAST>), dim3(tblk(A.capC)), dim3(ME_THREADS), 0, s,
                        A, T);
+    // the big tiles' colour order and records, spread over the GPU before either tile kernel starts (on
+    // the side stream they would wait for the small tiles' persistent workgroups to leave the CUs)
+    hipLaunchKernelGGL(k_t_bigord, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(1024), 0, s, A, T);
+    hipLaunchKernelGGL(k_t_bigfill, dim3(ntile < A.big_grid ? ntile : A.big_grid, T_FILL_CHUNKS), dim3(T_FILL_WG), 0,
+                       s, A, T);
     if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(s2, fork, 0) != hipSuccess) return ME_ERUNTIME;
     // the big tiles (the critical path when there are any) are submitted first, so that their
     // workgroups find CUs with free LDS before the small tiles fill them
-    // the big tiles' colour order and records, spread over the GPU beside the small tiles' start
-    hipLaunchKernelGGL(k_t_bigord, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(1024), 0, s2, A, T);
-    hipLaunchKernelGGL(k_t_bigfill, dim3(ntile < A.big_grid ? ntile : A.big_grid, T_FILL_CHUNKS), dim3(T_FILL_WG), 0,
-                       s2, A, T);
     if (prof_rec(A, ME_K_TILE_BIG, s2)) return ME_ERUNTIME;
     hipLaunchKernelGGL((k_t_big<SOLVER, FAST>), dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(T_BIGT_WG), dyn, s2,
                        A, T);
