@@ -138,6 +138,9 @@ struct me_engine {
     int big_grid = 0;  // workgroups of k_t_big / k_t_color: one per CU their LDS allows
     int small_grid = 0;  // workgroups of the persistent k_t_small: as many as can be resident
     int small_grid_fast = 0;  // the same for its fast-arithmetic variant
+    // whether the largest scene could hold an island beyond the big tile's LDS (beyond the LDS colouring
+    // limits): islands never cross scenes, so below these sizes the giant-island kernels are not launched
+    int giant_possible = 1, gcol_possible = 1;
 };
 
 namespace {
@@ -352,6 +355,31 @@ Bodies bodies_of(me_engine *e) {
     b.mu = e->b.mu;
     b.kin = e->b.kinematic; b.scene = e->b.scene; b.rest = e->b.rest;
     return b;
+}
+
+// giant_possible / gcol_possible from the bodies per scene (read once from the device): an island
+// of a scene of n bodies has at most n bodies and n(n-1)/2 + n pairs. Scene ids outside [0, N) keep
+// both set.
+int scene_bounds(me_engine *e) {
+    const int N = e->N;
+    long long mx = N;
+    if (e->b.scene) {
+        std::vector<int> sc((size_t)N), cnt((size_t)N, 0);
+        HIP_TRY(hipMemcpy(sc.data(), e->b.scene, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost));
+        bool in_range = true;
+        for (int i = 0; i < N && in_range; ++i) {
+            if (sc[i] < 0 || sc[i] >= N) in_range = false;
+            else cnt[sc[i]]++;
+        }
+        if (in_range) {
+            mx = 0;
+            for (int c : cnt) mx = c > mx ? c : mx;
+        }
+    }
+    const long long pairs = mx * (mx - 1) / 2 + mx;
+    e->giant_possible = (mx > T_BIG_B || pairs > T_BIG_P) ? 1 : 0;
+    e->gcol_possible = (mx > T_COLB || pairs > T_COLP) ? 1 : 0;
+    return ME_OK;
 }
 
 int read_stat(me_engine *e, hipStream_t s) {
@@ -787,6 +815,8 @@ TiledArgs tiled_args_of(me_engine *e, float sdt) {
     a.giant_grid = e->giant_grid;
     a.big_grid = e->big_grid;
     a.fast = (e->cfg.flags & ME_FLAG_FAST_MATH) != 0;
+    a.giant_possible = e->giant_possible;
+    a.gcol_possible = e->gcol_possible;
     a.small_grid = a.fast ? e->small_grid_fast : e->small_grid;
     a.giant_wait_ticks = e->giant_wait_ticks;
     a.taped = e->taping ? 1 : 0;
@@ -1190,7 +1220,8 @@ int me_create(const me_config *cfg, const me_bodies *bodies, void *workspace, si
         release(e);
         return fail(ME_ERUNTIME, "workspace initialisation failed");
     }
-    rc = bead_half_extents(e, s);
+    rc = scene_bounds(e);
+    if (rc == ME_OK) rc = bead_half_extents(e, s);
     if (rc) {
         release(e);
         return rc;
@@ -1217,6 +1248,8 @@ int me_set_bodies(me_engine *e, const me_bodies *bodies) {
     if (e->P > 0 && !bodies->rest) return fail(ME_EARG, "bead geometry needs rest positions");
     e->b = *bodies;
     drop_graphs(e);  // captured kernels hold the old body pointers
+    int rc = scene_bounds(e);
+    if (rc) return rc;
     return bead_half_extents(e, nullptr);
 }
 

@@ -105,6 +105,8 @@ struct TiledArgs {
     int big_grid;                    // workgroups of the big-tile kernels (one per CU the LDS allows)
     int small_grid;                  // workgroups of the persistent small-tile kernel (all resident ones)
     int fast;                        // fast arithmetic (ME_FLAG_FAST_MATH): the *_f contact updates
+    int giant_possible, gcol_possible;  // the largest scene could hold an island beyond the big tile's LDS
+                                        // (beyond the LDS colouring limits): islands never cross scenes
     long long giant_wait_ticks;      // bound of a grid barrier's wait, in wall-clock counter ticks
     AdjTape tape;
 };
@@ -1529,20 +1531,33 @@ __global__ void k_t_biglist(int ntile, int *istat, const float *invM, const floa
 // front in turn finish the launch close to max(largest tile, total work / workgroups) -- the
 // longest-processing-time rule -- where a fixed stride could stack two large tiles on one workgroup.
 // One thread per entry: the big list's first, then the small list's.
-__global__ void k_t_bigsort(int ntile, const int *istat, TiledBuffers T) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nbig = istat[IS_NBIG], nsmall = istat[IS_NSMALL];
-    const bool big = i < nbig;
-    const int n = big ? nbig : nsmall;
-    if (!big) i -= nbig;
-    if (i >= n || i >= ntile) return;
+// grid (tiles / 256, 2): y = 0 ranks the big list, y = 1 the small list; a list of up to T_SORT_LDS
+// entries is staged in LDS first (its weights and tile ids), so each thread's pass over it reads LDS
+#define T_SORT_LDS 4096
+__global__ void __launch_bounds__(256) k_t_bigsort(int ntile, const int *istat, TiledBuffers T) {
+    __shared__ int wl[T_SORT_LDS], ul[T_SORT_LDS];
+    const bool big = blockIdx.y == 0;
+    int n = big ? istat[IS_NBIG] : istat[IS_NSMALL];
+    if (n > ntile) n = ntile;
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;
     const int *list = big ? T.biglist : T.smalllist;
+    const bool lds = n <= T_SORT_LDS;
+    if (lds) {
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const int u = list[j];
+            ul[j] = u;
+            wl[j] = T.tp1[u] - T.tp0[u];
+        }
+        __syncthreads();
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     const int t = list[i];
     const int w = T.tp1[t] - T.tp0[t];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-        int u = list[j];
-        int wu = T.tp1[u] - T.tp0[u];
+        const int u = lds ? ul[j] : list[j];
+        const int wu = lds ? wl[j] : T.tp1[u] - T.tp0[u];
         rank += (wu > w || (wu == w && u < t)) ? 1 : 0;
     }
     (big ? T.bigq : T.smallq)[rank] = t;
@@ -1604,7 +1619,7 @@ inline int launch_solve(const TiledArgs &A, TiledBuffers &T, int ntile, hipStrea
     if (prof_rec(A, ME_K_TILE_SMALL, s)) return ME_ERUNTIME;
     // a tile beyond the big tile's LDS (more than T_BIG_B bodies or T_BIG_P pairs) is possible: the grid
     // sweep, after k_t_big, which leaves such tiles to it
-    if (A.N > T_BIG_B || A.capP > T_BIG_P) {
+    if (A.giant_possible && (A.N > T_BIG_B || A.capP > T_BIG_P)) {
         if (prof_rec(A, ME_K_TILE_GIANT, s2)) return ME_ERUNTIME;
         hipLaunchKernelGGL((k_t_giant<SOLVER, FAST>), dim3(A.giant_grid), dim3(T_BIG_WG), 0, s2, A, T);
         if (prof_rec(A, ME_K_TILE_GIANT, s2)) return ME_ERUNTIME;
@@ -1656,9 +1671,11 @@ inline int tiled_prepare(TiledBuffers &T, const TiledArgs &A, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_t_biglist, dim3(tblk(ntile > N ? ntile : N)), dim3(ME_THREADS), 0, s, ntile, A.istat, A.b.invM,
                        A.invIw, T);
-    hipLaunchKernelGGL(k_t_bigsort, dim3(tblk(ntile)), dim3(ME_THREADS), 0, s, ntile, A.istat, T);
+    hipLaunchKernelGGL(k_t_bigsort, dim3(tblk(ntile), 2), dim3(ME_THREADS), 0, s, ntile, A.istat, T);
     hipLaunchKernelGGL(k_t_color_small, dim3(ntile), dim3(T_SMALL_P), 0, s, A, T);
-    const int giant = N > T_COLB;  // a tile beyond the LDS colouring limits: coloured by a grid of workgroups
+    // a tile beyond the LDS colouring limits (possible only in a scene of that size): coloured by a grid
+    // of workgroups
+    const int giant = A.gcol_possible && N > T_COLB;
     if (prof_rec(A, ME_K_COLOR, s)) return ME_ERUNTIME;
     hipLaunchKernelGGL(k_t_color<1024>, dim3(ntile < A.big_grid ? ntile : A.big_grid), dim3(1024), T_COL_DYN, s, A, T,
                        giant);
