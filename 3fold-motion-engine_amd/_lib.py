@@ -78,6 +78,7 @@ def lib():
         L.me_tape_size.argtypes = [vp, C.POINTER(C.c_size_t)]
         L.me_adopt.argtypes = [vp]
         L.me_state_size.argtypes = [vp, C.POINTER(C.c_size_t)]
+        L.me_state_copy.argtypes = [vp, vp, vp, vp]
         L.me_ncp_workspace_size.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_size_t)]
         L.me_ncp_sensitivity.argtypes = [C.c_int32, C.c_int32, C.c_int32] + [vp] * 6 + [C.c_double] * 3 + \
             [vp] * 5 + [C.c_size_t, vp]
@@ -116,7 +117,7 @@ def check(rc: int, what: str) -> None:
 
 EXPORTED = ("me_workspace_size", "me_create", "me_destroy", "me_set_bodies", "me_step",
             "me_buffer", "me_read_counters", "me_profile", "me_profile_read", "me_last_error",
-            "me_tape_size", "me_step_taped", "me_adjoint_substep", "me_adopt", "me_state_size",
+            "me_tape_size", "me_step_taped", "me_adjoint_substep", "me_adopt", "me_state_size", "me_state_copy",
             "me_ncp_workspace_size", "me_ncp_sensitivity", "me_pgs_workspace_size", "me_pgs_create",
             "me_pgs_destroy", "me_pgs_prepare", "me_pgs_iterate", "me_pgs_snapshot", "me_pgs_residual",
             "me_pgs_buffer", "me_pgs_info", "me_ncp_sensitivity_noalloc", "me_ncp_svd_workspace_size",

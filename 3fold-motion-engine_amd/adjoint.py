@@ -84,13 +84,19 @@ class Rollout:
     # engine state: body tensors + the workspace's state block (status words, warm store, colour
     # cache, and what the reverse of a recorded substep reads); the rest of the workspace is scratch
     def _snap(self):
+        """Bodies and the state block; the block by the used items of its arrays (me_state_copy)."""
         e = self.eng
-        return tuple(t.clone() for t in (e.x, e.q, e.v, e.w, self._state))
+        blk = _torch().empty_like(self._state)
+        _lib.check(_lib.lib().me_state_copy(e._h, C.c_void_p(blk.data_ptr()), C.c_void_p(self._state.data_ptr()),
+                                            C.c_void_p(e._stream())), "me_state_copy")
+        return tuple(t.clone() for t in (e.x, e.q, e.v, e.w)) + (blk,)
 
     def _restore(self, s):
         e = self.eng
-        for dst, src in zip((e.x, e.q, e.v, e.w, self._state), s):
+        for dst, src in zip((e.x, e.q, e.v, e.w), s[:4]):
             dst.copy_(src)
+        _lib.check(_lib.lib().me_state_copy(e._h, C.c_void_p(self._state.data_ptr()), C.c_void_p(s[4].data_ptr()),
+                                            C.c_void_p(e._stream())), "me_state_copy")
 
     def _check(self):
         ovf = self.eng.refresh_counters(raise_on_overflow=False)["overflow"]

@@ -279,6 +279,39 @@ __global__ void __launch_bounds__(ME_THREADS) k_scan1(const int *in, int *out, i
     }
 }
 
+// ── the engine state block, copied by its used items ──
+// The state block (the workspace's leading bytes that carry a substep's state to the next and to the
+// adjoint of a recorded substep) is a sequence of arrays; a segment records one array's offset and
+// size and which status word counts its used items (of `unit` bytes), -1 for all of it. Copying only
+// the used items of the capacity-sized arrays is what makes a state snapshot cheap.
+#define ME_STATE_SEGS 64
+struct StateSeg {
+    unsigned long long off, bytes, unit;
+    int slot;
+};
+struct StateSegs {
+    StateSeg s[ME_STATE_SEGS];
+    int n = 0, overflow = 0;
+};
+
+// grid (blocks, segments): segment blockIdx.y, its used bytes (from the source's status words, rounded
+// up to 16 within the array) in 16-byte units
+__global__ void __launch_bounds__(ME_THREADS) k_state_copy(char *dst, const char *src, StateSegs segs) {
+    const StateSeg g = segs.s[blockIdx.y];
+    unsigned long long used = g.bytes;
+    if (g.slot >= 0) {
+        const int c = reinterpret_cast<const int *>(src)[g.slot];  // istat leads the block
+        const unsigned long long u = (unsigned long long)(c > 0 ? c : 0) * g.unit;
+        used = u < used ? u : used;
+    }
+    const unsigned long long n16 = (used + 15) / 16;
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src + g.off);
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst + g.off);
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n16;
+         i += (unsigned long long)gridDim.x * blockDim.x)
+        d4[i] = s4[i];
+}
+
 // ── A2 gravity, A3 world inverse inertia, box AABBs, velocity snapshot ──
 struct Bodies {
     float *x, *q, *v, *w;

@@ -55,10 +55,23 @@ inline int pow2_at_least(long long n) {
 struct Bump {
     char *base;
     size_t off = 0;
+    StateSegs *segs = nullptr;  // when set, every take is recorded as a state segment
     template <class T>
-    T *take(size_t count) {
+    T *take(size_t count) { return takec<T>(count, -1, 1); }
+    // count/per items of per elements each, of which status word `slot` counts the used items (-1: all)
+    template <class T>
+    T *takec(size_t count, int slot, size_t per) {
         off = (off + 255) & ~size_t(255);
         T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        if (segs && segs->n < ME_STATE_SEGS) {
+            StateSeg &g = segs->s[segs->n++];
+            g.off = off;
+            g.bytes = sizeof(T) * (count ? count : 1);
+            g.unit = sizeof(T) * per;
+            g.slot = slot;
+        } else if (segs) {
+            segs->overflow = 1;
+        }
         off += sizeof(T) * (count ? count : 1);
         return p;
     }
@@ -113,6 +126,7 @@ struct me_engine {
     int *istat;
     int *hstat;  // pinned host mirror
     size_t state_bytes = 0;  // leading workspace bytes that carry state between substeps
+    StateSegs segs{};        // the arrays of those bytes and the status words counting their used items
     hipStream_t side = nullptr;  // second stream for the big-tile sweeps (default priority)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // whole-step graphs (ME_FLAG_GRAPH): captured and launched on gs, keyed by (dt, substeps)
@@ -191,6 +205,8 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     size_t C = (size_t)e->capC, Pc = (size_t)e->capP;
     size_t ns = P > 0 ? 1 : 4 * (size_t)e->cap_cand;
     Bump B{base};
+    e->segs = StateSegs{};
+    B.segs = &e->segs;
     // ---- state: what a substep leaves for the next one (status words, warm store, colour cache)
     // and what the adjoint of a recorded substep reads (contacts, pairs, candidates, tiles,
     // per-contact constants, world inertia, pseudo-velocities); a snapshot of these leading bytes
@@ -199,43 +215,44 @@ void layout(me_engine *e, char *base, size_t *bytes) {
     e->invIw = B.take<float>(9 * (size_t)N);
     e->pv = B.take<float>(3 * (size_t)N);
     e->po = B.take<float>(3 * (size_t)N);
-    e->cand_a = B.take<int>((size_t)e->cap_cand);
-    e->cand_b = B.take<int>((size_t)e->cap_cand);
-    e->S.n = B.take<int>((size_t)e->cap_cand);
-    e->slot_off = B.take<int>((size_t)e->cap_cand);
-    e->Cc.key = B.take<int64_t>(C);
-    e->Cc.bi = B.take<int>(C);
-    e->Cc.bj = B.take<int>(C);
-    e->Cc.pA = B.take<float>(3 * C);
-    e->Cc.pB = B.take<float>(3 * C);
-    e->Cc.n = B.take<float>(3 * C);
-    e->Cc.pen = B.take<float>(C);
-    e->Cc.mu = B.take<float>(C);
-    e->Cc.jn = B.take<float>(C);
-    e->Cc.jt1 = B.take<float>(C);
-    e->Cc.jt2 = B.take<float>(C);
-    e->Cc.jp = B.take<float>(C);
-    e->Pp.pbi = B.take<int>(Pc);
-    e->Pp.pbj = B.take<int>(Pc);
-    e->Pp.pstart = B.take<int>(Pc);
-    e->Pp.pcount = B.take<int>(Pc);
-    e->Pp.col = B.take<int>(Pc);
-    e->Pp.porder = B.take<int>(Pc);
-    e->cache_a = B.take<int>(Pc);
-    e->cache_b = B.take<int>(Pc);
-    e->cache_col = B.take<int>(Pc);
+    e->cand_a = B.takec<int>((size_t)e->cap_cand, IS_CAND, 1);
+    e->cand_b = B.takec<int>((size_t)e->cap_cand, IS_CAND, 1);
+    e->S.n = B.takec<int>((size_t)e->cap_cand, IS_CAND, 1);
+    e->slot_off = B.takec<int>((size_t)e->cap_cand, IS_CAND, 1);
+    e->Cc.key = B.takec<int64_t>(C, IS_C, 1);
+    e->Cc.bi = B.takec<int>(C, IS_C, 1);
+    e->Cc.bj = B.takec<int>(C, IS_C, 1);
+    e->Cc.pA = B.takec<float>(3 * C, IS_C, 3);
+    e->Cc.pB = B.takec<float>(3 * C, IS_C, 3);
+    e->Cc.n = B.takec<float>(3 * C, IS_C, 3);
+    e->Cc.pen = B.takec<float>(C, IS_C, 1);
+    e->Cc.mu = B.takec<float>(C, IS_C, 1);
+    e->Cc.jn = B.takec<float>(C, IS_C, 1);
+    e->Cc.jt1 = B.takec<float>(C, IS_C, 1);
+    e->Cc.jt2 = B.takec<float>(C, IS_C, 1);
+    e->Cc.jp = B.takec<float>(C, IS_C, 1);
+    e->Pp.pbi = B.takec<int>(Pc, IS_P, 1);
+    e->Pp.pbj = B.takec<int>(Pc, IS_P, 1);
+    e->Pp.pstart = B.takec<int>(Pc, IS_P, 1);
+    e->Pp.pcount = B.takec<int>(Pc, IS_P, 1);
+    e->Pp.col = B.takec<int>(Pc, IS_P, 1);
+    e->Pp.porder = B.takec<int>(Pc, IS_P, 1);
+    e->cache_a = B.takec<int>(Pc, IS_CACHED_P, 1);
+    e->cache_b = B.takec<int>(Pc, IS_CACHED_P, 1);
+    e->cache_col = B.takec<int>(Pc, IS_CACHED_P, 1);
     e->cache_invM = B.take<float>((size_t)N);
     e->ccount = B.take<int>(ME_NCOLMAX + 1);
     e->cstart = B.take<int>(ME_NCOLMAX + 1);
     e->cfill = B.take<int>(ME_NCOLMAX + 1);
     e->csub = B.take<int>(ME_NCOLMAX + 1);
     e->cbase = B.take<int>(ME_NCOLMAX + 2);
-    e->wkey = B.take<int64_t>(C);
-    e->wjn = B.take<float>(C);
-    e->wjt1 = B.take<float>(C);
-    e->wjt2 = B.take<float>(C);
+    e->wkey = B.takec<int64_t>(C, IS_WN, 1);
+    e->wjn = B.takec<float>(C, IS_WN, 1);
+    e->wjt1 = B.takec<float>(C, IS_WN, 1);
+    e->wjt2 = B.takec<float>(C, IS_WN, 1);
     tiled_layout_state(e->tb, B, N, (int)Pc, (int)C);
     e->state_bytes = (B.off + 255) & ~size_t(255);
+    B.segs = nullptr;
     // ---- scratch: rewritten by every substep before it is read ----
     e->Rw = B.take<float>(9 * (size_t)N);
     e->aabb = B.take<float>(6 * (size_t)N);
@@ -1512,6 +1529,16 @@ int me_state_size(me_engine *e, size_t *bytes) {
     if (!e || !bytes) return fail(ME_EARG, "NULL argument");
     *bytes = e->state_bytes;
     return ME_OK;
+}
+
+int me_state_copy(me_engine *e, void *dst, const void *src, void *stream) {
+    if (!e || !dst || !src) return fail(ME_EARG, "NULL argument");
+    if (std::this_thread::get_id() != e->owner) return fail(ME_ETHREAD, "engine used from a thread that does not own it");
+    if (((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return fail(ME_EARG, "dst and src must be 16-byte aligned");
+    if (e->segs.overflow || e->segs.n < 1) return fail(ME_ERUNTIME, "state segment table incomplete");
+    hipLaunchKernelGGL(k_state_copy, dim3(64, e->segs.n), dim3(ME_THREADS), 0, (hipStream_t)stream, (char *)dst,
+                       (const char *)src, e->segs);
+    return hipGetLastError() == hipSuccess ? ME_OK : fail(ME_ERUNTIME, "state copy launch failed");
 }
 
 int me_tape_size(me_engine *e, size_t *bytes) {

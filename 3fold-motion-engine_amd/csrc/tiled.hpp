@@ -117,12 +117,12 @@ inline void tiled_layout_state(TiledBuffers &t, BumpT &B, int N, int capP, int c
     size_t n = (size_t)N, p = (size_t)capP;
     t.bsorted = B.template take<int>(n);
     t.bslot = B.template take<int>(n);
-    t.psorted = B.template take<int>(p);
+    t.psorted = B.template takec<int>(p, IS_P, 1);
     t.tb0 = B.template take<int>(n);
     t.tb1 = B.template take<int>(n);
     t.tp0 = B.template take<int>(n);
     t.tp1 = B.template take<int>(n);
-    t.prep = B.template take<float>(20 * (size_t)capC);
+    t.prep = B.template takec<float>(20 * (size_t)capC, IS_C, 20);
     t.ccls = B.template take<int>(ME_NCOLMAX + 1);
 }
 

@@ -214,6 +214,13 @@ int me_profile_read(me_engine *e, int kernel, double *total_ms, int64_t *launche
  * before reading. A copy of these bytes plus the body arrays is a complete checkpoint. */
 int me_state_size(me_engine *e, size_t *bytes);
 
+/* Copy an engine's state block (me_state_size bytes; dst and src each the workspace or a caller's
+ * snapshot of it, 16-byte aligned) by the used items of its capacity-sized arrays: the contacts, pairs,
+ * candidates, warm store and colour cache as far as the status words of src count them, the rest
+ * whole. No counterpart in the reference: for the adjoint's per-substep snapshots, which copied the
+ * whole block. Queued on `stream`. */
+int me_state_copy(me_engine *e, void *dst, const void *src, void *stream);
+
 /* NCP-level sensitivity, batched, float64 (replaces the reference's BatchProxADMMGPU.build_active +
  * grad_affine, ncp/ncp_batch_gpu.py:1160-1187 with kernels k_active :555-597 and k_solvesig :716-767,
  * and its CPU form ncp/ncp_ref.py: classify :251, _active_system :786, sensitivity :907).
