@@ -148,8 +148,10 @@ __device__ __forceinline__ float sat_overlap(f3 ax, const m3 &RA, f3 hA, const m
 }
 
 // box A vs box B, normal from B to A
+// A and B: the clipping polygons' storage (the caller's LDS: dynamically indexed, they would
+// otherwise sit in scratch memory)
 __device__ __forceinline__ int box_box(f3 cA, const m3 &RA, f3 hA, f3 cB, const m3 &RB, f3 hB, float margin,
-                                       f3 *pout, f3 *nout, float *pen, int *code) {
+                                       f3 *pout, f3 *nout, float *pen, int *code, Poly &A, Poly &B) {
     f3 d = cA - cB;
     float best = 3.4e38f;
     int bfrom = -1;
@@ -190,7 +192,6 @@ __device__ __forceinline__ int box_box(f3 cA, const m3 &RA, f3 hA, f3 cB, const 
             float val = dot(col(incR, i) * s, ref_n);
             if (val < bv) { bv = val; bi = i; bs = s; }
         }
-    Poly A, B;
     face_poly(incc, incR, inch, bi, bs, A.p);
     A.n = 4;
     for (int k = 0; k < 4; ++k) { A.code[k] = k; A.edge[k] = k; }
