@@ -46,8 +46,8 @@ __device__ __forceinline__ void face_poly(f3 c, const m3 &R, f3 h, int a, float 
 
 struct Poly {
     f3 p[ME_MAXPOLY];
-    int code[ME_MAXPOLY];
-    int edge[ME_MAXPOLY];
+    unsigned char code[ME_MAXPOLY];  // feature codes < 40
+    unsigned char edge[ME_MAXPOLY];  // edge ids < 8
     int n;
 };
 
@@ -81,8 +81,7 @@ __device__ __forceinline__ void clip_poly(const Poly &in, f3 sn, float d, int pl
 
 // deepest first (stable by gap), then greedily the point with the largest squared distance to
 // the kept ones; returns the kept indices ordered by feature code
-__device__ __forceinline__ int reduce4(int n, const f3 *p, const float *gap, const int *code, int *keep) {
-    int ord[ME_MAXPOLY];
+__device__ __forceinline__ int reduce4(int n, const f3 *p, const float *gap, const int *code, int *keep, int *ord) {
     for (int i = 0; i < n; ++i) ord[i] = i;
     for (int a = 1; a < n; ++a) {
         int t = ord[a], b = a - 1;
@@ -133,8 +132,8 @@ __device__ __forceinline__ int box_ground(f3 c, const m3 &R, f3 h, float margin,
         f3 vk = box_vertex(c, R, h, k);
         if (vk.z < margin) { p[n] = vk; gap[n] = vk.z; cd[n] = k; n++; }
     }
-    int keep[ME_MAXPOLY];
-    int nk = reduce4(n, p, gap, cd, keep);
+    int keep[ME_MAXPOLY], ord[ME_MAXPOLY];
+    int nk = reduce4(n, p, gap, cd, keep, ord);
     for (int i = 0; i < nk; ++i) { pout[i] = p[keep[i]]; pen[i] = -gap[keep[i]]; code[i] = cd[keep[i]]; }
     return nk;
 }
@@ -148,10 +147,18 @@ __device__ __forceinline__ float sat_overlap(f3 ax, const m3 &RA, f3 hA, const m
 }
 
 // box A vs box B, normal from B to A
-// A and B: the clipping polygons' storage (the caller's LDS: dynamically indexed, they would
-// otherwise sit in scratch memory)
+// the dynamically indexed arrays of box_box: the clipping polygons and the clipped points (the
+// caller's LDS: in private memory they would sit in scratch)
+struct ClipWork {
+    Poly A, B;
+    f3 p[ME_MAXPOLY];
+    float gap[ME_MAXPOLY];
+    int cd[ME_MAXPOLY], keep[ME_MAXPOLY], ord[ME_MAXPOLY];
+};
+
 __device__ __forceinline__ int box_box(f3 cA, const m3 &RA, f3 hA, f3 cB, const m3 &RB, f3 hB, float margin,
-                                       f3 *pout, f3 *nout, float *pen, int *code, Poly &A, Poly &B) {
+                                       f3 *pout, f3 *nout, float *pen, int *code, ClipWork &W) {
+    Poly &A = W.A, &B = W.B;
     f3 d = cA - cB;
     float best = 3.4e38f;
     int bfrom = -1;
@@ -209,9 +216,9 @@ __device__ __forceinline__ int box_box(f3 cA, const m3 &RA, f3 hA, f3 cB, const 
         Poly *t = cur; cur = nxt; nxt = t;
         if (cur->n == 0) return 0;
     }
-    f3 p[ME_MAXPOLY];
-    float gap[ME_MAXPOLY];
-    int cd[ME_MAXPOLY];
+    f3 *p = W.p;
+    float *gap = W.gap;
+    int *cd = W.cd;
     int n2 = 0;
     for (int i = 0; i < cur->n; ++i) {
         bool dup = false;
@@ -228,8 +235,8 @@ __device__ __forceinline__ int box_box(f3 cA, const m3 &RA, f3 hA, f3 cB, const 
             n2++;
         }
     }
-    int keep[ME_MAXPOLY];
-    int nk = reduce4(n2, p, gap, cd, keep);
+    int *keep = W.keep;
+    int nk = reduce4(n2, p, gap, cd, keep, W.ord);
     int face = (refIsB << 11) | ((ai * 2 + (ref_sign > 0.0f ? 1 : 0)) << 8) | ((bi * 2 + (bs > 0.0f ? 1 : 0)) << 5);
     for (int i = 0; i < nk; ++i) {
         pout[i] = p[keep[i]];

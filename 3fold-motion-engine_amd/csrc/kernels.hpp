@@ -542,11 +542,11 @@ struct Slots {
 };
 
 // ── narrow phase per candidate pair, up to 4 contacts in fixed slots ──
-#define ME_NARROW_WG 64  // threads of a k_narrow workgroup: two clipping polygons each in LDS (41 KB)
+#define ME_NARROW_WG 64  // threads of a k_narrow workgroup: each one's clipping work in LDS (58 KB)
 __global__ void __launch_bounds__(ME_NARROW_WG) k_narrow(int cap, const int *istat, Bodies b, const float *Rw,
                                                          const int *cand_a, const int *cand_b, float margin, int N,
                                                          Slots S) {
-    __shared__ Poly polys[2][ME_NARROW_WG];
+    __shared__ ClipWork work[ME_NARROW_WG];
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int n = istat[IS_CAND];
     if (n > cap) n = cap;
@@ -564,8 +564,7 @@ __global__ void __launch_bounds__(ME_NARROW_WG) k_narrow(int cap, const int *ist
         for (int i = 0; i < k; ++i) nn[i] = mk(0.0f, 0.0f, 1.0f);
     } else {
         m3 RB = ldm(Rw, bb);
-        k = box_box(cA, RA, hA, ld3(b.x, bb), RB, ld3(b.half, bb), margin, pp, nn, pen, code, polys[0][threadIdx.x],
-                    polys[1][threadIdx.x]);
+        k = box_box(cA, RA, hA, ld3(b.x, bb), RB, ld3(b.half, bb), margin, pp, nn, pen, code, work[threadIdx.x]);
     }
     float mu = pair_mu(b.mu, a, bb);
     S.n[p] = k;
