@@ -618,7 +618,54 @@ __device__ __forceinline__ void t_color_tile(const TiledArgs &A, const TiledBuff
         else gcol[T.psorted[p0 + k]] = -1;
     }
     int nround = 0;
-    for (int cur = 0; cur <= np; ++cur) {
+    if (lds && np <= T_COLP) {
+        // the LDS rounds with each thread's pairs (keys, body slots; -1 for a static side) in registers,
+        // read once instead of every round; the claims by tile slot (slot j is the tile's body j)
+        constexpr int PER = T_COLP / WG;
+        unsigned long long ck[PER];
+        int sa[PER], sb[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int k = threadIdx.x + j * WG;
+            ck[j] = 0ull;
+            sa[j] = sb[j] = -1;
+            if (k < np) {
+                const int p = T.psorted[p0 + k];
+                const int a = A.Pp.pbi[p], b = A.Pp.pbj[p];
+                ck[j] = (unsigned long long)jp_key(p);
+                if (invM[a] > 0.0f) sa[j] = T.bslot[a] - b0;
+                if (b >= 0 && invM[b] > 0.0f) sb[j] = T.bslot[b] - b0;
+            }
+        }
+        for (int cur = 0; cur <= np; ++cur) {
+            for (int j = threadIdx.x; j < nb; j += WG) claim[j] = ~0ull;
+            if (threadIdx.x == 0) left = 0;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int k = threadIdx.x + j * WG;
+                if (k >= np || colL[k] >= 0) continue;
+                if (sa[j] >= 0) atomicMin(&claim[sa[j]], ck[j]);
+                if (sb[j] >= 0) atomicMin(&claim[sb[j]], ck[j]);
+            }
+            __syncthreads();
+            int mine = 0;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int k = threadIdx.x + j * WG;
+                if (k >= np || colL[k] >= 0) continue;
+                const bool ok = (sa[j] < 0 || claim[sa[j]] == ck[j]) && (sb[j] < 0 || claim[sb[j]] == ck[j]);
+                if (ok) colL[k] = cur;
+                else ++mine;
+            }
+            if (mine) atomicAdd(&left, mine);
+            __syncthreads();
+            int l = left;
+            __syncthreads();
+            if (l == 0) { nround = cur + 1; break; }
+        }
+    }
+    for (int cur = 0; !(lds && np <= T_COLP) && cur <= np; ++cur) {
         for (int j = threadIdx.x; j < nb; j += WG) claim[slot(T.bsorted[b0 + j])] = ~0ull;
         if (threadIdx.x == 0) left = 0;
         __syncthreads();
