@@ -542,11 +542,18 @@ struct Slots {
 };
 
 // ── narrow phase per candidate pair, up to 4 contacts in fixed slots ──
-#define ME_NARROW_WG 64  // threads of a k_narrow workgroup: each one's clipping work in LDS (58 KB)
-__global__ void __launch_bounds__(ME_NARROW_WG) k_narrow(int cap, const int *istat, Bodies b, const float *Rw,
-                                                         const int *cand_a, const int *cand_b, float margin, int N,
-                                                         Slots S) {
-    __shared__ ClipWork work[ME_NARROW_WG];
+// LDSW: each thread's clipping work in LDS (64-thread workgroups, 58 KB each: at most two per CU),
+// for scenes whose candidates fit about one round of such workgroups; else in private memory
+// (scratch), 256 threads per workgroup and many workgroups per CU
+#define ME_NARROW_WG 64
+#define ME_NARROW_LDS_MAXN 12288  // bodies up to which k_narrow<true> runs (about 2.3 candidates per body)
+template <bool LDSW>
+__global__ void __launch_bounds__(LDSW ? ME_NARROW_WG : ME_THREADS) k_narrow(int cap, const int *istat, Bodies b,
+                                                                             const float *Rw, const int *cand_a,
+                                                                             const int *cand_b, float margin, int N,
+                                                                             Slots S) {
+    __shared__ ClipWork work[LDSW ? ME_NARROW_WG : 1];
+    ClipWork own;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     int n = istat[IS_CAND];
     if (n > cap) n = cap;
@@ -564,7 +571,8 @@ __global__ void __launch_bounds__(ME_NARROW_WG) k_narrow(int cap, const int *ist
         for (int i = 0; i < k; ++i) nn[i] = mk(0.0f, 0.0f, 1.0f);
     } else {
         m3 RB = ldm(Rw, bb);
-        k = box_box(cA, RA, hA, ld3(b.x, bb), RB, ld3(b.half, bb), margin, pp, nn, pen, code, work[threadIdx.x]);
+        k = box_box(cA, RA, hA, ld3(b.x, bb), RB, ld3(b.half, bb), margin, pp, nn, pen, code,
+                    LDSW ? work[threadIdx.x] : own);
     }
     float mu = pair_mu(b.mu, a, bb);
     S.n[p] = k;

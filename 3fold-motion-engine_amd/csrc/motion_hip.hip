@@ -507,8 +507,12 @@ int generate(me_engine *e, hipStream_t s) {
     if (rc) return rc;
     int cc = e->cap_cand;
     prof_mark(e, ME_K_NARROW, s);
-    hipLaunchKernelGGL(k_narrow, dim3((cc + ME_NARROW_WG - 1) / ME_NARROW_WG), dim3(ME_NARROW_WG), 0, s, cc, e->istat, b,
-                       e->Rw, e->cand_a, e->cand_b, e->cfg.margin, N, e->S);
+    if (N <= ME_NARROW_LDS_MAXN)
+        hipLaunchKernelGGL(k_narrow<true>, dim3((cc + ME_NARROW_WG - 1) / ME_NARROW_WG), dim3(ME_NARROW_WG), 0, s, cc,
+                           e->istat, b, e->Rw, e->cand_a, e->cand_b, e->cfg.margin, N, e->S);
+    else
+        hipLaunchKernelGGL(k_narrow<false>, dim3(nblk(cc)), dim3(ME_THREADS), 0, s, cc, e->istat, b, e->Rw, e->cand_a,
+                           e->cand_b, e->cfg.margin, N, e->S);
     prof_mark(e, ME_K_NARROW, s);
     // contact slots and non-empty pairs in one scan, which leaves the clamped counts (k_clamp_counts)
     rc = scan2(e, SC_NARROW, e->S.n, e->slot_off, e->pair_off, e->istat + IS_CAND, 0, cc, e->istat + IS_C,
