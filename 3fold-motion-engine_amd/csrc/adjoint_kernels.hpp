@@ -458,9 +458,13 @@ __device__ __forceinline__ void adj_narrow_body(int a, int bb, int k, const floa
     }
 }
 
+// two threads per candidate: the even one differentiates with respect to body a, the odd one to body
+// b (each re-runs the clipping in dual numbers seeded for its own body)
 __global__ void k_adj_narrow(TiledArgs A, int cap, const int *cand_a, const int *cand_b, const int *slot_n,
                              const int *slot_off, float margin) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = t >> 1;
+    const bool sideA = (t & 1) == 0;
     int n = A.istat[IS_CAND];
     if (n > cap) n = cap;
     if (p >= n) return;
@@ -468,16 +472,12 @@ __global__ void k_adj_narrow(TiledArgs A, int cap, const int *cand_a, const int 
     if (k <= 0) return;
     int a = cand_a[p], bb = cand_b[p], o = slot_off[p];
     if (o + k > A.capC) return;
+    const int body = sideA ? a : bb;
+    if (body < 0 || !(A.b.invM[body] > 0.0f)) return;
     const float *gb = A.tape.gbar + 7 * (size_t)o;
     float g[7];
-    if (A.b.invM[a] > 0.0f) {
-        adj_narrow_body(a, bb, k, gb, margin, A.b, A.tape.xpre, A.tape.qpre, true, g, A.istat + IS_OVF);
-        for (int m = 0; m < 7; ++m) fx_add(A.tape.acc, ADJ_X + m, a, g[m], A.istat);
-    }
-    if (bb >= 0 && A.b.invM[bb] > 0.0f) {
-        adj_narrow_body(a, bb, k, gb, margin, A.b, A.tape.xpre, A.tape.qpre, false, g, A.istat + IS_OVF);
-        for (int m = 0; m < 7; ++m) fx_add(A.tape.acc, ADJ_X + m, bb, g[m], A.istat);
-    }
+    adj_narrow_body(a, bb, k, gb, margin, A.b, A.tape.xpre, A.tape.qpre, sideA, g, A.istat + IS_OVF);
+    for (int m = 0; m < 7; ++m) fx_add(A.tape.acc, ADJ_X + m, body, g[m], A.istat);
 }
 
 // the body's fixed-point sums into its adjoints (centre, orientation, friction), then world inverse

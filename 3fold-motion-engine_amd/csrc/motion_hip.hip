@@ -1587,7 +1587,7 @@ int me_adjoint_substep(me_engine *e, double dt, void *tape, size_t bytes, float 
     hipLaunchKernelGGL(k_adj_integ, dim3(nblk(N)), dim3(ME_THREADS), 0, s, A, xbar, qbar, vbar, wbar);
     hipLaunchKernelGGL(k_adj_tiles, dim3(ntile), dim3(T_SMALL_P), 0, s, A, e->tb, vbar, wbar, jbar_end);
     hipLaunchKernelGGL(k_adj_contacts, dim3(nblk(capC)), dim3(ME_THREADS), 0, s, A, jbar_prev);
-    hipLaunchKernelGGL(k_adj_narrow, dim3(nblk(e->cap_cand)), dim3(ME_THREADS), 0, s, A, e->cap_cand, e->cand_a,
+    hipLaunchKernelGGL(k_adj_narrow, dim3(nblk(2LL * e->cap_cand)), dim3(ME_THREADS), 0, s, A, e->cap_cand, e->cand_a,
                        e->cand_b, e->S.n, e->slot_off, e->cfg.margin);
     hipLaunchKernelGGL(k_adj_body, dim3(nblk(N)), dim3(ME_THREADS), 0, s, A, xbar, qbar, mubar);
     LAUNCH_CHECK();
