@@ -60,6 +60,7 @@ struct TiledBuffers {
     int *giantlist;              // tiles beyond the big tile's LDS (IS_NGIANT of them): k_t_giant's
     int *gcollist;               // tiles beyond the LDS colouring limits (IS_NGCOL): k_t_color_giant's
     int *gcst;                   // giant tile: colour-class starts (ME_NCOLMAX + 1), class count at [ME_NCOLMAX + 1]
+    int *gcnt;                   // giant tile: pairs per class with two dynamic bodies, then with one (ME_NCOLMAX + 1 each)
     unsigned *gbar;              // giant tiles: grid-barrier counts of the sweep [0] and colouring [1]
                                  // launches, and the colouring's round counters [2..4]
 };
@@ -169,6 +170,7 @@ inline void tiled_layout_scratch(TiledBuffers &t, BumpT &B, int N, int capP, int
     t.giantlist = B.template take<int>(n);
     t.gcollist = B.template take<int>(n);
     t.gcst = B.template take<int>(ME_NCOLMAX + 2);
+    t.gcnt = B.template take<int>(2 * (ME_NCOLMAX + 1));
     t.gbar = B.template take<unsigned>(8);
 }
 
@@ -1549,6 +1551,7 @@ __global__ void __launch_bounds__(T_SMALL_P) k_t_color_small(TiledArgs A, TiledB
 __global__ void k_t_biglist(int ntile, int *istat, const float *invM, const float *invIw, TiledBuffers T) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < 5) T.gbar[t] = 0u;  // the giant-tile grid barriers and round counters of this substep start at zero
+    for (int j = t; j < 2 * (ME_NCOLMAX + 1); j += gridDim.x * blockDim.x) T.gcnt[j] = 0;  // its class totals
     if (t == 0) istat[IS_CCDIFF] = 0;  // read for the last time by k_t_mode: clear for the next substep
     if (t < istat[IS_NBODYSORT]) {
         int i = T.bsorted[t];
